@@ -1,0 +1,431 @@
+// C ABI of the engine: device context (tables uploaded once), applies,
+// device-resident TFQMR steady-state solver, FP32-in-FP64 iterative
+// refinement, RK4 / Chebyshev propagation and observables.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "heomq.h"
+#include "heomq_kernels.hpp"
+#include "heomq_model.hpp"
+#include "heomq_solver.hpp"
+#include "heomq_ctx.hpp"
+
+using namespace heomq;
+
+namespace {
+
+void build_records(const Model& m, std::vector<DevRec>& rec) {
+  const int M = m.M;
+  rec.assign(6 * (size_t)M, DevRec{});
+  auto set = [&](DevRec& r, std::complex<double> cL, std::complex<double> cR, int oL, int oR) {
+    r.cLr = cL.real(); r.cLi = cL.imag(); r.cRr = cR.real(); r.cRi = cR.imag();
+    r.opL = oL; r.opR = oR; r.pad0 = r.pad1 = 0;
+  };
+  const std::complex<double> I(0, 1);
+  for (int k = 0; k < M; ++k) {
+    // forward lower (calc_ax_tfqmr_omp.f90:247-258): -i*ff*cb op(c) X + i*fr*cd X op(c)
+    set(rec[k], -I * m.cb[k], I * m.cd[k], op_low(m, k), op_low(m, k));
+    // forward upper (:278-279, :342-353): -i*ff op'(c) X + i*fr X op'(c)
+    set(rec[M + k], -I, I, op_up(m, k), op_up(m, k));
+    for (int kind = 0; kind < 2; ++kind) {
+      const DevRec& f = rec[kind * M + k];
+      const std::complex<double> cL(f.cLr, f.cLi), cR(f.cRr, f.cRi);
+      // transpose of a direct term: conj(c) * Op^+ (the operator id with the dagger bit flipped)
+      set(rec[(2 + kind) * M + k], std::conj(cL), std::conj(cR), f.opL ^ 1, f.opR ^ 1);
+      // transpose of a term on X^H: a*Op*X^H -> a * W^H * Op, b*X^H*Op -> b * Op * W^H
+      set(rec[(4 + kind) * M + k], cR, cL, f.opR, f.opL);
+    }
+  }
+}
+
+void build_opmap(const Model& m, std::vector<int8_t>& map) {
+  const int d = m.d, nop = 2 * m.norbs * m.nspin;
+  map.assign((size_t)nop * 4 * d, 0);
+  for (int orb = 0; orb < m.norbs; ++orb)
+    for (int s = 0; s < m.nspin; ++s)
+      for (int t = 0; t < 2; ++t) {
+        const int o = 2 * (orb * m.nspin + s) + t;
+        int8_t* pL = &map[(size_t)o * 4 * d];
+        int8_t* sL = pL + d;
+        int8_t* pR = pL + 2 * d;
+        int8_t* sR = pL + 3 * d;
+        for (int i = 0; i < d; ++i) { pL[i] = -1; sL[i] = 0; pR[i] = -1; sR[i] = 0; }
+        for (int r = 0; r < d; ++r)
+          for (int c = 0; c < d; ++c) {
+            const double v = t == 0 ? m.amsv(orb, s, r, c) : m.amsv(orb, s, c, r);   // op(r, c)
+            if (v == 0.0) continue;
+            pL[r] = (int8_t)c;                    // (op X)(r, j) = v * X(c, j)
+            sL[r] = (int8_t)(v > 0 ? 1 : -1);
+            pR[c] = (int8_t)r;                    // (X op)(i, c) = v * X(i, r)
+            sR[c] = (int8_t)(v > 0 ? 1 : -1);
+          }
+      }
+}
+
+void build_hcsr(const std::vector<cplx>& H, int d, std::vector<int32_t>& nnz, std::vector<int32_t>& col,
+                std::vector<double>& val) {
+  nnz.assign(d, 0);
+  col.assign((size_t)d * d, 0);
+  val.assign((size_t)2 * d * d, 0.0);
+  for (int i = 0; i < d; ++i)
+    for (int k = 0; k < d; ++k) {
+      const cplx h = H[i + (size_t)k * d];
+      if (h == cplx(0, 0)) continue;
+      const int t = nnz[i]++;
+      col[(size_t)i * d + t] = k;
+      val[2 * ((size_t)i * d + t)] = h.real();
+      val[2 * ((size_t)i * d + t) + 1] = h.imag();
+    }
+}
+
+// per-ADO diagonal rates gamma_K = sum_r gdraw[K_r]
+void build_gamma(const Model& m, const std::vector<cplx>& gdraw, std::vector<double>& gam, bool conj) {
+  gam.assign(2 * (size_t)m.N, 0.0);
+  for (int n = 1; n <= m.L; ++n)
+    for (int64_t g = m.first[n]; g < m.first[n + 1]; ++g) {
+      cplx s(0, 0);
+      const int32_t* K = &m.tiers[n][(g - m.first[n]) * n];
+      for (int r = 0; r < n; ++r) s += gdraw[K[r]];
+      gam[2 * g] = s.real();
+      gam[2 * g + 1] = conj ? -s.imag() : s.imag();
+    }
+}
+
+int build_entries(const Model& m, std::vector<int64_t>& rowf, std::vector<DevEntry>& entf,
+                  std::vector<int64_t>& rowt, std::vector<DevEntry>& entt) {
+  const int M = m.M;
+  rowf.assign(m.N + 1, 0);
+  entf.clear();
+  entf.reserve(m.lower.size() + m.upper.size());
+  auto flags = [](int itype, int ff, int fr) {
+    return (uint8_t)((itype ? 1 : 0) | (ff < 0 ? 2 : 0) | (fr < 0 ? 4 : 0));
+  };
+  for (int64_t g = 0; g < m.N; ++g) {
+    rowf[g] = (int64_t)entf.size();
+    for (int64_t p = m.lower_off[g]; p < m.lower_off[g + 1]; ++p) {
+      const LowerEntry& e = m.lower[p];
+      entf.push_back(DevEntry{e.nbr, (uint16_t)e.drawer, flags(e.itype, e.ff, e.fr), 0});
+    }
+    if (g < m.n_upper_ados)
+      for (int k = 0; k < M; ++k) {
+        const UpperEntry& e = m.upper[(size_t)g * M + k];
+        if (e.nbr < 0) continue;
+        entf.push_back(DevEntry{e.nbr, (uint16_t)(M + k), flags(e.itype, e.ff, e.fr), 0});
+      }
+  }
+  rowf[m.N] = (int64_t)entf.size();
+  if (6 * (int64_t)M > 65535) return fail(HEOMQ_UNSUPPORTED, "too many drawers for the record table");
+  // reverse every forward entry: K <- J becomes J <- K
+  rowt.assign(m.N + 1, 0);
+  for (const DevEntry& e : entf) rowt[e.nbr + 1]++;
+  for (int64_t g = 0; g < m.N; ++g) rowt[g + 1] += rowt[g];
+  entt.assign(entf.size(), DevEntry{});
+  std::vector<int64_t> fill(rowt.begin(), rowt.end() - 1);
+  for (int64_t g = 0; g < m.N; ++g)
+    for (int64_t p = rowf[g]; p < rowf[g + 1]; ++p) {
+      const DevEntry& e = entf[p];
+      const int kind = e.rec >= M ? 1 : 0;
+      const int k = e.rec - kind * M;
+      const bool dag = e.flags & 1;
+      const bool nL = e.flags & 2, nR = e.flags & 4;
+      DevEntry t;
+      t.nbr = (int32_t)g;
+      t.rec = (uint16_t)((dag ? 4 + kind : 2 + kind) * M + k);
+      t.flags = dag ? (uint8_t)(1 | (nR ? 2 : 0) | (nL ? 4 : 0)) : (uint8_t)((nL ? 2 : 0) | (nR ? 4 : 0));
+      t.pad = 0;
+      entt[fill[e.nbr]++] = t;
+    }
+  return HEOMQ_OK;
+}
+
+int upload_coefs(heomq_ctx* c, const Model& m) {
+  std::vector<DevRec> rec;
+  build_records(m, rec);
+  HIPCHECK(upload(c->rec, rec));
+  std::vector<double> gam;
+  build_gamma(m, m.gdraw, gam, false);
+  HIPCHECK(upload(c->fwd.gam, gam));
+  build_gamma(m, m.gdraw, gam, true);
+  HIPCHECK(upload(c->trn.gam, gam));
+  std::vector<int32_t> nnz, col;
+  std::vector<double> val;
+  build_hcsr(m.H, m.d, nnz, col, val);
+  HIPCHECK(upload(c->hnnz, nnz));
+  HIPCHECK(upload(c->hcol, col));
+  HIPCHECK(upload(c->hval, val));
+  return HEOMQ_OK;
+}
+
+void fill_op(heomq_ctx* c, OperatorTables& t, const DevBuf& rec, bool transposed, double hsign) {
+  DevOperator& o = t.op;
+  o.n_ado = c->N;
+  o.rowptr = t.rowptr.as<int64_t>();
+  o.ent = t.ent.as<DevEntry>();
+  o.rec = rec.as<DevRec>();
+  o.gam = t.gam.as<double>();
+  o.opmap = c->opmap.as<int8_t>();
+  o.hnnz = c->hnnz.as<int32_t>();
+  o.hcol = c->hcol.as<int32_t>();
+  o.hval = c->hval.as<double>();
+  o.nop = 2 * c->norbs * c->nspin;
+  o.d = c->d;
+  o.hsign = hsign;
+  o.transposed = transposed;
+  o.parity = c->parity.as<int8_t>();
+  o.tb = c->tb;
+}
+
+int64_t trace_pos(const heomq_ctx* c, int trace) {
+  if (trace == HEOMQ_TRACE_LAST) return c->N - 1;
+  if (trace == HEOMQ_TRACE_RDO) return 0;
+  return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* heomq_last_error(void) { return last_error().c_str(); }
+int heomq_version(void) { return 10000; }
+
+int heomq_model_build(const heomq_params* p, heomq_model** out) {
+  if (!p || !out) return fail(HEOMQ_BAD_ARG, "null argument");
+  auto* h = new heomq_model;
+  const int rc = build_model(*p, h->m);
+  if (rc) { delete h; return rc; }
+  *out = h;
+  return HEOMQ_OK;
+}
+
+void heomq_model_free(heomq_model* m) { delete m; }
+
+int heomq_model_info(const heomq_model* h, int64_t* info, int64_t* first) {
+  if (!h) return fail(HEOMQ_BAD_ARG, "null model");
+  const Model& m = h->m;
+  if (info) {
+    info[0] = m.d; info[1] = m.M; info[2] = m.L; info[3] = m.N;
+    info[4] = m.ncor; info[5] = m.nopr;
+    info[6] = (int64_t)m.lower.size();
+    int64_t nu = 0;
+    for (const auto& e : m.upper) nu += e.nbr >= 0;
+    info[7] = nu;
+  }
+  if (first) for (int n = 0; n <= m.L + 1; ++n) first[n] = m.first[n];
+  return HEOMQ_OK;
+}
+
+int heomq_model_coefs(const heomq_model* h, double* cb, double* cd, double* gam) {
+  if (!h) return fail(HEOMQ_BAD_ARG, "null model");
+  const Model& m = h->m;
+  for (int k = 0; k < m.M; ++k) {
+    if (cb) { cb[2 * k] = m.cb[k].real(); cb[2 * k + 1] = m.cb[k].imag(); }
+    if (cd) { cd[2 * k] = m.cd[k].real(); cd[2 * k + 1] = m.cd[k].imag(); }
+    if (gam) { gam[2 * k] = m.gdraw[k].real(); gam[2 * k + 1] = m.gdraw[k].imag(); }
+  }
+  return HEOMQ_OK;
+}
+
+int heomq_model_operators(const heomq_model* h, double* H, double* ams) {
+  if (!h) return fail(HEOMQ_BAD_ARG, "null model");
+  const Model& m = h->m;
+  if (H) for (size_t t = 0; t < m.H.size(); ++t) { H[2 * t] = m.H[t].real(); H[2 * t + 1] = m.H[t].imag(); }
+  if (ams) std::memcpy(ams, m.ams.data(), m.ams.size() * sizeof(double));
+  return HEOMQ_OK;
+}
+
+int heomq_model_tier(const heomq_model* h, int tier, int32_t* tuples) {
+  if (!h || tier < 0 || tier > h->m.L) return fail(HEOMQ_BAD_ARG, "bad tier");
+  const auto& t = h->m.tiers[tier];
+  if (tuples && !t.empty()) std::memcpy(tuples, t.data(), t.size() * sizeof(int32_t));
+  return HEOMQ_OK;
+}
+
+int heomq_model_coupling(const heomq_model* h, int32_t* lower, int32_t* upper) {
+  if (!h) return fail(HEOMQ_BAD_ARG, "null model");
+  const Model& m = h->m;
+  if (lower)
+    for (size_t p = 0; p < m.lower.size(); ++p) {
+      lower[4 * p] = m.lower[p].nbr; lower[4 * p + 1] = m.lower[p].itype;
+      lower[4 * p + 2] = m.lower[p].ff; lower[4 * p + 3] = m.lower[p].fr;
+    }
+  if (upper)
+    for (size_t p = 0; p < m.upper.size(); ++p) {
+      upper[4 * p] = m.upper[p].nbr; upper[4 * p + 1] = m.upper[p].itype;
+      upper[4 * p + 2] = m.upper[p].ff; upper[4 * p + 3] = m.upper[p].fr;
+    }
+  return HEOMQ_OK;
+}
+
+int heomq_model_set_params(heomq_model* h, const heomq_params* p) {
+  if (!h || !p) return fail(HEOMQ_BAD_ARG, "null argument");
+  return set_params(h->m, *p);
+}
+
+int heomq_model_from_reference(
+    int32_t nrho, int64_t nunk, int32_t ntier, int32_t norbs, int32_t nspin, int32_t nalf,
+    int32_t ncor, int32_t igroundsteady, const int32_t* indextable, const int64_t* nfirst,
+    const int64_t* nlast, const int64_t* ifirst, const int64_t* indexcoef, const int8_t* itypecoef,
+    const int8_t* ifactfcoef, const int8_t* ifactrcoef, const int64_t* index_coef_ref,
+    const int32_t* indexdraw, const double* cb, const double* cd, const double* cgamma,
+    const double* dipm, const int32_t* ilead, const int32_t* mspin, const int32_t* morbs,
+    const int32_t* mpm, const double* eleadinfty, const double* hs, const double* amsall,
+    heomq_model** out) {
+  if (!out) return fail(HEOMQ_BAD_ARG, "null output");
+  auto* h = new heomq_model;
+  const int rc = model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady,
+                                      indextable, nfirst, nlast, ifirst, indexcoef, itypecoef,
+                                      ifactfcoef, ifactrcoef, index_coef_ref, indexdraw, cb, cd, cgamma,
+                                      dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, h->m);
+  if (rc) { delete h; return rc; }
+  *out = h;
+  return HEOMQ_OK;
+}
+
+int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
+  if (!h || !out) return fail(HEOMQ_BAD_ARG, "null argument");
+  const Model& m = h->m;
+  if (m.d != 4 && m.d != 16 && m.d != 64) return fail(HEOMQ_UNSUPPORTED, "d must be 4, 16 or 64");
+  HIPCHECK(hipSetDevice(device));
+  auto c = std::make_unique<heomq_ctx>();
+  c->device = device;
+  c->d = m.d; c->M = m.M; c->L = m.L; c->N = m.N;
+  c->nspin = m.nspin; c->norbs = m.norbs; c->nalf = m.nalf;
+  c->first = m.first;
+  c->sgn = m.sgn; c->spin = m.spin; c->orb = m.orb; c->alf = m.alf;
+  c->ams = m.ams;
+  c->tiers = m.tiers;
+  // tier-1 position of each sigma=+ drawer (td/calcj.f90 reads these ADOs)
+  c->tier1_of.assign(m.M, -1);
+  c->tier1_dag.assign(m.M, 0);
+  c->tier1_fact.assign(m.M, 1);
+  if (m.L >= 1)
+    for (int k = 0; k < m.M; ++k) {
+      const UpperEntry& e = m.upper[k];   // tier-0 upper entries are the tier-1 lookups
+      c->tier1_of[k] = e.nbr;
+      c->tier1_dag[k] = e.itype;
+      c->tier1_fact[k] = e.ff;
+    }
+  std::vector<int64_t> rowf, rowt;
+  std::vector<DevEntry> entf, entt;
+  if (int rc = build_entries(m, rowf, entf, rowt, entt)) return rc;
+  HIPCHECK(upload(c->fwd.rowptr, rowf));
+  HIPCHECK(upload(c->fwd.ent, entf));
+  HIPCHECK(upload(c->trn.rowptr, rowt));
+  HIPCHECK(upload(c->trn.ent, entt));
+  std::vector<int8_t> map;
+  build_opmap(m, map);
+  HIPCHECK(upload(c->opmap, map));
+  if (int rc = upload_coefs(c.get(), m)) return rc;
+  {
+    std::vector<int8_t> par(m.d);
+    for (int i = 0; i < m.d; ++i) {
+      int ne = 0, v = i;
+      for (int o = 0; o < m.norbs; ++o, v /= 4) ne += (v % 4 == 1) + (v % 4 == 2) + 2 * (v % 4 == 3);
+      par[i] = (int8_t)(ne & 1);
+    }
+    HIPCHECK(upload(c->parity, par));
+    c->tb.ntier = m.L + 1;
+    for (int n = 0; n <= m.L + 1 && n < 12; ++n) c->tb.first[n] = m.first[n];
+  }
+  fill_op(c.get(), c->fwd, c->rec, false, -1.0);
+  fill_op(c.get(), c->trn, c->rec, true, +1.0);
+  // derivative operator shares the forward entries; records/H/gamma set per call
+  c->dpar.op = c->fwd.op;
+  HIPCHECK(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
+  HIPCHECK(c->scal.alloc(sizeof(TfqmrScal) * 2));
+  HIPCHECK(hipMemset(c->scal.p, 0, sizeof(TfqmrScal) * 2));
+  HIPCHECK(c->partial.alloc(sizeof(double) * 2 * kMaxRedBlocks));
+  HIPCHECK(c->red.alloc(sizeof(double) * 4 + sizeof(unsigned) * 4));
+  HIPCHECK(hipMemset(c->red.p, 0, c->red.bytes));
+  HIPCHECK(hipDeviceSynchronize());
+  *out = c.release();
+  return HEOMQ_OK;
+}
+
+void heomq_ctx_free(heomq_ctx* c) { delete c; }
+
+int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
+  if (!c || !h) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (h->m.N != c->N || h->m.M != c->M) return fail(HEOMQ_BAD_ARG, "model shape differs from context");
+  HIPCHECK(hipSetDevice(c->device));
+  if (int rc = upload_coefs(c, h->m)) return rc;
+  fill_op(c, c->fwd, c->rec, false, -1.0);
+  fill_op(c, c->trn, c->rec, true, +1.0);
+  HIPCHECK(hipDeviceSynchronize());
+  return HEOMQ_OK;
+}
+
+int64_t heomq_ctx_vector_len(const heomq_ctx* c) { return c ? c->N * c->d * c->d : -1; }
+
+int heomq_apply(heomq_ctx* c, const double* x, double* y, int trace, void* stream) {
+  if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  HIPCHECK(launch_apply_f64(c->fwd.op, x, y, trace_pos(c, trace), (hipStream_t)stream));
+  return HEOMQ_OK;
+}
+
+int heomq_apply_transpose(heomq_ctx* c, const double* x, double* y, int trace, void* stream) {
+  if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  HIPCHECK(launch_apply_f64(c->trn.op, x, y, trace_pos(c, trace), (hipStream_t)stream));
+  return HEOMQ_OK;
+}
+
+int heomq_apply_f32(heomq_ctx* c, const float* x, float* y, int trace, void* stream) {
+  if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  HIPCHECK(launch_apply_f32(c->fwd.op, x, y, trace_pos(c, trace), (hipStream_t)stream));
+  return HEOMQ_OK;
+}
+
+int heomq_apply_dparam(heomq_ctx* c, const double* x, double* y, const double* dH, const double* dcb,
+                       const double* dcd, const double* dgam, void* stream) {
+  if (!c || !x || !y || !dH || !dcb || !dcd || !dgam) return fail(HEOMQ_BAD_ARG, "null argument");
+  const int M = c->M, d = c->d;
+  // records: lower-tier kind with derivative coefficients; upper-tier terms
+  // (coefficients -i, +i) do not depend on parameters and are switched off
+  std::vector<DevRec> rec(6 * (size_t)M, DevRec{});
+  const std::complex<double> I(0, 1);
+  std::vector<cplx> gd(M);
+  for (int k = 0; k < M; ++k) {
+    const cplx b(dcb[2 * k], dcb[2 * k + 1]), dd(dcd[2 * k], dcd[2 * k + 1]);
+    const cplx cL = -I * b, cR = I * dd;
+    const int o = 2 * (c->orb[k] * c->nspin + c->spin[k]) + (c->sgn[k] == 0 ? 1 : 0);
+    rec[k] = DevRec{cL.real(), cL.imag(), cR.real(), cR.imag(), o, o, 0, 0};
+    rec[M + k] = DevRec{0, 0, 0, 0, -1, -1, 0, 0};
+    gd[k] = cplx(dgam[2 * k], dgam[2 * k + 1]);
+  }
+  HIPCHECK(upload(c->rec_d, rec));
+  std::vector<double> gam(2 * (size_t)c->N, 0.0);
+  for (int n = 1; n <= c->L; ++n)
+    for (int64_t g = c->first[n]; g < c->first[n + 1]; ++g) {
+      cplx s(0, 0);
+      const int32_t* K = &c->tiers[n][(g - c->first[n]) * n];
+      for (int r = 0; r < n; ++r) s += gd[K[r]];
+      gam[2 * g] = s.real();
+      gam[2 * g + 1] = s.imag();
+    }
+  HIPCHECK(upload(c->dpar.gam, gam));
+  std::vector<cplx> Hd((size_t)d * d);
+  for (size_t t = 0; t < Hd.size(); ++t) Hd[t] = cplx(dH[2 * t], dH[2 * t + 1]);
+  std::vector<int32_t> nnz, col;
+  std::vector<double> val;
+  build_hcsr(Hd, d, nnz, col, val);
+  HIPCHECK(upload(c->hnnz_d, nnz));
+  HIPCHECK(upload(c->hcol_d, col));
+  HIPCHECK(upload(c->hval_d, val));
+  DevOperator op = c->fwd.op;
+  op.rec = c->rec_d.as<DevRec>();
+  op.gam = c->dpar.gam.as<double>();
+  op.hnnz = c->hnnz_d.as<int32_t>();
+  op.hcol = c->hcol_d.as<int32_t>();
+  op.hval = c->hval_d.as<double>();
+  HIPCHECK(launch_apply_f64(op, x, y, -1, (hipStream_t)stream));
+  HIPCHECK(hipStreamSynchronize((hipStream_t)stream));   // tables above are reused by the next call
+  return HEOMQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Internal definitions shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "heomq.h"
+#include "heomq_kernels.hpp"
+#include "heomq_model.hpp"
+#include "heomq_solver.hpp"
+
+struct heomq_model {
+  heomq::Model m;
+};
+
+#define HIPCHECK(expr)                                                                \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return heomq::fail(HEOMQ_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    release();
+    bytes = std::max<size_t>(b, 16);
+    return hipMalloc(&p, bytes);
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <typename T>
+inline hipError_t upload(DevBuf& b, const std::vector<T>& v) {
+  hipError_t e = b.alloc(v.size() * sizeof(T));
+  if (e != hipSuccess || v.empty()) return e;
+  return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+inline int op_low(const heomq::Model& m, int k) {   // lower-tier operator: c^+ for sigma = +, c for sigma = -
+  return 2 * (m.orb[k] * m.nspin + m.spin[k]) + (m.sgn[k] == 0 ? 1 : 0);
+}
+inline int op_up(const heomq::Model& m, int k) {    // upper-tier operator: c for sigma = +, c^+ for sigma = -
+  return 2 * (m.orb[k] * m.nspin + m.spin[k]) + (m.sgn[k] == 0 ? 0 : 1);
+}
+
+struct OperatorTables {
+  DevBuf rowptr, ent, gam;
+  heomq::DevOperator op;
+};
+
+struct heomq_ctx {
+  int device = 0;
+  int d = 0, M = 0, L = 0, nspin = 2, norbs = 1, nalf = 2;
+  int64_t N = 0, n1first = 1, n1count = 0;
+  std::vector<int64_t> first;
+  std::vector<int> sgn, spin, orb, alf;
+  // tier-1 lookup of drawers with sign + (for observables): ADO index
+  std::vector<int64_t> tier1_of;     // per drawer: ADO index or -1
+  std::vector<int> tier1_dag;        // 1 if stored as its Hermitian partner
+  std::vector<int> tier1_fact;
+  std::vector<double> ams;           // host copy for observables
+  std::vector<std::vector<int32_t>> tiers;   // drawer tuples (for per-ADO rates)
+  // shared device tables
+  DevBuf rec, rec_d, opmap, hnnz, hcol, hval, hnnz_d, hcol_d, hval_d;
+  OperatorTables fwd, trn, dpar;
+  // host-side entry lists (kept for re-upload of coefficients)
+  std::vector<int32_t> lower_drawers;
+  // solver workspace
+  DevBuf ws64, ws32, scal, partial, red;
+  DevBuf parity;                     // electron-number parity of each Fock state
+  void* pinned = nullptr;            // small pinned host buffer for scalar read-back
+  ~heomq_ctx() {
+    if (pinned) (void)hipHostFree(pinned);
+  }
+  heomq::TierBounds tb{};
+  hipStream_t own_stream = nullptr;
+};
+

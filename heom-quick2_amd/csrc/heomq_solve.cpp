@@ -1,0 +1,395 @@
+// Solver-level entry points: device TFQMR (the solve_tfqmr.f90 + dutfx.f90
+// drop-in), FP32-inside-FP64 iterative refinement, RK4 (td/intstp.f90),
+// Chebyshev (td/chebyshev.f90 + prechebyshev.f90), observables
+// (td/calcj.f90, main/calcocc.f90) and the host-pointer apply drop-ins.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "heomq_ctx.hpp"
+
+using namespace heomq;
+
+namespace {
+
+template <typename T>
+hipError_t apply_T(heomq_ctx* c, bool trans, const T* x, T* y, int64_t tpos, hipStream_t s, bool proj);
+// Solvers iterate with the projected operator Pi*A (Pi: parity sectors +
+// Hermitian tier 0).  A maps range(Pi) into itself and the physical steady
+// state is its unique solution there, so every Krylov vector stays physical
+// and round-off in the operator's non-physical null directions is removed at
+// every apply.  For the adjoint, (Pi A Pi)^T = Pi A^T Pi.
+template <>
+hipError_t apply_T<double>(heomq_ctx* c, bool trans, const double* x, double* y, int64_t tpos, hipStream_t s,
+                           bool proj) {
+  return launch_apply_f64(trans ? c->trn.op : c->fwd.op, x, y, tpos, s, proj);
+}
+template <>
+hipError_t apply_T<float>(heomq_ctx* c, bool trans, const float* x, float* y, int64_t tpos, hipStream_t s,
+                          bool proj) {
+  return launch_apply_f32(trans ? c->trn.op : c->fwd.op, x, y, tpos, s, proj);
+}
+
+int64_t tpos_of(const heomq_ctx* c, int trace) {
+  if (trace == HEOMQ_TRACE_LAST) return c->N - 1;
+  if (trace == HEOMQ_TRACE_RDO) return 0;
+  return -1;
+}
+
+template <typename T>
+struct Work {
+  T *w, *y1, *y2, *u1, *u2, *v, *d, *rt;
+};
+
+template <typename T>
+int carve(DevBuf& buf, int64_t n, int count, std::vector<T*>& out) {
+  HIPCHECK(buf.alloc(sizeof(T) * (size_t)n * count + 256 * count));
+  out.clear();
+  char* base = static_cast<char*>(buf.p);
+  const size_t stride = ((sizeof(T) * (size_t)n + 255) / 256) * 256;
+  for (int k = 0; k < count; ++k) out.push_back(reinterpret_cast<T*>(base + k * stride));
+  return HEOMQ_OK;
+}
+
+struct RunStats {
+  int iters = 0, applies = 0;
+  double relres = 0;
+};
+
+// TFQMR on A x = b with every vector and scalar on the device.  Restarts from
+// the true residual when the quasi-residual bound is met but ||b - Ax|| is not
+// below tol*||b|| (at most 3 times; stagnation stops it).
+// project: 0 none, 2 iterate with the projected operator Pi*A (parity sectors
+// + Hermitian tier 0; needs the trace row on the tier-0 block).
+template <typename T>
+int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool trans, int64_t tpos,
+              double tol, int maxiter, hipStream_t st, RunStats& rs, int project) {
+  TfqmrScal* s = c->scal.as<TfqmrScal>();
+  double* part = c->partial.as<double>();
+  double* red = c->red.as<double>();
+  unsigned* ticket = reinterpret_cast<unsigned*>(red + 4);
+  HIPCHECK(vec_randn<T>(st, n, W.rt, 0x5EED1234ull));
+  const int check_every = 8;
+  double best = std::numeric_limits<double>::infinity();
+  int base_iters = 0;
+  int stalls = 0;
+  while (true) {
+    if (project) HIPCHECK(vec_project<T>(st, c->N, c->d, x, c->parity.as<int8_t>(), c->tb, true));
+    HIPCHECK(apply_T<T>(c, trans, x, W.u1, tpos, st, project != 0));
+    rs.applies++;
+    HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, ticket, part)));
+    double* hr = static_cast<double*>(c->pinned);
+    HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const double rr = hr[0], bb = hr[1];
+    rs.relres = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    if (rr <= tol * tol * bb || rr == 0.0) return HEOMQ_OK;
+    if (base_iters >= maxiter) break;
+    if (rr >= best && ++stalls > 3) break;
+    best = std::min(best, rr);
+    HIPCHECK(tfq_reset(st, s, tol * std::sqrt(bb)));
+    HIPCHECK(tfq_init_r<T>(st, n, b, W.u1, W.w, W.y1, W.d, W.rt, s, part));
+    HIPCHECK(apply_T<T>(c, trans, W.y1, W.u1, tpos, st, project != 0));
+    rs.applies++;
+    HIPCHECK(tfq_init_v<T>(st, n, W.u1, W.v, W.rt, s, part));
+    TfqmrScal& hs = *reinterpret_cast<TfqmrScal*>(static_cast<char*>(c->pinned) + 1024);
+    while (true) {
+      for (int k = 0; k < check_every; ++k) {
+        HIPCHECK(tfq_y2<T>(st, n, W.y1, W.v, W.y2, s));
+        HIPCHECK(apply_T<T>(c, trans, W.y2, W.u2, tpos, st, project != 0));
+        HIPCHECK(tfq_half<T>(st, n, x, 0, W.w, W.u1, W.d, W.y1, s, part));
+        HIPCHECK(tfq_half<T>(st, n, x, 1, W.w, W.u2, W.d, W.y2, s, part));
+        HIPCHECK(tfq_rho<T>(st, n, x, W.d, W.w, W.rt, s, part));
+        HIPCHECK(tfq_y1<T>(st, n, W.w, W.y2, W.y1, s));
+        HIPCHECK(apply_T<T>(c, trans, W.y1, W.u1, tpos, st, project != 0));
+        HIPCHECK(tfq_v<T>(st, n, W.u1, W.u2, W.v, W.rt, s, part));
+      }
+      HIPCHECK(hipMemcpyAsync(&hs, s, sizeof(TfqmrScal), hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      rs.iters = base_iters + hs.iter;
+      rs.applies += 2 * check_every;
+      if (hs.flag != 0 || rs.iters >= maxiter) break;
+    }
+    base_iters = rs.iters;
+    if (hs.flag == 2 && ++stalls > 3) return fail(HEOMQ_BREAKDOWN, "TFQMR breakdown");
+  }
+  HIPCHECK(apply_T<T>(c, trans, x, W.u1, tpos, st, project != 0));
+  HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, ticket, part)));
+  double* hr = static_cast<double*>(c->pinned);
+  HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  rs.relres = hr[1] > 0 ? std::sqrt(hr[0] / hr[1]) : std::sqrt(hr[0]);
+  if (hr[0] <= tol * tol * hr[1]) return HEOMQ_OK;
+  return fail(HEOMQ_NOT_CONVERGED, "TFQMR did not reach the tolerance");
+}
+
+int inner_maxit() {
+  static const int v = [] {
+    const char* e = std::getenv("HEOMQ_INNER_MAXIT");
+    const int k = e ? std::atoi(e) : 0;
+    return k > 0 ? k : 64;
+  }();
+  return v;
+}
+
+int e_trace_into(heomq_ctx* c, double* b, int64_t n, int trace, hipStream_t st) {
+  HIPCHECK(vec_fill<double>(st, n, b, 0.0));
+  const int64_t pos = tpos_of(c, trace);
+  if (pos < 0) return fail(HEOMQ_BAD_ARG, "steady state needs a trace row");
+  const int64_t off = 2 * (pos * c->d * c->d + (int64_t)c->d * c->d - 1);
+  HIPCHECK(vec_set_elem(st, b, off, 1.0));
+  return HEOMQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int maxiter, int trace,
+                       int transpose, double* stats, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (trace != HEOMQ_TRACE_LAST && trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "trace row required");
+  hipStream_t st = (hipStream_t)stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = 2 * c->N * c->d * c->d;
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64, n, 9, v)) return rc;
+  Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+  const double* bb = b;
+  if (!bb) {
+    if (int rc = e_trace_into(c, v[8], n, trace, st)) return rc;
+    bb = v[8];
+  }
+  RunStats rs;
+  const int rc = tfqmr_run<double>(c, x, bb, W, n, transpose != 0, tpos_of(c, trace), tol, maxiter, st, rs,
+                                   trace == HEOMQ_TRACE_RDO ? 2 : 0);
+  HIPCHECK(hipStreamSynchronize(st));
+  if (stats) {
+    stats[0] = rs.iters;
+    stats[1] = rs.applies;
+    stats[2] = rs.relres;
+    stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return rc;
+}
+
+int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double tol, double inner_tol,
+                             int maxiter, int trace, double* stats, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (trace != HEOMQ_TRACE_LAST && trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "trace row required");
+  hipStream_t st = (hipStream_t)stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = 2 * c->N * c->d * c->d;
+  const int64_t tpos = tpos_of(c, trace);
+  std::vector<double*> v64;
+  if (int rc = carve<double>(c->ws64, n, 2, v64)) return rc;
+  std::vector<float*> v32;
+  if (int rc = carve<float>(c->ws32, n, 10, v32)) return rc;
+  Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6], v32[7]};
+  float* x32 = v32[8];
+  float* r32 = v32[9];
+  double* ax = v64[0];
+  const double* bb = b;
+  if (!bb) {
+    if (int rc = e_trace_into(c, v64[1], n, trace, st)) return rc;
+    bb = v64[1];
+  }
+  double* red = c->red.as<double>();
+  unsigned* ticket = reinterpret_cast<unsigned*>(red + 4);
+  double* part = c->partial.as<double>();
+  int outer = 0, inner_total = 0, applies = 0;
+  double relres = 0;
+  int rc = HEOMQ_NOT_CONVERGED;
+  const bool proj = trace == HEOMQ_TRACE_RDO;
+  for (outer = 0; outer < 50; ++outer) {
+    if (proj) HIPCHECK(vec_project<double>(st, c->N, c->d, x, c->parity.as<int8_t>(), c->tb, true));
+    HIPCHECK(launch_apply_f64(c->fwd.op, x, ax, tpos, st, proj));
+    applies++;
+    HIPCHECK((vec_resid<double, double>(st, n, bb, ax, red, ticket, part)));
+    double* hr = static_cast<double*>(c->pinned);
+    HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    relres = std::sqrt(hr[0] / hr[1]);
+    if (stats && outer < 40) stats[8 + outer] = relres;   // refinement history
+    if (hr[0] <= tol * tol * hr[1]) { rc = HEOMQ_OK; break; }
+    if (inner_total >= maxiter) break;
+    const double rn = std::sqrt(hr[0]);
+    HIPCHECK(vec_resid_to_f32(st, n, bb, ax, r32, 1.0 / rn));
+    HIPCHECK(vec_fill<float>(st, n, x32, 0.0f));
+    RunStats rs;
+    if (stats && outer < 40) stats[48 + outer] = 0;
+    // Each FP32 inner solve is capped: FP32 TFQMR recurrences reach an
+    // attainable-accuracy floor, and the FP64 residual of the next refinement
+    // step is the better restart.
+    const int cap = std::min(inner_maxit(), maxiter - inner_total);
+    const int irc = tfqmr_run<float>(c, x32, r32, W, n, false, tpos, inner_tol, cap, st, rs, proj ? 2 : 0);
+    if (irc != HEOMQ_OK && irc != HEOMQ_NOT_CONVERGED) return irc;
+    inner_total += rs.iters;
+    applies += rs.applies;
+    if (stats && outer < 16) stats[48 + outer] = rs.iters;
+    HIPCHECK(vec_axpy_f32_to_f64(st, n, x, x32, rn));
+  }
+  HIPCHECK(hipStreamSynchronize(st));
+  if (stats) {
+    stats[0] = inner_total;
+    stats[1] = applies;
+    stats[2] = relres;
+    stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats[4] = outer;
+  }
+  if (rc != HEOMQ_OK) return fail(HEOMQ_NOT_CONVERGED, "mixed-precision refinement did not converge");
+  return rc;
+}
+
+int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = 2 * c->N * c->d * c->d;
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64, n, 3, v)) return rc;
+  double *rho0 = v[0], *tmp = v[1], *k = v[2];
+  const double dt2 = dt / 2, dt3 = dt / 3, dt6 = dt / 6;
+  for (int step = 0; step < nstep; ++step) {   // td/intstp.f90:20-100
+    HIPCHECK(hipMemcpyAsync(rho0, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(launch_apply_f64(c->fwd.op, rho0, k, -1, st));
+    HIPCHECK(vec_lincomb3(st, n, tmp, 1.0, rho0, dt2, k, 0.0, nullptr));
+    HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, dt6, k, 0.0, nullptr));
+    HIPCHECK(launch_apply_f64(c->fwd.op, tmp, k, -1, st));
+    HIPCHECK(vec_lincomb3(st, n, tmp, 1.0, rho0, dt2, k, 0.0, nullptr));
+    HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, dt3, k, 0.0, nullptr));
+    HIPCHECK(launch_apply_f64(c->fwd.op, tmp, k, -1, st));
+    HIPCHECK(vec_lincomb3(st, n, tmp, 1.0, rho0, dt, k, 0.0, nullptr));
+    HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, dt3, k, 0.0, nullptr));
+    HIPCHECK(launch_apply_f64(c->fwd.op, tmp, k, -1, st));
+    HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, dt6, k, 0.0, nullptr));
+  }
+  return HEOMQ_OK;
+}
+
+// Chebyshev propagation with the reference's recurrence P_n = P_{n-2} +
+// (2/delta) L P_{n-1} and weights J_n(delta*dt) (td/chebyshev.f90:126-200);
+// delta is the spread of Re(gamma_K) over the hierarchy and the expansion is
+// cut where |J_n| / sum |J_m| < 1e-21 (td/prechebyshev.f90:71-134).
+int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = 2 * c->N * c->d * c->d;
+  std::vector<double> gam(2 * (size_t)c->N);
+  HIPCHECK(hipMemcpy(gam.data(), c->fwd.gam.p, sizeof(double) * gam.size(), hipMemcpyDeviceToHost));
+  double rmax = 0.0, rmin = 0.0;
+  for (int64_t a = 0; a < c->N; ++a) {
+    rmax = std::max(rmax, gam[2 * a]);
+    rmin = std::min(rmin, gam[2 * a]);
+  }
+  const double delta = rmax - rmin;
+  if (!(delta > 0)) return fail(HEOMQ_BAD_ARG, "Chebyshev: zero spectral width");
+  const double arg = delta * dt;
+  std::vector<double> J;
+  double sum = 0.0;
+  for (int k = 0; k < 100000; ++k) {
+    const double j = std::cyl_bessel_j((double)k, arg);
+    J.push_back(j);
+    sum += std::fabs(j);
+    if (k > arg && std::fabs(j) / sum < 1e-21) break;
+  }
+  const int ncheby = (int)J.size();
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64, n, 3, v)) return rc;
+  double *pm2 = v[0], *pm1 = v[1], *lp = v[2];
+  for (int step = 0; step < nstep; ++step) {
+    HIPCHECK(hipMemcpyAsync(pm2, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));   // P0
+    HIPCHECK(vec_lincomb3(st, n, x, J[0], pm2, 0.0, nullptr, 0.0, nullptr));
+    HIPCHECK(launch_apply_f64(c->fwd.op, pm2, lp, -1, st));
+    HIPCHECK(vec_lincomb3(st, n, pm1, 1.0 / delta, lp, 0.0, nullptr, 0.0, nullptr));        // P1
+    if (ncheby > 1) HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, 2.0 * J[1], pm1, 0.0, nullptr));
+    for (int k = 2; k < ncheby; ++k) {
+      HIPCHECK(launch_apply_f64(c->fwd.op, pm1, lp, -1, st));
+      // P_k = P_{k-2} + (2/delta) L P_{k-1}, written over P_{k-2}
+      HIPCHECK(vec_lincomb3(st, n, pm2, 1.0, pm2, 2.0 / delta, lp, 0.0, nullptr));
+      HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, 2.0 * J[k], pm2, 0.0, nullptr));
+      std::swap(pm1, pm2);
+    }
+  }
+  if (stats) {
+    stats[0] = ncheby;
+    stats[1] = delta;
+  }
+  return HEOMQ_OK;
+}
+
+int heomq_observables(heomq_ctx* c, const double* x, double* current, double* occ, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int d = c->d;
+  const int64_t nado = c->L >= 1 ? c->first[2] : 1;
+  std::vector<double> h(2 * (size_t)nado * d * d);
+  HIPCHECK(hipStreamSynchronize(st));
+  HIPCHECK(hipMemcpy(h.data(), x, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  using C = std::complex<double>;
+  auto X = [&](int64_t a, int i, int j) { return C(h[2 * (a * d * d + i + j * d)], h[2 * (a * d * d + i + j * d) + 1]); };
+  auto cv = [&](int orb, int s, int i, int j) { return c->ams[((size_t)(orb * c->nspin + s) * d + j) * d + i]; };
+  if (current) {
+    for (int t = 0; t < c->nalf * c->nspin; ++t) current[t] = 0.0;
+    for (int k = 0; k < c->M; ++k) {
+      if (c->sgn[k] != 0 || c->tier1_of[k] < 0) continue;
+      const int64_t a = c->tier1_of[k];
+      C tr(0, 0);   // tr(c X), X = x_a or fact * x_a^H
+      for (int i = 0; i < d; ++i)
+        for (int kk = 0; kk < d; ++kk) {
+          const double cik = cv(c->orb[k], c->spin[k], i, kk);
+          if (cik == 0.0) continue;
+          const C xv = c->tier1_dag[k] ? (double)c->tier1_fact[k] * std::conj(X(a, i, kk)) : X(a, kk, i);
+          tr += cik * xv;
+        }
+      current[c->alf[k] * c->nspin + c->spin[k]] += -2.0 * tr.imag() * kJconst;   // td/calcj.f90:28-37,68
+    }
+  }
+  if (occ)
+    for (int o = 0; o < c->norbs; ++o)
+      for (int s = 0; s < c->nspin; ++s) {
+        double acc = 0.0;   // tr(c^+ c rho0)
+        for (int i = 0; i < d; ++i)
+          for (int j = 0; j < d; ++j) {
+            double nij = 0.0;
+            for (int k = 0; k < d; ++k) nij += cv(o, s, k, i) * cv(o, s, k, j);
+            if (nij != 0.0) acc += nij * X(0, j, i).real();
+          }
+        occ[o * c->nspin + s] = acc;
+      }
+  return HEOMQ_OK;
+}
+
+static int host_apply(heomq_ctx* c, const double* xh, double* yh, int trace, bool trans) {
+  const int64_t n = 2 * c->N * c->d * c->d;
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64, n, 2, v)) return rc;
+  HIPCHECK(hipMemcpy(v[0], xh, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHECK(launch_apply_f64(trans ? c->trn.op : c->fwd.op, v[0], v[1], tpos_of(c, trace), nullptr));
+  HIPCHECK(hipMemcpy(yh, v[1], sizeof(double) * n, hipMemcpyDeviceToHost));
+  return HEOMQ_OK;
+}
+
+int heomq_calc_ax_tfqmr(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_host) {
+  if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_ax_tfqmr_omp");
+  return host_apply(c, x_host, y_host, HEOMQ_TRACE_LAST, false);
+}
+
+int heomq_calc_atrx(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_host) {
+  if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_atrx_bicg_omp");
+  return host_apply(c, x_host, y_host, HEOMQ_TRACE_LAST, true);
+}
+
+int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host) {
+  if (!c || !rhotmp_host || !rhorhs_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  return host_apply(c, rhotmp_host, rhorhs_host, HEOMQ_TRACE_NONE, false);
+}
+
+}  // extern "C"

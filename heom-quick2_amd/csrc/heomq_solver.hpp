@@ -1,0 +1,60 @@
+// Device scalars and launchers of the TFQMR / propagation vector kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "heomq_kernels.hpp"
+
+namespace heomq {
+
+constexpr int kMaxRedBlocks = 1024;
+
+// TFQMR scalars, resident on the device (Freund 1993 notation).
+struct TfqmrScal {
+  double rho, sigma, alpha, beta;
+  double theta, tau, eta, dcoef;
+  double rr, target;      // ||r0||^2, stop when tau*sqrt(2m+1) <= target
+  int32_t iter, flag;     // flag: 0 running, 1 bound met, 2 breakdown
+  unsigned ticket;        // last-workgroup counter of the fused reductions
+  unsigned pad;
+};
+
+template <typename T>
+hipError_t vec_project(hipStream_t, int64_t nado, int d, T* x, const int8_t* par, const TierBounds& tb,
+                       bool hermitian);
+
+template <typename T>
+hipError_t tfq_init_r(hipStream_t, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, const T* rt,
+                      TfqmrScal* s, double* partial);
+template <typename T>
+hipError_t tfq_init_v(hipStream_t, int64_t n, const T* u1, T* v, const T* rt, TfqmrScal* s, double* partial);
+template <typename T>
+hipError_t tfq_y2(hipStream_t, int64_t n, const T* y1, const T* v, T* y2, const TfqmrScal* s);
+template <typename T>
+hipError_t tfq_half(hipStream_t, int64_t n, T* x, int xupd, T* w, const T* u, T* d, const T* yj,
+                    TfqmrScal* s, double* partial);
+template <typename T>
+hipError_t tfq_rho(hipStream_t, int64_t n, T* x, const T* d, const T* w, const T* rt, TfqmrScal* s,
+                   double* partial);
+template <typename T>
+hipError_t tfq_y1(hipStream_t, int64_t n, const T* w, const T* y2, T* y1, const TfqmrScal* s);
+template <typename T>
+hipError_t tfq_v(hipStream_t, int64_t n, const T* u1, const T* u2, T* v, const T* rt, TfqmrScal* s,
+                 double* partial);
+template <typename T>
+hipError_t vec_fill(hipStream_t, int64_t n, T* v, T val);
+template <typename T>
+hipError_t vec_randn(hipStream_t, int64_t n, T* v, uint64_t seed);
+template <typename TB, typename TA>
+hipError_t vec_resid(hipStream_t, int64_t n, const TB* b, const TA* ax, double* out, unsigned* ticket,
+                     double* partial);
+hipError_t vec_resid_to_f32(hipStream_t, int64_t n, const double* b, const double* ax, float* r32, double scale);
+hipError_t vec_axpy_f32_to_f64(hipStream_t, int64_t n, double* x, const float* dx, double alpha);
+hipError_t tfq_reset(hipStream_t, TfqmrScal* s, double target);
+hipError_t vec_set_elem(hipStream_t, double* v, int64_t off, double val);
+hipError_t vec_lincomb3(hipStream_t, int64_t n, double* out, double a, const double* x, double b,
+                        const double* y, double c, const double* z);
+
+}  // namespace heomq

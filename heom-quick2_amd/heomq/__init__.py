@@ -1,0 +1,334 @@
+"""heomq -- MI355X-native HEOM Liouvillian engine (host binding).
+
+Python face of libheomq.so.  The model tables are built natively
+(``Model``), uploaded once to the GPU (``Context``), and every apply / solve
+runs as HIP kernels on torch-owned device buffers.  Vectors are complex128
+(or complex64) tensors of shape ``(N, d, d)`` in the reference's Fortran
+element order: ``v[a, j, i]`` is element (i, j) of ADO a, i.e. the memory of
+``rcgtmp(dim0, dim0, nunk)`` in tfqmr_double/calc_ax_tfqmr_omp.f90.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes as C
+import weakref
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (HAM_DQD, HAM_SIAM, TRACE_LAST, TRACE_NONE, TRACE_RDO, HeomqError, check, lib)
+
+__all__ = ["Params", "Model", "Context", "siam", "two_impurity", "HeomqError",
+           "TRACE_NONE", "TRACE_LAST", "TRACE_RDO", "HBAR", "JCONST"]
+
+HBAR = 0.658211928
+JCONST = 1.602177e5
+
+
+@dataclass
+class Params:
+    """Physical model (the reference's positional input + $para1/$para4)."""
+    norbs: int = 1
+    nalf: int = 2
+    nmats: int = 8
+    ntop: int = 3
+    bandwidth: tuple = (10.0, 10.0)
+    linewidth: tuple = ((0.1,), (0.1,))
+    temperature: tuple = (0.05, 0.05)
+    engyshift: tuple = ((0.0, 0.0), (0.0, 0.0))
+    ham: int = HAM_SIAM
+    hpar: tuple = ()
+    lsdraw: bool = False
+    steady: bool = True
+
+    def to_c(self) -> _lib.CParams:
+        p = _lib.CParams()
+        p.norbs, p.nspin, p.nalf, p.nmats, p.ntop = self.norbs, 2, self.nalf, self.nmats, self.ntop
+        p.lsdraw, p.ham, p.steady = int(self.lsdraw), int(self.ham), int(self.steady)
+        for a in range(min(self.nalf, 4)):
+            p.bandwidth[a] = self.bandwidth[a]
+            p.temperature[a] = self.temperature[a]
+            for m in range(min(self.norbs, len(self.linewidth[a]), 4)):
+                p.linewidth[a][m] = self.linewidth[a][m]
+            for s in range(2):
+                p.engyshift[a][s] = self.engyshift[a][s]
+        for i, v in enumerate(self.hpar[:16]):
+            p.hpar[i] = v
+        return p
+
+
+def siam(eps=-1.0, U=2.0, gamma=0.1, W=10.0, T=0.05, V=0.0, nk=8, ntop=3, gl=None, gr=None,
+         lsdraw=False) -> Params:
+    """Single-impurity Anderson model; bias split +V/2 on the left lead, -V/2 on the right."""
+    gl = gamma if gl is None else gl
+    gr = gamma if gr is None else gr
+    return Params(norbs=1, nalf=2, nmats=nk, ntop=ntop, bandwidth=(W, W), linewidth=((gl,), (gr,)),
+                  temperature=(T, T), engyshift=((V / 2, V / 2), (-V / 2, -V / 2)), ham=HAM_SIAM,
+                  hpar=(eps, eps, U, 0.0), lsdraw=lsdraw)
+
+
+def two_impurity(eps=-1.5, U=3.0, t=0.5, u12=0.0, gamma=0.2, W=10.0, T=0.1, V=0.4, nk=6, ntop=3,
+                 lsdraw=False) -> Params:
+    """Two-impurity model ($para4): both dots coupled to both leads with the same Gamma."""
+    return Params(norbs=2, nalf=2, nmats=nk, ntop=ntop, bandwidth=(W, W),
+                  linewidth=((gamma, gamma), (gamma, gamma)), temperature=(T, T),
+                  engyshift=((V / 2, V / 2), (-V / 2, -V / 2)), ham=HAM_DQD,
+                  hpar=(eps, eps, eps, eps, U, U, u12, t), lsdraw=lsdraw)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_lib.D)
+
+
+_LIVE_CTX: "weakref.WeakSet" = weakref.WeakSet()
+_LIVE_MODELS: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    """Free device contexts (then models) while the HIP runtime is still up."""
+    for c in list(_LIVE_CTX):
+        c.close()
+    for m in list(_LIVE_MODELS):
+        m.close()
+
+
+class Model:
+    """Host tables: bath coefficients, operators, hierarchy, coupling table."""
+
+    def __init__(self, params: Params | None = None, _handle=None):
+        self._h = C.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            p = params.to_c()
+            check(lib().heomq_model_build(C.byref(p), C.byref(self._h)))
+        self.params = params
+        info = np.zeros(8, np.int64)
+        first = np.zeros(16, np.int64)
+        check(lib().heomq_model_info(self._h, info.ctypes.data_as(_lib.I64), first.ctypes.data_as(_lib.I64)))
+        self.d, self.M, self.L, self.N, self.ncor, self.nopr = (int(v) for v in info[:6])
+        self.n_lower, self.n_upper = int(info[6]), int(info[7])
+        self.first = first[: self.L + 2].copy()
+        _LIVE_MODELS.add(self)
+
+    def close(self):
+        if self._h:
+            lib().heomq_model_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @classmethod
+    def from_reference(cls, **arrays) -> "Model":
+        """Build from the reference's own module arrays (see include/heomq.h)."""
+        a = arrays
+        h = C.c_void_p()
+        keep = []
+
+        def ptr(name, dtype, ctype):
+            v = np.ascontiguousarray(a[name], dtype=dtype)
+            keep.append(v)
+            return v.ctypes.data_as(ctype)
+
+        check(lib().heomq_model_from_reference(
+            int(a["nrho"]), int(a["nunk"]), int(a["ntier"]), int(a["norbs"]), int(a["nspin"]),
+            int(a["nalf"]), int(a["ncor"]), int(a["igroundsteady"]),
+            ptr("indextable", np.int32, _lib.I32), ptr("nfirst", np.int64, _lib.I64),
+            ptr("nlast", np.int64, _lib.I64), ptr("ifirst", np.int64, _lib.I64),
+            ptr("indexcoef", np.int64, _lib.I64), ptr("itypecoef", np.int8, _lib.I8),
+            ptr("ifactfcoef", np.int8, _lib.I8), ptr("ifactrcoef", np.int8, _lib.I8),
+            ptr("index_coef_ref", np.int64, _lib.I64), ptr("indexdraw", np.int32, _lib.I32),
+            ptr("cb", np.complex128, _lib.D), ptr("cd", np.complex128, _lib.D),
+            ptr("cgamma", np.complex128, _lib.D), ptr("dipm", np.float64, _lib.D),
+            ptr("ilead", np.int32, _lib.I32), ptr("mspin", np.int32, _lib.I32),
+            ptr("morbs", np.int32, _lib.I32), ptr("mpm", np.int32, _lib.I32),
+            ptr("eleadinfty", np.float64, _lib.D), ptr("hs", np.complex128, _lib.D),
+            ptr("amsall", np.float64, _lib.D), C.byref(h)))
+        return cls(None, _handle=h)
+
+    def set_params(self, params: Params):
+        p = params.to_c()
+        check(lib().heomq_model_set_params(self._h, C.byref(p)))
+        self.params = params
+
+    def coefs(self):
+        cb = np.zeros(self.M, np.complex128)
+        cd = np.zeros(self.M, np.complex128)
+        g = np.zeros(self.M, np.complex128)
+        check(lib().heomq_model_coefs(self._h, _dp(cb.view(np.float64)), _dp(cd.view(np.float64)),
+                                      _dp(g.view(np.float64))))
+        return cb, cd, g
+
+    def operators(self):
+        Hf = np.zeros(self.d * self.d, np.complex128)
+        nops = self.nopr // (2 * (self.params.nalf if self.params else 2))
+        ams = np.zeros(self.d * self.d * nops, np.float64)
+        check(lib().heomq_model_operators(self._h, _dp(Hf.view(np.float64)), _dp(ams)))
+        H = Hf.reshape(self.d, self.d).T.copy()                     # column-major -> H[i, j]
+        A = ams.reshape(nops, self.d, self.d).transpose(0, 2, 1).copy()   # A[m*nspin+s][i, j]
+        return H, A
+
+    def tier(self, n: int) -> np.ndarray:
+        cnt = int(self.first[n + 1] - self.first[n])
+        out = np.zeros((cnt, n), np.int32)
+        check(lib().heomq_model_tier(self._h, n, out.ctypes.data_as(_lib.I32)))
+        return out
+
+    def coupling(self):
+        lower = np.zeros((self.n_lower_total(), 4), np.int32)
+        upper = np.zeros((int(self.first[self.L]) * self.M, 4), np.int32)
+        check(lib().heomq_model_coupling(self._h, lower.ctypes.data_as(_lib.I32), upper.ctypes.data_as(_lib.I32)))
+        return lower, upper
+
+    def n_lower_total(self) -> int:
+        return self.n_lower
+
+
+class Context:
+    """Device-resident tables on one GPU plus the solver workspace."""
+
+    def __init__(self, model: Model, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("heomq.Context needs a GPU (HIP device); none is visible")
+        self.torch = torch
+        self.model = model
+        self.device = device
+        self._h = C.c_void_p()
+        torch.cuda.set_device(device)
+        check(lib().heomq_ctx_create(model._h, device, C.byref(self._h)))
+        self.N, self.d = model.N, model.d
+        _LIVE_CTX.add(self)
+
+    def close(self):
+        if self._h:
+            self.torch.cuda.synchronize(self.device)
+            lib().heomq_ctx_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- helpers ----
+    def _stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def zeros(self, dtype=None):
+        dtype = dtype or self.torch.complex128
+        return self.torch.zeros((self.N, self.d, self.d), dtype=dtype, device=f"cuda:{self.device}")
+
+    def _check_vec(self, v, dtype):
+        t = self.torch
+        if v.dtype != dtype or not v.is_cuda or not v.is_contiguous() or v.numel() != self.N * self.d * self.d:
+            raise HeomqError(_lib.BAD_DIM, f"expected contiguous {dtype} device tensor with {self.N * self.d * self.d} elements")
+        return C.c_void_p(v.data_ptr())
+
+    def refresh(self, model: Model | None = None):
+        check(lib().heomq_ctx_refresh(self._h, (model or self.model)._h))
+
+    # ---- applies ----
+    def apply(self, x, y=None, trace=TRACE_LAST):
+        t = self.torch
+        y = self.zeros() if y is None else y
+        check(lib().heomq_apply(self._h, self._check_vec(x, t.complex128), self._check_vec(y, t.complex128),
+                                trace, self._stream()))
+        return y
+
+    def apply_transpose(self, x, y=None, trace=TRACE_LAST):
+        t = self.torch
+        y = self.zeros() if y is None else y
+        check(lib().heomq_apply_transpose(self._h, self._check_vec(x, t.complex128),
+                                          self._check_vec(y, t.complex128), trace, self._stream()))
+        return y
+
+    def apply_f32(self, x, y=None, trace=TRACE_LAST):
+        t = self.torch
+        y = self.zeros(t.complex64) if y is None else y
+        check(lib().heomq_apply_f32(self._h, self._check_vec(x, t.complex64), self._check_vec(y, t.complex64),
+                                    trace, self._stream()))
+        return y
+
+    def apply_dparam(self, x, dH, dcb, dcd, dgam, y=None):
+        t = self.torch
+        y = self.zeros() if y is None else y
+        dH = np.asfortranarray(dH, dtype=np.complex128)
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (dcb, dcd, dgam)]
+        check(lib().heomq_apply_dparam(self._h, self._check_vec(x, t.complex128), self._check_vec(y, t.complex128),
+                                       _dp(dH.reshape(-1, order="F").view(np.float64)),
+                                       *[_dp(a.view(np.float64)) for a in arrs], self._stream()))
+        return y
+
+    # ---- solvers ----
+    def initial_guess(self):
+        x = self.zeros()
+        x[0] = self.torch.eye(self.d, dtype=self.torch.complex128, device=x.device) / self.d
+        return x
+
+    def solve_steady(self, x=None, b=None, tol=1e-12, maxiter=20000, trace=TRACE_RDO, transpose=False,
+                     raise_on_fail=True):
+        """TFQMR steady state on the device; returns (x, stats dict)."""
+        t = self.torch
+        x = self.initial_guess() if x is None else x
+        stats = np.zeros(8)
+        bp = self._check_vec(b, t.complex128) if b is not None else None
+        rc = lib().heomq_solve_steady(self._h, self._check_vec(x, t.complex128), bp, tol, maxiter, trace,
+                                      int(transpose), _dp(stats), self._stream())
+        info = dict(iters=int(stats[0]), applies=int(stats[1]), relres=stats[2], seconds=stats[3], status=rc)
+        if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
+            check(rc)
+        return x, info
+
+    def solve_steady_mixed(self, x=None, b=None, tol=1e-12, inner_tol=1e-5, maxiter=20000, trace=TRACE_RDO,
+                           raise_on_fail=True):
+        """FP32 TFQMR inner solves inside FP64 iterative refinement."""
+        t = self.torch
+        x = self.initial_guess() if x is None else x
+        stats = np.zeros(64)
+        bp = self._check_vec(b, t.complex128) if b is not None else None
+        rc = lib().heomq_solve_steady_mixed(self._h, self._check_vec(x, t.complex128), bp, tol, inner_tol,
+                                            maxiter, trace, _dp(stats), self._stream())
+        outer = int(stats[4])
+        info = dict(iters=int(stats[0]), applies=int(stats[1]), relres=stats[2], seconds=stats[3], outer=outer,
+                    history=stats[8:8 + min(outer + 1, 40)].tolist(), inner=stats[48:48 + min(outer, 16)].tolist(),
+                    status=rc)
+        if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
+            raise HeomqError(rc, lib().heomq_last_error().decode() + f" {info}")
+        return x, info
+
+    def rk4(self, x, dt, nstep):
+        check(lib().heomq_rk4(self._h, self._check_vec(x, self.torch.complex128), dt, nstep, self._stream()))
+        return x
+
+    def chebyshev(self, x, dt, nstep):
+        stats = np.zeros(4)
+        check(lib().heomq_chebyshev(self._h, self._check_vec(x, self.torch.complex128), dt, nstep, _dp(stats),
+                                    self._stream()))
+        return x, dict(ncheby=int(stats[0]), delta=stats[1])
+
+    def observables(self, x):
+        p = self.model.params
+        nalf = p.nalf if p else 2
+        norbs = p.norbs if p else 1
+        cur = np.zeros((nalf, 2))
+        occ = np.zeros((norbs, 2))
+        check(lib().heomq_observables(self._h, self._check_vec(x, self.torch.complex128), _dp(cur), _dp(occ),
+                                      self._stream()))
+        return cur, occ
+
+    # ---- reference-named host drop-ins (numpy, Fortran order) ----
+    def calc_ax_tfqmr_omp(self, dim0: int, rcgtmp: np.ndarray, rcgrhs: np.ndarray):
+        """rcgrhs = A rcgtmp with the trace row (calc_ax_tfqmr_omp.f90:1-510)."""
+        check(lib().heomq_calc_ax_tfqmr(self._h, dim0, _dp(rcgtmp.view(np.float64)), _dp(rcgrhs.view(np.float64))))
+
+    def calcderiv_omp(self, rhotmp: np.ndarray, rhorhs: np.ndarray):
+        """rhorhs = L rhotmp, no trace row (td/calcderiv_omp.f90)."""
+        check(lib().heomq_calcderiv(self._h, _dp(rhotmp.view(np.float64)), _dp(rhorhs.view(np.float64))))

@@ -1,0 +1,197 @@
+/*
+ * heomq.h -- C ABI of the MI355X-native HEOM Liouvillian engine.
+ *
+ * One data-parallel hot path of HEOM-QUICK2 (Fortran/OpenMP, MKL) is rebuilt
+ * here for one gfx950 GPU: the action of the fermionic HEOM Liouvillian on the
+ * auxiliary-density-operator (ADO) hierarchy, its real transpose, and the
+ * steady-state / propagation loops that consume them.  Every entry point takes
+ * plain pointers and sizes; device pointers are HIP device addresses, streams
+ * are hipStream_t passed as void*.  No torch types appear here.
+ *
+ * Vector layout: an ADO vector is complex (re, im interleaved) in the
+ * reference's Fortran order x(i, j, a) -- element (i, j) of ADO a at complex
+ * offset i + j*d + a*d*d -- which is also the real (Re, Im)-interleaved TFQMR
+ * vector of tfqmr_double/solve_tfqmr.f90:104-113.  FP64 entry points take
+ * double*, FP32 ones float*.
+ *
+ * Error behaviour mirrors the reference's fatal checks (e.g. dim0 != nrho,
+ * tfqmr_double/calc_ax_tfqmr_omp.f90:44-48) as status codes instead of STOP;
+ * heomq_last_error() gives the message.  Calls on one context are not
+ * reentrant (the reference applies are not either: `save` counters and shared
+ * module scratch, calc_ax_tfqmr_omp.f90:34-35,61).
+ */
+#ifndef HEOMQ_H
+#define HEOMQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  NOT_CONVERGED / BREAKDOWN follow DUTFX's INFO(1) meanings
+ * (tfqmr_double/dutfx.f90:137-147: 4 = no convergence, 8 = breakdown). */
+#define HEOMQ_OK             0
+#define HEOMQ_BAD_DIM        1
+#define HEOMQ_UNSUPPORTED    2
+#define HEOMQ_HIP_ERROR      3
+#define HEOMQ_NOT_CONVERGED  4
+#define HEOMQ_BREAKDOWN      8
+#define HEOMQ_STOPPED       16
+#define HEOMQ_BAD_ARG       32
+
+/* Trace (normalisation) row placement of the steady-state operator. */
+#define HEOMQ_TRACE_NONE  0   /* plain L x (td/calcderiv_omp.f90)                         */
+#define HEOMQ_TRACE_LAST  1   /* y(d,d,last ADO) = tr x0 (calc_ax_tfqmr_omp.f90:492-495)  */
+#define HEOMQ_TRACE_RDO   2   /* y(d,d,tier 0) = tr x0 (a dependent row for every tier)   */
+
+/* Hamiltonian kinds: SIAM ($para1, main/calchs.f90:257-387) and the
+ * two-impurity model ($para4, main/calchs.f90:463-567). */
+#define HEOMQ_HAM_SIAM 0
+#define HEOMQ_HAM_DQD  1
+
+typedef struct heomq_params {
+  int32_t norbs, nspin, nalf;   /* nspin must be 2 */
+  int32_t nmats;                /* Pade terms Nk; one Drude pole is added */
+  int32_t ntop;                 /* top tier L (reference ntier = L + 1)   */
+  int32_t lsdraw;               /* 1: keep ADOs with repeated drawers     */
+  int32_t ham;                  /* HEOMQ_HAM_*                            */
+  int32_t steady;               /* 1: bias enters gamma (igroundsteady=1) */
+  double bandwidth[4];          /* W per lead                              */
+  double linewidth[4][4];       /* Gamma[lead][orbital]                    */
+  double temperature[4];        /* k_B T per lead                          */
+  double engyshift[4][2];       /* bias per [lead][spin]                   */
+  /* SIAM: eup, edown, uu, tupdn.  DQD: e1up, e1down, e2up, e2down, uu1, uu2,
+   * u12, t12.  Energies in meV; divided by hbar inside.                    */
+  double hpar[16];
+} heomq_params;
+
+typedef struct heomq_model heomq_model;   /* host-side tables   */
+typedef struct heomq_ctx   heomq_ctx;     /* device-resident    */
+
+const char* heomq_last_error(void);
+int heomq_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Host model: bath decomposition, fermion operators, H, hierarchy and the
+ * tier-index (coupling) table.  Replaces the preparation phase the reference
+ * runs before its solvers: main/evaluatepara.f90:1157-2232 (counts, drawers,
+ * cb/cd/cgamma), main/zpout_psd.f90 (Pade), main/calchs.f90 (H),
+ * main/calcams.f90 + buildoperator.f90 (c operators), main/ball2drawer.f90 +
+ * fermion/sortindextable3.f90 (hierarchy), main/buildcoefindex.f90 with
+ * findlowertier/findhighertier/findindex (coupling table).
+ * ------------------------------------------------------------------------- */
+int heomq_model_build(const heomq_params* p, heomq_model** out);
+void heomq_model_free(heomq_model* m);
+/* info[0..7] = d, M, L, N (ADOs), ncor, nopr, n_lower_entries, n_upper_entries;
+ * first[0..L+1] = first ADO of each tier (0-based, first[L+1] = N). */
+int heomq_model_info(const heomq_model* m, int64_t* info, int64_t* first);
+/* Per-drawer coefficients (complex, interleaved): cb, cd, gamma (incl. bias). */
+int heomq_model_coefs(const heomq_model* m, double* cb, double* cd, double* gam);
+/* H/hbar (d*d complex, column-major) and c_{m,s} (norbs*nspin*d*d real). */
+int heomq_model_operators(const heomq_model* m, double* H, double* ams);
+/* Drawer tuples of tier n (N_n * n int32, 0-based drawer ids). */
+int heomq_model_tier(const heomq_model* m, int tier, int32_t* tuples);
+/* Coupling table, 0-based: lower[a*n + r] and upper[(a - 0)*M + e] rows of
+ * (nbr, itype, ffront, frear) as int32 quadruples; absent upper entries have
+ * nbr = -1.  lower covers tiers >= 1 (ordered by ADO), upper tiers < L. */
+int heomq_model_coupling(const heomq_model* m, int32_t* lower, int32_t* upper);
+/* Replace physical parameters that leave the hierarchy shape unchanged
+ * (energies, Gamma, T, bias).  Used by sweeps and finite differences. */
+int heomq_model_set_params(heomq_model* m, const heomq_params* p);
+
+/* ---------------------------------------------------------------------------
+ * Drop-in loader from the reference's own module arrays (1-based Fortran
+ * values, as built by the reference's preparation phase; module/matmod.f90).
+ * Called once by the ISO_C_BINDING shim (fortran/heomq_iface.f90) with
+ * c_loc of:  indextable, nfirst/nlast, ifirst (per tier), indexcoef (int64),
+ * itypecoef/ifactfcoef/ifactrcoef (int8), index_coef_ref (int64),
+ * indexdraw (position of each drawer in tier 1), cb/cd (complex,
+ * (norbs,nspin,ncor,nalf,nsgn)), cgamma(nvar), dipm(nvar), ilead/mspin/morbs/
+ * mpm (nvar), eleadinfty(nalf,nspin), hs(nrho,nrho), amsall(nrho,nrho,norbs,
+ * nspin).  Only the default-flag subset is accepted (psdjob, no lscale,
+ * offcor, lscreen, lsimple, lfilter, ltrun_der, lhf, lhb): the shim rejects
+ * the rest with HEOMQ_UNSUPPORTED as the reference would need them.
+ * ------------------------------------------------------------------------- */
+int heomq_model_from_reference(
+    int32_t nrho, int64_t nunk, int32_t ntier, int32_t norbs, int32_t nspin,
+    int32_t nalf, int32_t ncor, int32_t igroundsteady,
+    const int32_t* indextable, const int64_t* nfirst, const int64_t* nlast,
+    const int64_t* ifirst, const int64_t* indexcoef, const int8_t* itypecoef,
+    const int8_t* ifactfcoef, const int8_t* ifactrcoef,
+    const int64_t* index_coef_ref, const int32_t* indexdraw,
+    const double* cb, const double* cd, const double* cgamma,
+    const double* dipm, const int32_t* ilead, const int32_t* mspin,
+    const int32_t* morbs, const int32_t* mpm, const double* eleadinfty,
+    const double* hs, const double* amsall, heomq_model** out);
+
+/* ---------------------------------------------------------------------------
+ * Device context: uploads the tables once (heomq_ctx_load in the reference's
+ * terms) and owns the solver workspace.  dtype: 0 = FP64, 1 = FP32 apply.
+ * ------------------------------------------------------------------------- */
+int heomq_ctx_create(const heomq_model* m, int device, heomq_ctx** out);
+void heomq_ctx_free(heomq_ctx* c);
+/* Re-upload coefficients after heomq_model_set_params (same shape). */
+int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* m);
+/* Number of complex elements N*d*d of one ADO vector. */
+int64_t heomq_ctx_vector_len(const heomq_ctx* c);
+
+/* y = L x on device buffers (FP64), the drop-in for
+ * tfqmr_double/calc_ax_tfqmr_omp.f90 (trace = HEOMQ_TRACE_LAST) and
+ * td/calcderiv_omp.f90 (HEOMQ_TRACE_NONE). */
+int heomq_apply(heomq_ctx* c, const double* x, double* y, int trace, void* stream);
+/* y = L^T x, the real transpose (Re<w, Lx> = Re<L^T w, x>); the drop-in for
+ * bicg/calc_atrx_bicg_omp.f90 up to its unknown convention. */
+int heomq_apply_transpose(heomq_ctx* c, const double* x, double* y, int trace, void* stream);
+/* FP32 (complex64) apply, same layout with float. */
+int heomq_apply_f32(heomq_ctx* c, const float* x, float* y, int trace, void* stream);
+/* y = dL/dtheta x with L's coefficients replaced by their derivatives
+ * (dH, dcb, dcd, dgamma per drawer; complex interleaved; dH d*d). */
+int heomq_apply_dparam(heomq_ctx* c, const double* x, double* y, const double* dH,
+                       const double* dcb, const double* dcd, const double* dgam,
+                       void* stream);
+
+/* Host-pointer apply: the calc_ax_tfqmr_omp(dim0, rcgtmp, rcgrhs) drop-in
+ * (copies in, applies, copies out).  dim0 must equal d. */
+int heomq_calc_ax_tfqmr(heomq_ctx* c, int32_t dim0, const double* x_host,
+                        double* y_host);
+/* Host-pointer transpose (calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs) position). */
+int heomq_calc_atrx(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_host);
+/* calcderiv_omp(tt): rhorhs = L(rhotmp) for fixed H (host pointers). */
+int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host);
+
+/* ---------------------------------------------------------------------------
+ * Steady state: solve A x = e_trace on the device with TFQMR (Freund 1993),
+ * the solver-level drop-in for tfqmr_double/solve_tfqmr.f90 + dutfx.f90.
+ * x (device, in/out): initial guess in, solution out.  Converged when
+ * ||b - A x||_2 <= tol * ||b||_2 on the (Re, Im) vector.  transpose = 1 solves
+ * with A^T (the adjoint solve).  b may be NULL (then b = e_trace).
+ * stats[0] = iterations, stats[1] = applies, stats[2] = final relative
+ * residual, stats[3] = seconds.  Returns HEOMQ_NOT_CONVERGED if maxiter hit.
+ * ------------------------------------------------------------------------- */
+int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol,
+                       int maxiter, int trace, int transpose, double* stats,
+                       void* stream);
+/* Mixed precision: FP32 TFQMR inner solves (tolerance inner_tol) inside FP64
+ * iterative refinement; residuals, updates, dots and norms in FP64. */
+int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double tol,
+                             double inner_tol, int maxiter, int trace,
+                             double* stats, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Propagation (td/intstp.f90 RK4 and td/chebyshev.f90 Chebyshev), fixed H.
+ * x (device) is advanced in place by nstep steps of dt.
+ * ------------------------------------------------------------------------- */
+int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream);
+int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats,
+                    void* stream);
+
+/* Observables from a device state: I[alpha*nspin + s] (td/calcj.f90, times
+ * jconst) and occupation occ[m*nspin + s] (main/calcocc.f90). */
+int heomq_observables(heomq_ctx* c, const double* x, double* current, double* occ,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HEOMQ_H */

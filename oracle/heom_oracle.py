@@ -1,0 +1,790 @@
+"""CPU oracle for the HEOM Liouvillian apply path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker, never the product: only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+it.  It is an independent numpy restatement of the reference algorithm
+(HEOM-QUICK2, Fortran) for one hot path: the action of the fermionic HEOM
+Liouvillian on the auxiliary-density-operator (ADO) hierarchy, its real
+transpose, the steady-state solve built on it and the observables read from it.
+
+Every function cites the reference file:line it restates.  How each function
+is pinned (closed forms, brute force, invariants) is listed in
+``tests/test_oracle.py``; the reference itself cannot be built in this image
+(it needs Intel MKL), so no function is pinned against compiled reference
+output.
+
+Conventions
+-----------
+* Drawers (dissipation modes) are 0-based ids ``k = ((((sgn*nspin + s)*norbs
+  + m)*nalf + a)*ncor + j)`` -- sign slowest, exponent fastest, the order of
+  ``main/evaluatepara.f90:1701-1760``.  ``bar(k)`` flips the sign
+  (``evaluatepara.f90:1766-1772``).
+* An ADO of tier n is a sorted drawer tuple; tiers are stored tier-major, each
+  tier in lexicographic order (``fermion/sortindextable3.f90``).
+* Matrices in this module are indexed mathematically, ``X[a, i, j]`` for ADO a.
+  The Fortran layout of the reference (``x(i, j, a)``, column-major) is the
+  same memory as a C-ordered ``(N, d, d)`` array indexed ``[a, j, i]``;
+  :func:`to_fortran` / :func:`from_fortran` convert.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HBAR = 0.658211928          # include/sizes:23 (meV*ps with funits=1)
+JCONST = 1.602177e5         # include/sizes:27
+
+
+# --------------------------------------------------------------------------
+# Pade spectrum decomposition of the Fermi function
+# --------------------------------------------------------------------------
+def pade_psd(n: int):
+    """Pade [N-1/N] poles/residues of 1/(1+e^x).  Restates main/zpout_psd.f90:34-114.
+
+    Returns (eta, xi): f(x) ~= 1/2 - sum_j 2*eta_j*x / (x^2 + xi_j^2).
+    """
+    dim0, dim1 = 2 * n, 2 * n - 1
+    a0 = np.zeros((dim0, dim0))
+    for i in range(dim0 - 1):           # 1-based (2i-1)(2j-1) off-diagonals
+        v = 1.0 / math.sqrt((2.0 * (i + 1) - 1.0) * (2.0 * (i + 2) - 1.0))
+        a0[i, i + 1] = a0[i + 1, i] = v
+    a1 = np.zeros((dim1, dim1))
+    for i in range(dim1 - 1):           # 1-based (2i+1)(2j+1)
+        v = 1.0 / math.sqrt((2.0 * (i + 1) + 1.0) * (2.0 * (i + 2) + 1.0))
+        a1[i, i + 1] = a1[i + 1, i] = v
+    e0 = np.linalg.eigvalsh(a0)[:n]     # ascending; the n negative ones
+    e1 = np.linalg.eigvalsh(a1)[:n - 1]
+    x0 = 4.0 / e0 ** 2                  # eigXi  (zpout_psd.f90:79-81)
+    x1 = 4.0 / e1 ** 2                  # eigZeta
+    eta = np.ones(n)
+    for j in range(n):                  # zpout_psd.f90:88-96
+        for k in range(j):
+            eta[j] *= (x1[k] - x0[j]) / (x0[k] - x0[j])
+        for k in range(j + 1, n):
+            eta[j] *= (x1[k - 1] - x0[j]) / (x0[k] - x0[j])
+    eta = 0.5 * n * (2 * n + 1) * eta   # zpout_psd.f90:99
+    xi = np.sqrt(x0)
+    return eta, xi
+
+
+def fermi_pade(z, eta, xi):
+    """Pade Fermi function at complex argument.  Restates main/fermifunc.f90:18-23."""
+    z = np.asarray(z, dtype=complex)
+    out = 0.5 + 0j * z
+    for e, x in zip(eta, xi):
+        out = out - e / (z + 1j * x) - e / (z - 1j * x)
+    return out
+
+
+# --------------------------------------------------------------------------
+# Model parameters and derived tables
+# --------------------------------------------------------------------------
+@dataclass
+class Params:
+    norbs: int = 1
+    nspin: int = 2
+    nalf: int = 2
+    nmats: int = 8                      # Pade terms Nk (one Drude pole is added)
+    bandwidth: tuple = (10.0, 10.0)     # W per lead
+    linewidth: tuple = ((0.1,), (0.1,))  # Gamma[lead][orbital]
+    temperature: tuple = (0.05, 0.05)   # k_B T per lead
+    engyshift: tuple = ((0.0, 0.0), (0.0, 0.0))  # bias per [lead][spin]
+    ham: str = "siam"                   # "siam" ($para1) or "dqd" ($para4)
+    hpar: dict = field(default_factory=dict)
+    ntop: int = 3                       # top tier L (reference ntier = L+1)
+    lsdraw: bool = False                # allow repeated drawers (reference default True)
+    steady: bool = True                 # igroundsteady=1: bias enters gamma
+
+
+def siam_params(eps=-1.0, U=2.0, gamma=0.1, W=10.0, T=0.05, V=0.0, nk=8, ntop=3,
+                gl=None, gr=None, lsdraw=False):
+    """Single-impurity Anderson model, bias split +V/2 (left) / -V/2 (right)."""
+    gl = gamma if gl is None else gl
+    gr = gamma if gr is None else gr
+    return Params(norbs=1, nspin=2, nalf=2, nmats=nk, bandwidth=(W, W),
+                  linewidth=((gl,), (gr,)), temperature=(T, T),
+                  engyshift=((V / 2, V / 2), (-V / 2, -V / 2)), ham="siam",
+                  hpar=dict(eup=eps, edown=eps, uu=U), ntop=ntop, lsdraw=lsdraw)
+
+
+def dqd_params(eps=-1.5, U=3.0, t=0.5, u12=0.0, gamma=0.2, W=10.0, T=0.1, V=0.4,
+               nk=6, ntop=3, lsdraw=False):
+    """Two-impurity (serial double dot, both dots coupled to both leads)."""
+    return Params(norbs=2, nspin=2, nalf=2, nmats=nk, bandwidth=(W, W),
+                  linewidth=((gamma, gamma), (gamma, gamma)), temperature=(T, T),
+                  engyshift=((V / 2, V / 2), (-V / 2, -V / 2)), ham="dqd",
+                  hpar=dict(e1up=eps, e1down=eps, e2up=eps, e2down=eps, uu1=U, uu2=U,
+                            u12=u12, t12=t), ntop=ntop, lsdraw=lsdraw)
+
+
+# ---- fermion operators ----------------------------------------------------
+def _decode(idx1, nrho, norbs):
+    """Occupations (up, down) per orbital of 1-based Fock index (swapspin.f90:24-55)."""
+    occ = []
+    table = {0: [0, 0], 1: [1, 0], 2: [0, 1], 3: [1, 1]}
+    for ni in range(1, norbs + 1):
+        len0 = nrho // 4 ** (norbs - ni + 1)
+        occ.append(list(table[(idx1 - 1) // len0 % 4]))
+    return occ
+
+
+def _encode(occ, nrho, norbs):
+    nj = 1
+    for ni in range(norbs, 0, -1):
+        len0 = nrho // 4 ** (norbs - ni + 1)
+        u, dn = occ[ni - 1]
+        nj += len0 * {(0, 0): 0, (1, 0): 1, (0, 1): 2, (1, 1): 3}[(u, dn)]
+    return nj
+
+
+def _swapspin(nrho, norbs, ir, ic, iorbs):
+    """main/swapspin.f90:1-143 (1-based indices in/out)."""
+    out, sg = [], 1
+    for idx in (ir, ic):
+        occ = _decode(idx, nrho, norbs)
+        if occ[iorbs - 1] == [1, 1]:
+            sg = -sg
+        occ[iorbs - 1] = occ[iorbs - 1][::-1]
+        out.append(_encode(occ, nrho, norbs))
+    return out[0], out[1], sg
+
+
+def _swapms(nrho, norbs, ir, ic, iorbs, ispin):
+    """main/swapms.f90:144-229, nspin=2 branch (swap orbitals iorbs-1, iorbs of one spin)."""
+    out, sg = [], 1
+    s = ispin - 1
+    for idx in (ir, ic):
+        occ = _decode(idx, nrho, norbs)
+        if occ[iorbs - 2][s] == 1 and occ[iorbs - 1][s] == 1:
+            sg = -sg
+        occ[iorbs - 2][s], occ[iorbs - 1][s] = occ[iorbs - 1][s], occ[iorbs - 2][s]
+        out.append(_encode(occ, nrho, norbs))
+    return out[0], out[1], sg
+
+
+def annihilators(norbs, nspin=2):
+    """c_{m,s} in the occupation basis, ams[m][s] (d x d real).
+
+    Restates main/calcams.f90:79-103 (c_{1,up}) and main/buildoperator.f90:168-226
+    (the others by spin/orbital swaps with fermionic signs); amsall = amsori
+    (main/checkanticomm.f90:69).
+    """
+    assert nspin == 2
+    nrho = 4 ** norbs
+    c1 = np.zeros((nrho, nrho))
+    len0 = nrho // 4 ** norbs
+    for ni in range(1, 4 ** (norbs - 1) + 1):
+        irow0 = (ni - 1) * len0 * 4 + 1
+        irow1 = irow0 + len0 * 2
+        icol0 = irow0 + len0
+        icol1 = irow0 + len0 * 3
+        for nj in range(len0):
+            c1[irow0 + nj - 1, icol0 + nj - 1] = 1.0
+            c1[irow1 + nj - 1, icol1 + nj - 1] = 1.0
+    ams = [[None, None] for _ in range(norbs)]
+    ams[0][0] = c1
+    ent = {1: [(r + 1, c + 1, 1) for r, c in zip(*np.nonzero(c1))]}
+    # c_{1,down} (buildoperator.f90:190-199)
+    ent2 = []
+    for (r, c, s) in ent[1]:
+        r2, c2, sg = _swapspin(nrho, norbs, r, c, 1)
+        ent2.append((r2, c2, sg * s))
+    cur = {1: ent[1], 2: ent2}
+    ams[0][1] = _mat(ent2, nrho)
+    ispin = 2
+    for ni in range(2, norbs + 1):           # buildoperator.f90:202-226
+        new = []
+        for (r, c, s) in cur[ispin]:
+            r2, c2, sg = _swapms(nrho, norbs, r, c, ni, ispin)
+            new.append((r2, c2, sg * s))
+        cur[ispin] = new
+        ams[ni - 1][ispin - 1] = _mat(new, nrho)
+        ispin = 3 - ispin
+        new = []
+        for (r, c, s) in cur[3 - ispin]:
+            r2, c2, sg = _swapspin(nrho, norbs, r, c, ni)
+            new.append((r2, c2, sg * s))
+        cur[ispin] = new
+        ams[ni - 1][ispin - 1] = _mat(new, nrho)
+    return ams
+
+
+def _mat(ent, nrho):
+    m = np.zeros((nrho, nrho))
+    for r, c, s in ent:
+        m[r - 1, c - 1] = s
+    return m
+
+
+def hamiltonian(p: Params, ams):
+    """System Hamiltonian divided by hbar.
+
+    SIAM: main/calchs.f90:368-387 (para1); two-impurity: calchs.f90:514-567 (para4).
+    """
+    d = ams[0][0].shape[0]
+    h = np.zeros((d, d), dtype=complex)
+    n = lambda m, s: ams[m][s].T @ ams[m][s]
+    hp = p.hpar
+    if p.ham == "siam":
+        h += hp.get("eup", 0.0) * n(0, 0) + hp.get("edown", 0.0) * n(0, 1)
+        h += hp.get("uu", 0.0) * n(0, 0) @ n(0, 1)
+        sf = ams[0][0].T @ ams[0][1]
+        h += hp.get("tupdn", 0.0) * (sf + sf.T)
+    elif p.ham == "dqd":
+        h += hp["e1up"] * n(0, 0) + hp["e1down"] * n(0, 1)
+        h += hp["uu1"] * n(0, 0) @ n(0, 1)
+        h += hp["e2up"] * n(1, 0) + hp["e2down"] * n(1, 1)
+        h += hp["uu2"] * n(1, 0) @ n(1, 1)
+        n1 = n(0, 0) + n(0, 1)
+        n2 = n(1, 0) + n(1, 1)
+        h += hp.get("u12", 0.0) * n1 @ n2
+        hop = (ams[0][0].T @ ams[1][0] + ams[1][0].T @ ams[0][0]
+               + ams[0][1].T @ ams[1][1] + ams[1][1].T @ ams[0][1])
+        h += hp["t12"] * hop
+    else:
+        raise ValueError(p.ham)
+    return h / HBAR
+
+
+class Model:
+    """All tables the apply reads (the reference's module matmod, restated)."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        self.norbs, self.nspin, self.nalf = p.norbs, p.nspin, p.nalf
+        self.ncor = 1 + p.nmats                      # one Drude pole (evaluatepara.f90:1166-1190)
+        self.nopr = 2 * p.norbs * p.nspin * p.nalf   # evaluatepara.f90:1205
+        self.M = self.nopr * self.ncor               # evaluatepara.f90:1231
+        self.d = (2 * p.nspin) ** p.norbs            # evaluatepara.f90:1232
+        self.L = p.ntop
+        self._drawers()
+        self._bath()
+        self.ams = annihilators(p.norbs, p.nspin)
+        self.H = hamiltonian(p, self.ams)
+        self._hierarchy()
+        self._coupling()
+
+    # --- drawer attributes (evaluatepara.f90:1701-1772) ---
+    def _drawers(self):
+        p, M = self.p, None
+        rows = []
+        for sg in range(2):
+            for s in range(p.nspin):
+                for m in range(p.norbs):
+                    for a in range(p.nalf):
+                        for j in range(self.ncor):
+                            rows.append((sg, s, m, a, j))
+        self.attr = np.array(rows, dtype=np.int64)
+        self.sgn, self.spin, self.orb, self.alf, self.cor = self.attr.T
+        half = len(rows) // 2
+        self.bar = np.array([k + half if k < half else k - half for k in range(len(rows))])
+        self.dipm = np.where(self.sgn == 0, 1.0, -1.0)
+        self.opr = np.arange(len(rows)) // self.ncor      # operator id (look4drawer.f90:24)
+
+    # --- bath coefficients (evaluatepara.f90:1897-1982, 2210-2227, 2297, 2431) ---
+    def _bath(self):
+        p = self.p
+        eta, xi = pade_psd(p.nmats)
+        self.pade = (eta, xi)
+        cb = np.zeros(self.M, complex)
+        cd = np.zeros(self.M, complex)
+        cg = np.zeros(self.M, complex)
+        for k in range(self.M):
+            sg, s, m, a, j = self.attr[k]
+            W = p.bandwidth[a]
+            T = p.temperature[a]
+            zxip = p.linewidth[a][m] * W ** 2 * 0.5                 # :1511
+            dpm = 1.0 if sg == 0 else -1.0
+            if j == 0:                                              # Drude pole :1920-1953
+                cg[k] = complex(-W, dpm * 0.0) / HBAR
+                f1 = fermi_pade((1.0 / T) * complex(dpm * 0.0, W), eta, xi)
+                cb[k] = zxip / W * f1 / HBAR ** 2
+                f2 = fermi_pade((1.0 / T) * complex(-dpm * 0.0, W), eta, xi)
+                cd[k] = np.conj(zxip / W * f2) / HBAR ** 2
+            else:                                                   # Pade poles :1964-1982
+                jm = j - 1
+                pole = 1j * xi[jm] if sg == 0 else -1j * xi[jm]     # zpout_psd.f90:110-114
+                pole_b = -pole
+                coef = -eta[jm]
+                cg[k] = dpm * 1j * T * pole / HBAR
+                cb[k] = 1j * 2.0 * coef * zxip * T / ((pole * T - 0.0) ** 2 + W ** 2) / HBAR ** 2
+                cd[k] = np.conj(1j * 2.0 * coef * zxip * T) / np.conj((pole_b * T - 0.0) ** 2 + W ** 2) / HBAR ** 2
+        self.cb, self.cd, self.cgamma = cb, cd, cg
+        self.eleadinfty = np.array(p.engyshift, dtype=float) / HBAR   # fieldtype 0, t -> inf
+        gk = cg.copy()
+        if p.steady:                                              # calc_ax_tfqmr_omp.f90:190-193
+            gk = gk + 1j * self.dipm * self.eleadinfty[self.alf, self.spin]
+        self.gdraw = gk
+
+    # --- hierarchy enumeration (ball2drawer.f90, findoprindex.f90, sortindextable3.f90) ---
+    def _recorded_opset(self, ops):
+        half = self.nopr // 2
+        comp = tuple(sorted(o + half if o < half else o - half for o in ops))
+        return tuple(ops) <= comp
+
+    def _hierarchy(self):
+        M, L = self.M, self.L
+        tiers = [np.zeros((1, 0), dtype=np.int64)]
+        gen = itertools.combinations_with_replacement if self.p.lsdraw else itertools.combinations
+        for n in range(1, L + 1):
+            keep = []
+            for K in gen(range(M), n):
+                ops = tuple(k // self.ncor for k in K)
+                if self._recorded_opset(ops):
+                    keep.append(K)
+            tiers.append(np.array(keep, dtype=np.int64).reshape(-1, n))
+        self.tiers = tiers
+        self.first = np.cumsum([0] + [len(t) for t in tiers])
+        self.N = int(self.first[-1])
+        self.index = [{tuple(K): i for i, K in enumerate(t)} for t in tiers]
+
+    def tuple_of(self, a):
+        n = int(np.searchsorted(self.first, a, side="right") - 1)
+        return n, tuple(self.tiers[n][a - self.first[n]])
+
+    @staticmethod
+    def _parity(seq):
+        inv = 0
+        for i in range(len(seq)):
+            for j in range(i + 1, len(seq)):
+                if seq[i] > seq[j]:
+                    inv += 1
+        return -1 if inv % 2 else 1
+
+    def lookup(self, K):
+        """Stored ADO for drawer tuple K: (global idx, itype, fact) or None.
+
+        Restates main/findindex.f90:51-76 (direct record, then the reversed,
+        sign-flipped complement sorted with swap parity `fact`).
+        """
+        n = len(K)
+        Ks = tuple(sorted(K))
+        i = self.index[n].get(Ks)
+        if i is not None:
+            return int(self.first[n] + i), 0, 1
+        comp = [int(self.bar[k]) for k in reversed(Ks)]
+        fact = self._parity(comp)
+        i = self.index[n].get(tuple(sorted(comp)))
+        if i is None:
+            return None
+        return int(self.first[n] + i), 1, fact
+
+    # --- coupling table (buildcoefindex.f90:203-362, findlowertier/findhighertier) ---
+    def _coupling(self):
+        M, L = self.M, self.L
+        lower, upper = [], []
+        for n in range(0, L + 1):
+            T = self.tiers[n]
+            lo = np.zeros((len(T), n, 3), dtype=np.int64)      # (nbr, itype, ff, fr) packed below
+            lof = np.zeros((len(T), n, 2), dtype=np.int64)
+            for a, K in enumerate(T):
+                K = tuple(K)
+                for r in range(n):
+                    rest = K[:r] + K[r + 1:]
+                    nb, it, fact = self.lookup(rest) if n > 1 else (0, 0, 1)
+                    ff = 1 if r % 2 == 0 else -1                    # buildcoefindex.f90:267-271
+                    fr = 1 if (n - 1 - r) % 2 == 0 else -1          # :272-276
+                    if it == 1:
+                        ff *= fact
+                        fr *= fact
+                    lo[a, r] = (nb, it, 0)
+                    lof[a, r] = (ff, fr)
+            lower.append((lo[..., :2].copy(), lof))
+            if n < L:
+                up = np.full((len(T), M, 2), -1, dtype=np.int64)
+                upf = np.zeros((len(T), M, 2), dtype=np.int64)
+                for a, K in enumerate(T):
+                    K = tuple(K)
+                    for e in range(M):
+                        if not self.p.lsdraw and e in K:
+                            continue                                 # Pauli: no such ADO
+                        fr_list = list(K) + [e]
+                        ff_list = [e] + list(K)
+                        ffact = self._parity(ff_list)                # findhighertier.f90:52-65
+                        rfact = self._parity(fr_list)                # :69-82
+                        res = self.lookup(tuple(K) + (e,))
+                        if res is None:
+                            raise RuntimeError("higher-tier ADO not found")
+                        nb, it, fact = res
+                        up[a, e] = (nb, it)
+                        upf[a, e] = (ffact * fact, rfact * fact)
+                upper.append((up, upf))
+        self.lower, self.upper = lower, upper
+
+    def sizes(self):
+        return [len(t) for t in self.tiers]
+
+
+# --------------------------------------------------------------------------
+# The apply and its transpose
+# --------------------------------------------------------------------------
+def _cop(model, k, left):
+    """Operator multiplying a neighbour for drawer k (lower-tier term)."""
+    c = model.ams[model.orb[k]][model.spin[k]]
+    return c.T if model.sgn[k] == 0 else c
+
+
+def apply(model: Model, x, trace_row="last", H=None, gdraw=None, cb=None, cd=None):
+    """y = L x, the dense TFQMR-convention apply.
+
+    Restates tfqmr_double/calc_ax_tfqmr_omp.f90: same tier :198-203, lower
+    tier :221-271, upper tier :275-357, tier 0 :99-149 (written through the
+    generic upper-tier rule with complement lookups, which is the same sum),
+    trace row :492-495.  ``trace_row`` is "last" (reference: element (d,d)
+    of the last ADO), "rdo" (element (d,d) of the tier-0 block, a row that is
+    linearly dependent in L for every truncation tier), or None
+    (calcderiv_omp: no normalisation row).  Optional H/gdraw/cb/cd override
+    the model's coefficients (used for parameter derivatives).
+    """
+    H = model.H if H is None else H
+    gdraw = model.gdraw if gdraw is None else gdraw
+    cb = model.cb if cb is None else cb
+    cd = model.cd if cd is None else cd
+    d = model.d
+    y = np.zeros_like(x)
+    for n in range(model.L + 1):
+        s0, s1 = model.first[n], model.first[n + 1]
+        X = x[s0:s1]
+        K = model.tiers[n]
+        gam = gdraw[K].sum(axis=1) if n > 0 else np.zeros(len(K), complex)
+        yt = -1j * (H @ X - X @ H) + gam[:, None, None] * X
+        lo, lof = model.lower[n]
+        for r in range(n):
+            nb, it = lo[:, r, 0], lo[:, r, 1]
+            ff, fr = lof[:, r, 0], lof[:, r, 1]
+            Xn = x[nb].copy()
+            Xn[it == 1] = np.conj(np.transpose(Xn[it == 1], (0, 2, 1)))
+            kr = K[:, r]
+            C = np.stack([model.ams[model.orb[k]][model.spin[k]] for k in kr]) if len(kr) else np.zeros((0, d, d))
+            op = np.where((model.sgn[kr] == 0)[:, None, None], np.transpose(C, (0, 2, 1)), C)
+            a = (-1j * ff * cb[kr])[:, None, None]
+            b = (1j * fr * cd[kr])[:, None, None]
+            yt += a * (op @ Xn) + b * (Xn @ op)
+        if n < model.L:
+            up, upf = model.upper[n]
+            for e in range(model.M):
+                nb, it = up[:, e, 0], up[:, e, 1]
+                ok = nb >= 0
+                if not ok.any():
+                    continue
+                Xn = np.zeros((len(K), d, d), complex)
+                Xn[ok] = x[nb[ok]]
+                m = ok & (it == 1)
+                Xn[m] = np.conj(np.transpose(Xn[m], (0, 2, 1)))
+                c = model.ams[model.orb[e]][model.spin[e]]
+                op = c if model.sgn[e] == 0 else c.T                  # :278-279
+                ff = upf[:, e, 0][:, None, None]
+                fr = upf[:, e, 1][:, None, None]
+                yt += -1j * ff * (op @ Xn) + 1j * fr * (Xn @ op)
+        y[s0:s1] = yt
+    if trace_row == "last":
+        y[-1, d - 1, d - 1] = np.trace(x[0])
+    elif trace_row == "rdo":
+        y[0, d - 1, d - 1] = np.trace(x[0])
+    elif trace_row is not None:
+        raise ValueError(trace_row)
+    return y
+
+
+def apply_transpose(model: Model, w, trace_row="last"):
+    """z = L^T w, the real transpose of :func:`apply` (Re<w, Lx> = Re<L^T w, x>).
+
+    Scatter form of the adjoint of each term of calc_ax_tfqmr_omp.f90: a term
+    y_K += a*A*X*B contributes conj(a)*A^H*w_K*B^H to x_J (X = x_J) or
+    a*B*w_K^H*A (X = x_J^H).  The reference's own transpose
+    (bicg/calc_atrx_bicg_omp.f90) is the same operator in BiCG's unknown
+    convention.
+    """
+    H, d = model.H, model.d
+    w = w.copy()
+    tr = 0j
+    if trace_row in ("last", "rdo"):
+        pos = -1 if trace_row == "last" else 0
+        tr = w[pos, d - 1, d - 1]
+        w[pos, d - 1, d - 1] = 0.0
+    z = np.zeros_like(w)
+    for n in range(model.L + 1):
+        s0, s1 = model.first[n], model.first[n + 1]
+        W = w[s0:s1]
+        K = model.tiers[n]
+        gam = model.gdraw[K].sum(axis=1) if n > 0 else np.zeros(len(K), complex)
+        z[s0:s1] += 1j * (H @ W - W @ H) + np.conj(gam)[:, None, None] * W
+        lo, lof = model.lower[n]
+        for r in range(n):
+            nb, it = lo[:, r, 0], lo[:, r, 1]
+            ff, fr = lof[:, r, 0], lof[:, r, 1]
+            kr = K[:, r]
+            C = np.stack([model.ams[model.orb[k]][model.spin[k]] for k in kr]) if len(kr) else np.zeros((0, d, d))
+            op = np.where((model.sgn[kr] == 0)[:, None, None], np.transpose(C, (0, 2, 1)), C)
+            opH = np.transpose(op, (0, 2, 1))
+            a = (-1j * ff * model.cb[kr])[:, None, None]
+            b = (1j * fr * model.cd[kr])[:, None, None]
+            direct = np.conj(a) * (opH @ W) + np.conj(b) * (W @ opH)
+            Wh = np.conj(np.transpose(W, (0, 2, 1)))
+            dag = a * (Wh @ op) + b * (op @ Wh)
+            contrib = np.where((it == 0)[:, None, None], direct, dag)
+            np.add.at(z, nb, contrib)
+        if n < model.L:
+            up, upf = model.upper[n]
+            Wh = np.conj(np.transpose(W, (0, 2, 1)))
+            for e in range(model.M):
+                nb, it = up[:, e, 0], up[:, e, 1]
+                ok = nb >= 0
+                if not ok.any():
+                    continue
+                c = model.ams[model.orb[e]][model.spin[e]]
+                op = c if model.sgn[e] == 0 else c.T
+                ff = upf[:, e, 0][:, None, None]
+                fr = upf[:, e, 1][:, None, None]
+                a, b = -1j * ff, 1j * fr
+                direct = np.conj(a) * (op.T @ W) + np.conj(b) * (W @ op.T)
+                dag = a * (Wh @ op) + b * (op @ Wh)
+                contrib = np.where((it == 0)[:, None, None], direct, dag)
+                np.add.at(z, nb[ok], contrib[ok])
+    if trace_row in ("last", "rdo"):
+        z[0] += tr * np.eye(d)
+    return z
+
+
+# --------------------------------------------------------------------------
+# Layout helpers, observables
+# --------------------------------------------------------------------------
+def to_fortran(x):
+    """(N, d, d) math-indexed -> complex128 array in the reference's x(i,j,a) memory order."""
+    return np.ascontiguousarray(np.transpose(x, (0, 2, 1)))
+
+
+def from_fortran(xf):
+    return np.ascontiguousarray(np.transpose(xf, (0, 2, 1)))
+
+
+def seeded_ado_vector(model: Model, seed: int):
+    """Complex-normal ADO vector; tier-0 block Hermitianised and trace-normalised."""
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((model.N, model.d, model.d)) + 1j * rng.standard_normal((model.N, model.d, model.d))
+    r0 = x[0]
+    while True:
+        r0 = 0.5 * (r0 + r0.conj().T)
+        if abs(np.trace(r0)) >= 0.1:
+            break
+        r0 = rng.standard_normal((model.d, model.d)) + 1j * rng.standard_normal((model.d, model.d))
+    x[0] = r0 / np.trace(r0)
+    return x
+
+
+def current(model: Model, x):
+    """Lead currents I[alpha, spin] = -2 Im sum_{m,j} tr(c_ms rho_(+,m,s,alpha,j)) * jconst.
+
+    Restates td/calcj.f90:15-37,68 and utility/zams_trace_ab.f90.
+    """
+    I = np.zeros((model.nalf, model.nspin))
+    for k in range(model.M):
+        if model.sgn[k] != 0:
+            continue
+        nb, it, fact = model.lookup((k,))
+        X = x[nb] if it == 0 else fact * x[nb].conj().T
+        c = model.ams[model.orb[k]][model.spin[k]]
+        I[model.alf[k], model.spin[k]] += -2.0 * np.imag(np.trace(c @ X))
+    return I * JCONST
+
+
+def occupation(model: Model, x):
+    """<n_{m,s}> = tr(c^+ c rho0) (main/calcocc.f90:20-36)."""
+    occ = np.zeros((model.norbs, model.nspin))
+    for m in range(model.norbs):
+        for s in range(model.nspin):
+            c = model.ams[m][s]
+            occ[m, s] = np.real(np.trace(c.T @ c @ x[0]))
+    return occ
+
+
+# --------------------------------------------------------------------------
+# Solvers
+# --------------------------------------------------------------------------
+def as_real(x):
+    return np.ascontiguousarray(to_fortran(x)).view(np.float64).ravel()
+
+
+def from_real(v, model):
+    return from_fortran(v.view(np.complex128).reshape(model.N, model.d, model.d))
+
+
+def tfqmr(matvec, b, tol=1e-12, maxiter=20000, x0=None, max_restarts=3, project=None,
+          restart_every=None):
+    """Transpose-free QMR (Freund, SIAM J. Sci. Comput. 14 (1993) 470), real vectors.
+
+    Textbook form; the reference drives the same method through its reverse-
+    communication kernel (tfqmr_double/dutfx.f90) with a random shadow vector
+    (dutfx.f90:178).  Stops when the true residual ||b - A x|| <= tol*||b||.
+    When the quasi-residual bound is met but the true residual is not (the
+    attainable-accuracy floor), the iteration restarts from the true residual
+    at most ``max_restarts`` times.  ``project`` (optional) maps the iterate
+    onto an invariant subspace that holds the solution; it is applied at every
+    restart, and ``restart_every`` forces a restart after that many
+    iterations.  Returns (x, iters, relres).
+    """
+    x = np.zeros_like(b) if x0 is None else x0.copy()
+    nb = np.linalg.norm(b)
+    rt = np.random.default_rng(1).standard_normal(b.shape)
+    total = 0
+    best = np.inf
+    nrest = 0
+    while True:
+        if project is not None:
+            x = project(x)
+        r = b - matvec(x)
+        rr = np.linalg.norm(r)
+        if rr <= tol * nb or total >= maxiter:
+            return x, total, rr / nb
+        if rr >= best:
+            nrest += 1
+            if nrest > max_restarts:
+                return x, total, rr / nb
+        best = min(best, rr)
+        start = total
+        w = r.copy()
+        y1 = r.copy()
+        v = matvec(y1)
+        u1 = v.copy()
+        d = np.zeros_like(b)
+        tau = rr
+        theta = eta = 0.0
+        rho = rt @ r
+        while total < maxiter:
+            total += 1
+            sigma = rt @ v
+            alpha = rho / sigma
+            y2 = y1 - alpha * v
+            u2 = matvec(y2)
+            for (yj, uj) in ((y1, u1), (y2, u2)):
+                w = w - alpha * uj
+                d = yj + (theta ** 2 * eta / alpha) * d
+                theta = np.linalg.norm(w) / tau
+                c = 1.0 / math.sqrt(1.0 + theta ** 2)
+                tau = tau * theta * c
+                eta = c ** 2 * alpha
+                x = x + eta * d
+            if tau * math.sqrt(2 * (total - start) + 1) <= tol * nb:
+                break
+            if restart_every and total - start >= restart_every:
+                break
+            rho_n = rt @ w
+            beta = rho_n / rho
+            rho = rho_n
+            y1 = w + beta * y2
+            u1 = matvec(y1)
+            v = u1 + beta * (u2 + beta * v)
+
+
+
+def steady_state(model: Model, tol=1e-12, maxiter=20000, trace_row="rdo"):
+    """Steady state: solve A x = e_trace with A the trace-row apply.
+
+    solve_tfqmr.f90:97-179 solves A*dx = e_trace - A*x0 from a physical
+    trace-one guess x0 and sets rho = x0 + dx; the same is done here with
+    x0 = identity/d on tier 0.  Starting from a trace-one guess keeps the
+    Krylov space inside the Hermitian, parity-allowed, traceless sector.
+    """
+    d = model.d
+    nvec = model.N * d * d * 2
+    x0 = np.zeros((model.N, d, d), complex)
+    x0[0] = np.eye(d) / d
+    e = np.zeros((model.N, d, d), complex)
+    e[-1 if trace_row == "last" else 0, d - 1, d - 1] = 1.0
+    b = as_real(e - apply(model, x0, trace_row=trace_row))
+    proj = lambda v: as_real(physical_projection(model, from_real(v, model)))
+    if trace_row == "rdo":
+        # iterate with Pi*A: A maps the physical sector into itself and the
+        # steady state is its unique solution there
+        mv = lambda v: proj(as_real(apply(model, from_real(v, model), trace_row=trace_row)))
+        xr, it, rel = tfqmr(mv, b, tol=tol, maxiter=maxiter, project=proj)
+    else:
+        mv = lambda v: as_real(apply(model, from_real(v, model), trace_row=trace_row))
+        xr, it, rel = tfqmr(mv, b, tol=tol, maxiter=maxiter, project=proj, restart_every=100)
+    return x0 + from_real(xr, model), it, rel
+
+
+def fock_parity(model: Model):
+    """Electron-number parity of each occupation-basis state (calcams.f90 ordering)."""
+    par = []
+    for i in range(model.d):
+        n, v = 0, i
+        for _ in range(model.norbs):
+            dig = v % 4
+            n += (dig == 1) + (dig == 2) + 2 * (dig == 3)
+            v //= 4
+        par.append(n % 2)
+    return np.array(par)
+
+
+def physical_projection(model: Model, x, hermitian=True):
+    """Project onto the sector that holds every physical ADO vector.
+
+    A tier-n ADO couples states i, j only when parity(i) + parity(j) + n is
+    even (each c or c^+ changes the electron number by one), and the tier-0
+    block is Hermitian.  The Liouvillian maps this sector into itself.
+    """
+    par = fock_parity(model)
+    y = x.copy()
+    if hermitian:
+        y[0] = 0.5 * (y[0] + y[0].conj().T)
+    for n in range(model.L + 1):
+        mask = ((par[:, None] + par[None, :] + n) % 2) == 0
+        y[model.first[n]:model.first[n + 1]] *= mask
+    return y
+
+
+def dense_matrix(model: Model, trace_row="last", transpose=False):
+    """Full real matrix of the apply (tiny models only), column by column."""
+    n = model.N * model.d * model.d * 2
+    A = np.zeros((n, n))
+    f = apply_transpose if transpose else apply
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = as_real(f(model, from_real(e, model), trace_row=trace_row))
+    return A
+
+
+# --------------------------------------------------------------------------
+# Closed form at U = 0: Landauer with the same Lorentzian and Pade Fermi function
+# --------------------------------------------------------------------------
+def landauer_siam(p: Params, npts=400001):
+    """Noninteracting single level: current per spin into the dot from each lead
+    (same units as :func:`current`) and occupation per spin.
+
+    Lead alpha has linewidth Gamma_a W^2/((w-V_a)^2+W^2) and Fermi function
+    f_pade((w-V_a)/T), i.e. the rigidly shifted band that the bias gauge term
+    i*dipm*V/hbar in gamma_K represents.  Integrated on omega = s*tan(theta).
+    """
+    eta, xi = pade_psd(p.nmats)
+    eps = p.hpar["eup"]
+    th = np.linspace(-np.pi / 2, np.pi / 2, npts)[1:-1]
+    sc = max(p.bandwidth) * 0.5
+    w = sc * np.tan(th)
+    jac = sc / np.cos(th) ** 2
+    G = []
+    S = np.zeros_like(w, dtype=complex)
+    F = []
+    for a in range(p.nalf):
+        W = p.bandwidth[a]
+        V = p.engyshift[a][0]
+        ga = p.linewidth[a][0]
+        Ga = ga * W ** 2 / ((w - V) ** 2 + W ** 2)
+        S += 0.5 * ga * W / (w - V + 1j * W)
+        G.append(Ga)
+        F.append(np.real(fermi_pade((w - V) / p.temperature[a], eta, xi)))
+    den = np.abs(w - eps - S) ** 2
+    h = th[1] - th[0]
+    # composite Simpson on the uniform theta grid
+    wts = np.ones_like(th)
+    wts[1:-1:2] = 4
+    wts[2:-1:2] = 2
+    wts *= h / 3
+    trans = G[0] * G[1] / den
+    i_l = np.sum(wts * jac * trans * (F[0] - F[1])) / (2 * np.pi) / HBAR
+    n_occ = np.sum(wts * jac * (G[0] * F[0] + G[1] * F[1]) / den) / (2 * np.pi)
+    return i_l * JCONST, n_occ

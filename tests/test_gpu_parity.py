@@ -1,0 +1,111 @@
+"""GPU parity: the HIP apply / transpose / solvers against the CPU oracle.
+
+Every comparison runs the native library through its C ABI on cuda:0 and
+checks it against oracle/heom_oracle.py on the same seeded inputs.  FP64
+applies are held to 1e-12 relative, steady-state RDO and
+current to 1e-10 against the oracle's own TFQMR solution.
+"""
+import numpy as np
+import pytest
+
+import heomq
+from oracle import heom_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "siam_nk3_L3": (lambda: heomq.siam(nk=3, ntop=3, V=0.3, U=1.5), lambda: O.siam_params(nk=3, ntop=3, V=0.3, U=1.5)),
+    "siam_nk2_L2_rep": (lambda: heomq.siam(nk=2, ntop=2, V=-0.4, lsdraw=True), lambda: O.siam_params(nk=2, ntop=2, V=-0.4, lsdraw=True)),
+    "dqd_nk2_L2": (lambda: heomq.two_impurity(nk=2, ntop=2), lambda: O.dqd_params(nk=2, ntop=2)),
+}
+
+_cache = {}
+
+
+def setup(name):
+    if name not in _cache:
+        import torch
+        pf, of = CASES[name]
+        m = heomq.Model(pf())
+        ctx = heomq.Context(m)
+        o = O.Model(of())
+        _cache[name] = (m, ctx, o, torch)
+    return _cache[name]
+
+
+def to_dev(torch, x):
+    return torch.from_numpy(O.to_fortran(x)).to("cuda:0")
+
+
+def from_dev(t):
+    return O.from_fortran(t.cpu().numpy())
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("trace", ["last", "rdo", None])
+def test_apply_fp64(name, trace):
+    m, ctx, o, torch = setup(name)
+    print(f"case {name} trace {trace}", flush=True)
+    x = O.seeded_ado_vector(o, 11)
+    tr = {"last": heomq.TRACE_LAST, "rdo": heomq.TRACE_RDO, None: heomq.TRACE_NONE}[trace]
+    y = from_dev(ctx.apply(to_dev(torch, x), trace=tr))
+    yo = O.apply(o, x, trace_row=trace)
+    assert rel(y, yo) <= 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("trace", ["last", "rdo", None])
+def test_apply_transpose_fp64(name, trace):
+    m, ctx, o, torch = setup(name)
+    w = O.seeded_ado_vector(o, 12)
+    tr = {"last": heomq.TRACE_LAST, "rdo": heomq.TRACE_RDO, None: heomq.TRACE_NONE}[trace]
+    z = from_dev(ctx.apply_transpose(to_dev(torch, w), trace=tr))
+    zo = O.apply_transpose(o, w, trace_row=trace)
+    assert rel(z, zo) <= 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_apply_fp32(name):
+    m, ctx, o, torch = setup(name)
+    x = O.seeded_ado_vector(o, 13)
+    xd = to_dev(torch, x).to(torch.complex64)
+    y = from_dev(ctx.apply_f32(xd, trace=heomq.TRACE_RDO)).astype(np.complex128)
+    yo = O.apply(o, O.from_fortran(xd.cpu().numpy()).astype(np.complex128), trace_row="rdo")
+    assert rel(y, yo) <= 2e-6
+
+
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
+def test_steady_state_fp64(name):
+    m, ctx, o, torch = setup(name)
+    xs, st = ctx.solve_steady(tol=1e-12, raise_on_fail=False)
+    print(name, "device", st, flush=True)
+    assert st["status"] == 0
+    xs = from_dev(xs)
+    xo, it, relres = O.steady_state(o, tol=1e-12)
+    print(name, st, it, relres, flush=True)
+    assert np.abs(xs[0] - xo[0]).max() <= 1e-10
+    cur, occ = ctx.observables(to_dev(torch, xs))
+    assert np.abs(cur - O.current(o, xo)).max() <= 1e-10 * max(1.0, np.abs(O.current(o, xo)).max())
+    assert np.abs(occ - O.occupation(o, xo)).max() <= 1e-10
+
+
+def test_steady_state_mixed():
+    m, ctx, o, torch = setup("siam_nk3_L3")
+    x64, _ = ctx.solve_steady(tol=1e-12)
+    xm, st = ctx.solve_steady_mixed(tol=1e-12, inner_tol=1e-5)
+    print(st, flush=True)
+    assert st["outer"] >= 2
+    assert (xm[0] - x64[0]).abs().max().item() <= 1e-10
+
+
+def test_dot_product_transpose_device():
+    m, ctx, o, torch = setup("dqd_nk2_L2")
+    x = to_dev(torch, O.seeded_ado_vector(o, 21))
+    w = to_dev(torch, O.seeded_ado_vector(o, 22))
+    lhs = torch.vdot(w.flatten(), ctx.apply(x, trace=heomq.TRACE_LAST).flatten()).real.item()
+    rhs = torch.vdot(ctx.apply_transpose(w, trace=heomq.TRACE_LAST).flatten(), x.flatten()).real.item()
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)

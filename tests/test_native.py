@@ -1,0 +1,131 @@
+"""CPU tests of the native library: exports, host tables against the oracle,
+the reference-array drop-in loader and the error behaviour (no GPU needed)."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import heomq
+from heomq import _lib
+from oracle import heom_oracle as O
+from ref_arrays import reference_arrays
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def header_symbols():
+    text = (ROOT / "include" / "heomq.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(heomq_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    h = ctypes.CDLL(str(_lib.LIB_PATH))
+    syms = header_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(h, s)]
+    assert not missing, missing
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+
+
+def test_fortran_shim_exports_reference_names():
+    path = ROOT / "heom-quick2_amd" / "heomq" / "libheomq_fortran.so"
+    if not path.exists():
+        pytest.skip("Fortran compiler not available at build time")
+    h = ctypes.CDLL(str(path))
+    for name in ("calc_ax_tfqmr_omp_", "calc_atrx_bicg_omp_", "calcderiv_omp_", "heomq_fortran_init_",
+                 "solve_tfqmr_gpu_"):
+        assert hasattr(h, name), name
+
+
+CASES = [
+    (heomq.siam(nk=3, ntop=3, V=0.3, U=1.5), O.siam_params(nk=3, ntop=3, V=0.3, U=1.5)),
+    (heomq.two_impurity(nk=2, ntop=2), O.dqd_params(nk=2, ntop=2)),
+    (heomq.siam(nk=2, ntop=3, lsdraw=True), O.siam_params(nk=2, ntop=3, lsdraw=True)),
+    (heomq.siam(nk=1, ntop=4, T=0.2), O.siam_params(nk=1, ntop=4, T=0.2)),
+]
+
+
+@pytest.mark.parametrize("pp,op", CASES)
+def test_native_tables_match_oracle(pp, op):
+    m = heomq.Model(pp)
+    o = O.Model(op)
+    assert m.N == o.N and list(m.first) == list(o.first)
+    cb, cd, g = m.coefs()
+    assert np.abs(cb - o.cb).max() <= 1e-13 * np.abs(o.cb).max()
+    assert np.abs(cd - o.cd).max() <= 1e-13 * np.abs(o.cd).max()
+    assert np.abs(g - o.gdraw).max() <= 1e-12 * np.abs(o.gdraw).max()
+    H, A = m.operators()
+    assert np.abs(H - o.H).max() == 0.0
+    for mm in range(o.norbs):
+        for s in range(2):
+            assert np.array_equal(A[mm * 2 + s], o.ams[mm][s])
+    for n in range(1, m.L + 1):
+        assert np.array_equal(m.tier(n), o.tiers[n])
+    lo, up = m.coupling()
+    olo = np.concatenate([np.concatenate([o.lower[n][0], o.lower[n][1]], axis=2).reshape(-1, 4)
+                          for n in range(1, o.L + 1)])
+    assert np.array_equal(lo, olo)
+    oup = np.concatenate([np.concatenate([o.upper[n][0], o.upper[n][1]], axis=2).reshape(-1, 4)
+                          for n in range(0, o.L)])
+    mask = oup[:, 0] >= 0
+    assert np.array_equal(up[mask], oup[mask]) and (up[~mask, 0] == -1).all()
+
+
+def test_pade_native_matches_numpy_eigensolver():
+    # the native Sturm-bisection eigenvalues reproduce LAPACK's through cb/cgamma
+    for nk in (1, 4, 8, 16, 30):
+        m = heomq.Model(heomq.siam(nk=nk, ntop=1))
+        o = O.Model(O.siam_params(nk=nk, ntop=1))
+        _, _, g = m.coefs()
+        assert np.abs(g - o.gdraw).max() <= 1e-12 * np.abs(o.gdraw).max()
+
+
+@pytest.mark.parametrize("pp,op", [CASES[2], (heomq.two_impurity(nk=1, ntop=2, lsdraw=True),
+                                              O.dqd_params(nk=1, ntop=2, lsdraw=True))])
+def test_reference_loader_round_trip(pp, op):
+    o = O.Model(op)
+    m_ref = heomq.Model.from_reference(**reference_arrays(o))
+    m = heomq.Model(pp)
+    assert m_ref.N == m.N and list(m_ref.first) == list(m.first)
+    lo1, up1 = m_ref.coupling()
+    lo2, up2 = m.coupling()
+    assert np.array_equal(lo1, lo2)
+    assert np.array_equal(up1, up2)
+    assert np.abs(m_ref.coefs()[0] - m.coefs()[0]).max() <= 1e-13 * np.abs(m.coefs()[0]).max()
+    assert np.abs(m_ref.coefs()[2] - m.coefs()[2]).max() <= 1e-13 * np.abs(m.coefs()[2]).max()
+
+
+def test_reference_loader_rejects_bad_dims():
+    o = O.Model(O.siam_params(nk=1, ntop=2, lsdraw=True))
+    arr = reference_arrays(o)
+    arr["nrho"] = 8
+    with pytest.raises(heomq.HeomqError) as e:
+        heomq.Model.from_reference(**arr)
+    assert e.value.code == _lib.BAD_DIM
+
+
+@pytest.mark.parametrize("bad", [dict(norbs=4), dict(nmats=0), dict(ntop=0)])
+def test_bad_params_raise(bad):
+    p = heomq.siam()
+    for k, v in bad.items():
+        setattr(p, k, v)
+    with pytest.raises(heomq.HeomqError):
+        heomq.Model(p)
+
+
+def test_set_params_keeps_shape():
+    m = heomq.Model(heomq.siam(nk=2, ntop=2))
+    m.set_params(heomq.siam(nk=2, ntop=2, V=0.5, eps=-0.2))
+    o = O.Model(O.siam_params(nk=2, ntop=2, V=0.5, eps=-0.2))
+    assert np.abs(m.coefs()[2] - o.gdraw).max() <= 1e-12 * np.abs(o.gdraw).max()
+    with pytest.raises(heomq.HeomqError):
+        m.set_params(heomq.siam(nk=3, ntop=2))
+
+
+def test_cfg_sizes():
+    # hierarchy sizes of the benchmark configurations (unique-index, Hermitian-paired)
+    assert heomq.Model(heomq.siam()).N == 31297
+    assert heomq.Model(heomq.two_impurity()).N == 117321

@@ -1,0 +1,218 @@
+"""Pins of the CPU oracle (oracle/heom_oracle.py) by closed forms, brute force and invariants.
+
+The reference (Fortran + Intel MKL) cannot be built in this image and holds no
+golden vectors, so each oracle function is checked against mathematics a
+plausible mistake would break:
+
+* Pade decomposition       -> the Fermi function itself; pole count/positivity.
+* fermion operators        -> canonical anticommutation relations.
+* hierarchy enumeration    -> brute-force counts; every tuple stored or its
+                              Hermitian partner stored, never both for odd tiers.
+* coupling signs           -> permutation parity recomputed by brute force.
+* apply                    -> trace preservation, Hermiticity of (Lx)_0,
+                              dot-product test against the transpose, dense
+                              brute-force transpose on a tiny model.
+* steady state (U = 0)     -> Landauer closed form with the same Lorentzian
+                              band and the same Pade Fermi function (the
+                              noninteracting HEOM is exact from tier 2 on).
+* steady state (U > 0)     -> tr rho = 1, Hermitian, I_L = -I_R, I(V=0) = 0,
+                              <n> = 1 at particle-hole symmetry.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from oracle import heom_oracle as O
+
+
+# ---------------------------------------------------------------- Pade / bath
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 12])
+def test_pade_matches_fermi(n):
+    eta, xi = O.pade_psd(n)
+    assert np.all(eta > 0) and np.all(np.diff(xi) > 0)
+    x = np.linspace(-2.0 * n, 2.0 * n, 401)
+    f = 1.0 / (1.0 + np.exp(x))
+    err = np.abs(O.fermi_pade(x, eta, xi).real - f).max()
+    # [N-1/N] Pade is accurate well inside |x| < 2N; bound tightens with N
+    assert err < {1: 0.2, 2: 0.05, 4: 2e-3, 8: 1e-5, 12: 1e-7}[n]
+
+
+def test_pade_first_pole_near_matsubara():
+    # the lowest Pade pole converges to the lowest Matsubara frequency pi
+    eta, xi = O.pade_psd(10)
+    assert abs(xi[0] - math.pi) < 1e-10 and abs(eta[0] - 1.0) < 1e-10
+
+
+def test_bath_coefficients_reproduce_correlation():
+    # C(t) = sum_k cb_k exp(gamma_k t) against direct quadrature of
+    # (1/pi) int J(w) f_pade(w/T) e^{i w t} dw / hbar^2 (Lorentzian J)
+    p = O.siam_params(nk=6, T=0.3, W=4.0, gamma=0.7)
+    m = O.Model(p)
+    eta, xi = m.pade
+    W, T, G = 4.0, 0.3, 0.7
+    w = np.linspace(-2000, 2000, 4_000_001)
+    J = 0.5 * G * W ** 2 / (w ** 2 + W ** 2)
+    f = O.fermi_pade(w / T, eta, xi).real
+    for t in (0.05, 0.3, 1.0):
+        # the Pade Fermi function tends to 1/2 at large |w|; that part is done in closed form
+        tail = 0.25 * G * W * np.exp(-W * t / O.HBAR)
+        ref = (np.trapezoid(J * (f - 0.5) * np.exp(1j * w * t / O.HBAR), w) / np.pi + tail) / O.HBAR ** 2
+        sel = (m.sgn == 0) & (m.alf == 0) & (m.spin == 0)
+        ours = np.sum(m.cb[sel] * np.exp(m.cgamma[sel] * t))
+        assert abs(ours - ref) < 2e-5 * abs(ref)
+
+
+# ---------------------------------------------------------------- operators
+@pytest.mark.parametrize("norbs", [1, 2, 3])
+def test_anticommutators(norbs):
+    ams = O.annihilators(norbs)
+    ops = [ams[m][s] for m in range(norbs) for s in range(2)]
+    d = ops[0].shape[0]
+    for i, a in enumerate(ops):
+        for j, b in enumerate(ops):
+            assert np.abs(a @ b + b @ a).max() == 0
+            assert np.abs(a @ b.T + b.T @ a - (i == j) * np.eye(d)).max() == 0
+
+
+def test_siam_operator_convention():
+    # c_up = |0><u| + |d><2|, c_down = |0><d| - |u><2|  (calcams.f90 + swapspin.f90)
+    ams = O.annihilators(1)
+    cu, cd = ams[0]
+    assert cu[0, 1] == 1 and cu[2, 3] == 1 and np.count_nonzero(cu) == 2
+    assert cd[0, 2] == 1 and cd[1, 3] == -1 and np.count_nonzero(cd) == 2
+
+
+# ---------------------------------------------------------------- hierarchy
+@pytest.mark.parametrize("nk,ntop,lsdraw", [(1, 3, False), (2, 3, False), (1, 3, True), (2, 2, True)])
+def test_hierarchy_counts_bruteforce(nk, ntop, lsdraw):
+    m = O.Model(O.siam_params(nk=nk, ntop=ntop, lsdraw=lsdraw))
+    gen = itertools.combinations_with_replacement if lsdraw else itertools.combinations
+    for n in range(1, ntop + 1):
+        stored = set(map(tuple, m.tiers[n]))
+        allk = set(gen(range(m.M), n))
+        comp = {tuple(sorted(m.bar[k] for k in K)) for K in stored}
+        assert stored | comp == allk                     # every ADO reachable
+        if n % 2 == 1:
+            assert not (stored & comp)                    # odd tiers: exactly one of each pair
+            assert 2 * len(stored) == len(allk)
+        if not lsdraw:
+            assert len(allk) == math.comb(m.M, n)
+
+
+def test_coupling_signs_bruteforce():
+    m = O.Model(O.siam_params(nk=2, ntop=3))
+
+    def perm_sign(seq):
+        s = 1
+        seq = list(seq)
+        for i in range(len(seq)):
+            for j in range(i + 1, len(seq)):
+                if seq[i] > seq[j]:
+                    s = -s
+        return s
+
+    for n in range(0, m.L):
+        up, upf = m.upper[n]
+        for a, K in enumerate(m.tiers[n][:40]):
+            K = tuple(K)
+            for e in range(m.M):
+                nb = up[a, e, 0]
+                if e in K:
+                    assert nb == -1
+                    continue
+                target = tuple(sorted(K + (e,)))
+                _, T = m.tuple_of(nb)
+                if up[a, e, 1] == 0:
+                    assert T == target and upf[a, e, 0] == perm_sign((e,) + K)
+                else:
+                    # stored partner: reversed, sign-flipped, sorted with parity
+                    rev = [m.bar[k] for k in reversed(target)]
+                    assert tuple(sorted(rev)) == T
+                    assert upf[a, e, 0] == perm_sign((e,) + K) * perm_sign(rev)
+                    assert upf[a, e, 1] == perm_sign(K + (e,)) * perm_sign(rev)
+
+
+# ---------------------------------------------------------------- apply
+SMALL = [O.siam_params(nk=2, ntop=3, V=0.3, U=1.5), O.dqd_params(nk=1, ntop=2)]
+
+
+@pytest.mark.parametrize("p", SMALL)
+def test_trace_preservation_and_hermiticity(p):
+    m = O.Model(p)
+    x = O.seeded_ado_vector(m, 3)
+    y = O.apply(m, x, trace_row=None)
+    assert abs(np.trace(y[0])) < 1e-12 * np.abs(y).max()
+    assert np.abs(y[0] - y[0].conj().T).max() < 1e-12 * np.abs(y).max()
+
+
+@pytest.mark.parametrize("p", SMALL)
+@pytest.mark.parametrize("trace", [None, "last", "rdo"])
+def test_dot_product_transpose(p, trace):
+    m = O.Model(p)
+    x = O.seeded_ado_vector(m, 4)
+    w = O.seeded_ado_vector(m, 5)
+    lhs = np.real(np.vdot(w, O.apply(m, x, trace_row=trace)))
+    rhs = np.real(np.vdot(O.apply_transpose(m, w, trace_row=trace), x))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_dense_transpose_bruteforce():
+    m = O.Model(O.siam_params(nk=1, ntop=2, V=0.2))
+    A = O.dense_matrix(m, trace_row="last")
+    AT = O.dense_matrix(m, trace_row="last", transpose=True)
+    assert np.abs(A.T - AT).max() < 1e-13 * np.abs(A).max()
+
+
+def test_apply_linear_and_real_linear():
+    m = O.Model(SMALL[0])
+    x = O.seeded_ado_vector(m, 6)
+    z = O.seeded_ado_vector(m, 7)
+    y = O.apply(m, 2.5 * x - 0.5 * z)
+    assert np.abs(y - (2.5 * O.apply(m, x) - 0.5 * O.apply(m, z))).max() < 1e-11 * np.abs(y).max()
+    # not complex-linear: the Hermitian-partner terms conjugate their input
+    yi = O.apply(m, 1j * x, trace_row=None)
+    assert np.abs(yi - 1j * O.apply(m, x, trace_row=None)).max() > 1e-3
+
+
+# ---------------------------------------------------------------- steady state
+@pytest.mark.parametrize("ntop", [2, 3])
+def test_noninteracting_landauer(ntop):
+    p = O.siam_params(nk=2, ntop=ntop, U=0.0, eps=-0.3, V=0.5, gamma=0.3, T=0.2)
+    m = O.Model(p)
+    x, it, rel = O.steady_state(m, tol=1e-12)
+    assert rel <= 1e-12
+    i_l, n_occ = O.landauer_siam(p)
+    I = O.current(m, x)
+    assert np.abs(I[0] - i_l).max() <= 1e-9 * abs(i_l)
+    assert np.abs(I[1] + i_l).max() <= 1e-9 * abs(i_l)
+    assert np.abs(O.occupation(m, x) - n_occ).max() <= 1e-9
+
+
+def test_landauer_zero_bias():
+    p = O.siam_params(nk=2, ntop=2, U=0.0, eps=-0.3, V=0.0, gamma=0.3, T=0.2)
+    i_l, _ = O.landauer_siam(p)
+    assert abs(i_l) < 1e-9
+
+
+def test_interacting_invariants():
+    p = O.siam_params(nk=2, ntop=3, U=2.0, eps=-1.0, V=0.0, gamma=0.2, T=0.1)
+    m = O.Model(p)
+    x, it, rel = O.steady_state(m, tol=1e-12)
+    assert abs(np.trace(x[0]) - 1) < 1e-11
+    assert np.abs(x[0] - x[0].conj().T).max() < 1e-12
+    assert np.abs(O.current(m, x)).max() < 1e-7                  # V = 0
+    assert abs(O.occupation(m, x).sum() - 1.0) < 1e-10           # particle-hole symmetric
+    p2 = O.siam_params(nk=2, ntop=3, U=2.0, eps=-0.7, V=0.4, gamma=0.2, T=0.1)
+    m2 = O.Model(p2)
+    x2, _, _ = O.steady_state(m2, tol=1e-12)
+    I = O.current(m2, x2)
+    assert np.abs(I[0] + I[1]).max() < 1e-8 * np.abs(I).max()  # current conservation
+
+
+def test_trace_row_placements_agree_for_odd_top_tier():
+    m = O.Model(O.siam_params(nk=2, ntop=3, V=0.3, U=1.5))
+    xr, _, _ = O.steady_state(m, tol=1e-12, trace_row="rdo")
+    xl, _, _ = O.steady_state(m, tol=1e-12, trace_row="last")
+    assert np.abs(xr[0] - xl[0]).max() < 1e-10
