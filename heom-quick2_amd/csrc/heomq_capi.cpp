@@ -7,6 +7,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <memory>
 #include <vector>
 
@@ -181,6 +183,245 @@ void fill_op(heomq_ctx* c, OperatorTables& t, const DevBuf& rec, bool transposed
   o.tb = c->tb;
 }
 
+// Symmetry-packed layout: sectors, packed offsets, packed coupling entries.
+int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
+                 const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt) {
+  std::vector<DevRec> recs;
+  build_records(m, recs);
+  std::vector<int8_t> omap;
+  build_opmap(m, omap);
+  const int d = m.d, dd = d * d;
+  std::vector<int> fN(d), fS(d);
+  for (int i = 0; i < d; ++i) {
+    int n = 0, s2 = 0, v = i;
+    for (int o = 0; o < m.norbs; ++o, v /= 4) {
+      const int dig = v % 4;
+      const int up = dig == 1 || dig == 3, dn = dig == 2 || dig == 3;
+      n += up + dn;
+      s2 += up - dn;
+    }
+    fN[i] = n;
+    fS[i] = s2;
+  }
+  c->cons_n = c->cons_s = true;
+  for (int i = 0; i < d; ++i)
+    for (int k = 0; k < d; ++k)
+      if (m.H[i + (size_t)k * d] != cplx(0, 0)) {
+        if (fN[i] != fN[k]) c->cons_n = false;
+        if (fS[i] != fS[k]) c->cons_s = false;
+      }
+  // per-ADO (dN, d2S)
+  std::vector<std::pair<int, int>> key(m.N, {0, 0});
+  for (int n = 1; n <= m.L; ++n)
+    for (int64_t g = m.first[n]; g < m.first[n + 1]; ++g) {
+      const int32_t* K = &m.tiers[n][(g - m.first[n]) * n];
+      int dN = 0, dS = 0;
+      for (int r = 0; r < n; ++r) {
+        const int q = m.sgn[K[r]] == 0 ? 1 : -1;
+        dN += q;
+        dS += q * (m.spin[K[r]] == 0 ? 1 : -1);
+      }
+      if (!c->cons_n) dN = ((dN % 2) + 2) % 2;
+      if (!c->cons_s) dS = 0;
+      key[g] = {dN, dS};
+    }
+  std::vector<std::pair<int, int>> keys;
+  std::vector<uint8_t> sec(m.N);
+  for (int64_t g = 0; g < m.N; ++g) {
+    auto it = std::find(keys.begin(), keys.end(), key[g]);
+    if (it == keys.end()) {
+      if (keys.size() >= 255) return fail(HEOMQ_UNSUPPORTED, "too many symmetry sectors");
+      keys.push_back(key[g]);
+      it = keys.end() - 1;
+    }
+    sec[g] = (uint8_t)(it - keys.begin());
+  }
+  const int S = (int)keys.size();
+  std::vector<int16_t> elems((size_t)S * dd, -1), pos((size_t)S * dd, -1);
+  std::vector<int> nnz(S, 0);
+  for (int s = 0; s < S; ++s)
+    for (int j = 0; j < d; ++j)
+      for (int i = 0; i < d; ++i) {
+        const int dn = fN[i] - fN[j], ds = fS[i] - fS[j];
+        const bool okn = c->cons_n ? dn == keys[s].first : (((dn - keys[s].first) % 2) == 0);
+        const bool oks = c->cons_s ? ds == keys[s].second : true;
+        if (!okn || !oks) continue;
+        pos[(size_t)s * dd + i + j * d] = (int16_t)nnz[s];
+        elems[(size_t)s * dd + nnz[s]] = (int16_t)(i + j * d);
+        nnz[s]++;
+      }
+  std::vector<int64_t> off(m.N + 1, 0);
+  for (int64_t g = 0; g < m.N; ++g) off[g + 1] = off[g] + nnz[sec[g]];
+  const int64_t total = off[m.N];
+  if (total >= ((int64_t)1 << 31)) return fail(HEOMQ_UNSUPPORTED, "packed vector too long");
+  const int64_t nblk = (total + 255) / 256;
+  std::vector<int32_t> block((size_t)nblk + 1, 0);
+  {
+    int64_t a = 0;
+    for (int64_t b = 0; b <= nblk; ++b) {
+      const int64_t t = std::min<int64_t>(b * 256, total - 1);
+      while (a + 1 < m.N && off[a + 1] <= t) ++a;
+      block[b] = (int32_t)a;
+    }
+  }
+  // heavy / light split at the top tier
+  const int64_t n_heavy = m.L >= 1 ? m.first[m.L] : m.N;
+  const int64_t light_begin = off[n_heavy];
+  const int64_t nlb = (total - light_begin + 255) / 256;
+  std::vector<int32_t> lblock((size_t)std::max<int64_t>(nlb, 0) + 1, (int32_t)(m.N - 1));
+  {
+    int64_t a = n_heavy;
+    for (int64_t b = 0; b <= nlb; ++b) {
+      const int64_t t = std::min<int64_t>(light_begin + b * 256, total - 1);
+      while (a + 1 < m.N && off[a + 1] <= t) ++a;
+      lblock[b] = (int32_t)a;
+    }
+  }
+  HIPCHECK(upload(c->p_lblock, lblock));
+  // i(HX - XH) per sector as CSR over packed slots
+  std::vector<int32_t> hrow((size_t)S * (dd + 1), 0);
+  std::vector<int16_t> hq;
+  std::vector<double> hc;
+  for (int s = 0; s < S; ++s) {
+    for (int t = 0; t < nnz[s]; ++t) {
+      hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
+      const int e = elems[(size_t)s * dd + t];
+      const int i = e % d, j = e / d;
+      std::vector<std::pair<int, cplx>> terms;
+      auto addterm = [&](int q, cplx v) {
+        if (q < 0 || v == cplx(0, 0)) return;
+        for (auto& tm : terms)
+          if (tm.first == q) { tm.second += v; return; }
+        terms.push_back({q, v});
+      };
+      for (int k = 0; k < d; ++k) {
+        const cplx hik = m.H[i + (size_t)k * d], hkj = m.H[k + (size_t)j * d];
+        if (hik != cplx(0, 0)) addterm(pos[(size_t)s * dd + k + j * d], cplx(0, 1) * hik);
+        if (hkj != cplx(0, 0)) addterm(pos[(size_t)s * dd + i + k * d], -cplx(0, 1) * hkj);
+      }
+      for (auto& tm : terms) {
+        if (tm.second == cplx(0, 0)) continue;
+        hq.push_back((int16_t)tm.first);
+        hc.push_back(tm.second.real());
+        hc.push_back(tm.second.imag());
+      }
+    }
+    for (int t = nnz[s]; t <= dd; ++t) hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
+  }
+  if (hq.empty()) { hq.push_back(0); hc.push_back(0); hc.push_back(0); }
+  // gather maps: output slot t of sector s -> signed slot of the neighbour (sector s2)
+  std::vector<int16_t> maps;
+  std::map<std::tuple<int, int, int, int, int>, int32_t> mapidx;
+  auto getmap = [&](int s, int s2, int o, int dag, int side) -> int32_t {
+    if (o < 0) return -1;
+    const auto key = std::make_tuple(s, s2, o, dag, side);
+    auto it = mapidx.find(key);
+    if (it != mapidx.end()) return it->second;
+    const int32_t at = (int32_t)maps.size();
+    const int8_t* pL = &omap[(size_t)o * 4 * d];
+    const int8_t* sL = pL + d;
+    const int8_t* pR = pL + 2 * d;
+    const int8_t* sR = pL + 3 * d;
+    bool any = false;
+    for (int t = 0; t < nnz[s]; ++t) {
+      const int e = elems[(size_t)s * dd + t];
+      const int i = e % d, j = e / d;
+      int v = 0;
+      if (side == 0) {
+        const int pl = pL[i];
+        if (pl >= 0) {
+          const int q = pos[(size_t)s2 * dd + (dag ? (j + pl * d) : (pl + j * d))];
+          if (q >= 0) v = (q + 1) * sL[i];
+        }
+      } else {
+        const int pr = pR[j];
+        if (pr >= 0) {
+          const int q = pos[(size_t)s2 * dd + (dag ? (pr + i * d) : (i + pr * d))];
+          if (q >= 0) v = (q + 1) * sR[j];
+        }
+      }
+      any |= v != 0;
+      maps.push_back((int16_t)v);
+    }
+    const int32_t res = any ? at : -1;
+    if (!any) maps.resize(at);
+    mapidx[key] = res;
+    return res;
+  };
+  auto conv = [&](const std::vector<int64_t>& rows, const std::vector<DevEntry>& e) {
+    std::vector<DevEntryP> out(e.size());
+    for (int64_t g = 0; g < m.N; ++g)
+      for (int64_t p = rows[g]; p < rows[g + 1]; ++p) {
+        const DevRec& r = recs[e[p].rec];
+        const int dag = e[p].flags & 1;
+        const int s = sec[g], s2 = sec[e[p].nbr];
+        out[p] = DevEntryP{(int32_t)off[e[p].nbr], getmap(s, s2, r.opL, dag, 0), getmap(s, s2, r.opR, dag, 1),
+                           e[p].rec, e[p].flags, (uint8_t)s2};
+      }
+    return out;
+  };
+  const auto pentf = conv(rowf, entf);
+  const auto pentt = conv(rowt, entt);
+  if (maps.empty()) maps.push_back(0);
+  HIPCHECK(upload(c->p_off, off));
+  HIPCHECK(upload(c->p_sec, sec));
+  HIPCHECK(upload(c->p_elems, elems));
+  HIPCHECK(upload(c->p_pos, pos));
+  HIPCHECK(upload(c->p_block, block));
+  HIPCHECK(upload(c->p_entf, pentf));
+  HIPCHECK(upload(c->p_entt, pentt));
+  HIPCHECK(upload(c->p_hrow, hrow));
+  HIPCHECK(upload(c->p_hq, hq));
+  HIPCHECK(upload(c->p_hc, hc));
+  HIPCHECK(upload(c->p_maps, maps));
+  c->ptotal = total;
+  c->nsec = S;
+  const int tslot = pos[(size_t)sec[0] * dd + (d - 1) + (d - 1) * d];
+  for (int w = 0; w < 2; ++w) {
+    DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
+    p.n_ado = m.N;
+    p.total = total;
+    p.off = c->p_off.as<int64_t>();
+    p.sec = c->p_sec.as<uint8_t>();
+    p.sec_elems = c->p_elems.as<int16_t>();
+    p.sec_pos = c->p_pos.as<int16_t>();
+    p.block_ado = c->p_block.as<int32_t>();
+    p.rowptr = (w == 0 ? c->fwd.rowptr : c->trn.rowptr).as<int64_t>();
+    p.ent = (w == 0 ? c->p_entf : c->p_entt).as<DevEntryP>();
+    p.trace_slot = tslot;
+    p.hrow = c->p_hrow.as<int32_t>();
+    p.hq = c->p_hq.as<int16_t>();
+    p.hc = c->p_hc.as<double>();
+    p.maps = c->p_maps.as<int16_t>();
+    p.n_heavy = n_heavy;
+    p.light_begin = light_begin;
+    p.light_block_ado = c->p_lblock.as<int32_t>();
+    const auto& rows = w == 0 ? rowf : rowt;
+    int mx = 0;
+    for (int64_t g = n_heavy; g < m.N; ++g) mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
+    p.max_light_couplings = mx;
+  }
+  return HEOMQ_OK;
+}
+
+// coefficient pointers of the packed operators follow the dense ones
+void sync_packed(heomq_ctx* c) {
+  for (int w = 0; w < 2; ++w) {
+    DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
+    const DevOperator& o = w == 0 ? c->fwd.op : c->trn.op;
+    p.rec = o.rec;
+    p.gam = o.gam;
+    p.opmap = o.opmap;
+    p.hnnz = o.hnnz;
+    p.hcol = o.hcol;
+    p.hval = o.hval;
+    p.nop = o.nop;
+    p.d = o.d;
+    p.hsign = o.hsign;
+    p.transposed = o.transposed;
+  }
+}
+
 int64_t trace_pos(const heomq_ctx* c, int trace) {
   if (trace == HEOMQ_TRACE_LAST) return c->N - 1;
   if (trace == HEOMQ_TRACE_RDO) return 0;
@@ -337,6 +578,8 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
   fill_op(c.get(), c->trn, c->rec, true, +1.0);
   // derivative operator shares the forward entries; records/H/gamma set per call
   c->dpar.op = c->fwd.op;
+  if (int rc = build_packed(c.get(), m, rowf, entf, rowt, entt)) return rc;
+  sync_packed(c.get());
   HIPCHECK(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
   HIPCHECK(c->scal.alloc(sizeof(TfqmrScal) * 2));
   HIPCHECK(hipMemset(c->scal.p, 0, sizeof(TfqmrScal) * 2));
@@ -357,6 +600,7 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
   if (int rc = upload_coefs(c, h->m)) return rc;
   fill_op(c, c->fwd, c->rec, false, -1.0);
   fill_op(c, c->trn, c->rec, true, +1.0);
+  sync_packed(c);
   HIPCHECK(hipDeviceSynchronize());
   return HEOMQ_OK;
 }

@@ -81,8 +81,14 @@ struct heomq_ctx {
   // host-side entry lists (kept for re-upload of coefficients)
   std::vector<int32_t> lower_drawers;
   // solver workspace
-  DevBuf ws64, ws32, scal, partial, red;
+  DevBuf ws64, ws64b, ws32, ws_io, scal, partial, red;
   DevBuf parity;                     // electron-number parity of each Fock state
+  // symmetry-packed layout (solver path)
+  DevBuf p_off, p_sec, p_elems, p_pos, p_block, p_lblock, p_entf, p_entt, p_hrow, p_hq, p_hc, p_maps;
+  heomq::DevPacked pfwd, ptrn;
+  int64_t ptotal = 0;
+  int nsec = 0;
+  bool cons_n = true, cons_s = true;
   void* pinned = nullptr;            // small pinned host buffer for scalar read-back
   ~heomq_ctx() {
     if (pinned) (void)hipHostFree(pinned);

@@ -50,6 +50,70 @@ struct DevOperator {
   TierBounds tb{};
 };
 
+// ---------------------------------------------------------------------------
+// Symmetry-packed layout (solver path).  H and the bath couplings conserve the
+// electron number N and S_z, so element (i, j) of ADO K can be nonzero only if
+// N_i - N_j = dN_K and 2(S_z,i - S_z,j) = d2S_K, with (dN_K, d2S_K) the sums of
+// q = +-1 (sigma = +/-) and q*(+-1 for spin up/down) over K's drawers.  ADOs
+// with the same (dN, d2S) share one "sector": an ordered list of allowed
+// elements.  A packed vector stores, ADO after ADO, only those elements.
+// ---------------------------------------------------------------------------
+struct DevEntryP {       // packed coupling (16 bytes)
+  int32_t base;          // neighbour's first packed element
+  int32_t mapL, mapR;    // offsets of the gather maps (output slot -> signed neighbour slot); -1: no term
+  uint16_t rec;          // coefficient record (cL, cR)
+  uint8_t flags;         // bit0 X^H, bit1 sL = -1, bit2 sR = -1
+  uint8_t nsec;
+};
+
+struct DevPacked {
+  int64_t n_ado = 0, total = 0;      // ADOs, packed complex elements
+  const int64_t* off = nullptr;      // packed offset of each ADO (n_ado + 1)
+  const uint8_t* sec = nullptr;      // sector of each ADO
+  const int16_t* sec_elems = nullptr;   // [S][d*d]: element index i + j*d of slot t
+  const int16_t* sec_pos = nullptr;     // [S][d*d]: slot of element i + j*d, -1 if forbidden
+  const int32_t* block_ado = nullptr;   // ADO of packed element 256*b (one per 256-thread block, + sentinel)
+  const int64_t* rowptr = nullptr;
+  const DevEntryP* ent = nullptr;
+  const DevRec* rec = nullptr;
+  const double* gam = nullptr;
+  const int8_t* opmap = nullptr;
+  const int32_t* hnnz = nullptr;
+  const int32_t* hcol = nullptr;
+  const double* hval = nullptr;
+  int nop = 0, d = 0;
+  double hsign = -1.0;
+  bool transposed = false;
+  int32_t trace_slot = -1;           // packed index (within ADO 0) of element (d-1, d-1)
+  // split of the work: ADOs below n_heavy (the tiers under the top one, with
+  // ~M couplings each) run one workgroup per ADO with the coupling list sliced
+  // over the lanes; the top tier (few couplings) runs one thread per element
+  // starting at packed element light_begin, with its own block table.
+  int64_t n_heavy = 0;
+  int64_t light_begin = 0;
+  const int32_t* light_block_ado = nullptr;
+  int max_light_couplings = 0;       // largest coupling count of a top-tier ADO (<= 8)
+  // slot-space tables: i(HX - XH) per sector as CSR over packed slots, and the
+  // gather maps (int16 value (q + 1) * sign, 0 = no term) referenced by entries
+  const int32_t* hrow = nullptr;     // [S][d*d + 1] row starts into hq / hc
+  const int16_t* hq = nullptr;
+  const double* hc = nullptr;        // complex coefficients of i(HX - XH)
+  const int16_t* maps = nullptr;
+};
+
+// Projected apply on packed vectors: y = Pi A x with the trace row on the
+// tier-0 block (rdo = true) or none, tier-0 output Hermitianised.
+hipError_t launch_apply_packed_f64(const DevPacked& op, const double* x, double* y, bool rdo, hipStream_t s);
+hipError_t launch_apply_packed_f32(const DevPacked& op, const float* x, float* y, bool rdo, hipStream_t s);
+// dense (N*d*d complex) <-> packed conversions; unpack writes zeros off-sector.
+template <typename TD, typename TP>
+hipError_t launch_pack(const DevPacked& op, const TD* dense, TP* packed, hipStream_t s);
+template <typename TP, typename TD>
+hipError_t launch_unpack(const DevPacked& op, const TP* packed, TD* dense, hipStream_t s);
+// Hermitianise the tier-0 block of a packed vector in place.
+template <typename T>
+hipError_t launch_herm_packed(const DevPacked& op, T* x, hipStream_t s);
+
 // y = A x.  trace_pos >= 0: apply the trace row at element (d-1, d-1) of ADO
 // trace_pos (forward: overwrite with tr x0; transposed: treat that element
 // of x as zero and add it to the diagonal of output block 0).

@@ -20,6 +20,16 @@ namespace {
 
 template <typename T>
 hipError_t apply_T(heomq_ctx* c, bool trans, const T* x, T* y, int64_t tpos, hipStream_t s, bool proj);
+template <typename T>
+hipError_t apply_packed_T(heomq_ctx* c, bool trans, const T* x, T* y, bool rdo, hipStream_t s);
+template <>
+hipError_t apply_packed_T<double>(heomq_ctx* c, bool trans, const double* x, double* y, bool rdo, hipStream_t s) {
+  return launch_apply_packed_f64(trans ? c->ptrn : c->pfwd, x, y, rdo, s);
+}
+template <>
+hipError_t apply_packed_T<float>(heomq_ctx* c, bool trans, const float* x, float* y, bool rdo, hipStream_t s) {
+  return launch_apply_packed_f32(trans ? c->ptrn : c->pfwd, x, y, rdo, s);
+}
 // Solvers iterate with the projected operator Pi*A (Pi: parity sectors +
 // Hermitian tier 0).  A maps range(Pi) into itself and the physical steady
 // state is its unique solution there, so every Krylov vector stays physical
@@ -65,11 +75,16 @@ struct RunStats {
 // TFQMR on A x = b with every vector and scalar on the device.  Restarts from
 // the true residual when the quasi-residual bound is met but ||b - Ax|| is not
 // below tol*||b|| (at most 3 times; stagnation stops it).
-// project: 0 none, 2 iterate with the projected operator Pi*A (parity sectors
-// + Hermitian tier 0; needs the trace row on the tier-0 block).
+// mode: 0 dense vectors and the plain operator (any trace row); 1 symmetry-
+// packed vectors and the projected operator Pi*A (trace row on the tier-0
+// block), the iterate's tier-0 block re-Hermitianised at every restart.
 template <typename T>
 int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool trans, int64_t tpos,
-              double tol, int maxiter, hipStream_t st, RunStats& rs, int project) {
+              double tol, int maxiter, hipStream_t st, RunStats& rs, int mode) {
+  auto A = [&](const T* in, T* out) -> hipError_t {
+    return mode == 1 ? apply_packed_T<T>(c, trans, in, out, tpos >= 0, st)
+                     : apply_T<T>(c, trans, in, out, tpos, st, false);
+  };
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   double* part = c->partial.as<double>();
   double* red = c->red.as<double>();
@@ -80,8 +95,8 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
   int base_iters = 0;
   int stalls = 0;
   while (true) {
-    if (project) HIPCHECK(vec_project<T>(st, c->N, c->d, x, c->parity.as<int8_t>(), c->tb, true));
-    HIPCHECK(apply_T<T>(c, trans, x, W.u1, tpos, st, project != 0));
+    if (mode == 1) HIPCHECK(launch_herm_packed<T>(c->pfwd, x, st));
+    HIPCHECK(A(x, W.u1));
     rs.applies++;
     HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, ticket, part)));
     double* hr = static_cast<double*>(c->pinned);
@@ -95,19 +110,19 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
     best = std::min(best, rr);
     HIPCHECK(tfq_reset(st, s, tol * std::sqrt(bb)));
     HIPCHECK(tfq_init_r<T>(st, n, b, W.u1, W.w, W.y1, W.d, W.rt, s, part));
-    HIPCHECK(apply_T<T>(c, trans, W.y1, W.u1, tpos, st, project != 0));
+    HIPCHECK(A(W.y1, W.u1));
     rs.applies++;
     HIPCHECK(tfq_init_v<T>(st, n, W.u1, W.v, W.rt, s, part));
     TfqmrScal& hs = *reinterpret_cast<TfqmrScal*>(static_cast<char*>(c->pinned) + 1024);
     while (true) {
       for (int k = 0; k < check_every; ++k) {
         HIPCHECK(tfq_y2<T>(st, n, W.y1, W.v, W.y2, s));
-        HIPCHECK(apply_T<T>(c, trans, W.y2, W.u2, tpos, st, project != 0));
+        HIPCHECK(A(W.y2, W.u2));
         HIPCHECK(tfq_half<T>(st, n, x, 0, W.w, W.u1, W.d, W.y1, s, part));
         HIPCHECK(tfq_half<T>(st, n, x, 1, W.w, W.u2, W.d, W.y2, s, part));
         HIPCHECK(tfq_rho<T>(st, n, x, W.d, W.w, W.rt, s, part));
         HIPCHECK(tfq_y1<T>(st, n, W.w, W.y2, W.y1, s));
-        HIPCHECK(apply_T<T>(c, trans, W.y1, W.u1, tpos, st, project != 0));
+        HIPCHECK(A(W.y1, W.u1));
         HIPCHECK(tfq_v<T>(st, n, W.u1, W.u2, W.v, W.rt, s, part));
       }
       HIPCHECK(hipMemcpyAsync(&hs, s, sizeof(TfqmrScal), hipMemcpyDeviceToHost, st));
@@ -119,7 +134,7 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
     base_iters = rs.iters;
     if (hs.flag == 2 && ++stalls > 3) return fail(HEOMQ_BREAKDOWN, "TFQMR breakdown");
   }
-  HIPCHECK(apply_T<T>(c, trans, x, W.u1, tpos, st, project != 0));
+  HIPCHECK(A(x, W.u1));
   HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, ticket, part)));
   double* hr = static_cast<double*>(c->pinned);
   HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -157,24 +172,38 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
   if (trace != HEOMQ_TRACE_LAST && trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "trace row required");
   hipStream_t st = (hipStream_t)stream;
   const auto t0 = std::chrono::steady_clock::now();
-  const int64_t n = 2 * c->N * c->d * c->d;
+  const bool packed = trace == HEOMQ_TRACE_RDO;
+  // packed: the solve runs on the symmetry-packed physical sector with Pi*A
+  const int64_t n = packed ? 2 * c->ptotal : 2 * c->N * c->d * c->d;
   std::vector<double*> v;
-  if (int rc = carve<double>(c->ws64, n, 9, v)) return rc;
+  if (int rc = carve<double>(c->ws64, n, packed ? 10 : 9, v)) return rc;
   Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
-  const double* bb = b;
-  if (!bb) {
-    if (int rc = e_trace_into(c, v[8], n, trace, st)) return rc;
-    bb = v[8];
+  double* bb = v[8];
+  double* xs = packed ? v[9] : x;
+  if (packed) {
+    HIPCHECK((launch_pack<double, double>(c->pfwd, x, xs, st)));
+    if (b) {
+      HIPCHECK((launch_pack<double, double>(c->pfwd, b, bb, st)));
+    } else {
+      HIPCHECK(vec_fill<double>(st, n, bb, 0.0));
+      HIPCHECK(vec_set_elem(st, bb, 2 * (int64_t)c->pfwd.trace_slot, 1.0));
+    }
+  } else if (b) {
+    HIPCHECK(hipMemcpyAsync(bb, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  } else if (int rc = e_trace_into(c, bb, n, trace, st)) {
+    return rc;
   }
   RunStats rs;
-  const int rc = tfqmr_run<double>(c, x, bb, W, n, transpose != 0, tpos_of(c, trace), tol, maxiter, st, rs,
-                                   trace == HEOMQ_TRACE_RDO ? 2 : 0);
+  const int rc = tfqmr_run<double>(c, xs, bb, W, n, transpose != 0, tpos_of(c, trace), tol, maxiter, st, rs,
+                                   packed ? 1 : 0);
+  if (packed) HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
     stats[0] = rs.iters;
     stats[1] = rs.applies;
     stats[2] = rs.relres;
     stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats[5] = (double)n;
   }
   return rc;
 }
@@ -182,34 +211,37 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
 int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double tol, double inner_tol,
                              int maxiter, int trace, double* stats, void* stream) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
-  if (trace != HEOMQ_TRACE_LAST && trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "trace row required");
+  if (trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "mixed precision solves use the tier-0 trace row");
   hipStream_t st = (hipStream_t)stream;
   const auto t0 = std::chrono::steady_clock::now();
-  const int64_t n = 2 * c->N * c->d * c->d;
-  const int64_t tpos = tpos_of(c, trace);
+  const int64_t n = 2 * c->ptotal;
   std::vector<double*> v64;
-  if (int rc = carve<double>(c->ws64, n, 2, v64)) return rc;
+  if (int rc = carve<double>(c->ws64, n, 3, v64)) return rc;
   std::vector<float*> v32;
   if (int rc = carve<float>(c->ws32, n, 10, v32)) return rc;
   Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6], v32[7]};
   float* x32 = v32[8];
   float* r32 = v32[9];
   double* ax = v64[0];
-  const double* bb = b;
-  if (!bb) {
-    if (int rc = e_trace_into(c, v64[1], n, trace, st)) return rc;
-    bb = v64[1];
+  double* bb = v64[1];
+  double* xs = v64[2];
+  HIPCHECK((launch_pack<double, double>(c->pfwd, x, xs, st)));
+  if (b) {
+    HIPCHECK((launch_pack<double, double>(c->pfwd, b, bb, st)));
+  } else {
+    HIPCHECK(vec_fill<double>(st, n, bb, 0.0));
+    HIPCHECK(vec_set_elem(st, bb, 2 * (int64_t)c->pfwd.trace_slot, 1.0));
   }
   double* red = c->red.as<double>();
   unsigned* ticket = reinterpret_cast<unsigned*>(red + 4);
   double* part = c->partial.as<double>();
-  int outer = 0, inner_total = 0, applies = 0;
-  double relres = 0;
+  int outer = 0, inner_total = 0, applies = 0, fp64_iters = 0;
+  double relres = 0, prev = 0;
   int rc = HEOMQ_NOT_CONVERGED;
-  const bool proj = trace == HEOMQ_TRACE_RDO;
-  for (outer = 0; outer < 50; ++outer) {
-    if (proj) HIPCHECK(vec_project<double>(st, c->N, c->d, x, c->parity.as<int8_t>(), c->tb, true));
-    HIPCHECK(launch_apply_f64(c->fwd.op, x, ax, tpos, st, proj));
+  bool finish64 = false;
+  for (outer = 0; outer < 60; ++outer) {
+    HIPCHECK(launch_herm_packed<double>(c->pfwd, xs, st));
+    HIPCHECK(launch_apply_packed_f64(c->pfwd, xs, ax, true, st));   // FP64 residual
     applies++;
     HIPCHECK((vec_resid<double, double>(st, n, bb, ax, red, ticket, part)));
     double* hr = static_cast<double*>(c->pinned);
@@ -219,22 +251,36 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
     if (stats && outer < 40) stats[8 + outer] = relres;   // refinement history
     if (hr[0] <= tol * tol * hr[1]) { rc = HEOMQ_OK; break; }
     if (inner_total >= maxiter) break;
+    // FP32 corrections stop contracting once eps32 * ||A|| ||dx|| reaches the
+    // residual; the rest of the way is done by FP64 TFQMR from this iterate.
+    if (outer >= 2 && relres > 0.5 * prev) { finish64 = true; break; }
+    prev = relres;
     const double rn = std::sqrt(hr[0]);
     HIPCHECK(vec_resid_to_f32(st, n, bb, ax, r32, 1.0 / rn));
     HIPCHECK(vec_fill<float>(st, n, x32, 0.0f));
     RunStats rs;
-    if (stats && outer < 40) stats[48 + outer] = 0;
     // Each FP32 inner solve is capped: FP32 TFQMR recurrences reach an
     // attainable-accuracy floor, and the FP64 residual of the next refinement
     // step is the better restart.
     const int cap = std::min(inner_maxit(), maxiter - inner_total);
-    const int irc = tfqmr_run<float>(c, x32, r32, W, n, false, tpos, inner_tol, cap, st, rs, proj ? 2 : 0);
+    const int irc = tfqmr_run<float>(c, x32, r32, W, n, false, 0, inner_tol, cap, st, rs, 1);
     if (irc != HEOMQ_OK && irc != HEOMQ_NOT_CONVERGED) return irc;
     inner_total += rs.iters;
     applies += rs.applies;
     if (stats && outer < 16) stats[48 + outer] = rs.iters;
-    HIPCHECK(vec_axpy_f32_to_f64(st, n, x, x32, rn));
+    HIPCHECK(vec_axpy_f32_to_f64(st, n, xs, x32, rn));   // FP64 update
   }
+  if (finish64) {
+    std::vector<double*> w64;
+    if (int e = carve<double>(c->ws64b, n, 8, w64)) return e;
+    Work<double> W64{w64[0], w64[1], w64[2], w64[3], w64[4], w64[5], w64[6], w64[7]};
+    RunStats rs;
+    rc = tfqmr_run<double>(c, xs, bb, W64, n, false, 0, tol, std::max(maxiter - inner_total, 1), st, rs, 1);
+    fp64_iters = rs.iters;
+    applies += rs.applies;
+    relres = rs.relres;
+  }
+  HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
     stats[0] = inner_total;
@@ -242,9 +288,63 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
     stats[2] = relres;
     stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     stats[4] = outer;
+    stats[6] = fp64_iters;
   }
   if (rc != HEOMQ_OK) return fail(HEOMQ_NOT_CONVERGED, "mixed-precision refinement did not converge");
   return rc;
+}
+
+int heomq_apply_packed(heomq_ctx* c, const double* x, double* y, int trace, int transpose, void* stream) {
+  if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = 2 * c->ptotal;
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws_io, n, 2, v)) return rc;
+  const DevPacked& op = transpose ? c->ptrn : c->pfwd;
+  HIPCHECK((launch_pack<double, double>(op, x, v[0], st)));
+  HIPCHECK(launch_apply_packed_f64(op, v[0], v[1], trace == HEOMQ_TRACE_RDO, st));
+  HIPCHECK((launch_unpack<double, double>(op, v[1], y, st)));
+  HIPCHECK(hipStreamSynchronize(st));
+  return HEOMQ_OK;
+}
+
+int heomq_apply_packed_raw(heomq_ctx* c, const double* xp, double* yp, int trace, int transpose, void* stream) {
+  if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
+  HIPCHECK(launch_apply_packed_f64(transpose ? c->ptrn : c->pfwd, xp, yp, trace == HEOMQ_TRACE_RDO,
+                                   (hipStream_t)stream));
+  return HEOMQ_OK;
+}
+
+int heomq_apply_packed_raw_f32(heomq_ctx* c, const float* xp, float* yp, int trace, int transpose, void* stream) {
+  if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
+  HIPCHECK(launch_apply_packed_f32(transpose ? c->ptrn : c->pfwd, xp, yp, trace == HEOMQ_TRACE_RDO,
+                                   (hipStream_t)stream));
+  return HEOMQ_OK;
+}
+
+int heomq_pack(heomq_ctx* c, const double* dense, double* packed, void* stream) {
+  if (!c || !dense || !packed) return fail(HEOMQ_BAD_ARG, "null argument");
+  HIPCHECK((launch_pack<double, double>(c->pfwd, dense, packed, (hipStream_t)stream)));
+  return HEOMQ_OK;
+}
+
+int heomq_unpack(heomq_ctx* c, const double* packed, double* dense, void* stream) {
+  if (!c || !dense || !packed) return fail(HEOMQ_BAD_ARG, "null argument");
+  HIPCHECK((launch_unpack<double, double>(c->pfwd, packed, dense, (hipStream_t)stream)));
+  return HEOMQ_OK;
+}
+
+int heomq_packed_info(const heomq_ctx* c, int64_t* info) {
+  if (!c || !info) return fail(HEOMQ_BAD_ARG, "null argument");
+  info[0] = c->ptotal;
+  info[1] = c->nsec;
+  info[2] = c->cons_n;
+  info[3] = c->cons_s;
+  info[4] = c->N * c->d * c->d;
+  return HEOMQ_OK;
 }
 
 int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream) {

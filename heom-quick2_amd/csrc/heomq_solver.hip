@@ -1,9 +1,9 @@
 // Vector kernels of the device-resident TFQMR steady-state solver, the RK4 and
 // Chebyshev propagators.  The solver keeps every vector and every scalar on
-// the GPU: each fused kernel does its vector updates and one FP64 reduction,
-// and the last workgroup to finish (ticket counter) folds the per-workgroup
-// partials in a fixed order and advances the TFQMR scalars, so a whole
-// iteration is launch-only (no host round trip) and deterministic.
+// the GPU: each fused kernel does its vector updates (16-byte complex-pair
+// accesses) and writes per-workgroup FP64 partial sums; a one-block finalize
+// kernel folds the partials in a fixed order and advances the TFQMR scalars.
+// A whole iteration is launch-only (no host round trip) and deterministic.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -30,183 +30,231 @@ __device__ inline void block_sum2(double& a, double& b) {
   }
 }
 
-// Stores this block's partials; returns true in thread 0 of the last block,
-// with the ordered totals in (a, b).
-__device__ inline bool finish_reduction(double& a, double& b, double* partial, unsigned* ticket) {
+// Phase 1 of every reduction: each block writes its (a, b) partial sums.
+__device__ inline void block_partial(double a, double b, double* partial) {
   block_sum2(a, b);
-  __shared__ bool last;
   if (threadIdx.x == 0) {
     partial[2 * blockIdx.x] = a;
     partial[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  // ordered fold by one wave: lane l sums partials l, l+64, ... then a fixed tree
-  double ta = 0, tb = 0;
-  if (threadIdx.x < 64)
-    for (unsigned t = threadIdx.x; t < gridDim.x; t += 64) {
-      ta += ((volatile double*)partial)[2 * t];
-      tb += ((volatile double*)partial)[2 * t + 1];
-    }
-  if (threadIdx.x < 64) {
-    for (int off = 32; off > 0; off >>= 1) {
-      ta += __shfl_down(ta, off, 64);
-      tb += __shfl_down(tb, off, 64);
-    }
-  }
-  if (threadIdx.x == 0) {
-    a = ta;
-    b = tb;
-    *ticket = 0u;   // ready for the next launch
-  }
-  return threadIdx.x == 0;
 }
+
+// Vector kernels walk complex pairs (16-byte accesses).
+template <typename T> struct W2;
+template <> struct W2<double> { using type = double2; };
+template <> struct W2<float> { using type = float2; };
 
 #define GRID_LOOP(i, n) \
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
-// r = b - Ax (stored in w and y1), ||r||^2 and rt.r; sets tau, rho, nb-based target.
+// r = b - Ax (stored in w and y1), d = 0; partials of ||r||^2 and rt.r.
 template <typename T>
-__global__ void k_init_r(int64_t n, const T* __restrict__ b, const T* __restrict__ ax, T* __restrict__ w,
-                         T* __restrict__ y1, T* __restrict__ d, const T* __restrict__ rt, TfqmrScal* s,
-                         double* partial) {
+__global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restrict__ ax, T* __restrict__ w,
+                         T* __restrict__ y1, T* __restrict__ d, const T* __restrict__ rt, double* partial) {
+  using V = typename W2<T>::type;
   double rr = 0, rrt = 0;
-  GRID_LOOP(i, n) {
-    const T r = b[i] - ax[i];
-    w[i] = r;
-    y1[i] = r;
-    d[i] = 0;
-    rr += (double)r * r;
-    rrt += (double)rt[i] * r;
+  GRID_LOOP(i, n2) {
+    const V bb = reinterpret_cast<const V*>(b)[i], aa = reinterpret_cast<const V*>(ax)[i];
+    const V tt = reinterpret_cast<const V*>(rt)[i];
+    V r;
+    r.x = bb.x - aa.x;
+    r.y = bb.y - aa.y;
+    reinterpret_cast<V*>(w)[i] = r;
+    reinterpret_cast<V*>(y1)[i] = r;
+    V z;
+    z.x = 0;
+    z.y = 0;
+    reinterpret_cast<V*>(d)[i] = z;
+    rr += (double)r.x * r.x + (double)r.y * r.y;
+    rrt += (double)tt.x * r.x + (double)tt.y * r.y;
   }
-  if (finish_reduction(rr, rrt, partial, &s->ticket)) {
-    s->rr = rr;
-    s->tau = sqrt(rr);
-    s->rho = rrt;
-    s->theta = 0;
-    s->eta = 0;
-    s->dcoef = 0;
-    s->flag = (rrt == 0.0 && rr > 0.0) ? 2 : 0;
-  }
+  block_partial(rr, rrt, partial);
 }
 
-// v = u1 (= A y1) and sigma = rt.v -> alpha.
+// v = u1; partial of rt.v
 template <typename T>
-__global__ void k_init_v(int64_t n, const T* __restrict__ u1, T* __restrict__ v, const T* __restrict__ rt,
-                         TfqmrScal* s, double* partial) {
-  double sg = 0, z = 0;
-  GRID_LOOP(i, n) {
-    const T t = u1[i];
-    v[i] = t;
-    sg += (double)rt[i] * t;
+__global__ void k_init_v(int64_t n2, const T* __restrict__ u1, T* __restrict__ v, const T* __restrict__ rt,
+                         double* partial) {
+  using V = typename W2<T>::type;
+  double sg = 0;
+  GRID_LOOP(i, n2) {
+    const V t = reinterpret_cast<const V*>(u1)[i];
+    const V r = reinterpret_cast<const V*>(rt)[i];
+    reinterpret_cast<V*>(v)[i] = t;
+    sg += (double)r.x * t.x + (double)r.y * t.y;
   }
-  if (finish_reduction(sg, z, partial, &s->ticket)) {
-    s->sigma = sg;
-    if (sg == 0.0) { s->flag = 2; return; }
-    s->alpha = s->rho / sg;
-    s->dcoef = 0;
-  }
+  block_partial(sg, 0.0, partial);
 }
 
 template <typename T>
-__global__ void k_y2(int64_t n, const T* __restrict__ y1, const T* __restrict__ v, T* __restrict__ y2,
+__global__ void k_y2(int64_t n2, const T* __restrict__ y1, const T* __restrict__ v, T* __restrict__ y2,
                      const TfqmrScal* s) {
+  using V = typename W2<T>::type;
   if (s->flag) return;
   const T alpha = (T)s->alpha;
-  GRID_LOOP(i, n) y2[i] = y1[i] - alpha * v[i];
+  GRID_LOOP(i, n2) {
+    const V a = reinterpret_cast<const V*>(y1)[i], b = reinterpret_cast<const V*>(v)[i];
+    V o;
+    o.x = a.x - alpha * b.x;
+    o.y = a.y - alpha * b.y;
+    reinterpret_cast<V*>(y2)[i] = o;
+  }
 }
 
-// One TFQMR half step: [x += eta d], w -= alpha u, d = y + dcoef d, ||w||^2,
-// then theta, c, tau, eta, dcoef (Freund 1993; Saad, Iterative Methods 7.4.2).
+// One TFQMR half step: [x += eta d], w -= alpha u, d = y + dcoef d; partial ||w||^2.
 template <typename T>
-__global__ void k_half(int64_t n, T* __restrict__ x, int xupd, T* __restrict__ w, const T* __restrict__ u,
-                       T* __restrict__ d, const T* __restrict__ yj, TfqmrScal* s, double* partial) {
+__global__ void k_half(int64_t n2, T* __restrict__ x, int xupd, T* __restrict__ w, const T* __restrict__ u,
+                       T* __restrict__ d, const T* __restrict__ yj, const TfqmrScal* s, double* partial) {
+  using V = typename W2<T>::type;
   if (s->flag) return;
   const T alpha = (T)s->alpha, dcoef = (T)s->dcoef, eta = (T)s->eta;
-  double ww = 0, z = 0;
-  GRID_LOOP(i, n) {
-    const T di = d[i];
-    if (xupd) x[i] += eta * di;
-    const T wi = w[i] - alpha * u[i];
-    w[i] = wi;
-    d[i] = yj[i] + dcoef * di;
-    ww += (double)wi * wi;
+  double ww = 0;
+  GRID_LOOP(i, n2) {
+    const V di = reinterpret_cast<const V*>(d)[i];
+    if (xupd) {
+      V xi = reinterpret_cast<V*>(x)[i];
+      xi.x += eta * di.x;
+      xi.y += eta * di.y;
+      reinterpret_cast<V*>(x)[i] = xi;
+    }
+    V wi = reinterpret_cast<V*>(w)[i];
+    const V ui = reinterpret_cast<const V*>(u)[i];
+    wi.x -= alpha * ui.x;
+    wi.y -= alpha * ui.y;
+    reinterpret_cast<V*>(w)[i] = wi;
+    const V yi = reinterpret_cast<const V*>(yj)[i];
+    V dn;
+    dn.x = yi.x + dcoef * di.x;
+    dn.y = yi.y + dcoef * di.y;
+    reinterpret_cast<V*>(d)[i] = dn;
+    ww += (double)wi.x * wi.x + (double)wi.y * wi.y;
   }
-  if (finish_reduction(ww, z, partial, &s->ticket)) {
-    const double theta = sqrt(ww) / s->tau;
-    const double c = 1.0 / sqrt(1.0 + theta * theta);
-    s->tau = s->tau * theta * c;
-    s->theta = theta;
-    s->eta = c * c * s->alpha;
-    s->dcoef = theta * theta * s->eta / s->alpha;
-  }
+  block_partial(ww, 0.0, partial);
 }
 
-// x += eta d; rho_new = rt.w; beta; convergence bound tau*sqrt(2m+1) <= target.
+// x += eta d; partial of rt.w
 template <typename T>
-__global__ void k_rho(int64_t n, T* __restrict__ x, const T* __restrict__ d, const T* __restrict__ w,
-                      const T* __restrict__ rt, TfqmrScal* s, double* partial) {
+__global__ void k_rho(int64_t n2, T* __restrict__ x, const T* __restrict__ d, const T* __restrict__ w,
+                      const T* __restrict__ rt, const TfqmrScal* s, double* partial) {
+  using V = typename W2<T>::type;
   if (s->flag) return;
   const T eta = (T)s->eta;
-  double r = 0, z = 0;
-  GRID_LOOP(i, n) {
-    x[i] += eta * d[i];
-    r += (double)rt[i] * w[i];
+  double r = 0;
+  GRID_LOOP(i, n2) {
+    V xi = reinterpret_cast<V*>(x)[i];
+    const V di = reinterpret_cast<const V*>(d)[i];
+    xi.x += eta * di.x;
+    xi.y += eta * di.y;
+    reinterpret_cast<V*>(x)[i] = xi;
+    const V ti = reinterpret_cast<const V*>(rt)[i], wi = reinterpret_cast<const V*>(w)[i];
+    r += (double)ti.x * wi.x + (double)ti.y * wi.y;
   }
-  if (finish_reduction(r, z, partial, &s->ticket)) {
-    s->iter += 1;
-    if (s->tau * sqrt(2.0 * s->iter + 1.0) <= s->target) { s->flag = 1; return; }
-    if (s->rho == 0.0) { s->flag = 2; return; }
-    s->beta = r / s->rho;
-    s->rho = r;
-  }
+  block_partial(r, 0.0, partial);
 }
 
 template <typename T>
-__global__ void k_y1(int64_t n, const T* __restrict__ w, const T* __restrict__ y2, T* __restrict__ y1,
+__global__ void k_y1(int64_t n2, const T* __restrict__ w, const T* __restrict__ y2, T* __restrict__ y1,
                      const TfqmrScal* s) {
+  using V = typename W2<T>::type;
   if (s->flag) return;
   const T beta = (T)s->beta;
-  GRID_LOOP(i, n) y1[i] = w[i] + beta * y2[i];
+  GRID_LOOP(i, n2) {
+    const V a = reinterpret_cast<const V*>(w)[i], b = reinterpret_cast<const V*>(y2)[i];
+    V o;
+    o.x = a.x + beta * b.x;
+    o.y = a.y + beta * b.y;
+    reinterpret_cast<V*>(y1)[i] = o;
+  }
 }
 
-// v = u1 + beta (u2 + beta v); sigma = rt.v -> alpha, dcoef.
+// v = u1 + beta (u2 + beta v); partial of rt.v
 template <typename T>
-__global__ void k_v(int64_t n, const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ v,
-                    const T* __restrict__ rt, TfqmrScal* s, double* partial) {
+__global__ void k_v(int64_t n2, const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ v,
+                    const T* __restrict__ rt, const TfqmrScal* s, double* partial) {
+  using V = typename W2<T>::type;
   if (s->flag) return;
   const T beta = (T)s->beta;
-  double sg = 0, z = 0;
-  GRID_LOOP(i, n) {
-    const T t = u1[i] + beta * (u2[i] + beta * v[i]);
-    v[i] = t;
-    sg += (double)rt[i] * t;
+  double sg = 0;
+  GRID_LOOP(i, n2) {
+    const V a = reinterpret_cast<const V*>(u1)[i], b = reinterpret_cast<const V*>(u2)[i];
+    const V c = reinterpret_cast<const V*>(v)[i], r = reinterpret_cast<const V*>(rt)[i];
+    V t;
+    t.x = a.x + beta * (b.x + beta * c.x);
+    t.y = a.y + beta * (b.y + beta * c.y);
+    reinterpret_cast<V*>(v)[i] = t;
+    sg += (double)r.x * t.x + (double)r.y * t.y;
   }
-  if (finish_reduction(sg, z, partial, &s->ticket)) {
-    s->sigma = sg;
-    if (sg == 0.0) { s->flag = 2; return; }
-    s->alpha = s->rho / sg;
-    s->dcoef = s->theta * s->theta * s->eta / s->alpha;
-  }
+  block_partial(sg, 0.0, partial);
 }
 
-// ||b - ax||^2 and ||b||^2 into out[0], out[1] (FP64).
+// partials of ||b - ax||^2 and ||b||^2 (FP64)
 template <typename TB, typename TA>
-__global__ void k_resid(int64_t n, const TB* __restrict__ b, const TA* __restrict__ ax, double* out,
-                        unsigned* ticket, double* partial) {
+__global__ void k_resid(int64_t n2, const TB* __restrict__ b, const TA* __restrict__ ax, double* partial) {
   double rr = 0, bb = 0;
-  GRID_LOOP(i, n) {
-    const double r = (double)b[i] - (double)ax[i];
-    rr += r * r;
-    bb += (double)b[i] * (double)b[i];
+  GRID_LOOP(i, n2) {
+    const double b0 = (double)b[2 * i], b1 = (double)b[2 * i + 1];
+    const double r0 = b0 - (double)ax[2 * i], r1 = b1 - (double)ax[2 * i + 1];
+    rr += r0 * r0 + r1 * r1;
+    bb += b0 * b0 + b1 * b1;
   }
-  if (finish_reduction(rr, bb, partial, ticket)) {
-    out[0] = rr;
-    out[1] = bb;
+  block_partial(rr, bb, partial);
+}
+
+// Phase 2: one block folds the partials in a fixed order and advances the
+// scalars (Freund 1993; Saad, Iterative Methods 7.4.2).
+enum FinalizeOp { F_INIT_R = 0, F_INIT_V, F_HALF, F_RHO, F_V, F_RESID };
+
+__global__ void k_finalize(int op, int nblk, const double* __restrict__ partial, TfqmrScal* s, double* out) {
+  if (op != F_INIT_R && op != F_INIT_V && op != F_RESID && s->flag) return;   // s unused for F_RESID
+  double a = 0, b = 0;
+  for (int t = threadIdx.x; t < nblk; t += blockDim.x) {
+    a += partial[2 * t];
+    b += partial[2 * t + 1];
+  }
+  block_sum2(a, b);
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case F_INIT_R:
+      s->rr = a;
+      s->tau = sqrt(a);
+      s->rho = b;
+      s->theta = 0;
+      s->eta = 0;
+      s->dcoef = 0;
+      s->flag = (b == 0.0 && a > 0.0) ? 2 : 0;
+      break;
+    case F_INIT_V:
+      s->sigma = a;
+      if (a == 0.0) { s->flag = 2; break; }
+      s->alpha = s->rho / a;
+      s->dcoef = 0;
+      break;
+    case F_HALF: {
+      const double theta = sqrt(a) / s->tau;
+      const double c = 1.0 / sqrt(1.0 + theta * theta);
+      s->tau = s->tau * theta * c;
+      s->theta = theta;
+      s->eta = c * c * s->alpha;
+      s->dcoef = theta * theta * s->eta / s->alpha;
+      break;
+    }
+    case F_RHO:
+      s->iter += 1;
+      if (s->tau * sqrt(2.0 * s->iter + 1.0) <= s->target) { s->flag = 1; break; }
+      if (s->rho == 0.0) { s->flag = 2; break; }
+      s->beta = a / s->rho;
+      s->rho = a;
+      break;
+    case F_V:
+      s->sigma = a;
+      if (a == 0.0) { s->flag = 2; break; }
+      s->alpha = s->rho / a;
+      s->dcoef = s->theta * s->theta * s->eta / s->alpha;
+      break;
+    case F_RESID:
+      out[0] = a;
+      out[1] = b;
+      break;
   }
 }
 
@@ -315,44 +363,62 @@ static int grid_for(int64_t n) {
 
 #define L1(kern, n, ...) hipLaunchKernelGGL(kern, dim3(grid_for<T>(n)), dim3(kRedThreads), 0, st, n, __VA_ARGS__)
 
+static hipError_t finalize(hipStream_t st, int op, int nblk, const double* partial, TfqmrScal* s, double* out) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedThreads), 0, st, op, nblk, partial, s, out);
+  return hipGetLastError();
+}
+
+// n counts reals; the vector kernels walk n/2 complex pairs
 template <typename T>
 hipError_t tfq_init_r(hipStream_t st, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, const T* rt,
                       TfqmrScal* s, double* partial) {
-  L1((k_init_r<T>), n, b, ax, w, y1, d, rt, s, partial);
-  return hipGetLastError();
+  const int64_t n2 = n / 2;
+  L1((k_init_r<T>), n2, b, ax, w, y1, d, rt, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return finalize(st, F_INIT_R, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
 hipError_t tfq_init_v(hipStream_t st, int64_t n, const T* u1, T* v, const T* rt, TfqmrScal* s, double* partial) {
-  L1((k_init_v<T>), n, u1, v, rt, s, partial);
-  return hipGetLastError();
+  const int64_t n2 = n / 2;
+  L1((k_init_v<T>), n2, u1, v, rt, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return finalize(st, F_INIT_V, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
 hipError_t tfq_y2(hipStream_t st, int64_t n, const T* y1, const T* v, T* y2, const TfqmrScal* s) {
-  L1((k_y2<T>), n, y1, v, y2, s);
+  const int64_t n2 = n / 2;
+  L1((k_y2<T>), n2, y1, v, y2, s);
   return hipGetLastError();
 }
 template <typename T>
 hipError_t tfq_half(hipStream_t st, int64_t n, T* x, int xupd, T* w, const T* u, T* d, const T* yj,
                     TfqmrScal* s, double* partial) {
-  L1((k_half<T>), n, x, xupd, w, u, d, yj, s, partial);
-  return hipGetLastError();
+  const int64_t n2 = n / 2;
+  L1((k_half<T>), n2, x, xupd, w, u, d, yj, s, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return finalize(st, F_HALF, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
 hipError_t tfq_rho(hipStream_t st, int64_t n, T* x, const T* d, const T* w, const T* rt, TfqmrScal* s,
                    double* partial) {
-  L1((k_rho<T>), n, x, d, w, rt, s, partial);
-  return hipGetLastError();
+  const int64_t n2 = n / 2;
+  L1((k_rho<T>), n2, x, d, w, rt, s, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return finalize(st, F_RHO, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
 hipError_t tfq_y1(hipStream_t st, int64_t n, const T* w, const T* y2, T* y1, const TfqmrScal* s) {
-  L1((k_y1<T>), n, w, y2, y1, s);
+  const int64_t n2 = n / 2;
+  L1((k_y1<T>), n2, w, y2, y1, s);
   return hipGetLastError();
 }
 template <typename T>
 hipError_t tfq_v(hipStream_t st, int64_t n, const T* u1, const T* u2, T* v, const T* rt, TfqmrScal* s,
                  double* partial) {
-  L1((k_v<T>), n, u1, u2, v, rt, s, partial);
-  return hipGetLastError();
+  const int64_t n2 = n / 2;
+  L1((k_v<T>), n2, u1, u2, v, rt, s, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return finalize(st, F_V, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
 hipError_t vec_fill(hipStream_t st, int64_t n, T* v, T val) {
@@ -368,8 +434,11 @@ template <typename TB, typename TA>
 hipError_t vec_resid(hipStream_t st, int64_t n, const TB* b, const TA* ax, double* out, unsigned* ticket,
                      double* partial) {
   using T = TB;
-  L1((k_resid<TB, TA>), n, b, ax, out, ticket, partial);
-  return hipGetLastError();
+  (void)ticket;
+  const int64_t n2 = n / 2;
+  L1((k_resid<TB, TA>), n2, b, ax, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return finalize(st, F_RESID, grid_for<T>(n2), partial, nullptr, out);
 }
 hipError_t vec_resid_to_f32(hipStream_t st, int64_t n, const double* b, const double* ax, float* r32, double scale) {
   using T = double;
@@ -384,10 +453,8 @@ hipError_t vec_axpy_f32_to_f64(hipStream_t st, int64_t n, double* x, const float
 template <typename T>
 hipError_t vec_project(hipStream_t st, int64_t nado, int d, T* x, const int8_t* par, const TierBounds& tb,
                        bool hermitian) {
-  const int64_t n = nado * d * d;
   L1((k_project_parity<T>), nado, d, x, par, tb);
   if (hermitian) hipLaunchKernelGGL((k_hermitize0<T>), dim3((d * d + 255) / 256), dim3(256), 0, st, d, x);
-  (void)n;
   return hipGetLastError();
 }
 template hipError_t vec_project<double>(hipStream_t, int64_t, int, double*, const int8_t*, const TierBounds&, bool);

@@ -204,6 +204,7 @@ class Context:
         torch.cuda.set_device(device)
         check(lib().heomq_ctx_create(model._h, device, C.byref(self._h)))
         self.N, self.d = model.N, model.d
+        self._ptotal = self.packed_info()["packed"]
         _LIVE_CTX.add(self)
 
     def close(self):
@@ -257,6 +258,55 @@ class Context:
                                     trace, self._stream()))
         return y
 
+    def apply_packed(self, x, y=None, trace=TRACE_RDO, transpose=False):
+        """Projected apply through the symmetry-packed layout (dense in/out)."""
+        t = self.torch
+        y = self.zeros() if y is None else y
+        check(lib().heomq_apply_packed(self._h, self._check_vec(x, t.complex128), self._check_vec(y, t.complex128),
+                                       trace, int(transpose), self._stream()))
+        return y
+
+    def packed_zeros(self, dtype=None):
+        n = self.packed_info()["packed"]
+        return self.torch.zeros(n, dtype=dtype or self.torch.complex128, device=f"cuda:{self.device}")
+
+    def _check_packed(self, v, dtype):
+        if v.dtype != dtype or not v.is_cuda or not v.is_contiguous() or v.numel() != self._ptotal:
+            raise HeomqError(_lib.BAD_DIM, f"expected contiguous {dtype} device tensor with {self._ptotal} elements")
+        return C.c_void_p(v.data_ptr())
+
+    def apply_packed_raw(self, xp, yp, trace=TRACE_RDO, transpose=False):
+        t = self.torch
+        if xp.dtype == t.complex64:
+            check(lib().heomq_apply_packed_raw_f32(self._h, self._check_packed(xp, t.complex64),
+                                                   self._check_packed(yp, t.complex64), trace, int(transpose),
+                                                   self._stream()))
+        else:
+            check(lib().heomq_apply_packed_raw(self._h, self._check_packed(xp, t.complex128),
+                                               self._check_packed(yp, t.complex128), trace, int(transpose),
+                                               self._stream()))
+        return yp
+
+    def pack(self, x, xp=None):
+        t = self.torch
+        xp = self.packed_zeros() if xp is None else xp
+        check(lib().heomq_pack(self._h, self._check_vec(x, t.complex128), self._check_packed(xp, t.complex128),
+                               self._stream()))
+        return xp
+
+    def unpack(self, xp, x=None):
+        t = self.torch
+        x = self.zeros() if x is None else x
+        check(lib().heomq_unpack(self._h, self._check_packed(xp, t.complex128), self._check_vec(x, t.complex128),
+                                 self._stream()))
+        return x
+
+    def packed_info(self):
+        info = np.zeros(8, np.int64)
+        check(lib().heomq_packed_info(self._h, info.ctypes.data_as(_lib.I64)))
+        return dict(packed=int(info[0]), sectors=int(info[1]), conserves_n=bool(info[2]),
+                    conserves_sz=bool(info[3]), dense=int(info[4]))
+
     def apply_dparam(self, x, dH, dcb, dcd, dgam, y=None):
         t = self.torch
         y = self.zeros() if y is None else y
@@ -299,7 +349,7 @@ class Context:
         outer = int(stats[4])
         info = dict(iters=int(stats[0]), applies=int(stats[1]), relres=stats[2], seconds=stats[3], outer=outer,
                     history=stats[8:8 + min(outer + 1, 40)].tolist(), inner=stats[48:48 + min(outer, 16)].tolist(),
-                    status=rc)
+                    fp64_iters=int(stats[6]), status=rc)
         if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
             raise HeomqError(rc, lib().heomq_last_error().decode() + f" {info}")
         return x, info

@@ -18,7 +18,8 @@ INCLUDE = PKG_DIR.parent.parent / "include"
 LIB_PATH = PKG_DIR / "libheomq.so"
 FLIB_PATH = PKG_DIR / "libheomq_fortran.so"
 
-SOURCES = ["heomq_model.cpp", "heomq_apply.hip", "heomq_solver.hip", "heomq_capi.cpp", "heomq_solve.cpp"]
+SOURCES = ["heomq_model.cpp", "heomq_apply.hip", "heomq_packed.hip", "heomq_solver.hip", "heomq_capi.cpp",
+           "heomq_solve.cpp"]
 
 
 def _hipcc() -> str:

@@ -151,6 +151,23 @@ int heomq_apply_dparam(heomq_ctx* c, const double* x, double* y, const double* d
                        const double* dcb, const double* dcd, const double* dgam,
                        void* stream);
 
+/* Projected apply in the solvers' symmetry-packed layout (H and the bath
+ * conserve N and S_z, so element (i, j) of ADO K is kept only if N_i - N_j
+ * and 2(S_z,i - S_z,j) equal the charge / spin carried by K's drawers; the
+ * tier-0 block is Hermitianised).  Dense in, dense out (off-sector elements of
+ * y are zero); trace is HEOMQ_TRACE_RDO or HEOMQ_TRACE_NONE.  No counterpart
+ * in the reference: it is the operator the device solvers iterate with. */
+int heomq_apply_packed(heomq_ctx* c, const double* x, double* y, int trace, int transpose, void* stream);
+/* The same projected operator on packed device vectors (ptotal complex
+ * elements each, see heomq_packed_info), and the dense <-> packed maps. */
+int heomq_apply_packed_raw(heomq_ctx* c, const double* xp, double* yp, int trace, int transpose, void* stream);
+int heomq_apply_packed_raw_f32(heomq_ctx* c, const float* xp, float* yp, int trace, int transpose, void* stream);
+int heomq_pack(heomq_ctx* c, const double* dense, double* packed, void* stream);
+int heomq_unpack(heomq_ctx* c, const double* packed, double* dense, void* stream);
+/* info[0] = packed complex elements, [1] = sectors, [2] = N conserved,
+ * [3] = S_z conserved, [4] = dense complex elements N*d*d. */
+int heomq_packed_info(const heomq_ctx* c, int64_t* info);
+
 /* Host-pointer apply: the calc_ax_tfqmr_omp(dim0, rcgtmp, rcgrhs) drop-in
  * (copies in, applies, copies out).  dim0 must equal d. */
 int heomq_calc_ax_tfqmr(heomq_ctx* c, int32_t dim0, const double* x_host,

@@ -707,34 +707,71 @@ def steady_state(model: Model, tol=1e-12, maxiter=20000, trace_row="rdo"):
     return x0 + from_real(xr, model), it, rel
 
 
-def fock_parity(model: Model):
-    """Electron-number parity of each occupation-basis state (calcams.f90 ordering)."""
-    par = []
+def fock_quantum_numbers(model: Model):
+    """Electron number N and 2*S_z of each occupation-basis state (calcams.f90 ordering)."""
+    N, S2 = [], []
     for i in range(model.d):
-        n, v = 0, i
+        n = s2 = 0
+        v = i
         for _ in range(model.norbs):
             dig = v % 4
-            n += (dig == 1) + (dig == 2) + 2 * (dig == 3)
+            up, dn = dig in (1, 3), dig in (2, 3)
+            n += up + dn
+            s2 += up - dn
             v //= 4
-        par.append(n % 2)
-    return np.array(par)
+        N.append(n)
+        S2.append(s2)
+    return np.array(N), np.array(S2)
+
+
+def fock_parity(model: Model):
+    return fock_quantum_numbers(model)[0] % 2
+
+
+def ado_sectors(model: Model):
+    """(dN, d2Sz) carried by every ADO: sum over its drawers of q = +1 (sigma=+) / -1,
+    times +-1 for the spin in the second component."""
+    q = np.where(model.sgn == 0, 1, -1)
+    qs = q * np.where(model.spin == 0, 1, -1)
+    dN = np.zeros(model.N, np.int64)
+    dS = np.zeros(model.N, np.int64)
+    for n in range(1, model.L + 1):
+        K = model.tiers[n]
+        dN[model.first[n]:model.first[n + 1]] = q[K].sum(axis=1)
+        dS[model.first[n]:model.first[n + 1]] = qs[K].sum(axis=1)
+    return dN, dS
+
+
+def conserved(model: Model):
+    """Which of N, S_z the Hamiltonian conserves (H never mixes states that differ)."""
+    N, S2 = fock_quantum_numbers(model)
+    nz = np.abs(model.H) > 0
+    return bool(np.all(N[:, None] == N[None, :]) or np.all((N[:, None] == N[None, :]) | ~nz)), \
+        bool(np.all((S2[:, None] == S2[None, :]) | ~nz))
 
 
 def physical_projection(model: Model, x, hermitian=True):
     """Project onto the sector that holds every physical ADO vector.
 
-    A tier-n ADO couples states i, j only when parity(i) + parity(j) + n is
-    even (each c or c^+ changes the electron number by one), and the tier-0
-    block is Hermitian.  The Liouvillian maps this sector into itself.
+    Each c (c^+) lowers (raises) N by one and 2S_z by +-1, H conserves N (and
+    S_z when it has no spin-flip term), so element (i, j) of ADO K can be
+    nonzero only if N_i - N_j = dN_K and 2(S_z,i - S_z,j) = d2Sz_K (the sums of
+    q = +-1 over K's drawers); the tier-0 block is Hermitian.  The Liouvillian
+    maps this sector into itself.
     """
-    par = fock_parity(model)
+    N, S2 = fock_quantum_numbers(model)
+    cN, cS = conserved(model)
+    dN, dS = ado_sectors(model)
     y = x.copy()
     if hermitian:
         y[0] = 0.5 * (y[0] + y[0].conj().T)
-    for n in range(model.L + 1):
-        mask = ((par[:, None] + par[None, :] + n) % 2) == 0
-        y[model.first[n]:model.first[n + 1]] *= mask
-    return y
+    okN = (N[:, None] - N[None, :])[None] == dN[:, None, None]
+    if not cN:
+        okN = ((N[:, None] - N[None, :])[None] - dN[:, None, None]) % 2 == 0
+    ok = okN
+    if cS:
+        ok = ok & ((S2[:, None] - S2[None, :])[None] == dS[:, None, None])
+    return y * ok
 
 
 def dense_matrix(model: Model, trace_row="last", transpose=False):
