@@ -82,6 +82,7 @@ struct heomq_ctx {
   std::vector<int32_t> lower_drawers;
   // solver workspace
   DevBuf ws64, ws64b, ws32, ws_io, scal, partial, red;
+  DevBuf ws_host;                    // dense state for the host-pointer solve
   DevBuf parity;                     // electron-number parity of each Fock state
   // symmetry-packed layout (solver path)
   DevBuf p_off, p_sec, p_elems, p_pos, p_block, p_lblock, p_entf, p_entt, p_hrow, p_hq, p_hc, p_maps;

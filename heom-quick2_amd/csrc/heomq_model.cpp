@@ -551,14 +551,26 @@ int model_from_reference(
     const int n = it - 1;
     m.first[n] = nfirst[it - 1] - 1;
     const int64_t cnt = nlast[it - 1] - nfirst[it - 1] + 1;
+    if (cnt < (n == 0 ? 1 : 0) || m.first[n] != (n == 0 ? 0 : nlast[it - 2]) || nlast[it - 1] > nunk)
+      return fail(HEOMQ_BAD_ARG, "nfirst/nlast do not tile 1..nunk tier by tier");
     if (n > 0) {
       m.tiers[n].resize((size_t)cnt * n);
       for (int64_t a = 0; a < cnt; ++a)
-        for (int r = 0; r < n; ++r)
-          m.tiers[n][a * n + r] = indextable[ifirst[it - 1] - 1 + a * n + r] - 1;
+        for (int r = 0; r < n; ++r) {
+          const int32_t k = indextable[ifirst[it - 1] - 1 + a * n + r] - 1;
+          // drawer tuples are sorted, ids in 1..nvar (fermion/sortindextable3.f90)
+          if (k < 0 || k >= M || (r > 0 && k < m.tiers[n][a * n + r - 1]))
+            return fail(HEOMQ_BAD_ARG, "indextable: drawer id out of range or tuple not sorted (tier " +
+                                           std::to_string(n) + ", ADO " + std::to_string(a + 1) + ")");
+          m.tiers[n][a * n + r] = k;
+        }
     }
   }
   m.first[m.L + 1] = nunk;
+  if (nlast[ntier - 1] != nunk) return fail(HEOMQ_BAD_ARG, "nlast(ntier) != nunk");
+  auto bad_entry = [&](int32_t nb, int8_t it, bool upper) {
+    return nb < (upper ? -1 : 0) || nb >= nunk || (it != 0 && it != 1);
+  };
   // coupling
   m.lower.clear();
   m.lower_off.assign(m.N + 1, 0);
@@ -567,6 +579,8 @@ int model_from_reference(
   const int half = M / 2;
   for (int e = 0; e < M; ++e) {        // tier 0 (calc_ax_tfqmr_omp.f90:108-149 via indexdraw)
     const int pe = e < half ? e : e - half;
+    if (m.L >= 1 && (indexdraw[pe] < 1 || indexdraw[pe] > nlast[1] - nfirst[1] + 1))
+      return fail(HEOMQ_BAD_ARG, "indexdraw out of tier-1 range");
     const int32_t nb = (int32_t)(nfirst[1] - 1 + indexdraw[pe] - 1);
     m.upper[e] = UpperEntry{nb, (int8_t)(e < half ? 0 : 1), 1, 1};
   }
@@ -575,13 +589,19 @@ int model_from_reference(
     for (int64_t g = m.first[n]; g < m.first[n + 1]; ++g) {
       m.lower_off[g] = (int64_t)m.lower.size();
       int64_t lnl = index_coef_ref[g] - 1;   // 0-based position into the coupling stream
-      for (int r = 0; r < n; ++r, ++lnl)
-        m.lower.push_back(LowerEntry{(int32_t)(indexcoef[lnl] - 1), itypecoef[lnl], ifactfcoef[lnl],
-                                     ifactrcoef[lnl], m.tiers[n][(g - m.first[n]) * n + r]});
+      if (lnl < 0) return fail(HEOMQ_BAD_ARG, "index_coef_ref < 1");
+      for (int r = 0; r < n; ++r, ++lnl) {
+        const int32_t nb = (int32_t)(indexcoef[lnl] - 1);
+        if (bad_entry(nb, itypecoef[lnl], false)) return fail(HEOMQ_BAD_ARG, "indexcoef: bad lower-tier entry");
+        m.lower.push_back(LowerEntry{nb, itypecoef[lnl], ifactfcoef[lnl], ifactrcoef[lnl],
+                                     m.tiers[n][(g - m.first[n]) * n + r]});
+      }
       if (n < m.L)
-        for (int e = 0; e < M; ++e, ++lnl)
-          m.upper[(size_t)g * M + e] = UpperEntry{(int32_t)(indexcoef[lnl] - 1), itypecoef[lnl],
-                                                  ifactfcoef[lnl], ifactrcoef[lnl]};
+        for (int e = 0; e < M; ++e, ++lnl) {
+          const int32_t nb = (int32_t)(indexcoef[lnl] - 1);
+          if (bad_entry(nb, itypecoef[lnl], true)) return fail(HEOMQ_BAD_ARG, "indexcoef: bad upper-tier entry");
+          m.upper[(size_t)g * M + e] = UpperEntry{nb, itypecoef[lnl], ifactfcoef[lnl], ifactrcoef[lnl]};
+        }
     }
   }
   m.lower_off[m.N] = (int64_t)m.lower.size();

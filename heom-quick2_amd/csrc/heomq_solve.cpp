@@ -396,7 +396,7 @@ int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats
     const double j = std::cyl_bessel_j((double)k, arg);
     J.push_back(j);
     sum += std::fabs(j);
-    if (k > arg && std::fabs(j) / sum < 1e-21) break;
+    if (std::fabs(j) / sum < 1e-21) break;   // the term that meets the threshold is kept
   }
   const int ncheby = (int)J.size();
   std::vector<double*> v;
@@ -490,6 +490,18 @@ int heomq_calc_atrx(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_
 int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host) {
   if (!c || !rhotmp_host || !rhorhs_host) return fail(HEOMQ_BAD_ARG, "null argument");
   return host_apply(c, rhotmp_host, rhorhs_host, HEOMQ_TRACE_NONE, false);
+}
+
+int heomq_solve_steady_host(heomq_ctx* c, double* rho_host, double tol, int maxiter, double* stats) {
+  if (!c || !rho_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  const int64_t n = 2 * c->N * c->d * c->d;
+  HIPCHECK(c->ws_host.alloc(sizeof(double) * (size_t)n));
+  double* x = c->ws_host.as<double>();
+  HIPCHECK(hipMemcpy(x, rho_host, sizeof(double) * n, hipMemcpyHostToDevice));
+  const int rc = heomq_solve_steady(c, x, nullptr, tol, maxiter, HEOMQ_TRACE_RDO, 0, stats, nullptr);
+  if (rc != HEOMQ_OK && rc != HEOMQ_NOT_CONVERGED) return rc;
+  HIPCHECK(hipMemcpy(rho_host, x, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return rc;
 }
 
 }  // extern "C"

@@ -1,0 +1,198 @@
+!
+! Drop-in Fortran layer over libheomq (ISO_C_BINDING).
+!
+! The reference calls its Liouvillian apply through fixed subroutine
+! signatures from its solvers (tfqmr_double/solve_tfqmr.f90:127,140,202,204,
+! 267,269,...; bicg/solvebicg.f90:69,98,119; td/intstp.f90:32-93).  This file
+! exports the same names and argument meanings, backed by the MI355X engine:
+!
+!   calc_ax_tfqmr_omp(dim0, rcgtmp, rcgrhs)   tfqmr_double/calc_ax_tfqmr_omp.f90:1,15-17
+!   calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs)  bicg/calc_atrx_bicg_omp.f90:1,35-36
+!   calcderiv_omp(tt)                         td/calcderiv_omp.f90:1-7 (module arrays
+!                                             rhotmp -> rhorhs, registered once)
+!   solve_tfqmr_gpu(rho, crit, maxit, info)   solver-level replacement of solve_tfqmr
+!
+! heomq_fortran_init uploads the reference's own module arrays once (after
+! buildcoefindex), as heomq_model_from_reference + heomq_ctx_create.  Errors
+! print to unit 6 and stop (ERROR STOP, so the exit status is nonzero), as
+! the reference's own checks do
+! (e.g. calc_ax_tfqmr_omp.f90:44-48).
+!
+module heomq_iface
+  use iso_c_binding
+  implicit none
+  type(c_ptr), save :: heomq_model_h = c_null_ptr
+  type(c_ptr), save :: heomq_ctx_h = c_null_ptr
+  type(c_ptr), save :: td_rhotmp = c_null_ptr, td_rhorhs = c_null_ptr
+
+  interface
+    function c_last_error() bind(C, name='heomq_last_error') result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+
+    integer(c_int) function c_model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igs, &
+        indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
+        indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, out) &
+        bind(C, name='heomq_model_from_reference')
+      import :: c_int, c_int32_t, c_int64_t, c_int8_t, c_double, c_double_complex, c_ptr
+      integer(c_int32_t), value :: nrho, ntier, norbs, nspin, nalf, ncor, igs
+      integer(c_int64_t), value :: nunk
+      integer(c_int32_t) :: indextable(*), indexdraw(*), ilead(*), mspin(*), morbs(*), mpm(*)
+      integer(c_int64_t) :: nfirst(*), nlast(*), ifirst(*), indexcoef(*), index_coef_ref(*)
+      integer(c_int8_t) :: itypecoef(*), ifactfcoef(*), ifactrcoef(*)
+      complex(c_double_complex) :: cb(*), cd(*), cgamma(*), hs(*)
+      real(c_double) :: dipm(*), eleadinfty(*), amsall(*)
+      type(c_ptr) :: out
+    end function
+
+    integer(c_int) function c_ctx_create(m, device, out) bind(C, name='heomq_ctx_create')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: device
+      type(c_ptr) :: out
+    end function
+
+    integer(c_int) function c_calc_ax(ctx, dim0, x, y) bind(C, name='heomq_calc_ax_tfqmr')
+      import :: c_int, c_int32_t, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dim0
+      complex(c_double_complex) :: x(*), y(*)
+    end function
+
+    integer(c_int) function c_calc_atrx(ctx, dim0, x, y) bind(C, name='heomq_calc_atrx')
+      import :: c_int, c_int32_t, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dim0
+      complex(c_double_complex) :: x(*), y(*)
+    end function
+
+    integer(c_int) function c_calcderiv(ctx, x, y) bind(C, name='heomq_calcderiv')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, x, y
+    end function
+
+    integer(c_int) function c_solve_host(ctx, rho, crit, maxit, stats) bind(C, name='heomq_solve_steady_host')
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: rho(*)
+      real(c_double), value :: crit
+      integer(c_int), value :: maxit
+      real(c_double) :: stats(*)
+    end function
+  end interface
+
+contains
+
+  subroutine heomq_fail(where, rc)
+    character(len=*), intent(in) :: where
+    integer(c_int), intent(in) :: rc
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: k
+    type(c_ptr) :: p
+    p = c_last_error()
+    call c_f_pointer(p, msg, [512])
+    text = ' '
+    do k = 1, 512
+      if (msg(k) == c_null_char) exit
+      text(k:k) = msg(k)
+    end do
+    write(6,*)
+    write(6,*) ' error! ', where, ' status ', rc, ' : ', trim(text)
+    error stop 1
+  end subroutine heomq_fail
+
+end module heomq_iface
+
+! One-time upload of the reference's module arrays (module/matmod.f90).
+subroutine heomq_fortran_init(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady, &
+    indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
+    indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, device)
+  use heomq_iface
+  implicit none
+  integer(c_int32_t), intent(in) :: nrho, ntier, norbs, nspin, nalf, ncor, igroundsteady, device
+  integer(c_int64_t), intent(in) :: nunk
+  integer(c_int32_t), intent(in) :: indextable(*), indexdraw(*), ilead(*), mspin(*), morbs(*), mpm(*)
+  integer(c_int64_t), intent(in) :: nfirst(*), nlast(*), ifirst(*), indexcoef(*), index_coef_ref(*)
+  integer(c_int8_t), intent(in) :: itypecoef(*), ifactfcoef(*), ifactrcoef(*)
+  complex(c_double_complex), intent(in) :: cb(*), cd(*), cgamma(*), hs(*)
+  real(c_double), intent(in) :: dipm(*), eleadinfty(*), amsall(*)
+  integer(c_int) :: rc
+  rc = c_model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady, &
+       indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
+       indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, heomq_model_h)
+  if (rc /= 0) call heomq_fail('heomq_fortran_init (tables)', rc)
+  rc = c_ctx_create(heomq_model_h, int(device, c_int), heomq_ctx_h)
+  if (rc /= 0) call heomq_fail('heomq_fortran_init (device)', rc)
+end subroutine heomq_fortran_init
+
+! tfqmr_double/calc_ax_tfqmr_omp.f90:1,15-17 -- y = A x with the trace row
+subroutine calc_ax_tfqmr_omp(dim0, rcgtmp, rcgrhs)
+  use heomq_iface
+  implicit none
+  integer, intent(in) :: dim0
+  complex*16, intent(in) :: rcgtmp(*)
+  complex*16, intent(out) :: rcgrhs(*)
+  integer(c_int) :: rc
+  rc = c_calc_ax(heomq_ctx_h, int(dim0, c_int32_t), rcgtmp, rcgrhs)
+  if (rc /= 0) call heomq_fail('calc_ax_tfqmr_omp', rc)
+end subroutine calc_ax_tfqmr_omp
+
+! bicg/calc_atrx_bicg_omp.f90:1,35-36 position -- the real transpose of the
+! TFQMR-convention operator (the reference's own transpose works in BiCG's
+! unknown convention: lower triangle of rho0, calc_atrx_bicg_omp.f90:103-111)
+subroutine calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs)
+  use heomq_iface
+  implicit none
+  integer, intent(in) :: dim0
+  complex*16, intent(inout) :: rcgtmp(*)
+  complex*16, intent(inout) :: rcgrhs(*)
+  integer(c_int) :: rc
+  rc = c_calc_atrx(heomq_ctx_h, int(dim0, c_int32_t), rcgtmp, rcgrhs)
+  if (rc /= 0) call heomq_fail('calc_atrx_bicg_omp', rc)
+end subroutine calc_atrx_bicg_omp
+
+! td/calcderiv_omp.f90 reads module array rhotmp and writes rhorhs; the
+! caller registers their addresses once with heomq_register_td_arrays.
+subroutine heomq_register_td_arrays(rhotmp, rhorhs)
+  use heomq_iface
+  implicit none
+  complex*16, intent(in), target :: rhotmp(*)
+  complex*16, intent(in), target :: rhorhs(*)
+  td_rhotmp = c_loc(rhotmp)
+  td_rhorhs = c_loc(rhorhs)
+end subroutine heomq_register_td_arrays
+
+subroutine calcderiv_omp(tt)
+  use heomq_iface
+  implicit none
+  real*8, intent(in) :: tt
+  integer(c_int) :: rc
+  if (.not. c_associated(td_rhotmp)) then
+    write(6,*) ' error! calcderiv_omp: rhotmp/rhorhs not registered '
+    error stop 1
+  end if
+  rc = c_calcderiv(heomq_ctx_h, td_rhotmp, td_rhorhs)
+  if (rc /= 0) call heomq_fail('calcderiv_omp', rc)
+  if (tt < 0.d0) write(6,*) ' calcderiv_omp: fixed-H propagation, tt ignored '
+end subroutine calcderiv_omp
+
+! Solver-level drop-in for tfqmr_double/solve_tfqmr.f90: rho (nrho,nrho,nunk)
+! holds the initial guess on entry and the steady state on exit; crit is the
+! relative residual target; info(1) = status (0 ok, 4 not converged, 8
+! breakdown as DUTFX), info(2) = iterations.
+subroutine solve_tfqmr_gpu(rho, crit, maxit, info)
+  use heomq_iface
+  implicit none
+  complex*16, intent(inout) :: rho(*)
+  real*8, intent(in) :: crit
+  integer, intent(in) :: maxit
+  integer, intent(out) :: info(2)
+  real(c_double) :: stats(8)
+  integer(c_int) :: rc
+  rc = c_solve_host(heomq_ctx_h, rho, real(crit, c_double), int(maxit, c_int), stats)
+  info(1) = int(rc)
+  info(2) = int(stats(1))
+  if (rc /= 0 .and. rc /= 4) call heomq_fail('solve_tfqmr_gpu', rc)
+end subroutine solve_tfqmr_gpu

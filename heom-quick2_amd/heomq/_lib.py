@@ -72,6 +72,7 @@ SIGNATURES = {
     "heomq_calcderiv": (C.c_int, [P, D, D]),
     "heomq_solve_steady": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, D, P]),
     "heomq_solve_steady_mixed": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_int, C.c_int, D, P]),
+    "heomq_solve_steady_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),
     "heomq_rk4": (C.c_int, [P, P, C.c_double, C.c_int, P]),
     "heomq_chebyshev": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_observables": (C.c_int, [P, P, D, D, P]),

@@ -194,6 +194,14 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol,
 int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double tol,
                              double inner_tol, int maxiter, int trace,
                              double* stats, void* stream);
+/* Host-pointer steady state (the solve_tfqmr drop-in of the Fortran shim):
+ * rho_host (d, d, N) complex holds the initial guess on entry (a trace-one
+ * tier-0 block, Hermitian as tfqmr_double/solve_tfqmr.f90:97-103 makes it)
+ * and the solution on exit;
+ * solved with the tier-0 trace row on the symmetry-packed layout.  stats as
+ * heomq_solve_steady (at least 6 doubles). */
+int heomq_solve_steady_host(heomq_ctx* c, double* rho_host, double tol, int maxiter,
+                            double* stats);
 
 /* ---------------------------------------------------------------------------
  * Propagation (td/intstp.f90 RK4 and td/chebyshev.f90 Chebyshev), fixed H.

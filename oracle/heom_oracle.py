@@ -787,6 +787,72 @@ def dense_matrix(model: Model, trace_row="last", transpose=False):
 
 
 # --------------------------------------------------------------------------
+# Propagation (fixed H)
+# --------------------------------------------------------------------------
+def rk4(model: Model, x, dt, nstep):
+    """Classic RK4 on dx/dt = L x (td/intstp.f90:20-100): k1..k4 from
+    calcderiv at x, x + dt/2 k1, x + dt/2 k2, x + dt k3."""
+    x = x.copy()
+    for _ in range(nstep):
+        k1 = apply(model, x, trace_row=None)
+        k2 = apply(model, x + 0.5 * dt * k1, trace_row=None)
+        k3 = apply(model, x + 0.5 * dt * k2, trace_row=None)
+        k4 = apply(model, x + dt * k3, trace_row=None)
+        x = x + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return x
+
+
+def ado_gamma(model: Model):
+    """gamma_K = sum of the drawers' gamma (incl. steady bias) per ADO."""
+    g = np.zeros(model.N, complex)
+    for n in range(1, model.L + 1):
+        g[model.first[n]:model.first[n + 1]] = model.gdraw[model.tiers[n]].sum(axis=1)
+    return g
+
+
+def cheby_delta(model: Model):
+    """Spectral width of the real part of diag(L) over the hierarchy, the
+    tier-0 block (gamma = 0) included (td/prechebyshev.f90:71-109).  For a
+    real H the commutator adds nothing to Re diag(L)."""
+    re = ado_gamma(model).real
+    return max(0.0, re.max()) - min(0.0, re.min())
+
+
+def cheby_terms(arg, dpre=1e-21):
+    """Bessel weights J_0 .. J_{n-1}(arg), n the first index whose |J_n| /
+    sum_{m<=n} |J_m| < dpre (td/prechebyshev.f90:122-134)."""
+    from scipy.special import jv
+    J, tot, k = [], 0.0, 0
+    while True:
+        j = float(jv(k, arg))
+        J.append(j)
+        tot += abs(j)
+        k += 1
+        if abs(j) / tot < dpre:
+            return np.array(J)
+
+
+def chebyshev(model: Model, x, dt, nstep):
+    """Chebyshev step x <- sum_k c_k J_k(delta dt) P_k (td/chebyshev.f90:126-200):
+    P_0 = x, P_1 = L x / delta, P_k = P_{k-2} + (2/delta) L P_{k-1},
+    c_0 = 1, c_k = 2.  P_k = (-i)^k T_k(iL/delta) x, so the sum is the
+    Jacobi-Anger series of exp(L dt)."""
+    delta = cheby_delta(model)
+    J = cheby_terms(delta * dt)
+    x = x.copy()
+    for _ in range(nstep):
+        pm2 = x
+        pm1 = apply(model, pm2, trace_row=None) / delta
+        out = J[0] * pm2 + 2 * J[1] * pm1
+        for k in range(2, len(J)):
+            pk = pm2 + (2.0 / delta) * apply(model, pm1, trace_row=None)
+            out = out + 2 * J[k] * pk
+            pm2, pm1 = pm1, pk
+        x = out
+    return x, len(J), delta
+
+
+# --------------------------------------------------------------------------
 # Closed form at U = 0: Landauer with the same Lorentzian and Pade Fermi function
 # --------------------------------------------------------------------------
 def landauer_siam(p: Params, npts=400001):

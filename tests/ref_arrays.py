@@ -9,8 +9,9 @@ def reference_arrays(o):
     ntier = L + 1
     nfirst = np.array([o.first[n] + 1 for n in range(ntier)], np.int64)
     nlast = np.array([o.first[n + 1] for n in range(ntier)], np.int64)
-    ifirst, table, pos = [], [0], 1
-    ifirst.append(1)
+    # indextable(1) = 0 is tier 0's empty record (main/ball2drawer.f90:40-44);
+    # tier 1 starts at position 2
+    ifirst, table = [1], [0]
     pos = 2
     for n in range(1, ntier):
         ifirst.append(pos)
@@ -49,7 +50,7 @@ def reference_arrays(o):
             ams[s, m] = o.ams[m][s].T          # C order [s][m][j][i] == Fortran (i,j,m,s)
     return dict(
         nrho=d, nunk=o.N, ntier=ntier, norbs=norbs, nspin=nspin, nalf=nalf, ncor=ncor,
-        igroundsteady=1, indextable=np.array(table[1:], np.int32), nfirst=nfirst, nlast=nlast,
+        igroundsteady=1, indextable=np.array(table, np.int32), nfirst=nfirst, nlast=nlast,
         ifirst=np.array(ifirst, np.int64), indexcoef=np.array(indexcoef, np.int64),
         itypecoef=np.array(itype, np.int8), ifactfcoef=np.array(ff, np.int8),
         ifactrcoef=np.array(fr, np.int8), index_coef_ref=ref, indexdraw=indexdraw,

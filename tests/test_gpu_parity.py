@@ -126,3 +126,23 @@ def test_dot_product_transpose_device():
     lhs = torch.vdot(w.flatten(), ctx.apply(x, trace=heomq.TRACE_LAST).flatten()).real.item()
     rhs = torch.vdot(ctx.apply_transpose(w, trace=heomq.TRACE_LAST).flatten(), x.flatten()).real.item()
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
+def test_rk4_device(name):
+    """td/intstp.f90 RK4 on the device vs the oracle's RK4 (same steps)."""
+    m, ctx, o, torch = setup(name)
+    x = O.seeded_ado_vector(o, 41)
+    xd = ctx.rk4(to_dev(torch, x), 0.02, 3)
+    assert rel(from_dev(xd), O.rk4(o, x, 0.02, 3)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
+def test_chebyshev_device(name):
+    """td/chebyshev.f90 step on the device vs the oracle's recurrence."""
+    m, ctx, o, torch = setup(name)
+    x = O.seeded_ado_vector(o, 42)
+    xd, info = ctx.chebyshev(to_dev(torch, x), 0.02, 2)
+    xo, ncheby, delta = O.chebyshev(o, x, 0.02, 2)
+    assert info["ncheby"] == ncheby and abs(info["delta"] - delta) <= 1e-12 * delta
+    assert rel(from_dev(xd), xo) <= 1e-12

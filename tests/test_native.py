@@ -90,12 +90,27 @@ def test_reference_loader_round_trip(pp, op):
     m_ref = heomq.Model.from_reference(**reference_arrays(o))
     m = heomq.Model(pp)
     assert m_ref.N == m.N and list(m_ref.first) == list(m.first)
+    for n in range(m.L + 1):
+        assert np.array_equal(m_ref.tier(n), m.tier(n))
+    for a, b in zip(m_ref.operators(), m.operators()):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
     lo1, up1 = m_ref.coupling()
     lo2, up2 = m.coupling()
     assert np.array_equal(lo1, lo2)
     assert np.array_equal(up1, up2)
     assert np.abs(m_ref.coefs()[0] - m.coefs()[0]).max() <= 1e-13 * np.abs(m.coefs()[0]).max()
     assert np.abs(m_ref.coefs()[2] - m.coefs()[2]).max() <= 1e-13 * np.abs(m.coefs()[2]).max()
+
+
+def test_reference_loader_rejects_shifted_indextable():
+    # indextable without tier 0's empty record (ball2drawer.f90:40-44) shifts
+    # every tuple; the loader must refuse it instead of reading past the end
+    o = O.Model(O.siam_params(nk=1, ntop=2, lsdraw=True))
+    arr = reference_arrays(o)
+    arr["indextable"] = np.ascontiguousarray(np.concatenate([arr["indextable"][1:], [10 ** 6]]).astype(np.int32))
+    with pytest.raises(heomq.HeomqError) as e:
+        heomq.Model.from_reference(**arr)
+    assert e.value.code == _lib.BAD_ARG
 
 
 def test_reference_loader_rejects_bad_dims():

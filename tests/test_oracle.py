@@ -216,3 +216,58 @@ def test_trace_row_placements_agree_for_odd_top_tier():
     xr, _, _ = O.steady_state(m, tol=1e-12, trace_row="rdo")
     xl, _, _ = O.steady_state(m, tol=1e-12, trace_row="last")
     assert np.abs(xr[0] - xl[0]).max() < 1e-10
+
+
+def _tiny():
+    return O.Model(O.siam_params(nk=1, ntop=2, V=0.3, U=1.5, T=0.2))
+
+
+def test_rk4_is_fourth_order_taylor_polynomial():
+    # for a linear ODE one RK4 step is exactly sum_{k<=4} (h A)^k / k! applied to x
+    o = _tiny()
+    A = O.dense_matrix(o, trace_row=None)
+    x = O.seeded_ado_vector(o, 3)
+    h = 0.05
+    v = O.as_real(x)
+    ref, term = v.copy(), v.copy()
+    for k in range(1, 5):
+        term = h * (A @ term) / k
+        ref = ref + term
+    got = O.as_real(O.rk4(o, x, h, 1))
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_chebyshev_matches_matrix_exponential():
+    # the Jacobi-Anger series of exp(L dt) is exact for any spectrum.  The
+    # damping puts L's spectrum on the real axis, where the Chebyshev
+    # polynomials of iL/delta grow like (1 + sqrt 2)^k, so the 1e-21 Bessel-tail
+    # cut leaves round-off only for delta*dt of order one (the reference's
+    # recommended step); at delta*dt = 15 the same cut is visibly short.
+    from scipy.linalg import expm
+    o = _tiny()
+    A = O.dense_matrix(o, trace_row=None)
+    x = O.seeded_ado_vector(o, 4)
+    for dt in (0.01, 0.05):
+        ref = expm(A * dt) @ O.as_real(x)
+        got, ncheby, delta = O.chebyshev(o, x, dt, 2)
+        ref = expm(A * dt) @ ref
+        assert ncheby > delta * dt
+        assert np.abs(O.as_real(got) - ref).max() <= 1e-13 * np.abs(ref).max()
+        # trace of the RDO is conserved by the exact propagator
+        assert abs(np.trace(got[0]) - np.trace(x[0])) <= 1e-13
+    got, _, _ = O.chebyshev(o, x, 0.5, 1)
+    assert np.abs(O.as_real(got) - expm(A * 0.5) @ O.as_real(x)).max() > 1e-6
+
+
+def test_cheby_delta_is_width_of_real_diagonal():
+    # brute force over the dense matrix's diagonal (real part of L's diagonal)
+    o = _tiny()
+    A = O.dense_matrix(o, trace_row=None)
+    dg = np.diag(A)
+    assert abs(O.cheby_delta(o) - (dg.max() - min(dg.min(), 0.0))) <= 1e-12 * abs(dg).max()
+
+
+def test_bessel_weights_sum_rule():
+    # J_0(z) + 2 sum_k J_2k(z) = 1 (generating function at theta = 0)
+    J = O.cheby_terms(7.3)
+    assert abs(J[0] + 2 * J[2::2].sum() - 1.0) <= 1e-14
