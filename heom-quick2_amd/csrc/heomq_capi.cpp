@@ -183,6 +183,64 @@ void fill_op(heomq_ctx* c, OperatorTables& t, const DevBuf& rec, bool transposed
   o.tb = c->tb;
 }
 
+// i(HX - XH) per symmetry sector as a CSR over packed slots (row t of sector
+// s: the slots q and coefficients v with (i(HX - XH))_t = sum v * X_q).  Built
+// at context creation and again whenever H changes; H must keep the N / S_z
+// conservation the sector layout was built for.
+int build_packed_h(heomq_ctx* c, const Model& m) {
+  const int d = m.d, dd = d * d, S = c->nsec;
+  for (int i = 0; i < d; ++i)
+    for (int k = 0; k < d; ++k)
+      if (m.H[i + (size_t)k * d] != cplx(0, 0)) {
+        if ((c->cons_n && c->fock_n[i] != c->fock_n[k]) || (c->cons_s && c->fock_s[i] != c->fock_s[k]))
+          return fail(HEOMQ_UNSUPPORTED,
+                      "H no longer conserves the quantum numbers the packed layout uses: create a new context");
+      }
+  const auto& elems = c->sec_elems_h;
+  const auto& pos = c->sec_pos_h;
+  std::vector<int32_t> hrow((size_t)S * (dd + 1), 0);
+  std::vector<int16_t> hq;
+  std::vector<double> hc;
+  for (int s = 0; s < S; ++s) {
+    const int nnz = c->sec_nnz_h[s];
+    for (int t = 0; t < nnz; ++t) {
+      hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
+      const int e = elems[(size_t)s * dd + t];
+      const int i = e % d, j = e / d;
+      std::vector<std::pair<int, cplx>> terms;
+      auto addterm = [&](int q, cplx v) {
+        if (q < 0 || v == cplx(0, 0)) return;
+        for (auto& tm : terms)
+          if (tm.first == q) { tm.second += v; return; }
+        terms.push_back({q, v});
+      };
+      for (int k = 0; k < d; ++k) {
+        const cplx hik = m.H[i + (size_t)k * d], hkj = m.H[k + (size_t)j * d];
+        if (hik != cplx(0, 0)) addterm(pos[(size_t)s * dd + k + j * d], cplx(0, 1) * hik);
+        if (hkj != cplx(0, 0)) addterm(pos[(size_t)s * dd + i + k * d], -cplx(0, 1) * hkj);
+      }
+      for (auto& tm : terms) {
+        if (tm.second == cplx(0, 0)) continue;
+        hq.push_back((int16_t)tm.first);
+        hc.push_back(tm.second.real());
+        hc.push_back(tm.second.imag());
+      }
+    }
+    for (int t = nnz; t <= dd; ++t) hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
+  }
+  if (hq.empty()) { hq.push_back(0); hc.push_back(0); hc.push_back(0); }
+  HIPCHECK(upload(c->p_hrow, hrow));
+  HIPCHECK(upload(c->p_hq, hq));
+  HIPCHECK(upload(c->p_hc, hc));
+  for (int w = 0; w < 2; ++w) {
+    DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
+    p.hrow = c->p_hrow.as<int32_t>();
+    p.hq = c->p_hq.as<int16_t>();
+    p.hc = c->p_hc.as<double>();
+  }
+  return HEOMQ_OK;
+}
+
 // Symmetry-packed layout: sectors, packed offsets, packed coupling entries.
 int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
                  const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt) {
@@ -250,6 +308,11 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
         elems[(size_t)s * dd + nnz[s]] = (int16_t)(i + j * d);
         nnz[s]++;
       }
+  c->sec_elems_h = elems;
+  c->sec_pos_h = pos;
+  c->sec_nnz_h = nnz;
+  c->fock_n = fN;
+  c->fock_s = fS;
   std::vector<int64_t> off(m.N + 1, 0);
   for (int64_t g = 0; g < m.N; ++g) off[g + 1] = off[g] + nnz[sec[g]];
   const int64_t total = off[m.N];
@@ -278,37 +341,6 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     }
   }
   HIPCHECK(upload(c->p_lblock, lblock));
-  // i(HX - XH) per sector as CSR over packed slots
-  std::vector<int32_t> hrow((size_t)S * (dd + 1), 0);
-  std::vector<int16_t> hq;
-  std::vector<double> hc;
-  for (int s = 0; s < S; ++s) {
-    for (int t = 0; t < nnz[s]; ++t) {
-      hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
-      const int e = elems[(size_t)s * dd + t];
-      const int i = e % d, j = e / d;
-      std::vector<std::pair<int, cplx>> terms;
-      auto addterm = [&](int q, cplx v) {
-        if (q < 0 || v == cplx(0, 0)) return;
-        for (auto& tm : terms)
-          if (tm.first == q) { tm.second += v; return; }
-        terms.push_back({q, v});
-      };
-      for (int k = 0; k < d; ++k) {
-        const cplx hik = m.H[i + (size_t)k * d], hkj = m.H[k + (size_t)j * d];
-        if (hik != cplx(0, 0)) addterm(pos[(size_t)s * dd + k + j * d], cplx(0, 1) * hik);
-        if (hkj != cplx(0, 0)) addterm(pos[(size_t)s * dd + i + k * d], -cplx(0, 1) * hkj);
-      }
-      for (auto& tm : terms) {
-        if (tm.second == cplx(0, 0)) continue;
-        hq.push_back((int16_t)tm.first);
-        hc.push_back(tm.second.real());
-        hc.push_back(tm.second.imag());
-      }
-    }
-    for (int t = nnz[s]; t <= dd; ++t) hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
-  }
-  if (hq.empty()) { hq.push_back(0); hc.push_back(0); hc.push_back(0); }
   // gather maps: output slot t of sector s -> signed slot of the neighbour (sector s2)
   std::vector<int16_t> maps;
   std::map<std::tuple<int, int, int, int, int>, int32_t> mapidx;
@@ -370,12 +402,10 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
   HIPCHECK(upload(c->p_block, block));
   HIPCHECK(upload(c->p_entf, pentf));
   HIPCHECK(upload(c->p_entt, pentt));
-  HIPCHECK(upload(c->p_hrow, hrow));
-  HIPCHECK(upload(c->p_hq, hq));
-  HIPCHECK(upload(c->p_hc, hc));
   HIPCHECK(upload(c->p_maps, maps));
-  c->ptotal = total;
   c->nsec = S;
+  if (int rc = build_packed_h(c, m)) return rc;
+  c->ptotal = total;
   const int tslot = pos[(size_t)sec[0] * dd + (d - 1) + (d - 1) * d];
   for (int w = 0; w < 2; ++w) {
     DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
@@ -389,9 +419,6 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     p.rowptr = (w == 0 ? c->fwd.rowptr : c->trn.rowptr).as<int64_t>();
     p.ent = (w == 0 ? c->p_entf : c->p_entt).as<DevEntryP>();
     p.trace_slot = tslot;
-    p.hrow = c->p_hrow.as<int32_t>();
-    p.hq = c->p_hq.as<int16_t>();
-    p.hc = c->p_hc.as<double>();
     p.maps = c->p_maps.as<int16_t>();
     p.n_heavy = n_heavy;
     p.light_begin = light_begin;
@@ -597,6 +624,7 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
   if (!c || !h) return fail(HEOMQ_BAD_ARG, "null argument");
   if (h->m.N != c->N || h->m.M != c->M) return fail(HEOMQ_BAD_ARG, "model shape differs from context");
   HIPCHECK(hipSetDevice(c->device));
+  if (int rc = build_packed_h(c, h->m)) return rc;   // checks conservation before anything changes
   if (int rc = upload_coefs(c, h->m)) return rc;
   fill_op(c, c->fwd, c->rec, false, -1.0);
   fill_op(c, c->trn, c->rec, true, +1.0);

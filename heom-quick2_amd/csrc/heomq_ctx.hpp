@@ -83,6 +83,9 @@ struct heomq_ctx {
   // solver workspace
   DevBuf ws64, ws64b, ws32, ws_io, scal, partial, red;
   DevBuf ws_host;                    // dense state for the host-pointer solve
+  // host copies of the sector tables (the packed H table is rebuilt from them)
+  std::vector<int16_t> sec_elems_h, sec_pos_h;
+  std::vector<int> sec_nnz_h, fock_n, fock_s;
   DevBuf parity;                     // electron-number parity of each Fock state
   // symmetry-packed layout (solver path)
   DevBuf p_off, p_sec, p_elems, p_pos, p_block, p_lblock, p_entf, p_entt, p_hrow, p_hq, p_hc, p_maps;

@@ -787,6 +787,78 @@ def dense_matrix(model: Model, trace_row="last", transpose=False):
 
 
 # --------------------------------------------------------------------------
+# Sensitivities of the steady current (adjoint; finite differences)
+# --------------------------------------------------------------------------
+def current_weights(model: Model, lead=0):
+    """g with I_lead (summed over spin) = Re <g, x> for every x, found by
+    probing :func:`current` (real-linear in x) on each element of the tier-1
+    ADOs, the only ones it reads (td/calcj.f90:15-37)."""
+    g = np.zeros((model.N, model.d, model.d), complex)
+    e = np.zeros_like(g)
+    for a in range(model.first[1], model.first[2]):
+        for i in range(model.d):
+            for j in range(model.d):
+                for ph in (1.0, 1j):
+                    e[a, i, j] = ph
+                    g[a, i, j] += ph * current(model, e)[lead].sum()
+                    e[a, i, j] = 0
+    return g
+
+
+def _projected(model: Model, transpose=False):
+    proj = lambda v: as_real(physical_projection(model, from_real(v, model)))
+    f = apply_transpose if transpose else apply
+    mv = lambda v: proj(as_real(f(model, from_real(proj(v), model), trace_row="rdo")))
+    return mv, proj
+
+
+def current_gradient_adjoint(params_fn, theta, lead=0, tol=1e-13):
+    """dI_lead/dtheta from one transposed solve.
+
+    A(theta) x = e_trace with the theta-independent trace row; I = Re<g, x>;
+    A^T lam = g (the transposed projected operator Pi A^T Pi, built on
+    apply_transpose -- the role calc_atrx_bicg_omp plays in the reference's
+    only transposed path); dI/dtheta_i = -Re<lam, Pi (dA/dtheta_i) x> with the
+    trace-row element of the derivative zero.  Every theta_i used here enters
+    A affinely (H linear in the level energies and U, cb/cd linear in Gamma,
+    gamma affine in the bias), so dA/dtheta_i x = A(theta + e_i) x - A(theta) x
+    exactly.  Returns (I, grad)."""
+    theta = np.asarray(theta, float)
+    model = Model(params_fn(theta))
+    x, _, _ = steady_state(model, tol=tol)
+    g = current_weights(model, lead)
+    mvT, proj = _projected(model, transpose=True)
+    lam_r, _, _ = tfqmr(mvT, proj(as_real(g)), tol=tol, project=proj)
+    lam = from_real(lam_r, model)
+    d = model.d
+    ax = apply(model, x, trace_row=None)
+    grad = np.zeros(len(theta))
+    for i in range(len(theta)):
+        tp = theta.copy()
+        tp[i] += 1.0
+        z = physical_projection(model, apply(Model(params_fn(tp)), x, trace_row=None) - ax)
+        z[0, d - 1, d - 1] = 0.0
+        grad[i] = -np.real(np.vdot(lam, z))
+    return float(current(model, x)[lead].sum()), grad
+
+
+def current_gradient_fd(params_fn, theta, lead=0, h=1e-5, tol=1e-13):
+    """Central finite differences of I_lead with a full steady solve per point."""
+    theta = np.asarray(theta, float)
+    grad = np.zeros(len(theta))
+    for i in range(len(theta)):
+        vals = []
+        for sgn in (1.0, -1.0):
+            tp = theta.copy()
+            tp[i] += sgn * h
+            m = Model(params_fn(tp))
+            x, _, _ = steady_state(m, tol=tol)
+            vals.append(current(m, x)[lead].sum())
+        grad[i] = (vals[0] - vals[1]) / (2 * h)
+    return grad
+
+
+# --------------------------------------------------------------------------
 # Propagation (fixed H)
 # --------------------------------------------------------------------------
 def rk4(model: Model, x, dt, nstep):

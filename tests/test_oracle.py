@@ -271,3 +271,39 @@ def test_bessel_weights_sum_rule():
     # J_0(z) + 2 sum_k J_2k(z) = 1 (generating function at theta = 0)
     J = O.cheby_terms(7.3)
     assert abs(J[0] + 2 * J[2::2].sum() - 1.0) <= 1e-14
+
+
+def _siam_theta(th, **kw):
+    return O.siam_params(eps=th[0], U=th[1], gl=th[2], gr=th[3], V=th[4], **kw)
+
+
+def test_parameters_enter_affinely():
+    # the adjoint's exact derivative rule dA x = A(theta + e_i) x - A(theta) x
+    # needs A affine in each of (eps, U, Gamma_L, Gamma_R, V)
+    th = np.array([-0.7, 1.5, 0.1, 0.15, 0.3])
+    pf = lambda t: _siam_theta(t, nk=1, ntop=2, T=0.2)
+    m0 = O.Model(pf(th))
+    x = O.seeded_ado_vector(m0, 5)
+    a0 = O.apply(m0, x, trace_row=None)
+    for i in range(5):
+        t1, t2 = th.copy(), th.copy()
+        t1[i] += 1.0
+        t2[i] += 2.5
+        d1 = O.apply(O.Model(pf(t1)), x, trace_row=None) - a0
+        d2 = O.apply(O.Model(pf(t2)), x, trace_row=None) - a0
+        assert np.abs(d2 - 2.5 * d1).max() <= 1e-12 * max(np.abs(a0).max(), 1.0)
+
+
+def test_current_weights_reproduce_current():
+    m = O.Model(O.siam_params(nk=1, ntop=2, V=0.3))
+    g = O.current_weights(m, 0)
+    x = O.seeded_ado_vector(m, 6)
+    assert abs(np.real(np.vdot(g, x)) - O.current(m, x)[0].sum()) <= 1e-9 * np.abs(g).max()
+
+
+def test_adjoint_gradient_matches_finite_differences():
+    th = np.array([-0.7, 1.5, 0.1, 0.15, 0.3])
+    pf = lambda t: _siam_theta(t, nk=1, ntop=2, T=0.2)
+    I, ga = O.current_gradient_adjoint(pf, th)
+    gf = O.current_gradient_fd(pf, th, h=1e-5)
+    assert np.abs(ga - gf).max() <= 1e-6 * np.abs(gf).max()
