@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-applies", type=int, default=2)
+    ap.add_argument("--no-ref", action="store_true",
+                    help="skip the long reference solve (rdo_err is then not measured); for profiling runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the batched and single-hierarchy "
@@ -71,8 +73,10 @@ def main():
     dev = torch.device("cuda:0")
 
     # reference state for the RDO error: a much longer solve of the same system
-    xref, stref = ctx.solve_steady(tol=1e-14, maxiter=6000, raise_on_fail=False)
-    rho_ref = xref[0].clone()
+    rho_ref, stref = None, None
+    if not args.no_ref:
+        xref, stref = ctx.solve_steady(tol=1e-14, maxiter=6000, raise_on_fail=False)
+        rho_ref = xref[0].clone()
 
     for _ in range(args.warmup):
         ctx.solve_steady(tol=args.tol)
@@ -83,7 +87,8 @@ def main():
     for _ in range(args.steps):
         x, st = ctx.solve_steady(tol=args.tol)
         stats.append(st)
-        rdo_err = max(rdo_err, (x[0] - rho_ref).abs().max().item())
+        if rho_ref is not None:
+            rdo_err = max(rdo_err, (x[0] - rho_ref).abs().max().item())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     solves_per_s = args.steps / elapsed
@@ -135,8 +140,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wname, "d": d, "M": model.M, "L": model.L, "N_ado": N, "tol": args.tol,
                    "solver": "TFQMR (device-resident, FP64)"},
-        "rdo_err": rdo_err,
-        "rdo_ref": {"iters": stref["iters"], "relres": stref["relres"]},
+        "rdo_err": rdo_err if rho_ref is not None else None,
+        "rdo_ref": {"iters": stref["iters"], "relres": stref["relres"]} if stref else None,
         "iters_per_solve": iters,
         "applies_per_solve": applies,
         "current_L": float(cur[0].sum()),

@@ -228,22 +228,72 @@ int build_packed_h(heomq_ctx* c, const Model& m) {
     }
     for (int t = nnz; t <= dd; ++t) hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
   }
+  // fixed-width copy when all rows are short
+  int width = 0;
+  for (int s = 0; s < S; ++s)
+    for (int t = 0; t < dd; ++t)
+      width = std::max(width, hrow[(size_t)s * (dd + 1) + t + 1] - hrow[(size_t)s * (dd + 1) + t]);
+  const int kMaxEll = 16;
+  const int ew = width <= kMaxEll ? std::max(width, 1) : 0;
+  std::vector<int16_t> hq_ell;
+  std::vector<double> hc_ell;
+  if (ew > 0) {
+    hq_ell.assign((size_t)S * dd * ew, 0);
+    hc_ell.assign((size_t)S * dd * ew * 2, 0.0);
+    for (int s = 0; s < S; ++s)
+      for (int t = 0; t < dd; ++t) {
+        const int r0 = hrow[(size_t)s * (dd + 1) + t], r1 = hrow[(size_t)s * (dd + 1) + t + 1];
+        for (int u = r0; u < r1; ++u) {
+          const size_t at = ((size_t)s * dd + t) * ew + (u - r0);
+          hq_ell[at] = hq[u];
+          hc_ell[2 * at] = hc[2 * (size_t)u];
+          hc_ell[2 * at + 1] = hc[2 * (size_t)u + 1];
+        }
+      }
+  } else {
+    hq_ell.assign(1, 0);
+    hc_ell.assign(2, 0.0);
+  }
   if (hq.empty()) { hq.push_back(0); hc.push_back(0); hc.push_back(0); }
   HIPCHECK(upload(c->p_hrow, hrow));
   HIPCHECK(upload(c->p_hq, hq));
   HIPCHECK(upload(c->p_hc, hc));
+  HIPCHECK(upload(c->p_hq_ell, hq_ell));
+  HIPCHECK(upload(c->p_hc_ell, hc_ell));
   for (int w = 0; w < 2; ++w) {
     DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
     p.hrow = c->p_hrow.as<int32_t>();
     p.hq = c->p_hq.as<int16_t>();
     p.hc = c->p_hc.as<double>();
+    p.hell_w = ew;
+    p.hq_ell = c->p_hq_ell.as<int16_t>();
+    p.hc_ell = c->p_hc_ell.as<double>();
   }
   return HEOMQ_OK;
 }
 
 // Symmetry-packed layout: sectors, packed offsets, packed coupling entries.
-int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
-                 const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt) {
+// Host-side symmetry-packed layout (sectors, offsets, block tables, packed
+// entries and gather maps); built without a device, uploaded by build_packed.
+struct PackedHost {
+  bool cons_n = true, cons_s = true;
+  std::vector<int> fN, fS;           // electron number and 2 S_z of each Fock state
+  int S = 0;                         // sectors
+  std::vector<int16_t> elems, pos;   // [S][d*d]
+  std::vector<int> nnz;              // elements per sector
+  std::vector<uint8_t> sec;          // sector of each ADO
+  std::vector<int64_t> off;          // N + 1
+  int64_t total = 0;
+  std::vector<int32_t> block, lblock, lnz;
+  int64_t n_heavy = 0, light_begin = 0;
+  std::vector<DevEntryP> pentf, pentt;
+  std::vector<int16_t> maps;
+  int tslot = -1;
+  int max_light[2] = {0, 0};
+};
+
+int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
+                      const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt, PackedHost& ph) {
   std::vector<DevRec> recs;
   build_records(m, recs);
   std::vector<int8_t> omap;
@@ -261,12 +311,12 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     fN[i] = n;
     fS[i] = s2;
   }
-  c->cons_n = c->cons_s = true;
+  ph.cons_n = ph.cons_s = true;
   for (int i = 0; i < d; ++i)
     for (int k = 0; k < d; ++k)
       if (m.H[i + (size_t)k * d] != cplx(0, 0)) {
-        if (fN[i] != fN[k]) c->cons_n = false;
-        if (fS[i] != fS[k]) c->cons_s = false;
+        if (fN[i] != fN[k]) ph.cons_n = false;
+        if (fS[i] != fS[k]) ph.cons_s = false;
       }
   // per-ADO (dN, d2S)
   std::vector<std::pair<int, int>> key(m.N, {0, 0});
@@ -279,8 +329,8 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
         dN += q;
         dS += q * (m.spin[K[r]] == 0 ? 1 : -1);
       }
-      if (!c->cons_n) dN = ((dN % 2) + 2) % 2;
-      if (!c->cons_s) dS = 0;
+      if (!ph.cons_n) dN = ((dN % 2) + 2) % 2;
+      if (!ph.cons_s) dS = 0;
       key[g] = {dN, dS};
     }
   std::vector<std::pair<int, int>> keys;
@@ -301,18 +351,13 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     for (int j = 0; j < d; ++j)
       for (int i = 0; i < d; ++i) {
         const int dn = fN[i] - fN[j], ds = fS[i] - fS[j];
-        const bool okn = c->cons_n ? dn == keys[s].first : (((dn - keys[s].first) % 2) == 0);
-        const bool oks = c->cons_s ? ds == keys[s].second : true;
+        const bool okn = ph.cons_n ? dn == keys[s].first : (((dn - keys[s].first) % 2) == 0);
+        const bool oks = ph.cons_s ? ds == keys[s].second : true;
         if (!okn || !oks) continue;
         pos[(size_t)s * dd + i + j * d] = (int16_t)nnz[s];
         elems[(size_t)s * dd + nnz[s]] = (int16_t)(i + j * d);
         nnz[s]++;
       }
-  c->sec_elems_h = elems;
-  c->sec_pos_h = pos;
-  c->sec_nnz_h = nnz;
-  c->fock_n = fN;
-  c->fock_s = fS;
   std::vector<int64_t> off(m.N + 1, 0);
   for (int64_t g = 0; g < m.N; ++g) off[g + 1] = off[g] + nnz[sec[g]];
   const int64_t total = off[m.N];
@@ -331,16 +376,30 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
   const int64_t n_heavy = m.L >= 1 ? m.first[m.L] : m.N;
   const int64_t light_begin = off[n_heavy];
   const int64_t nlb = (total - light_begin + 255) / 256;
-  std::vector<int32_t> lblock((size_t)std::max<int64_t>(nlb, 0) + 1, (int32_t)(m.N - 1));
+  // top-tier ADOs with at least one allowed element (an ADO whose drawers carry
+  // a charge no pair of Fock states can match holds none)
+  std::vector<int32_t> lnz;
+  for (int64_t a = n_heavy; a < m.N; ++a)
+    if (off[a + 1] > off[a]) lnz.push_back((int32_t)a);
+  std::vector<int32_t> lblock((size_t)std::max<int64_t>(nlb, 0) + 1, 0);
   {
-    int64_t a = n_heavy;
-    for (int64_t b = 0; b <= nlb; ++b) {
+    size_t q = 0;
+    for (int64_t b = 0; b <= nlb && !lnz.empty(); ++b) {
       const int64_t t = std::min<int64_t>(light_begin + b * 256, total - 1);
-      while (a + 1 < m.N && off[a + 1] <= t) ++a;
-      lblock[b] = (int32_t)a;
+      while (q + 1 < lnz.size() && off[lnz[q + 1]] <= t) ++q;
+      lblock[b] = (int32_t)q;
+    }
+    // what the light kernel assumes: every block's element range lies in the
+    // ADOs lnz[lo .. hi] and there are at most 257 of them (its LDS tables)
+    for (int64_t b = 0; b < nlb; ++b) {
+      const int32_t lo = lblock[b], hi = lblock[b + 1];
+      const int64_t t0 = light_begin + b * 256, t1 = std::min<int64_t>(t0 + 255, total - 1);
+      if (hi < lo || hi - lo + 1 > 257 || hi >= (int32_t)lnz.size() || off[lnz[lo]] > t0 ||
+          off[lnz[hi] + 1] <= t1)
+        return fail(HEOMQ_UNSUPPORTED, "packed layout: light block table inconsistent");
     }
   }
-  HIPCHECK(upload(c->p_lblock, lblock));
+  if (lnz.empty()) lnz.push_back(0);
   // gather maps: output slot t of sector s -> signed slot of the neighbour (sector s2)
   std::vector<int16_t> maps;
   std::map<std::tuple<int, int, int, int, int>, int32_t> mapidx;
@@ -392,25 +451,62 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
       }
     return out;
   };
-  const auto pentf = conv(rowf, entf);
-  const auto pentt = conv(rowt, entt);
+  ph.pentf = conv(rowf, entf);
+  ph.pentt = conv(rowt, entt);
   if (maps.empty()) maps.push_back(0);
-  HIPCHECK(upload(c->p_off, off));
-  HIPCHECK(upload(c->p_sec, sec));
-  HIPCHECK(upload(c->p_elems, elems));
-  HIPCHECK(upload(c->p_pos, pos));
-  HIPCHECK(upload(c->p_block, block));
-  HIPCHECK(upload(c->p_entf, pentf));
-  HIPCHECK(upload(c->p_entt, pentt));
-  HIPCHECK(upload(c->p_maps, maps));
-  c->nsec = S;
+  ph.fN = std::move(fN);
+  ph.fS = std::move(fS);
+  ph.S = S;
+  ph.elems = std::move(elems);
+  ph.pos = std::move(pos);
+  ph.nnz = std::move(nnz);
+  ph.sec = std::move(sec);
+  ph.off = std::move(off);
+  ph.total = total;
+  ph.block = std::move(block);
+  ph.lblock = std::move(lblock);
+  ph.lnz = std::move(lnz);
+  ph.n_heavy = n_heavy;
+  ph.light_begin = light_begin;
+  ph.maps = std::move(maps);
+  ph.tslot = ph.pos[(size_t)ph.sec[0] * dd + (d - 1) + (d - 1) * d];
+  for (int w = 0; w < 2; ++w) {
+    const auto& rows = w == 0 ? rowf : rowt;
+    int mx = 0;
+    for (int64_t g = n_heavy; g < m.N; ++g) mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
+    ph.max_light[w] = mx;
+  }
+  return HEOMQ_OK;
+}
+
+int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
+                 const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt) {
+  PackedHost ph;
+  if (int rc = build_packed_host(m, rowf, entf, rowt, entt, ph)) return rc;
+  c->cons_n = ph.cons_n;
+  c->cons_s = ph.cons_s;
+  c->sec_elems_h = ph.elems;
+  c->sec_pos_h = ph.pos;
+  c->sec_nnz_h = ph.nnz;
+  c->fock_n = ph.fN;
+  c->fock_s = ph.fS;
+  HIPCHECK(upload(c->p_lblock, ph.lblock));
+  HIPCHECK(upload(c->p_lnz, ph.lnz));
+  HIPCHECK(upload(c->p_off, ph.off));
+  HIPCHECK(upload(c->p_sec, ph.sec));
+  HIPCHECK(upload(c->p_elems, ph.elems));
+  HIPCHECK(upload(c->p_pos, ph.pos));
+  HIPCHECK(upload(c->p_block, ph.block));
+  HIPCHECK(upload(c->p_entf, ph.pentf));
+  HIPCHECK(upload(c->p_entt, ph.pentt));
+  HIPCHECK(upload(c->p_maps, ph.maps));
+  c->nsec = ph.S;
   if (int rc = build_packed_h(c, m)) return rc;
-  c->ptotal = total;
-  const int tslot = pos[(size_t)sec[0] * dd + (d - 1) + (d - 1) * d];
+  c->ptotal = ph.total;
   for (int w = 0; w < 2; ++w) {
     DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
     p.n_ado = m.N;
-    p.total = total;
+    p.total = ph.total;
     p.off = c->p_off.as<int64_t>();
     p.sec = c->p_sec.as<uint8_t>();
     p.sec_elems = c->p_elems.as<int16_t>();
@@ -418,15 +514,13 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     p.block_ado = c->p_block.as<int32_t>();
     p.rowptr = (w == 0 ? c->fwd.rowptr : c->trn.rowptr).as<int64_t>();
     p.ent = (w == 0 ? c->p_entf : c->p_entt).as<DevEntryP>();
-    p.trace_slot = tslot;
+    p.trace_slot = ph.tslot;
     p.maps = c->p_maps.as<int16_t>();
-    p.n_heavy = n_heavy;
-    p.light_begin = light_begin;
+    p.n_heavy = ph.n_heavy;
+    p.light_begin = ph.light_begin;
     p.light_block_ado = c->p_lblock.as<int32_t>();
-    const auto& rows = w == 0 ? rowf : rowt;
-    int mx = 0;
-    for (int64_t g = n_heavy; g < m.N; ++g) mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
-    p.max_light_couplings = mx;
+    p.light_nz = c->p_lnz.as<int32_t>();
+    p.max_light_couplings = ph.max_light[w];
   }
   return HEOMQ_OK;
 }
@@ -527,6 +621,32 @@ int heomq_model_coupling(const heomq_model* h, int32_t* lower, int32_t* upper) {
       upper[4 * p] = m.upper[p].nbr; upper[4 * p + 1] = m.upper[p].itype;
       upper[4 * p + 2] = m.upper[p].ff; upper[4 * p + 3] = m.upper[p].fr;
     }
+  return HEOMQ_OK;
+}
+
+int heomq_model_packed_layout(const heomq_model* h, int64_t* info) {
+  if (!h || !info) return fail(HEOMQ_BAD_ARG, "null argument");
+  const Model& m = h->m;
+  std::vector<int64_t> rowf, rowt;
+  std::vector<DevEntry> entf, entt;
+  if (int rc = build_entries(m, rowf, entf, rowt, entt)) return rc;
+  PackedHost ph;
+  if (int rc = build_packed_host(m, rowf, entf, rowt, entt, ph)) return rc;
+  int64_t empty = 0, worst = 0;
+  for (int64_t a = 0; a < m.N; ++a) empty += ph.off[a + 1] == ph.off[a];
+  for (size_t b = 0; b + 1 < ph.lblock.size(); ++b) worst = std::max<int64_t>(worst, ph.lblock[b + 1] - ph.lblock[b] + 1);
+  info[0] = ph.total;
+  info[1] = ph.S;
+  info[2] = ph.cons_n;
+  info[3] = ph.cons_s;
+  info[4] = m.N * (int64_t)m.d * m.d;
+  info[5] = ph.n_heavy;
+  info[6] = ph.light_begin;
+  info[7] = (int64_t)ph.lblock.size() - 1;
+  info[8] = worst;
+  info[9] = empty;
+  info[10] = ph.max_light[0];
+  info[11] = (int64_t)ph.maps.size();
   return HEOMQ_OK;
 }
 

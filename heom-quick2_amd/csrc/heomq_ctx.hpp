@@ -89,6 +89,7 @@ struct heomq_ctx {
   DevBuf parity;                     // electron-number parity of each Fock state
   // symmetry-packed layout (solver path)
   DevBuf p_off, p_sec, p_elems, p_pos, p_block, p_lblock, p_entf, p_entt, p_hrow, p_hq, p_hc, p_maps;
+  DevBuf p_hq_ell, p_hc_ell, p_lnz;
   heomq::DevPacked pfwd, ptrn;
   int64_t ptotal = 0;
   int nsec = 0;

@@ -91,10 +91,18 @@ struct DevPacked {
   // starting at packed element light_begin, with its own block table.
   int64_t n_heavy = 0;
   int64_t light_begin = 0;
-  const int32_t* light_block_ado = nullptr;
+  const int32_t* light_nz = nullptr;         // top-tier ADOs that hold >= 1 packed element, in order
+  const int32_t* light_block_ado = nullptr;  // per light block: index into light_nz of the ADO holding its
+                                             // first element (+ one entry for the end); a block spans at most
+                                             // 257 of these ADOs
   int max_light_couplings = 0;       // largest coupling count of a top-tier ADO (<= 8)
   // slot-space tables: i(HX - XH) per sector as CSR over packed slots, and the
   // gather maps (int16 value (q + 1) * sign, 0 = no term) referenced by entries
+  // the same rows padded to a fixed width hell_w (slot -1 / coefficient 0 as
+  // padding) when every row is short: index loads then depend only on (s, t)
+  int hell_w = 0;                    // 0: use the CSR form
+  const int16_t* hq_ell = nullptr;   // [S][d*d][hell_w]
+  const double* hc_ell = nullptr;    // [S][d*d][hell_w] complex
   const int32_t* hrow = nullptr;     // [S][d*d + 1] row starts into hq / hc
   const int16_t* hq = nullptr;
   const double* hc = nullptr;        // complex coefficients of i(HX - XH)

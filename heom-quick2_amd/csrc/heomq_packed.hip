@@ -34,71 +34,166 @@ __device__ inline typename P2<T>::type ld(const typename P2<T>::type* xv, int64_
   return v;
 }
 
-// gamma_K x_t + hsign * i(HX - XH)_t for packed slot t of the ADO at `base`
+// gamma_K x_t + hsign * i(HX - XH)_t for packed slot t of the ADO at `base`.
+// The CSR row of i(HX - XH) is read four terms at a time so that its index
+// and data loads are in flight together.
 template <typename T, bool TRANS>
-__device__ inline void diag_term(const DevPacked& op, const typename P2<T>::type* xv, int64_t a, int64_t base,
-                                 int s, int t, int64_t zero_at, T& ar, T& ai) {
+__device__ inline void diag_term(const DevPacked& op, const typename P2<T>::type* __restrict__ xv, T gr, T gi,
+                                 int64_t base, int s, int t, int64_t zero_at, T& ar, T& ai) {
   const int dd = op.d * op.d;
   const auto xe = ld<T, TRANS>(xv, base + t, zero_at);
-  const T gr = (T)op.gam[2 * a], gi = (T)op.gam[2 * a + 1];
   ar = gr * xe.x - gi * xe.y;
   ai = gr * xe.y + gi * xe.x;
-  const int r0 = op.hrow[s * (dd + 1) + t], r1 = op.hrow[s * (dd + 1) + t + 1];
   T cr = 0, ci = 0;
-  for (int u = r0; u < r1; ++u) {
-    const auto v = ld<T, TRANS>(xv, base + op.hq[u], zero_at);
-    const T hr = (T)op.hc[2 * u], hi = (T)op.hc[2 * u + 1];
-    cr += hr * v.x - hi * v.y;
-    ci += hr * v.y + hi * v.x;
+  const int W = op.hell_w;
+  if (W > 0) {   // fixed-width rows: index and coefficient loads depend only on (s, t)
+    const int64_t row = ((int64_t)s * dd + t) * W;
+    for (int k0 = 0; k0 < W; k0 += 4) {
+      int q[4];
+      T hr[4], hi[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const bool ok = k0 + k < W;
+        q[k] = ok ? op.hq_ell[row + k0 + k] : 0;
+        hr[k] = ok ? (T)op.hc_ell[2 * (row + k0 + k)] : (T)0;
+        hi[k] = ok ? (T)op.hc_ell[2 * (row + k0 + k) + 1] : (T)0;
+      }
+      typename P2<T>::type v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = ld<T, TRANS>(xv, base + q[k], zero_at);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        cr += hr[k] * v[k].x - hi[k] * v[k].y;
+        ci += hr[k] * v[k].y + hi[k] * v[k].x;
+      }
+    }
+    const T hs = (T)op.hsign;
+    ar += hs * cr;
+    ai += hs * ci;
+    return;
+  }
+  const int r0 = op.hrow[s * (dd + 1) + t], r1 = op.hrow[s * (dd + 1) + t + 1];
+  for (int u0 = r0; u0 < r1; u0 += 4) {
+    int q[4];
+    T hr[4], hi[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool ok = u0 + k < r1;
+      q[k] = ok ? op.hq[u0 + k] : 0;
+      hr[k] = ok ? (T)op.hc[2 * (u0 + k)] : (T)0;
+      hi[k] = ok ? (T)op.hc[2 * (u0 + k) + 1] : (T)0;
+    }
+    typename P2<T>::type v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ld<T, TRANS>(xv, base + q[k], zero_at);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      cr += hr[k] * v[k].x - hi[k] * v[k].y;
+      ci += hr[k] * v[k].y + hi[k] * v[k].x;
+    }
   }
   const T hs = (T)op.hsign;
   ar += hs * cr;
   ai += hs * ci;
 }
 
-template <typename T, bool TRANS>
-__device__ inline void coupling_term(const DevEntryP& en, const DevRec& rc, const int16_t* __restrict__ maps,
-                                     const typename P2<T>::type* xv, int t, int64_t zero_at, T& ar, T& ai) {
-  const bool dag = en.flags & 1;
-  if (en.mapL >= 0) {
-    const int mv = maps[en.mapL + t];
-    if (mv != 0) {
-      auto v = ld<T, TRANS>(xv, (int64_t)en.base + (mv > 0 ? mv : -mv) - 1, zero_at);
-      if (dag) v.y = -v.y;
-      const T sg = ((mv > 0) != ((en.flags & 2) != 0)) ? (T)1 : (T)-1;
-      ar += sg * ((T)rc.cLr * v.x - (T)rc.cLi * v.y);
-      ai += sg * ((T)rc.cLr * v.y + (T)rc.cLi * v.x);
-    }
+// Coefficients (cL, cR) of a record: the first 32 bytes of DevRec.
+struct CoefLR {
+  double2 l, r;
+};
+__device__ inline CoefLR load_coef(const DevRec* rec, int k) {
+  const double2* p = reinterpret_cast<const double2*>(rec + k);
+  return CoefLR{p[0], p[1]};
+}
+
+// U couplings at once: every entry's gather-map values, then every neighbour
+// element, are loaded before any of them is used (independent loads in
+// flight instead of a dependent chain per coupling).
+template <typename T, bool TRANS, int U>
+__device__ inline void coupling_block(const DevEntryP* en, const CoefLR* cf, int nvalid,
+                                      const int16_t* __restrict__ maps, const typename P2<T>::type* __restrict__ xv,
+                                      int t, int64_t zero_at, T& ar, T& ai) {
+  using V = typename P2<T>::type;
+  int mL[U], mR[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const bool ok = u < nvalid;
+    mL[u] = (ok && en[u].mapL >= 0) ? (int)maps[en[u].mapL + t] : 0;
+    mR[u] = (ok && en[u].mapR >= 0) ? (int)maps[en[u].mapR + t] : 0;
   }
-  if (en.mapR >= 0) {
-    const int mv = maps[en.mapR + t];
-    if (mv != 0) {
-      auto v = ld<T, TRANS>(xv, (int64_t)en.base + (mv > 0 ? mv : -mv) - 1, zero_at);
-      if (dag) v.y = -v.y;
-      const T sg = ((mv > 0) != ((en.flags & 4) != 0)) ? (T)1 : (T)-1;
-      ar += sg * ((T)rc.cRr * v.x - (T)rc.cRi * v.y);
-      ai += sg * ((T)rc.cRr * v.y + (T)rc.cRi * v.x);
-    }
+  V vL[U], vR[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t b = u < nvalid ? (int64_t)en[u].base : 0;
+    vL[u] = mL[u] != 0 ? ld<T, TRANS>(xv, b + (mL[u] > 0 ? mL[u] : -mL[u]) - 1, zero_at) : V{0, 0};
+    vR[u] = mR[u] != 0 ? ld<T, TRANS>(xv, b + (mR[u] > 0 ? mR[u] : -mR[u]) - 1, zero_at) : V{0, 0};
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (mL[u] == 0 && mR[u] == 0) continue;
+    const uint8_t fl = en[u].flags;
+    const T cj = (fl & 1) ? (T)-1 : (T)1;   // X^H: conjugate the gathered element
+    const T sL = ((mL[u] > 0) != ((fl & 2) != 0)) ? (T)1 : (T)-1;
+    const T sR = ((mR[u] > 0) != ((fl & 4) != 0)) ? (T)1 : (T)-1;
+    const T lx = vL[u].x, ly = cj * vL[u].y, rx = vR[u].x, ry = cj * vR[u].y;
+    const T clr = (T)cf[u].l.x, cli = (T)cf[u].l.y, crr = (T)cf[u].r.x, cri = (T)cf[u].r.y;
+    ar += sL * (clr * lx - cli * ly) + sR * (crr * rx - cri * ry);
+    ai += sL * (clr * ly + cli * lx) + sR * (crr * ry + cri * rx);
   }
 }
 
-// Top-tier ADOs: one thread per packed element.
-template <typename T, bool TRANS>
+// Top-tier ADOs: one thread per packed element.  The headers (offset, first
+// and last entry, sector, gamma) of the non-empty ADOs a block covers (at most
+// 257, checked when the tables are built) are staged in LDS and the element's
+// ADO is found there; the couplings are then read U at a time (U = the top
+// tier's coupling count, at most 4 per step).
+template <typename T, bool TRANS, int U>
 __global__ void __launch_bounds__(256)
 k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at) {
   using V = typename P2<T>::type;
+  __shared__ int64_t s_off[257];
+  __shared__ int64_t s_p0[257], s_p1[257];
+  __shared__ int s_sec[257];
+  __shared__ double2 s_gam[257];
+  const int lo = op.light_block_ado[blockIdx.x], hi = op.light_block_ado[blockIdx.x + 1];
+  const int cnt = hi - lo + 1;
+  for (int k = threadIdx.x; k < cnt; k += 256) {
+    const int64_t a = op.light_nz[lo + k];
+    s_off[k] = op.off[a];
+    s_p0[k] = op.rowptr[a];
+    s_p1[k] = op.rowptr[a + 1];
+    s_sec[k] = op.sec[a];
+    s_gam[k] = reinterpret_cast<const double2*>(op.gam)[a];
+  }
+  __syncthreads();
   const int64_t t = op.light_begin + (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= op.total) return;   // no barrier in this kernel
-  const V* __restrict__ xv = reinterpret_cast<const V*>(x);
-  const int64_t a = find_ado_tab(op, op.light_block_ado, t);
-  const int64_t base = op.off[a];
+  if (t >= op.total) return;   // after the only barrier
+  int l = 0, h = cnt - 1;
+  while (l < h) {
+    const int mid = (l + h + 1) >> 1;
+    if (s_off[mid] <= t) l = mid; else h = mid - 1;
+  }
+  const int64_t base = s_off[l];
   const int slot = (int)(t - base);
+  const int64_t p0 = s_p0[l], p1 = s_p1[l];
+  const V* __restrict__ xv = reinterpret_cast<const V*>(x);
   T ar, ai;
-  diag_term<T, TRANS>(op, xv, a, base, op.sec[a], slot, zero_at, ar, ai);
-  const int64_t p0 = op.rowptr[a], p1 = op.rowptr[a + 1];
-  for (int64_t p = p0; p < p1; ++p) {
-    const DevEntryP en = op.ent[p];
-    coupling_term<T, TRANS>(en, op.rec[en.rec], op.maps, xv, slot, zero_at, ar, ai);
+  diag_term<T, TRANS>(op, xv, (T)s_gam[l].x, (T)s_gam[l].y, base, s_sec[l], slot, zero_at, ar, ai);
+  for (int64_t p = p0; p < p1; p += U) {
+    const int nv = (int)(p1 - p < U ? p1 - p : U);
+    DevEntryP en[U];
+    CoefLR cf[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (u < nv) {
+        en[u] = op.ent[p + u];
+        cf[u] = load_coef(op.rec, en[u].rec);
+      } else {
+        en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
+        cf[u] = CoefLR{};
+      }
+    }
+    coupling_block<T, TRANS, U>(en, cf, nv, op.maps, xv, slot, zero_at, ar, ai);
   }
   V o;
   o.x = ar;
@@ -106,26 +201,28 @@ k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, i
   reinterpret_cast<V*>(y)[t] = o;
 }
 
-// ADOs below the top tier: one workgroup per ADO.  EL lanes cover the ADO's
-// packed elements, 256/EL slices split its coupling list (staged in LDS).
+template <typename T>
+__device__ void tier0_fixup_block(const DevPacked& op, const T* __restrict__ x, T* __restrict__ y, int rdo);
+
+// ADOs below the top tier: one workgroup per ADO.  The block's 256 lanes are
+// laid out as (slice, slot): S = 256 / nnz slices, each walking every S-th
+// coupling of the ADO (entries and coefficients staged in LDS), four
+// couplings per step; the slices are summed through LDS in a fixed order.
+// Block 0 (the tier-0 ADO) then applies the trace row and Hermitianises.
 template <typename T, bool TRANS>
 __global__ void __launch_bounds__(256)
-k_apply_packed_heavy(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at) {
+k_apply_packed_heavy(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
   using V = typename P2<T>::type;
   constexpr int kStage = 256;
   __shared__ T red_r[256], red_i[256];
   __shared__ DevEntryP s_en[kStage];
-  __shared__ DevRec s_rc[kStage];
+  __shared__ CoefLR s_cf[kStage];
   const int64_t a = blockIdx.x;
   const int64_t base = op.off[a];
   const int nnz = (int)(op.off[a + 1] - base);
-  if (nnz == 0) return;   // uniform across the block, before any barrier
-  int EL = 1;
-  while (EL < nnz && EL < 256) EL <<= 1;
-  const int S = 256 / EL;
-  const int lane = threadIdx.x % EL, slice = threadIdx.x / EL;
   const V* __restrict__ xv = reinterpret_cast<const V*>(x);
   const int s = op.sec[a];
+  const T gr = (T)op.gam[2 * a], gi = (T)op.gam[2 * a + 1];
   const int64_t p0 = op.rowptr[a], p1 = op.rowptr[a + 1];
   const int nent = (int)(p1 - p0);
   const bool staged = nent <= kStage;
@@ -133,39 +230,60 @@ k_apply_packed_heavy(const DevPacked op, const T* __restrict__ x, T* __restrict_
     for (int k = threadIdx.x; k < nent; k += 256) {
       const DevEntryP e0 = op.ent[p0 + k];
       s_en[k] = e0;
-      s_rc[k] = op.rec[e0.rec];
+      s_cf[k] = load_coef(op.rec, e0.rec);
     }
   __syncthreads();
-  for (int t0 = 0; t0 < nnz; t0 += EL) {
+  constexpr int U = 4;
+  for (int t0 = 0; t0 < nnz; t0 += 256) {   // nnz is uniform over the block: every thread takes the same trips
+    const int w = nnz - t0 < 256 ? nnz - t0 : 256;   // slots in this pass
+    const int S = 256 / w;
+    const int lane = threadIdx.x % w, slice = threadIdx.x / w;
     const int t = t0 + lane;
     T ar = 0, ai = 0;
-    if (t < nnz) {
-      if (slice == 0) diag_term<T, TRANS>(op, xv, a, base, s, t, zero_at, ar, ai);
-      for (int k = slice; k < nent; k += S) {
-        if (staged) {
-          coupling_term<T, TRANS>(s_en[k], s_rc[k], op.maps, xv, t, zero_at, ar, ai);
-        } else {
-          const DevEntryP en = op.ent[p0 + k];
-          coupling_term<T, TRANS>(en, op.rec[en.rec], op.maps, xv, t, zero_at, ar, ai);
+    if (slice < S) {
+      if (slice == 0) diag_term<T, TRANS>(op, xv, gr, gi, base, s, t, zero_at, ar, ai);
+      for (int k0 = slice; k0 < nent; k0 += S * U) {
+        DevEntryP en[U];
+        CoefLR cf[U];
+        int nv = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + u * S;
+          if (k < nent) {
+            en[u] = staged ? s_en[k] : op.ent[p0 + k];
+            cf[u] = staged ? s_cf[k] : load_coef(op.rec, en[u].rec);
+            nv = u + 1;
+          } else {
+            en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
+            cf[u] = CoefLR{};
+          }
         }
+        coupling_block<T, TRANS, U>(en, cf, nv, op.maps, xv, t, zero_at, ar, ai);
       }
     }
     red_r[threadIdx.x] = ar;
     red_i[threadIdx.x] = ai;
     __syncthreads();
-    if (slice == 0 && t < nnz) {
-      T sr = 0, si = 0;
-      for (int k = 0; k < S; ++k) {   // fixed order: deterministic
-        sr += red_r[k * EL + lane];
-        si += red_i[k * EL + lane];
+    // pairwise tree over the S slices (fixed pairing: deterministic); S and
+    // every trip count below are uniform over the block
+    for (int act = S; act > 1;) {
+      const int half = (act + 1) >> 1;
+      if (slice < act - half) {
+        red_r[threadIdx.x] += red_r[threadIdx.x + half * w];
+        red_i[threadIdx.x] += red_i[threadIdx.x + half * w];
       }
+      __syncthreads();
+      act = half;
+    }
+    if (threadIdx.x < w) {
       V o;
-      o.x = sr;
-      o.y = si;
+      o.x = red_r[threadIdx.x];
+      o.y = red_i[threadIdx.x];
       reinterpret_cast<V*>(y)[base + t] = o;
     }
     __syncthreads();
   }
+  if (a == 0) tier0_fixup_block<T>(op, x, y, rdo);   // block-uniform branch
 }
 
 // Tier-0 fix-ups (one block): trace row (forward: y(d,d) = tr x0; transposed:
@@ -192,7 +310,7 @@ __device__ inline void herm0(const DevPacked& op, T* y) {
 }
 
 template <typename T>
-__global__ void k_tier0_fixup(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int rdo) {
+__device__ void tier0_fixup_block(const DevPacked& op, const T* __restrict__ x, T* __restrict__ y, int rdo) {
   const int D = op.d, DD = D * D;
   const int16_t* pos = op.sec_pos + op.sec[0] * DD;
   __shared__ T tr[2];
@@ -220,6 +338,11 @@ __global__ void k_tier0_fixup(const DevPacked op, const T* __restrict__ x, T* __
     __syncthreads();
   }
   herm0<T>(op, y);
+}
+
+template <typename T>
+__global__ void k_tier0_fixup(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int rdo) {
+  tier0_fixup_block<T>(op, x, y, rdo);
 }
 
 template <typename T>
@@ -268,23 +391,33 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
   if (op.n_heavy > 0) {
     if (op.transposed)
       hipLaunchKernelGGL((k_apply_packed_heavy<T, true>), dim3((unsigned)op.n_heavy), dim3(256), 0, s, op, x, y,
-                         zero_at);
+                         zero_at, rdo ? 1 : 0);
     else
       hipLaunchKernelGGL((k_apply_packed_heavy<T, false>), dim3((unsigned)op.n_heavy), dim3(256), 0, s, op, x, y,
-                         zero_at);
+                         zero_at, rdo ? 1 : 0);
     err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
   const int64_t light = op.total - op.light_begin;
   if (light > 0) {
-    const int64_t blocks = (light + 255) / 256;
-    if (op.transposed)
-      hipLaunchKernelGGL((k_apply_packed<T, true>), dim3((unsigned)blocks), dim3(256), 0, s, op, x, y, zero_at);
-    else
-      hipLaunchKernelGGL((k_apply_packed<T, false>), dim3((unsigned)blocks), dim3(256), 0, s, op, x, y, zero_at);
+    const dim3 grid((unsigned)((light + 255) / 256));
+    const int u = op.max_light_couplings;
+#define HEOMQ_LIGHT(UU)                                                                              \
+  do {                                                                                               \
+    if (op.transposed)                                                                               \
+      hipLaunchKernelGGL((k_apply_packed<T, true, UU>), grid, dim3(256), 0, s, op, x, y, zero_at);   \
+    else                                                                                             \
+      hipLaunchKernelGGL((k_apply_packed<T, false, UU>), grid, dim3(256), 0, s, op, x, y, zero_at);  \
+  } while (0)
+    if (u <= 1) HEOMQ_LIGHT(1);
+    else if (u == 2) HEOMQ_LIGHT(2);
+    else if (u == 3) HEOMQ_LIGHT(3);
+    else HEOMQ_LIGHT(4);
+#undef HEOMQ_LIGHT
     err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
+  if (op.n_heavy > 0) return hipSuccess;   // block 0 of the heavy kernel did the tier-0 fix-up
   hipLaunchKernelGGL((k_tier0_fixup<T>), dim3(1), dim3(256), 0, s, op, x, y, rdo ? 1 : 0);
   return hipGetLastError();
 }

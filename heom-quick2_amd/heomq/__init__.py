@@ -186,6 +186,14 @@ class Model:
         check(lib().heomq_model_coupling(self._h, lower.ctypes.data_as(_lib.I32), upper.ctypes.data_as(_lib.I32)))
         return lower, upper
 
+    def packed_layout(self) -> dict:
+        """The symmetry-packed layout the device solvers use, built on the host."""
+        info = np.zeros(12, np.int64)
+        check(lib().heomq_model_packed_layout(self._h, info.ctypes.data_as(_lib.I64)))
+        keys = ("packed", "sectors", "n_conserved", "sz_conserved", "dense", "n_heavy", "light_begin",
+                "light_blocks", "max_ados_per_light_block", "empty_ados", "max_light_couplings", "map_entries")
+        return dict(zip(keys, (int(v) for v in info)))
+
     def n_lower_total(self) -> int:
         return self.n_lower
 

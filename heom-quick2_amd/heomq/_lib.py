@@ -49,6 +49,7 @@ SIGNATURES = {
     "heomq_model_tier": (C.c_int, [P, C.c_int, I32]),
     "heomq_model_coupling": (C.c_int, [P, I32, I32]),
     "heomq_model_set_params": (C.c_int, [P, C.POINTER(CParams)]),
+    "heomq_model_packed_layout": (C.c_int, [P, I64]),
     "heomq_model_from_reference": (C.c_int, [
         C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
         I32, I64, I64, I64, I64, I8, I8, I8, I64, I32, D, D, D, D, I32, I32, I32, I32, D, D, D,

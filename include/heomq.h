@@ -96,6 +96,15 @@ int heomq_model_tier(const heomq_model* m, int tier, int32_t* tuples);
  * (nbr, itype, ffront, frear) as int32 quadruples; absent upper entries have
  * nbr = -1.  lower covers tiers >= 1 (ordered by ADO), upper tiers < L. */
 int heomq_model_coupling(const heomq_model* m, int32_t* lower, int32_t* upper);
+/* Symmetry-packed layout the device solvers use, built on the host (no
+ * device needed; the same construction a context runs, with its consistency
+ * checks).  info[0] = packed complex elements, [1] = sectors, [2] = N
+ * conserved, [3] = S_z conserved, [4] = dense complex elements, [5] = ADOs
+ * below the top tier, [6] = first top-tier packed element, [7] = top-tier
+ * blocks of 256 elements, [8] = most non-empty ADOs one such block spans,
+ * [9] = ADOs with no allowed element, [10] = most couplings of a top-tier
+ * ADO, [11] = gather-map entries.  No counterpart in the reference. */
+int heomq_model_packed_layout(const heomq_model* m, int64_t* info);
 /* Replace physical parameters that leave the hierarchy shape unchanged
  * (energies, Gamma, T, bias).  Used by sweeps and finite differences. */
 int heomq_model_set_params(heomq_model* m, const heomq_params* p);

@@ -144,3 +144,25 @@ def test_cfg_sizes():
     # hierarchy sizes of the benchmark configurations (unique-index, Hermitian-paired)
     assert heomq.Model(heomq.siam()).N == 31297
     assert heomq.Model(heomq.two_impurity()).N == 117321
+
+
+@pytest.mark.parametrize("pp,op", CASES + [(heomq.siam(nk=3, ntop=3, V=0.3, U=1.5, T=0.2),
+                                            O.siam_params(nk=3, ntop=3, V=0.3, U=1.5, T=0.2))])
+def test_packed_layout_host(pp, op):
+    """The packed layout built on the host: element count equals the oracle's
+    selection rules, and the top-tier block tables fit the light kernel's
+    257-entry LDS tables even where many ADOs hold no element (d = 4)."""
+    lay = heomq.Model(pp).packed_layout()
+    o = O.Model(op)
+    mask = O.physical_projection(o, np.ones((o.N, o.d, o.d), complex), hermitian=False) != 0
+    assert lay["packed"] == int(mask.sum())
+    assert lay["empty_ados"] == int((mask.reshape(o.N, -1).sum(1) == 0).sum())
+    assert 1 <= lay["max_ados_per_light_block"] <= 257
+    assert lay["n_conserved"] == 1 and lay["sz_conserved"] == 1
+
+
+def test_packed_layout_benchmark_sizes():
+    lay = heomq.Model(heomq.two_impurity()).packed_layout()
+    assert lay["packed"] == 1797356 and lay["max_light_couplings"] == 3
+    lay = heomq.Model(heomq.siam()).packed_layout()
+    assert lay["packed"] == 37012 and lay["empty_ados"] > 0 and lay["max_ados_per_light_block"] <= 257
