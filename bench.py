@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--cpu-applies", type=int, default=2)
     ap.add_argument("--no-ref", action="store_true",
                     help="skip the long reference solve (rdo_err is then not measured); for profiling runs")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="profiling: a 64 MiB calibration copy, then --apply-reps packed applies; no solve, no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the batched and single-hierarchy "
@@ -63,6 +65,8 @@ def main():
     import heomq
 
     params, wname = workload(args.workload)
+    if args.apply_only:
+        return apply_only(params, args.apply_reps)
     if args.tol is None:
         # the two-impurity TFQMR residual floors near 3e-8; 1e-7 already gives
         # an RDO error far below 1e-10 against the long reference solve below
@@ -150,6 +154,24 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def apply_only(params, reps):
+    """Profiling driver: one device copy of 64 MiB (the counter calibration:
+    FETCH_SIZE / WRITE_SIZE of a kernel whose traffic is known), then packed
+    applies on a trace-one state."""
+    import torch
+    import heomq
+    src = torch.ones(1 << 23, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    ctx = heomq.Context(heomq.Model(params))
+    xp = ctx.pack(ctx.initial_guess())
+    yp = ctx.packed_zeros()
+    for _ in range(reps):
+        ctx.apply_packed_raw(xp, yp, trace=heomq.TRACE_RDO)
+    torch.cuda.synchronize()
+    print(f"apply-only: {reps} packed applies, {ctx.packed_info()['packed']} packed elements", flush=True)
 
 
 def cpu_baseline(params, wname, napply, applies_per_solve):

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -235,17 +237,18 @@ int build_packed_h(heomq_ctx* c, const Model& m) {
       width = std::max(width, hrow[(size_t)s * (dd + 1) + t + 1] - hrow[(size_t)s * (dd + 1) + t]);
   const int kMaxEll = 16;
   const int ew = width <= kMaxEll ? std::max(width, 1) : 0;
+  const int ewp = (ew + 7) / 8 * 8;
   std::vector<int16_t> hq_ell;
   std::vector<double> hc_ell;
   if (ew > 0) {
-    hq_ell.assign((size_t)S * dd * ew, 0);
+    hq_ell.assign((size_t)S * dd * ewp, 0);
     hc_ell.assign((size_t)S * dd * ew * 2, 0.0);
     for (int s = 0; s < S; ++s)
       for (int t = 0; t < dd; ++t) {
         const int r0 = hrow[(size_t)s * (dd + 1) + t], r1 = hrow[(size_t)s * (dd + 1) + t + 1];
         for (int u = r0; u < r1; ++u) {
           const size_t at = ((size_t)s * dd + t) * ew + (u - r0);
-          hq_ell[at] = hq[u];
+          hq_ell[((size_t)s * dd + t) * ewp + (u - r0)] = hq[u];
           hc_ell[2 * at] = hc[2 * (size_t)u];
           hc_ell[2 * at + 1] = hc[2 * (size_t)u + 1];
         }
@@ -266,6 +269,7 @@ int build_packed_h(heomq_ctx* c, const Model& m) {
     p.hq = c->p_hq.as<int16_t>();
     p.hc = c->p_hc.as<double>();
     p.hell_w = ew;
+    p.hell_wp = ewp;
     p.hq_ell = c->p_hq_ell.as<int16_t>();
     p.hc_ell = c->p_hc_ell.as<double>();
   }
@@ -287,9 +291,10 @@ struct PackedHost {
   std::vector<int32_t> block, lblock, lnz;
   int64_t n_heavy = 0, light_begin = 0;
   std::vector<DevEntryP> pentf, pentt;
-  std::vector<int16_t> maps;
+  std::vector<int32_t> maps;
   int tslot = -1;
   int max_light[2] = {0, 0};
+  int light_rec_lo[2] = {0, 0}, light_rec_n[2] = {0, 0};
 };
 
 int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
@@ -400,39 +405,39 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
     }
   }
   if (lnz.empty()) lnz.push_back(0);
-  // gather maps: output slot t of sector s -> signed slot of the neighbour (sector s2)
-  std::vector<int16_t> maps;
+  // paired gather maps: output slot t of sector s -> the signed slots in the
+  // neighbour (sector s2) of its left-operator and right-operator terms
+  std::vector<int32_t> maps;
   std::map<std::tuple<int, int, int, int, int>, int32_t> mapidx;
-  auto getmap = [&](int s, int s2, int o, int dag, int side) -> int32_t {
-    if (o < 0) return -1;
-    const auto key = std::make_tuple(s, s2, o, dag, side);
-    auto it = mapidx.find(key);
-    if (it != mapidx.end()) return it->second;
-    const int32_t at = (int32_t)maps.size();
+  auto side_value = [&](int s, int s2, int o, int dag, int side, int t) -> int {
+    if (o < 0) return 0;
     const int8_t* pL = &omap[(size_t)o * 4 * d];
     const int8_t* sL = pL + d;
     const int8_t* pR = pL + 2 * d;
     const int8_t* sR = pL + 3 * d;
+    const int e = elems[(size_t)s * dd + t];
+    const int i = e % d, j = e / d;
+    if (side == 0) {
+      const int pl = pL[i];
+      if (pl < 0) return 0;
+      const int q = pos[(size_t)s2 * dd + (dag ? (j + pl * d) : (pl + j * d))];
+      return q >= 0 ? (q + 1) * sL[i] : 0;
+    }
+    const int pr = pR[j];
+    if (pr < 0) return 0;
+    const int q = pos[(size_t)s2 * dd + (dag ? (pr + i * d) : (i + pr * d))];
+    return q >= 0 ? (q + 1) * sR[j] : 0;
+  };
+  auto getmap = [&](int s, int s2, int oL, int oR, int dag) -> int32_t {
+    const auto key = std::make_tuple(s, s2, oL, oR, dag);
+    auto it = mapidx.find(key);
+    if (it != mapidx.end()) return it->second;
+    const int32_t at = (int32_t)maps.size();
     bool any = false;
     for (int t = 0; t < nnz[s]; ++t) {
-      const int e = elems[(size_t)s * dd + t];
-      const int i = e % d, j = e / d;
-      int v = 0;
-      if (side == 0) {
-        const int pl = pL[i];
-        if (pl >= 0) {
-          const int q = pos[(size_t)s2 * dd + (dag ? (j + pl * d) : (pl + j * d))];
-          if (q >= 0) v = (q + 1) * sL[i];
-        }
-      } else {
-        const int pr = pR[j];
-        if (pr >= 0) {
-          const int q = pos[(size_t)s2 * dd + (dag ? (pr + i * d) : (i + pr * d))];
-          if (q >= 0) v = (q + 1) * sR[j];
-        }
-      }
-      any |= v != 0;
-      maps.push_back((int16_t)v);
+      const int vl = side_value(s, s2, oL, dag, 0, t), vr = side_value(s, s2, oR, dag, 1, t);
+      any |= (vl != 0 || vr != 0);
+      maps.push_back((int32_t)((uint32_t)(uint16_t)(int16_t)vl | ((uint32_t)(uint16_t)(int16_t)vr << 16)));
     }
     const int32_t res = any ? at : -1;
     if (!any) maps.resize(at);
@@ -446,8 +451,8 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
         const DevRec& r = recs[e[p].rec];
         const int dag = e[p].flags & 1;
         const int s = sec[g], s2 = sec[e[p].nbr];
-        out[p] = DevEntryP{(int32_t)off[e[p].nbr], getmap(s, s2, r.opL, dag, 0), getmap(s, s2, r.opR, dag, 1),
-                           e[p].rec, e[p].flags, (uint8_t)s2};
+        out[p] = DevEntryP{(int32_t)off[e[p].nbr], getmap(s, s2, r.opL, r.opR, dag), -1, e[p].rec, e[p].flags,
+                           (uint8_t)s2};
       }
     return out;
   };
@@ -472,9 +477,18 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
   ph.tslot = ph.pos[(size_t)ph.sec[0] * dd + (d - 1) + (d - 1) * d];
   for (int w = 0; w < 2; ++w) {
     const auto& rows = w == 0 ? rowf : rowt;
-    int mx = 0;
-    for (int64_t g = n_heavy; g < m.N; ++g) mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
+    const auto& ents = w == 0 ? ph.pentf : ph.pentt;
+    int mx = 0, rlo = INT32_MAX, rhi = -1;
+    for (int64_t g = n_heavy; g < m.N; ++g) {
+      mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
+      for (int64_t p = rows[g]; p < rows[g + 1]; ++p) {
+        rlo = std::min<int>(rlo, ents[p].rec);
+        rhi = std::max<int>(rhi, ents[p].rec);
+      }
+    }
     ph.max_light[w] = mx;
+    ph.light_rec_lo[w] = rhi >= 0 ? rlo : 0;
+    ph.light_rec_n[w] = rhi >= 0 ? rhi - rlo + 1 : 0;
   }
   return HEOMQ_OK;
 }
@@ -515,7 +529,9 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     p.rowptr = (w == 0 ? c->fwd.rowptr : c->trn.rowptr).as<int64_t>();
     p.ent = (w == 0 ? c->p_entf : c->p_entt).as<DevEntryP>();
     p.trace_slot = ph.tslot;
-    p.maps = c->p_maps.as<int16_t>();
+    p.maps = c->p_maps.as<int32_t>();
+    p.light_rec_lo = ph.light_rec_lo[w];
+    p.light_rec_n = ph.light_rec_n[w];
     p.n_heavy = ph.n_heavy;
     p.light_begin = ph.light_begin;
     p.light_block_ado = c->p_lblock.as<int32_t>();
@@ -728,6 +744,7 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
   if (int rc = build_packed(c.get(), m, rowf, entf, rowt, entt)) return rc;
   sync_packed(c.get());
   HIPCHECK(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
+  HIPCHECK(hipStreamCreateWithFlags(&c->solve_stream, hipStreamNonBlocking));
   HIPCHECK(c->scal.alloc(sizeof(TfqmrScal) * 2));
   HIPCHECK(hipMemset(c->scal.p, 0, sizeof(TfqmrScal) * 2));
   HIPCHECK(c->partial.alloc(sizeof(double) * 2 * kMaxRedBlocks));
@@ -744,6 +761,7 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
   if (!c || !h) return fail(HEOMQ_BAD_ARG, "null argument");
   if (h->m.N != c->N || h->m.M != c->M) return fail(HEOMQ_BAD_ARG, "model shape differs from context");
   HIPCHECK(hipSetDevice(c->device));
+  c->drop_graphs();   // captured chunks hold table pointers and values' addresses
   if (int rc = build_packed_h(c, h->m)) return rc;   // checks conservation before anything changes
   if (int rc = upload_coefs(c, h->m)) return rc;
   fill_op(c, c->fwd, c->rec, false, -1.0);

@@ -95,10 +95,33 @@ struct heomq_ctx {
   int nsec = 0;
   bool cons_n = true, cons_s = true;
   void* pinned = nullptr;            // small pinned host buffer for scalar read-back
+  heomq::TierBounds tb{};
+  // Solves asked for on the null stream run on this non-blocking stream (the
+  // null stream cannot be captured into a graph).
+  hipStream_t solve_stream = nullptr;
+  // Captured TFQMR iteration chunks.  Every pointer a chunk's kernels take
+  // follows from the key (iterate, right-hand side, workspace base, length)
+  // and the context's tables, so a key match means the graph is valid; the
+  // cache is dropped whenever the tables are re-uploaded.
+  struct ChunkGraph {
+    const void* x;
+    const void* b;
+    const void* work;
+    int64_t n, tpos;
+    int mode, trans, bytes, stream_id;
+    hipStream_t stream;
+    hipGraphExec_t exec;
+  };
+  std::vector<ChunkGraph> graphs;
+  bool use_graphs = true;            // false: launch the chunk's kernels one by one
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
   ~heomq_ctx() {
+    drop_graphs();
+    if (solve_stream) (void)hipStreamDestroy(solve_stream);
     if (pinned) (void)hipHostFree(pinned);
   }
-  heomq::TierBounds tb{};
-  hipStream_t own_stream = nullptr;
 };
 

@@ -60,7 +60,9 @@ struct DevOperator {
 // ---------------------------------------------------------------------------
 struct DevEntryP {       // packed coupling (16 bytes)
   int32_t base;          // neighbour's first packed element
-  int32_t mapL, mapR;    // offsets of the gather maps (output slot -> signed neighbour slot); -1: no term
+  int32_t map;           // offset of the paired gather map (output slot -> two signed neighbour
+                         // slots, left and right operator, as int16 halves); -1: no term
+  int32_t spare;
   uint16_t rec;          // coefficient record (cL, cR)
   uint8_t flags;         // bit0 X^H, bit1 sL = -1, bit2 sR = -1
   uint8_t nsec;
@@ -97,16 +99,19 @@ struct DevPacked {
                                              // 257 of these ADOs
   int max_light_couplings = 0;       // largest coupling count of a top-tier ADO (<= 8)
   // slot-space tables: i(HX - XH) per sector as CSR over packed slots, and the
-  // gather maps (int16 value (q + 1) * sign, 0 = no term) referenced by entries
+  // gather maps (value (q + 1) * sign, 0 = no term) referenced by entries
   // the same rows padded to a fixed width hell_w (slot -1 / coefficient 0 as
   // padding) when every row is short: index loads then depend only on (s, t)
   int hell_w = 0;                    // 0: use the CSR form
-  const int16_t* hq_ell = nullptr;   // [S][d*d][hell_w]
+  int hell_wp = 0;                   // hell_w rounded up to 8 (index rows are read 8 at a time)
+  const int16_t* hq_ell = nullptr;   // [S][d*d][hell_wp]
   const double* hc_ell = nullptr;    // [S][d*d][hell_w] complex
   const int32_t* hrow = nullptr;     // [S][d*d + 1] row starts into hq / hc
   const int16_t* hq = nullptr;
   const double* hc = nullptr;        // complex coefficients of i(HX - XH)
-  const int16_t* maps = nullptr;
+  const int32_t* maps = nullptr;     // paired gather maps (int16 left | int16 right << 16)
+  // coefficient records the top-tier entries use: [light_rec_lo, light_rec_lo + light_rec_n)
+  int light_rec_lo = 0, light_rec_n = 0;
 };
 
 // Projected apply on packed vectors: y = Pi A x with the trace row on the

@@ -34,62 +34,55 @@ __device__ inline typename P2<T>::type ld(const typename P2<T>::type* xv, int64_
   return v;
 }
 
+// Element `g` of x, from the LDS window [win_lo, win_lo + win_n) when it lies
+// there (the window already holds the trace-row zero of the transposed apply).
+template <typename T, bool TRANS>
+__device__ inline typename P2<T>::type ldw(const typename P2<T>::type* __restrict__ xv, int64_t g, int64_t zero_at,
+                                          const typename P2<T>::type* s_x, int64_t win_lo, int win_n) {
+  const int64_t r = g - win_lo;
+  if (s_x != nullptr && r >= 0 && r < win_n) return s_x[r];
+  return ld<T, TRANS>(xv, g, zero_at);
+}
+
 // gamma_K x_t + hsign * i(HX - XH)_t for packed slot t of the ADO at `base`.
-// The CSR row of i(HX - XH) is read four terms at a time so that its index
-// and data loads are in flight together.
+// Fixed-width rows: the eight slot indices of a row come in one 16-byte load,
+// the coefficients follow, and the neighbour elements (same ADO) are read from
+// the LDS window when it holds them.
 template <typename T, bool TRANS>
 __device__ inline void diag_term(const DevPacked& op, const typename P2<T>::type* __restrict__ xv, T gr, T gi,
-                                 int64_t base, int s, int t, int64_t zero_at, T& ar, T& ai) {
+                                 int64_t base, int s, int t, int64_t zero_at, T& ar, T& ai,
+                                 const typename P2<T>::type* s_x = nullptr, int64_t win_lo = 0, int win_n = 0) {
+  using V = typename P2<T>::type;
   const int dd = op.d * op.d;
-  const auto xe = ld<T, TRANS>(xv, base + t, zero_at);
+  const V xe = ldw<T, TRANS>(xv, base + t, zero_at, s_x, win_lo, win_n);
   ar = gr * xe.x - gi * xe.y;
   ai = gr * xe.y + gi * xe.x;
   T cr = 0, ci = 0;
   const int W = op.hell_w;
-  if (W > 0) {   // fixed-width rows: index and coefficient loads depend only on (s, t)
-    const int64_t row = ((int64_t)s * dd + t) * W;
-    for (int k0 = 0; k0 < W; k0 += 4) {
-      int q[4];
-      T hr[4], hi[4];
+  if (W > 0) {
+    const int64_t row = (int64_t)s * dd + t;
+    const int4* qrow = reinterpret_cast<const int4*>(op.hq_ell + row * op.hell_wp);
+    const double2* crow = reinterpret_cast<const double2*>(op.hc_ell) + row * W;
+    for (int k0 = 0; k0 < W; k0 += 8) {
+      const int4 qq = qrow[k0 >> 3];
+      const int packed[4] = {qq.x, qq.y, qq.z, qq.w};
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const bool ok = k0 + k < W;
-        q[k] = ok ? op.hq_ell[row + k0 + k] : 0;
-        hr[k] = ok ? (T)op.hc_ell[2 * (row + k0 + k)] : (T)0;
-        hi[k] = ok ? (T)op.hc_ell[2 * (row + k0 + k) + 1] : (T)0;
-      }
-      typename P2<T>::type v[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) v[k] = ld<T, TRANS>(xv, base + q[k], zero_at);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        cr += hr[k] * v[k].x - hi[k] * v[k].y;
-        ci += hr[k] * v[k].y + hi[k] * v[k].x;
+      for (int k = 0; k < 8; ++k) {
+        if (k0 + k >= W) break;
+        const int q = (int)(int16_t)((packed[k >> 1] >> ((k & 1) * 16)) & 0xffff);
+        const double2 h = crow[k0 + k];
+        const V v = ldw<T, TRANS>(xv, base + q, zero_at, s_x, win_lo, win_n);
+        cr += (T)h.x * v.x - (T)h.y * v.y;
+        ci += (T)h.x * v.y + (T)h.y * v.x;
       }
     }
-    const T hs = (T)op.hsign;
-    ar += hs * cr;
-    ai += hs * ci;
-    return;
-  }
-  const int r0 = op.hrow[s * (dd + 1) + t], r1 = op.hrow[s * (dd + 1) + t + 1];
-  for (int u0 = r0; u0 < r1; u0 += 4) {
-    int q[4];
-    T hr[4], hi[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const bool ok = u0 + k < r1;
-      q[k] = ok ? op.hq[u0 + k] : 0;
-      hr[k] = ok ? (T)op.hc[2 * (u0 + k)] : (T)0;
-      hi[k] = ok ? (T)op.hc[2 * (u0 + k) + 1] : (T)0;
-    }
-    typename P2<T>::type v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = ld<T, TRANS>(xv, base + q[k], zero_at);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      cr += hr[k] * v[k].x - hi[k] * v[k].y;
-      ci += hr[k] * v[k].y + hi[k] * v[k].x;
+  } else {
+    const int r0 = op.hrow[s * (dd + 1) + t], r1 = op.hrow[s * (dd + 1) + t + 1];
+    for (int u = r0; u < r1; ++u) {
+      const V v = ldw<T, TRANS>(xv, base + op.hq[u], zero_at, s_x, win_lo, win_n);
+      const T hr = (T)op.hc[2 * u], hi = (T)op.hc[2 * u + 1];
+      cr += hr * v.x - hi * v.y;
+      ci += hr * v.y + hi * v.x;
     }
   }
   const T hs = (T)op.hsign;
@@ -106,20 +99,20 @@ __device__ inline CoefLR load_coef(const DevRec* rec, int k) {
   return CoefLR{p[0], p[1]};
 }
 
-// U couplings at once: every entry's gather-map values, then every neighbour
-// element, are loaded before any of them is used (independent loads in
-// flight instead of a dependent chain per coupling).
+// U couplings at once: every entry's paired gather-map value (one 32-bit load
+// for the left and right operator), then every neighbour element, are loaded
+// before any of them is used.
 template <typename T, bool TRANS, int U>
 __device__ inline void coupling_block(const DevEntryP* en, const CoefLR* cf, int nvalid,
-                                      const int16_t* __restrict__ maps, const typename P2<T>::type* __restrict__ xv,
+                                      const int32_t* __restrict__ maps, const typename P2<T>::type* __restrict__ xv,
                                       int t, int64_t zero_at, T& ar, T& ai) {
   using V = typename P2<T>::type;
   int mL[U], mR[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const bool ok = u < nvalid;
-    mL[u] = (ok && en[u].mapL >= 0) ? (int)maps[en[u].mapL + t] : 0;
-    mR[u] = (ok && en[u].mapR >= 0) ? (int)maps[en[u].mapR + t] : 0;
+    const int mv = (u < nvalid && en[u].map >= 0) ? maps[en[u].map + t] : 0;
+    mL[u] = (int)(int16_t)(mv & 0xffff);
+    mR[u] = (int)(int16_t)((uint32_t)mv >> 16);
   }
   V vL[U], vR[U];
 #pragma unroll
@@ -142,20 +135,37 @@ __device__ inline void coupling_block(const DevEntryP* en, const CoefLR* cf, int
   }
 }
 
-// Top-tier ADOs: one thread per packed element.  The headers (offset, first
-// and last entry, sector, gamma) of the non-empty ADOs a block covers (at most
-// 257, checked when the tables are built) are staged in LDS and the element's
-// ADO is found there; the couplings are then read U at a time (U = the top
+constexpr int kLightRec = 384;   // coefficient records a top-tier block stages in LDS
+
+// Top-tier ADOs: one thread per packed element.  A block stages in LDS the
+// headers (offset, first and last entry, sector, gamma) of the non-empty ADOs
+// it covers (at most 257, checked when the tables are built), its own 256
+// elements of x (the commutator's neighbours), and the coefficient records
+// the top tier uses; the couplings are then read U at a time (U = the top
 // tier's coupling count, at most 4 per step).
+// LDS carve-up of the light and heavy bodies (they share one buffer).
+constexpr size_t kAlign16(size_t b) { return (b + 15) / 16 * 16; }
+constexpr size_t kLightOffP0 = kAlign16(257 * 8), kLightOffP1 = kLightOffP0 + kAlign16(257 * 8),
+                 kLightOffSec = kLightOffP1 + kAlign16(257 * 8), kLightOffGam = kLightOffSec + kAlign16(257 * 4),
+                 kLightOffX = kLightOffGam + kAlign16(257 * 16), kLightOffRec = kLightOffX + 256 * 16,
+                 kLightSmem = kLightOffRec + kLightRec * sizeof(CoefLR);
+constexpr size_t kHeavyOffRi = 256 * 8, kHeavyOffEn = 2 * 256 * 8, kHeavyOffCf = kHeavyOffEn + 256 * sizeof(DevEntryP),
+                 kHeavyOffX = kHeavyOffCf + 256 * sizeof(CoefLR), kHeavySmem = kHeavyOffX + 256 * 16;
+constexpr size_t kApplySmem = kLightSmem > kHeavySmem ? kLightSmem : kHeavySmem;
+
 template <typename T, bool TRANS, int U>
-__global__ void __launch_bounds__(256)
-k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at) {
+__device__ inline void light_body(const DevPacked& op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at,
+                                  int bid, char* smem) {
   using V = typename P2<T>::type;
-  __shared__ int64_t s_off[257];
-  __shared__ int64_t s_p0[257], s_p1[257];
-  __shared__ int s_sec[257];
-  __shared__ double2 s_gam[257];
-  const int lo = op.light_block_ado[blockIdx.x], hi = op.light_block_ado[blockIdx.x + 1];
+  int64_t* s_off = reinterpret_cast<int64_t*>(smem);
+  int64_t* s_p0 = reinterpret_cast<int64_t*>(smem + kLightOffP0);
+  int64_t* s_p1 = reinterpret_cast<int64_t*>(smem + kLightOffP1);
+  int* s_sec = reinterpret_cast<int*>(smem + kLightOffSec);
+  double2* s_gam = reinterpret_cast<double2*>(smem + kLightOffGam);
+  V* s_x = reinterpret_cast<V*>(smem + kLightOffX);
+  CoefLR* s_rec = reinterpret_cast<CoefLR*>(smem + kLightOffRec);
+  const V* __restrict__ xv = reinterpret_cast<const V*>(x);
+  const int lo = op.light_block_ado[bid], hi = op.light_block_ado[bid + 1];
   const int cnt = hi - lo + 1;
   for (int k = threadIdx.x; k < cnt; k += 256) {
     const int64_t a = op.light_nz[lo + k];
@@ -165,8 +175,14 @@ k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, i
     s_sec[k] = op.sec[a];
     s_gam[k] = reinterpret_cast<const double2*>(op.gam)[a];
   }
+  const bool rec_staged = op.light_rec_n <= kLightRec;
+  if (rec_staged)
+    for (int k = threadIdx.x; k < op.light_rec_n; k += 256) s_rec[k] = load_coef(op.rec, op.light_rec_lo + k);
+  const int64_t win_lo = op.light_begin + (int64_t)bid * 256;
+  const int win_n = (int)(op.total - win_lo < 256 ? op.total - win_lo : 256);
+  if (threadIdx.x < win_n) s_x[threadIdx.x] = ld<T, TRANS>(xv, win_lo + threadIdx.x, zero_at);
   __syncthreads();
-  const int64_t t = op.light_begin + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t t = win_lo + threadIdx.x;
   if (t >= op.total) return;   // after the only barrier
   int l = 0, h = cnt - 1;
   while (l < h) {
@@ -176,9 +192,9 @@ k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, i
   const int64_t base = s_off[l];
   const int slot = (int)(t - base);
   const int64_t p0 = s_p0[l], p1 = s_p1[l];
-  const V* __restrict__ xv = reinterpret_cast<const V*>(x);
   T ar, ai;
-  diag_term<T, TRANS>(op, xv, (T)s_gam[l].x, (T)s_gam[l].y, base, s_sec[l], slot, zero_at, ar, ai);
+  diag_term<T, TRANS>(op, xv, (T)s_gam[l].x, (T)s_gam[l].y, base, s_sec[l], slot, zero_at, ar, ai, s_x, win_lo,
+                      win_n);
   for (int64_t p = p0; p < p1; p += U) {
     const int nv = (int)(p1 - p < U ? p1 - p : U);
     DevEntryP en[U];
@@ -187,7 +203,7 @@ k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, i
     for (int u = 0; u < U; ++u) {
       if (u < nv) {
         en[u] = op.ent[p + u];
-        cf[u] = load_coef(op.rec, en[u].rec);
+        cf[u] = rec_staged ? s_rec[en[u].rec - op.light_rec_lo] : load_coef(op.rec, en[u].rec);
       } else {
         en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
         cf[u] = CoefLR{};
@@ -210,14 +226,16 @@ __device__ void tier0_fixup_block(const DevPacked& op, const T* __restrict__ x, 
 // couplings per step; the slices are summed through LDS in a fixed order.
 // Block 0 (the tier-0 ADO) then applies the trace row and Hermitianises.
 template <typename T, bool TRANS>
-__global__ void __launch_bounds__(256)
-k_apply_packed_heavy(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
+__device__ inline void heavy_body(const DevPacked& op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at,
+                                  int rdo, int bid, char* smem) {
   using V = typename P2<T>::type;
   constexpr int kStage = 256;
-  __shared__ T red_r[256], red_i[256];
-  __shared__ DevEntryP s_en[kStage];
-  __shared__ CoefLR s_cf[kStage];
-  const int64_t a = blockIdx.x;
+  T* red_r = reinterpret_cast<T*>(smem);
+  T* red_i = reinterpret_cast<T*>(smem + kHeavyOffRi);
+  DevEntryP* s_en = reinterpret_cast<DevEntryP*>(smem + kHeavyOffEn);
+  CoefLR* s_cf = reinterpret_cast<CoefLR*>(smem + kHeavyOffCf);
+  V* s_x = reinterpret_cast<V*>(smem + kHeavyOffX);
+  const int64_t a = bid;
   const int64_t base = op.off[a];
   const int nnz = (int)(op.off[a + 1] - base);
   const V* __restrict__ xv = reinterpret_cast<const V*>(x);
@@ -232,6 +250,8 @@ k_apply_packed_heavy(const DevPacked op, const T* __restrict__ x, T* __restrict_
       s_en[k] = e0;
       s_cf[k] = load_coef(op.rec, e0.rec);
     }
+  const int win_n = nnz < 256 ? nnz : 256;
+  if (threadIdx.x < win_n) s_x[threadIdx.x] = ld<T, TRANS>(xv, base + threadIdx.x, zero_at);
   __syncthreads();
   constexpr int U = 4;
   for (int t0 = 0; t0 < nnz; t0 += 256) {   // nnz is uniform over the block: every thread takes the same trips
@@ -241,7 +261,7 @@ k_apply_packed_heavy(const DevPacked op, const T* __restrict__ x, T* __restrict_
     const int t = t0 + lane;
     T ar = 0, ai = 0;
     if (slice < S) {
-      if (slice == 0) diag_term<T, TRANS>(op, xv, gr, gi, base, s, t, zero_at, ar, ai);
+      if (slice == 0) diag_term<T, TRANS>(op, xv, gr, gi, base, s, t, zero_at, ar, ai, s_x, base, win_n);
       for (int k0 = slice; k0 < nent; k0 += S * U) {
         DevEntryP en[U];
         CoefLR cf[U];
@@ -284,6 +304,19 @@ k_apply_packed_heavy(const DevPacked op, const T* __restrict__ x, T* __restrict_
     __syncthreads();
   }
   if (a == 0) tier0_fixup_block<T>(op, x, y, rdo);   // block-uniform branch
+}
+
+// One launch for the whole apply: blocks [0, n_heavy) take the ADOs below the
+// top tier (dispatched first: their work per block is the longest), the rest
+// the top tier's elements, so the two phases overlap on the device.
+template <typename T, bool TRANS, int U>
+__global__ void __launch_bounds__(256)
+k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
+  __shared__ __align__(16) char smem[kApplySmem];
+  if ((int64_t)blockIdx.x < op.n_heavy)   // block-uniform
+    heavy_body<T, TRANS>(op, x, y, zero_at, rdo, (int)blockIdx.x, smem);
+  else
+    light_body<T, TRANS, U>(op, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem);
 }
 
 // Tier-0 fix-ups (one block): trace row (forward: y(d,d) = tr x0; transposed:
@@ -387,37 +420,28 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
   if (rdo && op.trace_slot < 0) return hipErrorInvalidValue;
   if (op.total <= 0) return hipSuccess;
   const int64_t zero_at = (op.transposed && rdo) ? op.trace_slot : -1;   // ADO 0 starts at packed offset 0
-  hipError_t err;
-  if (op.n_heavy > 0) {
-    if (op.transposed)
-      hipLaunchKernelGGL((k_apply_packed_heavy<T, true>), dim3((unsigned)op.n_heavy), dim3(256), 0, s, op, x, y,
-                         zero_at, rdo ? 1 : 0);
-    else
-      hipLaunchKernelGGL((k_apply_packed_heavy<T, false>), dim3((unsigned)op.n_heavy), dim3(256), 0, s, op, x, y,
-                         zero_at, rdo ? 1 : 0);
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
-  }
   const int64_t light = op.total - op.light_begin;
-  if (light > 0) {
-    const dim3 grid((unsigned)((light + 255) / 256));
+  const int64_t blocks = op.n_heavy + (light > 0 ? (light + 255) / 256 : 0);
+  if (blocks > 0) {
+    const dim3 grid((unsigned)blocks);
     const int u = op.max_light_couplings;
-#define HEOMQ_LIGHT(UU)                                                                              \
-  do {                                                                                               \
-    if (op.transposed)                                                                               \
-      hipLaunchKernelGGL((k_apply_packed<T, true, UU>), grid, dim3(256), 0, s, op, x, y, zero_at);   \
-    else                                                                                             \
-      hipLaunchKernelGGL((k_apply_packed<T, false, UU>), grid, dim3(256), 0, s, op, x, y, zero_at);  \
+    const int r = rdo ? 1 : 0;
+#define HEOMQ_APPLY(UU)                                                                                  \
+  do {                                                                                                   \
+    if (op.transposed)                                                                                   \
+      hipLaunchKernelGGL((k_apply_packed<T, true, UU>), grid, dim3(256), 0, s, op, x, y, zero_at, r);    \
+    else                                                                                                 \
+      hipLaunchKernelGGL((k_apply_packed<T, false, UU>), grid, dim3(256), 0, s, op, x, y, zero_at, r);   \
   } while (0)
-    if (u <= 1) HEOMQ_LIGHT(1);
-    else if (u == 2) HEOMQ_LIGHT(2);
-    else if (u == 3) HEOMQ_LIGHT(3);
-    else HEOMQ_LIGHT(4);
-#undef HEOMQ_LIGHT
-    err = hipGetLastError();
+    if (u <= 1) HEOMQ_APPLY(1);
+    else if (u == 2) HEOMQ_APPLY(2);
+    else if (u == 3) HEOMQ_APPLY(3);
+    else HEOMQ_APPLY(4);
+#undef HEOMQ_APPLY
+    const hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
-  if (op.n_heavy > 0) return hipSuccess;   // block 0 of the heavy kernel did the tier-0 fix-up
+  if (op.n_heavy > 0) return hipSuccess;   // block 0 (the tier-0 ADO) did the tier-0 fix-up
   hipLaunchKernelGGL((k_tier0_fixup<T>), dim3(1), dim3(256), 0, s, op, x, y, rdo ? 1 : 0);
   return hipGetLastError();
 }

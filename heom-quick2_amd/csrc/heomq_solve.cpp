@@ -54,8 +54,9 @@ int64_t tpos_of(const heomq_ctx* c, int trace) {
 
 template <typename T>
 struct Work {
-  T *w, *y1, *y2, *u1, *u2, *v, *d, *rt;
+  T *w, *y1, *y2, *u1, *u2, *v, *d;
 };
+constexpr int kWorkVecs = 7;
 
 template <typename T>
 int carve(DevBuf& buf, int64_t n, int count, std::vector<T*>& out) {
@@ -71,6 +72,53 @@ struct RunStats {
   int iters = 0, applies = 0;
   double relres = 0;
 };
+
+// The TFQMR iteration, `iters` times: step1, A y2, step2, step3, A y1, v.
+template <typename T, typename Apply>
+hipError_t record_chunk(heomq_ctx* c, T* x, const Work<T>& W, int64_t n, hipStream_t st, Apply& A, int iters) {
+  TfqmrScal* s = c->scal.as<TfqmrScal>();
+  double* part = c->partial.as<double>();
+  for (int k = 0; k < iters; ++k) {
+    if (hipError_t e = tfq_step1<T>(st, n, W.y1, W.v, W.y2, W.w, W.u1, W.d, s, part)) return e;
+    if (hipError_t e = A(W.y2, W.u2)) return e;
+    if (hipError_t e = tfq_step2<T>(st, n, x, W.w, W.u2, W.d, W.y2, s, part)) return e;
+    if (hipError_t e = tfq_step3<T>(st, n, x, W.d, W.w, W.y2, W.y1, s)) return e;
+    if (hipError_t e = A(W.y1, W.u1)) return e;
+    if (hipError_t e = tfq_v<T>(st, n, W.u1, W.u2, W.v, s, part)) return e;
+  }
+  return hipSuccess;
+}
+
+// The chunk of `iters` iterations as an instantiated graph (captured once per
+// set of buffers, then replayed): one launch per chunk instead of ~11 per
+// iteration.  The kernels only read the device scalars, so a replay after
+// convergence or breakdown does nothing.
+template <typename T, typename Apply>
+int chunk_graph(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool trans, int64_t tpos, int mode,
+                hipStream_t st, Apply& A, int iters, hipGraphExec_t* out) {
+  for (const auto& g : c->graphs)
+    if (g.x == x && g.b == b && g.work == W.w && g.n == n && g.tpos == tpos && g.mode == mode &&
+        g.trans == (int)trans && g.bytes == (int)sizeof(T) && g.stream == st) {
+      *out = g.exec;
+      return HEOMQ_OK;
+    }
+  HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const hipError_t rec = record_chunk<T>(c, x, W, n, st, A, iters);
+  hipGraph_t graph = nullptr;
+  const hipError_t end = hipStreamEndCapture(st, &graph);
+  if (rec != hipSuccess || end != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(HEOMQ_HIP_ERROR, std::string("TFQMR chunk capture: ") + hipGetErrorString(rec != hipSuccess ? rec : end));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (inst != hipSuccess) return fail(HEOMQ_HIP_ERROR, std::string("TFQMR chunk instantiate: ") + hipGetErrorString(inst));
+  if (c->graphs.size() >= 16) c->drop_graphs();
+  c->graphs.push_back({x, b, W.w, n, tpos, mode, (int)trans, (int)sizeof(T), 0, st, exec});
+  *out = exec;
+  return HEOMQ_OK;
+}
 
 // TFQMR on A x = b with every vector and scalar on the device.  Restarts from
 // the true residual when the quasi-residual bound is met but ||b - Ax|| is not
@@ -88,8 +136,6 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   double* part = c->partial.as<double>();
   double* red = c->red.as<double>();
-  unsigned* ticket = reinterpret_cast<unsigned*>(red + 4);
-  HIPCHECK(vec_randn<T>(st, n, W.rt, 0x5EED1234ull));
   const int check_every = 8;
   double best = std::numeric_limits<double>::infinity();
   int base_iters = 0;
@@ -98,7 +144,7 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
     if (mode == 1) HIPCHECK(launch_herm_packed<T>(c->pfwd, x, st));
     HIPCHECK(A(x, W.u1));
     rs.applies++;
-    HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, ticket, part)));
+    HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, part)));
     double* hr = static_cast<double*>(c->pinned);
     HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -109,22 +155,19 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
     if (rr >= best && ++stalls > 3) break;
     best = std::min(best, rr);
     HIPCHECK(tfq_reset(st, s, tol * std::sqrt(bb)));
-    HIPCHECK(tfq_init_r<T>(st, n, b, W.u1, W.w, W.y1, W.d, W.rt, s, part));
+    HIPCHECK(tfq_init_r<T>(st, n, b, W.u1, W.w, W.y1, W.d, s, part));
     HIPCHECK(A(W.y1, W.u1));
     rs.applies++;
-    HIPCHECK(tfq_init_v<T>(st, n, W.u1, W.v, W.rt, s, part));
+    HIPCHECK(tfq_init_v<T>(st, n, W.u1, W.v, s, part));
     TfqmrScal& hs = *reinterpret_cast<TfqmrScal*>(static_cast<char*>(c->pinned) + 1024);
+    hipGraphExec_t chunk = nullptr;
+    if (c->use_graphs)
+      if (int rc = chunk_graph<T>(c, x, b, W, n, trans, tpos, mode, st, A, check_every, &chunk)) return rc;
     while (true) {
-      for (int k = 0; k < check_every; ++k) {
-        HIPCHECK(tfq_y2<T>(st, n, W.y1, W.v, W.y2, s));
-        HIPCHECK(A(W.y2, W.u2));
-        HIPCHECK(tfq_half<T>(st, n, x, 0, W.w, W.u1, W.d, W.y1, s, part));
-        HIPCHECK(tfq_half<T>(st, n, x, 1, W.w, W.u2, W.d, W.y2, s, part));
-        HIPCHECK(tfq_rho<T>(st, n, x, W.d, W.w, W.rt, s, part));
-        HIPCHECK(tfq_y1<T>(st, n, W.w, W.y2, W.y1, s));
-        HIPCHECK(A(W.y1, W.u1));
-        HIPCHECK(tfq_v<T>(st, n, W.u1, W.u2, W.v, W.rt, s, part));
-      }
+      if (chunk)
+        HIPCHECK(hipGraphLaunch(chunk, st));
+      else
+        HIPCHECK((record_chunk<T>(c, x, W, n, st, A, check_every)));
       HIPCHECK(hipMemcpyAsync(&hs, s, sizeof(TfqmrScal), hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
       rs.iters = base_iters + hs.iter;
@@ -135,7 +178,7 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
     if (hs.flag == 2 && ++stalls > 3) return fail(HEOMQ_BREAKDOWN, "TFQMR breakdown");
   }
   HIPCHECK(A(x, W.u1));
-  HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, ticket, part)));
+  HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, part)));
   double* hr = static_cast<double*>(c->pinned);
   HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
@@ -171,15 +214,19 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
   if (trace != HEOMQ_TRACE_LAST && trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "trace row required");
   hipStream_t st = (hipStream_t)stream;
+  if (st == nullptr) {   // the iteration is replayed as a graph, which the null stream cannot capture
+    HIPCHECK(hipStreamSynchronize(nullptr));
+    st = c->solve_stream;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   const bool packed = trace == HEOMQ_TRACE_RDO;
   // packed: the solve runs on the symmetry-packed physical sector with Pi*A
   const int64_t n = packed ? 2 * c->ptotal : 2 * c->N * c->d * c->d;
   std::vector<double*> v;
-  if (int rc = carve<double>(c->ws64, n, packed ? 10 : 9, v)) return rc;
-  Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
-  double* bb = v[8];
-  double* xs = packed ? v[9] : x;
+  if (int rc = carve<double>(c->ws64, n, kWorkVecs + (packed ? 2 : 1), v)) return rc;
+  Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6]};
+  double* bb = v[kWorkVecs];
+  double* xs = packed ? v[kWorkVecs + 1] : x;
   if (packed) {
     HIPCHECK((launch_pack<double, double>(c->pfwd, x, xs, st)));
     if (b) {
@@ -213,15 +260,19 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
   if (trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "mixed precision solves use the tier-0 trace row");
   hipStream_t st = (hipStream_t)stream;
+  if (st == nullptr) {
+    HIPCHECK(hipStreamSynchronize(nullptr));
+    st = c->solve_stream;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t n = 2 * c->ptotal;
   std::vector<double*> v64;
   if (int rc = carve<double>(c->ws64, n, 3, v64)) return rc;
   std::vector<float*> v32;
-  if (int rc = carve<float>(c->ws32, n, 10, v32)) return rc;
-  Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6], v32[7]};
-  float* x32 = v32[8];
-  float* r32 = v32[9];
+  if (int rc = carve<float>(c->ws32, n, kWorkVecs + 2, v32)) return rc;
+  Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6]};
+  float* x32 = v32[kWorkVecs];
+  float* r32 = v32[kWorkVecs + 1];
   double* ax = v64[0];
   double* bb = v64[1];
   double* xs = v64[2];
@@ -233,7 +284,6 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
     HIPCHECK(vec_set_elem(st, bb, 2 * (int64_t)c->pfwd.trace_slot, 1.0));
   }
   double* red = c->red.as<double>();
-  unsigned* ticket = reinterpret_cast<unsigned*>(red + 4);
   double* part = c->partial.as<double>();
   int outer = 0, inner_total = 0, applies = 0, fp64_iters = 0;
   double relres = 0, prev = 0;
@@ -243,7 +293,7 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
     HIPCHECK(launch_herm_packed<double>(c->pfwd, xs, st));
     HIPCHECK(launch_apply_packed_f64(c->pfwd, xs, ax, true, st));   // FP64 residual
     applies++;
-    HIPCHECK((vec_resid<double, double>(st, n, bb, ax, red, ticket, part)));
+    HIPCHECK((vec_resid<double, double>(st, n, bb, ax, red, part)));
     double* hr = static_cast<double*>(c->pinned);
     HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -272,8 +322,8 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   }
   if (finish64) {
     std::vector<double*> w64;
-    if (int e = carve<double>(c->ws64b, n, 8, w64)) return e;
-    Work<double> W64{w64[0], w64[1], w64[2], w64[3], w64[4], w64[5], w64[6], w64[7]};
+    if (int e = carve<double>(c->ws64b, n, kWorkVecs, w64)) return e;
+    Work<double> W64{w64[0], w64[1], w64[2], w64[3], w64[4], w64[5], w64[6]};
     RunStats rs;
     rc = tfqmr_run<double>(c, xs, bb, W64, n, false, 0, tol, std::max(maxiter - inner_total, 1), st, rs, 1);
     fp64_iters = rs.iters;
@@ -292,6 +342,12 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   }
   if (rc != HEOMQ_OK) return fail(HEOMQ_NOT_CONVERGED, "mixed-precision refinement did not converge");
   return rc;
+}
+
+int heomq_ctx_set_graphs(heomq_ctx* c, int enable) {
+  if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
+  c->use_graphs = enable != 0;
+  return HEOMQ_OK;
 }
 
 int heomq_apply_packed(heomq_ctx* c, const double* x, double* y, int trace, int transpose, void* stream) {

@@ -47,15 +47,25 @@ template <> struct W2<float> { using type = float2; };
 #define GRID_LOOP(i, n) \
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
+// Shadow vector of TFQMR, generated where it is used instead of stored: a
+// +-1 sign per real element from a splitmix64 hash of its index (the same
+// bits for any grid and any call).
+__device__ inline double shadow(int64_t k) {
+  uint64_t z = 0x5EED1234ull + 0x9E3779B97F4A7C15ull * (uint64_t)(k + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (z >> 63) ? -1.0 : 1.0;
+}
+
 // r = b - Ax (stored in w and y1), d = 0; partials of ||r||^2 and rt.r.
 template <typename T>
 __global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restrict__ ax, T* __restrict__ w,
-                         T* __restrict__ y1, T* __restrict__ d, const T* __restrict__ rt, double* partial) {
+                         T* __restrict__ y1, T* __restrict__ d, double* partial) {
   using V = typename W2<T>::type;
   double rr = 0, rrt = 0;
   GRID_LOOP(i, n2) {
     const V bb = reinterpret_cast<const V*>(b)[i], aa = reinterpret_cast<const V*>(ax)[i];
-    const V tt = reinterpret_cast<const V*>(rt)[i];
     V r;
     r.x = bb.x - aa.x;
     r.y = bb.y - aa.y;
@@ -66,123 +76,126 @@ __global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restric
     z.y = 0;
     reinterpret_cast<V*>(d)[i] = z;
     rr += (double)r.x * r.x + (double)r.y * r.y;
-    rrt += (double)tt.x * r.x + (double)tt.y * r.y;
+    rrt += shadow(2 * i) * r.x + shadow(2 * i + 1) * r.y;
   }
   block_partial(rr, rrt, partial);
 }
 
 // v = u1; partial of rt.v
 template <typename T>
-__global__ void k_init_v(int64_t n2, const T* __restrict__ u1, T* __restrict__ v, const T* __restrict__ rt,
-                         double* partial) {
+__global__ void k_init_v(int64_t n2, const T* __restrict__ u1, T* __restrict__ v, double* partial) {
   using V = typename W2<T>::type;
   double sg = 0;
   GRID_LOOP(i, n2) {
     const V t = reinterpret_cast<const V*>(u1)[i];
-    const V r = reinterpret_cast<const V*>(rt)[i];
     reinterpret_cast<V*>(v)[i] = t;
-    sg += (double)r.x * t.x + (double)r.y * t.y;
+    sg += shadow(2 * i) * t.x + shadow(2 * i + 1) * t.y;
   }
   block_partial(sg, 0.0, partial);
 }
 
+// First half of an iteration, fused with the next search direction:
+// y2 = y1 - alpha v;  w -= alpha u1;  d = y1 + dcoef d;  partial ||w||^2.
 template <typename T>
-__global__ void k_y2(int64_t n2, const T* __restrict__ y1, const T* __restrict__ v, T* __restrict__ y2,
-                     const TfqmrScal* s) {
+__global__ void k_step1(int64_t n2, const T* __restrict__ y1, const T* __restrict__ v, T* __restrict__ y2,
+                        T* __restrict__ w, const T* __restrict__ u1, T* __restrict__ d, const TfqmrScal* s,
+                        double* partial) {
   using V = typename W2<T>::type;
   if (s->flag) return;
-  const T alpha = (T)s->alpha;
+  const T alpha = (T)s->alpha, dcoef = (T)s->dcoef;
+  double ww = 0;
   GRID_LOOP(i, n2) {
     const V a = reinterpret_cast<const V*>(y1)[i], b = reinterpret_cast<const V*>(v)[i];
     V o;
     o.x = a.x - alpha * b.x;
     o.y = a.y - alpha * b.y;
     reinterpret_cast<V*>(y2)[i] = o;
-  }
-}
-
-// One TFQMR half step: [x += eta d], w -= alpha u, d = y + dcoef d; partial ||w||^2.
-template <typename T>
-__global__ void k_half(int64_t n2, T* __restrict__ x, int xupd, T* __restrict__ w, const T* __restrict__ u,
-                       T* __restrict__ d, const T* __restrict__ yj, const TfqmrScal* s, double* partial) {
-  using V = typename W2<T>::type;
-  if (s->flag) return;
-  const T alpha = (T)s->alpha, dcoef = (T)s->dcoef, eta = (T)s->eta;
-  double ww = 0;
-  GRID_LOOP(i, n2) {
-    const V di = reinterpret_cast<const V*>(d)[i];
-    if (xupd) {
-      V xi = reinterpret_cast<V*>(x)[i];
-      xi.x += eta * di.x;
-      xi.y += eta * di.y;
-      reinterpret_cast<V*>(x)[i] = xi;
-    }
     V wi = reinterpret_cast<V*>(w)[i];
-    const V ui = reinterpret_cast<const V*>(u)[i];
+    const V ui = reinterpret_cast<const V*>(u1)[i];
     wi.x -= alpha * ui.x;
     wi.y -= alpha * ui.y;
     reinterpret_cast<V*>(w)[i] = wi;
-    const V yi = reinterpret_cast<const V*>(yj)[i];
+    const V di = reinterpret_cast<const V*>(d)[i];
     V dn;
-    dn.x = yi.x + dcoef * di.x;
-    dn.y = yi.y + dcoef * di.y;
+    dn.x = a.x + dcoef * di.x;
+    dn.y = a.y + dcoef * di.y;
     reinterpret_cast<V*>(d)[i] = dn;
     ww += (double)wi.x * wi.x + (double)wi.y * wi.y;
   }
   block_partial(ww, 0.0, partial);
 }
 
-// x += eta d; partial of rt.w
+// Second half: x += eta d;  w -= alpha u2;  d = y2 + dcoef d;  partials of
+// ||w||^2 and rt.w (w is final for this iteration).
 template <typename T>
-__global__ void k_rho(int64_t n2, T* __restrict__ x, const T* __restrict__ d, const T* __restrict__ w,
-                      const T* __restrict__ rt, const TfqmrScal* s, double* partial) {
+__global__ void k_step2(int64_t n2, T* __restrict__ x, T* __restrict__ w, const T* __restrict__ u2,
+                        T* __restrict__ d, const T* __restrict__ y2, const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
   if (s->flag) return;
-  const T eta = (T)s->eta;
-  double r = 0;
+  const T alpha = (T)s->alpha, dcoef = (T)s->dcoef, eta = (T)s->eta;
+  double ww = 0, rw = 0;
+  GRID_LOOP(i, n2) {
+    const V di = reinterpret_cast<const V*>(d)[i];
+    V xi = reinterpret_cast<V*>(x)[i];
+    xi.x += eta * di.x;
+    xi.y += eta * di.y;
+    reinterpret_cast<V*>(x)[i] = xi;
+    V wi = reinterpret_cast<V*>(w)[i];
+    const V ui = reinterpret_cast<const V*>(u2)[i];
+    wi.x -= alpha * ui.x;
+    wi.y -= alpha * ui.y;
+    reinterpret_cast<V*>(w)[i] = wi;
+    const V yi = reinterpret_cast<const V*>(y2)[i];
+    V dn;
+    dn.x = yi.x + dcoef * di.x;
+    dn.y = yi.y + dcoef * di.y;
+    reinterpret_cast<V*>(d)[i] = dn;
+    ww += (double)wi.x * wi.x + (double)wi.y * wi.y;
+    rw += shadow(2 * i) * wi.x + shadow(2 * i + 1) * wi.y;
+  }
+  block_partial(ww, rw, partial);
+}
+
+// x += eta d;  y1 = w + beta y2 (not on the iteration that met the bound).
+template <typename T>
+__global__ void k_step3(int64_t n2, T* __restrict__ x, const T* __restrict__ d, const T* __restrict__ w,
+                        const T* __restrict__ y2, T* __restrict__ y1, const TfqmrScal* s) {
+  using V = typename W2<T>::type;
+  if (s->flag) return;
+  const T eta = (T)s->eta, beta = (T)s->beta;
+  const bool next = s->stop == 0;
   GRID_LOOP(i, n2) {
     V xi = reinterpret_cast<V*>(x)[i];
     const V di = reinterpret_cast<const V*>(d)[i];
     xi.x += eta * di.x;
     xi.y += eta * di.y;
     reinterpret_cast<V*>(x)[i] = xi;
-    const V ti = reinterpret_cast<const V*>(rt)[i], wi = reinterpret_cast<const V*>(w)[i];
-    r += (double)ti.x * wi.x + (double)ti.y * wi.y;
-  }
-  block_partial(r, 0.0, partial);
-}
-
-template <typename T>
-__global__ void k_y1(int64_t n2, const T* __restrict__ w, const T* __restrict__ y2, T* __restrict__ y1,
-                     const TfqmrScal* s) {
-  using V = typename W2<T>::type;
-  if (s->flag) return;
-  const T beta = (T)s->beta;
-  GRID_LOOP(i, n2) {
-    const V a = reinterpret_cast<const V*>(w)[i], b = reinterpret_cast<const V*>(y2)[i];
-    V o;
-    o.x = a.x + beta * b.x;
-    o.y = a.y + beta * b.y;
-    reinterpret_cast<V*>(y1)[i] = o;
+    if (next) {
+      const V a = reinterpret_cast<const V*>(w)[i], b = reinterpret_cast<const V*>(y2)[i];
+      V o;
+      o.x = a.x + beta * b.x;
+      o.y = a.y + beta * b.y;
+      reinterpret_cast<V*>(y1)[i] = o;
+    }
   }
 }
 
 // v = u1 + beta (u2 + beta v); partial of rt.v
 template <typename T>
 __global__ void k_v(int64_t n2, const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ v,
-                    const T* __restrict__ rt, const TfqmrScal* s, double* partial) {
+                    const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
-  if (s->flag) return;
+  if (s->flag || s->stop) return;
   const T beta = (T)s->beta;
   double sg = 0;
   GRID_LOOP(i, n2) {
     const V a = reinterpret_cast<const V*>(u1)[i], b = reinterpret_cast<const V*>(u2)[i];
-    const V c = reinterpret_cast<const V*>(v)[i], r = reinterpret_cast<const V*>(rt)[i];
+    const V c = reinterpret_cast<const V*>(v)[i];
     V t;
     t.x = a.x + beta * (b.x + beta * c.x);
     t.y = a.y + beta * (b.y + beta * c.y);
     reinterpret_cast<V*>(v)[i] = t;
-    sg += (double)r.x * t.x + (double)r.y * t.y;
+    sg += shadow(2 * i) * t.x + shadow(2 * i + 1) * t.y;
   }
   block_partial(sg, 0.0, partial);
 }
@@ -202,7 +215,17 @@ __global__ void k_resid(int64_t n2, const TB* __restrict__ b, const TA* __restri
 
 // Phase 2: one block folds the partials in a fixed order and advances the
 // scalars (Freund 1993; Saad, Iterative Methods 7.4.2).
-enum FinalizeOp { F_INIT_R = 0, F_INIT_V, F_HALF, F_RHO, F_V, F_RESID };
+enum FinalizeOp { F_INIT_R = 0, F_INIT_V, F_HALF, F_HALF_RHO, F_V, F_RESID };
+
+// One quasi-minimisation half step from ||w||^2 = a.
+__device__ inline void half_step(TfqmrScal* s, double a) {
+  const double theta = sqrt(a) / s->tau;
+  const double c = 1.0 / sqrt(1.0 + theta * theta);
+  s->tau = s->tau * theta * c;
+  s->theta = theta;
+  s->eta = c * c * s->alpha;
+  s->dcoef = theta * theta * s->eta / s->alpha;
+}
 
 __global__ void k_finalize(int op, int nblk, const double* __restrict__ partial, TfqmrScal* s, double* out) {
   if (op != F_INIT_R && op != F_INIT_V && op != F_RESID && s->flag) return;   // s unused for F_RESID
@@ -229,23 +252,20 @@ __global__ void k_finalize(int op, int nblk, const double* __restrict__ partial,
       s->alpha = s->rho / a;
       s->dcoef = 0;
       break;
-    case F_HALF: {
-      const double theta = sqrt(a) / s->tau;
-      const double c = 1.0 / sqrt(1.0 + theta * theta);
-      s->tau = s->tau * theta * c;
-      s->theta = theta;
-      s->eta = c * c * s->alpha;
-      s->dcoef = theta * theta * s->eta / s->alpha;
+    case F_HALF:
+      half_step(s, a);
       break;
-    }
-    case F_RHO:
+    case F_HALF_RHO:   // second half step, then rho_new = rt.w = b
+      half_step(s, a);
       s->iter += 1;
-      if (s->tau * sqrt(2.0 * s->iter + 1.0) <= s->target) { s->flag = 1; break; }
+      // bound met: the x update of this iteration still runs (k_step3); F_V then stops
+      if (s->tau * sqrt(2.0 * s->iter + 1.0) <= s->target) { s->stop = 1; break; }
       if (s->rho == 0.0) { s->flag = 2; break; }
-      s->beta = a / s->rho;
-      s->rho = a;
+      s->beta = b / s->rho;
+      s->rho = b;
       break;
     case F_V:
+      if (s->stop) { s->flag = 1; break; }
       s->sigma = a;
       if (a == 0.0) { s->flag = 2; break; }
       s->alpha = s->rho / a;
@@ -342,7 +362,7 @@ __global__ void k_tfq_reset(TfqmrScal* s, double target) {
   s->target = target;
   s->iter = 0;
   s->flag = 0;
-  s->ticket = 0;
+  s->stop = 0;
 }
 
 template <typename T>
@@ -370,53 +390,47 @@ static hipError_t finalize(hipStream_t st, int op, int nblk, const double* parti
 
 // n counts reals; the vector kernels walk n/2 complex pairs
 template <typename T>
-hipError_t tfq_init_r(hipStream_t st, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, const T* rt,
-                      TfqmrScal* s, double* partial) {
+hipError_t tfq_init_r(hipStream_t st, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, TfqmrScal* s,
+                      double* partial) {
   const int64_t n2 = n / 2;
-  L1((k_init_r<T>), n2, b, ax, w, y1, d, rt, partial);
+  L1((k_init_r<T>), n2, b, ax, w, y1, d, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_INIT_R, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
-hipError_t tfq_init_v(hipStream_t st, int64_t n, const T* u1, T* v, const T* rt, TfqmrScal* s, double* partial) {
+hipError_t tfq_init_v(hipStream_t st, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial) {
   const int64_t n2 = n / 2;
-  L1((k_init_v<T>), n2, u1, v, rt, partial);
+  L1((k_init_v<T>), n2, u1, v, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_INIT_V, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
-hipError_t tfq_y2(hipStream_t st, int64_t n, const T* y1, const T* v, T* y2, const TfqmrScal* s) {
+hipError_t tfq_step1(hipStream_t st, int64_t n, const T* y1, const T* v, T* y2, T* w, const T* u1, T* d,
+                     TfqmrScal* s, double* partial) {
   const int64_t n2 = n / 2;
-  L1((k_y2<T>), n2, y1, v, y2, s);
-  return hipGetLastError();
-}
-template <typename T>
-hipError_t tfq_half(hipStream_t st, int64_t n, T* x, int xupd, T* w, const T* u, T* d, const T* yj,
-                    TfqmrScal* s, double* partial) {
-  const int64_t n2 = n / 2;
-  L1((k_half<T>), n2, x, xupd, w, u, d, yj, s, partial);
+  L1((k_step1<T>), n2, y1, v, y2, w, u1, d, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_HALF, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
-hipError_t tfq_rho(hipStream_t st, int64_t n, T* x, const T* d, const T* w, const T* rt, TfqmrScal* s,
-                   double* partial) {
+hipError_t tfq_step2(hipStream_t st, int64_t n, T* x, T* w, const T* u2, T* d, const T* y2, TfqmrScal* s,
+                     double* partial) {
   const int64_t n2 = n / 2;
-  L1((k_rho<T>), n2, x, d, w, rt, s, partial);
+  L1((k_step2<T>), n2, x, w, u2, d, y2, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_RHO, grid_for<T>(n2), partial, s, nullptr);
+  return finalize(st, F_HALF_RHO, grid_for<T>(n2), partial, s, nullptr);
 }
 template <typename T>
-hipError_t tfq_y1(hipStream_t st, int64_t n, const T* w, const T* y2, T* y1, const TfqmrScal* s) {
+hipError_t tfq_step3(hipStream_t st, int64_t n, T* x, const T* d, const T* w, const T* y2, T* y1,
+                     const TfqmrScal* s) {
   const int64_t n2 = n / 2;
-  L1((k_y1<T>), n2, w, y2, y1, s);
+  L1((k_step3<T>), n2, x, d, w, y2, y1, s);
   return hipGetLastError();
 }
 template <typename T>
-hipError_t tfq_v(hipStream_t st, int64_t n, const T* u1, const T* u2, T* v, const T* rt, TfqmrScal* s,
-                 double* partial) {
+hipError_t tfq_v(hipStream_t st, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial) {
   const int64_t n2 = n / 2;
-  L1((k_v<T>), n2, u1, u2, v, rt, s, partial);
+  L1((k_v<T>), n2, u1, u2, v, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_V, grid_for<T>(n2), partial, s, nullptr);
 }
@@ -431,10 +445,8 @@ hipError_t vec_randn(hipStream_t st, int64_t n, T* v, uint64_t seed) {
   return hipGetLastError();
 }
 template <typename TB, typename TA>
-hipError_t vec_resid(hipStream_t st, int64_t n, const TB* b, const TA* ax, double* out, unsigned* ticket,
-                     double* partial) {
+hipError_t vec_resid(hipStream_t st, int64_t n, const TB* b, const TA* ax, double* out, double* partial) {
   using T = TB;
-  (void)ticket;
   const int64_t n2 = n / 2;
   L1((k_resid<TB, TA>), n2, b, ax, partial);
   if (hipError_t e = hipGetLastError()) return e;
@@ -477,24 +489,22 @@ hipError_t vec_lincomb3(hipStream_t st, int64_t n, double* out, double a, const 
 }
 
 #define INST(T)                                                                                          \
-  template hipError_t tfq_init_r<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, T*, const T*,       \
-                                    TfqmrScal*, double*);                                               \
-  template hipError_t tfq_init_v<T>(hipStream_t, int64_t, const T*, T*, const T*, TfqmrScal*, double*);  \
-  template hipError_t tfq_y2<T>(hipStream_t, int64_t, const T*, const T*, T*, const TfqmrScal*);         \
-  template hipError_t tfq_half<T>(hipStream_t, int64_t, T*, int, T*, const T*, T*, const T*, TfqmrScal*, \
-                                  double*);                                                             \
-  template hipError_t tfq_rho<T>(hipStream_t, int64_t, T*, const T*, const T*, const T*, TfqmrScal*,     \
-                                 double*);                                                              \
-  template hipError_t tfq_y1<T>(hipStream_t, int64_t, const T*, const T*, T*, const TfqmrScal*);         \
-  template hipError_t tfq_v<T>(hipStream_t, int64_t, const T*, const T*, T*, const T*, TfqmrScal*,       \
-                               double*);                                                                \
+  template hipError_t tfq_init_r<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, T*, TfqmrScal*,     \
+                                    double*);                                                           \
+  template hipError_t tfq_init_v<T>(hipStream_t, int64_t, const T*, T*, TfqmrScal*, double*);            \
+  template hipError_t tfq_step1<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, const T*, T*,       \
+                                   TfqmrScal*, double*);                                                \
+  template hipError_t tfq_step2<T>(hipStream_t, int64_t, T*, T*, const T*, T*, const T*, TfqmrScal*,     \
+                                   double*);                                                            \
+  template hipError_t tfq_step3<T>(hipStream_t, int64_t, T*, const T*, const T*, const T*, T*,           \
+                                   const TfqmrScal*);                                                   \
+  template hipError_t tfq_v<T>(hipStream_t, int64_t, const T*, const T*, T*, TfqmrScal*, double*);       \
   template hipError_t vec_fill<T>(hipStream_t, int64_t, T*, T);                                           \
   template hipError_t vec_randn<T>(hipStream_t, int64_t, T*, uint64_t);
 INST(double)
 INST(float)
 template hipError_t vec_resid<double, double>(hipStream_t, int64_t, const double*, const double*, double*,
-                                              unsigned*, double*);
-template hipError_t vec_resid<float, float>(hipStream_t, int64_t, const float*, const float*, double*,
-                                            unsigned*, double*);
+                                              double*);
+template hipError_t vec_resid<float, float>(hipStream_t, int64_t, const float*, const float*, double*, double*);
 
 }  // namespace heomq

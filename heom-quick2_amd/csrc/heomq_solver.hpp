@@ -17,39 +17,39 @@ struct TfqmrScal {
   double theta, tau, eta, dcoef;
   double rr, target;      // ||r0||^2, stop when tau*sqrt(2m+1) <= target
   int32_t iter, flag;     // flag: 0 running, 1 bound met, 2 breakdown
-  unsigned ticket;        // last-workgroup counter of the fused reductions
-  unsigned pad;
+  int32_t stop;           // bound met this iteration: finish its x update, then flag = 1
+  int32_t pad;
 };
 
 template <typename T>
 hipError_t vec_project(hipStream_t, int64_t nado, int d, T* x, const int8_t* par, const TierBounds& tb,
                        bool hermitian);
 
+// One TFQMR iteration (Freund 1993) is: tfq_step1, A y2 -> u2, tfq_step2,
+// tfq_step3, A y1 -> u1, tfq_v.  Every reduction is followed by its one-block
+// finalize; the shadow vector is generated in the kernels.
 template <typename T>
-hipError_t tfq_init_r(hipStream_t, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, const T* rt,
-                      TfqmrScal* s, double* partial);
+hipError_t tfq_init_r(hipStream_t, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, TfqmrScal* s,
+                      double* partial);
 template <typename T>
-hipError_t tfq_init_v(hipStream_t, int64_t n, const T* u1, T* v, const T* rt, TfqmrScal* s, double* partial);
+hipError_t tfq_init_v(hipStream_t, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial);
 template <typename T>
-hipError_t tfq_y2(hipStream_t, int64_t n, const T* y1, const T* v, T* y2, const TfqmrScal* s);
+hipError_t tfq_step1(hipStream_t, int64_t n, const T* y1, const T* v, T* y2, T* w, const T* u1, T* d,
+                     TfqmrScal* s, double* partial);
 template <typename T>
-hipError_t tfq_half(hipStream_t, int64_t n, T* x, int xupd, T* w, const T* u, T* d, const T* yj,
-                    TfqmrScal* s, double* partial);
+hipError_t tfq_step2(hipStream_t, int64_t n, T* x, T* w, const T* u2, T* d, const T* y2, TfqmrScal* s,
+                     double* partial);
 template <typename T>
-hipError_t tfq_rho(hipStream_t, int64_t n, T* x, const T* d, const T* w, const T* rt, TfqmrScal* s,
-                   double* partial);
+hipError_t tfq_step3(hipStream_t, int64_t n, T* x, const T* d, const T* w, const T* y2, T* y1,
+                     const TfqmrScal* s);
 template <typename T>
-hipError_t tfq_y1(hipStream_t, int64_t n, const T* w, const T* y2, T* y1, const TfqmrScal* s);
-template <typename T>
-hipError_t tfq_v(hipStream_t, int64_t n, const T* u1, const T* u2, T* v, const T* rt, TfqmrScal* s,
-                 double* partial);
+hipError_t tfq_v(hipStream_t, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial);
 template <typename T>
 hipError_t vec_fill(hipStream_t, int64_t n, T* v, T val);
 template <typename T>
 hipError_t vec_randn(hipStream_t, int64_t n, T* v, uint64_t seed);
 template <typename TB, typename TA>
-hipError_t vec_resid(hipStream_t, int64_t n, const TB* b, const TA* ax, double* out, unsigned* ticket,
-                     double* partial);
+hipError_t vec_resid(hipStream_t, int64_t n, const TB* b, const TA* ax, double* out, double* partial);
 hipError_t vec_resid_to_f32(hipStream_t, int64_t n, const double* b, const double* ax, float* r32, double scale);
 hipError_t vec_axpy_f32_to_f64(hipStream_t, int64_t n, double* x, const float* dx, double alpha);
 hipError_t tfq_reset(hipStream_t, TfqmrScal* s, double target);

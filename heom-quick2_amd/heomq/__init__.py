@@ -241,6 +241,10 @@ class Context:
             raise HeomqError(_lib.BAD_DIM, f"expected contiguous {dtype} device tensor with {self.N * self.d * self.d} elements")
         return C.c_void_p(v.data_ptr())
 
+    def set_graphs(self, enable: bool):
+        """Replay solver iterations from captured graphs (default) or launch eagerly."""
+        check(lib().heomq_ctx_set_graphs(self._h, int(enable)))
+
     def refresh(self, model: Model | None = None):
         check(lib().heomq_ctx_refresh(self._h, (model or self.model)._h))
 

@@ -142,6 +142,10 @@ int heomq_ctx_create(const heomq_model* m, int device, heomq_ctx** out);
 void heomq_ctx_free(heomq_ctx* c);
 /* Re-upload coefficients after heomq_model_set_params (same shape). */
 int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* m);
+/* Solver iterations are replayed from captured HIP graphs (one launch per
+ * eight TFQMR iterations; solves requested on the null stream run on a
+ * context-owned stream).  enable = 0 launches the same kernels one by one. */
+int heomq_ctx_set_graphs(heomq_ctx* c, int enable);
 /* Number of complex elements N*d*d of one ADO vector. */
 int64_t heomq_ctx_vector_len(const heomq_ctx* c);
 

@@ -146,3 +146,24 @@ def test_chebyshev_device(name):
     xo, ncheby, delta = O.chebyshev(o, x, 0.02, 2)
     assert info["ncheby"] == ncheby and abs(info["delta"] - delta) <= 1e-12 * delta
     assert rel(from_dev(xd), xo) <= 1e-12
+
+
+def test_graph_replays_match_eager():
+    """The solver's captured chunk graphs, replayed many times with host
+    synchronisations between replays, give the same iterates as launching the
+    same kernels eagerly; a second solve (cached graph) and a transposed solve
+    on the same context agree too."""
+    m, ctx, o, torch = setup("siam_nk3_L3")
+    ctx.set_graphs(False)
+    xe, se = ctx.solve_steady(tol=1e-12)
+    be = O.seeded_ado_vector(o, 61)
+    be[0] = 0
+    le, _ = ctx.solve_steady(x=ctx.zeros(), b=to_dev(torch, be), tol=1e-12, transpose=True)
+    ctx.set_graphs(True)
+    for rep in range(2):
+        xg, sg = ctx.solve_steady(tol=1e-12)
+        print("eager", se, "graph", sg, flush=True)
+        assert sg["iters"] == se["iters"]
+        assert (xg - xe).abs().max().item() <= 1e-14
+        lg, _ = ctx.solve_steady(x=ctx.zeros(), b=to_dev(torch, be), tol=1e-12, transpose=True)
+        assert (lg - le).abs().max().item() <= 1e-14 * max(le.abs().max().item(), 1.0)
