@@ -45,7 +45,7 @@ k_apply(const DevOperator op, const T* __restrict__ x, T* __restrict__ y, int64_
     for (int k = 0; k < EPT; ++k) {
       const int e = lane + k * LANES;
       V v = xv[a * E + e];
-      if (TRANS && a == trace_pos && e == E - 1) { v.x = 0; v.y = 0; }
+      if (TRANS && a == trace_pos && e == op.trace_elem) { v.x = 0; v.y = 0; }
       xs[slot * E + e] = v;
     }
   }
@@ -104,7 +104,7 @@ k_apply(const DevOperator op, const T* __restrict__ x, T* __restrict__ y, int64_
         const int idx = dag ? (j + pl * D) : (pl + j * D);
         V v = xn[idx];
         if (dag) v.y = -v.y;
-        if (zchk && idx == E - 1) { v.x = 0; v.y = 0; }
+        if (zchk && idx == op.trace_elem) { v.x = 0; v.y = 0; }
         const T s = (T)mL[D + i];
         ar[k] += s * (cLr * v.x - cLi * v.y);
         ai[k] += s * (cLr * v.y + cLi * v.x);
@@ -114,7 +114,7 @@ k_apply(const DevOperator op, const T* __restrict__ x, T* __restrict__ y, int64_
         const int idx = dag ? (pr + i * D) : (i + pr * D);
         V v = xn[idx];
         if (dag) v.y = -v.y;
-        if (zchk && idx == E - 1) { v.x = 0; v.y = 0; }
+        if (zchk && idx == op.trace_elem) { v.x = 0; v.y = 0; }
         const T s = (T)mR[3 * D + j];
         ar[k] += s * (cRr * v.x - cRi * v.y);
         ai[k] += s * (cRr * v.y + cRi * v.x);
@@ -157,24 +157,24 @@ __global__ void k_hermitize_block0(T* __restrict__ y, int d) {
 
 // Forward trace row: y(d,d,trace_pos) = tr x0 (calc_ax_tfqmr_omp.f90:492-495).
 template <typename T>
-__global__ void k_trace_fwd(const T* __restrict__ x, T* __restrict__ y, int d, int64_t pos) {
+__global__ void k_trace_fwd(const T* __restrict__ x, T* __restrict__ y, int d, int64_t pos, int elem) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   T sr = 0, si = 0;
   for (int i = 0; i < d; ++i) {
     sr += x[2 * (i + i * d)];
     si += x[2 * (i + i * d) + 1];
   }
-  const int64_t o = 2 * ((int64_t)pos * d * d + d * d - 1);
+  const int64_t o = 2 * ((int64_t)pos * d * d + elem);
   y[o] = sr;
   y[o + 1] = si;
 }
 
 // Transposed trace row: z0(i,i) += w(d,d,trace_pos).
 template <typename T>
-__global__ void k_trace_trn(const T* __restrict__ w, T* __restrict__ z, int d, int64_t pos) {
+__global__ void k_trace_trn(const T* __restrict__ w, T* __restrict__ z, int d, int64_t pos, int elem) {
   const int i = threadIdx.x;
   if (i >= d || blockIdx.x != 0) return;
-  const int64_t o = 2 * ((int64_t)pos * d * d + d * d - 1);
+  const int64_t o = 2 * ((int64_t)pos * d * d + elem);
   z[2 * (i + i * d)] += w[o];
   z[2 * (i + i * d) + 1] += w[o + 1];
 }
@@ -196,9 +196,9 @@ static hipError_t launch_d(const DevOperator& op, const T* x, T* y, int64_t trac
   if (err != hipSuccess) return err;
   if (trace_pos >= 0) {
     if (op.transposed)
-      hipLaunchKernelGGL((k_trace_trn<T>), dim3(1), dim3(64), 0, s, x, y, D, trace_pos);
+      hipLaunchKernelGGL((k_trace_trn<T>), dim3(1), dim3(64), 0, s, x, y, D, trace_pos, op.trace_elem);
     else
-      hipLaunchKernelGGL((k_trace_fwd<T>), dim3(1), dim3(64), 0, s, x, y, D, trace_pos);
+      hipLaunchKernelGGL((k_trace_fwd<T>), dim3(1), dim3(64), 0, s, x, y, D, trace_pos, op.trace_elem);
     err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
@@ -228,6 +228,80 @@ hipError_t launch_apply_f64(const DevOperator& op, const double* x, double* y, i
 hipError_t launch_apply_f32(const DevOperator& op, const float* x, float* y, int64_t trace_pos, hipStream_t s,
                             bool project) {
   return launch_any<float>(op, x, y, trace_pos, s, project);
+}
+
+// ---------------------------------------------------------------------------
+// BiCG convention around the same operator (tier-0 block only; element (i, j)
+// at i + j*d; lower triangle i > j).
+// ---------------------------------------------------------------------------
+// x0: real diagonal, upper = conj(lower)^T (calc_ax_bicg_omp.f90:70-77)
+__global__ void k_bicg_herm_in(int d, double* __restrict__ x) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d * d) return;
+  const int i = e % d, j = e / d;
+  if (i == j) x[2 * e + 1] = 0.0;
+  else if (i < j) {   // upper element reads the lower one, which no thread writes
+    const int t = j + i * d;
+    x[2 * e] = x[2 * t];
+    x[2 * e + 1] = -x[2 * t + 1];
+  }
+}
+// y0 (or w0): upper triangle zeroed, diagonal real (calc_ax_bicg_omp.f90:330-337;
+// calc_atrx_bicg_omp.f90:103-110)
+__global__ void k_bicg_lower(int d, double* __restrict__ y) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d * d) return;
+  const int i = e % d, j = e / d;
+  if (i == j) y[2 * e + 1] = 0.0;
+  else if (i < j) { y[2 * e] = 0.0; y[2 * e + 1] = 0.0; }
+}
+// z0 folded onto its lower triangle: the transpose of k_bicg_herm_in
+// (lower += conj(upper)^T, diagonal real, upper zero); each lower thread owns
+// its mirror element
+__global__ void k_bicg_fold(int d, double* __restrict__ z) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d * d) return;
+  const int i = e % d, j = e / d;
+  if (i == j) z[2 * e + 1] = 0.0;
+  else if (i > j) {
+    const int t = j + i * d;
+    z[2 * e] += z[2 * t];
+    z[2 * e + 1] -= z[2 * t + 1];
+    z[2 * t] = 0.0;
+    z[2 * t + 1] = 0.0;
+  }
+}
+
+hipError_t launch_herm_from_lower_f64(int d, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_herm_in, dim3((d * d + 255) / 256), dim3(256), 0, s, d, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_lower_tier0_f64(int d, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_lower, dim3((d * d + 255) / 256), dim3(256), 0, s, d, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_bicg_f64(const DevOperator& op, double* x, double* y, hipStream_t s) {
+  const int d = op.d;
+  hipLaunchKernelGGL(k_bicg_herm_in, dim3((d * d + 255) / 256), dim3(256), 0, s, d, x);
+  if (hipError_t e = hipGetLastError()) return e;
+  DevOperator o = op;
+  o.trace_elem = 0;   // y(1,1) of tier 0 = tr rho0 (calc_ax_bicg_omp.f90:321-324)
+  if (hipError_t e = launch_any<double>(o, x, y, 0, s, false)) return e;
+  hipLaunchKernelGGL(k_bicg_lower, dim3((d * d + 255) / 256), dim3(256), 0, s, d, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_transpose_bicg_f64(const DevOperator& op, double* w, double* z, hipStream_t s) {
+  const int d = op.d;
+  hipLaunchKernelGGL(k_bicg_lower, dim3((d * d + 255) / 256), dim3(256), 0, s, d, w);
+  if (hipError_t e = hipGetLastError()) return e;
+  DevOperator o = op;
+  o.trace_elem = 0;   // w(1,1) is the trace row's dual: kept out of L^T, added to the diagonal of z0
+  if (hipError_t e = launch_any<double>(o, w, z, 0, s, false)) return e;
+  hipLaunchKernelGGL(k_bicg_fold, dim3((d * d + 255) / 256), dim3(256), 0, s, d, z);
+  return hipGetLastError();
 }
 
 }  // namespace heomq

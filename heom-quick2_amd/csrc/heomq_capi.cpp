@@ -183,6 +183,7 @@ void fill_op(heomq_ctx* c, OperatorTables& t, const DevBuf& rec, bool transposed
   o.transposed = transposed;
   o.parity = c->parity.as<int8_t>();
   o.tb = c->tb;
+  o.trace_elem = c->d * c->d - 1;
 }
 
 // i(HX - XH) per symmetry sector as a CSR over packed slots (row t of sector

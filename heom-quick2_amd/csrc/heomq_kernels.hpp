@@ -48,6 +48,8 @@ struct DevOperator {
   bool transposed = false;
   const int8_t* parity = nullptr;    // electron-number parity of each Fock state [d]
   TierBounds tb{};
+  int trace_elem = 0;                // element (i + j*d) of the trace-row ADO the trace row replaces:
+                                     // d*d - 1 (TFQMR convention) or 0 (BiCG convention)
 };
 
 // ---------------------------------------------------------------------------
@@ -137,5 +139,18 @@ hipError_t launch_apply_f64(const DevOperator& op, const double* x, double* y,
                             int64_t trace_pos, hipStream_t s, bool project = false);
 hipError_t launch_apply_f32(const DevOperator& op, const float* x, float* y,
                             int64_t trace_pos, hipStream_t s, bool project = false);
+
+// BiCG convention (bicg/calc_ax_bicg_omp.f90, calc_atrx_bicg_omp.f90).  Forward:
+// x's tier 0 is rebuilt in place from its lower triangle, y = L x, y(1,1) of
+// tier 0 = tr rho0, then y's tier-0 upper triangle is zeroed and its diagonal
+// made real.  Transposed (op = the transposed operator): w's tier 0 is reduced
+// in place to its lower triangle with a real diagonal, z = L^T w with w(1,1)
+// moved onto the diagonal of z0, and z0 is folded back onto its lower triangle.
+hipError_t launch_apply_bicg_f64(const DevOperator& op, double* x, double* y, hipStream_t s);
+hipError_t launch_apply_transpose_bicg_f64(const DevOperator& op, double* w, double* z, hipStream_t s);
+// rho0 of a vector rebuilt from its lower triangle (real diagonal, upper = conj lower^T).
+hipError_t launch_herm_from_lower_f64(int d, double* x, hipStream_t s);
+// rho0 of a vector reduced to its lower triangle with a real diagonal.
+hipError_t launch_lower_tier0_f64(int d, double* x, hipStream_t s);
 
 }  // namespace heomq

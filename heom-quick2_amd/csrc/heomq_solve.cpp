@@ -344,6 +344,80 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   return rc;
 }
 
+// ---- BiCG convention (bicg/) ----
+int heomq_apply_bicg(heomq_ctx* c, double* x, double* y, void* stream) {
+  if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  HIPCHECK(launch_apply_bicg_f64(c->fwd.op, x, y, (hipStream_t)stream));
+  return HEOMQ_OK;
+}
+
+int heomq_apply_transpose_bicg(heomq_ctx* c, double* w, double* z, void* stream) {
+  if (!c || !w || !z) return fail(HEOMQ_BAD_ARG, "null argument");
+  HIPCHECK(launch_apply_transpose_bicg_f64(c->trn.op, w, z, (hipStream_t)stream));
+  return HEOMQ_OK;
+}
+
+// solvebicg.f90 on the device: x (dense, in/out) starts from its tier-0 lower
+// triangle (:52-58), b = e_(1,1) (:60-61), stop when ||r|| <= crit (absolute,
+// :124-126); x's rho0 is returned Hermitian (:108-113).
+int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* stats, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = 2 * c->N * c->d * c->d;
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64, n, 6, v)) return rc;
+  double *r = v[0], *sv = v[1], *dv = v[2], *f = v[3], *q = v[4], *qt = v[5];
+  BicgScal* sc = reinterpret_cast<BicgScal*>(c->scal.p);
+  double* part = c->partial.as<double>();
+  HIPCHECK(launch_lower_tier0_f64(c->d, x, st));
+  HIPCHECK(launch_apply_bicg_f64(c->fwd.op, x, q, st));
+  HIPCHECK(bicg_init(st, n, q, 0, r, sv, dv, f, sc, crit, part));
+  int applies = 1, iters = 0, flag = 0;
+  BicgScal& hs = *reinterpret_cast<BicgScal*>(static_cast<char*>(c->pinned) + 2048);
+  HIPCHECK(hipMemcpyAsync(&hs, sc, sizeof(BicgScal), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  flag = hs.flag;
+  const int check_every = 8;
+  while (flag == 0 && iters < maxiter) {
+    for (int k = 0; k < check_every; ++k) {
+      HIPCHECK(launch_apply_bicg_f64(c->fwd.op, dv, q, st));
+      HIPCHECK(bicg_alpha(st, n, f, q, sc, part));
+      HIPCHECK(launch_apply_transpose_bicg_f64(c->trn.op, f, qt, st));
+      HIPCHECK(bicg_update(st, n, x, dv, r, q, sv, qt, sc, part));
+      HIPCHECK(bicg_dirs(st, n, r, sv, dv, f, sc));
+    }
+    HIPCHECK(hipMemcpyAsync(&hs, sc, sizeof(BicgScal), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    iters = hs.iter;
+    flag = hs.flag;
+    applies = 1 + 2 * iters;
+  }
+  HIPCHECK(launch_herm_from_lower_f64(c->d, x, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (stats) {
+    stats[0] = iters;
+    stats[1] = applies;
+    stats[2] = std::sqrt(hs.rr);
+    stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (flag == 2) return fail(HEOMQ_BREAKDOWN, "BiCG breakdown");
+  if (flag != 1) return fail(HEOMQ_NOT_CONVERGED, "BiCG did not reach the criterion");
+  return HEOMQ_OK;
+}
+
+int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats) {
+  if (!c || !rho_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  const int64_t n = 2 * c->N * c->d * c->d;
+  HIPCHECK(c->ws_host.alloc(sizeof(double) * (size_t)n));
+  double* x = c->ws_host.as<double>();
+  HIPCHECK(hipMemcpy(x, rho_host, sizeof(double) * n, hipMemcpyHostToDevice));
+  const int rc = heomq_solve_bicg(c, x, crit, maxiter, stats, nullptr);
+  if (rc != HEOMQ_OK && rc != HEOMQ_NOT_CONVERGED) return rc;
+  HIPCHECK(hipMemcpy(rho_host, x, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return rc;
+}
+
 int heomq_ctx_set_graphs(heomq_ctx* c, int enable) {
   if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
   c->use_graphs = enable != 0;
@@ -541,6 +615,32 @@ int heomq_calc_atrx(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_
   if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
   if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_atrx_bicg_omp");
   return host_apply(c, x_host, y_host, HEOMQ_TRACE_LAST, true);
+}
+
+static int host_bicg(heomq_ctx* c, double* xh, double* yh, bool trans) {
+  const int64_t n = 2 * c->N * c->d * c->d;
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64, n, 2, v)) return rc;
+  HIPCHECK(hipMemcpy(v[0], xh, sizeof(double) * n, hipMemcpyHostToDevice));
+  if (trans)
+    HIPCHECK(launch_apply_transpose_bicg_f64(c->trn.op, v[0], v[1], nullptr));
+  else
+    HIPCHECK(launch_apply_bicg_f64(c->fwd.op, v[0], v[1], nullptr));
+  HIPCHECK(hipMemcpy(xh, v[0], sizeof(double) * n, hipMemcpyDeviceToHost));   // the input is inout
+  HIPCHECK(hipMemcpy(yh, v[1], sizeof(double) * n, hipMemcpyDeviceToHost));
+  return HEOMQ_OK;
+}
+
+int heomq_calc_ax_bicg(heomq_ctx* c, int32_t dim0, double* x_host, double* y_host) {
+  if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_ax_bicg_omp");
+  return host_bicg(c, x_host, y_host, false);
+}
+
+int heomq_calc_atrx_bicg(heomq_ctx* c, int32_t dim0, double* x_host, double* y_host) {
+  if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_atrx_bicg_omp");
+  return host_bicg(c, x_host, y_host, true);
 }
 
 int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host) {

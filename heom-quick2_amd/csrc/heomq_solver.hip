@@ -507,4 +507,151 @@ template hipError_t vec_resid<double, double>(hipStream_t, int64_t, const double
                                               double*);
 template hipError_t vec_resid<float, float>(hipStream_t, int64_t, const float*, const float*, double*, double*);
 
+// ---------------------------------------------------------------------------
+// BiCG (bicg/solvebicg.f90): real inner products of the (Re, Im) vectors
+// (cdotlong, bicg/ddotlong.f90:18-31), absolute stop ||r|| <= crit.
+// ---------------------------------------------------------------------------
+__global__ void k_bicg_init(int64_t n2, const double* __restrict__ q, int64_t e_pair, double* __restrict__ r,
+                            double* __restrict__ s, double* __restrict__ d, double* __restrict__ f, double* partial) {
+  double rr = 0;
+  GRID_LOOP(i, n2) {
+    const double2 qi = reinterpret_cast<const double2*>(q)[i];
+    double2 ri;
+    ri.x = (i == e_pair ? 1.0 : 0.0) - qi.x;
+    ri.y = -qi.y;
+    reinterpret_cast<double2*>(r)[i] = ri;
+    reinterpret_cast<double2*>(s)[i] = ri;
+    reinterpret_cast<double2*>(d)[i] = ri;
+    reinterpret_cast<double2*>(f)[i] = ri;
+    rr += ri.x * ri.x + ri.y * ri.y;
+  }
+  block_partial(rr, 0.0, partial);
+}
+
+__global__ void k_bicg_fq(int64_t n2, const double* __restrict__ f, const double* __restrict__ q,
+                          const BicgScal* sc, double* partial) {
+  if (sc->flag) return;
+  double a = 0;
+  GRID_LOOP(i, n2) {
+    const double2 fi = reinterpret_cast<const double2*>(f)[i], qi = reinterpret_cast<const double2*>(q)[i];
+    a += fi.x * qi.x + fi.y * qi.y;
+  }
+  block_partial(a, 0.0, partial);
+}
+
+__global__ void k_bicg_update(int64_t n2, double* __restrict__ x, const double* __restrict__ d,
+                              double* __restrict__ r, const double* __restrict__ q, double* __restrict__ s,
+                              const double* __restrict__ qt, const BicgScal* sc, double* partial) {
+  if (sc->flag) return;
+  const double alpha = sc->alpha;
+  double rr = 0, sr = 0;
+  GRID_LOOP(i, n2) {
+    double2 xi = reinterpret_cast<double2*>(x)[i];
+    const double2 di = reinterpret_cast<const double2*>(d)[i];
+    xi.x += alpha * di.x;
+    xi.y += alpha * di.y;
+    reinterpret_cast<double2*>(x)[i] = xi;
+    double2 ri = reinterpret_cast<double2*>(r)[i];
+    const double2 qi = reinterpret_cast<const double2*>(q)[i];
+    ri.x -= alpha * qi.x;
+    ri.y -= alpha * qi.y;
+    reinterpret_cast<double2*>(r)[i] = ri;
+    double2 si = reinterpret_cast<double2*>(s)[i];
+    const double2 ti = reinterpret_cast<const double2*>(qt)[i];
+    si.x -= alpha * ti.x;
+    si.y -= alpha * ti.y;
+    reinterpret_cast<double2*>(s)[i] = si;
+    rr += ri.x * ri.x + ri.y * ri.y;
+    sr += si.x * ri.x + si.y * ri.y;
+  }
+  block_partial(rr, sr, partial);
+}
+
+__global__ void k_bicg_dirs(int64_t n2, const double* __restrict__ r, const double* __restrict__ s,
+                            double* __restrict__ d, double* __restrict__ f, const BicgScal* sc) {
+  if (sc->flag) return;
+  const double beta = sc->beta;
+  GRID_LOOP(i, n2) {
+    const double2 ri = reinterpret_cast<const double2*>(r)[i], si = reinterpret_cast<const double2*>(s)[i];
+    double2 di = reinterpret_cast<double2*>(d)[i], fi = reinterpret_cast<double2*>(f)[i];
+    di.x = ri.x + beta * di.x;
+    di.y = ri.y + beta * di.y;
+    fi.x = si.x + beta * fi.x;
+    fi.y = si.y + beta * fi.y;
+    reinterpret_cast<double2*>(d)[i] = di;
+    reinterpret_cast<double2*>(f)[i] = fi;
+  }
+}
+
+enum BicgOp { B_INIT = 0, B_ALPHA, B_UPDATE };
+
+__global__ void k_bicg_finalize(int op, int nblk, const double* __restrict__ partial, BicgScal* sc, double crit) {
+  if (op != B_INIT && sc->flag) return;
+  double a = 0, b = 0;
+  for (int t = threadIdx.x; t < nblk; t += blockDim.x) {
+    a += partial[2 * t];
+    b += partial[2 * t + 1];
+  }
+  block_sum2(a, b);
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case B_INIT:   // solvebicg.f90:58-66: s = r, so <s, r> = ||r||^2
+      sc->rr = a;
+      sc->rho = a;
+      sc->crit = crit;
+      sc->iter = 0;
+      sc->flag = sqrt(a) <= crit ? 1 : 0;
+      sc->alpha = sc->beta = 0;
+      break;
+    case B_ALPHA:
+      if (a == 0.0) { sc->flag = 2; break; }
+      sc->alpha = sc->rho / a;
+      break;
+    case B_UPDATE:
+      sc->rr = a;
+      sc->iter += 1;
+      if (sqrt(a) <= sc->crit) { sc->flag = 1; break; }
+      if (sc->rho == 0.0) { sc->flag = 2; break; }
+      sc->beta = b / sc->rho;
+      sc->rho = b;
+      break;
+  }
+}
+
+static hipError_t bfin(hipStream_t st, int op, int nblk, const double* partial, BicgScal* sc, double crit) {
+  hipLaunchKernelGGL(k_bicg_finalize, dim3(1), dim3(kRedThreads), 0, st, op, nblk, partial, sc, crit);
+  return hipGetLastError();
+}
+
+hipError_t bicg_init(hipStream_t st, int64_t n, const double* q, int64_t e_off, double* r, double* s, double* d,
+                     double* f, BicgScal* sc, double crit, double* partial) {
+  using T = double;
+  const int64_t n2 = n / 2;
+  L1(k_bicg_init, n2, q, e_off / 2, r, s, d, f, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return bfin(st, B_INIT, grid_for<T>(n2), partial, sc, crit);
+}
+hipError_t bicg_alpha(hipStream_t st, int64_t n, const double* f, const double* q, BicgScal* sc, double* partial) {
+  using T = double;
+  const int64_t n2 = n / 2;
+  L1(k_bicg_fq, n2, f, q, sc, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return bfin(st, B_ALPHA, grid_for<T>(n2), partial, sc, 0.0);
+}
+hipError_t bicg_update(hipStream_t st, int64_t n, double* x, const double* d, double* r, const double* q, double* s,
+                       const double* qt, BicgScal* sc, double* partial) {
+  using T = double;
+  const int64_t n2 = n / 2;
+  L1(k_bicg_update, n2, x, d, r, q, s, qt, sc, partial);
+  if (hipError_t e = hipGetLastError()) return e;
+  return bfin(st, B_UPDATE, grid_for<T>(n2), partial, sc, 0.0);
+}
+hipError_t bicg_dirs(hipStream_t st, int64_t n, const double* r, const double* s, double* d, double* f,
+                     const BicgScal* sc) {
+  using T = double;
+  const int64_t n2 = n / 2;
+  L1(k_bicg_dirs, n2, r, s, d, f, sc);
+  return hipGetLastError();
+}
+
 }  // namespace heomq

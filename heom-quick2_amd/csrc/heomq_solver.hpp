@@ -21,6 +21,25 @@ struct TfqmrScal {
   int32_t pad;
 };
 
+// BiCG scalars (bicg/solvebicg.f90 notation: rho = <s, r>, alpha, beta).
+struct BicgScal {
+  double rho, alpha, beta, rr, crit;
+  int32_t iter, flag;     // flag: 0 running, 1 ||r|| <= crit, 2 breakdown
+};
+
+// BiCG vector steps on complex vectors of n reals (each followed by its finalize):
+//   bicg_init:  r = e - q (e: 1 at real element e_off), s = d = f = r;  ||r||^2
+//   bicg_alpha: <f, q> -> alpha = rho / <f, q>
+//   bicg_update: x += alpha d, r -= alpha q, s -= alpha qt;  ||r||^2, <s, r> -> beta
+//   bicg_dirs:  d = r + beta d, f = s + beta f
+hipError_t bicg_init(hipStream_t, int64_t n, const double* q, int64_t e_off, double* r, double* s, double* d,
+                     double* f, BicgScal* sc, double crit, double* partial);
+hipError_t bicg_alpha(hipStream_t, int64_t n, const double* f, const double* q, BicgScal* sc, double* partial);
+hipError_t bicg_update(hipStream_t, int64_t n, double* x, const double* d, double* r, const double* q, double* s,
+                       const double* qt, BicgScal* sc, double* partial);
+hipError_t bicg_dirs(hipStream_t, int64_t n, const double* r, const double* s, double* d, double* f,
+                     const BicgScal* sc);
+
 template <typename T>
 hipError_t vec_project(hipStream_t, int64_t nado, int d, T* x, const int8_t* par, const TierBounds& tb,
                        bool hermitian);

@@ -7,10 +7,12 @@
 ! exports the same names and argument meanings, backed by the MI355X engine:
 !
 !   calc_ax_tfqmr_omp(dim0, rcgtmp, rcgrhs)   tfqmr_double/calc_ax_tfqmr_omp.f90:1,15-17
+!   calc_ax_bicg_omp(dim0, rcgtmp, rcgrhs)    bicg/calc_ax_bicg_omp.f90:1,16-17
 !   calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs)  bicg/calc_atrx_bicg_omp.f90:1,35-36
 !   calcderiv_omp(tt)                         td/calcderiv_omp.f90:1-7 (module arrays
 !                                             rhotmp -> rhorhs, registered once)
 !   solve_tfqmr_gpu(rho, crit, maxit, info)   solver-level replacement of solve_tfqmr
+!   solve_bicg_gpu(rho, crit, maxit, info)    solver-level replacement of solvebicg
 !
 ! heomq_fortran_init uploads the reference's own module arrays once (after
 ! buildcoefindex), as heomq_model_from_reference + heomq_ctx_create.  Errors
@@ -60,11 +62,27 @@ module heomq_iface
       complex(c_double_complex) :: x(*), y(*)
     end function
 
-    integer(c_int) function c_calc_atrx(ctx, dim0, x, y) bind(C, name='heomq_calc_atrx')
+    integer(c_int) function c_calc_ax_bicg(ctx, dim0, x, y) bind(C, name='heomq_calc_ax_bicg')
       import :: c_int, c_int32_t, c_ptr, c_double_complex
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: dim0
       complex(c_double_complex) :: x(*), y(*)
+    end function
+
+    integer(c_int) function c_calc_atrx_bicg(ctx, dim0, x, y) bind(C, name='heomq_calc_atrx_bicg')
+      import :: c_int, c_int32_t, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dim0
+      complex(c_double_complex) :: x(*), y(*)
+    end function
+
+    integer(c_int) function c_solve_bicg_host(ctx, rho, crit, maxit, stats) bind(C, name='heomq_solve_bicg_host')
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: rho(*)
+      real(c_double), value :: crit
+      integer(c_int), value :: maxit
+      real(c_double) :: stats(*)
     end function
 
     integer(c_int) function c_calcderiv(ctx, x, y) bind(C, name='heomq_calcderiv')
@@ -139,9 +157,22 @@ subroutine calc_ax_tfqmr_omp(dim0, rcgtmp, rcgrhs)
   if (rc /= 0) call heomq_fail('calc_ax_tfqmr_omp', rc)
 end subroutine calc_ax_tfqmr_omp
 
-! bicg/calc_atrx_bicg_omp.f90:1,35-36 position -- the real transpose of the
-! TFQMR-convention operator (the reference's own transpose works in BiCG's
-! unknown convention: lower triangle of rho0, calc_atrx_bicg_omp.f90:103-111)
+! bicg/calc_ax_bicg_omp.f90:1,16-17 -- the BiCG-convention apply; rcgtmp's
+! rho0 is rebuilt from its lower triangle in place, as the reference does
+subroutine calc_ax_bicg_omp(dim0, rcgtmp, rcgrhs)
+  use heomq_iface
+  implicit none
+  integer, intent(in) :: dim0
+  complex*16, intent(inout) :: rcgtmp(*)
+  complex*16, intent(inout) :: rcgrhs(*)
+  integer(c_int) :: rc
+  rc = c_calc_ax_bicg(heomq_ctx_h, int(dim0, c_int32_t), rcgtmp, rcgrhs)
+  if (rc /= 0) call heomq_fail('calc_ax_bicg_omp', rc)
+end subroutine calc_ax_bicg_omp
+
+! bicg/calc_atrx_bicg_omp.f90:1,35-36 -- the real transpose of the BiCG-
+! convention apply; rcgtmp's rho0 is reduced in place to its lower triangle
+! with a real diagonal (calc_atrx_bicg_omp.f90:103-110)
 subroutine calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs)
   use heomq_iface
   implicit none
@@ -149,7 +180,7 @@ subroutine calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs)
   complex*16, intent(inout) :: rcgtmp(*)
   complex*16, intent(inout) :: rcgrhs(*)
   integer(c_int) :: rc
-  rc = c_calc_atrx(heomq_ctx_h, int(dim0, c_int32_t), rcgtmp, rcgrhs)
+  rc = c_calc_atrx_bicg(heomq_ctx_h, int(dim0, c_int32_t), rcgtmp, rcgrhs)
   if (rc /= 0) call heomq_fail('calc_atrx_bicg_omp', rc)
 end subroutine calc_atrx_bicg_omp
 
@@ -196,3 +227,21 @@ subroutine solve_tfqmr_gpu(rho, crit, maxit, info)
   info(2) = int(stats(1))
   if (rc /= 0 .and. rc /= 4) call heomq_fail('solve_tfqmr_gpu', rc)
 end subroutine solve_tfqmr_gpu
+
+! Solver-level drop-in for bicg/solvebicg.f90: rho holds the initial guess on
+! entry and the steady state on exit (rho0 Hermitian); crit is the absolute
+! ||r|| criterion of solvebicg.f90:124-126; info(1) = status, info(2) = iterations.
+subroutine solve_bicg_gpu(rho, crit, maxit, info)
+  use heomq_iface
+  implicit none
+  complex*16, intent(inout) :: rho(*)
+  real*8, intent(in) :: crit
+  integer, intent(in) :: maxit
+  integer, intent(out) :: info(2)
+  real(c_double) :: stats(8)
+  integer(c_int) :: rc
+  rc = c_solve_bicg_host(heomq_ctx_h, rho, real(crit, c_double), int(maxit, c_int), stats)
+  info(1) = int(rc)
+  info(2) = int(stats(1))
+  if (rc /= 0 .and. rc /= 4) call heomq_fail('solve_bicg_gpu', rc)
+end subroutine solve_bicg_gpu

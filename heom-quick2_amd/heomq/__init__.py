@@ -329,6 +329,34 @@ class Context:
                                        *[_dp(a.view(np.float64)) for a in arrs], self._stream()))
         return y
 
+    def apply_bicg(self, x, y=None):
+        """BiCG-convention apply (calc_ax_bicg_omp); x's tier 0 is rebuilt in place from its lower triangle."""
+        t = self.torch
+        y = self.zeros() if y is None else y
+        check(lib().heomq_apply_bicg(self._h, self._check_vec(x, t.complex128), self._check_vec(y, t.complex128),
+                                     self._stream()))
+        return y
+
+    def apply_transpose_bicg(self, w, z=None):
+        """Transpose of the BiCG-convention apply (calc_atrx_bicg_omp); w's tier 0 is reduced in place."""
+        t = self.torch
+        z = self.zeros() if z is None else z
+        check(lib().heomq_apply_transpose_bicg(self._h, self._check_vec(w, t.complex128),
+                                               self._check_vec(z, t.complex128), self._stream()))
+        return z
+
+    def solve_bicg(self, x=None, crit=1e-10, maxiter=20000, raise_on_fail=True):
+        """Steady state by BiCG (solvebicg.f90 convention: absolute ||r|| <= crit)."""
+        t = self.torch
+        x = self.initial_guess() if x is None else x
+        stats = np.zeros(8)
+        rc = lib().heomq_solve_bicg(self._h, self._check_vec(x, t.complex128), crit, maxiter, _dp(stats),
+                                    self._stream())
+        info = dict(iters=int(stats[0]), applies=int(stats[1]), resid=stats[2], seconds=stats[3], status=rc)
+        if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
+            check(rc)
+        return x, info
+
     # ---- solvers ----
     def initial_guess(self):
         x = self.zeros()

@@ -72,6 +72,12 @@ SIGNATURES = {
     "heomq_calc_ax_tfqmr": (C.c_int, [P, C.c_int32, D, D]),
     "heomq_calc_atrx": (C.c_int, [P, C.c_int32, D, D]),
     "heomq_calcderiv": (C.c_int, [P, D, D]),
+    "heomq_apply_bicg": (C.c_int, [P, P, P, P]),
+    "heomq_apply_transpose_bicg": (C.c_int, [P, P, P, P]),
+    "heomq_calc_ax_bicg": (C.c_int, [P, C.c_int32, D, D]),
+    "heomq_calc_atrx_bicg": (C.c_int, [P, C.c_int32, D, D]),
+    "heomq_solve_bicg": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
+    "heomq_solve_bicg_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),
     "heomq_solve_steady": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, D, P]),
     "heomq_solve_steady_mixed": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_int, C.c_int, D, P]),
     "heomq_solve_steady_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),

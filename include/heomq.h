@@ -187,6 +187,19 @@ int heomq_calc_ax_tfqmr(heomq_ctx* c, int32_t dim0, const double* x_host,
                         double* y_host);
 /* Host-pointer transpose (calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs) position). */
 int heomq_calc_atrx(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_host);
+/* BiCG convention on device buffers: y = A_B x, the operator of
+ * bicg/calc_ax_bicg_omp.f90 (x's tier 0 is rebuilt in place from its lower
+ * triangle, y(1,1) of tier 0 = tr rho0, y's tier-0 upper triangle zero and
+ * diagonal real), and z = A_B^T w, the operator of bicg/calc_atrx_bicg_omp.f90
+ * (w's tier 0 is reduced in place to its lower triangle with a real diagonal).
+ * Both inputs are modified as the reference's are (intent inout). */
+int heomq_apply_bicg(heomq_ctx* c, double* x, double* y, void* stream);
+int heomq_apply_transpose_bicg(heomq_ctx* c, double* w, double* z, void* stream);
+/* Host-pointer drop-ins: calc_ax_bicg_omp(dim0, rcgtmp, rcgrhs) and
+ * calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs) (calc_ax_bicg_omp.f90:16-17,
+ * calc_atrx_bicg_omp.f90:35-36: both arguments inout). */
+int heomq_calc_ax_bicg(heomq_ctx* c, int32_t dim0, double* x_host, double* y_host);
+int heomq_calc_atrx_bicg(heomq_ctx* c, int32_t dim0, double* x_host, double* y_host);
 /* calcderiv_omp(tt): rhorhs = L(rhotmp) for fixed H (host pointers). */
 int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host);
 
@@ -207,6 +220,13 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol,
 int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double tol,
                              double inner_tol, int maxiter, int trace,
                              double* stats, void* stream);
+/* Steady state by BiCG in the reference's convention (bicg/solvebicg.f90):
+ * x (device, dense, in/out) holds the initial guess, b = e_(1,1) of tier 0,
+ * stop when ||b - A_B x||_2 <= crit (absolute, as solvebicg.f90:124-126);
+ * x's rho0 is returned Hermitian.  stats as heomq_solve_steady. */
+int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* stats, void* stream);
+/* The same on a host array (the solvebicg drop-in of the Fortran layer). */
+int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats);
 /* Host-pointer steady state (the solve_tfqmr drop-in of the Fortran shim):
  * rho_host (d, d, N) complex holds the initial guess on entry (a trace-one
  * tier-0 block, Hermitian as tfqmr_double/solve_tfqmr.f90:97-103 makes it)

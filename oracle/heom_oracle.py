@@ -787,6 +787,83 @@ def dense_matrix(model: Model, trace_row="last", transpose=False):
 
 
 # --------------------------------------------------------------------------
+# BiCG convention (bicg/calc_ax_bicg_omp.f90, calc_atrx_bicg_omp.f90, solvebicg.f90)
+# --------------------------------------------------------------------------
+def _herm_from_lower(x0):
+    """rho0 rebuilt from its lower triangle: real diagonal, upper = conj(lower)^T
+    (calc_ax_bicg_omp.f90:70-77; math indices, row i > column j is lower)."""
+    L = np.tril(x0, -1)
+    return L + L.conj().T + np.diag(np.real(np.diag(x0)))
+
+
+def apply_bicg(model: Model, x):
+    """BiCG-convention apply (calc_ax_bicg_omp.f90): tier 0 of x is
+    Hermitianised from its lower triangle (the caller's x is left untouched
+    here; the reference overwrites it), y = L x, y(1,1) of tier 0 = tr rho0
+    (:321-324), then the upper triangle of y's tier 0 is zeroed and its
+    diagonal made real (:330-337)."""
+    xh = x.copy()
+    xh[0] = _herm_from_lower(x[0])
+    y = apply(model, xh, trace_row=None)
+    y[0, 0, 0] = np.trace(xh[0])
+    y[0] = np.tril(y[0], -1) + np.diag(np.real(np.diag(y[0])))
+    return y
+
+
+def apply_transpose_bicg(model: Model, w):
+    """The real transpose of :func:`apply_bicg` (the operator of
+    calc_atrx_bicg_omp.f90): w's tier 0 reduced to its lower triangle with a
+    real diagonal (:103-110), element (1,1) taken out of L^T's input and added
+    to the diagonal (the trace row), then the transpose of the lower-triangle
+    Hermitianisation folds the result back onto the lower triangle."""
+    d = model.d
+    wp = w.copy()
+    wp[0] = np.tril(w[0], -1) + np.diag(np.real(np.diag(w[0])))
+    t = np.real(wp[0, 0, 0])
+    wp[0, 0, 0] = 0
+    z = apply_transpose(model, wp, trace_row=None)
+    z[0] += t * np.eye(d)
+    z0 = z[0]
+    z[0] = np.tril(z0, -1) + np.tril(z0.T.conj(), -1) + np.diag(np.real(np.diag(z0)))
+    return z
+
+
+def bicg(model: Model, tol=1e-10, maxiter=20000, x0=None):
+    """Steady state by BiCG in the reference's convention (solvebicg.f90):
+    b = e_(1,1) of tier 0, shadow s0 = r0, absolute stop ||r|| <= tol
+    (:124-150); the returned rho0 is Hermitianised from its lower triangle
+    (:108-113).  Returns (x, iters, ||r||)."""
+    d = model.d
+    x = np.zeros((model.N, d, d), complex) if x0 is None else x0.copy()
+    if x0 is None:
+        x[0] = np.eye(d) / d
+    x[0] = np.tril(x[0], -1) + np.diag(np.real(np.diag(x[0])))
+    dot = lambda a, b: float(np.real(np.vdot(a, b)))
+    r = -apply_bicg(model, x)
+    r[0, 0, 0] += 1.0
+    if np.sqrt(dot(r, r)) <= tol:
+        x[0] = _herm_from_lower(x[0])
+        return x, 0, np.sqrt(dot(r, r))
+    s, dk, f = r.copy(), r.copy(), r.copy()
+    it = 0
+    while True:
+        q = apply_bicg(model, dk)
+        rho = dot(s, r)
+        alpha = rho / dot(f, q)
+        x = x + alpha * dk
+        r = r - alpha * q
+        s = s - alpha * apply_transpose_bicg(model, f)
+        err = np.sqrt(dot(r, r))
+        if err <= tol or it >= maxiter:
+            x[0] = _herm_from_lower(x[0])
+            return x, it + 1, err
+        beta = dot(s, r) / rho
+        dk = r + beta * dk
+        f = s + beta * f
+        it += 1
+
+
+# --------------------------------------------------------------------------
 # Sensitivities of the steady current (adjoint; finite differences)
 # --------------------------------------------------------------------------
 def current_weights(model: Model, lead=0):

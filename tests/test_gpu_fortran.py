@@ -1,8 +1,9 @@
 """The Fortran drop-in layer end to end on the GPU.
 
 libheomq_fortran.so exports the reference's own subroutine names
-(calc_ax_tfqmr_omp, calc_atrx_bicg_omp, calcderiv_omp) plus the one-time
-loader heomq_fortran_init and the solver-level solve_tfqmr_gpu.  This test
+(calc_ax_tfqmr_omp, calc_ax_bicg_omp, calc_atrx_bicg_omp, calcderiv_omp) plus
+the one-time loader heomq_fortran_init and the solver-level solve_tfqmr_gpu
+and solve_bicg_gpu.  This test
 calls them exactly as a Fortran caller would (every argument by reference,
 arrays in the reference's module layout, 1-based tables from
 tests/ref_arrays.py) and checks the results against the oracle.
@@ -80,10 +81,35 @@ def test_calc_ax_tfqmr_omp():
     assert rel(call_apply(h.calc_ax_tfqmr_omp_, o, x), O.apply(o, x, trace_row="last")) <= 1e-12
 
 
+def test_calc_ax_bicg_omp():
+    h, o = shim()
+    x = O.seeded_ado_vector(o, 34)
+    x[0] += np.triu(np.full((o.d, o.d), 0.3 + 0.7j), 1)    # the upper triangle is ignored
+    assert rel(call_apply(h.calc_ax_bicg_omp_, o, x), O.apply_bicg(o, x)) <= 1e-12
+
+
 def test_calc_atrx_bicg_omp():
     h, o = shim()
     w = O.seeded_ado_vector(o, 32)
-    assert rel(call_apply(h.calc_atrx_bicg_omp_, o, w), O.apply_transpose(o, w, trace_row="last")) <= 1e-12
+    assert rel(call_apply(h.calc_atrx_bicg_omp_, o, w), O.apply_transpose_bicg(o, w)) <= 1e-12
+
+
+def test_solve_bicg_gpu():
+    h, o = shim()
+    rho = np.zeros((o.N, o.d, o.d), complex)
+    rho[0] = np.eye(o.d) / o.d
+    rf = O.to_fortran(rho).ravel().copy()
+    crit, maxit = C.c_double(1e-10), C.c_int(5000)
+    info = (C.c_int * 2)()
+    h.solve_bicg_gpu_(rf.ctypes.data_as(C.c_void_p), C.byref(crit), C.byref(maxit), info)
+    print("solve_bicg_gpu info", list(info), flush=True)
+    assert info[0] == 0
+    xs = O.from_fortran(rf.reshape(o.N, o.d, o.d))
+    # the steady state is unique: BiCG's absolute ||r|| <= 1e-10 leaves rho0
+    # within 1e-9 of the oracle's TFQMR solution (1e-9 criterion: 6e-10 on the CPU)
+    xo, it, relres = O.steady_state(o, tol=1e-13)
+    assert np.abs(xs[0] - xo[0]).max() <= 1e-9
+    assert np.abs(xs[0] - xs[0].conj().T).max() == 0.0
 
 
 def test_calcderiv_omp():

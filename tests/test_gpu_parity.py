@@ -167,3 +167,33 @@ def test_graph_replays_match_eager():
         assert (xg - xe).abs().max().item() <= 1e-14
         lg, _ = ctx.solve_steady(x=ctx.zeros(), b=to_dev(torch, be), tol=1e-12, transpose=True)
         assert (lg - le).abs().max().item() <= 1e-14 * max(le.abs().max().item(), 1.0)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_bicg_convention_applies(name):
+    """calc_ax_bicg_omp / calc_atrx_bicg_omp conventions on the device vs the oracle."""
+    m, ctx, o, torch = setup(name)
+    x = O.seeded_ado_vector(o, 71)
+    x[0] += np.triu(np.full((o.d, o.d), 0.2 - 0.5j), 1)
+    xd = to_dev(torch, x)
+    y = from_dev(ctx.apply_bicg(xd))
+    assert rel(y, O.apply_bicg(o, x)) <= 1e-12
+    xh = from_dev(xd)                                  # input rebuilt in place from its lower triangle
+    assert np.abs(xh[0] - O._herm_from_lower(x[0])).max() <= 1e-15
+    w = O.seeded_ado_vector(o, 72)
+    z = from_dev(ctx.apply_transpose_bicg(to_dev(torch, w)))
+    assert rel(z, O.apply_transpose_bicg(o, w)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
+def test_bicg_steady_state(name):
+    """solvebicg on the device reaches the unique steady state (the oracle's
+    TFQMR solution; the oracle's own BiCG is pinned to it on the CPU)."""
+    m, ctx, o, torch = setup(name)
+    x, st = ctx.solve_bicg(crit=1e-10, maxiter=5000)
+    print(name, st, flush=True)
+    assert st["status"] == 0 and st["resid"] <= 1e-10
+    xb = from_dev(x)
+    assert np.abs(xb[0] - xb[0].conj().T).max() == 0.0
+    xt, _, _ = O.steady_state(o, tol=1e-13)
+    assert np.abs(xb[0] - xt[0]).max() <= 1e-9

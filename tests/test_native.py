@@ -35,8 +35,8 @@ def test_fortran_shim_exports_reference_names():
     if not path.exists():
         pytest.skip("Fortran compiler not available at build time")
     h = ctypes.CDLL(str(path))
-    for name in ("calc_ax_tfqmr_omp_", "calc_atrx_bicg_omp_", "calcderiv_omp_", "heomq_fortran_init_",
-                 "solve_tfqmr_gpu_"):
+    for name in ("calc_ax_tfqmr_omp_", "calc_ax_bicg_omp_", "calc_atrx_bicg_omp_", "calcderiv_omp_",
+                 "heomq_fortran_init_", "solve_tfqmr_gpu_", "solve_bicg_gpu_"):
         assert hasattr(h, name), name
 
 

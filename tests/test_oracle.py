@@ -307,3 +307,43 @@ def test_adjoint_gradient_matches_finite_differences():
     I, ga = O.current_gradient_adjoint(pf, th)
     gf = O.current_gradient_fd(pf, th, h=1e-5)
     assert np.abs(ga - gf).max() <= 1e-6 * np.abs(gf).max()
+
+
+def test_bicg_transpose_is_exact_transpose():
+    # brute force: the dense matrices of the BiCG-convention apply and of its
+    # transpose are transposes of each other
+    o = O.Model(O.siam_params(nk=1, ntop=1, V=0.3, U=1.5))
+    n = o.N * o.d * o.d * 2
+    A = np.zeros((n, n))
+    AT = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = O.as_real(O.apply_bicg(o, O.from_real(e, o)))
+        AT[:, j] = O.as_real(O.apply_transpose_bicg(o, O.from_real(e, o)))
+    assert np.abs(A.T - AT).max() <= 1e-12 * np.abs(A).max()
+
+
+def test_bicg_apply_agrees_with_tfqmr_apply_on_physical_states():
+    # for a Hermitian rho0 the two conventions differ only in where the trace
+    # row sits and in which tier-0 triangle is kept
+    o = O.Model(O.siam_params(nk=2, ntop=2, V=0.3, U=1.5))
+    x = O.seeded_ado_vector(o, 8)
+    yb = O.apply_bicg(o, x)
+    yn = O.apply(o, x, trace_row=None)
+    assert np.abs(yb[1:] - yn[1:]).max() <= 1e-13 * np.abs(yn).max()
+    low = np.tril(np.ones((o.d, o.d), bool), -1)
+    assert np.abs(yb[0][low] - yn[0][low]).max() <= 1e-13 * np.abs(yn).max()
+    assert abs(yb[0, 0, 0] - np.trace(x[0])) <= 1e-14
+
+
+def test_bicg_steady_state_matches_landauer():
+    # U = 0 closed form through the BiCG path (solvebicg.f90 convention)
+    p = O.siam_params(nk=3, ntop=2, V=0.4, U=0.0, eps=-0.3, T=0.2)
+    o = O.Model(p)
+    x, it, err = O.bicg(o, tol=1e-12, maxiter=3000)
+    assert err <= 1e-12
+    xt, _, _ = O.steady_state(o, tol=1e-13)
+    assert np.abs(x[0] - xt[0]).max() <= 1e-9
+    IL, n = O.landauer_siam(p)                  # per spin
+    assert np.abs(O.current(o, x)[0] - IL).max() <= 1e-8 * abs(IL)
