@@ -22,7 +22,7 @@
 
 using namespace heomq;
 
-namespace {
+namespace heomq {
 
 void build_records(const Model& m, std::vector<DevRec>& rec) {
   const int M = m.M;
@@ -100,6 +100,10 @@ void build_gamma(const Model& m, const std::vector<cplx>& gdraw, std::vector<dou
       gam[2 * g + 1] = conj ? -s.imag() : s.imag();
     }
 }
+
+}  // namespace heomq
+
+namespace {
 
 int build_entries(const Model& m, std::vector<int64_t>& rowf, std::vector<DevEntry>& entf,
                   std::vector<int64_t>& rowt, std::vector<DevEntry>& entt) {
@@ -505,6 +509,8 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
   c->sec_nnz_h = ph.nnz;
   c->fock_n = ph.fN;
   c->fock_s = ph.fS;
+  c->poff_h = ph.off;
+  c->psec_h = ph.sec;
   HIPCHECK(upload(c->p_lblock, ph.lblock));
   HIPCHECK(upload(c->p_lnz, ph.lnz));
   HIPCHECK(upload(c->p_off, ph.off));

@@ -86,6 +86,8 @@ struct heomq_ctx {
   // host copies of the sector tables (the packed H table is rebuilt from them)
   std::vector<int16_t> sec_elems_h, sec_pos_h;
   std::vector<int> sec_nnz_h, fock_n, fock_s;
+  std::vector<int64_t> poff_h;       // packed offset of each ADO (N + 1)
+  std::vector<uint8_t> psec_h;       // sector of each ADO
   DevBuf parity;                     // electron-number parity of each Fock state
   // symmetry-packed layout (solver path)
   DevBuf p_off, p_sec, p_elems, p_pos, p_block, p_lblock, p_entf, p_entt, p_hrow, p_hq, p_hc, p_maps;
@@ -125,3 +127,13 @@ struct heomq_ctx {
   }
 };
 
+// Currents I[alpha*nspin + s] and occupations from the dense tier-0/1 blocks
+// h (complex interleaved, first[2] * d * d elements), host side.
+void observables_host(const heomq_ctx* c, const std::vector<double>& h, double* current, double* occ);
+
+namespace heomq {
+// Coefficient records of the forward and transposed operators (6 M of them)
+// and the per-ADO diagonal rates gamma_K (conjugated for the transpose).
+void build_records(const Model& m, std::vector<DevRec>& rec);
+void build_gamma(const Model& m, const std::vector<cplx>& gdraw, std::vector<double>& gam, bool conj);
+}  // namespace heomq

@@ -114,6 +114,16 @@ struct DevPacked {
   const int32_t* maps = nullptr;     // paired gather maps (int16 left | int16 right << 16)
   // coefficient records the top-tier entries use: [light_rec_lo, light_rec_lo + light_rec_n)
   int light_rec_lo = 0, light_rec_n = 0;
+  // batch of problems sharing every structural table (grid y = problem):
+  // problem p's vectors start p * vstride complex elements in, its records,
+  // ADO rates and commutator coefficients p * rec_stride / gam_stride /
+  // hc_stride entries in (all 0 for one problem)
+  int64_t vstride = 0, rec_stride = 0, gam_stride = 0, hc_stride = 0;
+  int nprob = 1;
+  // batches: problem p is skipped when frozen_flag[p * frozen_stride] == 3
+  // (the solver's state word of a finished problem); null for no skipping
+  const int32_t* frozen_flag = nullptr;
+  int frozen_stride = 0;
 };
 
 // Projected apply on packed vectors: y = Pi A x with the trace row on the

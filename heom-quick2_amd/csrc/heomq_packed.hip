@@ -44,12 +44,20 @@ __device__ inline typename P2<T>::type ldw(const typename P2<T>::type* __restric
   return ld<T, TRANS>(xv, g, zero_at);
 }
 
+// The coefficient arrays of one problem of a batch.
+struct Coefs {
+  const DevRec* rec;
+  const double* gam;
+  const double* hc_ell;
+};
+
 // gamma_K x_t + hsign * i(HX - XH)_t for packed slot t of the ADO at `base`.
 // Fixed-width rows: the eight slot indices of a row come in one 16-byte load,
 // the coefficients follow, and the neighbour elements (same ADO) are read from
 // the LDS window when it holds them.
 template <typename T, bool TRANS>
-__device__ inline void diag_term(const DevPacked& op, const typename P2<T>::type* __restrict__ xv, T gr, T gi,
+__device__ inline void diag_term(const DevPacked& op, const double* __restrict__ hc_ell,
+                                 const typename P2<T>::type* __restrict__ xv, T gr, T gi,
                                  int64_t base, int s, int t, int64_t zero_at, T& ar, T& ai,
                                  const typename P2<T>::type* s_x = nullptr, int64_t win_lo = 0, int win_n = 0) {
   using V = typename P2<T>::type;
@@ -62,7 +70,7 @@ __device__ inline void diag_term(const DevPacked& op, const typename P2<T>::type
   if (W > 0) {
     const int64_t row = (int64_t)s * dd + t;
     const int4* qrow = reinterpret_cast<const int4*>(op.hq_ell + row * op.hell_wp);
-    const double2* crow = reinterpret_cast<const double2*>(op.hc_ell) + row * W;
+    const double2* crow = reinterpret_cast<const double2*>(hc_ell) + row * W;
     for (int k0 = 0; k0 < W; k0 += 8) {
       const int4 qq = qrow[k0 >> 3];
       const int packed[4] = {qq.x, qq.y, qq.z, qq.w};
@@ -154,8 +162,8 @@ constexpr size_t kHeavyOffRi = 256 * 8, kHeavyOffEn = 2 * 256 * 8, kHeavyOffCf =
 constexpr size_t kApplySmem = kLightSmem > kHeavySmem ? kLightSmem : kHeavySmem;
 
 template <typename T, bool TRANS, int U>
-__device__ inline void light_body(const DevPacked& op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at,
-                                  int bid, char* smem) {
+__device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
+                                  int64_t zero_at, int bid, char* smem) {
   using V = typename P2<T>::type;
   int64_t* s_off = reinterpret_cast<int64_t*>(smem);
   int64_t* s_p0 = reinterpret_cast<int64_t*>(smem + kLightOffP0);
@@ -173,11 +181,11 @@ __device__ inline void light_body(const DevPacked& op, const T* __restrict__ x, 
     s_p0[k] = op.rowptr[a];
     s_p1[k] = op.rowptr[a + 1];
     s_sec[k] = op.sec[a];
-    s_gam[k] = reinterpret_cast<const double2*>(op.gam)[a];
+    s_gam[k] = reinterpret_cast<const double2*>(cf_p.gam)[a];
   }
   const bool rec_staged = op.light_rec_n <= kLightRec;
   if (rec_staged)
-    for (int k = threadIdx.x; k < op.light_rec_n; k += 256) s_rec[k] = load_coef(op.rec, op.light_rec_lo + k);
+    for (int k = threadIdx.x; k < op.light_rec_n; k += 256) s_rec[k] = load_coef(cf_p.rec, op.light_rec_lo + k);
   const int64_t win_lo = op.light_begin + (int64_t)bid * 256;
   const int win_n = (int)(op.total - win_lo < 256 ? op.total - win_lo : 256);
   if (threadIdx.x < win_n) s_x[threadIdx.x] = ld<T, TRANS>(xv, win_lo + threadIdx.x, zero_at);
@@ -193,8 +201,8 @@ __device__ inline void light_body(const DevPacked& op, const T* __restrict__ x, 
   const int slot = (int)(t - base);
   const int64_t p0 = s_p0[l], p1 = s_p1[l];
   T ar, ai;
-  diag_term<T, TRANS>(op, xv, (T)s_gam[l].x, (T)s_gam[l].y, base, s_sec[l], slot, zero_at, ar, ai, s_x, win_lo,
-                      win_n);
+  diag_term<T, TRANS>(op, cf_p.hc_ell, xv, (T)s_gam[l].x, (T)s_gam[l].y, base, s_sec[l], slot, zero_at, ar, ai,
+                      s_x, win_lo, win_n);
   for (int64_t p = p0; p < p1; p += U) {
     const int nv = (int)(p1 - p < U ? p1 - p : U);
     DevEntryP en[U];
@@ -203,7 +211,7 @@ __device__ inline void light_body(const DevPacked& op, const T* __restrict__ x, 
     for (int u = 0; u < U; ++u) {
       if (u < nv) {
         en[u] = op.ent[p + u];
-        cf[u] = rec_staged ? s_rec[en[u].rec - op.light_rec_lo] : load_coef(op.rec, en[u].rec);
+        cf[u] = rec_staged ? s_rec[en[u].rec - op.light_rec_lo] : load_coef(cf_p.rec, en[u].rec);
       } else {
         en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
         cf[u] = CoefLR{};
@@ -226,8 +234,8 @@ __device__ void tier0_fixup_block(const DevPacked& op, const T* __restrict__ x, 
 // couplings per step; the slices are summed through LDS in a fixed order.
 // Block 0 (the tier-0 ADO) then applies the trace row and Hermitianises.
 template <typename T, bool TRANS>
-__device__ inline void heavy_body(const DevPacked& op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at,
-                                  int rdo, int bid, char* smem) {
+__device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
+                                  int64_t zero_at, int rdo, int bid, char* smem) {
   using V = typename P2<T>::type;
   constexpr int kStage = 256;
   T* red_r = reinterpret_cast<T*>(smem);
@@ -240,7 +248,7 @@ __device__ inline void heavy_body(const DevPacked& op, const T* __restrict__ x, 
   const int nnz = (int)(op.off[a + 1] - base);
   const V* __restrict__ xv = reinterpret_cast<const V*>(x);
   const int s = op.sec[a];
-  const T gr = (T)op.gam[2 * a], gi = (T)op.gam[2 * a + 1];
+  const T gr = (T)cf_p.gam[2 * a], gi = (T)cf_p.gam[2 * a + 1];
   const int64_t p0 = op.rowptr[a], p1 = op.rowptr[a + 1];
   const int nent = (int)(p1 - p0);
   const bool staged = nent <= kStage;
@@ -248,7 +256,7 @@ __device__ inline void heavy_body(const DevPacked& op, const T* __restrict__ x, 
     for (int k = threadIdx.x; k < nent; k += 256) {
       const DevEntryP e0 = op.ent[p0 + k];
       s_en[k] = e0;
-      s_cf[k] = load_coef(op.rec, e0.rec);
+      s_cf[k] = load_coef(cf_p.rec, e0.rec);
     }
   const int win_n = nnz < 256 ? nnz : 256;
   if (threadIdx.x < win_n) s_x[threadIdx.x] = ld<T, TRANS>(xv, base + threadIdx.x, zero_at);
@@ -261,7 +269,7 @@ __device__ inline void heavy_body(const DevPacked& op, const T* __restrict__ x, 
     const int t = t0 + lane;
     T ar = 0, ai = 0;
     if (slice < S) {
-      if (slice == 0) diag_term<T, TRANS>(op, xv, gr, gi, base, s, t, zero_at, ar, ai, s_x, base, win_n);
+      if (slice == 0) diag_term<T, TRANS>(op, cf_p.hc_ell, xv, gr, gi, base, s, t, zero_at, ar, ai, s_x, base, win_n);
       for (int k0 = slice; k0 < nent; k0 += S * U) {
         DevEntryP en[U];
         CoefLR cf[U];
@@ -271,7 +279,7 @@ __device__ inline void heavy_body(const DevPacked& op, const T* __restrict__ x, 
           const int k = k0 + u * S;
           if (k < nent) {
             en[u] = staged ? s_en[k] : op.ent[p0 + k];
-            cf[u] = staged ? s_cf[k] : load_coef(op.rec, en[u].rec);
+            cf[u] = staged ? s_cf[k] : load_coef(cf_p.rec, en[u].rec);
             nv = u + 1;
           } else {
             en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
@@ -313,10 +321,15 @@ template <typename T, bool TRANS, int U>
 __global__ void __launch_bounds__(256)
 k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
   __shared__ __align__(16) char smem[kApplySmem];
+  const int64_t p = blockIdx.y;   // problem of a batch
+  if (op.frozen_flag && op.frozen_flag[p * op.frozen_stride] == 3) return;   // finished: block-uniform, no barrier yet
+  x += 2 * p * op.vstride;
+  y += 2 * p * op.vstride;
+  const Coefs cf{op.rec + p * op.rec_stride, op.gam + 2 * p * op.gam_stride, op.hc_ell + 2 * p * op.hc_stride};
   if ((int64_t)blockIdx.x < op.n_heavy)   // block-uniform
-    heavy_body<T, TRANS>(op, x, y, zero_at, rdo, (int)blockIdx.x, smem);
+    heavy_body<T, TRANS>(op, cf, x, y, zero_at, rdo, (int)blockIdx.x, smem);
   else
-    light_body<T, TRANS, U>(op, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem);
+    light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem);
 }
 
 // Tier-0 fix-ups (one block): trace row (forward: y(d,d) = tr x0; transposed:
@@ -423,7 +436,7 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
   const int64_t light = op.total - op.light_begin;
   const int64_t blocks = op.n_heavy + (light > 0 ? (light + 255) / 256 : 0);
   if (blocks > 0) {
-    const dim3 grid((unsigned)blocks);
+    const dim3 grid((unsigned)blocks, (unsigned)op.nprob);
     const int u = op.max_light_couplings;
     const int r = rdo ? 1 : 0;
 #define HEOMQ_APPLY(UU)                                                                                  \
@@ -442,6 +455,7 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
     if (err != hipSuccess) return err;
   }
   if (op.n_heavy > 0) return hipSuccess;   // block 0 (the tier-0 ADO) did the tier-0 fix-up
+  if (op.nprob != 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_tier0_fixup<T>), dim3(1), dim3(256), 0, s, op, x, y, rdo ? 1 : 0);
   return hipGetLastError();
 }

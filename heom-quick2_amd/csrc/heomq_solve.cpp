@@ -561,6 +561,15 @@ int heomq_observables(heomq_ctx* c, const double* x, double* current, double* oc
   std::vector<double> h(2 * (size_t)nado * d * d);
   HIPCHECK(hipStreamSynchronize(st));
   HIPCHECK(hipMemcpy(h.data(), x, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  observables_host(c, h, current, occ);
+  return HEOMQ_OK;
+}
+
+}  // extern "C"
+
+// Currents and occupations from the dense tier-0 and tier-1 blocks (host copy).
+void observables_host(const heomq_ctx* c, const std::vector<double>& h, double* current, double* occ) {
+  const int d = c->d;
   using C = std::complex<double>;
   auto X = [&](int64_t a, int i, int j) { return C(h[2 * (a * d * d + i + j * d)], h[2 * (a * d * d + i + j * d) + 1]); };
   auto cv = [&](int orb, int s, int i, int j) { return c->ams[((size_t)(orb * c->nspin + s) * d + j) * d + i]; };
@@ -592,8 +601,9 @@ int heomq_observables(heomq_ctx* c, const double* x, double* current, double* oc
           }
         occ[o * c->nspin + s] = acc;
       }
-  return HEOMQ_OK;
 }
+
+extern "C" {
 
 static int host_apply(heomq_ctx* c, const double* xh, double* yh, int trace, bool trans) {
   const int64_t n = 2 * c->N * c->d * c->d;

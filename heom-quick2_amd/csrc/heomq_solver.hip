@@ -47,6 +47,11 @@ template <> struct W2<float> { using type = float2; };
 #define GRID_LOOP(i, n) \
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
+// Batched launches: blockIdx.y is the problem; its vectors start vs reals
+// after the previous problem's, its scalars and partial sums are its own.
+#define BATCH_PARTIAL() (partial += 2 * (int64_t)blockIdx.y * gridDim.x)
+#define BATCH_OFF() ((int64_t)blockIdx.y * vs)
+
 // Shadow vector of TFQMR, generated where it is used instead of stored: a
 // +-1 sign per real element from a splitmix64 hash of its index (the same
 // bits for any grid and any call).
@@ -61,8 +66,12 @@ __device__ inline double shadow(int64_t k) {
 // r = b - Ax (stored in w and y1), d = 0; partials of ||r||^2 and rt.r.
 template <typename T>
 __global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restrict__ ax, T* __restrict__ w,
-                         T* __restrict__ y1, T* __restrict__ d, double* partial) {
+                         T* __restrict__ y1, T* __restrict__ d, const TfqmrScal* s, double* partial, int64_t vs) {
   using V = typename W2<T>::type;
+  s += blockIdx.y;
+  BATCH_PARTIAL();
+  if (s->flag != kInitRequested) return;   // partials of this problem are not read
+  { const int64_t o = BATCH_OFF(); b += o; ax += o; w += o; y1 += o; d += o; }
   double rr = 0, rrt = 0;
   GRID_LOOP(i, n2) {
     const V bb = reinterpret_cast<const V*>(b)[i], aa = reinterpret_cast<const V*>(ax)[i];
@@ -83,12 +92,18 @@ __global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restric
 
 // v = u1; partial of rt.v
 template <typename T>
-__global__ void k_init_v(int64_t n2, const T* __restrict__ u1, T* __restrict__ v, double* partial) {
+__global__ void k_init_v(int64_t n2, const T* __restrict__ u1, T* __restrict__ v, T* __restrict__ u1_copy,
+                         const TfqmrScal* s, double* partial, int64_t vs) {
   using V = typename W2<T>::type;
+  s += blockIdx.y;
+  BATCH_PARTIAL();
+  if (s->flag != kInitialising) return;
+  { const int64_t o = BATCH_OFF(); u1 += o; v += o; if (u1_copy) u1_copy += o; }
   double sg = 0;
   GRID_LOOP(i, n2) {
     const V t = reinterpret_cast<const V*>(u1)[i];
     reinterpret_cast<V*>(v)[i] = t;
+    if (u1_copy) reinterpret_cast<V*>(u1_copy)[i] = t;
     sg += shadow(2 * i) * t.x + shadow(2 * i + 1) * t.y;
   }
   block_partial(sg, 0.0, partial);
@@ -99,9 +114,12 @@ __global__ void k_init_v(int64_t n2, const T* __restrict__ u1, T* __restrict__ v
 template <typename T>
 __global__ void k_step1(int64_t n2, const T* __restrict__ y1, const T* __restrict__ v, T* __restrict__ y2,
                         T* __restrict__ w, const T* __restrict__ u1, T* __restrict__ d, const TfqmrScal* s,
-                        double* partial) {
+                        double* partial, int64_t vs) {
   using V = typename W2<T>::type;
+  s += blockIdx.y;
+  BATCH_PARTIAL();
   if (s->flag) return;
+  { const int64_t o = BATCH_OFF(); y1 += o; v += o; y2 += o; w += o; u1 += o; d += o; }
   const T alpha = (T)s->alpha, dcoef = (T)s->dcoef;
   double ww = 0;
   GRID_LOOP(i, n2) {
@@ -129,9 +147,13 @@ __global__ void k_step1(int64_t n2, const T* __restrict__ y1, const T* __restric
 // ||w||^2 and rt.w (w is final for this iteration).
 template <typename T>
 __global__ void k_step2(int64_t n2, T* __restrict__ x, T* __restrict__ w, const T* __restrict__ u2,
-                        T* __restrict__ d, const T* __restrict__ y2, const TfqmrScal* s, double* partial) {
+                        T* __restrict__ d, const T* __restrict__ y2, const TfqmrScal* s, double* partial,
+                        int64_t vs) {
   using V = typename W2<T>::type;
+  s += blockIdx.y;
+  BATCH_PARTIAL();
   if (s->flag) return;
+  { const int64_t o = BATCH_OFF(); x += o; w += o; u2 += o; d += o; y2 += o; }
   const T alpha = (T)s->alpha, dcoef = (T)s->dcoef, eta = (T)s->eta;
   double ww = 0, rw = 0;
   GRID_LOOP(i, n2) {
@@ -159,9 +181,11 @@ __global__ void k_step2(int64_t n2, T* __restrict__ x, T* __restrict__ w, const 
 // x += eta d;  y1 = w + beta y2 (not on the iteration that met the bound).
 template <typename T>
 __global__ void k_step3(int64_t n2, T* __restrict__ x, const T* __restrict__ d, const T* __restrict__ w,
-                        const T* __restrict__ y2, T* __restrict__ y1, const TfqmrScal* s) {
+                        const T* __restrict__ y2, T* __restrict__ y1, const TfqmrScal* s, int64_t vs) {
   using V = typename W2<T>::type;
+  s += blockIdx.y;
   if (s->flag) return;
+  { const int64_t o = BATCH_OFF(); x += o; d += o; w += o; y2 += o; y1 += o; }
   const T eta = (T)s->eta, beta = (T)s->beta;
   const bool next = s->stop == 0;
   GRID_LOOP(i, n2) {
@@ -183,9 +207,12 @@ __global__ void k_step3(int64_t n2, T* __restrict__ x, const T* __restrict__ d, 
 // v = u1 + beta (u2 + beta v); partial of rt.v
 template <typename T>
 __global__ void k_v(int64_t n2, const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ v,
-                    const TfqmrScal* s, double* partial) {
+                    const TfqmrScal* s, double* partial, int64_t vs) {
   using V = typename W2<T>::type;
+  s += blockIdx.y;
+  BATCH_PARTIAL();
   if (s->flag || s->stop) return;
+  { const int64_t o = BATCH_OFF(); u1 += o; u2 += o; v += o; }
   const T beta = (T)s->beta;
   double sg = 0;
   GRID_LOOP(i, n2) {
@@ -202,7 +229,10 @@ __global__ void k_v(int64_t n2, const T* __restrict__ u1, const T* __restrict__ 
 
 // partials of ||b - ax||^2 and ||b||^2 (FP64)
 template <typename TB, typename TA>
-__global__ void k_resid(int64_t n2, const TB* __restrict__ b, const TA* __restrict__ ax, double* partial) {
+__global__ void k_resid(int64_t n2, const TB* __restrict__ b, const TA* __restrict__ ax, double* partial,
+                        int64_t vs) {
+  BATCH_PARTIAL();
+  { const int64_t o = BATCH_OFF(); b += o; ax += o; }
   double rr = 0, bb = 0;
   GRID_LOOP(i, n2) {
     const double b0 = (double)b[2 * i], b1 = (double)b[2 * i + 1];
@@ -227,7 +257,13 @@ __device__ inline void half_step(TfqmrScal* s, double a) {
   s->dcoef = theta * theta * s->eta / s->alpha;
 }
 
+// Block p finalizes problem p of a batch (one block for a single problem).
 __global__ void k_finalize(int op, int nblk, const double* __restrict__ partial, TfqmrScal* s, double* out) {
+  partial += 2 * (int64_t)blockIdx.x * nblk;
+  if (out) out += 2 * blockIdx.x;
+  if (s) s += blockIdx.x;
+  if (op == F_INIT_R && s->flag != kInitRequested) return;
+  if (op == F_INIT_V && s->flag != kInitialising) return;
   if (op != F_INIT_R && op != F_INIT_V && op != F_RESID && s->flag) return;   // s unused for F_RESID
   double a = 0, b = 0;
   for (int t = threadIdx.x; t < nblk; t += blockDim.x) {
@@ -244,13 +280,15 @@ __global__ void k_finalize(int op, int nblk, const double* __restrict__ partial,
       s->theta = 0;
       s->eta = 0;
       s->dcoef = 0;
-      s->flag = (b == 0.0 && a > 0.0) ? 2 : 0;
+      s->stop = 0;
+      s->flag = (b == 0.0 && a > 0.0) ? 2 : kInitialising;
       break;
     case F_INIT_V:
       s->sigma = a;
       if (a == 0.0) { s->flag = 2; break; }
       s->alpha = s->rho / a;
       s->dcoef = 0;
+      s->flag = 0;
       break;
     case F_HALF:
       half_step(s, a);
@@ -354,15 +392,33 @@ __global__ void k_hermitize0(int d, T* __restrict__ x) {
 }
 
 // Scalar state reset on the device (no host staging of pageable memory).
-__global__ void k_tfq_reset(TfqmrScal* s, double target) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  s->rho = s->sigma = s->alpha = s->beta = 0;
-  s->theta = s->tau = s->eta = s->dcoef = 0;
-  s->rr = 0;
-  s->target = target;
-  s->iter = 0;
-  s->flag = 0;
-  s->stop = 0;
+// Problems p < nprob with mask[p] != 0 (every problem if mask is null) are set
+// up for a (re)start: scalars cleared, flag = kInitRequested.  Problems left
+// out keep their state (a frozen problem stays frozen).
+__global__ void k_tfq_reset(TfqmrScal* s, double target, const int32_t* mask, int nprob) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob || (mask && !mask[p])) return;
+  TfqmrScal* q = s + p;
+  q->rho = q->sigma = q->alpha = q->beta = 0;
+  q->theta = q->tau = q->eta = q->dcoef = 0;
+  q->rr = 0;
+  q->target = target;
+  q->iter = 0;
+  q->stop = 0;
+  q->flag = kInitRequested;
+}
+
+// flag[p] = kFrozen where mask[p]: finished problems drop out of every kernel.
+__global__ void k_tfq_freeze(TfqmrScal* s, const int32_t* mask, int nprob) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nprob && mask[p]) s[p].flag = kFrozen;
+}
+
+// v[p * stride + idx[k]] = val[k] for every problem p and k < m.
+__global__ void k_set_pattern(double* v, int64_t stride, int nprob, SetPattern pat) {
+  const int p = blockIdx.x;
+  if (p >= nprob) return;
+  for (int k = threadIdx.x; k < pat.m; k += blockDim.x) v[p * stride + pat.idx[k]] = pat.val[k];
 }
 
 template <typename T>
@@ -382,57 +438,62 @@ static int grid_for(int64_t n) {
 }
 
 #define L1(kern, n, ...) hipLaunchKernelGGL(kern, dim3(grid_for<T>(n)), dim3(kRedThreads), 0, st, n, __VA_ARGS__)
+#define LB(kern, n, ...) \
+  hipLaunchKernelGGL(kern, dim3(grid_for<T>(n), bt.nprob), dim3(kRedThreads), 0, st, n, __VA_ARGS__)
 
-static hipError_t finalize(hipStream_t st, int op, int nblk, const double* partial, TfqmrScal* s, double* out) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kRedThreads), 0, st, op, nblk, partial, s, out);
+static hipError_t finalize(hipStream_t st, int op, int nblk, const double* partial, TfqmrScal* s, double* out,
+                           int nprob = 1) {
+  hipLaunchKernelGGL(k_finalize, dim3(nprob), dim3(kRedThreads), 0, st, op, nblk, partial, s, out);
   return hipGetLastError();
 }
 
-// n counts reals; the vector kernels walk n/2 complex pairs
+// n counts reals; the vector kernels walk n/2 complex pairs of each problem
 template <typename T>
 hipError_t tfq_init_r(hipStream_t st, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, TfqmrScal* s,
-                      double* partial) {
+                      double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  L1((k_init_r<T>), n2, b, ax, w, y1, d, partial);
+  LB((k_init_r<T>), n2, b, ax, w, y1, d, s, partial, bt.vs);
   if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_INIT_R, grid_for<T>(n2), partial, s, nullptr);
+  return finalize(st, F_INIT_R, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
 template <typename T>
-hipError_t tfq_init_v(hipStream_t st, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial) {
+hipError_t tfq_init_v(hipStream_t st, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial, Batch bt,
+                      T* u1_copy) {
   const int64_t n2 = n / 2;
-  L1((k_init_v<T>), n2, u1, v, partial);
+  LB((k_init_v<T>), n2, u1, v, u1_copy, s, partial, bt.vs);
   if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_INIT_V, grid_for<T>(n2), partial, s, nullptr);
+  return finalize(st, F_INIT_V, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
 template <typename T>
 hipError_t tfq_step1(hipStream_t st, int64_t n, const T* y1, const T* v, T* y2, T* w, const T* u1, T* d,
-                     TfqmrScal* s, double* partial) {
+                     TfqmrScal* s, double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  L1((k_step1<T>), n2, y1, v, y2, w, u1, d, s, partial);
+  LB((k_step1<T>), n2, y1, v, y2, w, u1, d, s, partial, bt.vs);
   if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_HALF, grid_for<T>(n2), partial, s, nullptr);
+  return finalize(st, F_HALF, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
 template <typename T>
 hipError_t tfq_step2(hipStream_t st, int64_t n, T* x, T* w, const T* u2, T* d, const T* y2, TfqmrScal* s,
-                     double* partial) {
+                     double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  L1((k_step2<T>), n2, x, w, u2, d, y2, s, partial);
+  LB((k_step2<T>), n2, x, w, u2, d, y2, s, partial, bt.vs);
   if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_HALF_RHO, grid_for<T>(n2), partial, s, nullptr);
+  return finalize(st, F_HALF_RHO, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
 template <typename T>
 hipError_t tfq_step3(hipStream_t st, int64_t n, T* x, const T* d, const T* w, const T* y2, T* y1,
-                     const TfqmrScal* s) {
+                     const TfqmrScal* s, Batch bt) {
   const int64_t n2 = n / 2;
-  L1((k_step3<T>), n2, x, d, w, y2, y1, s);
+  LB((k_step3<T>), n2, x, d, w, y2, y1, s, bt.vs);
   return hipGetLastError();
 }
 template <typename T>
-hipError_t tfq_v(hipStream_t st, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial) {
+hipError_t tfq_v(hipStream_t st, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial,
+                 Batch bt) {
   const int64_t n2 = n / 2;
-  L1((k_v<T>), n2, u1, u2, v, s, partial);
+  LB((k_v<T>), n2, u1, u2, v, s, partial, bt.vs);
   if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_V, grid_for<T>(n2), partial, s, nullptr);
+  return finalize(st, F_V, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
 template <typename T>
 hipError_t vec_fill(hipStream_t st, int64_t n, T* v, T val) {
@@ -445,12 +506,12 @@ hipError_t vec_randn(hipStream_t st, int64_t n, T* v, uint64_t seed) {
   return hipGetLastError();
 }
 template <typename TB, typename TA>
-hipError_t vec_resid(hipStream_t st, int64_t n, const TB* b, const TA* ax, double* out, double* partial) {
+hipError_t vec_resid(hipStream_t st, int64_t n, const TB* b, const TA* ax, double* out, double* partial, Batch bt) {
   using T = TB;
   const int64_t n2 = n / 2;
-  L1((k_resid<TB, TA>), n2, b, ax, partial);
+  LB((k_resid<TB, TA>), n2, b, ax, partial, bt.vs);
   if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_RESID, grid_for<T>(n2), partial, nullptr, out);
+  return finalize(st, F_RESID, grid_for<T>(n2), partial, nullptr, out, bt.nprob);
 }
 hipError_t vec_resid_to_f32(hipStream_t st, int64_t n, const double* b, const double* ax, float* r32, double scale) {
   using T = double;
@@ -472,8 +533,16 @@ hipError_t vec_project(hipStream_t st, int64_t nado, int d, T* x, const int8_t* 
 template hipError_t vec_project<double>(hipStream_t, int64_t, int, double*, const int8_t*, const TierBounds&, bool);
 template hipError_t vec_project<float>(hipStream_t, int64_t, int, float*, const int8_t*, const TierBounds&, bool);
 
-hipError_t tfq_reset(hipStream_t st, TfqmrScal* s, double target) {
-  hipLaunchKernelGGL(k_tfq_reset, dim3(1), dim3(64), 0, st, s, target);
+hipError_t tfq_reset(hipStream_t st, TfqmrScal* s, double target, const int32_t* mask, int nprob) {
+  hipLaunchKernelGGL(k_tfq_reset, dim3((nprob + 63) / 64), dim3(64), 0, st, s, target, mask, nprob);
+  return hipGetLastError();
+}
+hipError_t tfq_freeze(hipStream_t st, TfqmrScal* s, const int32_t* mask, int nprob) {
+  hipLaunchKernelGGL(k_tfq_freeze, dim3((nprob + 63) / 64), dim3(64), 0, st, s, mask, nprob);
+  return hipGetLastError();
+}
+hipError_t vec_set_pattern(hipStream_t st, double* v, int64_t stride, int nprob, const SetPattern& pat) {
+  hipLaunchKernelGGL(k_set_pattern, dim3(nprob), dim3(64), 0, st, v, stride, nprob, pat);
   return hipGetLastError();
 }
 hipError_t vec_set_elem(hipStream_t st, double* v, int64_t off, double val) {
@@ -490,22 +559,23 @@ hipError_t vec_lincomb3(hipStream_t st, int64_t n, double* out, double a, const 
 
 #define INST(T)                                                                                          \
   template hipError_t tfq_init_r<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, T*, TfqmrScal*,     \
-                                    double*);                                                           \
-  template hipError_t tfq_init_v<T>(hipStream_t, int64_t, const T*, T*, TfqmrScal*, double*);            \
+                                    double*, Batch);                                                    \
+  template hipError_t tfq_init_v<T>(hipStream_t, int64_t, const T*, T*, TfqmrScal*, double*, Batch, T*); \
   template hipError_t tfq_step1<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, const T*, T*,       \
-                                   TfqmrScal*, double*);                                                \
+                                   TfqmrScal*, double*, Batch);                                         \
   template hipError_t tfq_step2<T>(hipStream_t, int64_t, T*, T*, const T*, T*, const T*, TfqmrScal*,     \
-                                   double*);                                                            \
+                                   double*, Batch);                                                     \
   template hipError_t tfq_step3<T>(hipStream_t, int64_t, T*, const T*, const T*, const T*, T*,           \
-                                   const TfqmrScal*);                                                   \
-  template hipError_t tfq_v<T>(hipStream_t, int64_t, const T*, const T*, T*, TfqmrScal*, double*);       \
+                                   const TfqmrScal*, Batch);                                            \
+  template hipError_t tfq_v<T>(hipStream_t, int64_t, const T*, const T*, T*, TfqmrScal*, double*, Batch); \
   template hipError_t vec_fill<T>(hipStream_t, int64_t, T*, T);                                           \
   template hipError_t vec_randn<T>(hipStream_t, int64_t, T*, uint64_t);
 INST(double)
 INST(float)
 template hipError_t vec_resid<double, double>(hipStream_t, int64_t, const double*, const double*, double*,
-                                              double*);
-template hipError_t vec_resid<float, float>(hipStream_t, int64_t, const float*, const float*, double*, double*);
+                                              double*, Batch);
+template hipError_t vec_resid<float, float>(hipStream_t, int64_t, const float*, const float*, double*, double*,
+                                            Batch);
 
 // ---------------------------------------------------------------------------
 // BiCG (bicg/solvebicg.f90): real inner products of the (Re, Im) vectors

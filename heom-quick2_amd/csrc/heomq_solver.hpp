@@ -11,6 +11,17 @@ namespace heomq {
 
 constexpr int kMaxRedBlocks = 1024;
 
+// Device flag values of a TFQMR problem.  0 running, 1 quasi-residual bound
+// met, 2 breakdown; a (re)start goes kInitRequested -> kInitialising -> 0;
+// kFrozen problems (finished members of a batch) are skipped by every kernel.
+constexpr int32_t kInitRequested = 4, kInitialising = 5, kFrozen = 3;
+
+// A batch of problems of one shape: nprob copies of every vector, vs reals apart.
+struct Batch {
+  int nprob = 1;
+  int64_t vs = 0;
+};
+
 // TFQMR scalars, resident on the device (Freund 1993 notation).
 struct TfqmrScal {
   double rho, sigma, alpha, beta;
@@ -49,30 +60,41 @@ hipError_t vec_project(hipStream_t, int64_t nado, int d, T* x, const int8_t* par
 // finalize; the shadow vector is generated in the kernels.
 template <typename T>
 hipError_t tfq_init_r(hipStream_t, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, TfqmrScal* s,
-                      double* partial);
+                      double* partial, Batch bt = Batch{});
 template <typename T>
-hipError_t tfq_init_v(hipStream_t, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial);
+hipError_t tfq_init_v(hipStream_t, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial,
+                      Batch bt = Batch{}, T* u1_copy = nullptr);
 template <typename T>
 hipError_t tfq_step1(hipStream_t, int64_t n, const T* y1, const T* v, T* y2, T* w, const T* u1, T* d,
-                     TfqmrScal* s, double* partial);
+                     TfqmrScal* s, double* partial, Batch bt = Batch{});
 template <typename T>
 hipError_t tfq_step2(hipStream_t, int64_t n, T* x, T* w, const T* u2, T* d, const T* y2, TfqmrScal* s,
-                     double* partial);
+                     double* partial, Batch bt = Batch{});
 template <typename T>
 hipError_t tfq_step3(hipStream_t, int64_t n, T* x, const T* d, const T* w, const T* y2, T* y1,
-                     const TfqmrScal* s);
+                     const TfqmrScal* s, Batch bt = Batch{});
 template <typename T>
-hipError_t tfq_v(hipStream_t, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial);
+hipError_t tfq_v(hipStream_t, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial,
+                 Batch bt = Batch{});
 template <typename T>
 hipError_t vec_fill(hipStream_t, int64_t n, T* v, T val);
 template <typename T>
 hipError_t vec_randn(hipStream_t, int64_t n, T* v, uint64_t seed);
 template <typename TB, typename TA>
-hipError_t vec_resid(hipStream_t, int64_t n, const TB* b, const TA* ax, double* out, double* partial);
+hipError_t vec_resid(hipStream_t, int64_t n, const TB* b, const TA* ax, double* out, double* partial,
+                     Batch bt = Batch{});
 hipError_t vec_resid_to_f32(hipStream_t, int64_t n, const double* b, const double* ax, float* r32, double scale);
 hipError_t vec_axpy_f32_to_f64(hipStream_t, int64_t n, double* x, const float* dx, double alpha);
-hipError_t tfq_reset(hipStream_t, TfqmrScal* s, double target);
+hipError_t tfq_reset(hipStream_t, TfqmrScal* s, double target, const int32_t* mask = nullptr, int nprob = 1);
+hipError_t tfq_freeze(hipStream_t, TfqmrScal* s, const int32_t* mask, int nprob);
 hipError_t vec_set_elem(hipStream_t, double* v, int64_t off, double val);
+// A few scattered values written into every problem's copy of a vector.
+struct SetPattern {
+  int m = 0;
+  int64_t idx[64];
+  double val[64];
+};
+hipError_t vec_set_pattern(hipStream_t, double* v, int64_t stride, int nprob, const SetPattern& pat);
 hipError_t vec_lincomb3(hipStream_t, int64_t n, double* out, double a, const double* x, double b,
                         const double* y, double c, const double* z);
 

@@ -77,17 +77,84 @@ def two_impurity(eps=-1.5, U=3.0, t=0.5, u12=0.0, gamma=0.2, W=10.0, T=0.1, V=0.
                   hpar=(eps, eps, eps, eps, U, U, u12, t), lsdraw=lsdraw)
 
 
+class Batch:
+    """Many problems of one hierarchy shape solved together on one GPU.
+
+    Problems must share norbs, nalf, nmats (Pade terms), ntop, lsdraw and the
+    Hamiltonian kind; energies, Gamma, temperature and bias may differ.
+    """
+
+    def __init__(self, params_list, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("heomq.Batch needs a GPU (HIP device); none is visible")
+        self.torch = torch
+        self.params = list(params_list)
+        arr = (_lib.CParams * len(self.params))(*[p.to_c() for p in self.params])
+        self._h = C.c_void_p()
+        torch.cuda.set_device(device)
+        check(lib().heomq_batch_create(arr, len(self.params), device, C.byref(self._h)))
+        info = np.zeros(8, np.int64)
+        check(lib().heomq_batch_info(self._h, info.ctypes.data_as(_lib.I64)))
+        self.nprob, self.N, self.packed, self.d = (int(v) for v in info[:4])
+        self.device = device
+        _LIVE_BATCH.add(self)
+
+    def close(self):
+        if self._h:
+            self.torch.cuda.synchronize(self.device)
+            lib().heomq_batch_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply_packed(self, xp, yp, trace=TRACE_RDO):
+        """y_p = Pi A_p x_p for every problem; xp, yp: (nprob, packed) complex128 device tensors."""
+        t = self.torch
+        for v in (xp, yp):
+            if v.dtype != t.complex128 or not v.is_cuda or not v.is_contiguous() or v.numel() != self.nprob * self.packed:
+                raise HeomqError(_lib.BAD_DIM, "expected (nprob, packed) complex128 device tensors")
+        check(lib().heomq_batch_apply_packed(self._h, C.c_void_p(xp.data_ptr()), C.c_void_p(yp.data_ptr()), trace,
+                                             C.c_void_p(t.cuda.current_stream(self.device).cuda_stream)))
+        return yp
+
+    def solve(self, tol=1e-10, maxiter=20000, raise_on_fail=True):
+        """Steady states of all problems: (rdo (P, d, d) math-indexed, current (P, nalf, 2), occ (P, norbs, 2), stats)."""
+        P, d = self.nprob, self.d
+        nalf = self.params[0].nalf
+        norbs = self.params[0].norbs
+        rdo = np.zeros((P, d, d), np.complex128)
+        cur = np.zeros((P, nalf, 2))
+        occ = np.zeros((P, norbs, 2))
+        stats = np.zeros((P, 4))
+        rc = lib().heomq_batch_solve(self._h, tol, maxiter, _dp(rdo.view(np.float64)), _dp(cur), _dp(occ), _dp(stats),
+                                     C.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream))
+        if rc != _lib.OK and raise_on_fail:
+            check(rc)
+        rdo = np.ascontiguousarray(rdo.transpose(0, 2, 1))   # column-major blocks -> [p, i, j]
+        info = dict(iters=stats[:, 0].astype(int), relres=stats[:, 1], status=stats[:, 2].astype(int),
+                    restarts=stats[:, 3].astype(int), rc=rc)
+        return rdo, cur, occ, info
+
+
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(_lib.D)
 
 
 _LIVE_CTX: "weakref.WeakSet" = weakref.WeakSet()
 _LIVE_MODELS: "weakref.WeakSet" = weakref.WeakSet()
+_LIVE_BATCH: "weakref.WeakSet" = weakref.WeakSet()
 
 
 @atexit.register
 def _release_all():
-    """Free device contexts (then models) while the HIP runtime is still up."""
+    """Free batches and device contexts (then models) while the HIP runtime is still up."""
+    for b in list(_LIVE_BATCH):
+        b.close()
     for c in list(_LIVE_CTX):
         c.close()
     for m in list(_LIVE_MODELS):

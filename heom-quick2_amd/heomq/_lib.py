@@ -81,6 +81,11 @@ SIGNATURES = {
     "heomq_solve_steady": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, D, P]),
     "heomq_solve_steady_mixed": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_int, C.c_int, D, P]),
     "heomq_solve_steady_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),
+    "heomq_batch_create": (C.c_int, [C.POINTER(CParams), C.c_int, C.c_int, C.POINTER(P)]),
+    "heomq_batch_free": (None, [P]),
+    "heomq_batch_info": (C.c_int, [P, I64]),
+    "heomq_batch_apply_packed": (C.c_int, [P, P, P, C.c_int, P]),
+    "heomq_batch_solve": (C.c_int, [P, C.c_double, C.c_int, D, D, D, D, P]),
     "heomq_rk4": (C.c_int, [P, P, C.c_double, C.c_int, P]),
     "heomq_chebyshev": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_observables": (C.c_int, [P, P, D, D, P]),

@@ -237,6 +237,31 @@ int heomq_solve_steady_host(heomq_ctx* c, double* rho_host, double tol, int maxi
                             double* stats);
 
 /* ---------------------------------------------------------------------------
+ * Batched steady states.  The reference solves one parameter point per run
+ * (a shell loop over generated inputs, tools/make_input_didv/makeinput.f90);
+ * here nprob problems of one hierarchy shape (same norbs, nalf, nmats, ntop,
+ * lsdraw, Hamiltonian kind; energies, Gamma, T and bias free) share every
+ * structural table and are solved together, one launch per solver step for
+ * the whole batch.  No counterpart in the reference.
+ * ------------------------------------------------------------------------- */
+typedef struct heomq_batch heomq_batch;
+int heomq_batch_create(const heomq_params* params, int nprob, int device, heomq_batch** out);
+void heomq_batch_free(heomq_batch* b);
+/* info[0] = problems, [1] = ADOs per problem, [2] = packed elements per
+ * problem, [3] = d, [4] = commutator row width. */
+int heomq_batch_info(const heomq_batch* b, int64_t* info);
+/* y_p = Pi A_p x_p for every problem at once; problem p's packed vector starts
+ * at complex element p * info[2]. */
+int heomq_batch_apply_packed(heomq_batch* b, const double* xp, double* yp, int trace, void* stream);
+/* TFQMR steady state of every problem (tier-0 trace row, projected operator,
+ * trace-one start), to ||b - A x|| <= tol ||b||.  Host outputs: rdo (nprob *
+ * d * d complex, column-major tier-0 blocks), current (nprob * nalf * nspin,
+ * td/calcj.f90 units), occ (nprob * norbs * nspin), stats (nprob * 4:
+ * iterations, relative residual, status, restarts).  Any output may be NULL. */
+int heomq_batch_solve(heomq_batch* b, double tol, int maxiter, double* rdo, double* current, double* occ,
+                      double* stats, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Propagation (td/intstp.f90 RK4 and td/chebyshev.f90 Chebyshev), fixed H.
  * x (device) is advanced in place by nstep steps of dt.
  * ------------------------------------------------------------------------- */
