@@ -36,6 +36,7 @@ struct heomq_batch {
   hipStream_t stream = nullptr;
   hipGraphExec_t chunk = nullptr;
   void* pinned = nullptr;
+  bool precond = false;           // Jacobi scaling of the iteration (every H diagonal)
   ~heomq_batch() {
     if (chunk) (void)hipGraphExecDestroy(chunk);
     if (stream) (void)hipStreamDestroy(stream);
@@ -152,13 +153,19 @@ int heomq_batch_create(const heomq_params* params, int nprob, int device, heomq_
   B->bf.hc_stride = (int64_t)c->nsec * c->d * c->d * width;
   B->bf.nprob = nprob;
   B->n = 2 * c->ptotal;
-  // workspace: x, b, w, y1, y2, u1, u2, v, d, ax for every problem
-  HIPCHECK(B->work.alloc(sizeof(double) * (size_t)B->n * nprob * 10));
+  // workspace: x, b, w, y1, y2, u1, u2, v, d, ax, xbest for every problem
+  HIPCHECK(B->work.alloc(sizeof(double) * (size_t)B->n * nprob * 11));
+  bool diag = true;
+  for (const auto& H : Hs)
+    for (int i = 0; i < c->d; ++i)
+      for (int k = 0; k < c->d; ++k)
+        if (i != k && H[i + (size_t)k * c->d] != cplx(0, 0)) diag = false;
+  B->precond = diag;
   HIPCHECK(B->scal.alloc(sizeof(TfqmrScal) * (size_t)nprob));
   HIPCHECK(B->partial.alloc(sizeof(double) * 2 * kMaxRedBlocks * (size_t)nprob));
   HIPCHECK(B->out.alloc(sizeof(double) * 2 * (size_t)nprob));
-  HIPCHECK(B->mask.alloc(sizeof(int32_t) * (size_t)nprob));
-  HIPCHECK(hipHostMalloc(&B->pinned, (sizeof(TfqmrScal) + 2 * sizeof(double) + sizeof(int32_t)) * (size_t)nprob + 256,
+  HIPCHECK(B->mask.alloc(2 * sizeof(int32_t) * (size_t)nprob));
+  HIPCHECK(hipHostMalloc(&B->pinned, (sizeof(TfqmrScal) + 2 * sizeof(double) + 2 * sizeof(int32_t)) * (size_t)nprob + 256,
                          hipHostMallocDefault));
   HIPCHECK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
   B->bf.frozen_flag = reinterpret_cast<const int32_t*>(B->scal.as<char>() + offsetof(TfqmrScal, flag));
@@ -204,7 +211,7 @@ int heomq_batch_solve(heomq_batch* B, double tol, int maxiter, double* rdo, doub
   double* base = B->work.as<double>();
   double *x = base, *b = base + n * P, *w = base + 2 * n * P, *y1 = base + 3 * n * P, *y2 = base + 4 * n * P,
          *u1 = base + 5 * n * P, *u2 = base + 6 * n * P, *v = base + 7 * n * P, *dv = base + 8 * n * P,
-         *ax = base + 9 * n * P;
+         *ax = base + 9 * n * P, *xbest = base + 10 * n * P;
   TfqmrScal* s = B->scal.as<TfqmrScal>();
   double* part = B->partial.as<double>();
   double* outv = B->out.as<double>();
@@ -212,8 +219,13 @@ int heomq_batch_solve(heomq_batch* B, double tol, int maxiter, double* rdo, doub
   TfqmrScal* hs = static_cast<TfqmrScal*>(B->pinned);
   double* hout = reinterpret_cast<double*>(hs + P);
   int32_t* hmask = reinterpret_cast<int32_t*>(hout + 2 * P);
-  const DevPacked& op = B->bf;
-  auto A = [&](const double* in, double* o) { return launch_apply_packed_f64(op, in, o, true, st); };
+  int32_t* hmask2 = hmask + P;
+  int32_t* mask2 = mask + P;
+  DevPacked op_it = B->bf, op0 = B->bf;
+  op_it.precond = B->precond ? 1 : 0;   // b = e_trace sits on the unscaled trace row
+  op0.precond = 0;
+  auto Ait = [&](const double* in, double* o) { return launch_apply_packed_f64(op_it, in, o, true, st); };
+  auto A0 = [&](const double* in, double* o) { return launch_apply_packed_f64(op0, in, o, true, st); };
 
   // x0: trace-one identity / d on tier 0 (a Hermitian physical start); b = e_trace
   HIPCHECK(vec_fill<double>(st, n * P, x, 0.0));
@@ -224,31 +236,38 @@ int heomq_batch_solve(heomq_batch* B, double tol, int maxiter, double* rdo, doub
     if (q >= 0 && px.m < 64) { px.idx[px.m] = 2 * q; px.val[px.m] = 1.0 / d; px.m++; }
   }
   pb.m = 1;
-  pb.idx[0] = 2 * (int64_t)op.trace_slot;
+  pb.idx[0] = 2 * (int64_t)op_it.trace_slot;
   pb.val[0] = 1.0;
   HIPCHECK(vec_set_pattern(st, x, n, P, px));
   HIPCHECK(vec_set_pattern(st, b, n, P, pb));
+  HIPCHECK(hipMemcpyAsync(xbest, x, sizeof(double) * n * P, hipMemcpyDeviceToDevice, st));
 
   auto init = [&](const int32_t* m) -> int {   // (re)start the masked problems from their current x
-    HIPCHECK(tfq_reset(st, s, tol, m, P));   // ||b|| = 1: the stop target is tol
-    HIPCHECK(A(x, ax));
+    HIPCHECK(tfq_reset(st, s, tol, m, P, m == nullptr));   // ||b|| = 1: the stop target is tol
+    if (B->precond) {   // true ||r||^2 of every problem, for the scaled-target ratio
+      HIPCHECK(A0(x, ax));
+      HIPCHECK((vec_resid<double, double>(st, n, b, ax, outv, part, bt)));
+    }
+    HIPCHECK(Ait(x, ax));
     HIPCHECK(tfq_init_r<double>(st, n, b, ax, w, y1, dv, s, part, bt));
-    HIPCHECK(A(y1, ax));
+    if (B->precond) HIPCHECK(tfq_scale_targets(st, s, outv, P));
+    HIPCHECK(Ait(y1, ax));
     HIPCHECK(tfq_init_v<double>(st, n, ax, v, s, part, bt, u1));
     return HEOMQ_OK;
   };
+  HIPCHECK(tfq_set_unscaled(st, s, nullptr, P));   // every problem starts with the scaled iteration
   if (int rc = init(nullptr)) return rc;
 
-  const int check_every = 8;
+  const int check_every = 8, verify_every = 32;
   if (!B->chunk) {
     HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t rec = hipSuccess;
     for (int k = 0; k < check_every && rec == hipSuccess; ++k) {
       if ((rec = tfq_step1<double>(st, n, y1, v, y2, w, u1, dv, s, part, bt))) break;
-      if ((rec = A(y2, u2))) break;
+      if ((rec = Ait(y2, u2))) break;
       if ((rec = tfq_step2<double>(st, n, x, w, u2, dv, y2, s, part, bt))) break;
       if ((rec = tfq_step3<double>(st, n, x, dv, w, y2, y1, s, bt))) break;
-      if ((rec = A(y1, u1))) break;
+      if ((rec = Ait(y1, u1))) break;
       rec = tfq_v<double>(st, n, u1, u2, v, s, part, bt);
     }
     hipGraph_t g = nullptr;
@@ -263,46 +282,75 @@ int heomq_batch_solve(heomq_batch* B, double tol, int maxiter, double* rdo, doub
   }
 
   const auto t0 = std::chrono::steady_clock::now();
-  std::vector<int> status(P, 0), restarts(P, 0), total(P, 0);
+  std::vector<int> status(P, 0), restarts(P, 0), total(P, 0), stalls(P, 0), unscaled(P, 0);
   std::vector<double> best(P, std::numeric_limits<double>::infinity()), relres(P, 0.0);
   int active = P;
-  while (active > 0) {
+  for (int replay = 1; active > 0; ++replay) {
     HIPCHECK(hipGraphLaunch(B->chunk, st));
     HIPCHECK(hipMemcpyAsync(hs, s, sizeof(TfqmrScal) * P, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    bool check = false;
+    const bool verify = replay % verify_every == 0;
+    bool check = verify;
     for (int p = 0; p < P; ++p)
       if (status[p] == 0 && (hs[p].flag == 1 || hs[p].flag == 2 || total[p] + hs[p].iter >= maxiter)) check = true;
     if (!check) continue;
-    // true residuals of the stopped problems (every unfrozen problem is computed)
-    HIPCHECK(A(x, ax));
+    // true residuals of every unfrozen problem (unscaled operator)
+    HIPCHECK(A0(x, ax));
     HIPCHECK((vec_resid<double, double>(st, n, b, ax, outv, part, bt)));
     HIPCHECK(hipMemcpyAsync(hout, outv, sizeof(double) * 2 * P, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    bool any_freeze = false, any_restart = false;
     std::vector<int32_t> fr(P, 0), rs(P, 0);
+    bool any_freeze = false, any_restart = false;
     for (int p = 0; p < P; ++p) {
       if (status[p] != 0) continue;
       const bool stopped = hs[p].flag == 1 || hs[p].flag == 2;
       const bool out_of_iters = total[p] + hs[p].iter >= maxiter;
-      if (!stopped && !out_of_iters) continue;
       const double rr = hout[2 * p], bb = hout[2 * p + 1];
-      relres[p] = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-      total[p] += hs[p].iter;
+      const double rel = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+      bool revert = false, restart = false, freeze = false;
+      const double prev = best[p];
       if (rr <= tol * tol * bb) {
-        status[p] = 1;   // converged
-      } else if (!out_of_iters && restarts[p] < 3 && rr < best[p]) {
-        best[p] = rr;
+        status[p] = 1;
+        relres[p] = rel;
+        freeze = true;
+      } else {
+        const bool improved = rr < prev;
+        if (improved) {
+          best[p] = rr;
+          relres[p] = rel;
+          HIPCHECK(hipMemcpyAsync(xbest + n * p, x + n * p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        }
+        revert = rr > 100.0 * prev;   // the recurrences diverged: back to the best iterate
+        restart = stopped || revert;
+        bool give_up = (restart && !improved && ++stalls[p] > 3) || out_of_iters;
+        if (give_up && !out_of_iters && B->precond && !unscaled[p]) {
+          unscaled[p] = 1;   // the scaled iteration stalled: continue unscaled from the best iterate
+          stalls[p] = 0;
+          revert = rr > best[p];
+          restart = true;
+          give_up = false;
+        }
+        if (give_up) {
+          status[p] = hs[p].flag == 2 ? 2 : 3;
+          revert = rr > best[p];
+          restart = false;
+          freeze = true;
+        }
+      }
+      if (revert) {
+        HIPCHECK(hipMemcpyAsync(x + n * p, xbest + n * p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        relres[p] = std::sqrt(best[p] / (bb > 0 ? bb : 1.0));
+      }
+      if (stopped || restart || freeze) total[p] += hs[p].iter;
+      if (freeze) {
+        fr[p] = 1;
+        any_freeze = true;
+        active--;
+      } else if (restart) {
         restarts[p]++;
         rs[p] = 1;
         any_restart = true;
-        continue;
-      } else {
-        status[p] = hs[p].flag == 2 ? 2 : 3;   // breakdown / not converged
       }
-      fr[p] = 1;
-      any_freeze = true;
-      active--;
     }
     if (any_freeze) {
       std::memcpy(hmask, fr.data(), sizeof(int32_t) * P);
@@ -311,8 +359,13 @@ int heomq_batch_solve(heomq_batch* B, double tol, int maxiter, double* rdo, doub
       HIPCHECK(hipStreamSynchronize(st));
     }
     if (any_restart) {
-      for (int p = 0; p < P; ++p)
-        if (rs[p]) HIPCHECK(launch_herm_packed<double>(c->pfwd, x + n * p, st));
+      for (int p = 0; p < P; ++p) {
+        if (!rs[p]) continue;
+        HIPCHECK(launch_herm_packed<double>(c->pfwd, x + n * p, st));
+      }
+      for (int p = 0; p < P; ++p) hmask2[p] = unscaled[p];   // the apply reads this word of each problem's scalars
+      HIPCHECK(hipMemcpyAsync(mask2, hmask2, sizeof(int32_t) * P, hipMemcpyHostToDevice, st));
+      HIPCHECK(tfq_set_unscaled(st, s, mask2, P));
       std::memcpy(hmask, rs.data(), sizeof(int32_t) * P);
       HIPCHECK(hipMemcpyAsync(mask, hmask, sizeof(int32_t) * P, hipMemcpyHostToDevice, st));
       if (int rc = init(mask)) return rc;

@@ -196,6 +196,10 @@ void fill_op(heomq_ctx* c, OperatorTables& t, const DevBuf& rec, bool transposed
 // conservation the sector layout was built for.
 int build_packed_h(heomq_ctx* c, const Model& m) {
   const int d = m.d, dd = d * d, S = c->nsec;
+  c->h_diagonal = true;
+  for (int i = 0; i < d; ++i)
+    for (int k = 0; k < d; ++k)
+      if (i != k && m.H[i + (size_t)k * d] != cplx(0, 0)) c->h_diagonal = false;
   for (int i = 0; i < d; ++i)
     for (int k = 0; k < d; ++k)
       if (m.H[i + (size_t)k * d] != cplx(0, 0)) {

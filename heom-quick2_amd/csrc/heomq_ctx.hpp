@@ -116,6 +116,10 @@ struct heomq_ctx {
   };
   std::vector<ChunkGraph> graphs;
   bool use_graphs = true;            // false: launch the chunk's kernels one by one
+  // Jacobi scaling of forward steady-state solves: -1 automatic (on when H is
+  // diagonal in the occupation basis), 0 off, 1 on
+  int precond = -1;
+  bool h_diagonal = false;
   void drop_graphs() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();

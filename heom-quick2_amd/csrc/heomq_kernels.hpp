@@ -121,9 +121,16 @@ struct DevPacked {
   int64_t vstride = 0, rec_stride = 0, gam_stride = 0, hc_stride = 0;
   int nprob = 1;
   // batches: problem p is skipped when frozen_flag[p * frozen_stride] == 3
-  // (the solver's state word of a finished problem); null for no skipping
+  // (the solver's state word of a finished problem), and is not scaled when
+  // frozen_flag[p * frozen_stride + 2] != 0 (its unscaled-iteration word);
+  // null for neither
   const int32_t* frozen_flag = nullptr;
   int frozen_stride = 0;
+  // 1: left Jacobi scaling, output element t divided by the operator's own
+  // diagonal gamma_K + hsign * c_tt (c_tt: x_t's coefficient in i(HX - XH)_t);
+  // elements whose diagonal vanishes (tier-0 populations) and the trace row
+  // are left unscaled
+  int precond = 0;
 };
 
 // Projected apply on packed vectors: y = Pi A x with the trace row on the

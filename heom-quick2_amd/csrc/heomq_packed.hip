@@ -59,13 +59,15 @@ template <typename T, bool TRANS>
 __device__ inline void diag_term(const DevPacked& op, const double* __restrict__ hc_ell,
                                  const typename P2<T>::type* __restrict__ xv, T gr, T gi,
                                  int64_t base, int s, int t, int64_t zero_at, T& ar, T& ai,
-                                 const typename P2<T>::type* s_x = nullptr, int64_t win_lo = 0, int win_n = 0) {
+                                 const typename P2<T>::type* s_x, int64_t win_lo, int win_n, T& csr, T& csi) {
   using V = typename P2<T>::type;
   const int dd = op.d * op.d;
   const V xe = ldw<T, TRANS>(xv, base + t, zero_at, s_x, win_lo, win_n);
   ar = gr * xe.x - gi * xe.y;
   ai = gr * xe.y + gi * xe.x;
   T cr = 0, ci = 0;
+  csr = 0;
+  csi = 0;
   const int W = op.hell_w;
   if (W > 0) {
     const int64_t row = (int64_t)s * dd + t;
@@ -82,6 +84,7 @@ __device__ inline void diag_term(const DevPacked& op, const double* __restrict__
         const V v = ldw<T, TRANS>(xv, base + q, zero_at, s_x, win_lo, win_n);
         cr += (T)h.x * v.x - (T)h.y * v.y;
         ci += (T)h.x * v.y + (T)h.y * v.x;
+        if (q == t) { csr += (T)h.x; csi += (T)h.y; }
       }
     }
   } else {
@@ -91,11 +94,24 @@ __device__ inline void diag_term(const DevPacked& op, const double* __restrict__
       const T hr = (T)op.hc[2 * u], hi = (T)op.hc[2 * u + 1];
       cr += hr * v.x - hi * v.y;
       ci += hr * v.y + hi * v.x;
+      if (op.hq[u] == t) { csr += hr; csi += hi; }
     }
   }
   const T hs = (T)op.hsign;
   ar += hs * cr;
   ai += hs * ci;
+}
+
+// Left Jacobi scaling of one output element: divide by gamma + hsign * c_tt.
+template <typename T>
+__device__ inline void jacobi_scale(const DevPacked& op, T gr, T gi, T csr, T csi, T& ar, T& ai) {
+  const T hs = (T)op.hsign;
+  const T dr = gr + hs * csr, di = gi + hs * csi;
+  const T n2 = dr * dr + di * di;
+  if (n2 <= (T)1e-24) return;
+  const T xr = (ar * dr + ai * di) / n2, xi = (ai * dr - ar * di) / n2;
+  ar = xr;
+  ai = xi;
 }
 
 // Coefficients (cL, cR) of a record: the first 32 bytes of DevRec.
@@ -163,7 +179,7 @@ constexpr size_t kApplySmem = kLightSmem > kHeavySmem ? kLightSmem : kHeavySmem;
 
 template <typename T, bool TRANS, int U>
 __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
-                                  int64_t zero_at, int bid, char* smem) {
+                                  int64_t zero_at, int bid, char* smem, bool scaled) {
   using V = typename P2<T>::type;
   int64_t* s_off = reinterpret_cast<int64_t*>(smem);
   int64_t* s_p0 = reinterpret_cast<int64_t*>(smem + kLightOffP0);
@@ -200,9 +216,9 @@ __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T
   const int64_t base = s_off[l];
   const int slot = (int)(t - base);
   const int64_t p0 = s_p0[l], p1 = s_p1[l];
-  T ar, ai;
+  T ar, ai, csr, csi;
   diag_term<T, TRANS>(op, cf_p.hc_ell, xv, (T)s_gam[l].x, (T)s_gam[l].y, base, s_sec[l], slot, zero_at, ar, ai,
-                      s_x, win_lo, win_n);
+                      s_x, win_lo, win_n, csr, csi);
   for (int64_t p = p0; p < p1; p += U) {
     const int nv = (int)(p1 - p < U ? p1 - p : U);
     DevEntryP en[U];
@@ -219,6 +235,7 @@ __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T
     }
     coupling_block<T, TRANS, U>(en, cf, nv, op.maps, xv, slot, zero_at, ar, ai);
   }
+  if (scaled) jacobi_scale<T>(op, (T)s_gam[l].x, (T)s_gam[l].y, csr, csi, ar, ai);
   V o;
   o.x = ar;
   o.y = ai;
@@ -235,7 +252,7 @@ __device__ void tier0_fixup_block(const DevPacked& op, const T* __restrict__ x, 
 // Block 0 (the tier-0 ADO) then applies the trace row and Hermitianises.
 template <typename T, bool TRANS>
 __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
-                                  int64_t zero_at, int rdo, int bid, char* smem) {
+                                  int64_t zero_at, int rdo, int bid, char* smem, bool scaled) {
   using V = typename P2<T>::type;
   constexpr int kStage = 256;
   T* red_r = reinterpret_cast<T*>(smem);
@@ -267,9 +284,10 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
     const int S = 256 / w;
     const int lane = threadIdx.x % w, slice = threadIdx.x / w;
     const int t = t0 + lane;
-    T ar = 0, ai = 0;
+    T ar = 0, ai = 0, csr = 0, csi = 0;
     if (slice < S) {
-      if (slice == 0) diag_term<T, TRANS>(op, cf_p.hc_ell, xv, gr, gi, base, s, t, zero_at, ar, ai, s_x, base, win_n);
+      if (slice == 0)
+        diag_term<T, TRANS>(op, cf_p.hc_ell, xv, gr, gi, base, s, t, zero_at, ar, ai, s_x, base, win_n, csr, csi);
       for (int k0 = slice; k0 < nent; k0 += S * U) {
         DevEntryP en[U];
         CoefLR cf[U];
@@ -303,10 +321,12 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
       __syncthreads();
       act = half;
     }
-    if (threadIdx.x < w) {
+    if (threadIdx.x < w) {   // slice 0: these lanes hold slot t's diagonal
+      T sr = red_r[threadIdx.x], si = red_i[threadIdx.x];
+      if (scaled) jacobi_scale<T>(op, gr, gi, csr, csi, sr, si);
       V o;
-      o.x = red_r[threadIdx.x];
-      o.y = red_i[threadIdx.x];
+      o.x = sr;
+      o.y = si;
       reinterpret_cast<V*>(y)[base + t] = o;
     }
     __syncthreads();
@@ -323,13 +343,14 @@ k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, i
   __shared__ __align__(16) char smem[kApplySmem];
   const int64_t p = blockIdx.y;   // problem of a batch
   if (op.frozen_flag && op.frozen_flag[p * op.frozen_stride] == 3) return;   // finished: block-uniform, no barrier yet
+  const bool scaled = op.precond && !(op.frozen_flag && op.frozen_flag[p * op.frozen_stride + 2]);
   x += 2 * p * op.vstride;
   y += 2 * p * op.vstride;
   const Coefs cf{op.rec + p * op.rec_stride, op.gam + 2 * p * op.gam_stride, op.hc_ell + 2 * p * op.hc_stride};
   if ((int64_t)blockIdx.x < op.n_heavy)   // block-uniform
-    heavy_body<T, TRANS>(op, cf, x, y, zero_at, rdo, (int)blockIdx.x, smem);
+    heavy_body<T, TRANS>(op, cf, x, y, zero_at, rdo, (int)blockIdx.x, smem, scaled);
   else
-    light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem);
+    light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem, scaled);
 }
 
 // Tier-0 fix-ups (one block): trace row (forward: y(d,d) = tr x0; transposed:

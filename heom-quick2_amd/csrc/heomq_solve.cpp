@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -21,14 +22,20 @@ namespace {
 template <typename T>
 hipError_t apply_T(heomq_ctx* c, bool trans, const T* x, T* y, int64_t tpos, hipStream_t s, bool proj);
 template <typename T>
-hipError_t apply_packed_T(heomq_ctx* c, bool trans, const T* x, T* y, bool rdo, hipStream_t s);
+hipError_t apply_packed_T(heomq_ctx* c, bool trans, const T* x, T* y, bool rdo, hipStream_t s, bool prec = false);
 template <>
-hipError_t apply_packed_T<double>(heomq_ctx* c, bool trans, const double* x, double* y, bool rdo, hipStream_t s) {
-  return launch_apply_packed_f64(trans ? c->ptrn : c->pfwd, x, y, rdo, s);
+hipError_t apply_packed_T<double>(heomq_ctx* c, bool trans, const double* x, double* y, bool rdo, hipStream_t s,
+                                  bool prec) {
+  DevPacked op = trans ? c->ptrn : c->pfwd;
+  op.precond = prec ? 1 : 0;
+  return launch_apply_packed_f64(op, x, y, rdo, s);
 }
 template <>
-hipError_t apply_packed_T<float>(heomq_ctx* c, bool trans, const float* x, float* y, bool rdo, hipStream_t s) {
-  return launch_apply_packed_f32(trans ? c->ptrn : c->pfwd, x, y, rdo, s);
+hipError_t apply_packed_T<float>(heomq_ctx* c, bool trans, const float* x, float* y, bool rdo, hipStream_t s,
+                                 bool prec) {
+  DevPacked op = trans ? c->ptrn : c->pfwd;
+  op.precond = prec ? 1 : 0;
+  return launch_apply_packed_f32(op, x, y, rdo, s);
 }
 // Solvers iterate with the projected operator Pi*A (Pi: parity sectors +
 // Hermitian tier 0).  A maps range(Pi) into itself and the physical steady
@@ -54,9 +61,10 @@ int64_t tpos_of(const heomq_ctx* c, int trace) {
 
 template <typename T>
 struct Work {
-  T *w, *y1, *y2, *u1, *u2, *v, *d;
+  T *w, *y1, *y2, *u1, *u2, *v, *d;   // the Krylov state
+  T *ax, *xbest;                      // true-residual product, best iterate so far
 };
-constexpr int kWorkVecs = 7;
+constexpr int kWorkVecs = 9;
 
 template <typename T>
 int carve(DevBuf& buf, int64_t n, int count, std::vector<T*>& out) {
@@ -128,62 +136,110 @@ int chunk_graph(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, boo
 // block), the iterate's tier-0 block re-Hermitianised at every restart.
 template <typename T>
 int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool trans, int64_t tpos,
-              double tol, int maxiter, hipStream_t st, RunStats& rs, int mode) {
-  auto A = [&](const T* in, T* out) -> hipError_t {
-    return mode == 1 ? apply_packed_T<T>(c, trans, in, out, tpos >= 0, st)
+              double tol, int maxiter, hipStream_t st, RunStats& rs, int mode, bool prec = false) {
+  // the iteration runs on the (optionally Jacobi-scaled) operator; every
+  // true residual ||b - A x|| uses the operator itself.  Left scaling keeps b
+  // only when the scaling is 1 on b's support: the callers enable it for
+  // b = e_trace, whose one element sits on the unscaled trace row.
+  auto Ait = [&](const T* in, T* out) -> hipError_t {
+    return mode == 1 ? apply_packed_T<T>(c, trans, in, out, tpos >= 0, st, prec)
+                     : apply_T<T>(c, trans, in, out, tpos, st, false);
+  };
+  auto A0 = [&](const T* in, T* out) -> hipError_t {
+    return mode == 1 ? apply_packed_T<T>(c, trans, in, out, tpos >= 0, st, false)
                      : apply_T<T>(c, trans, in, out, tpos, st, false);
   };
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   double* part = c->partial.as<double>();
   double* red = c->red.as<double>();
-  const int check_every = 8;
-  double best = std::numeric_limits<double>::infinity();
-  int base_iters = 0;
-  int stalls = 0;
-  while (true) {
-    if (mode == 1) HIPCHECK(launch_herm_packed<T>(c->pfwd, x, st));
-    HIPCHECK(A(x, W.u1));
+  double* hr = static_cast<double*>(c->pinned);
+  auto true_resid = [&](double& rr, double& bb) -> int {
+    HIPCHECK(A0(x, W.ax));
     rs.applies++;
-    HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, part)));
-    double* hr = static_cast<double*>(c->pinned);
+    HIPCHECK((vec_resid<T, T>(st, n, b, W.ax, red, part)));
     HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    const double rr = hr[0], bb = hr[1];
+    rr = hr[0];
+    bb = hr[1];
     rs.relres = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    return HEOMQ_OK;
+  };
+  const int check_every = 8;      // iterations per graph replay / host read of the scalars
+  const int verify_every = 32;    // replays between true-residual checks (256 iterations)
+  double best = std::numeric_limits<double>::infinity();
+  int base_iters = 0, stalls = 0;
+  while (true) {
+    if (mode == 1) HIPCHECK(launch_herm_packed<T>(c->pfwd, x, st));
+    double rr = 0, bb = 0;
+    if (int rc = true_resid(rr, bb)) return rc;
     if (rr <= tol * tol * bb || rr == 0.0) return HEOMQ_OK;
     if (base_iters >= maxiter) break;
-    if (rr >= best && ++stalls > 3) break;
-    best = std::min(best, rr);
-    HIPCHECK(tfq_reset(st, s, tol * std::sqrt(bb)));
-    HIPCHECK(tfq_init_r<T>(st, n, b, W.u1, W.w, W.y1, W.d, s, part));
-    HIPCHECK(A(W.y1, W.u1));
+    const double best_before = best;
+    if (rr < best) {
+      best = rr;
+      HIPCHECK(hipMemcpyAsync(W.xbest, x, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+    } else {
+      if (++stalls > 3) {
+        if (!prec) break;
+        prec = false;   // the scaled iteration stalled: continue unscaled from the best iterate
+        stalls = 0;
+      }
+      if (rr > 100.0 * best) {   // the recurrences diverged: restart from the best iterate
+        HIPCHECK(hipMemcpyAsync(x, W.xbest, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+        if (int rc = true_resid(rr, bb)) return rc;
+      }
+    }
+    HIPCHECK(tfq_reset(st, s, tol * std::sqrt(bb), nullptr, 1, base_iters == 0 && std::isinf(best_before)));
+    if (prec) {
+      HIPCHECK(Ait(x, W.ax));
+      rs.applies++;
+    }
+    HIPCHECK(tfq_init_r<T>(st, n, b, W.ax, W.w, W.y1, W.d, s, part));
+    TfqmrScal& hs = *reinterpret_cast<TfqmrScal*>(static_cast<char*>(c->pinned) + 1024);
+    if (prec) {   // the quasi-residual bound measures the scaled residual: scale the target with it
+      HIPCHECK(hipMemcpyAsync(&hs, s, sizeof(TfqmrScal), hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      if (rr > 0 && hs.rr > 0)
+        HIPCHECK(vec_set_elem(st, reinterpret_cast<double*>(s), offsetof(TfqmrScal, target) / sizeof(double),
+                              tol * std::sqrt(bb) * std::sqrt(hs.rr / rr)));
+    }
+    HIPCHECK(Ait(W.y1, W.u1));
     rs.applies++;
     HIPCHECK(tfq_init_v<T>(st, n, W.u1, W.v, s, part));
-    TfqmrScal& hs = *reinterpret_cast<TfqmrScal*>(static_cast<char*>(c->pinned) + 1024);
     hipGraphExec_t chunk = nullptr;
     if (c->use_graphs)
-      if (int rc = chunk_graph<T>(c, x, b, W, n, trans, tpos, mode, st, A, check_every, &chunk)) return rc;
-    while (true) {
+      if (int rc = chunk_graph<T>(c, x, b, W, n, trans, tpos, mode + 2 * (int)prec, st, Ait, check_every, &chunk))
+        return rc;
+    for (int replay = 1;; ++replay) {
       if (chunk)
         HIPCHECK(hipGraphLaunch(chunk, st));
       else
-        HIPCHECK((record_chunk<T>(c, x, W, n, st, A, check_every)));
+        HIPCHECK((record_chunk<T>(c, x, W, n, st, Ait, check_every)));
       HIPCHECK(hipMemcpyAsync(&hs, s, sizeof(TfqmrScal), hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
       rs.iters = base_iters + hs.iter;
       rs.applies += 2 * check_every;
       if (hs.flag != 0 || rs.iters >= maxiter) break;
+      if (replay % verify_every == 0) {   // converged already, or drifting away?
+        double r2 = 0, b2 = 0;
+        if (int rc = true_resid(r2, b2)) return rc;
+        if (r2 <= tol * tol * b2 || r2 > 100.0 * best) break;
+        if (r2 < best) {
+          best = r2;
+          HIPCHECK(hipMemcpyAsync(W.xbest, x, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+        }
+      }
     }
     base_iters = rs.iters;
     if (hs.flag == 2 && ++stalls > 3) return fail(HEOMQ_BREAKDOWN, "TFQMR breakdown");
   }
-  HIPCHECK(A(x, W.u1));
-  HIPCHECK((vec_resid<T, T>(st, n, b, W.u1, red, part)));
-  double* hr = static_cast<double*>(c->pinned);
-  HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHECK(hipStreamSynchronize(st));
-  rs.relres = hr[1] > 0 ? std::sqrt(hr[0] / hr[1]) : std::sqrt(hr[0]);
-  if (hr[0] <= tol * tol * hr[1]) return HEOMQ_OK;
+  double rr = 0, bb = 0;
+  if (int rc = true_resid(rr, bb)) return rc;
+  if (rr > best) {   // hand back the best iterate seen
+    HIPCHECK(hipMemcpyAsync(x, W.xbest, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+    if (int rc = true_resid(rr, bb)) return rc;
+  }
+  if (rr <= tol * tol * bb) return HEOMQ_OK;
   return fail(HEOMQ_NOT_CONVERGED, "TFQMR did not reach the tolerance");
 }
 
@@ -224,7 +280,7 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
   const int64_t n = packed ? 2 * c->ptotal : 2 * c->N * c->d * c->d;
   std::vector<double*> v;
   if (int rc = carve<double>(c->ws64, n, kWorkVecs + (packed ? 2 : 1), v)) return rc;
-  Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6]};
+  Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
   double* bb = v[kWorkVecs];
   double* xs = packed ? v[kWorkVecs + 1] : x;
   if (packed) {
@@ -241,8 +297,9 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
     return rc;
   }
   RunStats rs;
+  const bool prec = packed && !transpose && !b && (c->precond == 1 || (c->precond < 0 && c->h_diagonal));
   const int rc = tfqmr_run<double>(c, xs, bb, W, n, transpose != 0, tpos_of(c, trace), tol, maxiter, st, rs,
-                                   packed ? 1 : 0);
+                                   packed ? 1 : 0, prec);
   if (packed) HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
@@ -270,7 +327,7 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   if (int rc = carve<double>(c->ws64, n, 3, v64)) return rc;
   std::vector<float*> v32;
   if (int rc = carve<float>(c->ws32, n, kWorkVecs + 2, v32)) return rc;
-  Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6]};
+  Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6], v32[7], v32[8]};
   float* x32 = v32[kWorkVecs];
   float* r32 = v32[kWorkVecs + 1];
   double* ax = v64[0];
@@ -323,7 +380,7 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   if (finish64) {
     std::vector<double*> w64;
     if (int e = carve<double>(c->ws64b, n, kWorkVecs, w64)) return e;
-    Work<double> W64{w64[0], w64[1], w64[2], w64[3], w64[4], w64[5], w64[6]};
+    Work<double> W64{w64[0], w64[1], w64[2], w64[3], w64[4], w64[5], w64[6], w64[7], w64[8]};
     RunStats rs;
     rc = tfqmr_run<double>(c, xs, bb, W64, n, false, 0, tol, std::max(maxiter - inner_total, 1), st, rs, 1);
     fp64_iters = rs.iters;
@@ -416,6 +473,12 @@ int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxit
   if (rc != HEOMQ_OK && rc != HEOMQ_NOT_CONVERGED) return rc;
   HIPCHECK(hipMemcpy(rho_host, x, sizeof(double) * n, hipMemcpyDeviceToHost));
   return rc;
+}
+
+int heomq_ctx_set_precond(heomq_ctx* c, int mode) {
+  if (!c || mode < -1 || mode > 1) return fail(HEOMQ_BAD_ARG, "precond mode must be -1, 0 or 1");
+  c->precond = mode;
+  return HEOMQ_OK;
 }
 
 int heomq_ctx_set_graphs(heomq_ctx* c, int enable) {

@@ -55,8 +55,8 @@ template <> struct W2<float> { using type = float2; };
 // Shadow vector of TFQMR, generated where it is used instead of stored: a
 // +-1 sign per real element from a splitmix64 hash of its index (the same
 // bits for any grid and any call).
-__device__ inline double shadow(int64_t k) {
-  uint64_t z = 0x5EED1234ull + 0x9E3779B97F4A7C15ull * (uint64_t)(k + 1);
+__device__ inline double shadow(int64_t k, uint64_t seed) {
+  uint64_t z = 0x5EED1234ull + 0xD1B54A32D192ED03ull * seed + 0x9E3779B97F4A7C15ull * (uint64_t)(k + 1);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
@@ -71,6 +71,7 @@ __global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restric
   s += blockIdx.y;
   BATCH_PARTIAL();
   if (s->flag != kInitRequested) return;   // partials of this problem are not read
+  const uint64_t seed = s->seed;
   { const int64_t o = BATCH_OFF(); b += o; ax += o; w += o; y1 += o; d += o; }
   double rr = 0, rrt = 0;
   GRID_LOOP(i, n2) {
@@ -85,7 +86,7 @@ __global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restric
     z.y = 0;
     reinterpret_cast<V*>(d)[i] = z;
     rr += (double)r.x * r.x + (double)r.y * r.y;
-    rrt += shadow(2 * i) * r.x + shadow(2 * i + 1) * r.y;
+    rrt += shadow(2 * i, seed) * r.x + shadow(2 * i + 1, seed) * r.y;
   }
   block_partial(rr, rrt, partial);
 }
@@ -98,13 +99,14 @@ __global__ void k_init_v(int64_t n2, const T* __restrict__ u1, T* __restrict__ v
   s += blockIdx.y;
   BATCH_PARTIAL();
   if (s->flag != kInitialising) return;
+  const uint64_t seed = s->seed;
   { const int64_t o = BATCH_OFF(); u1 += o; v += o; if (u1_copy) u1_copy += o; }
   double sg = 0;
   GRID_LOOP(i, n2) {
     const V t = reinterpret_cast<const V*>(u1)[i];
     reinterpret_cast<V*>(v)[i] = t;
     if (u1_copy) reinterpret_cast<V*>(u1_copy)[i] = t;
-    sg += shadow(2 * i) * t.x + shadow(2 * i + 1) * t.y;
+    sg += shadow(2 * i, seed) * t.x + shadow(2 * i + 1, seed) * t.y;
   }
   block_partial(sg, 0.0, partial);
 }
@@ -153,6 +155,7 @@ __global__ void k_step2(int64_t n2, T* __restrict__ x, T* __restrict__ w, const 
   s += blockIdx.y;
   BATCH_PARTIAL();
   if (s->flag) return;
+  const uint64_t seed = s->seed;
   { const int64_t o = BATCH_OFF(); x += o; w += o; u2 += o; d += o; y2 += o; }
   const T alpha = (T)s->alpha, dcoef = (T)s->dcoef, eta = (T)s->eta;
   double ww = 0, rw = 0;
@@ -173,7 +176,7 @@ __global__ void k_step2(int64_t n2, T* __restrict__ x, T* __restrict__ w, const 
     dn.y = yi.y + dcoef * di.y;
     reinterpret_cast<V*>(d)[i] = dn;
     ww += (double)wi.x * wi.x + (double)wi.y * wi.y;
-    rw += shadow(2 * i) * wi.x + shadow(2 * i + 1) * wi.y;
+    rw += shadow(2 * i, seed) * wi.x + shadow(2 * i + 1, seed) * wi.y;
   }
   block_partial(ww, rw, partial);
 }
@@ -212,6 +215,7 @@ __global__ void k_v(int64_t n2, const T* __restrict__ u1, const T* __restrict__ 
   s += blockIdx.y;
   BATCH_PARTIAL();
   if (s->flag || s->stop) return;
+  const uint64_t seed = s->seed;
   { const int64_t o = BATCH_OFF(); u1 += o; u2 += o; v += o; }
   const T beta = (T)s->beta;
   double sg = 0;
@@ -222,7 +226,7 @@ __global__ void k_v(int64_t n2, const T* __restrict__ u1, const T* __restrict__ 
     t.x = a.x + beta * (b.x + beta * c.x);
     t.y = a.y + beta * (b.y + beta * c.y);
     reinterpret_cast<V*>(v)[i] = t;
-    sg += shadow(2 * i) * t.x + shadow(2 * i + 1) * t.y;
+    sg += shadow(2 * i, seed) * t.x + shadow(2 * i + 1, seed) * t.y;
   }
   block_partial(sg, 0.0, partial);
 }
@@ -395,10 +399,11 @@ __global__ void k_hermitize0(int d, T* __restrict__ x) {
 // Problems p < nprob with mask[p] != 0 (every problem if mask is null) are set
 // up for a (re)start: scalars cleared, flag = kInitRequested.  Problems left
 // out keep their state (a frozen problem stays frozen).
-__global__ void k_tfq_reset(TfqmrScal* s, double target, const int32_t* mask, int nprob) {
+__global__ void k_tfq_reset(TfqmrScal* s, double target, const int32_t* mask, int nprob, int fresh) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nprob || (mask && !mask[p])) return;
   TfqmrScal* q = s + p;
+  q->seed = fresh ? 0 : q->seed + 1;
   q->rho = q->sigma = q->alpha = q->beta = 0;
   q->theta = q->tau = q->eta = q->dcoef = 0;
   q->rr = 0;
@@ -406,6 +411,22 @@ __global__ void k_tfq_reset(TfqmrScal* s, double target, const int32_t* mask, in
   q->iter = 0;
   q->stop = 0;
   q->flag = kInitRequested;
+}
+
+// target[p] = target[p] * sqrt(rr[p] / rr_true[p]) for problems just (re)started
+// (flag kInitialising): the quasi-residual bound of a scaled iteration measures
+// the scaled residual, so its stop target follows the ratio at the start.
+__global__ void k_tfq_scale_targets(TfqmrScal* s, const double* __restrict__ rr_true, int nprob) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob || s[p].flag != kInitialising) return;
+  const double t = rr_true[2 * p];
+  if (t > 0 && s[p].rr > 0) s[p].target *= sqrt(s[p].rr / t);
+}
+
+// unscaled[p] = mask[p] (0 for every problem when mask is null).
+__global__ void k_tfq_set_unscaled(TfqmrScal* s, const int32_t* mask, int nprob) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nprob) s[p].unscaled = mask ? mask[p] : 0;
 }
 
 // flag[p] = kFrozen where mask[p]: finished problems drop out of every kernel.
@@ -533,8 +554,16 @@ hipError_t vec_project(hipStream_t st, int64_t nado, int d, T* x, const int8_t* 
 template hipError_t vec_project<double>(hipStream_t, int64_t, int, double*, const int8_t*, const TierBounds&, bool);
 template hipError_t vec_project<float>(hipStream_t, int64_t, int, float*, const int8_t*, const TierBounds&, bool);
 
-hipError_t tfq_reset(hipStream_t st, TfqmrScal* s, double target, const int32_t* mask, int nprob) {
-  hipLaunchKernelGGL(k_tfq_reset, dim3((nprob + 63) / 64), dim3(64), 0, st, s, target, mask, nprob);
+hipError_t tfq_reset(hipStream_t st, TfqmrScal* s, double target, const int32_t* mask, int nprob, bool fresh) {
+  hipLaunchKernelGGL(k_tfq_reset, dim3((nprob + 63) / 64), dim3(64), 0, st, s, target, mask, nprob, fresh ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t tfq_scale_targets(hipStream_t st, TfqmrScal* s, const double* rr_true, int nprob) {
+  hipLaunchKernelGGL(k_tfq_scale_targets, dim3((nprob + 63) / 64), dim3(64), 0, st, s, rr_true, nprob);
+  return hipGetLastError();
+}
+hipError_t tfq_set_unscaled(hipStream_t st, TfqmrScal* s, const int32_t* mask, int nprob) {
+  hipLaunchKernelGGL(k_tfq_set_unscaled, dim3((nprob + 63) / 64), dim3(64), 0, st, s, mask, nprob);
   return hipGetLastError();
 }
 hipError_t tfq_freeze(hipStream_t st, TfqmrScal* s, const int32_t* mask, int nprob) {

@@ -29,7 +29,8 @@ struct TfqmrScal {
   double rr, target;      // ||r0||^2, stop when tau*sqrt(2m+1) <= target
   int32_t iter, flag;     // flag: 0 running, 1 bound met, 2 breakdown
   int32_t stop;           // bound met this iteration: finish its x update, then flag = 1
-  int32_t pad;
+  int32_t unscaled;       // batches: 1 = this problem iterates without the Jacobi scaling
+  uint64_t seed;          // shadow vector of this (re)start: a new one at every restart
 };
 
 // BiCG scalars (bicg/solvebicg.f90 notation: rho = <s, r>, alpha, beta).
@@ -85,8 +86,14 @@ hipError_t vec_resid(hipStream_t, int64_t n, const TB* b, const TA* ax, double* 
                      Batch bt = Batch{});
 hipError_t vec_resid_to_f32(hipStream_t, int64_t n, const double* b, const double* ax, float* r32, double scale);
 hipError_t vec_axpy_f32_to_f64(hipStream_t, int64_t n, double* x, const float* dx, double alpha);
-hipError_t tfq_reset(hipStream_t, TfqmrScal* s, double target, const int32_t* mask = nullptr, int nprob = 1);
+// fresh: the first start of a solve (shadow seed 0); otherwise a restart (next seed)
+hipError_t tfq_reset(hipStream_t, TfqmrScal* s, double target, const int32_t* mask = nullptr, int nprob = 1,
+                     bool fresh = true);
 hipError_t tfq_freeze(hipStream_t, TfqmrScal* s, const int32_t* mask, int nprob);
+// unscaled[p] = mask[p] (0 for all with a null mask)
+hipError_t tfq_set_unscaled(hipStream_t, TfqmrScal* s, const int32_t* mask, int nprob);
+// target *= sqrt(rr / rr_true[2p]) for problems in kInitialising (scaled iterations)
+hipError_t tfq_scale_targets(hipStream_t, TfqmrScal* s, const double* rr_true, int nprob);
 hipError_t vec_set_elem(hipStream_t, double* v, int64_t off, double val);
 // A few scattered values written into every problem's copy of a vector.
 struct SetPattern {

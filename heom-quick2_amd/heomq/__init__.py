@@ -308,6 +308,10 @@ class Context:
             raise HeomqError(_lib.BAD_DIM, f"expected contiguous {dtype} device tensor with {self.N * self.d * self.d} elements")
         return C.c_void_p(v.data_ptr())
 
+    def set_precond(self, mode: int):
+        """Jacobi scaling of forward steady solves: -1 automatic (diagonal H), 0 off, 1 on."""
+        check(lib().heomq_ctx_set_precond(self._h, int(mode)))
+
     def set_graphs(self, enable: bool):
         """Replay solver iterations from captured graphs (default) or launch eagerly."""
         check(lib().heomq_ctx_set_graphs(self._h, int(enable)))

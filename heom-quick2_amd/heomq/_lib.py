@@ -59,6 +59,7 @@ SIGNATURES = {
     "heomq_ctx_refresh": (C.c_int, [P, P]),
     "heomq_ctx_vector_len": (C.c_int64, [P]),
     "heomq_ctx_set_graphs": (C.c_int, [P, C.c_int]),
+    "heomq_ctx_set_precond": (C.c_int, [P, C.c_int]),
     "heomq_apply": (C.c_int, [P, P, P, C.c_int, P]),
     "heomq_apply_transpose": (C.c_int, [P, P, P, C.c_int, P]),
     "heomq_apply_f32": (C.c_int, [P, P, P, C.c_int, P]),

@@ -146,6 +146,11 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* m);
  * eight TFQMR iterations; solves requested on the null stream run on a
  * context-owned stream).  enable = 0 launches the same kernels one by one. */
 int heomq_ctx_set_graphs(heomq_ctx* c, int enable);
+/* Jacobi (diagonal) left scaling of the forward steady-state TFQMR with the
+ * default right-hand side: -1 automatic (on when H is diagonal in the
+ * occupation basis, e.g. the single-impurity model), 0 off, 1 on.  The
+ * convergence test always uses the unscaled residual ||b - A x||. */
+int heomq_ctx_set_precond(heomq_ctx* c, int mode);
 /* Number of complex elements N*d*d of one ADO vector. */
 int64_t heomq_ctx_vector_len(const heomq_ctx* c);
 

@@ -197,3 +197,20 @@ def test_bicg_steady_state(name):
     assert np.abs(xb[0] - xb[0].conj().T).max() == 0.0
     xt, _, _ = O.steady_state(o, tol=1e-13)
     assert np.abs(xb[0] - xt[0]).max() <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
+def test_jacobi_scaling_on_and_off(name):
+    """The steady state does not depend on the iteration's diagonal scaling
+    (automatic for diagonal H; forced on for the two-impurity H with hopping)."""
+    m, ctx, o, torch = setup(name)
+    xo, _, _ = O.steady_state(o, tol=1e-13)
+    its = {}
+    for mode in (0, 1):
+        ctx.set_precond(mode)
+        x, st = ctx.solve_steady(tol=1e-12)
+        its[mode] = st["iters"]
+        assert st["status"] == 0
+        assert np.abs(from_dev(x)[0] - xo[0]).max() <= 1e-10
+    ctx.set_precond(-1)
+    print(name, "iterations plain / scaled:", its, flush=True)
