@@ -50,7 +50,7 @@ def main():
                     help="relative residual target (default: 1e-7 two-impurity, 1e-12 SIAM)")
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-applies", type=int, default=2)
+    ap.add_argument("--cpu-applies", type=int, default=5)
     ap.add_argument("--no-ref", action="store_true",
                     help="skip the long reference solve (rdo_err is then not measured); for profiling runs")
     ap.add_argument("--apply-only", action="store_true",
@@ -121,7 +121,7 @@ def main():
     achieved = bytes_alg / t_apply / 1e9
     dense_bytes = 2 * 16 * d * d * N        # the same state in the dense d x d layout
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                 "kernel": f"k_apply_packed<{d},double,false>", "t_apply_us": round(t_apply * 1e6, 2),
                 "bytes_alg": bytes_alg, "packed_elems": P, "dense_elems": info["dense"],
                 "dense_equiv_GBs": round(dense_bytes / t_apply / 1e9, 1)}
@@ -158,6 +158,17 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(wname):
+    """HBM bytes per packed-apply launch from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    import re
+    f = ROOT / "profiles" / f"packed_apply_{wname}_traffic.md"
+    if not f.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", f.read_text())
+    return int(m.group(1)) if m else None
 
 
 def sweep_points(npts, seed=0x5EED0004):
@@ -267,6 +278,8 @@ def apply_only(params, reps):
     import heomq
     src = torch.ones(1 << 23, dtype=torch.float64, device="cuda:0")
     dst = torch.empty_like(src)
+    evict = torch.empty(1 << 25, dtype=torch.float64, device="cuda:0")
+    evict.fill_(0.0)   # 256 MiB written after src: src is no longer in L2 when it is copied
     dst.copy_(src)
     ctx = heomq.Context(heomq.Model(params))
     xp = ctx.pack(ctx.initial_guess())
@@ -278,31 +291,28 @@ def apply_only(params, reps):
 
 
 def cpu_baseline(params, wname, napply, applies_per_solve):
-    """Time the numpy oracle's apply on the host (single thread)."""
+    """Time the oracle's OpenMP C port of the reference apply (tfqmr_double/
+    calc_ax_tfqmr_omp.f90 on the reference's own tables) on the host cores."""
     import numpy as np
-    try:
-        from threadpoolctl import threadpool_limits
-    except Exception:  # pragma: no cover
-        threadpool_limits = None
     from oracle import heom_oracle as O
-    if params.ham == 1:
-        op = O.dqd_params()
-    else:
-        op = O.siam_params()
+    from oracle import heom_port as P
+    op = O.dqd_params() if params.ham == 1 else O.siam_params()
     om = O.Model(op)
-    x = O.seeded_ado_vector(om, 7)
-    ctx = threadpool_limits(1) if threadpool_limits else None
+    port = P.Port(om)
+    threads = min(port.threads(), 16)
+    port.lib.heom_port_set_threads(threads)
+    xf = np.ascontiguousarray(O.to_fortran(O.seeded_ado_vector(om, 7)))
+    yf = np.zeros_like(xf)
+    port.apply_fortran(xf, yf, "rdo")   # first touch
     t0 = time.perf_counter()
     for _ in range(napply):
-        O.apply(om, x, trace_row="rdo")
+        port.apply_fortran(xf, yf, "rdo")
     t = (time.perf_counter() - t0) / napply
-    if ctx is not None:
-        ctx.unregister() if hasattr(ctx, "unregister") else None
-    return {"value": round(1.0 / (t * applies_per_solve), 6), "unit": "solves/s", "cores": 1,
+    return {"value": round(1.0 / (t * applies_per_solve), 6), "unit": "solves/s", "cores": threads,
             "kind": "port",
-            "sample": f"{napply} numpy-oracle applies of the same workload ({wname}), "
-                      f"{t:.2f} s each, scaled by the GPU solve's {applies_per_solve:.0f} applies per solve "
-                      f"(vector updates not counted)"}
+            "sample": f"{napply} applies of the same workload ({wname}) by the OpenMP C port of "
+                      f"calc_ax_tfqmr_omp on the reference's tables, {t * 1e3:.1f} ms each on {threads} threads, "
+                      f"scaled by the GPU solve's {applies_per_solve:.0f} applies per solve (vector updates not counted)"}
 
 
 if __name__ == "__main__":

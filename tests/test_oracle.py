@@ -347,3 +347,17 @@ def test_bicg_steady_state_matches_landauer():
     assert np.abs(x[0] - xt[0]).max() <= 1e-9
     IL, n = O.landauer_siam(p)                  # per spin
     assert np.abs(O.current(o, x)[0] - IL).max() <= 1e-8 * abs(IL)
+
+
+@pytest.mark.parametrize("p", [O.siam_params(nk=2, ntop=3, V=0.3, U=1.5), O.dqd_params(nk=1, ntop=2),
+                               O.siam_params(nk=2, ntop=2, lsdraw=True)])
+def test_c_port_matches_numpy_oracle(p):
+    # the OpenMP C port of calc_ax_tfqmr_omp (the CPU baseline) on the
+    # reference's own table layout agrees with the pinned numpy oracle
+    from oracle import heom_port as P
+    o = O.Model(p)
+    port = P.Port(o)
+    x = O.seeded_ado_vector(o, 3)
+    for tr in ("last", "rdo", None):
+        y, yo = port.apply(x, tr), O.apply(o, x, trace_row=tr)
+        assert np.abs(y - yo).max() <= 1e-14 * np.abs(yo).max()
