@@ -754,6 +754,7 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
   c->dpar.op = c->fwd.op;
   if (int rc = build_packed(c.get(), m, rowf, entf, rowt, entt)) return rc;
   sync_packed(c.get());
+  if (int rc = build_jacobi(c.get(), m)) return rc;
   HIPCHECK(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
   HIPCHECK(hipStreamCreateWithFlags(&c->solve_stream, hipStreamNonBlocking));
   HIPCHECK(c->scal.alloc(sizeof(TfqmrScal) * 2));
@@ -778,6 +779,7 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
   fill_op(c, c->fwd, c->rec, false, -1.0);
   fill_op(c, c->trn, c->rec, true, +1.0);
   sync_packed(c);
+  if (int rc = build_jacobi(c, h->m)) return rc;
   HIPCHECK(hipDeviceSynchronize());
   return HEOMQ_OK;
 }

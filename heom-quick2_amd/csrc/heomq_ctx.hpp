@@ -120,6 +120,14 @@ struct heomq_ctx {
   // diagonal in the occupation basis), 0 off, 1 on
   int precond = -1;
   bool h_diagonal = false;
+  // block-Jacobi iteration (jacobi/): eigensystem of H per conserved group of
+  // Fock states, the default shift, per-ADO rates on the host (for the shift's
+  // degeneracy check) and the iteration's own workspace
+  DevBuf j_u, j_e, j_grp, j_gn, ws_jac;
+  heomq::DevJacobi pjac;
+  double ejac_default = 0.0, ejac_gap = 0.0, ejac_cb = 0.0;
+  int jac_nnz = 0;
+  std::vector<double> gam_h;
   void drop_graphs() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
@@ -140,4 +148,8 @@ namespace heomq {
 // and the per-ADO diagonal rates gamma_K (conjugated for the transpose).
 void build_records(const Model& m, std::vector<DevRec>& rec);
 void build_gamma(const Model& m, const std::vector<cplx>& gdraw, std::vector<double>& gam, bool conj);
+// Eigensystem of H for the block-Jacobi iteration (uploaded into c->pjac) and
+// the default shift of jacobi/prelude_jacobi.f90; run whenever H or the bath
+// coefficients change.
+int build_jacobi(heomq_ctx* c, const Model& m);
 }  // namespace heomq

@@ -170,4 +170,24 @@ hipError_t launch_herm_from_lower_f64(int d, double* x, hipStream_t s);
 // rho0 of a vector reduced to its lower triangle with a real diagonal.
 hipError_t launch_lower_tier0_f64(int d, double* x, hipStream_t s);
 
+// Same-tier block inverse of the block-Jacobi iteration, in the eigenbasis of
+// H.  H is block-diagonal over groups of Fock states sharing the quantum
+// numbers the packed layout conserves; the eigenvectors (columns of u) of
+// group g are supported on g, so element (i, j) of U^H X U only needs the
+// elements (k, l) of X with k in group(i) and l in group(j), all of which the
+// packed layout stores whenever it stores (i, j).
+struct DevJacobi {
+  const double* u = nullptr;        // d*d complex, column-major: u[k + i*d] = U_ki
+  const double* e = nullptr;        // d eigenvalues of H
+  const int16_t* grp = nullptr;     // [d][gmax]: the states of group(i)
+  const int8_t* gn = nullptr;       // group size of each state
+  int gmax = 0;
+  double ejac = 0.0;                // the iteration's shift (> 0)
+};
+
+// t = (U^H r U)_ij / (ejac - gamma_K + i (E_i - E_j)) on the packed layout
+hipError_t launch_jacobi_fwd(const DevPacked& op, const DevJacobi& jc, const double* r, double* t, hipStream_t s);
+// x += U t U^H on the packed layout
+hipError_t launch_jacobi_back(const DevPacked& op, const DevJacobi& jc, const double* t, double* x, hipStream_t s);
+
 }  // namespace heomq

@@ -431,6 +431,75 @@ __global__ void k_pack(const DevPacked op, const TD* __restrict__ dense, TP* __r
   packed[2 * t + 1] = (TP)dense[2 * src + 1];
 }
 
+// Block-Jacobi same-tier inverse (jacobi/calc_ax_jacobi_omp.f90:276-298 on
+// tier >= 1, :95-108 on tier 0): forward half.  Thread = packed element
+// (i, j) of ADO a; the transform sums over the H-groups of i and j only.
+__global__ void k_jacobi_fwd(const DevPacked op, const DevJacobi jc, const double* __restrict__ r,
+                             double* __restrict__ t) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= op.total) return;
+  const int64_t a = find_ado(op, g);
+  const int d = op.d, DD = d * d, s = op.sec[a];
+  const int64_t o = op.off[a];
+  const int e = op.sec_elems[s * DD + (int)(g - o)];
+  const int i = e % d, j = e / d;
+  const int ni = jc.gn[i], nj = jc.gn[j];
+  const double2* rv = reinterpret_cast<const double2*>(r);
+  const double2* uv = reinterpret_cast<const double2*>(jc.u);
+  double ar = 0.0, ai = 0.0;
+  for (int p = 0; p < ni; ++p) {
+    const int k = jc.grp[i * jc.gmax + p];
+    const double2 uki = uv[k + i * d];            // conj(U_ki) multiplies
+    double br = 0.0, bi = 0.0;                    // sum_l X_kl U_lj
+    for (int q = 0; q < nj; ++q) {
+      const int l = jc.grp[j * jc.gmax + q];
+      const double2 x = rv[o + op.sec_pos[s * DD + k + l * d]];
+      const double2 ulj = uv[l + j * d];
+      br += x.x * ulj.x - x.y * ulj.y;
+      bi += x.x * ulj.y + x.y * ulj.x;
+    }
+    ar += uki.x * br + uki.y * bi;
+    ai += uki.x * bi - uki.y * br;
+  }
+  // w = 1 / (ejac - gamma_K + i (E_i - E_j))
+  const double wr = jc.ejac - op.gam[2 * a], wi = (jc.e[i] - jc.e[j]) - op.gam[2 * a + 1];
+  const double den = wr * wr + wi * wi;
+  t[2 * g] = (ar * wr + ai * wi) / den;
+  t[2 * g + 1] = (ai * wr - ar * wi) / den;
+}
+
+// Back half: x_ij += sum_{p in group(i), q in group(j)} U_ip t_pq conj(U_jq).
+__global__ void k_jacobi_back(const DevPacked op, const DevJacobi jc, const double* __restrict__ t,
+                              double* __restrict__ x) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= op.total) return;
+  const int64_t a = find_ado(op, g);
+  const int d = op.d, DD = d * d, s = op.sec[a];
+  const int64_t o = op.off[a];
+  const int e = op.sec_elems[s * DD + (int)(g - o)];
+  const int i = e % d, j = e / d;
+  const int ni = jc.gn[i], nj = jc.gn[j];
+  const double2* tv = reinterpret_cast<const double2*>(t);
+  const double2* uv = reinterpret_cast<const double2*>(jc.u);
+  double ar = 0.0, ai = 0.0;
+  for (int p = 0; p < ni; ++p) {
+    const int pp = jc.grp[i * jc.gmax + p];
+    const double2 uip = uv[i + pp * d];
+    double br = 0.0, bi = 0.0;                    // sum_q t_pq conj(U_jq)
+    for (int q = 0; q < nj; ++q) {
+      const int qq = jc.grp[j * jc.gmax + q];
+      const double2 v = tv[o + op.sec_pos[s * DD + pp + qq * d]];
+      const double2 ujq = uv[j + qq * d];
+      br += v.x * ujq.x + v.y * ujq.y;
+      bi += v.y * ujq.x - v.x * ujq.y;
+    }
+    ar += uip.x * br - uip.y * bi;
+    ai += uip.x * bi + uip.y * br;
+  }
+  x[2 * g] += ar;
+  x[2 * g + 1] += ai;
+}
+
 template <typename TP, typename TD>
 __global__ void k_unpack(const DevPacked op, const TP* __restrict__ packed, TD* __restrict__ dense) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -502,6 +571,18 @@ hipError_t launch_unpack(const DevPacked& op, const TP* packed, TD* dense, hipSt
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || op.total <= 0) return e;
   hipLaunchKernelGGL((k_unpack<TP, TD>), dim3((unsigned)((op.total + 255) / 256)), dim3(256), 0, s, op, packed, dense);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_fwd(const DevPacked& op, const DevJacobi& jc, const double* r, double* t, hipStream_t s) {
+  if (op.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jacobi_fwd, dim3((unsigned)((op.total + 255) / 256)), dim3(256), 0, s, op, jc, r, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_back(const DevPacked& op, const DevJacobi& jc, const double* t, double* x, hipStream_t s) {
+  if (op.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jacobi_back, dim3((unsigned)((op.total + 255) / 256)), dim3(256), 0, s, op, jc, t, x);
   return hipGetLastError();
 }
 

@@ -13,6 +13,8 @@
 !                                             rhotmp -> rhorhs, registered once)
 !   solve_tfqmr_gpu(rho, crit, maxit, info)   solver-level replacement of solve_tfqmr
 !   solve_bicg_gpu(rho, crit, maxit, info)    solver-level replacement of solvebicg
+!   calc_ax_jacobi_omp(dim0, rcgtmp, rcgrhs)  jacobi/calc_ax_jacobi_omp.f90:1,15-17
+!   solve_jacobi_gpu(rho, crit, maxit, info)  solver-level replacement of solve_jacobi
 !
 ! heomq_fortran_init uploads the reference's own module arrays once (after
 ! buildcoefindex), as heomq_model_from_reference + heomq_ctx_create.  Errors
@@ -77,6 +79,22 @@ module heomq_iface
     end function
 
     integer(c_int) function c_solve_bicg_host(ctx, rho, crit, maxit, stats) bind(C, name='heomq_solve_bicg_host')
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex) :: rho(*)
+      real(c_double), value :: crit
+      integer(c_int), value :: maxit
+      real(c_double) :: stats(*)
+    end function
+
+    integer(c_int) function c_calc_ax_jacobi(ctx, dim0, x, y) bind(C, name='heomq_calc_ax_jacobi')
+      import :: c_int, c_int32_t, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: dim0
+      complex(c_double_complex) :: x(*), y(*)
+    end function
+
+    integer(c_int) function c_solve_jacobi_host(ctx, rho, crit, maxit, stats) bind(C, name='heomq_solve_jacobi_host')
       import :: c_int, c_ptr, c_double, c_double_complex
       type(c_ptr), value :: ctx
       complex(c_double_complex) :: rho(*)
@@ -245,3 +263,36 @@ subroutine solve_bicg_gpu(rho, crit, maxit, info)
   info(2) = int(stats(1))
   if (rc /= 0 .and. rc /= 4) call heomq_fail('solve_bicg_gpu', rc)
 end subroutine solve_bicg_gpu
+
+! jacobi/calc_ax_jacobi_omp.f90:1,15-17 -- one block-Jacobi sweep: rcgrhs is
+! the new iterate from rcgtmp (default shift of prelude_jacobi.f90:84-113)
+subroutine calc_ax_jacobi_omp(dim0, rcgtmp, rcgrhs)
+  use heomq_iface
+  implicit none
+  integer, intent(in) :: dim0
+  complex*16, intent(in) :: rcgtmp(*)
+  complex*16, intent(out) :: rcgrhs(*)
+  integer(c_int) :: rc
+  rc = c_calc_ax_jacobi(heomq_ctx_h, int(dim0, c_int32_t), rcgtmp, rcgrhs)
+  if (rc /= 0) call heomq_fail('calc_ax_jacobi_omp', rc)
+end subroutine calc_ax_jacobi_omp
+
+! solve_jacobi on the device (jacobi/solve_jacobi.f90): rho (the full
+! hierarchy, reference layout) holds the start on entry -- its rho0 made
+! Hermitian from the lower triangle and required to have unit trace -- and
+! the last iterate on return.  info(1) = status (0 converged, 4 not
+! converged), info(2) = sweeps.
+subroutine solve_jacobi_gpu(rho, crit, maxit, info)
+  use heomq_iface
+  implicit none
+  complex*16, intent(inout) :: rho(*)
+  real*8, intent(in) :: crit
+  integer, intent(in) :: maxit
+  integer, intent(out) :: info(2)
+  real(c_double) :: stats(8)
+  integer(c_int) :: rc
+  rc = c_solve_jacobi_host(heomq_ctx_h, rho, real(crit, c_double), int(maxit, c_int), stats)
+  info(1) = int(rc)
+  info(2) = int(stats(1))
+  if (rc /= 0 .and. rc /= 4) call heomq_fail('solve_jacobi_gpu', rc)
+end subroutine solve_jacobi_gpu

@@ -261,6 +261,16 @@ class Model:
                 "light_blocks", "max_ados_per_light_block", "empty_ados", "max_light_couplings", "map_entries")
         return dict(zip(keys, (int(v) for v in info)))
 
+    def jacobi_info(self):
+        """(info dict, eigenvalues of H, eigenvectors of H) of the block-Jacobi
+        iteration, built on the host (prelude_jacobi.f90:43-113)."""
+        info = np.zeros(5)
+        ev = np.zeros(self.d)
+        u = np.zeros(self.d * self.d, np.complex128)
+        check(lib().heomq_model_jacobi_info(self._h, _dp(info), _dp(ev), _dp(u.view(np.float64))))
+        keys = ("ejac", "gap", "cb_term", "nnzjac", "group_max")
+        return dict(zip(keys, (float(v) for v in info))), ev, u.reshape(self.d, self.d, order="F")
+
     def n_lower_total(self) -> int:
         return self.n_lower
 
@@ -424,6 +434,33 @@ class Context:
         rc = lib().heomq_solve_bicg(self._h, self._check_vec(x, t.complex128), crit, maxiter, _dp(stats),
                                     self._stream())
         info = dict(iters=int(stats[0]), applies=int(stats[1]), resid=stats[2], seconds=stats[3], status=rc)
+        if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
+            check(rc)
+        return x, info
+
+    def jacobi_info(self) -> dict:
+        info = np.zeros(5)
+        check(lib().heomq_jacobi_info(self._h, _dp(info)))
+        return dict(zip(("ejac", "gap", "cb_term", "nnzjac", "group_max"), (float(v) for v in info)))
+
+    def jacobi_step(self, x, y=None, ejac=0.0):
+        """One block-Jacobi sweep (calc_ax_jacobi_omp): y = new iterate from x."""
+        t = self.torch
+        y = self.zeros() if y is None else y
+        check(lib().heomq_jacobi_step(self._h, self._check_vec(x, t.complex128), self._check_vec(y, t.complex128),
+                                      float(ejac), self._stream()))
+        return y
+
+    def solve_jacobi(self, x=None, crit=1e-8, maxiter=20000, ejac=0.0, raise_on_fail=True):
+        """Steady state by the block-Jacobi iteration (solve_jacobi.f90: absolute
+        trace-row residual <= crit, checked every 50 sweeps)."""
+        t = self.torch
+        x = self.initial_guess() if x is None else x
+        stats = np.zeros(8)
+        rc = lib().heomq_solve_jacobi(self._h, self._check_vec(x, t.complex128), crit, maxiter, float(ejac),
+                                      _dp(stats), self._stream())
+        info = dict(iters=int(stats[0]), applies=int(stats[1]), resid=stats[2], seconds=stats[3], ejac=stats[4],
+                    best=stats[5], status=rc)
         if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
             check(rc)
         return x, info

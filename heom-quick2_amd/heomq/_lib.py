@@ -79,6 +79,12 @@ SIGNATURES = {
     "heomq_calc_atrx_bicg": (C.c_int, [P, C.c_int32, D, D]),
     "heomq_solve_bicg": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_solve_bicg_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),
+    "heomq_model_jacobi_info": (C.c_int, [P, D, D, D]),
+    "heomq_jacobi_info": (C.c_int, [P, D]),
+    "heomq_jacobi_step": (C.c_int, [P, P, P, C.c_double, P]),
+    "heomq_calc_ax_jacobi": (C.c_int, [P, C.c_int32, D, D]),
+    "heomq_solve_jacobi": (C.c_int, [P, P, C.c_double, C.c_int, C.c_double, D, P]),
+    "heomq_solve_jacobi_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),
     "heomq_solve_steady": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, D, P]),
     "heomq_solve_steady_mixed": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_int, C.c_int, D, P]),
     "heomq_solve_steady_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),

@@ -232,6 +232,39 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
 int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* stats, void* stream);
 /* The same on a host array (the solvebicg drop-in of the Fortran layer). */
 int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats);
+/*
+ * Block-Jacobi iteration (jacobi/solve_jacobi.f90, calc_ax_jacobi_omp.f90,
+ * prelude_jacobi.f90; J. Chem. Phys. 147, 044105 (2017), Eq. 5).  One sweep
+ * replaces every ADO K by (ejac - S_K)^-1 (ejac x_K + C_K(x)), S_K X =
+ * -i[H, X] + gamma_K X the same-tier operator and C_K the tier couplings,
+ * computed on the symmetry-packed layout with S_K diagonalised through the
+ * eigenvectors of H.  ejac <= 0 selects the reference's default shift
+ * (prelude_jacobi.f90:84-113): max(spread of the nonzero eigenvalues of
+ * [H, .], sqrt((ntier - 1) * max |cb|)).
+ */
+/* info[0] default ejac, [1] eigenvalue spread of [H, .], [2] sqrt((ntier-1)
+ * max|cb|), [3] nonzero eigenvalues of [H, .] (nnzjac), [4] largest group of
+ * Fock states H couples.  evals (d) and evecs (d*d complex, column-major) may
+ * be NULL.  Host only: needs no GPU.  Replaces prelude_jacobi.f90:43-113. */
+int heomq_model_jacobi_info(const heomq_model* m, double* info, double* evals, double* evecs);
+/* The same five numbers for a context's current parameters. */
+int heomq_jacobi_info(const heomq_ctx* c, double* info);
+/* y = one sweep of x (dense device vectors, N*d*d complex each): the operator
+ * calc_ax_jacobi_omp(dim0, rcgtmp, rcgrhs) applies (calc_ax_jacobi_omp.f90:1-324). */
+int heomq_jacobi_step(heomq_ctx* c, const double* x, double* y, double ejac, void* stream);
+/* calc_ax_jacobi_omp on host arrays (the Fortran drop-in), default ejac. */
+int heomq_calc_ax_jacobi(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_host);
+/* solve_jacobi on the device: rho0 of x is made Hermitian from its lower
+ * triangle and must have unit trace (else HEOMQ_BAD_ARG, as
+ * solve_jacobi.f90:70-85 stops); sweeps until ||A x - e||_2 (the
+ * calc_ax_tfqmr_omp trace row, last ADO), checked after sweeps 1, 51, 101, ...
+ * (:183-229), is <= crit or maxiter sweeps are done.  x returns the last
+ * iterate.  stats (6): sweeps, applies, final residual, seconds, ejac used,
+ * smallest residual seen. */
+int heomq_solve_jacobi(heomq_ctx* c, double* x, double crit, int maxiter, double ejac, double* stats,
+                       void* stream);
+/* The same on a host array (solve_jacobi_gpu of the Fortran layer). */
+int heomq_solve_jacobi_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats);
 /* Host-pointer steady state (the solve_tfqmr drop-in of the Fortran shim):
  * rho_host (d, d, N) complex holds the initial guess on entry (a trace-one
  * tier-0 block, Hermitian as tfqmr_double/solve_tfqmr.f90:97-103 makes it)

@@ -1002,6 +1002,109 @@ def chebyshev(model: Model, x, dt, nstep):
 
 
 # --------------------------------------------------------------------------
+# Block-Jacobi iteration (jacobi/prelude_jacobi.f90, calc_ax_jacobi_omp.f90,
+# solve_jacobi.f90; J. Chem. Phys. 147, 044105 (2017), Eq. 5)
+# --------------------------------------------------------------------------
+def liouville_commutator(model: Model):
+    """d^2 x d^2 matrix of X -> HX - XH on column-major vec(X): zhil2liou('l')
+    minus zhil2liou('r') (prelude_jacobi.f90:43-45)."""
+    d = model.d
+    return np.kron(np.eye(d), model.H) - np.kron(model.H.T, np.eye(d))
+
+
+def jacobi_ejac(model: Model):
+    """Default shift (prelude_jacobi.f90:48-113): the eigenvalues of the
+    Hermitian commutator matrix with |lambda| > dpico = 1e-12 (:62-70), their
+    spread, sqrt((ntier - 1) * max |cb|) (:88-105), and the larger of the two.
+    Returns (ejac, spread, cb_term, number of nonzero eigenvalues)."""
+    lam = np.linalg.eigvalsh(liouville_commutator(model))
+    nz = lam[np.abs(lam) > 1e-12]
+    gap = float(nz.max() - nz.min()) if nz.size else 0.0
+    cbt = math.sqrt(model.L * float(np.abs(model.cb).max()))
+    return max(gap, cbt), gap, cbt, int(nz.size)
+
+
+def same_tier_matrix(model: Model, gamma):
+    """S_K on column-major vec(X): -i[H, X] + gamma_K X (calc_ax_tfqmr_omp.f90:180-203)."""
+    return -1j * liouville_commutator(model) + gamma * np.eye(model.d ** 2)
+
+
+def _jacobi_blocks(model: Model, ejac):
+    """Per-ADO (S_K, (ejac - S_K)^-1), shared between ADOs of equal gamma_K."""
+    key = ("_jacobi_blocks", ejac)
+    cached = getattr(model, "_jac_cache", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    d = model.d
+    gam = ado_gamma(model)
+    mats, blocks = {}, []
+    for K in range(model.N):
+        g = complex(gam[K])
+        if abs(ejac - g) <= 1e-12:
+            raise ValueError("ejac equals an ADO's gamma_K")
+        if g not in mats:
+            S = same_tier_matrix(model, g)
+            mats[g] = (S, np.linalg.inv(ejac * np.eye(d * d) - S))
+        blocks.append(mats[g])
+    model._jac_cache = (key, blocks)
+    return blocks
+
+
+def jacobi_sweep(model: Model, x, ejac):
+    """One sweep, calc_ax_jacobi_omp.f90: per ADO K the right-hand side
+    R_K = ejac x_K + C_K(x) (:63-88 tier 0, :130-272 higher tiers; C_K the
+    lower- and upper-tier couplings, here (A_0 x)_K - S_K x_K with A_0 the
+    apply without a trace row), then x_K <- (ejac - S_K)^-1 R_K (:95-108,
+    :276-298, there through the eigenvectors of [H, .]; here a dense solve).
+    Rates within 1e-12 of ejac are not supported (the reference's
+    1/(i lambda) branch, :285-287)."""
+    d = model.d
+    ax = apply(model, x, trace_row=None)
+    blocks = _jacobi_blocks(model, ejac)
+    out = np.empty_like(x)
+    for K in range(model.N):
+        S, Minv = blocks[K]
+        xv = x[K].reshape(-1, order="F")
+        cK = ax[K].reshape(-1, order="F") - S @ xv
+        out[K] = (Minv @ (ejac * xv + cK)).reshape(d, d, order="F")
+    return out
+
+
+def jacobi_residual(model: Model, x):
+    """||A x - e||_2 on the (Re, Im) vector with calc_ax_tfqmr_omp's trace row
+    (solve_jacobi.f90:186-190: the real part of the last element minus 1)."""
+    r = apply(model, x, trace_row="last")
+    r[-1, model.d - 1, model.d - 1] -= 1.0
+    return float(np.linalg.norm(as_real(r)))
+
+
+def jacobi_solve(model: Model, x0, crit=1e-10, maxiter=20000, ejac=None):
+    """solve_jacobi.f90: rho0 made Hermitian from its lower triangle with a real
+    diagonal (:70-76) and required to have unit trace (:77-85); sweeps with the
+    residual checked after sweeps 1, 51, 101, ... (:183-229) until it is
+    <= crit or maxiter.  Returns (x, sweeps, residual)."""
+    x = x0.copy()
+    x[0] = _herm_from_lower(x[0])
+    if abs(np.trace(x[0]) - 1.0) > 1e-12:
+        raise ValueError("rho is not normalized")
+    ejac = jacobi_ejac(model)[0] if ejac is None else ejac
+    res = jacobi_residual(model, x)
+    if res <= crit:
+        return x, 0, res
+    it = 0
+    while True:
+        x = jacobi_sweep(model, x, ejac)
+        if it % 50 == 0:
+            res = jacobi_residual(model, x)
+            if res <= crit:
+                break
+        if it >= maxiter:
+            break
+        it += 1
+    return x, it, res
+
+
+# --------------------------------------------------------------------------
 # Closed form at U = 0: Landauer with the same Lorentzian and Pade Fermi function
 # --------------------------------------------------------------------------
 def landauer_siam(p: Params, npts=400001):

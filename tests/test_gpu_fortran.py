@@ -1,9 +1,9 @@
 """The Fortran drop-in layer end to end on the GPU.
 
 libheomq_fortran.so exports the reference's own subroutine names
-(calc_ax_tfqmr_omp, calc_ax_bicg_omp, calc_atrx_bicg_omp, calcderiv_omp) plus
-the one-time loader heomq_fortran_init and the solver-level solve_tfqmr_gpu
-and solve_bicg_gpu.  This test
+(calc_ax_tfqmr_omp, calc_ax_bicg_omp, calc_atrx_bicg_omp, calc_ax_jacobi_omp,
+calcderiv_omp) plus the one-time loader heomq_fortran_init and the
+solver-level solve_tfqmr_gpu, solve_bicg_gpu and solve_jacobi_gpu.  This test
 calls them exactly as a Fortran caller would (every argument by reference,
 arrays in the reference's module layout, 1-based tables from
 tests/ref_arrays.py) and checks the results against the oracle.
@@ -138,3 +138,26 @@ def test_solve_tfqmr_gpu():
     xo, it, relres = O.steady_state(o, tol=1e-12)
     assert np.abs(xs[0] - xo[0]).max() <= 1e-10
     assert abs(np.trace(xs[0]) - 1.0) <= 1e-12
+
+
+def test_calc_ax_jacobi_omp():
+    h, o = shim()
+    x = O.physical_projection(o, O.seeded_ado_vector(o, 35))
+    yo = O.jacobi_sweep(o, x, O.jacobi_ejac(o)[0])
+    assert rel(call_apply(h.calc_ax_jacobi_omp_, o, x), yo) <= 1e-12
+
+
+def test_solve_jacobi_gpu():
+    h, o = shim()
+    rho = np.zeros((o.N, o.d, o.d), complex)
+    rho[0] = np.eye(o.d) / o.d
+    rf = O.to_fortran(rho).ravel().copy()
+    crit, maxit = C.c_double(1e-10), C.c_int(5000)
+    info = (C.c_int * 2)()
+    h.solve_jacobi_gpu_(rf.ctypes.data_as(C.c_void_p), C.byref(crit), C.byref(maxit), info)
+    print("solve_jacobi_gpu info", list(info), flush=True)
+    assert info[0] == 0
+    xs = O.from_fortran(rf.reshape(o.N, o.d, o.d))
+    assert O.jacobi_residual(o, xs) <= 1e-10
+    xo, it, res = O.steady_state(o, tol=1e-13)
+    assert np.abs(xs[0] - xo[0]).max() <= 1e-9

@@ -36,7 +36,8 @@ def test_fortran_shim_exports_reference_names():
         pytest.skip("Fortran compiler not available at build time")
     h = ctypes.CDLL(str(path))
     for name in ("calc_ax_tfqmr_omp_", "calc_ax_bicg_omp_", "calc_atrx_bicg_omp_", "calcderiv_omp_",
-                 "heomq_fortran_init_", "solve_tfqmr_gpu_", "solve_bicg_gpu_"):
+                 "heomq_fortran_init_", "solve_tfqmr_gpu_", "solve_bicg_gpu_", "calc_ax_jacobi_omp_",
+                 "solve_jacobi_gpu_"):
         assert hasattr(h, name), name
 
 
@@ -166,3 +167,28 @@ def test_packed_layout_benchmark_sizes():
     assert lay["packed"] == 1797356 and lay["max_light_couplings"] == 3
     lay = heomq.Model(heomq.siam()).packed_layout()
     assert lay["packed"] == 37012 and lay["empty_ados"] > 0 and lay["max_ados_per_light_block"] <= 257
+
+
+@pytest.mark.parametrize("pp,op", [(heomq.siam(nk=2, ntop=3, V=0.3, U=1.5), O.siam_params(nk=2, ntop=3, V=0.3, U=1.5)),
+                                   (heomq.two_impurity(nk=2, ntop=2), O.dqd_params(nk=2, ntop=2)),
+                                   (heomq.two_impurity(), None)])
+def test_jacobi_eigensystem_host(pp, op):
+    """The block-Jacobi eigensystem built on the host: H U = U diag(E), U unitary
+    and block-diagonal over the conserved groups, and the default shift equals
+    the oracle's (eigenvalues of the d^2 x d^2 commutator matrix)."""
+    m = heomq.Model(pp)
+    same = op is not None          # the benchmark-size model: only its H is rebuilt in the oracle
+    o = O.Model(op if same else O.dqd_params(nk=1, ntop=1))
+    info, E, U = m.jacobi_info()
+    H = o.H
+    assert np.abs(H @ U - U * E[None, :]).max() <= 1e-13 * max(np.abs(H).max(), 1.0)
+    assert np.abs(U.conj().T @ U - np.eye(o.d)).max() <= 1e-13
+    N, S2 = O.fock_quantum_numbers(o)
+    nz = np.abs(U) > 0
+    assert not np.any(nz & ((N[:, None] != N[None, :]) | (S2[:, None] != S2[None, :])))
+    ej, gap, cbt, nnz = O.jacobi_ejac(o)
+    assert abs(info["gap"] - gap) <= 1e-12 * max(gap, 1.0)
+    assert int(info["nnzjac"]) == nnz
+    if same:
+        assert abs(info["ejac"] - ej) <= 1e-12 * ej
+        assert abs(info["cb_term"] - cbt) <= 1e-12 * cbt

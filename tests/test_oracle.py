@@ -361,3 +361,56 @@ def test_c_port_matches_numpy_oracle(p):
     for tr in ("last", "rdo", None):
         y, yo = port.apply(x, tr), O.apply(o, x, trace_row=tr)
         assert np.abs(y - yo).max() <= 1e-14 * np.abs(yo).max()
+
+
+# ---- block-Jacobi iteration (jacobi/) ----
+@pytest.mark.parametrize("p", [O.siam_params(nk=2, ntop=2, V=0.3, U=1.5), O.dqd_params(nk=1, ntop=2)])
+def test_jacobi_ejac_closed_form(p):
+    # the eigenvalues of X -> HX - XH are E_i - E_j (E: eigenvalues of H), so
+    # the spread of the nonzero ones is 2 (E_max - E_min)
+    o = O.Model(p)
+    ej, gap, cbt, nnz = O.jacobi_ejac(o)
+    E = np.linalg.eigvalsh(o.H)
+    lam = (E[:, None] - E[None, :]).ravel()
+    assert abs(gap - 2 * (E.max() - E.min())) <= 1e-12 * gap
+    assert nnz == int((np.abs(lam) > 1e-12).sum())
+    assert abs(cbt - np.sqrt(o.L * np.abs(o.cb).max())) <= 1e-15 * cbt
+    assert ej == max(gap, cbt)
+
+
+@pytest.mark.parametrize("p", [O.siam_params(nk=2, ntop=2, V=0.3, U=1.5), O.dqd_params(nk=1, ntop=2)])
+def test_jacobi_same_tier_block_is_the_applys_diagonal_block(p):
+    # S_K is the block of the apply that maps ADO K to itself: apply a vector
+    # supported on ADO K alone and read ADO K back
+    o = O.Model(p)
+    gam = O.ado_gamma(o)
+    rng = np.random.default_rng(5)
+    for K in [0, 1, int(o.first[2]), o.N - 1]:
+        x = np.zeros((o.N, o.d, o.d), complex)
+        x[K] = rng.standard_normal((o.d, o.d)) + 1j * rng.standard_normal((o.d, o.d))
+        y = O.apply(o, x, trace_row=None)[K]
+        S = O.same_tier_matrix(o, gam[K])
+        assert np.abs(y.reshape(-1, order="F") - S @ x[K].reshape(-1, order="F")).max() <= 1e-12 * np.abs(y).max()
+
+
+def test_jacobi_sweep_fixed_point_and_convergence():
+    # the steady state is a fixed point of a sweep, and the iteration reaches
+    # it from the reference's start (identity / d on tier 0)
+    o = O.Model(O.siam_params(nk=2, ntop=2, V=0.3, U=1.5))
+    xs, _, _ = O.steady_state(o, tol=1e-13)
+    ej = O.jacobi_ejac(o)[0]
+    assert np.abs(O.jacobi_sweep(o, xs, ej) - xs).max() <= 1e-10 * np.abs(xs).max()
+    x0 = np.zeros((o.N, o.d, o.d), complex)
+    x0[0] = np.eye(o.d) / o.d
+    x, it, res = O.jacobi_solve(o, x0, crit=1e-10, maxiter=2000)
+    assert res <= 1e-10 and it < 2000
+    assert np.abs(x[0] - xs[0]).max() <= 1e-9
+
+
+def test_jacobi_sweep_shift_independent_fixed_point():
+    # a different shift changes the iterates, never the fixed point: a sweep
+    # with a wrong sign of the commutator or of gamma_K in S_K would move it
+    o = O.Model(O.dqd_params(nk=1, ntop=2))
+    xs, _, _ = O.steady_state(o, tol=1e-13)
+    for ej in (0.5, 3.0):
+        assert np.abs(O.jacobi_sweep(o, xs, ej) - xs).max() <= 1e-10 * np.abs(xs).max()
