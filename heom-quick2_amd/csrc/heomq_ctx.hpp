@@ -116,16 +116,21 @@ struct heomq_ctx {
   };
   std::vector<ChunkGraph> graphs;
   bool use_graphs = true;            // false: launch the chunk's kernels one by one
-  // Jacobi scaling of forward steady-state solves: -1 automatic (on when H is
-  // diagonal in the occupation basis), 0 off, 1 on
+  // left preconditioning of forward steady-state solves: -1 automatic (Jacobi
+  // scaling when H is diagonal in the occupation basis, block Jacobi
+  // otherwise), 0 none, 1 Jacobi scaling, 2 block Jacobi
   int precond = -1;
   bool h_diagonal = false;
   // block-Jacobi iteration (jacobi/): eigensystem of H per conserved group of
   // Fock states, the default shift, per-ADO rates on the host (for the shift's
   // degeneracy check) and the iteration's own workspace
   DevBuf j_u, j_e, j_grp, j_gn, ws_jac;
+  DevBuf j_trow, j_tslot, j_tcf, j_tcb, j_tde;   // slot-space term tables
   heomq::DevJacobi pjac;
   double ejac_default = 0.0, ejac_gap = 0.0, ejac_cb = 0.0;
+  // shift of the block-Jacobi preconditioner of TFQMR: (shift - S_K)^-1 per
+  // ADO; prec_shift_user 0 selects kPrecShiftFactor * ejac_default
+  double prec_shift = 0.0, prec_shift_user = 0.0;
   int jac_nnz = 0;
   std::vector<double> gam_h;
   void drop_graphs() {
@@ -152,4 +157,6 @@ void build_gamma(const Model& m, const std::vector<cplx>& gdraw, std::vector<dou
 // the default shift of jacobi/prelude_jacobi.f90; run whenever H or the bath
 // coefficients change.
 int build_jacobi(heomq_ctx* c, const Model& m);
+// default block-Jacobi preconditioner shift, as a multiple of the iteration's ejac
+constexpr double kPrecShiftFactor = 0.05;
 }  // namespace heomq

@@ -73,6 +73,8 @@ struct JacobiHost {
   std::vector<double> u, e;   // 2*d*d (column-major U) and d
   std::vector<int16_t> grp;   // [d][gmax]
   std::vector<int8_t> gn;
+  std::vector<std::vector<int>> groups;
+  std::vector<int> gid;       // group of each state
   int gmax = 0, nnz = 0;
   double gap = 0.0, cbterm = 0.0, ejac = 0.0;
 };
@@ -116,6 +118,8 @@ int build_jacobi_host(const Model& m, JacobiHost& jh) {
       if (key[k] == key[i]) { gid[k] = (int)groups.size(); g.push_back(k); }
     groups.push_back(g);
   }
+  jh.groups = groups;
+  jh.gid = gid;
   jh.gmax = 0;
   for (auto& g : groups) jh.gmax = std::max<int>(jh.gmax, (int)g.size());
   if (jh.gmax > 64) return fail(HEOMQ_UNSUPPORTED, "block-Jacobi: an H group is larger than 64 states");
@@ -187,8 +191,12 @@ int sweep(heomq_ctx* c, const DevJacobi& jc, double* x, double* r, double* t, hi
   DevPacked op = c->pfwd;
   op.precond = 0;
   HIPCHECK(launch_apply_packed_f64(op, x, r, false, st));
+  if (jc.windowed) {
+    HIPCHECK(launch_jacobi_win(op, jc, r, x, true, st));
+    return HEOMQ_OK;
+  }
   HIPCHECK(launch_jacobi_fwd(op, jc, r, t, st));
-  HIPCHECK(launch_jacobi_back(op, jc, t, x, st));
+  HIPCHECK(launch_jacobi_back(op, jc, t, x, true, st));
   return HEOMQ_OK;
 }
 
@@ -251,8 +259,66 @@ int build_jacobi(heomq_ctx* c, const Model& m) {
   c->pjac.grp = c->j_grp.as<int16_t>();
   c->pjac.gn = c->j_gn.as<int8_t>();
   c->pjac.gmax = jh.gmax;
+  // slot-space term tables of the transform (per sector and slot)
+  {
+    const int S = c->nsec;
+    std::vector<int32_t> trow((size_t)S * (dd + 1), 0);
+    std::vector<int16_t> tslot;
+    std::vector<double> tcf, tcb, tde((size_t)S * dd, 0.0);
+    auto U = [&](int a, int b) { return cplx(jh.u[2 * (a + (size_t)b * d)], jh.u[2 * (a + (size_t)b * d) + 1]); };
+    for (int s = 0; s < S; ++s) {
+      for (int t = 0; t < dd; ++t) {
+        trow[(size_t)s * (dd + 1) + t] = (int32_t)tslot.size();
+        if (t >= c->sec_nnz_h[s]) continue;
+        const int e = c->sec_elems_h[(size_t)s * dd + t], i = e % d, j = e / d;
+        tde[(size_t)s * dd + t] = jh.e[i] - jh.e[j];
+        for (int p = 0; p < jh.gn[i]; ++p)
+          for (int q = 0; q < jh.gn[j]; ++q) {
+            const int k = jh.grp[(size_t)i * jh.gmax + p], l = jh.grp[(size_t)j * jh.gmax + q];
+            const cplx f = std::conj(U(k, i)) * U(l, j), b = U(i, k) * std::conj(U(j, l));
+            tslot.push_back(c->sec_pos_h[(size_t)s * dd + k + l * d]);
+            tcf.push_back(f.real());
+            tcf.push_back(f.imag());
+            tcb.push_back(b.real());
+            tcb.push_back(b.imag());
+          }
+      }
+      trow[(size_t)s * (dd + 1) + dd] = (int32_t)tslot.size();
+    }
+    if (tslot.empty()) tslot.push_back(0), tcf.assign(2, 0.0), tcb.assign(2, 0.0);
+    HIPCHECK(upload(c->j_trow, trow));
+    HIPCHECK(upload(c->j_tslot, tslot));
+    HIPCHECK(upload(c->j_tcf, tcf));
+    HIPCHECK(upload(c->j_tcb, tcb));
+    HIPCHECK(upload(c->j_tde, tde));
+    c->pjac.trow = c->j_trow.as<int32_t>();
+    c->pjac.tslot = c->j_tslot.as<int16_t>();
+    c->pjac.tcf = c->j_tcf.as<double>();
+    c->pjac.tcb = c->j_tcb.as<double>();
+    c->pjac.tde = c->j_tde.as<double>();
+  }
+  // the one-kernel form: every 256-element block of the packed vector must lie
+  // in ADOs that fit its LDS window
+  {
+    const int64_t total = c->poff_h[m.N];
+    const int64_t nblk = (total + 255) / 256;
+    bool fits = total > 0;
+    int64_t a = 0;
+    std::vector<int64_t> blk_ado((size_t)nblk + 1, 0);
+    for (int64_t bb = 0; bb <= nblk && fits; ++bb) {
+      const int64_t t = std::min<int64_t>(bb * 256, total - 1);
+      while (a + 1 < m.N && c->poff_h[a + 1] <= t) ++a;
+      blk_ado[bb] = a;
+    }
+    for (int64_t bb = 0; bb < nblk && fits; ++bb) {
+      const int64_t lo = blk_ado[bb], hi = blk_ado[bb + 1];
+      if (hi - lo + 1 > 257 || c->poff_h[hi + 1] - c->poff_h[lo] > kJacobiWindow) fits = false;
+    }
+    c->pjac.windowed = fits ? 1 : 0;
+  }
   c->pjac.ejac = jh.ejac;
   c->ejac_default = jh.ejac;
+  c->prec_shift = c->prec_shift_user > 0.0 ? c->prec_shift_user : kPrecShiftFactor * jh.ejac;
   c->ejac_gap = jh.gap;
   c->ejac_cb = jh.cbterm;
   c->jac_nnz = jh.nnz;

@@ -183,11 +183,30 @@ struct DevJacobi {
   const int8_t* gn = nullptr;       // group size of each state
   int gmax = 0;
   double ejac = 0.0;                // the iteration's shift (> 0)
+  // 1: every 256-element block of the packed layout lies in ADOs spanning at
+  // most kJacobiWindow elements and 257 ADOs, so the one-kernel form applies
+  int windowed = 0;
+  // slot-space form of the transform, per sector s and slot t (element (i, j)):
+  // terms [trow[s][t], trow[s][t+1]) over (k, l) in group(i) x group(j), each
+  // the slot of (k, l) with the forward coefficient conj(U_ki) U_lj and the
+  // back coefficient U_ik conj(U_jl); tde[s][t] = E_i - E_j
+  const int32_t* trow = nullptr;    // [S][d*d + 1]
+  const int16_t* tslot = nullptr;
+  const double* tcf = nullptr;      // complex
+  const double* tcb = nullptr;      // complex
+  const double* tde = nullptr;      // [S][d*d]
 };
 
 // t = (U^H r U)_ij / (ejac - gamma_K + i (E_i - E_j)) on the packed layout
 hipError_t launch_jacobi_fwd(const DevPacked& op, const DevJacobi& jc, const double* r, double* t, hipStream_t s);
-// x += U t U^H on the packed layout
-hipError_t launch_jacobi_back(const DevPacked& op, const DevJacobi& jc, const double* t, double* x, hipStream_t s);
+// x += U t U^H (accumulate) or x = U t U^H on the packed layout
+hipError_t launch_jacobi_back(const DevPacked& op, const DevJacobi& jc, const double* t, double* x, bool accumulate,
+                              hipStream_t s);
+// The whole same-tier inverse in one kernel (jc.windowed): out (+)= U [(U^H in U) / w] U^H
+// per ADO; in and out must differ.  Without accumulate the tier-0 block of out
+// is made exactly Hermitian.
+hipError_t launch_jacobi_win(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
+                             hipStream_t s);
+constexpr int kJacobiWindow = 1024;   // LDS window of the one-kernel form (complex elements)
 
 }  // namespace heomq

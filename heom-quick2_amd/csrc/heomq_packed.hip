@@ -468,18 +468,12 @@ __global__ void k_jacobi_fwd(const DevPacked op, const DevJacobi jc, const doubl
   t[2 * g + 1] = (ai * wr - ar * wi) / den;
 }
 
-// Back half: x_ij += sum_{p in group(i), q in group(j)} U_ip t_pq conj(U_jq).
-__global__ void k_jacobi_back(const DevPacked op, const DevJacobi jc, const double* __restrict__ t,
-                              double* __restrict__ x) {
-  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g >= op.total) return;
-  const int64_t a = find_ado(op, g);
-  const int d = op.d, DD = d * d, s = op.sec[a];
-  const int64_t o = op.off[a];
-  const int e = op.sec_elems[s * DD + (int)(g - o)];
-  const int i = e % d, j = e / d;
+// Element (i, j) of U T U^H for one ADO block of the packed vector t
+// (block at o, sector s): sum_{p in group(i), q in group(j)} U_ip t_pq conj(U_jq).
+__device__ inline double2 jacobi_back_elem(const DevPacked& op, const DevJacobi& jc, const double2* tv, int64_t o,
+                                           int s, int i, int j) {
+  const int d = op.d, DD = d * d;
   const int ni = jc.gn[i], nj = jc.gn[j];
-  const double2* tv = reinterpret_cast<const double2*>(t);
   const double2* uv = reinterpret_cast<const double2*>(jc.u);
   double ar = 0.0, ai = 0.0;
   for (int p = 0; p < ni; ++p) {
@@ -496,8 +490,141 @@ __global__ void k_jacobi_back(const DevPacked op, const DevJacobi jc, const doub
     ar += uip.x * br - uip.y * bi;
     ai += uip.x * bi + uip.y * br;
   }
-  x[2 * g] += ar;
-  x[2 * g + 1] += ai;
+  return make_double2(ar, ai);
+}
+
+// Back half: x_ij (+)= (U t U^H)_ij (accumulate: a Jacobi sweep; otherwise the
+// preconditioned vector itself, whose tier-0 block is made exactly Hermitian:
+// the physical iterates of the solvers keep a Hermitian rho0, and round-off
+// in its anti-Hermitian part is not seen by the projected operator).
+__global__ void k_jacobi_back(const DevPacked op, const DevJacobi jc, const double* __restrict__ t,
+                              double* __restrict__ x, int accumulate) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= op.total) return;
+  const int64_t a = find_ado(op, g);
+  const int d = op.d, DD = d * d, s = op.sec[a];
+  const int64_t o = op.off[a];
+  const int e = op.sec_elems[s * DD + (int)(g - o)];
+  const int i = e % d, j = e / d;
+  const double2* tv = reinterpret_cast<const double2*>(t);
+  const double2 v = jacobi_back_elem(op, jc, tv, o, s, i, j);
+  double ar = v.x, ai = v.y;
+  if (!accumulate && a == 0) {   // ADO 0 holds every (j, i) with its (i, j)
+    const double2 m = jacobi_back_elem(op, jc, tv, o, s, j, i);
+    ar = 0.5 * (ar + m.x);
+    ai = 0.5 * (ai - m.y);
+  }
+  if (accumulate) {
+    ar += x[2 * g];
+    ai += x[2 * g + 1];
+  }
+  x[2 * g] = ar;
+  x[2 * g + 1] = ai;
+}
+
+// The same-tier inverse (shift - S_K)^-1 in one kernel (calc_ax_jacobi_omp.f90:
+// 276-298 restated in the eigenbasis of H): block b owns output elements
+// [256 b, 256 b + 256); the whole ADOs they belong to are staged in LDS, the
+// forward transform (U^H X U)_pq / w_pq of every staged element goes to a
+// second LDS array, and each thread back-transforms its own element from it.
+// Without accumulate the tier-0 block of the output is made exactly
+// Hermitian (z_ij from the average of (i, j) and conj of (j, i)).  in and out
+// must not alias: neighbouring blocks read the ADOs they share.
+__device__ inline int jac_find(const int64_t* offs, int nado, int64_t g) {
+  int lo = 0, hi = nado - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_jacobi_win(const DevPacked op, const DevJacobi jc,
+                                                    const double* __restrict__ in, double* __restrict__ out,
+                                                    int accumulate) {
+  __shared__ double2 xw[kJacobiWindow];
+  __shared__ double2 tw[kJacobiWindow];
+  __shared__ int64_t offs[260];
+  __shared__ double2 gamw[260];
+  __shared__ int secw[260];
+  const int64_t e0 = (int64_t)blockIdx.x * 256;
+  const int64_t a_lo = op.block_ado[blockIdx.x], a_hi = op.block_ado[blockIdx.x + 1];
+  const int nado = (int)(a_hi - a_lo + 1);   // <= 257, checked on the host
+  for (int k = threadIdx.x; k <= nado; k += 256) {
+    offs[k] = op.off[a_lo + k];
+    if (k < nado) {
+      gamw[k] = make_double2(op.gam[2 * (a_lo + k)], op.gam[2 * (a_lo + k) + 1]);
+      secw[k] = op.sec[a_lo + k];
+    }
+  }
+  __syncthreads();
+  const int64_t w0 = offs[0];
+  const int nw = (int)(offs[nado] - w0);     // <= kJacobiWindow, checked on the host
+  const double2* xin = reinterpret_cast<const double2*>(in);
+  for (int k = threadIdx.x; k < nw; k += 256) xw[k] = xin[w0 + k];
+  __syncthreads();
+  const int DD = op.d * op.d;
+  const double2* cf = reinterpret_cast<const double2*>(jc.tcf);
+  const double2* cb = reinterpret_cast<const double2*>(jc.tcb);
+  // forward: tw = (U^H X U) / (ejac - gamma_K + i (E_i - E_j)) over the window
+  for (int k = threadIdx.x; k < nw; k += 256) {
+    const int64_t g = w0 + k;
+    const int ia = jac_find(offs, nado, g);
+    const int s = secw[ia];
+    const int base = (int)(offs[ia] - w0);
+    const int t = (int)(g - offs[ia]);
+    const int r0 = jc.trow[s * (DD + 1) + t], r1 = jc.trow[s * (DD + 1) + t + 1];
+    double ar = 0.0, ai = 0.0;
+    for (int r = r0; r < r1; ++r) {
+      const double2 c = cf[r];
+      const double2 x = xw[base + jc.tslot[r]];
+      ar += c.x * x.x - c.y * x.y;
+      ai += c.x * x.y + c.y * x.x;
+    }
+    const double2 ga = gamw[ia];
+    const double wr = jc.ejac - ga.x, wi = jc.tde[s * DD + t] - ga.y;
+    const double den = wr * wr + wi * wi;
+    tw[k] = make_double2((ar * wr + ai * wi) / den, (ai * wr - ar * wi) / den);
+  }
+  __syncthreads();
+  const int64_t g = e0 + threadIdx.x;
+  if (g >= op.total) return;
+  const int ia = jac_find(offs, nado, g);
+  const int s = secw[ia];
+  const int base = (int)(offs[ia] - w0);
+  const int t = (int)(g - offs[ia]);
+  double2 z = make_double2(0.0, 0.0);
+  {
+    const int r0 = jc.trow[s * (DD + 1) + t], r1 = jc.trow[s * (DD + 1) + t + 1];
+    for (int r = r0; r < r1; ++r) {
+      const double2 c = cb[r];
+      const double2 v = tw[base + jc.tslot[r]];
+      z.x += c.x * v.x - c.y * v.y;
+      z.y += c.x * v.y + c.y * v.x;
+    }
+  }
+  if (!accumulate && a_lo + ia == 0) {   // tier 0: average with conj of the mirror element
+    const int e = op.sec_elems[s * DD + t];
+    const int i = e % op.d, j = e / op.d;
+    const int tm = op.sec_pos[s * DD + j + i * op.d];
+    const int r0 = jc.trow[s * (DD + 1) + tm], r1 = jc.trow[s * (DD + 1) + tm + 1];
+    double mr = 0.0, mi = 0.0;
+    for (int r = r0; r < r1; ++r) {
+      const double2 c = cb[r];
+      const double2 v = tw[base + jc.tslot[r]];
+      mr += c.x * v.x - c.y * v.y;
+      mi += c.x * v.y + c.y * v.x;
+    }
+    z.x = 0.5 * (z.x + mr);
+    z.y = 0.5 * (z.y - mi);
+  }
+  double2* xo = reinterpret_cast<double2*>(out);
+  if (accumulate) {
+    const double2 w = xo[g];
+    z.x += w.x;
+    z.y += w.y;
+  }
+  xo[g] = z;
 }
 
 template <typename TP, typename TD>
@@ -580,9 +707,20 @@ hipError_t launch_jacobi_fwd(const DevPacked& op, const DevJacobi& jc, const dou
   return hipGetLastError();
 }
 
-hipError_t launch_jacobi_back(const DevPacked& op, const DevJacobi& jc, const double* t, double* x, hipStream_t s) {
+hipError_t launch_jacobi_back(const DevPacked& op, const DevJacobi& jc, const double* t, double* x, bool accumulate,
+                              hipStream_t s) {
   if (op.total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_jacobi_back, dim3((unsigned)((op.total + 255) / 256)), dim3(256), 0, s, op, jc, t, x);
+  hipLaunchKernelGGL(k_jacobi_back, dim3((unsigned)((op.total + 255) / 256)), dim3(256), 0, s, op, jc, t, x,
+                     accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_win(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
+                             hipStream_t s) {
+  if (!jc.windowed || in == out) return hipErrorInvalidValue;
+  if (op.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jacobi_win, dim3((unsigned)((op.total + 255) / 256)), dim3(256), 0, s, op, jc, in, out,
+                     accumulate ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <cstddef>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -21,20 +22,33 @@ namespace {
 
 template <typename T>
 hipError_t apply_T(heomq_ctx* c, bool trans, const T* x, T* y, int64_t tpos, hipStream_t s, bool proj);
+// prec: 0 none; 1 left Jacobi scaling inside the apply; 2 left block-Jacobi
+// preconditioning y <- (shift - S_K)^-1 y per ADO after the apply (FP64
+// forward operator only; scratch holds the eigenbasis coefficients)
 template <typename T>
-hipError_t apply_packed_T(heomq_ctx* c, bool trans, const T* x, T* y, bool rdo, hipStream_t s, bool prec = false);
+hipError_t apply_packed_T(heomq_ctx* c, bool trans, const T* x, T* y, bool rdo, hipStream_t s, int prec = 0,
+                          T* scratch = nullptr);
 template <>
 hipError_t apply_packed_T<double>(heomq_ctx* c, bool trans, const double* x, double* y, bool rdo, hipStream_t s,
-                                  bool prec) {
+                                  int prec, double* scratch) {
   DevPacked op = trans ? c->ptrn : c->pfwd;
-  op.precond = prec ? 1 : 0;
-  return launch_apply_packed_f64(op, x, y, rdo, s);
+  op.precond = prec == 1 ? 1 : 0;
+  if (prec != 2) return launch_apply_packed_f64(op, x, y, rdo, s);
+  DevJacobi jc = c->pjac;
+  jc.ejac = c->prec_shift;
+  if (jc.windowed) {   // A x into the scratch, then y = M^-1 (A x) in one kernel
+    if (hipError_t e = launch_apply_packed_f64(op, x, scratch, rdo, s)) return e;
+    return launch_jacobi_win(op, jc, scratch, y, false, s);
+  }
+  if (hipError_t e = launch_apply_packed_f64(op, x, y, rdo, s)) return e;
+  if (hipError_t e = launch_jacobi_fwd(op, jc, y, scratch, s)) return e;
+  return launch_jacobi_back(op, jc, scratch, y, false, s);
 }
 template <>
 hipError_t apply_packed_T<float>(heomq_ctx* c, bool trans, const float* x, float* y, bool rdo, hipStream_t s,
-                                 bool prec) {
+                                 int prec, float*) {
   DevPacked op = trans ? c->ptrn : c->pfwd;
-  op.precond = prec ? 1 : 0;
+  op.precond = prec == 1 ? 1 : 0;
   return launch_apply_packed_f32(op, x, y, rdo, s);
 }
 // Solvers iterate with the projected operator Pi*A (Pi: parity sectors +
@@ -63,6 +77,8 @@ template <typename T>
 struct Work {
   T *w, *y1, *y2, *u1, *u2, *v, *d;   // the Krylov state
   T *ax, *xbest;                      // true-residual product, best iterate so far
+  T* pt = nullptr;                    // block-Jacobi preconditioning: eigenbasis scratch
+  T* bp = nullptr;                    //   and the preconditioned right-hand side
 };
 constexpr int kWorkVecs = 9;
 
@@ -136,19 +152,34 @@ int chunk_graph(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, boo
 // block), the iterate's tier-0 block re-Hermitianised at every restart.
 template <typename T>
 int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool trans, int64_t tpos,
-              double tol, int maxiter, hipStream_t st, RunStats& rs, int mode, bool prec = false) {
-  // the iteration runs on the (optionally Jacobi-scaled) operator; every
-  // true residual ||b - A x|| uses the operator itself.  Left scaling keeps b
-  // only when the scaling is 1 on b's support: the callers enable it for
-  // b = e_trace, whose one element sits on the unscaled trace row.
+              double tol, int maxiter, hipStream_t st, RunStats& rs, int mode, int prec = 0) {
+  // the iteration runs on the left-preconditioned operator M^-1 A (prec 1:
+  // Jacobi scaling, 2: block Jacobi; 0: A itself); every true residual
+  // ||b - A x|| uses the operator itself.  Jacobi scaling keeps b (the
+  // callers enable it for b = e_trace, whose one element sits on the
+  // unscaled trace row); block Jacobi iterates with M^-1 b in W.bp.
   auto Ait = [&](const T* in, T* out) -> hipError_t {
-    return mode == 1 ? apply_packed_T<T>(c, trans, in, out, tpos >= 0, st, prec)
+    return mode == 1 ? apply_packed_T<T>(c, trans, in, out, tpos >= 0, st, prec, W.pt)
                      : apply_T<T>(c, trans, in, out, tpos, st, false);
   };
   auto A0 = [&](const T* in, T* out) -> hipError_t {
-    return mode == 1 ? apply_packed_T<T>(c, trans, in, out, tpos >= 0, st, false)
+    return mode == 1 ? apply_packed_T<T>(c, trans, in, out, tpos >= 0, st, 0)
                      : apply_T<T>(c, trans, in, out, tpos, st, false);
   };
+  if (prec == 2) {
+    if (mode != 1 || trans || !W.pt || !W.bp || sizeof(T) != sizeof(double))
+      return fail(HEOMQ_BAD_ARG, "block-Jacobi preconditioning needs the forward FP64 packed operator");
+    DevJacobi jc = c->pjac;
+    jc.ejac = c->prec_shift;
+    if (jc.windowed) {
+      HIPCHECK(launch_jacobi_win(c->pfwd, jc, reinterpret_cast<const double*>(b), reinterpret_cast<double*>(W.bp),
+                                 false, st));
+    } else {
+      HIPCHECK(launch_jacobi_fwd(c->pfwd, jc, reinterpret_cast<const double*>(b), reinterpret_cast<double*>(W.pt), st));
+      HIPCHECK(launch_jacobi_back(c->pfwd, jc, reinterpret_cast<const double*>(W.pt),
+                                  reinterpret_cast<double*>(W.bp), false, st));
+    }
+  }
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   double* part = c->partial.as<double>();
   double* red = c->red.as<double>();
@@ -166,12 +197,16 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
   };
   const int check_every = 8;      // iterations per graph replay / host read of the scalars
   const int verify_every = 32;    // replays between true-residual checks (256 iterations)
+  static const bool log = std::getenv("HEOMQ_SOLVER_LOG") != nullptr;
   double best = std::numeric_limits<double>::infinity();
   int base_iters = 0, stalls = 0;
   while (true) {
     if (mode == 1) HIPCHECK(launch_herm_packed<T>(c->pfwd, x, st));
     double rr = 0, bb = 0;
     if (int rc = true_resid(rr, bb)) return rc;
+    if (log)
+      std::fprintf(stderr, "tfqmr: start iters %d relres %.3e best %.3e prec %d stalls %d\n", base_iters,
+                   bb > 0 ? std::sqrt(rr / bb) : 0.0, bb > 0 ? std::sqrt(best / bb) : 0.0, prec, stalls);
     if (rr <= tol * tol * bb || rr == 0.0) return HEOMQ_OK;
     if (base_iters >= maxiter) break;
     const double best_before = best;
@@ -181,7 +216,7 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
     } else {
       if (++stalls > 3) {
         if (!prec) break;
-        prec = false;   // the scaled iteration stalled: continue unscaled from the best iterate
+        prec = 0;   // the preconditioned iteration stalled: continue unpreconditioned from the best iterate
         stalls = 0;
       }
       if (rr > 100.0 * best) {   // the recurrences diverged: restart from the best iterate
@@ -194,7 +229,7 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
       HIPCHECK(Ait(x, W.ax));
       rs.applies++;
     }
-    HIPCHECK(tfq_init_r<T>(st, n, b, W.ax, W.w, W.y1, W.d, s, part));
+    HIPCHECK(tfq_init_r<T>(st, n, prec == 2 ? W.bp : b, W.ax, W.w, W.y1, W.d, s, part));
     TfqmrScal& hs = *reinterpret_cast<TfqmrScal*>(static_cast<char*>(c->pinned) + 1024);
     if (prec) {   // the quasi-residual bound measures the scaled residual: scale the target with it
       HIPCHECK(hipMemcpyAsync(&hs, s, sizeof(TfqmrScal), hipMemcpyDeviceToHost, st));
@@ -207,6 +242,8 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
     rs.applies++;
     HIPCHECK(tfq_init_v<T>(st, n, W.u1, W.v, s, part));
     hipGraphExec_t chunk = nullptr;
+    int no_progress = 0;
+    double last_check = rr;
     if (c->use_graphs)
       if (int rc = chunk_graph<T>(c, x, b, W, n, trans, tpos, mode + 2 * (int)prec, st, Ait, check_every, &chunk))
         return rc;
@@ -220,14 +257,22 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
       rs.iters = base_iters + hs.iter;
       rs.applies += 2 * check_every;
       if (hs.flag != 0 || rs.iters >= maxiter) break;
-      if (replay % verify_every == 0) {   // converged already, or drifting away?
+      if (replay % verify_every == 0) {   // converged already, drifting away or stagnating?
         double r2 = 0, b2 = 0;
         if (int rc = true_resid(r2, b2)) return rc;
+        if (log)
+          std::fprintf(stderr, "tfqmr: verify iters %d relres %.3e\n", rs.iters, b2 > 0 ? std::sqrt(r2 / b2) : 0.0);
         if (r2 <= tol * tol * b2 || r2 > 100.0 * best) break;
+        // less than a 10 % drop of the residual norm since the last check
+        // counts as no progress; two such checks in a row restart the
+        // recurrences (fresh shadow vector) from the best iterate
+        no_progress = r2 > 0.81 * last_check ? no_progress + 1 : 0;
+        last_check = r2;
         if (r2 < best) {
           best = r2;
           HIPCHECK(hipMemcpyAsync(W.xbest, x, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
         }
+        if (no_progress >= 2) break;
       }
     }
     base_iters = rs.iters;
@@ -279,7 +324,7 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
   // packed: the solve runs on the symmetry-packed physical sector with Pi*A
   const int64_t n = packed ? 2 * c->ptotal : 2 * c->N * c->d * c->d;
   std::vector<double*> v;
-  if (int rc = carve<double>(c->ws64, n, kWorkVecs + (packed ? 2 : 1), v)) return rc;
+  if (int rc = carve<double>(c->ws64, n, kWorkVecs + (packed ? 4 : 1), v)) return rc;
   Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
   double* bb = v[kWorkVecs];
   double* xs = packed ? v[kWorkVecs + 1] : x;
@@ -297,7 +342,18 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
     return rc;
   }
   RunStats rs;
-  const bool prec = packed && !transpose && !b && (c->precond == 1 || (c->precond < 0 && c->h_diagonal));
+  // automatic choice: Jacobi scaling when H is diagonal in the occupation
+  // basis, block Jacobi otherwise
+  int prec = 0;
+  if (packed && !transpose) {
+    const int want = c->precond >= 0 ? c->precond : (c->h_diagonal ? 1 : 2);
+    if (want == 1 && !b) prec = 1;   // the scaling keeps only b = e_trace
+    if (want == 2) prec = 2;
+  }
+  if (prec == 2) {
+    W.pt = v[kWorkVecs + 2];
+    W.bp = v[kWorkVecs + 3];
+  }
   const int rc = tfqmr_run<double>(c, xs, bb, W, n, transpose != 0, tpos_of(c, trace), tol, maxiter, st, rs,
                                    packed ? 1 : 0, prec);
   if (packed) HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
@@ -476,8 +532,21 @@ int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxit
 }
 
 int heomq_ctx_set_precond(heomq_ctx* c, int mode) {
-  if (!c || mode < -1 || mode > 1) return fail(HEOMQ_BAD_ARG, "precond mode must be -1, 0 or 1");
+  if (!c || mode < -1 || mode > 2) return fail(HEOMQ_BAD_ARG, "precond mode must be -1, 0, 1 or 2");
   c->precond = mode;
+  return HEOMQ_OK;
+}
+
+int heomq_ctx_set_precond_shift(heomq_ctx* c, double shift) {
+  if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (shift < 0.0) return fail(HEOMQ_BAD_ARG, "the block-Jacobi shift must be positive (0: default)");
+  const double v = shift > 0.0 ? shift : kPrecShiftFactor * c->ejac_default;
+  for (size_t a = 0; 2 * a + 1 < c->gam_h.size(); ++a)
+    if (std::abs(std::complex<double>(v - c->gam_h[2 * a], -c->gam_h[2 * a + 1])) <= 1e-12)
+      return fail(HEOMQ_UNSUPPORTED, "block-Jacobi shift equals an ADO's gamma_K");
+  c->drop_graphs();   // captured chunks hold the shift as a kernel argument
+  c->prec_shift_user = shift;
+  c->prec_shift = v;
   return HEOMQ_OK;
 }
 

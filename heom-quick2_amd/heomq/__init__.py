@@ -318,9 +318,13 @@ class Context:
             raise HeomqError(_lib.BAD_DIM, f"expected contiguous {dtype} device tensor with {self.N * self.d * self.d} elements")
         return C.c_void_p(v.data_ptr())
 
-    def set_precond(self, mode: int):
-        """Jacobi scaling of forward steady solves: -1 automatic (diagonal H), 0 off, 1 on."""
+    def set_precond(self, mode: int, shift: float | None = None):
+        """Left preconditioning of forward steady solves: -1 automatic (Jacobi
+        scaling for diagonal H, block Jacobi otherwise), 0 none, 1 Jacobi
+        scaling, 2 block Jacobi with the given shift (None / 0: default)."""
         check(lib().heomq_ctx_set_precond(self._h, int(mode)))
+        if shift is not None:
+            check(lib().heomq_ctx_set_precond_shift(self._h, float(shift)))
 
     def set_graphs(self, enable: bool):
         """Replay solver iterations from captured graphs (default) or launch eagerly."""

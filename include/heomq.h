@@ -146,11 +146,18 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* m);
  * eight TFQMR iterations; solves requested on the null stream run on a
  * context-owned stream).  enable = 0 launches the same kernels one by one. */
 int heomq_ctx_set_graphs(heomq_ctx* c, int enable);
-/* Jacobi (diagonal) left scaling of the forward steady-state TFQMR with the
- * default right-hand side: -1 automatic (on when H is diagonal in the
- * occupation basis, e.g. the single-impurity model), 0 off, 1 on.  The
- * convergence test always uses the unscaled residual ||b - A x||. */
+/* Left preconditioning of the forward steady-state TFQMR on the packed
+ * layout: 0 none; 1 Jacobi (diagonal) scaling, default right-hand side only;
+ * 2 block Jacobi, (shift - S_K)^-1 per ADO with S_K the same-tier operator
+ * (the sweep operator of jacobi/calc_ax_jacobi_omp.f90 used as M^-1); -1
+ * automatic: 1 when H is diagonal in the occupation basis (the single-
+ * impurity model), 2 otherwise.  The convergence test always uses the
+ * unpreconditioned residual ||b - A x||.  No counterpart in the reference,
+ * whose TFQMR is unpreconditioned. */
 int heomq_ctx_set_precond(heomq_ctx* c, int mode);
+/* Shift of the block-Jacobi preconditioner (> 0), or 0 for the default:
+ * 0.05 x the block-Jacobi iteration's default ejac. */
+int heomq_ctx_set_precond_shift(heomq_ctx* c, double shift);
 /* Number of complex elements N*d*d of one ADO vector. */
 int64_t heomq_ctx_vector_len(const heomq_ctx* c);
 
