@@ -125,7 +125,7 @@ struct heomq_ctx {
   // Fock states, the default shift, per-ADO rates on the host (for the shift's
   // degeneracy check) and the iteration's own workspace
   DevBuf j_u, j_e, j_grp, j_gn, ws_jac;
-  DevBuf j_trow, j_tslot, j_tcf, j_tcb, j_tde;   // slot-space term tables
+  DevBuf j_crow, j_cslot, j_ca, j_cc, j_rrow, j_rslot, j_rb, j_rd, j_tde;   // slot-space list tables
   heomq::DevJacobi pjac;
   double ejac_default = 0.0, ejac_gap = 0.0, ejac_cb = 0.0;
   // shift of the block-Jacobi preconditioner of TFQMR: (shift - S_K)^-1 per

@@ -259,42 +259,56 @@ int build_jacobi(heomq_ctx* c, const Model& m) {
   c->pjac.grp = c->j_grp.as<int16_t>();
   c->pjac.gn = c->j_gn.as<int8_t>();
   c->pjac.gmax = jh.gmax;
-  // slot-space term tables of the transform (per sector and slot)
+  // slot-space list tables of the one-kernel form (per sector and slot)
   {
     const int S = c->nsec;
-    std::vector<int32_t> trow((size_t)S * (dd + 1), 0);
-    std::vector<int16_t> tslot;
-    std::vector<double> tcf, tcb, tde((size_t)S * dd, 0.0);
+    std::vector<int32_t> crow((size_t)S * (dd + 1), 0), rrow((size_t)S * (dd + 1), 0);
+    std::vector<int16_t> cslot, rslot;
+    std::vector<double> ca, cc, rb, rd, tde((size_t)S * dd, 0.0);
     auto U = [&](int a, int b) { return cplx(jh.u[2 * (a + (size_t)b * d)], jh.u[2 * (a + (size_t)b * d) + 1]); };
+    auto push = [](std::vector<double>& v, cplx z) { v.push_back(z.real()); v.push_back(z.imag()); };
     for (int s = 0; s < S; ++s) {
       for (int t = 0; t < dd; ++t) {
-        trow[(size_t)s * (dd + 1) + t] = (int32_t)tslot.size();
+        crow[(size_t)s * (dd + 1) + t] = (int32_t)cslot.size();
+        rrow[(size_t)s * (dd + 1) + t] = (int32_t)rslot.size();
         if (t >= c->sec_nnz_h[s]) continue;
-        const int e = c->sec_elems_h[(size_t)s * dd + t], i = e % d, j = e / d;
-        tde[(size_t)s * dd + t] = jh.e[i] - jh.e[j];
-        for (int p = 0; p < jh.gn[i]; ++p)
-          for (int q = 0; q < jh.gn[j]; ++q) {
-            const int k = jh.grp[(size_t)i * jh.gmax + p], l = jh.grp[(size_t)j * jh.gmax + q];
-            const cplx f = std::conj(U(k, i)) * U(l, j), b = U(i, k) * std::conj(U(j, l));
-            tslot.push_back(c->sec_pos_h[(size_t)s * dd + k + l * d]);
-            tcf.push_back(f.real());
-            tcf.push_back(f.imag());
-            tcb.push_back(b.real());
-            tcb.push_back(b.imag());
-          }
+        const int e = c->sec_elems_h[(size_t)s * dd + t], r = e % d, col = e / d;
+        tde[(size_t)s * dd + t] = jh.e[r] - jh.e[col];
+        for (int q = 0; q < jh.gn[col]; ++q) {   // c' in group(c)
+          const int c2 = jh.grp[(size_t)col * jh.gmax + q];
+          cslot.push_back(c->sec_pos_h[(size_t)s * dd + r + c2 * d]);
+          push(ca, U(c2, col));
+          push(cc, std::conj(U(col, c2)));
+        }
+        for (int p = 0; p < jh.gn[r]; ++p) {     // r' in group(r)
+          const int r2 = jh.grp[(size_t)r * jh.gmax + p];
+          rslot.push_back(c->sec_pos_h[(size_t)s * dd + r2 + col * d]);
+          push(rb, std::conj(U(r2, r)));
+          push(rd, U(r, r2));
+        }
       }
-      trow[(size_t)s * (dd + 1) + dd] = (int32_t)tslot.size();
+      crow[(size_t)s * (dd + 1) + dd] = (int32_t)cslot.size();
+      rrow[(size_t)s * (dd + 1) + dd] = (int32_t)rslot.size();
     }
-    if (tslot.empty()) tslot.push_back(0), tcf.assign(2, 0.0), tcb.assign(2, 0.0);
-    HIPCHECK(upload(c->j_trow, trow));
-    HIPCHECK(upload(c->j_tslot, tslot));
-    HIPCHECK(upload(c->j_tcf, tcf));
-    HIPCHECK(upload(c->j_tcb, tcb));
+    if (cslot.empty()) cslot.push_back(0), ca.assign(2, 0.0), cc.assign(2, 0.0);
+    if (rslot.empty()) rslot.push_back(0), rb.assign(2, 0.0), rd.assign(2, 0.0);
+    HIPCHECK(upload(c->j_crow, crow));
+    HIPCHECK(upload(c->j_cslot, cslot));
+    HIPCHECK(upload(c->j_ca, ca));
+    HIPCHECK(upload(c->j_cc, cc));
+    HIPCHECK(upload(c->j_rrow, rrow));
+    HIPCHECK(upload(c->j_rslot, rslot));
+    HIPCHECK(upload(c->j_rb, rb));
+    HIPCHECK(upload(c->j_rd, rd));
     HIPCHECK(upload(c->j_tde, tde));
-    c->pjac.trow = c->j_trow.as<int32_t>();
-    c->pjac.tslot = c->j_tslot.as<int16_t>();
-    c->pjac.tcf = c->j_tcf.as<double>();
-    c->pjac.tcb = c->j_tcb.as<double>();
+    c->pjac.crow = c->j_crow.as<int32_t>();
+    c->pjac.cslot = c->j_cslot.as<int16_t>();
+    c->pjac.ca = c->j_ca.as<double>();
+    c->pjac.cc = c->j_cc.as<double>();
+    c->pjac.rrow = c->j_rrow.as<int32_t>();
+    c->pjac.rslot = c->j_rslot.as<int16_t>();
+    c->pjac.rb = c->j_rb.as<double>();
+    c->pjac.rd = c->j_rd.as<double>();
     c->pjac.tde = c->j_tde.as<double>();
   }
   // the one-kernel form: every 256-element block of the packed vector must lie

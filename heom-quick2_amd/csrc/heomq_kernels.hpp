@@ -186,14 +186,20 @@ struct DevJacobi {
   // 1: every 256-element block of the packed layout lies in ADOs spanning at
   // most kJacobiWindow elements and 257 ADOs, so the one-kernel form applies
   int windowed = 0;
-  // slot-space form of the transform, per sector s and slot t (element (i, j)):
-  // terms [trow[s][t], trow[s][t+1]) over (k, l) in group(i) x group(j), each
-  // the slot of (k, l) with the forward coefficient conj(U_ki) U_lj and the
-  // back coefficient U_ik conj(U_jl); tde[s][t] = E_i - E_j
-  const int32_t* trow = nullptr;    // [S][d*d + 1]
-  const int16_t* tslot = nullptr;
-  const double* tcf = nullptr;      // complex
-  const double* tcb = nullptr;      // complex
+  // slot-space tables of the one-kernel form, per sector s and slot t of
+  // element (r, c): the column list [crow[s][t], crow[s][t+1]) runs over
+  // c' in group(c) with the slot of (r, c') and the coefficients U_{c'c}
+  // (X -> X U) and conj(U_{cc'}) (X -> X U^H); the row list runs over r' in
+  // group(r) with the slot of (r', c) and conj(U_{r'r}) (X -> U^H X) and
+  // U_{rr'} (X -> U X); tde[s][t] = E_r - E_c
+  const int32_t* crow = nullptr;    // [S][d*d + 1]
+  const int16_t* cslot = nullptr;
+  const double* ca = nullptr;       // complex
+  const double* cc = nullptr;       // complex
+  const int32_t* rrow = nullptr;    // [S][d*d + 1]
+  const int16_t* rslot = nullptr;
+  const double* rb = nullptr;       // complex
+  const double* rd = nullptr;       // complex
   const double* tde = nullptr;      // [S][d*d]
 };
 
@@ -207,6 +213,6 @@ hipError_t launch_jacobi_back(const DevPacked& op, const DevJacobi& jc, const do
 // is made exactly Hermitian.
 hipError_t launch_jacobi_win(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
                              hipStream_t s);
-constexpr int kJacobiWindow = 1024;   // LDS window of the one-kernel form (complex elements)
+constexpr int kJacobiWindow = 384;    // LDS window of the one-kernel form (complex elements); d <= 16 needs <= 328
 
 }  // namespace heomq

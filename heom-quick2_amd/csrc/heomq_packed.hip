@@ -530,13 +530,19 @@ __global__ void k_jacobi_back(const DevPacked op, const DevJacobi jc, const doub
 // Without accumulate the tier-0 block of the output is made exactly
 // Hermitian (z_ij from the average of (i, j) and conj of (j, i)).  in and out
 // must not alias: neighbouring blocks read the ADOs they share.
-__device__ inline int jac_find(const int64_t* offs, int nado, int64_t g) {
-  int lo = 0, hi = nado - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (offs[mid] <= g) lo = mid; else hi = mid - 1;
+// sum over one slot-space list: c[r] * v[base + slot[r]] for r in [r0, r1)
+__device__ inline double2 jac_list(const int32_t* row, const int16_t* slot, const double* coef, int idx,
+                                   const double2* v, int base) {
+  const int r0 = row[idx], r1 = row[idx + 1];
+  const double2* cf = reinterpret_cast<const double2*>(coef);
+  double ar = 0.0, ai = 0.0;
+  for (int r = r0; r < r1; ++r) {
+    const double2 c = cf[r];
+    const double2 x = v[base + slot[r]];
+    ar += c.x * x.x - c.y * x.y;
+    ai += c.x * x.y + c.y * x.x;
   }
-  return lo;
+  return make_double2(ar, ai);
 }
 
 __global__ __launch_bounds__(256) void k_jacobi_win(const DevPacked op, const DevJacobi jc,
@@ -547,6 +553,7 @@ __global__ __launch_bounds__(256) void k_jacobi_win(const DevPacked op, const De
   __shared__ int64_t offs[260];
   __shared__ double2 gamw[260];
   __shared__ int secw[260];
+  __shared__ uint16_t adoi[kJacobiWindow];   // window element -> index of its ADO in offs
   const int64_t e0 = (int64_t)blockIdx.x * 256;
   const int64_t a_lo = op.block_ado[blockIdx.x], a_hi = op.block_ado[blockIdx.x + 1];
   const int nado = (int)(a_hi - a_lo + 1);   // <= 257, checked on the host
@@ -562,61 +569,44 @@ __global__ __launch_bounds__(256) void k_jacobi_win(const DevPacked op, const De
   const int nw = (int)(offs[nado] - w0);     // <= kJacobiWindow, checked on the host
   const double2* xin = reinterpret_cast<const double2*>(in);
   for (int k = threadIdx.x; k < nw; k += 256) xw[k] = xin[w0 + k];
+  for (int ia = threadIdx.x; ia < nado; ia += 256)
+    for (int k = (int)(offs[ia] - w0), k1 = (int)(offs[ia + 1] - w0); k < k1; ++k) adoi[k] = (uint16_t)ia;
   __syncthreads();
-  const int DD = op.d * op.d;
-  const double2* cf = reinterpret_cast<const double2*>(jc.tcf);
-  const double2* cb = reinterpret_cast<const double2*>(jc.tcb);
-  // forward: tw = (U^H X U) / (ejac - gamma_K + i (E_i - E_j)) over the window
+  const int R = op.d * op.d + 1;   // row length of the list tables
+  // X -> X U_b
   for (int k = threadIdx.x; k < nw; k += 256) {
-    const int64_t g = w0 + k;
-    const int ia = jac_find(offs, nado, g);
-    const int s = secw[ia];
-    const int base = (int)(offs[ia] - w0);
-    const int t = (int)(g - offs[ia]);
-    const int r0 = jc.trow[s * (DD + 1) + t], r1 = jc.trow[s * (DD + 1) + t + 1];
-    double ar = 0.0, ai = 0.0;
-    for (int r = r0; r < r1; ++r) {
-      const double2 c = cf[r];
-      const double2 x = xw[base + jc.tslot[r]];
-      ar += c.x * x.x - c.y * x.y;
-      ai += c.x * x.y + c.y * x.x;
-    }
-    const double2 ga = gamw[ia];
-    const double wr = jc.ejac - ga.x, wi = jc.tde[s * DD + t] - ga.y;
-    const double den = wr * wr + wi * wi;
-    tw[k] = make_double2((ar * wr + ai * wi) / den, (ai * wr - ar * wi) / den);
+    const int ia = adoi[k], base = (int)(offs[ia] - w0);
+    tw[k] = jac_list(jc.crow, jc.cslot, jc.ca, secw[ia] * R + (k - base), xw, base);
   }
   __syncthreads();
-  const int64_t g = e0 + threadIdx.x;
-  if (g >= op.total) return;
-  const int ia = jac_find(offs, nado, g);
-  const int s = secw[ia];
-  const int base = (int)(offs[ia] - w0);
-  const int t = (int)(g - offs[ia]);
-  double2 z = make_double2(0.0, 0.0);
-  {
-    const int r0 = jc.trow[s * (DD + 1) + t], r1 = jc.trow[s * (DD + 1) + t + 1];
-    for (int r = r0; r < r1; ++r) {
-      const double2 c = cb[r];
-      const double2 v = tw[base + jc.tslot[r]];
-      z.x += c.x * v.x - c.y * v.y;
-      z.y += c.x * v.y + c.y * v.x;
-    }
+  // -> (U_a^H .) / (shift - gamma_K + i (E_i - E_j))
+  for (int k = threadIdx.x; k < nw; k += 256) {
+    const int ia = adoi[k], base = (int)(offs[ia] - w0), s = secw[ia], t = k - base;
+    const double2 q = jac_list(jc.rrow, jc.rslot, jc.rb, s * R + t, tw, base);
+    const double2 ga = gamw[ia];
+    const double wr = jc.ejac - ga.x, wi = jc.tde[s * (R - 1) + t] - ga.y;
+    const double den = wr * wr + wi * wi;
+    xw[k] = make_double2((q.x * wr + q.y * wi) / den, (q.y * wr - q.x * wi) / den);
   }
+  __syncthreads();
+  // -> . U_b^H
+  for (int k = threadIdx.x; k < nw; k += 256) {
+    const int ia = adoi[k], base = (int)(offs[ia] - w0);
+    tw[k] = jac_list(jc.crow, jc.cslot, jc.cc, secw[ia] * R + (k - base), xw, base);
+  }
+  __syncthreads();
+  // -> U_a . for the block's own elements
+  const int64_t g = e0 + threadIdx.x;
+  if (g >= op.total) return;   // after the last barrier
+  const int k = (int)(g - w0);
+  const int ia = adoi[k], base = (int)(offs[ia] - w0), s = secw[ia], t = k - base;
+  double2 z = jac_list(jc.rrow, jc.rslot, jc.rd, s * R + t, tw, base);
   if (!accumulate && a_lo + ia == 0) {   // tier 0: average with conj of the mirror element
-    const int e = op.sec_elems[s * DD + t];
+    const int e = op.sec_elems[s * (R - 1) + t];
     const int i = e % op.d, j = e / op.d;
-    const int tm = op.sec_pos[s * DD + j + i * op.d];
-    const int r0 = jc.trow[s * (DD + 1) + tm], r1 = jc.trow[s * (DD + 1) + tm + 1];
-    double mr = 0.0, mi = 0.0;
-    for (int r = r0; r < r1; ++r) {
-      const double2 c = cb[r];
-      const double2 v = tw[base + jc.tslot[r]];
-      mr += c.x * v.x - c.y * v.y;
-      mi += c.x * v.y + c.y * v.x;
-    }
-    z.x = 0.5 * (z.x + mr);
-    z.y = 0.5 * (z.y - mi);
+    const double2 m = jac_list(jc.rrow, jc.rslot, jc.rd, s * R + op.sec_pos[s * (R - 1) + j + i * op.d], tw, base);
+    z.x = 0.5 * (z.x + m.x);
+    z.y = 0.5 * (z.y - m.y);
   }
   double2* xo = reinterpret_cast<double2*>(out);
   if (accumulate) {

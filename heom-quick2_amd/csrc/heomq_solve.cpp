@@ -579,6 +579,33 @@ int heomq_apply_packed_raw(heomq_ctx* c, const double* xp, double* yp, int trace
   return HEOMQ_OK;
 }
 
+int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int trace, int form, void* stream) {
+  if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
+  if (form < 1 || form > 2) return fail(HEOMQ_BAD_ARG, "form must be 1 or 2");
+  if (xp == yp) return fail(HEOMQ_BAD_ARG, "x and y must differ");
+  hipStream_t st = (hipStream_t)stream;
+  const bool rdo = trace == HEOMQ_TRACE_RDO;
+  DevPacked op = c->pfwd;
+  op.precond = 0;
+  DevJacobi jc = c->pjac;
+  jc.ejac = c->prec_shift;
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64b, 2 * c->ptotal, 1, v)) return rc;
+  HIPCHECK(launch_apply_packed_f64(op, xp, v[0], rdo, st));
+  if (form == 1) {
+    if (!jc.windowed) return fail(HEOMQ_UNSUPPORTED, "the one-kernel preconditioner does not apply here");
+    HIPCHECK(launch_jacobi_win(op, jc, v[0], yp, false, st));
+  } else {
+    HIPCHECK(hipMemcpyAsync(yp, v[0], sizeof(double) * 2 * c->ptotal, hipMemcpyDeviceToDevice, st));
+    std::vector<double*> w;
+    if (int rc = carve<double>(c->ws_io, 2 * c->ptotal, 1, w)) return rc;
+    HIPCHECK(launch_jacobi_fwd(op, jc, yp, w[0], st));
+    HIPCHECK(launch_jacobi_back(op, jc, w[0], yp, false, st));
+  }
+  return HEOMQ_OK;
+}
+
 int heomq_apply_packed_raw_f32(heomq_ctx* c, const float* xp, float* yp, int trace, int transpose, void* stream) {
   if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
   if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");

@@ -372,6 +372,16 @@ class Context:
             raise HeomqError(_lib.BAD_DIM, f"expected contiguous {dtype} device tensor with {self._ptotal} elements")
         return C.c_void_p(v.data_ptr())
 
+    def apply_packed_precond(self, xp, yp=None, trace=TRACE_RDO, form=1):
+        """y = M^-1 (Pi A x) on packed vectors with the block-Jacobi preconditioner
+        (form 1: LDS-window kernel; 2: two-kernel transform)."""
+        t = self.torch
+        yp = self.packed_zeros() if yp is None else yp
+        check(lib().heomq_apply_packed_precond(self._h, self._check_packed(xp, t.complex128),
+                                               self._check_packed(yp, t.complex128), trace, int(form),
+                                               self._stream()))
+        return yp
+
     def apply_packed_raw(self, xp, yp, trace=TRACE_RDO, transpose=False):
         t = self.torch
         if xp.dtype == t.complex64:

@@ -69,6 +69,7 @@ SIGNATURES = {
     "heomq_packed_info": (C.c_int, [P, I64]),
     "heomq_apply_packed_raw": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),
     "heomq_apply_packed_raw_f32": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),
+    "heomq_apply_packed_precond": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),
     "heomq_pack": (C.c_int, [P, P, P, P]),
     "heomq_unpack": (C.c_int, [P, P, P, P]),
     "heomq_calc_ax_tfqmr": (C.c_int, [P, C.c_int32, D, D]),

@@ -187,6 +187,14 @@ int heomq_apply_packed(heomq_ctx* c, const double* x, double* y, int trace, int 
  * elements each, see heomq_packed_info), and the dense <-> packed maps. */
 int heomq_apply_packed_raw(heomq_ctx* c, const double* xp, double* yp, int trace, int transpose, void* stream);
 int heomq_apply_packed_raw_f32(heomq_ctx* c, const float* xp, float* yp, int trace, int transpose, void* stream);
+/* yp = M^-1 (Pi A xp) on packed vectors, M^-1 the block-Jacobi inverse
+ * (shift - S_K)^-1 per ADO with the preconditioner shift (tier-0 output made
+ * Hermitian): the operator the block-Jacobi-preconditioned TFQMR iterates
+ * with.  form 1: apply, then the LDS-window kernel (the solver's form when
+ * every ADO range a 256-element block touches fits its LDS; else
+ * HEOMQ_UNSUPPORTED); 2: apply, then the two-kernel transform.  Both forms
+ * compute the same values.  No counterpart in the reference. */
+int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int trace, int form, void* stream);
 int heomq_pack(heomq_ctx* c, const double* dense, double* packed, void* stream);
 int heomq_unpack(heomq_ctx* c, const double* packed, double* dense, void* stream);
 /* info[0] = packed complex elements, [1] = sectors, [2] = N conserved,

@@ -1070,6 +1070,18 @@ def jacobi_sweep(model: Model, x, ejac):
     return out
 
 
+def block_jacobi_precondition(model: Model, y, shift):
+    """M^-1 y with M = shift - S_K on every ADO K (the sweep operator of
+    calc_ax_jacobi_omp.f90 used as a preconditioner), tier 0 made Hermitian."""
+    d = model.d
+    blocks = _jacobi_blocks(model, shift)
+    z = np.empty_like(y)
+    for K in range(model.N):
+        z[K] = (blocks[K][1] @ y[K].reshape(-1, order="F")).reshape(d, d, order="F")
+    z[0] = 0.5 * (z[0] + z[0].conj().T)
+    return z
+
+
 def jacobi_residual(model: Model, x):
     """||A x - e||_2 on the (Re, Im) vector with calc_ax_tfqmr_omp's trace row
     (solve_jacobi.f90:186-190: the real part of the last element minus 1)."""

@@ -121,3 +121,22 @@ def test_block_jacobi_preconditioned_tfqmr(name, shift):
     assert np.abs(xs[0] - ref[0]).max() <= 1e-10
     assert np.abs(xs[0] - xs[0].conj().T).max() <= 1e-14
     assert np.abs(O.current(o, xs) - O.current(o, ref)).max() <= 1e-10 * np.abs(O.current(o, ref)).max()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("form", [1, 2])
+def test_preconditioned_apply_matches_oracle(name, form):
+    """M^-1 (Pi A x) in both device forms (LDS-window kernel, two-kernel
+    transform) against the oracle's dense per-ADO solve."""
+    m, ctx, o, torch = setup(name)
+    print(f"case {name} form {form}", flush=True)
+    x = O.physical_projection(o, O.seeded_ado_vector(o, 43))
+    shift = 0.05 * O.jacobi_ejac(o)[0]
+    ctx.set_precond(-1, shift)
+    try:
+        yp = ctx.apply_packed_precond(ctx.pack(to_dev(torch, x)), form=form)
+        y = from_dev(ctx.unpack(yp))
+    finally:
+        ctx.set_precond(-1, 0.0)
+    yo = O.block_jacobi_precondition(o, O.physical_projection(o, O.apply(o, x, trace_row="rdo")), shift)
+    assert rel(y, yo) <= 1e-12

@@ -414,3 +414,18 @@ def test_jacobi_sweep_shift_independent_fixed_point():
     xs, _, _ = O.steady_state(o, tol=1e-13)
     for ej in (0.5, 3.0):
         assert np.abs(O.jacobi_sweep(o, xs, ej) - xs).max() <= 1e-10 * np.abs(xs).max()
+
+
+def test_block_jacobi_preconditioner_inverts_the_same_tier_blocks():
+    # M^-1 applied to M y returns y (tier 0 Hermitian), with M built from the
+    # apply's own diagonal blocks
+    o = O.Model(O.dqd_params(nk=1, ntop=2))
+    shift = 0.05 * O.jacobi_ejac(o)[0]
+    y = O.physical_projection(o, O.seeded_ado_vector(o, 9))
+    gam = O.ado_gamma(o)
+    My = np.empty_like(y)
+    for K in range(o.N):
+        S = O.same_tier_matrix(o, gam[K])
+        My[K] = ((shift * np.eye(o.d ** 2) - S) @ y[K].reshape(-1, order="F")).reshape(o.d, o.d, order="F")
+    z = O.block_jacobi_precondition(o, My, shift)
+    assert np.abs(z - y).max() <= 1e-12 * np.abs(y).max()
