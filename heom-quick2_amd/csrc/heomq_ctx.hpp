@@ -126,6 +126,11 @@ struct heomq_ctx {
   // degeneracy check) and the iteration's own workspace
   DevBuf j_u, j_e, j_grp, j_gn, ws_jac;
   DevBuf j_crow, j_cslot, j_ca, j_cc, j_rrow, j_rslot, j_rb, j_rd, j_tde;   // slot-space list tables
+  // sparse-storage drop-in (calc_ax_tfqmr_spa_omp): COO element -> dense
+  // index, positions of the tier-0 diagonal, dense scratch; spa_lunk < 0: no pattern
+  DevBuf spa_map, spa_diag, spa_dense, spa_io;
+  int64_t spa_lunk = -1;
+  int spa_ndiag = 0;
   heomq::DevJacobi pjac;
   double ejac_default = 0.0, ejac_gap = 0.0, ejac_cb = 0.0;
   // shift of the block-Jacobi preconditioner of TFQMR: (shift - S_K)^-1 per

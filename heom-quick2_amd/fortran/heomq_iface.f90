@@ -14,6 +14,10 @@
 !   solve_tfqmr_gpu(rho, crit, maxit, info)   solver-level replacement of solve_tfqmr
 !   solve_bicg_gpu(rho, crit, maxit, info)    solver-level replacement of solvebicg
 !   calc_ax_jacobi_omp(dim0, rcgtmp, rcgrhs)  jacobi/calc_ax_jacobi_omp.f90:1,15-17
+!   calc_ax_tfqmr_spa_omp(ldim, rcgtmp, rcgrhs)
+!                                             tfqmr_double/calc_ax_tfqmr_spa_omp.f90:1,16-18
+!                                             (pattern of module sparsemod registered
+!                                             once by heomq_register_spa_pattern)
 !   solve_jacobi_gpu(rho, crit, maxit, info)  solver-level replacement of solve_jacobi
 !
 ! heomq_fortran_init uploads the reference's own module arrays once (after
@@ -85,6 +89,21 @@ module heomq_iface
       real(c_double), value :: crit
       integer(c_int), value :: maxit
       real(c_double) :: stats(*)
+    end function
+
+    integer(c_int) function c_spa_pattern(ctx, lunk, nnz, ind, irow, icol) bind(C, name='heomq_spa_pattern')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: lunk
+      integer(c_int32_t) :: nnz(*), irow(*), icol(*)
+      integer(c_int64_t) :: ind(*)
+    end function
+
+    integer(c_int) function c_calc_ax_spa(ctx, ldim, x, y) bind(C, name='heomq_calc_ax_tfqmr_spa')
+      import :: c_int, c_int64_t, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ldim
+      complex(c_double_complex) :: x(*), y(*)
     end function
 
     integer(c_int) function c_calc_ax_jacobi(ctx, dim0, x, y) bind(C, name='heomq_calc_ax_jacobi')
@@ -296,3 +315,31 @@ subroutine solve_jacobi_gpu(rho, crit, maxit, info)
   info(2) = int(stats(1))
   if (rc /= 0 .and. rc /= 4) call heomq_fail('solve_jacobi_gpu', rc)
 end subroutine solve_jacobi_gpu
+
+! The sparse pattern of module sparsemod (fermion/prelude_spa.f90:532-749):
+! lunk_spa elements; nnz_spa and ind_spa per ADO; irow_spa / icol_spa
+! (module/sparsemod.f90:28-31).  Registered once, after prelude_spa.
+subroutine heomq_register_spa_pattern(lunk_spa, nnz_spa, ind_spa, irow_spa, icol_spa)
+  use heomq_iface
+  implicit none
+  integer*8, intent(in) :: lunk_spa
+  integer, intent(in) :: nnz_spa(*), irow_spa(*), icol_spa(*)
+  integer*8, intent(in) :: ind_spa(*)
+  integer(c_int) :: rc
+  rc = c_spa_pattern(heomq_ctx_h, int(lunk_spa, c_int64_t), nnz_spa, ind_spa, irow_spa, icol_spa)
+  if (rc /= 0) call heomq_fail('heomq_register_spa_pattern', rc)
+end subroutine heomq_register_spa_pattern
+
+! tfqmr_double/calc_ax_tfqmr_spa_omp.f90:1,16-18 -- the apply on COO storage:
+! rcgrhs(1:lunk_spa) = P A P^T rcgtmp, rcgrhs(lunk_spa + 1) = tr rho0;
+! ldim /= lunk_spa is fatal as in :61-65
+subroutine calc_ax_tfqmr_spa_omp(ldim, rcgtmp, rcgrhs)
+  use heomq_iface
+  implicit none
+  integer*8, intent(in) :: ldim
+  complex*16, intent(in) :: rcgtmp(*)
+  complex*16, intent(out) :: rcgrhs(*)
+  integer(c_int) :: rc
+  rc = c_calc_ax_spa(heomq_ctx_h, int(ldim, c_int64_t), rcgtmp, rcgrhs)
+  if (rc /= 0) call heomq_fail('calc_ax_tfqmr_spa_omp', rc)
+end subroutine calc_ax_tfqmr_spa_omp

@@ -452,6 +452,31 @@ class Context:
             check(rc)
         return x, info
 
+    def set_spa_pattern(self, nnz_spa, ind_spa, irow_spa, icol_spa):
+        """The reference's sparse (COO per ADO) pattern, 1-based as module sparsemod holds it."""
+        nnz = np.ascontiguousarray(nnz_spa, np.int32)
+        ind = np.ascontiguousarray(ind_spa, np.int64)
+        ir = np.ascontiguousarray(irow_spa, np.int32)
+        ic = np.ascontiguousarray(icol_spa, np.int32)
+        if nnz.size != self.N or ind.size != self.N or ir.size != ic.size:
+            raise HeomqError(_lib.BAD_DIM, "nnz_spa / ind_spa need N entries, irow_spa / icol_spa lunk_spa each")
+        self._spa_lunk = int(ir.size)
+        check(lib().heomq_spa_pattern(self._h, ir.size, nnz.ctypes.data_as(_lib.I32), ind.ctypes.data_as(_lib.I64),
+                                      ir.ctypes.data_as(_lib.I32), ic.ctypes.data_as(_lib.I32)))
+
+    def apply_spa(self, x_spa, y_spa=None):
+        """calc_ax_tfqmr_spa_omp on device vectors: y (lunk_spa + 1) from x (lunk_spa)."""
+        t = self.torch
+        n = getattr(self, "_spa_lunk", -1)
+        if y_spa is None:
+            y_spa = t.zeros(n + 1, dtype=t.complex128, device=f"cuda:{self.device}")
+        for v, m in ((x_spa, n), (y_spa, n + 1)):
+            if v.dtype != t.complex128 or not v.is_cuda or not v.is_contiguous() or v.numel() != m:
+                raise HeomqError(_lib.BAD_DIM, "apply_spa: contiguous complex128 device vectors of lunk_spa / lunk_spa + 1")
+        check(lib().heomq_apply_spa(self._h, C.c_void_p(x_spa.data_ptr()), C.c_void_p(y_spa.data_ptr()),
+                                    self._stream()))
+        return y_spa
+
     def jacobi_info(self) -> dict:
         info = np.zeros(5)
         check(lib().heomq_jacobi_info(self._h, _dp(info)))

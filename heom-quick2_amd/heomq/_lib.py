@@ -81,6 +81,9 @@ SIGNATURES = {
     "heomq_calc_atrx_bicg": (C.c_int, [P, C.c_int32, D, D]),
     "heomq_solve_bicg": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_solve_bicg_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),
+    "heomq_spa_pattern": (C.c_int, [P, C.c_int64, I32, I64, I32, I32]),
+    "heomq_apply_spa": (C.c_int, [P, P, P, P]),
+    "heomq_calc_ax_tfqmr_spa": (C.c_int, [P, C.c_int64, D, D]),
     "heomq_model_jacobi_info": (C.c_int, [P, D, D, D]),
     "heomq_jacobi_info": (C.c_int, [P, D]),
     "heomq_jacobi_step": (C.c_int, [P, P, P, C.c_double, P]),

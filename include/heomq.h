@@ -248,6 +248,24 @@ int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* 
 /* The same on a host array (the solvebicg drop-in of the Fortran layer). */
 int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats);
 /*
+ * Sparse-storage drop-in (tfqmr_double/calc_ax_tfqmr_spa_omp.f90): ADOs
+ * stored as COO lists over a fixed pattern (module sparsemod, built by
+ * fermion/prelude_spa.f90:532-749).  y = P A P^T x with P the pattern's
+ * selection, and y(lunk_spa + 1) = tr rho0 (:1061-1069).
+ */
+/* Set the pattern from the reference's arrays: nnz_spa and ind_spa (1-based
+ * start in rho_spa) per ADO (N each), irow_spa / icol_spa (1-based, lunk_spa
+ * each).  Rejected with HEOMQ_BAD_ARG: indices out of range, overlapping
+ * ADOs, an element listed twice, elements of no ADO. */
+int heomq_spa_pattern(heomq_ctx* c, int64_t lunk_spa, const int32_t* nnz_spa, const int64_t* ind_spa,
+                      const int32_t* irow_spa, const int32_t* icol_spa);
+/* Device vectors: x_spa (lunk_spa complex), y_spa (lunk_spa + 1 complex). */
+int heomq_apply_spa(heomq_ctx* c, const double* x_spa, double* y_spa, void* stream);
+/* calc_ax_tfqmr_spa_omp(ldim, rcgtmp, rcgrhs) on host arrays; ldim /= lunk_spa
+ * is HEOMQ_BAD_DIM, as calc_ax_tfqmr_spa_omp.f90:61-65 stops. */
+int heomq_calc_ax_tfqmr_spa(heomq_ctx* c, int64_t ldim, const double* x_host, double* y_host);
+
+/*
  * Block-Jacobi iteration (jacobi/solve_jacobi.f90, calc_ax_jacobi_omp.f90,
  * prelude_jacobi.f90; J. Chem. Phys. 147, 044105 (2017), Eq. 5).  One sweep
  * replaces every ADO K by (ejac - S_K)^-1 (ejac x_K + C_K(x)), S_K X =

@@ -1002,6 +1002,47 @@ def chebyshev(model: Model, x, dt, nstep):
 
 
 # --------------------------------------------------------------------------
+# Sparse (COO per ADO) storage: tfqmr_double/calc_ax_tfqmr_spa_omp.f90
+# --------------------------------------------------------------------------
+def spa_pattern_from_mask(mask):
+    """The reference's sparse pattern arrays (module sparsemod, 1-based) for a
+    boolean (N, d, d) mask: nnz_spa and ind_spa per ADO, irow_spa / icol_spa
+    per element, each ADO's elements in column-major order as
+    fermion/prelude_spa.f90:550-561 lists them."""
+    N, d, _ = mask.shape
+    nnz = np.zeros(N, np.int32)
+    ind = np.zeros(N, np.int64)
+    rows, cols = [], []
+    pos = 0
+    for a in range(N):
+        jj, ii = np.nonzero(mask[a].T)          # column-major: column slowest
+        nnz[a] = len(ii)
+        ind[a] = pos + 1 if len(ii) else 0
+        rows.extend(ii + 1)
+        cols.extend(jj + 1)
+        pos += len(ii)
+    return nnz, ind, np.array(rows, np.int32), np.array(cols, np.int32)
+
+
+def apply_spa(model: Model, pattern, x_spa):
+    """calc_ax_tfqmr_spa_omp: the operator on COO storage keeps only the
+    pattern's elements of the result (zmat_extract_coo), y = P A P^T x
+    (trace row none), and appends tr rho0 over the tier-0 diagonal elements
+    of the pattern as element lunk_spa + 1 (:1061-1069)."""
+    nnz, ind, irow, icol = pattern
+    d = model.d
+    x = np.zeros((model.N, d, d), complex)
+    a_of = np.repeat(np.arange(model.N), nnz)
+    x[a_of, irow - 1, icol - 1] = x_spa
+    y = apply(model, x, trace_row=None)
+    out = np.empty(len(x_spa) + 1, complex)
+    out[:-1] = y[a_of, irow - 1, icol - 1]
+    diag0 = (a_of == 0) & (irow == icol)
+    out[-1] = x_spa[diag0].sum()
+    return out
+
+
+# --------------------------------------------------------------------------
 # Block-Jacobi iteration (jacobi/prelude_jacobi.f90, calc_ax_jacobi_omp.f90,
 # solve_jacobi.f90; J. Chem. Phys. 147, 044105 (2017), Eq. 5)
 # --------------------------------------------------------------------------

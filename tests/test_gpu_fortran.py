@@ -161,3 +161,21 @@ def test_solve_jacobi_gpu():
     assert O.jacobi_residual(o, xs) <= 1e-10
     xo, it, res = O.steady_state(o, tol=1e-13)
     assert np.abs(xs[0] - xo[0]).max() <= 1e-9
+
+
+def test_calc_ax_tfqmr_spa_omp():
+    h, o = shim()
+    mask = O.physical_projection(o, np.ones((o.N, o.d, o.d), complex), hermitian=False) != 0
+    rng = np.random.default_rng(4)
+    mask &= rng.random(mask.shape) < 0.8        # a pattern smaller than the physical sector
+    mask[0] |= np.eye(o.d, dtype=bool)
+    nnz, ind, irow, icol = pat = O.spa_pattern_from_mask(mask)
+    lunk = C.c_int64(len(irow))
+    keep = [np.ascontiguousarray(a) for a in (nnz, ind, irow, icol)]
+    h.heomq_register_spa_pattern_(C.byref(lunk), *[a.ctypes.data_as(C.c_void_p) for a in keep])
+    rng2 = np.random.default_rng(5)
+    xs = rng2.standard_normal(len(irow)) + 1j * rng2.standard_normal(len(irow))
+    ys = np.zeros(len(irow) + 1, complex)
+    h.calc_ax_tfqmr_spa_omp_(C.byref(lunk), xs.ctypes.data_as(C.c_void_p), ys.ctypes.data_as(C.c_void_p))
+    yo = O.apply_spa(o, pat, xs)
+    assert np.abs(ys - yo).max() <= 1e-12 * np.abs(yo).max()

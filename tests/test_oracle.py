@@ -429,3 +429,38 @@ def test_block_jacobi_preconditioner_inverts_the_same_tier_blocks():
         My[K] = ((shift * np.eye(o.d ** 2) - S) @ y[K].reshape(-1, order="F")).reshape(o.d, o.d, order="F")
     z = O.block_jacobi_precondition(o, My, shift)
     assert np.abs(z - y).max() <= 1e-12 * np.abs(y).max()
+
+
+# ---- sparse (COO per ADO) storage ----
+def test_apply_spa_full_pattern_is_the_dense_apply():
+    o = O.Model(O.siam_params(nk=2, ntop=2, V=0.3, U=1.5))
+    pat = O.spa_pattern_from_mask(np.ones((o.N, o.d, o.d), bool))
+    x = O.seeded_ado_vector(o, 21)
+    xs = np.array([x[a, i - 1, j - 1] for a, i, j in zip(np.repeat(np.arange(o.N), pat[0]), pat[2], pat[3])])
+    y = O.apply_spa(o, pat, xs)
+    yd = O.apply(o, x, trace_row=None)
+    assert np.abs(y[:-1] - yd.transpose(0, 2, 1).reshape(-1)).max() <= 1e-13 * np.abs(yd).max()
+    assert abs(y[-1] - np.trace(x[0])) <= 1e-14
+
+
+def test_apply_spa_is_the_restricted_dense_matrix():
+    # brute force: column j of the sparse apply is the pattern's rows of column
+    # j of the dense matrix, for a pattern that drops elements at random
+    o = O.Model(O.siam_params(nk=1, ntop=1, V=0.3, U=1.5))
+    rng = np.random.default_rng(3)
+    mask = rng.random((o.N, o.d, o.d)) < 0.6
+    mask[0] |= np.eye(o.d, dtype=bool)
+    pat = O.spa_pattern_from_mask(mask)
+    a_of = np.repeat(np.arange(o.N), pat[0])
+    A = O.dense_matrix(o, trace_row=None)
+    n = len(a_of)
+    idx = lambda a, i, j: 2 * (a * o.d * o.d + i + j * o.d)   # as_real: Fortran order, (Re, Im)
+    for e in range(n):
+        for part in (0, 1):
+            xs = np.zeros(n, complex)
+            xs[e] = 1.0 if part == 0 else 1j
+            y = O.apply_spa(o, pat, xs)
+            col = A[:, idx(a_of[e], pat[2][e] - 1, pat[3][e] - 1) + part]
+            want = np.array([col[idx(a, i - 1, j - 1)] + 1j * col[idx(a, i - 1, j - 1) + 1]
+                             for a, i, j in zip(a_of, pat[2], pat[3])])
+            assert np.abs(y[:-1] - want).max() <= 1e-12 * max(np.abs(A).max(), 1.0)
