@@ -303,7 +303,6 @@ struct PackedHost {
   std::vector<int32_t> maps;
   int tslot = -1;
   int max_light[2] = {0, 0};
-  int light_rec_lo[2] = {0, 0}, light_rec_n[2] = {0, 0};
 };
 
 int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
@@ -486,18 +485,9 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
   ph.tslot = ph.pos[(size_t)ph.sec[0] * dd + (d - 1) + (d - 1) * d];
   for (int w = 0; w < 2; ++w) {
     const auto& rows = w == 0 ? rowf : rowt;
-    const auto& ents = w == 0 ? ph.pentf : ph.pentt;
-    int mx = 0, rlo = INT32_MAX, rhi = -1;
-    for (int64_t g = n_heavy; g < m.N; ++g) {
-      mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
-      for (int64_t p = rows[g]; p < rows[g + 1]; ++p) {
-        rlo = std::min<int>(rlo, ents[p].rec);
-        rhi = std::max<int>(rhi, ents[p].rec);
-      }
-    }
+    int mx = 0;
+    for (int64_t g = n_heavy; g < m.N; ++g) mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
     ph.max_light[w] = mx;
-    ph.light_rec_lo[w] = rhi >= 0 ? rlo : 0;
-    ph.light_rec_n[w] = rhi >= 0 ? rhi - rlo + 1 : 0;
   }
   return HEOMQ_OK;
 }
@@ -541,8 +531,6 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     p.ent = (w == 0 ? c->p_entf : c->p_entt).as<DevEntryP>();
     p.trace_slot = ph.tslot;
     p.maps = c->p_maps.as<int32_t>();
-    p.light_rec_lo = ph.light_rec_lo[w];
-    p.light_rec_n = ph.light_rec_n[w];
     p.n_heavy = ph.n_heavy;
     p.light_begin = ph.light_begin;
     p.light_block_ado = c->p_lblock.as<int32_t>();

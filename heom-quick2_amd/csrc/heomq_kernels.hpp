@@ -112,8 +112,6 @@ struct DevPacked {
   const int16_t* hq = nullptr;
   const double* hc = nullptr;        // complex coefficients of i(HX - XH)
   const int32_t* maps = nullptr;     // paired gather maps (int16 left | int16 right << 16)
-  // coefficient records the top-tier entries use: [light_rec_lo, light_rec_lo + light_rec_n)
-  int light_rec_lo = 0, light_rec_n = 0;
   // batch of problems sharing every structural table (grid y = problem):
   // problem p's vectors start p * vstride complex elements in, its records,
   // ADO rates and commutator coefficients p * rec_stride / gam_stride /

@@ -159,20 +159,18 @@ __device__ inline void coupling_block(const DevEntryP* en, const CoefLR* cf, int
   }
 }
 
-constexpr int kLightRec = 384;   // coefficient records a top-tier block stages in LDS
-
 // Top-tier ADOs: one thread per packed element.  A block stages in LDS the
 // headers (offset, first and last entry, sector, gamma) of the non-empty ADOs
-// it covers (at most 257, checked when the tables are built), its own 256
-// elements of x (the commutator's neighbours), and the coefficient records
-// the top tier uses; the couplings are then read U at a time (U = the top
-// tier's coupling count, at most 4 per step).
+// it covers (at most 257, checked when the tables are built) and its own 256
+// elements of x (the commutator's neighbours); the couplings are then read U
+// at a time (U = min(the top tier's coupling count, 2): two couplings in
+// flight keep the kernel at 72 VGPRs, 7 waves per SIMD; the coefficient
+// records come from L2, which holds them for every block).
 // LDS carve-up of the light and heavy bodies (they share one buffer).
 constexpr size_t kAlign16(size_t b) { return (b + 15) / 16 * 16; }
 constexpr size_t kLightOffP0 = kAlign16(257 * 8), kLightOffP1 = kLightOffP0 + kAlign16(257 * 8),
                  kLightOffSec = kLightOffP1 + kAlign16(257 * 8), kLightOffGam = kLightOffSec + kAlign16(257 * 4),
-                 kLightOffX = kLightOffGam + kAlign16(257 * 16), kLightOffRec = kLightOffX + 256 * 16,
-                 kLightSmem = kLightOffRec + kLightRec * sizeof(CoefLR);
+                 kLightOffX = kLightOffGam + kAlign16(257 * 16), kLightSmem = kLightOffX + 256 * 16;
 constexpr size_t kHeavyOffRi = 256 * 8, kHeavyOffEn = 2 * 256 * 8, kHeavyOffCf = kHeavyOffEn + 256 * sizeof(DevEntryP),
                  kHeavyOffX = kHeavyOffCf + 256 * sizeof(CoefLR), kHeavySmem = kHeavyOffX + 256 * 16;
 constexpr size_t kApplySmem = kLightSmem > kHeavySmem ? kLightSmem : kHeavySmem;
@@ -187,7 +185,6 @@ __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T
   int* s_sec = reinterpret_cast<int*>(smem + kLightOffSec);
   double2* s_gam = reinterpret_cast<double2*>(smem + kLightOffGam);
   V* s_x = reinterpret_cast<V*>(smem + kLightOffX);
-  CoefLR* s_rec = reinterpret_cast<CoefLR*>(smem + kLightOffRec);
   const V* __restrict__ xv = reinterpret_cast<const V*>(x);
   const int lo = op.light_block_ado[bid], hi = op.light_block_ado[bid + 1];
   const int cnt = hi - lo + 1;
@@ -199,9 +196,6 @@ __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T
     s_sec[k] = op.sec[a];
     s_gam[k] = reinterpret_cast<const double2*>(cf_p.gam)[a];
   }
-  const bool rec_staged = op.light_rec_n <= kLightRec;
-  if (rec_staged)
-    for (int k = threadIdx.x; k < op.light_rec_n; k += 256) s_rec[k] = load_coef(cf_p.rec, op.light_rec_lo + k);
   const int64_t win_lo = op.light_begin + (int64_t)bid * 256;
   const int win_n = (int)(op.total - win_lo < 256 ? op.total - win_lo : 256);
   if (threadIdx.x < win_n) s_x[threadIdx.x] = ld<T, TRANS>(xv, win_lo + threadIdx.x, zero_at);
@@ -227,7 +221,7 @@ __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T
     for (int u = 0; u < U; ++u) {
       if (u < nv) {
         en[u] = op.ent[p + u];
-        cf[u] = rec_staged ? s_rec[en[u].rec - op.light_rec_lo] : load_coef(cf_p.rec, en[u].rec);
+        cf[u] = load_coef(cf_p.rec, en[u].rec);
       } else {
         en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
         cf[u] = CoefLR{};
@@ -247,7 +241,7 @@ __device__ void tier0_fixup_block(const DevPacked& op, const T* __restrict__ x, 
 
 // ADOs below the top tier: one workgroup per ADO.  The block's 256 lanes are
 // laid out as (slice, slot): S = 256 / nnz slices, each walking every S-th
-// coupling of the ADO (entries and coefficients staged in LDS), four
+// coupling of the ADO (entries and coefficients staged in LDS), two
 // couplings per step; the slices are summed through LDS in a fixed order.
 // Block 0 (the tier-0 ADO) then applies the trace row and Hermitianises.
 template <typename T, bool TRANS>
@@ -278,7 +272,7 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
   const int win_n = nnz < 256 ? nnz : 256;
   if (threadIdx.x < win_n) s_x[threadIdx.x] = ld<T, TRANS>(xv, base + threadIdx.x, zero_at);
   __syncthreads();
-  constexpr int U = 4;
+  constexpr int U = 2;
   for (int t0 = 0; t0 < nnz; t0 += 256) {   // nnz is uniform over the block: every thread takes the same trips
     const int w = nnz - t0 < 256 ? nnz - t0 : 256;   // slots in this pass
     const int S = 256 / w;
@@ -654,9 +648,7 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
       hipLaunchKernelGGL((k_apply_packed<T, false, UU>), grid, dim3(256), 0, s, op, x, y, zero_at, r);   \
   } while (0)
     if (u <= 1) HEOMQ_APPLY(1);
-    else if (u == 2) HEOMQ_APPLY(2);
-    else if (u == 3) HEOMQ_APPLY(3);
-    else HEOMQ_APPLY(4);
+    else HEOMQ_APPLY(2);
 #undef HEOMQ_APPLY
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
