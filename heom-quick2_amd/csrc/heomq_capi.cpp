@@ -703,6 +703,8 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
   c->sgn = m.sgn; c->spin = m.spin; c->orb = m.orb; c->alf = m.alf;
   c->ams = m.ams;
   c->tiers = m.tiers;
+  c->steady_model = m.steady;
+  c->field_amp = m.eleadinfty;
   // tier-1 position of each sigma=+ drawer (td/calcj.f90 reads these ADOs)
   c->tier1_of.assign(m.M, -1);
   c->tier1_dag.assign(m.M, 0);
@@ -768,6 +770,8 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
   fill_op(c, c->trn, c->rec, true, +1.0);
   sync_packed(c);
   if (int rc = build_jacobi(c, h->m)) return rc;
+  c->steady_model = h->m.steady;
+  c->field_amp = h->m.eleadinfty;
   HIPCHECK(hipDeviceSynchronize());
   return HEOMQ_OK;
 }

@@ -126,6 +126,17 @@ struct heomq_ctx {
   // degeneracy check) and the iteration's own workspace
   DevBuf j_u, j_e, j_grp, j_gn, ws_jac;
   DevBuf j_crow, j_cslot, j_ca, j_cc, j_rrow, j_rslot, j_rb, j_rd, j_tde;   // slot-space list tables
+  // time-dependent lead energy shift eshift(alpha, s; t) of the propagators
+  // (main/getengyshift.f90): field_type -1 none, 0 amps (1 - exp(-t / tchar)),
+  // 1 amps sin(2 pi t / tchar) + amps_dc; zero for t >= t_off.  Amplitudes
+  // are the model's lead shifts (1/fs); td_cnt[a][alpha*nspin + s] sums
+  // dipm over ADO a's drawers of lead alpha and spin s; td_gam holds
+  // gamma_K(t) for the apply
+  int field_type = -1;
+  std::vector<double> field_amp, field_tchar, field_dc;
+  double field_toff = 1e99;
+  bool steady_model = true;
+  DevBuf td_cnt, td_gam;
   // sparse-storage drop-in (calc_ax_tfqmr_spa_omp): COO element -> dense
   // index, positions of the tier-0 diagonal, dense scratch; spa_lunk < 0: no pattern
   DevBuf spa_map, spa_diag, spa_dense, spa_io;
@@ -164,4 +175,8 @@ void build_gamma(const Model& m, const std::vector<cplx>& gdraw, std::vector<dou
 int build_jacobi(heomq_ctx* c, const Model& m);
 // default block-Jacobi preconditioner shift, as a multiple of the iteration's ejac
 constexpr double kPrecShiftFactor = 0.05;
+// The forward operator at time t (the context's own when no time-dependent
+// field is set; else gamma_K(t) written into the context's td_gam on st).
+int operator_at(heomq_ctx* c, double t, hipStream_t st, DevOperator& op);
+void field_shifts(const heomq_ctx* c, double t, double* es);
 }  // namespace heomq

@@ -636,7 +636,7 @@ int heomq_packed_info(const heomq_ctx* c, int64_t* info) {
   return HEOMQ_OK;
 }
 
-int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream) {
+int heomq_rk4_at(heomq_ctx* c, double* x, double t0, double dt, int nstep, void* stream) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
   const int64_t n = 2 * c->N * c->d * c->d;
@@ -644,21 +644,33 @@ int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream) {
   if (int rc = carve<double>(c->ws64, n, 3, v)) return rc;
   double *rho0 = v[0], *tmp = v[1], *k = v[2];
   const double dt2 = dt / 2, dt3 = dt / 3, dt6 = dt / 6;
+  // the derivative at time t (td/intstp.f90 calls calcderiv_omp(tt), (tt + dt2) twice, (tt + dt))
+  auto deriv = [&](double t, const double* in, double* out) -> int {
+    DevOperator op;
+    if (int rc = operator_at(c, t, st, op)) return rc;
+    HIPCHECK(launch_apply_f64(op, in, out, -1, st));
+    return HEOMQ_OK;
+  };
   for (int step = 0; step < nstep; ++step) {   // td/intstp.f90:20-100
+    const double t = t0 + step * dt;
     HIPCHECK(hipMemcpyAsync(rho0, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    HIPCHECK(launch_apply_f64(c->fwd.op, rho0, k, -1, st));
+    if (int rc = deriv(t, rho0, k)) return rc;
     HIPCHECK(vec_lincomb3(st, n, tmp, 1.0, rho0, dt2, k, 0.0, nullptr));
     HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, dt6, k, 0.0, nullptr));
-    HIPCHECK(launch_apply_f64(c->fwd.op, tmp, k, -1, st));
+    if (int rc = deriv(t + dt2, tmp, k)) return rc;
     HIPCHECK(vec_lincomb3(st, n, tmp, 1.0, rho0, dt2, k, 0.0, nullptr));
     HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, dt3, k, 0.0, nullptr));
-    HIPCHECK(launch_apply_f64(c->fwd.op, tmp, k, -1, st));
+    if (int rc = deriv(t + dt2, tmp, k)) return rc;
     HIPCHECK(vec_lincomb3(st, n, tmp, 1.0, rho0, dt, k, 0.0, nullptr));
     HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, dt3, k, 0.0, nullptr));
-    HIPCHECK(launch_apply_f64(c->fwd.op, tmp, k, -1, st));
+    if (int rc = deriv(t + dt, tmp, k)) return rc;
     HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, dt6, k, 0.0, nullptr));
   }
   return HEOMQ_OK;
+}
+
+int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream) {
+  return heomq_rk4_at(c, x, 0.0, dt, nstep, stream);
 }
 
 // Chebyshev propagation with the reference's recurrence P_n = P_{n-2} +

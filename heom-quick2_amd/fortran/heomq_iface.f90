@@ -10,7 +10,8 @@
 !   calc_ax_bicg_omp(dim0, rcgtmp, rcgrhs)    bicg/calc_ax_bicg_omp.f90:1,16-17
 !   calc_atrx_bicg_omp(dim0, rcgtmp, rcgrhs)  bicg/calc_atrx_bicg_omp.f90:1,35-36
 !   calcderiv_omp(tt)                         td/calcderiv_omp.f90:1-7 (module arrays
-!                                             rhotmp -> rhorhs, registered once)
+!                                             rhotmp -> rhorhs, registered once; the
+!                                             lead field of heomq_fortran_set_field at tt)
 !   solve_tfqmr_gpu(rho, crit, maxit, info)   solver-level replacement of solve_tfqmr
 !   solve_bicg_gpu(rho, crit, maxit, info)    solver-level replacement of solvebicg
 !   calc_ax_jacobi_omp(dim0, rcgtmp, rcgrhs)  jacobi/calc_ax_jacobi_omp.f90:1,15-17
@@ -120,6 +121,20 @@ module heomq_iface
       real(c_double), value :: crit
       integer(c_int), value :: maxit
       real(c_double) :: stats(*)
+    end function
+
+    integer(c_int) function c_calcderiv_tt(ctx, tt, x, y) bind(C, name='heomq_calcderiv_tt')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, x, y
+      real(c_double), value :: tt
+    end function
+
+    integer(c_int) function c_set_field(ctx, ftype, tchar, dc, toff) bind(C, name='heomq_ctx_set_field')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ftype
+      real(c_double) :: tchar(*), dc(*)
+      real(c_double), value :: toff
     end function
 
     integer(c_int) function c_calcderiv(ctx, x, y) bind(C, name='heomq_calcderiv')
@@ -241,10 +256,32 @@ subroutine calcderiv_omp(tt)
     write(6,*) ' error! calcderiv_omp: rhotmp/rhorhs not registered '
     error stop 1
   end if
-  rc = c_calcderiv(heomq_ctx_h, td_rhotmp, td_rhorhs)
+  rc = c_calcderiv_tt(heomq_ctx_h, real(tt, c_double), td_rhotmp, td_rhorhs)
   if (rc /= 0) call heomq_fail('calcderiv_omp', rc)
-  if (tt < 0.d0) write(6,*) ' calcderiv_omp: fixed-H propagation, tt ignored '
 end subroutine calcderiv_omp
+
+! The time-dependent lead shift of main/getengyshift.f90 (module variables
+! fieldtype, tchar(nalf,nspin) in fs, amps_dc(nalf,nspin) in 1/fs as
+! evaluatepara.f90:2372 stores it, t_off); the amplitudes are the engyshift
+! the context was built with.  Call once after heomq_fortran_init for a
+! time-dependent run (igroundsteady = 0).
+subroutine heomq_fortran_set_field(fieldtype, nalf, nspin, tchar, amps_dc, t_off)
+  use heomq_iface
+  implicit none
+  integer, intent(in) :: fieldtype, nalf, nspin
+  real*8, intent(in) :: tchar(nalf, nspin), amps_dc(nalf, nspin), t_off
+  real(c_double) :: tc(nalf * nspin), dc(nalf * nspin)
+  integer :: a, s
+  integer(c_int) :: rc
+  do a = 1, nalf
+    do s = 1, nspin
+      tc((a - 1) * nspin + s) = tchar(a, s)
+      dc((a - 1) * nspin + s) = amps_dc(a, s) * 0.658211928d0   ! back to eV (include/sizes:23)
+    end do
+  end do
+  rc = c_set_field(heomq_ctx_h, int(fieldtype, c_int), tc, dc, real(t_off, c_double))
+  if (rc /= 0) call heomq_fail('heomq_fortran_set_field', rc)
+end subroutine heomq_fortran_set_field
 
 ! Solver-level drop-in for tfqmr_double/solve_tfqmr.f90: rho (nrho,nrho,nunk)
 ! holds the initial guess on entry and the steady state on exit; crit is the

@@ -541,9 +541,28 @@ class Context:
             raise HeomqError(rc, lib().heomq_last_error().decode() + f" {info}")
         return x, info
 
-    def rk4(self, x, dt, nstep):
-        check(lib().heomq_rk4(self._h, self._check_vec(x, self.torch.complex128), dt, nstep, self._stream()))
+    def rk4(self, x, dt, nstep, t0=0.0):
+        """RK4 (td/intstp.f90) from time t0; the lead field, when set, enters at each stage time."""
+        check(lib().heomq_rk4_at(self._h, self._check_vec(x, self.torch.complex128), float(t0), dt, nstep,
+                                 self._stream()))
         return x
+
+    def set_field(self, fieldtype, tchar=None, amps_dc=None, t_off=1e99):
+        """Time-dependent lead shift eshift(alpha, s; t) (main/getengyshift.f90): fieldtype 0
+        amps (1 - exp(-t/tchar)), 1 amps sin(2 pi t/tchar) + amps_dc, -1 none; tchar in fs and
+        amps_dc in eV, shaped (nalf, nspin); the amplitudes are the model's engyshift."""
+        tc = None if tchar is None else np.ascontiguousarray(tchar, np.float64).ravel()
+        dc = None if amps_dc is None else np.ascontiguousarray(amps_dc, np.float64).ravel()
+        check(lib().heomq_ctx_set_field(self._h, int(fieldtype), None if tc is None else _dp(tc),
+                                        None if dc is None else _dp(dc), float(t_off)))
+
+    def calcderiv(self, x, tt=0.0, y=None):
+        """dx/dt at time tt (calcderiv_omp(tt)): the apply with no trace row and gamma_K(tt)."""
+        t = self.torch
+        y = self.zeros() if y is None else y
+        check(lib().heomq_calcderiv_at(self._h, float(tt), self._check_vec(x, t.complex128),
+                                       self._check_vec(y, t.complex128), self._stream()))
+        return y
 
     def chebyshev(self, x, dt, nstep):
         stats = np.zeros(4)

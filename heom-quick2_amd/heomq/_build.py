@@ -19,7 +19,7 @@ LIB_PATH = PKG_DIR / "libheomq.so"
 FLIB_PATH = PKG_DIR / "libheomq_fortran.so"
 
 SOURCES = ["heomq_model.cpp", "heomq_apply.hip", "heomq_packed.hip", "heomq_solver.hip", "heomq_capi.cpp",
-           "heomq_solve.cpp", "heomq_batch.cpp", "heomq_jacobi.cpp", "heomq_spa.hip"]
+           "heomq_solve.cpp", "heomq_batch.cpp", "heomq_jacobi.cpp", "heomq_spa.hip", "heomq_field.hip"]
 
 
 def _hipcc() -> str:

@@ -99,6 +99,10 @@ SIGNATURES = {
     "heomq_batch_apply_packed": (C.c_int, [P, P, P, C.c_int, P]),
     "heomq_batch_solve": (C.c_int, [P, C.c_double, C.c_int, D, D, D, D, P]),
     "heomq_rk4": (C.c_int, [P, P, C.c_double, C.c_int, P]),
+    "heomq_rk4_at": (C.c_int, [P, P, C.c_double, C.c_double, C.c_int, P]),
+    "heomq_ctx_set_field": (C.c_int, [P, C.c_int, D, D, C.c_double]),
+    "heomq_calcderiv_at": (C.c_int, [P, C.c_double, P, P, P]),
+    "heomq_calcderiv_tt": (C.c_int, [P, C.c_double, D, D]),
     "heomq_chebyshev": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_observables": (C.c_int, [P, P, D, D, P]),
 }

@@ -248,6 +248,25 @@ int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* 
 /* The same on a host array (the solvebicg drop-in of the Fortran layer). */
 int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats);
 /*
+ * Time-dependent lead energy shift of the propagators (main/getengyshift.f90,
+ * td/calcderiv_omp.f90:51-61,144-148): gamma_K(t) = sum_k cgamma_k +
+ * i dipm_k eshift(alpha_k, s_k; t) with eshift = amps (1 - exp(-t/tchar))
+ * (fieldtype 0) or amps sin(2 pi t / tchar) + amps_dc (fieldtype 1), zero
+ * for t >= t_off; amps are the model's lead shifts (engyshift).  Needs a
+ * model built with steady = 0 (no static bias in gamma).  tchar, amps_dc:
+ * nalf * nspin entries, [alpha * nspin + s], fs and eV (amps_dc may be NULL);
+ * fieldtype -1 removes the field.  Pulsed shifts (fieldtype 2) are
+ * HEOMQ_UNSUPPORTED.
+ */
+int heomq_ctx_set_field(heomq_ctx* c, int fieldtype, const double* tchar, const double* amps_dc, double t_off);
+/* y = L(tt) x (dense device vectors), the derivative calcderiv_omp(tt) forms. */
+int heomq_calcderiv_at(heomq_ctx* c, double tt, const double* x, double* y, void* stream);
+/* calcderiv_omp(tt) on host arrays (the Fortran drop-in's backend). */
+int heomq_calcderiv_tt(heomq_ctx* c, double tt, const double* rhotmp_host, double* rhorhs_host);
+/* RK4 of td/intstp.f90 from time t0 (the derivative at t, t + dt/2 twice, t + dt). */
+int heomq_rk4_at(heomq_ctx* c, double* x, double t0, double dt, int nstep, void* stream);
+
+/*
  * Sparse-storage drop-in (tfqmr_double/calc_ax_tfqmr_spa_omp.f90): ADOs
  * stored as COO lists over a fixed pattern (module sparsemod, built by
  * fermion/prelude_spa.f90:532-749).  y = P A P^T x with P the pattern's

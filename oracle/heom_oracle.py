@@ -951,6 +951,44 @@ def rk4(model: Model, x, dt, nstep):
     return x
 
 
+def engy_shift(model: Model, t, fieldtype, tchar, amps_dc=None, t_off=1e99):
+    """main/getengyshift.f90 for every (lead, spin), in 1/fs: fieldtype 0
+    amps (1 - exp(-t / tchar)), 1 amps sin(2 pi t / tchar) + amps_dc (eV in),
+    -1 zero; zero for t >= t_off.  amps = engyshift / hbar, the amplitude the
+    steady bias uses (evaluatepara.f90:2296,2431)."""
+    amps = model.eleadinfty
+    if t >= t_off or fieldtype == -1:
+        return np.zeros_like(amps)
+    tchar = np.asarray(tchar, float)
+    if fieldtype == 0:
+        return amps * (1.0 - np.exp(-t / tchar))
+    if fieldtype == 1:
+        dc = 0.0 if amps_dc is None else np.asarray(amps_dc, float) / HBAR
+        return amps * np.sin(2.0 * np.pi * t / tchar) + dc
+    raise ValueError("fieldtype")
+
+
+def gdraw_at(model: Model, eshift):
+    """Per-drawer rate with a lead shift (td/calcderiv_omp.f90:144-148):
+    cgamma + i dipm eshift(alpha, s)."""
+    return model.cgamma + 1j * model.dipm * eshift[model.alf, model.spin]
+
+
+def rk4_field(model: Model, x, t0, dt, nstep, field):
+    """td/intstp.f90 with the derivative calcderiv_omp(tt) at t, t + dt/2 (twice)
+    and t + dt; field = (fieldtype, tchar, amps_dc, t_off)."""
+    f = lambda t, v: apply(model, v, trace_row=None, gdraw=gdraw_at(model, engy_shift(model, t, *field)))
+    x = x.copy()
+    for step in range(nstep):
+        t = t0 + step * dt
+        k1 = f(t, x)
+        k2 = f(t + dt / 2, x + dt / 2 * k1)
+        k3 = f(t + dt / 2, x + dt / 2 * k2)
+        k4 = f(t + dt, x + dt * k3)
+        x = x + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return x
+
+
 def ado_gamma(model: Model):
     """gamma_K = sum of the drawers' gamma (incl. steady bias) per ADO."""
     g = np.zeros(model.N, complex)

@@ -1,0 +1,88 @@
+"""GPU parity of the time-dependent lead shift in the propagators:
+calcderiv_omp(tt) and the RK4 of td/intstp.f90 with eshift(alpha, s; t) of
+main/getengyshift.f90 (fieldtype 0 and 1), against the oracle."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import heomq
+from oracle import heom_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "siam_nk2_L2": (lambda: heomq.siam(nk=2, ntop=2, V=0.6, U=1.5, T=0.2),
+                    lambda: O.siam_params(nk=2, ntop=2, V=0.6, U=1.5, T=0.2)),
+    "dqd_nk1_L2": (lambda: heomq.two_impurity(nk=1, ntop=2), lambda: O.dqd_params(nk=1, ntop=2)),
+}
+
+_cache = {}
+
+
+def setup(name):
+    if name not in _cache:
+        import torch
+        pf, of = CASES[name]
+        m = heomq.Model(dataclasses.replace(pf(), steady=False))
+        ctx = heomq.Context(m)
+        o = O.Model(dataclasses.replace(of(), steady=False))
+        _cache[name] = (m, ctx, o, torch)
+    return _cache[name]
+
+
+def to_dev(torch, x):
+    return torch.from_numpy(O.to_fortran(x)).to("cuda:0")
+
+
+def from_dev(t):
+    return O.from_fortran(t.cpu().numpy())
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+FIELDS = {
+    "switch_on": (0, np.array([[3.0, 3.0], [5.0, 5.0]]), None, 1e99),
+    "ac_dc": (1, np.array([[2.0, 2.5], [3.0, 3.5]]), np.array([[0.05, 0.05], [-0.02, -0.02]]), 1e99),
+    "off": (0, np.array([[3.0, 3.0], [5.0, 5.0]]), None, 0.5),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("fname", list(FIELDS))
+def test_calcderiv_at_time(name, fname):
+    m, ctx, o, torch = setup(name)
+    ft, tc, dc, toff = FIELDS[fname]
+    print(f"case {name} field {fname}", flush=True)
+    ctx.set_field(ft, tc, dc, toff)
+    try:
+        x = O.seeded_ado_vector(o, 17)
+        for tt in (0.0, 0.7, 2.3):
+            y = from_dev(ctx.calcderiv(to_dev(torch, x), tt=tt))
+            yo = O.apply(o, x, trace_row=None, gdraw=O.gdraw_at(o, O.engy_shift(o, tt, ft, tc, dc, toff)))
+            assert rel(y, yo) <= 1e-12
+    finally:
+        ctx.set_field(-1)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_rk4_with_field(name):
+    m, ctx, o, torch = setup(name)
+    ft, tc, dc, toff = FIELDS["ac_dc"]
+    ctx.set_field(ft, tc, dc, toff)
+    try:
+        x0 = O.physical_projection(o, O.seeded_ado_vector(o, 18))
+        x = from_dev(ctx.rk4(to_dev(torch, x0), 0.02, 10, t0=0.3))
+        xo = O.rk4_field(o, x0, 0.3, 0.02, 10, (ft, tc, dc, toff))
+        assert rel(x, xo) <= 1e-12
+    finally:
+        ctx.set_field(-1)
+
+
+def test_field_rejected_on_a_steady_model():
+    ctx = heomq.Context(heomq.Model(heomq.siam(nk=1, ntop=1)))
+    with pytest.raises(heomq.HeomqError, match="steady"):
+        ctx.set_field(0, np.ones((2, 2)))
+    ctx.close()

@@ -464,3 +464,34 @@ def test_apply_spa_is_the_restricted_dense_matrix():
             want = np.array([col[idx(a, i - 1, j - 1)] + 1j * col[idx(a, i - 1, j - 1) + 1]
                              for a, i, j in zip(a_of, pat[2], pat[3])])
             assert np.abs(y[:-1] - want).max() <= 1e-12 * max(np.abs(A).max(), 1.0)
+
+
+# ---- time-dependent lead shift (getengyshift, calcderiv_omp(tt), intstp) ----
+def _td_pair():
+    import dataclasses
+    p = O.siam_params(nk=1, ntop=2, V=0.6, U=1.5, T=0.2)
+    return O.Model(dataclasses.replace(p, steady=False)), O.Model(p)
+
+
+def test_field_limits_are_the_steady_bias():
+    # the exponential switch-on reaches the steady bias for t >> tchar, the
+    # sine shift equals it at a quarter period and vanishes at half a period
+    td, st = _td_pair()
+    tc = np.full((2, 2), 3.0)
+    assert np.abs(O.gdraw_at(td, O.engy_shift(td, 1e6, 0, tc)) - st.gdraw).max() <= 1e-14 * np.abs(st.gdraw).max()
+    assert np.abs(O.gdraw_at(td, O.engy_shift(td, 0.75, 1, tc)) - st.gdraw).max() <= 1e-14 * np.abs(st.gdraw).max()
+    assert np.abs(O.gdraw_at(td, O.engy_shift(td, 1.5, 1, tc)) - td.cgamma).max() <= 1e-14 * np.abs(st.gdraw).max()
+    assert np.abs(O.engy_shift(td, 5.0, 0, tc, t_off=4.0)).max() == 0.0
+
+
+def test_rk4_with_field_is_fourth_order():
+    # a time-dependent rate must be sampled at the RK4 stage times for the
+    # global error to fall 16x per halving of dt
+    td, _ = _td_pair()
+    x0 = O.physical_projection(td, O.seeded_ado_vector(td, 6))
+    field = (1, np.full((2, 2), 2.0), None, 1e99)
+    T = 0.8
+    ref = O.rk4_field(td, x0, 0.1, T / 256, 256, field)
+    e1 = np.abs(O.rk4_field(td, x0, 0.1, T / 16, 16, field) - ref).max()
+    e2 = np.abs(O.rk4_field(td, x0, 0.1, T / 32, 32, field) - ref).max()
+    assert 12.0 < e1 / e2 < 20.0
