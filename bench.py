@@ -90,15 +90,15 @@ def main():
         ctx.solve_steady(tol=args.tol)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    stats = []
-    rdo_err = 0.0
+    stats, rdos = [], []
     for _ in range(args.steps):
         x, st = ctx.solve_steady(tol=args.tol)
         stats.append(st)
-        if rho_ref is not None:
-            rdo_err = max(rdo_err, (x[0] - rho_ref).abs().max().item())
+        rdos.append(x[0].clone())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    # outside the timed region: torch's first use of these kernels loads them
+    rdo_err = max((r - rho_ref).abs().max().item() for r in rdos) if rho_ref is not None else 0.0
     solves_per_s = args.steps / elapsed
     cur, occ = ctx.observables(x)
 

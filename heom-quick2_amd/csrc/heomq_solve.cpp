@@ -126,6 +126,8 @@ int chunk_graph(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, boo
       *out = g.exec;
       return HEOMQ_OK;
     }
+  if (std::getenv("HEOMQ_SOLVER_LOG"))
+    std::fprintf(stderr, "tfqmr: capturing a chunk graph (mode %d, %zu cached)\n", mode, c->graphs.size());
   HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   const hipError_t rec = record_chunk<T>(c, x, W, n, st, A, iters);
   hipGraph_t graph = nullptr;
