@@ -147,6 +147,9 @@ struct heomq_ctx {
   // shift of the block-Jacobi preconditioner of TFQMR: (shift - S_K)^-1 per
   // ADO; prec_shift_user 0 selects kPrecShiftFactor * ejac_default
   double prec_shift = 0.0, prec_shift_user = 0.0;
+  // seed of TFQMR's hashed +-1 shadow vector at a solve's first start (the
+  // reference draws it from DRANDN with seed 1, tfqmr_double/dutfx.f90:178)
+  uint64_t shadow_seed = 0;
   int jac_nnz = 0;
   std::vector<double> gam_h;
   void drop_graphs() {

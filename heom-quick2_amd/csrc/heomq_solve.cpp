@@ -226,7 +226,8 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
         if (int rc = true_resid(rr, bb)) return rc;
       }
     }
-    HIPCHECK(tfq_reset(st, s, tol * std::sqrt(bb), nullptr, 1, base_iters == 0 && std::isinf(best_before)));
+    HIPCHECK(tfq_reset(st, s, tol * std::sqrt(bb), nullptr, 1, base_iters == 0 && std::isinf(best_before),
+                       c->shadow_seed));
     if (prec) {
       HIPCHECK(Ait(x, W.ax));
       rs.applies++;
@@ -549,6 +550,12 @@ int heomq_ctx_set_precond_shift(heomq_ctx* c, double shift) {
   c->drop_graphs();   // captured chunks hold the shift as a kernel argument
   c->prec_shift_user = shift;
   c->prec_shift = v;
+  return HEOMQ_OK;
+}
+
+int heomq_ctx_set_shadow_seed(heomq_ctx* c, uint64_t seed) {
+  if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
+  c->shadow_seed = seed;
   return HEOMQ_OK;
 }
 

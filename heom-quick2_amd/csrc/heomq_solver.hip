@@ -399,11 +399,12 @@ __global__ void k_hermitize0(int d, T* __restrict__ x) {
 // Problems p < nprob with mask[p] != 0 (every problem if mask is null) are set
 // up for a (re)start: scalars cleared, flag = kInitRequested.  Problems left
 // out keep their state (a frozen problem stays frozen).
-__global__ void k_tfq_reset(TfqmrScal* s, double target, const int32_t* mask, int nprob, int fresh) {
+__global__ void k_tfq_reset(TfqmrScal* s, double target, const int32_t* mask, int nprob, int fresh,
+                            uint64_t first_seed) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nprob || (mask && !mask[p])) return;
   TfqmrScal* q = s + p;
-  q->seed = fresh ? 0 : q->seed + 1;
+  q->seed = fresh ? first_seed : q->seed + 1;
   q->rho = q->sigma = q->alpha = q->beta = 0;
   q->theta = q->tau = q->eta = q->dcoef = 0;
   q->rr = 0;
@@ -554,8 +555,10 @@ hipError_t vec_project(hipStream_t st, int64_t nado, int d, T* x, const int8_t* 
 template hipError_t vec_project<double>(hipStream_t, int64_t, int, double*, const int8_t*, const TierBounds&, bool);
 template hipError_t vec_project<float>(hipStream_t, int64_t, int, float*, const int8_t*, const TierBounds&, bool);
 
-hipError_t tfq_reset(hipStream_t st, TfqmrScal* s, double target, const int32_t* mask, int nprob, bool fresh) {
-  hipLaunchKernelGGL(k_tfq_reset, dim3((nprob + 63) / 64), dim3(64), 0, st, s, target, mask, nprob, fresh ? 1 : 0);
+hipError_t tfq_reset(hipStream_t st, TfqmrScal* s, double target, const int32_t* mask, int nprob, bool fresh,
+                     uint64_t first_seed) {
+  hipLaunchKernelGGL(k_tfq_reset, dim3((nprob + 63) / 64), dim3(64), 0, st, s, target, mask, nprob, fresh ? 1 : 0,
+                     first_seed);
   return hipGetLastError();
 }
 hipError_t tfq_scale_targets(hipStream_t st, TfqmrScal* s, const double* rr_true, int nprob) {

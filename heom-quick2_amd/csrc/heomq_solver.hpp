@@ -86,9 +86,9 @@ hipError_t vec_resid(hipStream_t, int64_t n, const TB* b, const TA* ax, double* 
                      Batch bt = Batch{});
 hipError_t vec_resid_to_f32(hipStream_t, int64_t n, const double* b, const double* ax, float* r32, double scale);
 hipError_t vec_axpy_f32_to_f64(hipStream_t, int64_t n, double* x, const float* dx, double alpha);
-// fresh: the first start of a solve (shadow seed 0); otherwise a restart (next seed)
+// fresh: the first start of a solve (shadow seed first_seed); otherwise a restart (next seed)
 hipError_t tfq_reset(hipStream_t, TfqmrScal* s, double target, const int32_t* mask = nullptr, int nprob = 1,
-                     bool fresh = true);
+                     bool fresh = true, uint64_t first_seed = 0);
 hipError_t tfq_freeze(hipStream_t, TfqmrScal* s, const int32_t* mask, int nprob);
 // unscaled[p] = mask[p] (0 for all with a null mask)
 hipError_t tfq_set_unscaled(hipStream_t, TfqmrScal* s, const int32_t* mask, int nprob);

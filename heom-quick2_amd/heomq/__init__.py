@@ -326,6 +326,10 @@ class Context:
         if shift is not None:
             check(lib().heomq_ctx_set_precond_shift(self._h, float(shift)))
 
+    def set_shadow_seed(self, seed: int):
+        """Seed of TFQMR's +-1 shadow vector at each solve's first start."""
+        check(lib().heomq_ctx_set_shadow_seed(self._h, int(seed)))
+
     def set_graphs(self, enable: bool):
         """Replay solver iterations from captured graphs (default) or launch eagerly."""
         check(lib().heomq_ctx_set_graphs(self._h, int(enable)))

@@ -61,6 +61,7 @@ SIGNATURES = {
     "heomq_ctx_set_graphs": (C.c_int, [P, C.c_int]),
     "heomq_ctx_set_precond": (C.c_int, [P, C.c_int]),
     "heomq_ctx_set_precond_shift": (C.c_int, [P, C.c_double]),
+    "heomq_ctx_set_shadow_seed": (C.c_int, [P, C.c_uint64]),
     "heomq_apply": (C.c_int, [P, P, P, C.c_int, P]),
     "heomq_apply_transpose": (C.c_int, [P, P, P, C.c_int, P]),
     "heomq_apply_f32": (C.c_int, [P, P, P, C.c_int, P]),

@@ -158,6 +158,10 @@ int heomq_ctx_set_precond(heomq_ctx* c, int mode);
 /* Shift of the block-Jacobi preconditioner (> 0), or 0 for the default:
  * 0.05 x the block-Jacobi iteration's default ejac. */
 int heomq_ctx_set_precond_shift(heomq_ctx* c, double shift);
+/* Seed of TFQMR's +-1 shadow vector at the first start of every solve
+ * (restarts take the following seeds); default 0.  The reference draws its
+ * shadow vector with DRANDN, seed 1 (tfqmr_double/dutfx.f90:178). */
+int heomq_ctx_set_shadow_seed(heomq_ctx* c, uint64_t seed);
 /* Number of complex elements N*d*d of one ADO vector. */
 int64_t heomq_ctx_vector_len(const heomq_ctx* c);
 
