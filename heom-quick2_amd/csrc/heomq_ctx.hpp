@@ -126,6 +126,7 @@ struct heomq_ctx {
   // degeneracy check) and the iteration's own workspace
   DevBuf j_u, j_e, j_grp, j_gn, ws_jac;
   DevBuf j_crow, j_cslot, j_ca, j_cc, j_rrow, j_rslot, j_rb, j_rd, j_tde;   // slot-space list tables
+  DevBuf j_blk, j_desc;                                                        // ADO-aligned form
   // time-dependent lead energy shift eshift(alpha, s; t) of the propagators
   // (main/getengyshift.f90): field_type -1 none, 0 amps (1 - exp(-t / tchar)),
   // 1 amps sin(2 pi t / tchar) + amps_dc; zero for t >= t_off.  Amplitudes

@@ -191,8 +191,8 @@ int sweep(heomq_ctx* c, const DevJacobi& jc, double* x, double* r, double* t, hi
   DevPacked op = c->pfwd;
   op.precond = 0;
   HIPCHECK(launch_apply_packed_f64(op, x, r, false, st));
-  if (jc.windowed) {
-    HIPCHECK(launch_jacobi_win(op, jc, r, x, true, st));
+  if (jacobi_one_kernel(jc)) {
+    HIPCHECK(launch_jacobi_1k(op, jc, r, x, true, st));
     return HEOMQ_OK;
   }
   HIPCHECK(launch_jacobi_fwd(op, jc, r, t, st));
@@ -329,6 +329,53 @@ int build_jacobi(heomq_ctx* c, const Model& m) {
       if (hi - lo + 1 > 257 || c->poff_h[hi + 1] - c->poff_h[lo] > kJacobiWindow) fits = false;
     }
     c->pjac.windowed = fits ? 1 : 0;
+  }
+  // the ADO-aligned form: runs of whole ADOs of at most 256 elements and 256
+  // ADOs per block, element descriptors with groups of at most 4 states
+  {
+    const int S = c->nsec;
+    bool ok = d <= 16 && jh.gmax <= 4;
+    std::vector<int64_t> blk;
+    for (int64_t a = 0; a < m.N && ok; ++a) {
+      const int64_t sz = c->poff_h[a + 1] - c->poff_h[a];
+      if (sz > 256) ok = false;
+      else if (blk.empty() || c->poff_h[a + 1] - blk[blk.size() - 1] > 256 || a - blk[blk.size() - 2] >= 256) {
+        blk.push_back(a);
+        blk.push_back(c->poff_h[a]);
+      }
+    }
+    std::vector<JacDesc> desc(ok ? (size_t)S * dd : 0);
+    for (int s = 0; s < S && ok; ++s)
+      for (int t = 0; t < c->sec_nnz_h[s]; ++t) {
+        const int e = c->sec_elems_h[(size_t)s * dd + t], r = e % d, col = e / d;
+        JacDesc& ds = desc[(size_t)s * dd + t];
+        ds.r = (uint8_t)r;
+        ds.c = (uint8_t)col;
+        ds.nq = (uint8_t)jh.gn[col];
+        ds.np = (uint8_t)jh.gn[r];
+        for (int q = 0; q < jh.gn[col]; ++q) {
+          const int c2 = jh.grp[(size_t)col * jh.gmax + q];
+          ds.c2[q] = (uint8_t)c2;
+          ds.cslot[q] = c->sec_pos_h[(size_t)s * dd + r + c2 * d];
+        }
+        for (int p = 0; p < jh.gn[r]; ++p) {
+          const int r2 = jh.grp[(size_t)r * jh.gmax + p];
+          ds.r2[p] = (uint8_t)r2;
+          ds.rslot[p] = c->sec_pos_h[(size_t)s * dd + r2 + col * d];
+        }
+      }
+    if (ok && !blk.empty() && c->poff_h[m.N] > 0) {
+      const int64_t nb = (int64_t)blk.size() / 2;
+      blk.push_back(m.N);
+      blk.push_back(c->poff_h[m.N]);
+      HIPCHECK(upload(c->j_blk, blk));
+      HIPCHECK(upload(c->j_desc, desc));
+      c->pjac.blk = c->j_blk.as<int64_t>();
+      c->pjac.desc = c->j_desc.as<JacDesc>();
+      c->pjac.ado_blocks = nb;
+    } else {
+      c->pjac.ado_blocks = 0;
+    }
   }
   c->pjac.ejac = jh.ejac;
   c->ejac_default = jh.ejac;

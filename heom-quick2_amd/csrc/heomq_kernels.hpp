@@ -174,6 +174,20 @@ hipError_t launch_lower_tier0_f64(int d, double* x, hipStream_t s);
 // group g are supported on g, so element (i, j) of U^H X U only needs the
 // elements (k, l) of X with k in group(i) and l in group(j), all of which the
 // packed layout stores whenever it stores (i, j).
+// One element (r, c) of a sector in the ADO-aligned block-Jacobi kernel: the
+// slots of (r, c') for the nq states c' of group(c) and of (r', c) for the np
+// states r' of group(r), with those states themselves (entries past nq / np
+// are padding).
+struct alignas(16) JacDesc {
+  int16_t cslot[4];
+  int16_t rslot[4];
+  uint8_t c2[4];
+  uint8_t r2[4];
+  uint8_t r, c, nq, np;
+  uint8_t pad[4];
+};
+static_assert(sizeof(JacDesc) == 32, "JacDesc is two 16-byte loads");
+
 struct DevJacobi {
   const double* u = nullptr;        // d*d complex, column-major: u[k + i*d] = U_ki
   const double* e = nullptr;        // d eigenvalues of H
@@ -199,6 +213,12 @@ struct DevJacobi {
   const double* rb = nullptr;       // complex
   const double* rd = nullptr;       // complex
   const double* tde = nullptr;      // [S][d*d]
+  // ADO-aligned one-kernel form (d <= 16, groups of <= 4 states): block b owns
+  // ADOs [blk[2b], blk[2b+2]), whose packed elements [blk[2b+1], blk[2b+3])
+  // number at most 256; 0 blocks when the layout does not allow it
+  int64_t ado_blocks = 0;
+  const int64_t* blk = nullptr;     // [ado_blocks + 1][2]: first ADO, first packed element
+  const JacDesc* desc = nullptr;    // [S][d*d]
 };
 
 // t = (U^H r U)_ij / (ejac - gamma_K + i (E_i - E_j)) on the packed layout
@@ -211,6 +231,13 @@ hipError_t launch_jacobi_back(const DevPacked& op, const DevJacobi& jc, const do
 // is made exactly Hermitian.
 hipError_t launch_jacobi_win(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
                              hipStream_t s);
+// The same with one block per run of whole ADOs (jc.ado_blocks > 0); in and out must differ.
+hipError_t launch_jacobi_ado(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
+                             hipStream_t s);
+// The one-kernel form the layout allows (ADO-aligned, else the window form).
+inline bool jacobi_one_kernel(const DevJacobi& jc) { return jc.ado_blocks > 0 || jc.windowed; }
+hipError_t launch_jacobi_1k(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
+                            hipStream_t s);
 constexpr int kJacobiWindow = 384;    // LDS window of the one-kernel form (complex elements); d <= 16 needs <= 328
 
 }  // namespace heomq

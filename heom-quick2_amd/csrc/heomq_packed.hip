@@ -611,6 +611,133 @@ __global__ __launch_bounds__(256) void k_jacobi_win(const DevPacked op, const De
   xo[g] = z;
 }
 
+// The same-tier inverse with blocks aligned to ADOs: block b owns whole ADOs
+// (at most 256 elements), one thread per element, so every element is
+// transformed once and the four stages run on LDS.  The element's slot lists
+// (JacDesc) are loaded as soon as its ADO is known, U and the eigenvalues are
+// staged in LDS, and the only dependent global loads are the block bounds, the
+// ADO headers and the descriptor.
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cmulc(double2 a, double2 b) {   // conj(a) b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const DevJacobi jc,
+                                                    const double* __restrict__ in, double* __restrict__ out,
+                                                    int accumulate) {
+  __shared__ double2 xw[256];
+  __shared__ double2 tw[256];
+  __shared__ double2 us[256];        // U, u[k + i d] = U_ki (d <= 16)
+  __shared__ double ew[16];
+  __shared__ int32_t offs[257];      // ADO starts relative to the block's first element
+  __shared__ double2 gamw[256];
+  __shared__ int secw[256];
+  const int tid = threadIdx.x, d = op.d, DD = d * d;
+  const int64_t a_lo = jc.blk[2 * blockIdx.x], w0 = jc.blk[2 * blockIdx.x + 1];
+  const int nado = (int)(jc.blk[2 * blockIdx.x + 2] - a_lo);   // 1..256, checked on the host
+  const int nw = (int)(jc.blk[2 * blockIdx.x + 3] - w0);       // <= 256
+  const double2* xin = reinterpret_cast<const double2*>(in);
+  const bool act = tid < nw;
+  double2 xv = make_double2(0.0, 0.0);
+  if (act) xv = xin[w0 + tid];
+  for (int k = tid; k <= nado; k += 256) {
+    offs[k] = (int32_t)(op.off[a_lo + k] - w0);
+    if (k < nado) {
+      gamw[k] = make_double2(op.gam[2 * (a_lo + k)], op.gam[2 * (a_lo + k) + 1]);
+      secw[k] = op.sec[a_lo + k];
+    }
+  }
+  if (tid < DD) us[tid] = reinterpret_cast<const double2*>(jc.u)[tid];
+  if (tid < d) ew[tid] = jc.e[tid];
+  __syncthreads();
+  // the ADO of this element: the last ia with offs[ia] <= tid
+  int ia = 0;
+  if (act) {
+    int hi = nado - 1;
+    while (ia < hi) {
+      const int mid = (ia + hi + 1) >> 1;
+      if (offs[mid] <= tid) ia = mid;
+      else hi = mid - 1;
+    }
+  }
+  const int base = offs[ia], s = secw[ia], t = tid - base;
+  JacDesc ds{};
+  if (act) ds = jc.desc[s * DD + t];
+  xw[tid] = xv;
+  __syncthreads();
+  // X -> X U_c
+  if (act) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < ds.nq) {
+        const double2 p = cmul(us[ds.c2[q] + ds.c * d], xw[base + ds.cslot[q]]);
+        acc.x += p.x;
+        acc.y += p.y;
+      }
+    tw[tid] = acc;
+  }
+  __syncthreads();
+  // -> (U_r^H .) / (shift - gamma_K + i (E_r - E_c))
+  if (act) {
+    double2 q = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p < ds.np) {
+        const double2 v = cmulc(us[ds.r2[p] + ds.r * d], tw[base + ds.rslot[p]]);
+        q.x += v.x;
+        q.y += v.y;
+      }
+    const double2 ga = gamw[ia];
+    const double wr = jc.ejac - ga.x, wi = (ew[ds.r] - ew[ds.c]) - ga.y;
+    const double den = wr * wr + wi * wi;
+    xw[tid] = make_double2((q.x * wr + q.y * wi) / den, (q.y * wr - q.x * wi) / den);
+  }
+  __syncthreads();
+  // -> . U_c^H
+  if (act) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < ds.nq) {
+        const double2 p = cmulc(us[ds.c + ds.c2[q] * d], xw[base + ds.cslot[q]]);
+        acc.x += p.x;
+        acc.y += p.y;
+      }
+    tw[tid] = acc;
+  }
+  __syncthreads();
+  // -> U_r .
+  double2 z = make_double2(0.0, 0.0);
+  if (act) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p < ds.np) {
+        const double2 v = cmul(us[ds.r + ds.r2[p] * d], tw[base + ds.rslot[p]]);
+        z.x += v.x;
+        z.y += v.y;
+      }
+  }
+  if (!accumulate && a_lo == 0) {   // tier 0: average with conj of the mirror element
+    xw[tid] = z;
+    __syncthreads();
+    const int tm = (act && ia == 0) ? op.sec_pos[s * DD + ds.c + ds.r * d] : -1;
+    if (tm >= 0) {
+      const double2 m = xw[base + tm];
+      z.x = 0.5 * (z.x + m.x);
+      z.y = 0.5 * (z.y - m.y);
+    }
+  }
+  if (!act) return;
+  double2* xo = reinterpret_cast<double2*>(out);
+  if (accumulate) {
+    const double2 w = xo[w0 + tid];
+    z.x += w.x;
+    z.y += w.y;
+  }
+  xo[w0 + tid] = z;
+}
+
 template <typename TP, typename TD>
 __global__ void k_unpack(const DevPacked op, const TP* __restrict__ packed, TD* __restrict__ dense) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -704,6 +831,19 @@ hipError_t launch_jacobi_win(const DevPacked& op, const DevJacobi& jc, const dou
   hipLaunchKernelGGL(k_jacobi_win, dim3((unsigned)((op.total + 255) / 256)), dim3(256), 0, s, op, jc, in, out,
                      accumulate ? 1 : 0);
   return hipGetLastError();
+}
+
+hipError_t launch_jacobi_ado(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
+                             hipStream_t s) {
+  if (jc.ado_blocks <= 0 || in == out || op.nprob != 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_jacobi_ado, dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in, out, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_1k(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
+                            hipStream_t s) {
+  if (jc.ado_blocks > 0 && op.nprob == 1) return launch_jacobi_ado(op, jc, in, out, accumulate, s);
+  return launch_jacobi_win(op, jc, in, out, accumulate, s);
 }
 
 template <typename T>

@@ -36,9 +36,9 @@ hipError_t apply_packed_T<double>(heomq_ctx* c, bool trans, const double* x, dou
   if (prec != 2) return launch_apply_packed_f64(op, x, y, rdo, s);
   DevJacobi jc = c->pjac;
   jc.ejac = c->prec_shift;
-  if (jc.windowed) {   // A x into the scratch, then y = M^-1 (A x) in one kernel
+  if (jacobi_one_kernel(jc)) {   // A x into the scratch, then y = M^-1 (A x) in one kernel
     if (hipError_t e = launch_apply_packed_f64(op, x, scratch, rdo, s)) return e;
-    return launch_jacobi_win(op, jc, scratch, y, false, s);
+    return launch_jacobi_1k(op, jc, scratch, y, false, s);
   }
   if (hipError_t e = launch_apply_packed_f64(op, x, y, rdo, s)) return e;
   if (hipError_t e = launch_jacobi_fwd(op, jc, y, scratch, s)) return e;
@@ -173,9 +173,9 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
       return fail(HEOMQ_BAD_ARG, "block-Jacobi preconditioning needs the forward FP64 packed operator");
     DevJacobi jc = c->pjac;
     jc.ejac = c->prec_shift;
-    if (jc.windowed) {
-      HIPCHECK(launch_jacobi_win(c->pfwd, jc, reinterpret_cast<const double*>(b), reinterpret_cast<double*>(W.bp),
-                                 false, st));
+    if (jacobi_one_kernel(jc)) {
+      HIPCHECK(launch_jacobi_1k(c->pfwd, jc, reinterpret_cast<const double*>(b), reinterpret_cast<double*>(W.bp),
+                                false, st));
     } else {
       HIPCHECK(launch_jacobi_fwd(c->pfwd, jc, reinterpret_cast<const double*>(b), reinterpret_cast<double*>(W.pt), st));
       HIPCHECK(launch_jacobi_back(c->pfwd, jc, reinterpret_cast<const double*>(W.pt),
@@ -591,7 +591,7 @@ int heomq_apply_packed_raw(heomq_ctx* c, const double* xp, double* yp, int trace
 int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int trace, int form, void* stream) {
   if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
   if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
-  if (form < 1 || form > 2) return fail(HEOMQ_BAD_ARG, "form must be 1 or 2");
+  if (form < 1 || form > 3) return fail(HEOMQ_BAD_ARG, "form must be 1, 2 or 3");
   if (xp == yp) return fail(HEOMQ_BAD_ARG, "x and y must differ");
   hipStream_t st = (hipStream_t)stream;
   const bool rdo = trace == HEOMQ_TRACE_RDO;
@@ -603,7 +603,10 @@ int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int t
   if (int rc = carve<double>(c->ws64b, 2 * c->ptotal, 1, v)) return rc;
   HIPCHECK(launch_apply_packed_f64(op, xp, v[0], rdo, st));
   if (form == 1) {
-    if (!jc.windowed) return fail(HEOMQ_UNSUPPORTED, "the one-kernel preconditioner does not apply here");
+    if (!jacobi_one_kernel(jc)) return fail(HEOMQ_UNSUPPORTED, "the one-kernel preconditioner does not apply here");
+    HIPCHECK(launch_jacobi_1k(op, jc, v[0], yp, false, st));
+  } else if (form == 3) {
+    if (!jc.windowed) return fail(HEOMQ_UNSUPPORTED, "the LDS-window preconditioner does not apply here");
     HIPCHECK(launch_jacobi_win(op, jc, v[0], yp, false, st));
   } else {
     HIPCHECK(hipMemcpyAsync(yp, v[0], sizeof(double) * 2 * c->ptotal, hipMemcpyDeviceToDevice, st));

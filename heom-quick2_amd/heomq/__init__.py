@@ -378,7 +378,7 @@ class Context:
 
     def apply_packed_precond(self, xp, yp=None, trace=TRACE_RDO, form=1):
         """y = M^-1 (Pi A x) on packed vectors with the block-Jacobi preconditioner
-        (form 1: LDS-window kernel; 2: two-kernel transform)."""
+        (form 1: the solver's one-kernel inverse; 2: two-kernel transform; 3: LDS-window kernel)."""
         t = self.torch
         yp = self.packed_zeros() if yp is None else yp
         check(lib().heomq_apply_packed_precond(self._h, self._check_packed(xp, t.complex128),

@@ -194,9 +194,11 @@ int heomq_apply_packed_raw_f32(heomq_ctx* c, const float* xp, float* yp, int tra
 /* yp = M^-1 (Pi A xp) on packed vectors, M^-1 the block-Jacobi inverse
  * (shift - S_K)^-1 per ADO with the preconditioner shift (tier-0 output made
  * Hermitian): the operator the block-Jacobi-preconditioned TFQMR iterates
- * with.  form 1: apply, then the LDS-window kernel (the solver's form when
- * every ADO range a 256-element block touches fits its LDS; else
- * HEOMQ_UNSUPPORTED); 2: apply, then the two-kernel transform.  Both forms
+ * with.  form 1: apply, then the one-kernel inverse the solver uses (blocks
+ * of whole ADOs when d <= 16 and H couples groups of <= 4 states, else the
+ * LDS-window kernel; HEOMQ_UNSUPPORTED when neither fits the layout); 2:
+ * apply, then the two-kernel transform; 3: apply, then the LDS-window kernel
+ * (when every ADO range a 256-element block touches fits its LDS).  All forms
  * compute the same values.  No counterpart in the reference. */
 int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int trace, int form, void* stream);
 int heomq_pack(heomq_ctx* c, const double* dense, double* packed, void* stream);

@@ -124,17 +124,25 @@ def test_block_jacobi_preconditioned_tfqmr(name, shift):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("form", [1, 2, 3])
 def test_preconditioned_apply_matches_oracle(name, form):
-    """M^-1 (Pi A x) in both device forms (LDS-window kernel, two-kernel
-    transform) against the oracle's dense per-ADO solve."""
+    """M^-1 (Pi A x) in every device form (the solver's one-kernel inverse on
+    blocks of whole ADOs, the two-kernel transform, the LDS-window kernel)
+    against the oracle's dense per-ADO solve."""
     m, ctx, o, torch = setup(name)
     print(f"case {name} form {form}", flush=True)
     x = O.physical_projection(o, O.seeded_ado_vector(o, 43))
     shift = 0.05 * O.jacobi_ejac(o)[0]
     ctx.set_precond(-1, shift)
     try:
-        yp = ctx.apply_packed_precond(ctx.pack(to_dev(torch, x)), form=form)
+        try:
+            yp = ctx.apply_packed_precond(ctx.pack(to_dev(torch, x)), form=form)
+        except heomq.HeomqError as e:
+            # the window kernel needs every 256-element block to touch <= 257
+            # ADOs: the SIAM layout (ADOs of 1-4 elements) does not allow it
+            if form == 3 and name.startswith("siam") and e.code == heomq._lib.UNSUPPORTED:
+                pytest.skip("LDS-window form does not fit this layout")
+            raise
         y = from_dev(ctx.unpack(yp))
     finally:
         ctx.set_precond(-1, 0.0)
