@@ -159,6 +159,62 @@ __device__ inline void coupling_block(const DevEntryP* en, const CoefLR* cf, int
   }
 }
 
+// One coupling's contribution from its gathered elements (signs in the map
+// values and the entry flags, coefficients cL, cR).
+template <typename T>
+__device__ inline void coupling_accum(int mL, int mR, typename P2<T>::type vL, typename P2<T>::type vR, uint8_t fl,
+                                      const CoefLR& cf, T& ar, T& ai) {
+  if (mL == 0 && mR == 0) return;
+  const T cj = (fl & 1) ? (T)-1 : (T)1;
+  const T sL = ((mL > 0) != ((fl & 2) != 0)) ? (T)1 : (T)-1;
+  const T sR = ((mR > 0) != ((fl & 4) != 0)) ? (T)1 : (T)-1;
+  const T lx = vL.x, ly = cj * vL.y, rx = vR.x, ry = cj * vR.y;
+  const T clr = (T)cf.l.x, cli = (T)cf.l.y, crr = (T)cf.r.x, cri = (T)cf.r.y;
+  ar += sL * (clr * lx - cli * ly) + sR * (crr * rx - cri * ry);
+  ai += sL * (clr * ly + cli * lx) + sR * (crr * ry + cri * rx);
+}
+
+// The couplings k0, k0 + S, k0 + 2S, ... of a heavy lane with the entries and
+// coefficients staged in LDS, U per batch, software-pipelined: the gather-map
+// values of batch b + 1 are loaded while batch b's elements are in flight, so
+// each batch waits for one global load instead of two (map, then element).
+template <typename T, bool TRANS, int U>
+__device__ inline void staged_couplings(const DevEntryP* s_en, const CoefLR* s_cf, int nent, int k0, int S,
+                                        const int32_t* __restrict__ maps, const typename P2<T>::type* __restrict__ xv,
+                                        int t, int64_t zero_at, T& ar, T& ai) {
+  using V = typename P2<T>::type;
+  int mv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int k = k0 + u * S;
+    mv[u] = (k < nent && s_en[k].map >= 0) ? maps[s_en[k].map + t] : 0;
+  }
+  for (; k0 < nent; k0 += S * U) {
+    V vL[U], vR[U];
+    int mL[U], mR[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * S;
+      const int64_t b = k < nent ? (int64_t)s_en[k].base : 0;
+      mL[u] = (int)(int16_t)(mv[u] & 0xffff);
+      mR[u] = (int)(int16_t)((uint32_t)mv[u] >> 16);
+      vL[u] = mL[u] != 0 ? ld<T, TRANS>(xv, b + (mL[u] > 0 ? mL[u] : -mL[u]) - 1, zero_at) : V{0, 0};
+      vR[u] = mR[u] != 0 ? ld<T, TRANS>(xv, b + (mR[u] > 0 ? mR[u] : -mR[u]) - 1, zero_at) : V{0, 0};
+    }
+    const int kn = k0 + S * U;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kn + u * S;
+      mv[u] = (k < nent && s_en[k].map >= 0) ? maps[s_en[k].map + t] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u * S;
+      if (k < nent) coupling_accum<T>(mL[u], mR[u], vL[u], vR[u], s_en[k].flags, s_cf[k], ar, ai);
+    }
+  }
+}
+
 // Top-tier ADOs: one thread per packed element.  A block stages in LDS the
 // headers (offset, first and last entry, sector, gamma) of the non-empty ADOs
 // it covers (at most 257, checked when the tables are built) and its own 256
@@ -282,23 +338,27 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
     if (slice < S) {
       if (slice == 0)
         diag_term<T, TRANS>(op, cf_p.hc_ell, xv, gr, gi, base, s, t, zero_at, ar, ai, s_x, base, win_n, csr, csi);
-      for (int k0 = slice; k0 < nent; k0 += S * U) {
-        DevEntryP en[U];
-        CoefLR cf[U];
-        int nv = 0;
+      if (staged) {
+        staged_couplings<T, TRANS, U>(s_en, s_cf, nent, slice, S, op.maps, xv, t, zero_at, ar, ai);
+      } else {   // entries from global memory (ADOs with more than 256 couplings)
+        for (int k0 = slice; k0 < nent; k0 += S * U) {
+          DevEntryP en[U];
+          CoefLR cf[U];
+          int nv = 0;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int k = k0 + u * S;
-          if (k < nent) {
-            en[u] = staged ? s_en[k] : op.ent[p0 + k];
-            cf[u] = staged ? s_cf[k] : load_coef(cf_p.rec, en[u].rec);
-            nv = u + 1;
-          } else {
-            en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
-            cf[u] = CoefLR{};
+          for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * S;
+            if (k < nent) {
+              en[u] = op.ent[p0 + k];
+              cf[u] = load_coef(cf_p.rec, en[u].rec);
+              nv = u + 1;
+            } else {
+              en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
+              cf[u] = CoefLR{};
+            }
           }
+          coupling_block<T, TRANS, U>(en, cf, nv, op.maps, xv, t, zero_at, ar, ai);
         }
-        coupling_block<T, TRANS, U>(en, cf, nv, op.maps, xv, t, zero_at, ar, ai);
       }
     }
     red_r[threadIdx.x] = ar;
@@ -330,9 +390,11 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
 
 // One launch for the whole apply: blocks [0, n_heavy) take the ADOs below the
 // top tier (dispatched first: their work per block is the longest), the rest
-// the top tier's elements, so the two phases overlap on the device.
+// the top tier's elements, so the two phases overlap on the device.  The LDS
+// buffer allows 7 blocks per CU; the register budget is held to the same 7
+// waves per SIMD (72 VGPRs).
 template <typename T, bool TRANS, int U>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
 k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
   __shared__ __align__(16) char smem[kApplySmem];
   const int64_t p = blockIdx.y;   // problem of a batch
