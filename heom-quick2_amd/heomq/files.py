@@ -1,0 +1,294 @@
+"""The reference's state and table files, read and written on the host.
+
+A state saved by the reference can seed a solve here, and a state solved here
+can be handed back to the reference (its ``initialize2`` / ``initialize2_spa``
+read it when a job restarts from a saved state).
+
+Formats (main/main.f90:1085-1120 and :1655-1700 write them;
+main/initialize2.f90:43-82 and fermion/initialize2_spa.f90:24-104 read them):
+
+``variables.gr`` / ``variables.st`` / ``variables.sav`` / ``variables_td.sav``
+    Fortran unformatted sequential records, each framed by its byte length as
+    a 4-byte integer before and after: (norbs, nspin, ncor, ntier, nalf) as
+    five int32; nunk as one int64; then one record per ADO holding
+    rho(1:nrho, 1:nrho, lni), complex*16 in column-major order.
+
+``rho_spa.gr`` / ``rho_spa.st`` / ``rho_spa.sav`` / ``rho_spa_td.sav``
+    ``form='binary'``: the same items with no record framing; for each ADO
+    with nnz_spa(lni) > 0, its nnz_spa(lni) complex*16 values in the order of
+    the sparse pattern (module sparsemod: nnz_spa, ind_spa, irow_spa,
+    icol_spa, 1-based).
+
+``indextable.sav``
+    ``form='binary'``: ntable as int64, then indextable(1:ntable) as int32.
+
+ADO order is the reference's (tier-major, drawer tuples sorted), the order
+of the dense vectors of this package: x[a] is ADO a, its memory the
+column-major d x d block.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+from typing import NamedTuple
+
+import numpy as np
+
+_I4 = np.dtype("<i4")
+_I8 = np.dtype("<i8")
+_C16 = np.dtype("<c16")
+
+
+class StateHeader(NamedTuple):
+    norbs: int
+    nspin: int
+    ncor: int
+    ntier: int     # number of tiers held, tier 0 included (the reference's ntier)
+    nalf: int
+    nunk: int      # ADOs held
+
+
+class FileFormatError(ValueError):
+    pass
+
+
+def header_of(model, nunk: int | None = None) -> StateHeader:
+    """The header the reference writes for a hierarchy like `model`'s."""
+    p = model.params
+    return StateHeader(int(p.norbs), 2, int(model.ncor), int(model.L) + 1, int(p.nalf),
+                       int(model.N if nunk is None else nunk))
+
+
+def indextable_of(model) -> np.ndarray:
+    """The reference's indextable for `model`'s hierarchy: tier 0's empty
+    record (a single 0, main/ball2drawer.f90:40-44), then every ADO's sorted
+    drawer tuple, 1-based, tier by tier."""
+    parts = [np.zeros(1, np.int32)]
+    for n in range(1, model.L + 1):
+        parts.append((model.tier(n).astype(np.int32) + 1).ravel())
+    return np.concatenate(parts)
+
+
+def _tier_lengths(model) -> np.ndarray:
+    return np.diff(np.asarray(model.first[: model.L + 2], np.int64))
+
+
+# ---------------------------------------------------------------------------
+# unformatted sequential records
+# ---------------------------------------------------------------------------
+def _record(payload: bytes) -> bytes:
+    n = len(payload)
+    if n >= 2 ** 31:
+        raise FileFormatError("record longer than 2 GiB: the reference's 4-byte framing cannot hold it")
+    m = np.array([n], _I4).tobytes()
+    return m + payload + m
+
+
+class _Records:
+    def __init__(self, data: bytes, path):
+        self.data, self.pos, self.path = data, 0, path
+
+    def next(self) -> memoryview:
+        d, p = self.data, self.pos
+        if p + 4 > len(d):
+            raise FileFormatError(f"{self.path}: file ends before a record")
+        n = int(np.frombuffer(d, _I4, 1, p)[0])
+        if n < 0 or p + 8 + n > len(d):
+            raise FileFormatError(f"{self.path}: record of {n} bytes at offset {p} runs past the end")
+        tail = int(np.frombuffer(d, _I4, 1, p + 4 + n)[0])
+        if tail != n:
+            raise FileFormatError(f"{self.path}: record framing {n} / {tail} at offset {p} do not match")
+        self.pos = p + 8 + n
+        return memoryview(d)[p + 4: p + 4 + n]
+
+
+def _check_against(h: StateHeader, model, name: str) -> tuple[int, bool]:
+    """The checks of main/initialize2.f90:47-67: same norbs, nspin, ncor, nalf,
+    more than one tier; returns (ADOs to read, lsmall)."""
+    m = header_of(model)
+    if (h.norbs, h.nspin, h.ncor, h.nalf) != (m.norbs, m.nspin, m.ncor, m.nalf) or h.ntier <= 1:
+        raise FileFormatError(f"error input file <{name}>: header {tuple(h[:5])}, model {tuple(m[:5])}")
+    lengths = _tier_lengths(model)
+    lnj = int(lengths[: min(h.ntier, m.ntier)].sum())
+    small = h.ntier < m.ntier
+    if small and lnj != h.nunk:
+        raise FileFormatError(f"incompatible <{name}>: {h.ntier} tiers should hold {lnj} ADOs, it holds {h.nunk}")
+    if not small and h.nunk < m.nunk:
+        raise FileFormatError(f"incompatible <{name}>: {h.nunk} ADOs, the model has {m.nunk}")
+    return (h.nunk if small else m.nunk), small
+
+
+def write_variables(path, x, header: StateHeader) -> None:
+    """Write a dense state as ``variables.*`` (main/main.f90:1112-1118).
+    x: (nunk, d, d) complex128 with x[a] in column-major memory order (as the
+    vectors of this package), a numpy array or a tensor on any device."""
+    x = _host_c16(x)
+    if x.ndim != 3 or x.shape[1] != x.shape[2] or x.shape[0] != header.nunk:
+        raise FileFormatError(f"x must be (nunk={header.nunk}, d, d), got {x.shape}")
+    with open(path, "wb") as f:
+        f.write(_record(np.array(header[:5], _I4).tobytes()))
+        f.write(_record(np.array([header.nunk], _I8).tobytes()))
+        for a in range(header.nunk):
+            f.write(_record(x[a].tobytes()))
+
+
+def read_variables(path, model=None, d: int | None = None):
+    """Read ``variables.*``; returns (header, x) with x (n, d, d) complex128.
+    With `model` the reference's restart rules apply (main/initialize2.f90:
+    47-82): the header must match, a file with fewer tiers fills the first
+    ADOs and leaves the rest zero, a file with more tiers is cut to the
+    model's ADOs; x then has the model's N ADOs."""
+    path = Path(path)
+    r = _Records(path.read_bytes(), path)
+    hdr = np.frombuffer(r.next(), _I4)
+    if hdr.size != 5:
+        raise FileFormatError(f"{path}: first record holds {hdr.size * 4} bytes, expected 20")
+    nu = np.frombuffer(r.next(), _I8)
+    if nu.size != 1:
+        raise FileFormatError(f"{path}: second record must hold nunk as one 8-byte integer")
+    h = StateHeader(*(int(v) for v in hdr), int(nu[0]))
+    if model is not None:
+        d = model.d
+        nread, small = _check_against(h, model, path.name)
+        nout = model.N
+    else:
+        nread = nout = h.nunk
+    x = np.zeros((nout, d or 0, d or 0), np.complex128) if d else None
+    for a in range(nread):
+        rec = np.frombuffer(r.next(), _C16)
+        if x is None:
+            dd = int(round(np.sqrt(rec.size)))
+            if dd * dd != rec.size:
+                raise FileFormatError(f"{path}: ADO record of {rec.size} values is not a square block")
+            d = dd
+            x = np.zeros((nout, d, d), np.complex128)
+        if rec.size != d * d:
+            raise FileFormatError(f"{path}: ADO {a + 1} holds {rec.size} values, expected {d * d}")
+        x[a] = rec.reshape(d, d)
+    if x is None:
+        x = np.zeros((nout, d or 0, d or 0), np.complex128)
+    return h, x
+
+
+# ---------------------------------------------------------------------------
+# form='binary' streams
+# ---------------------------------------------------------------------------
+def write_rho_spa(path, x_spa, nnz_spa, ind_spa, header: StateHeader) -> None:
+    """Write a sparse state as ``rho_spa.*`` (main/main.f90:1093-1104): the
+    header, nunk, then each ADO's nnz_spa values (ind_spa: 1-based starts)."""
+    xs = _host_c16(x_spa).ravel()
+    nnz = np.asarray(nnz_spa, np.int64)
+    ind = np.asarray(ind_spa, np.int64)
+    if nnz.size != header.nunk or ind.size != header.nunk:
+        raise FileFormatError("nnz_spa and ind_spa need nunk entries")
+    with open(path, "wb") as f:
+        f.write(np.array(header[:5], _I4).tobytes())
+        f.write(np.array([header.nunk], _I8).tobytes())
+        for a in range(header.nunk):
+            if nnz[a] > 0:
+                s = int(ind[a]) - 1
+                if s < 0 or s + nnz[a] > xs.size:
+                    raise FileFormatError(f"ADO {a + 1}: ind_spa / nnz_spa run outside x_spa")
+                f.write(xs[s: s + int(nnz[a])].tobytes())
+
+
+def read_rho_spa(path, nnz_spa, ind_spa, lunk_spa: int, model=None):
+    """Read ``rho_spa.*`` with the pattern it was written with
+    (fermion/initialize2_spa.f90:80-104); returns (header, x_spa) with x_spa
+    (lunk_spa,) complex128.  With `model`: the header checks and the
+    fewer-tiers rule of the reference (later ADOs left zero)."""
+    path = Path(path)
+    data = path.read_bytes()
+    if len(data) < 28:
+        raise FileFormatError(f"{path}: shorter than its header")
+    h = StateHeader(*(int(v) for v in np.frombuffer(data, _I4, 5, 0)), int(np.frombuffer(data, _I8, 1, 20)[0]))
+    nnz = np.asarray(nnz_spa, np.int64)
+    ind = np.asarray(ind_spa, np.int64)
+    nread = h.nunk
+    if model is not None:
+        nread, _ = _check_against(h, model, path.name)
+    if nnz.size < nread or ind.size < nread:
+        raise FileFormatError(f"{path}: the pattern covers {nnz.size} ADOs, the file needs {nread}")
+    x = np.zeros(int(lunk_spa), np.complex128)
+    pos = 28
+    for a in range(nread):
+        k = int(nnz[a])
+        if k <= 0:
+            continue
+        if pos + 16 * k > len(data):
+            raise FileFormatError(f"{path}: ends inside ADO {a + 1}")
+        s = int(ind[a]) - 1
+        if s < 0 or s + k > x.size:
+            raise FileFormatError(f"ADO {a + 1}: ind_spa / nnz_spa run outside lunk_spa")
+        x[s: s + k] = np.frombuffer(data, _C16, k, pos)
+        pos += 16 * k
+    return h, x
+
+
+def remap_rho_spa(x_spa0, pattern0, pattern1, lunk_spa1: int, d: int) -> np.ndarray:
+    """A sparse state moved from the pattern it was saved with to another one
+    (fermion/initialize2_spa.f90:42-74, iread_spa = 1): each ADO expanded to
+    its d x d block, then the new pattern's elements taken from it.
+    pattern = (nnz_spa, ind_spa, irow_spa, icol_spa), 1-based."""
+    nnz0, ind0, ir0, ic0 = (np.asarray(v, np.int64) for v in pattern0)
+    nnz1, ind1, ir1, ic1 = (np.asarray(v, np.int64) for v in pattern1)
+    xs0 = _host_c16(x_spa0).ravel()
+    out = np.zeros(int(lunk_spa1), np.complex128)
+    for a in range(min(nnz0.size, nnz1.size)):
+        blk = np.zeros((d, d), np.complex128)
+        if nnz0[a] > 0:
+            s0 = int(ind0[a]) - 1
+            e = slice(s0, s0 + int(nnz0[a]))
+            blk[ir0[e] - 1, ic0[e] - 1] = xs0[e]
+        if nnz1[a] > 0:
+            s1 = int(ind1[a]) - 1
+            e = slice(s1, s1 + int(nnz1[a]))
+            out[e] = blk[ir1[e] - 1, ic1[e] - 1]
+    return out
+
+
+def dense_to_spa(x, pattern) -> np.ndarray:
+    """The pattern's elements of a dense state (x[a] column-major)."""
+    nnz, ind, ir, ic = (np.asarray(v, np.int64) for v in pattern)
+    xd = _host_c16(x)
+    out = np.zeros(ir.size, np.complex128)
+    for a in range(nnz.size):
+        if nnz[a] > 0:
+            e = slice(int(ind[a]) - 1, int(ind[a]) - 1 + int(nnz[a]))
+            out[e] = xd[a].reshape(-1)[(ir[e] - 1) + (ic[e] - 1) * xd.shape[1]]
+    return out
+
+
+def spa_to_dense(x_spa, pattern, n_ado: int, d: int) -> np.ndarray:
+    """A dense state (x[a] column-major, zero off the pattern) from a sparse one."""
+    nnz, ind, ir, ic = (np.asarray(v, np.int64) for v in pattern)
+    xs = _host_c16(x_spa).ravel()
+    out = np.zeros((n_ado, d * d), np.complex128)
+    for a in range(min(n_ado, nnz.size)):
+        if nnz[a] > 0:
+            e = slice(int(ind[a]) - 1, int(ind[a]) - 1 + int(nnz[a]))
+            out[a, (ir[e] - 1) + (ic[e] - 1) * d] = xs[e]
+    return out.reshape(n_ado, d, d)
+
+
+def write_indextable(path, indextable) -> None:
+    """``indextable.sav`` (main/main.f90:1657-1661)."""
+    t = np.ascontiguousarray(indextable, _I4)
+    with open(path, "wb") as f:
+        f.write(np.array([t.size], _I8).tobytes())
+        f.write(t.tobytes())
+
+
+def read_indextable(path) -> np.ndarray:
+    data = Path(path).read_bytes()
+    if len(data) < 8:
+        raise FileFormatError(f"{path}: shorter than its length field")
+    n = int(np.frombuffer(data, _I8, 1, 0)[0])
+    if n < 0 or len(data) != 8 + 4 * n:
+        raise FileFormatError(f"{path}: {len(data)} bytes do not hold {n} int32 entries")
+    return np.frombuffer(data, _I4, n, 8).copy()
+
+
+def _host_c16(x) -> np.ndarray:
+    if hasattr(x, "detach"):   # a torch tensor, on any device
+        x = x.detach().to("cpu").numpy()
+    return np.ascontiguousarray(x, np.complex128)

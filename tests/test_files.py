@@ -1,0 +1,219 @@
+"""The reference's state and table files (heomq.files): byte layout, the
+restart rules of main/initialize2.f90:43-82 and fermion/initialize2_spa.f90,
+and the layout a Fortran compiler itself gives these records (amdflang, when
+the toolchain has it)."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import heomq
+from heomq import files as F
+from oracle import heom_oracle as O
+from oracle.ref_tables import reference_arrays
+
+
+def _state(n, d, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=(n, d, d)) + 1j * rng.normal(size=(n, d, d))
+
+
+def test_variables_round_trip_and_framing(tmp_path):
+    x = _state(5, 4, 1)
+    h = F.StateHeader(1, 2, 3, 3, 2, 5)
+    p = tmp_path / "variables.st"
+    F.write_variables(p, x, h)
+    raw = p.read_bytes()
+    i4 = np.frombuffer(raw, np.int32)
+    # record 1: 20 bytes of five int32, framed; record 2: 8 bytes of int64
+    assert list(i4[:7]) == [20, 1, 2, 3, 3, 2, 20]
+    assert i4[7] == 8 and np.frombuffer(raw[32:40], np.int64)[0] == 5 and i4[10] == 8
+    assert i4[11] == 16 * 16   # first ADO record: 4x4 complex*16
+    assert len(raw) == 28 + 16 + 5 * (8 + 256)
+    # ADO records hold rho(:,:,a) column-major: the memory of x[a]
+    assert np.array_equal(np.frombuffer(raw[48:48 + 256], np.complex128), x[0].ravel())
+    h2, x2 = F.read_variables(p)
+    assert h2 == h and np.array_equal(x2, x)
+
+
+def test_variables_rejects_broken_framing(tmp_path):
+    p = tmp_path / "variables.sav"
+    F.write_variables(p, _state(2, 2, 2), F.StateHeader(1, 2, 3, 2, 2, 2))
+    raw = bytearray(p.read_bytes())
+    raw[-1] ^= 1   # last trailer
+    p.write_bytes(bytes(raw))
+    with pytest.raises(F.FileFormatError):
+        F.read_variables(p)
+    p.write_bytes(bytes(raw[:-20]))
+    with pytest.raises(F.FileFormatError):
+        F.read_variables(p)
+
+
+def test_restart_rules_follow_initialize2(tmp_path):
+    """Fewer tiers: the first ADOs filled, the rest zero; more tiers: cut to
+    the model; a different bath or leads: refused (initialize2.f90:47-82)."""
+    small = heomq.Model(heomq.siam(nk=2, ntop=1))
+    big = heomq.Model(heomq.siam(nk=2, ntop=2))
+    xs = _state(small.N, small.d, 3)
+    p = tmp_path / "variables.sav"
+    F.write_variables(p, xs, F.header_of(small))
+    h, x = F.read_variables(p, model=big)
+    assert h.ntier == 2 and x.shape == (big.N, big.d, big.d)
+    assert np.array_equal(x[: small.N], xs) and not np.any(x[small.N:])
+    # the tiers the small hierarchy holds are the big one's first ADOs
+    assert np.array_equal(F.indextable_of(small), F.indextable_of(big)[: F.indextable_of(small).size])
+    xb = _state(big.N, big.d, 4)
+    F.write_variables(p, xb, F.header_of(big))
+    h, x = F.read_variables(p, model=small)
+    assert np.array_equal(x, xb[: small.N])
+    other = heomq.Model(heomq.siam(nk=3, ntop=2))   # one more Pade term: ncor differs
+    with pytest.raises(F.FileFormatError):
+        F.read_variables(p, model=other)
+    F.write_variables(p, xs[:3], F.StateHeader(*F.header_of(small)[:5], 3))   # ADO count of no tier
+    with pytest.raises(F.FileFormatError):
+        F.read_variables(p, model=big)
+
+
+def _pattern(n_ado, d, seed, density=0.5):
+    rng = np.random.default_rng(seed)
+    nnz, ind, ir, ic = [], [], [], []
+    pos = 1
+    for a in range(n_ado):
+        mask = rng.random((d, d)) < density
+        if a == 0:
+            mask[:] = True
+        j, i = np.nonzero(mask.T)   # (column, row) pairs in column-major element order
+        nnz.append(len(i))
+        ind.append(pos)
+        ir.extend((i + 1).tolist())
+        ic.extend((j + 1).tolist())
+        pos += len(i)
+    return (np.array(nnz, np.int32), np.array(ind, np.int64), np.array(ir, np.int32), np.array(ic, np.int32)), pos - 1
+
+
+def test_rho_spa_round_trip_remap_and_dense(tmp_path):
+    n, d = 7, 4
+    pat0, lunk0 = _pattern(n, d, 5)
+    pat1, lunk1 = _pattern(n, d, 6, density=0.7)
+    x = _state(n, d, 7)
+    xs0 = F.dense_to_spa(x, pat0)
+    assert xs0.size == lunk0
+    # dense -> sparse -> dense keeps exactly the pattern's elements
+    back = F.spa_to_dense(xs0, pat0, n, d)
+    for a in range(n):
+        s = slice(pat0[1][a] - 1, pat0[1][a] - 1 + pat0[0][a])
+        m = np.zeros(d * d, bool)
+        m[(pat0[2][s] - 1) + (pat0[3][s] - 1) * d] = True
+        assert np.array_equal(back[a].ravel()[m], x[a].ravel()[m]) and not np.any(back[a].ravel()[~m])
+    h = F.StateHeader(1, 2, 3, 3, 2, n)
+    p = tmp_path / "rho_spa.st"
+    F.write_rho_spa(p, xs0, pat0[0], pat0[1], h)
+    raw = p.read_bytes()
+    assert len(raw) == 20 + 8 + 16 * lunk0   # no record framing
+    assert list(np.frombuffer(raw[:20], np.int32)) == [1, 2, 3, 3, 2]
+    h2, xr = F.read_rho_spa(p, pat0[0], pat0[1], lunk0)
+    assert h2 == h and np.array_equal(xr, xs0)
+    # a state saved with one pattern, read into another (iread_spa = 1)
+    xs1 = F.remap_rho_spa(xr, pat0, pat1, lunk1, d)
+    assert np.array_equal(xs1, F.dense_to_spa(F.spa_to_dense(xs0, pat0, n, d), pat1))
+
+
+def test_indextable_matches_reference_tables(tmp_path):
+    m = heomq.Model(heomq.siam(nk=2, ntop=3))
+    o = O.Model(O.siam_params(nk=2, ntop=3))
+    t = F.indextable_of(m)
+    assert np.array_equal(t, reference_arrays(o)["indextable"])
+    p = tmp_path / "indextable.sav"
+    F.write_indextable(p, t)
+    raw = p.read_bytes()
+    assert np.frombuffer(raw[:8], np.int64)[0] == t.size and len(raw) == 8 + 4 * t.size
+    assert np.array_equal(F.read_indextable(p), t)
+
+
+FORTRAN = """
+program files_check
+implicit none
+integer :: norbs, nspin, ncor, ntier, nalf, ni, nj
+integer*8 :: nunk, lni
+complex*16 :: rho(3,3,4), acc
+! the reference's writes (main/main.f90), with known values
+norbs = 1; nspin = 2; ncor = 5; ntier = 3; nalf = 2; nunk = 4
+do lni = 1, nunk
+  do nj = 1, 3
+    do ni = 1, 3
+      rho(ni,nj,lni) = dcmplx(dble(ni + 10*nj), dble(lni) - 0.5d0)
+    end do
+  end do
+end do
+open(unit=25, file='variables.st', form='unformatted', status='unknown')
+rewind(25)
+write(25)norbs, nspin, ncor, ntier, nalf
+write(25)nunk
+do lni=1,nunk
+   write(25)((rho(ni,nj,lni), ni=1,3), nj=1,3)
+end do
+close(25)
+open(unit=51, file='rho_spa.st', form='binary', status='unknown')
+rewind(51)
+write(51)norbs, nspin, ncor, ntier, nalf
+write(51)nunk
+write(51)(rho(ni,1,1), ni=1,3)
+write(51)(rho(ni,2,3), ni=2,3)
+close(51)
+! the reference's read (main/initialize2.f90) of a file written by Python
+open(unit=15, file='variables.sav', form='unformatted', status='old')
+rewind(15)
+read(15)norbs, nspin, ncor, ntier, nalf
+read(15)nunk
+acc = (0.d0, 0.d0)
+do lni=1,nunk
+   read(15)((rho(ni,nj,lni), ni=1,3), nj=1,3)
+   do nj = 1, 3
+     do ni = 1, 3
+       acc = acc + rho(ni,nj,lni) * dble(ni + 3*(nj-1) + 9*(lni-1))
+     end do
+   end do
+end do
+close(15)
+write(6,'(5i6,i6)') norbs, nspin, ncor, ntier, nalf, nunk
+write(6,'(2es25.16)') dble(acc), dimag(acc)
+end program
+"""
+
+
+def _flang():
+    for c in ("/opt/rocm/bin/amdflang", shutil.which("amdflang"), shutil.which("flang"), shutil.which("gfortran")):
+        if c and Path(c).exists():
+            return c
+    return None
+
+
+@pytest.mark.skipif(_flang() is None, reason="no Fortran compiler in this toolchain")
+def test_layout_matches_a_fortran_compiler(tmp_path):
+    """Files written by Fortran's own unformatted / binary I/O read back here,
+    and a file written here reads back in Fortran."""
+    xs = _state(2, 3, 9)
+    F.write_variables(tmp_path / "variables.sav", xs, F.StateHeader(1, 2, 5, 2, 2, 2))
+    (tmp_path / "files_check.f90").write_text(FORTRAN)
+    subprocess.run([_flang(), "-O0", "files_check.f90", "-o", "files_check"], cwd=tmp_path, check=True,
+                   capture_output=True)
+    out = subprocess.run(["./files_check"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout.split()
+    assert [int(v) for v in out[:6]] == [1, 2, 5, 2, 2, 2]
+    # weight ni + 3 (nj - 1) + 9 (lni - 1): one plus the element's column-major offset in the file
+    acc = np.sum(xs.reshape(2, 9) * (np.arange(18).reshape(2, 9) + 1))
+    assert abs(float(out[6]) - acc.real) <= 1e-12 * np.abs(xs).sum() * 18
+    assert abs(float(out[7]) - acc.imag) <= 1e-12 * np.abs(xs).sum() * 18
+    rho = np.zeros((4, 3, 3), np.complex128)   # rho[lni-1] as x[a]: memory column-major
+    for lni in range(1, 5):
+        for nj in range(1, 4):
+            for ni in range(1, 4):
+                rho[lni - 1].reshape(-1)[(ni - 1) + 3 * (nj - 1)] = complex(ni + 10 * nj, lni - 0.5)
+    h, x = F.read_variables(tmp_path / "variables.st")
+    assert h == F.StateHeader(1, 2, 5, 3, 2, 4) and np.array_equal(x, rho)
+    nnz, ind = np.array([3, 0, 2, 0]), np.array([1, 4, 4, 6])
+    h, xsp = F.read_rho_spa(tmp_path / "rho_spa.st", nnz, ind, 5)
+    assert h.nunk == 4
+    assert np.array_equal(xsp[:3], rho[0].reshape(-1)[0:3])
+    assert np.array_equal(xsp[3:5], rho[2].reshape(-1)[4:6])
