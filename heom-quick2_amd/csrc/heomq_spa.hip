@@ -11,7 +11,10 @@
 //
 // Here the pattern becomes a map from COO element to dense index, and the
 // apply is scatter -> the dense device apply -> gather, plus the trace: the
-// arithmetic is the dense operator's, so P A P^T holds for any pattern.
+// arithmetic is the dense operator's, so P A P^T holds for any pattern.  The
+// derivative of the sparse propagators (td/calcderiv_spa_omp.f90: rhotmp_spa
+// -> rhorhs_spa, H and the lead shift at time tt, :84-98) is the same
+// composition around the derivative at tt, with no trace element.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -128,6 +131,44 @@ int heomq_apply_spa(heomq_ctx* c, const double* x_spa, double* y_spa, void* stre
   hipLaunchKernelGGL(k_spa_trace, dim3(1), dim3(256), 0, st, c->spa_ndiag, c->spa_diag.as<int64_t>(),
                      reinterpret_cast<const double2*>(x_spa), reinterpret_cast<double2*>(y_spa) + n);
   HIPCHECK(hipGetLastError());
+  return HEOMQ_OK;
+}
+
+int heomq_calcderiv_spa_at(heomq_ctx* c, double tt, const double* x_spa, double* y_spa, void* stream) {
+  if (!c || !x_spa || !y_spa) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c->spa_lunk < 0) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = c->spa_lunk, nd = 2 * c->N * c->d * c->d;
+  HIPCHECK(c->spa_dense.alloc(2 * sizeof(double) * (size_t)nd));
+  double* xd = c->spa_dense.as<double>();
+  double* yd = xd + nd;
+  hipLaunchKernelGGL(k_spa_zero, dim3(grid_of(nd)), dim3(256), 0, st, nd, xd);
+  HIPCHECK(hipGetLastError());
+  if (n > 0) {
+    hipLaunchKernelGGL(k_spa_scatter, dim3(grid_of(n)), dim3(256), 0, st, n, c->spa_map.as<int64_t>(),
+                       reinterpret_cast<const double2*>(x_spa), reinterpret_cast<double2*>(xd));
+    HIPCHECK(hipGetLastError());
+  }
+  if (int rc = heomq_calcderiv_at(c, tt, xd, yd, stream)) return rc;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_spa_gather, dim3(grid_of(n)), dim3(256), 0, st, n, c->spa_map.as<int64_t>(),
+                       reinterpret_cast<const double2*>(yd), reinterpret_cast<double2*>(y_spa));
+    HIPCHECK(hipGetLastError());
+  }
+  return HEOMQ_OK;
+}
+
+int heomq_calcderiv_spa_tt(heomq_ctx* c, double tt, const double* x_host, double* y_host) {
+  if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c->spa_lunk < 0) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
+  const int64_t n = c->spa_lunk;
+  if (n == 0) return HEOMQ_OK;
+  HIPCHECK(c->spa_io.alloc(sizeof(double) * 2 * (size_t)(2 * n + 1)));
+  double* xd = c->spa_io.as<double>();
+  double* yd = xd + 2 * n;
+  HIPCHECK(hipMemcpy(xd, x_host, sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+  if (int rc = heomq_calcderiv_spa_at(c, tt, xd, yd, nullptr)) return rc;
+  HIPCHECK(hipMemcpy(y_host, yd, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
   return HEOMQ_OK;
 }
 

@@ -15,6 +15,9 @@
 !   solve_tfqmr_gpu(rho, crit, maxit, info)   solver-level replacement of solve_tfqmr
 !   solve_bicg_gpu(rho, crit, maxit, info)    solver-level replacement of solvebicg
 !   calc_ax_jacobi_omp(dim0, rcgtmp, rcgrhs)  jacobi/calc_ax_jacobi_omp.f90:1,15-17
+!   calcderiv_spa_omp(tt)                     td/calcderiv_spa_omp.f90:1-11 (module
+!                                             arrays rhotmp_spa -> rhorhs_spa, registered
+!                                             once by heomq_register_td_spa_arrays)
 !   calc_ax_tfqmr_spa_omp(ldim, rcgtmp, rcgrhs)
 !                                             tfqmr_double/calc_ax_tfqmr_spa_omp.f90:1,16-18
 !                                             (pattern of module sparsemod registered
@@ -33,6 +36,7 @@ module heomq_iface
   type(c_ptr), save :: heomq_model_h = c_null_ptr
   type(c_ptr), save :: heomq_ctx_h = c_null_ptr
   type(c_ptr), save :: td_rhotmp = c_null_ptr, td_rhorhs = c_null_ptr
+  type(c_ptr), save :: td_rhotmp_spa = c_null_ptr, td_rhorhs_spa = c_null_ptr
 
   interface
     function c_last_error() bind(C, name='heomq_last_error') result(p)
@@ -124,6 +128,12 @@ module heomq_iface
     end function
 
     integer(c_int) function c_calcderiv_tt(ctx, tt, x, y) bind(C, name='heomq_calcderiv_tt')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, x, y
+      real(c_double), value :: tt
+    end function
+
+    integer(c_int) function c_calcderiv_spa_tt(ctx, tt, x, y) bind(C, name='heomq_calcderiv_spa_tt')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, x, y
       real(c_double), value :: tt
@@ -366,6 +376,32 @@ subroutine heomq_register_spa_pattern(lunk_spa, nnz_spa, ind_spa, irow_spa, icol
   rc = c_spa_pattern(heomq_ctx_h, int(lunk_spa, c_int64_t), nnz_spa, ind_spa, irow_spa, icol_spa)
   if (rc /= 0) call heomq_fail('heomq_register_spa_pattern', rc)
 end subroutine heomq_register_spa_pattern
+
+! td/calcderiv_spa_omp.f90 reads module array rhotmp_spa and writes
+! rhorhs_spa (module sparsemod); the caller registers their addresses once.
+subroutine heomq_register_td_spa_arrays(rhotmp_spa, rhorhs_spa)
+  use heomq_iface
+  implicit none
+  complex*16, intent(in), target :: rhotmp_spa(*)
+  complex*16, intent(in), target :: rhorhs_spa(*)
+  td_rhotmp_spa = c_loc(rhotmp_spa)
+  td_rhorhs_spa = c_loc(rhorhs_spa)
+end subroutine heomq_register_td_spa_arrays
+
+! td/calcderiv_spa_omp.f90:1-11 -- rhorhs_spa = P L(tt) P^T rhotmp_spa over
+! the registered sparse pattern (H fixed, the lead shift at tt)
+subroutine calcderiv_spa_omp(tt)
+  use heomq_iface
+  implicit none
+  real*8, intent(in) :: tt
+  integer(c_int) :: rc
+  if (.not. c_associated(td_rhotmp_spa)) then
+    write(6,*) ' error! calcderiv_spa_omp: rhotmp_spa/rhorhs_spa not registered '
+    error stop 1
+  end if
+  rc = c_calcderiv_spa_tt(heomq_ctx_h, real(tt, c_double), td_rhotmp_spa, td_rhorhs_spa)
+  if (rc /= 0) call heomq_fail('calcderiv_spa_omp', rc)
+end subroutine calcderiv_spa_omp
 
 ! tfqmr_double/calc_ax_tfqmr_spa_omp.f90:1,16-18 -- the apply on COO storage:
 ! rcgrhs(1:lunk_spa) = P A P^T rcgtmp, rcgrhs(lunk_spa + 1) = tr rho0;

@@ -481,6 +481,20 @@ class Context:
                                     self._stream()))
         return y_spa
 
+    def calcderiv_spa(self, x_spa, tt=0.0, y_spa=None):
+        """calcderiv_spa_omp(tt) on device vectors: y = P L(tt) P^T x over the
+        sparse pattern (lunk_spa each), the lead field (set_field) at tt."""
+        t = self.torch
+        n = getattr(self, "_spa_lunk", -1)
+        if y_spa is None:
+            y_spa = t.zeros(max(n, 0), dtype=t.complex128, device=f"cuda:{self.device}")
+        for v in (x_spa, y_spa):
+            if v.dtype != t.complex128 or not v.is_cuda or not v.is_contiguous() or v.numel() != n:
+                raise HeomqError(_lib.BAD_DIM, "calcderiv_spa: contiguous complex128 device vectors of lunk_spa")
+        check(lib().heomq_calcderiv_spa_at(self._h, float(tt), C.c_void_p(x_spa.data_ptr()),
+                                           C.c_void_p(y_spa.data_ptr()), self._stream()))
+        return y_spa
+
     def jacobi_info(self) -> dict:
         info = np.zeros(5)
         check(lib().heomq_jacobi_info(self._h, _dp(info)))

@@ -85,6 +85,8 @@ SIGNATURES = {
     "heomq_spa_pattern": (C.c_int, [P, C.c_int64, I32, I64, I32, I32]),
     "heomq_apply_spa": (C.c_int, [P, P, P, P]),
     "heomq_calc_ax_tfqmr_spa": (C.c_int, [P, C.c_int64, D, D]),
+    "heomq_calcderiv_spa_at": (C.c_int, [P, C.c_double, P, P, P]),
+    "heomq_calcderiv_spa_tt": (C.c_int, [P, C.c_double, D, D]),
     "heomq_model_jacobi_info": (C.c_int, [P, D, D, D]),
     "heomq_jacobi_info": (C.c_int, [P, D]),
     "heomq_jacobi_step": (C.c_int, [P, P, P, C.c_double, P]),

@@ -289,6 +289,13 @@ int heomq_apply_spa(heomq_ctx* c, const double* x_spa, double* y_spa, void* stre
 /* calc_ax_tfqmr_spa_omp(ldim, rcgtmp, rcgrhs) on host arrays; ldim /= lunk_spa
  * is HEOMQ_BAD_DIM, as calc_ax_tfqmr_spa_omp.f90:61-65 stops. */
 int heomq_calc_ax_tfqmr_spa(heomq_ctx* c, int64_t ldim, const double* x_host, double* y_host);
+/* The sparse propagators' derivative (td/calcderiv_spa_omp.f90:1-11,84-98):
+ * y_spa = P L(tt) P^T x_spa, lunk_spa complex each (no trace element), with
+ * the lead field of heomq_ctx_set_field at tt.  Device vectors. */
+int heomq_calcderiv_spa_at(heomq_ctx* c, double tt, const double* x_spa, double* y_spa, void* stream);
+/* calcderiv_spa_omp(tt) on host arrays (rhotmp_spa -> rhorhs_spa of module
+ * sparsemod; the Fortran drop-in's backend). */
+int heomq_calcderiv_spa_tt(heomq_ctx* c, double tt, const double* x_host, double* y_host);
 
 /*
  * Block-Jacobi iteration (jacobi/solve_jacobi.f90, calc_ax_jacobi_omp.f90,

@@ -1080,6 +1080,20 @@ def apply_spa(model: Model, pattern, x_spa):
     return out
 
 
+def calcderiv_spa(model: Model, pattern, x_spa, gdraw=None):
+    """td/calcderiv_spa_omp.f90: rhotmp_spa -> rhorhs_spa.  Each ADO expanded
+    from COO (zmat_coo2dns), the derivative of the dense hierarchy applied
+    (no trace row; H fixed, the drawers' rates with the lead shift at t in
+    ``gdraw``, :84-98), and the pattern's elements kept (zmat_extract_coo)."""
+    nnz, ind, irow, icol = pattern
+    d = model.d
+    x = np.zeros((model.N, d, d), complex)
+    a_of = np.repeat(np.arange(model.N), nnz)
+    x[a_of, irow - 1, icol - 1] = x_spa
+    y = apply(model, x, trace_row=None, gdraw=gdraw)
+    return y[a_of, irow - 1, icol - 1]
+
+
 # --------------------------------------------------------------------------
 # Block-Jacobi iteration (jacobi/prelude_jacobi.f90, calc_ax_jacobi_omp.f90,
 # solve_jacobi.f90; J. Chem. Phys. 147, 044105 (2017), Eq. 5)

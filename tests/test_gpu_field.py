@@ -86,3 +86,27 @@ def test_field_rejected_on_a_steady_model():
     with pytest.raises(heomq.HeomqError, match="steady"):
         ctx.set_field(0, np.ones((2, 2)))
     ctx.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_calcderiv_spa_at_time(name):
+    """calcderiv_spa_omp(tt): the sparse derivative over a pattern smaller
+    than the physical sector, with the lead shift at tt."""
+    m, ctx, o, torch = setup(name)
+    ft, tc, dc, toff = FIELDS["ac_dc"]
+    mask = O.physical_projection(o, np.ones((o.N, o.d, o.d), complex), hermitian=False) != 0
+    mask &= np.random.default_rng(11).random(mask.shape) < 0.75
+    mask[0] |= np.eye(o.d, dtype=bool)
+    pat = O.spa_pattern_from_mask(mask)
+    ctx.set_spa_pattern(*pat)
+    rng = np.random.default_rng(12)
+    xs = rng.standard_normal(len(pat[2])) + 1j * rng.standard_normal(len(pat[2]))
+    ctx.set_field(ft, tc, dc, toff)
+    try:
+        for tt in (0.0, 1.1):
+            print(f"case {name} calcderiv_spa tt {tt}", flush=True)
+            y = ctx.calcderiv_spa(torch.from_numpy(xs).to("cuda:0"), tt=tt).cpu().numpy()
+            yo = O.calcderiv_spa(o, pat, xs, gdraw=O.gdraw_at(o, O.engy_shift(o, tt, ft, tc, dc, toff)))
+            assert rel(y, yo) <= 1e-12
+    finally:
+        ctx.set_field(-1)

@@ -163,6 +163,25 @@ def test_solve_jacobi_gpu():
     assert np.abs(xs[0] - xo[0]).max() <= 1e-9
 
 
+def test_calcderiv_spa_omp():
+    h, o = shim()
+    mask = O.physical_projection(o, np.ones((o.N, o.d, o.d), complex), hermitian=False) != 0
+    mask &= np.random.default_rng(6).random(mask.shape) < 0.8
+    mask[0] |= np.eye(o.d, dtype=bool)
+    nnz, ind, irow, icol = pat = O.spa_pattern_from_mask(mask)
+    lunk = C.c_int64(len(irow))
+    keep = [np.ascontiguousarray(a) for a in (nnz, ind, irow, icol)]
+    h.heomq_register_spa_pattern_(C.byref(lunk), *[a.ctypes.data_as(C.c_void_p) for a in keep])
+    rng = np.random.default_rng(7)
+    rhotmp_spa = rng.standard_normal(len(irow)) + 1j * rng.standard_normal(len(irow))
+    rhorhs_spa = np.zeros(len(irow), complex)
+    h.heomq_register_td_spa_arrays_(rhotmp_spa.ctypes.data_as(C.c_void_p), rhorhs_spa.ctypes.data_as(C.c_void_p))
+    tt = C.c_double(0.0)
+    h.calcderiv_spa_omp_(C.byref(tt))
+    yo = O.calcderiv_spa(o, pat, rhotmp_spa)
+    assert np.abs(rhorhs_spa - yo).max() <= 1e-12 * np.abs(yo).max()
+
+
 def test_calc_ax_tfqmr_spa_omp():
     h, o = shim()
     mask = O.physical_projection(o, np.ones((o.N, o.d, o.d), complex), hermitian=False) != 0

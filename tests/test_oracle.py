@@ -466,6 +466,29 @@ def test_apply_spa_is_the_restricted_dense_matrix():
             assert np.abs(y[:-1] - want).max() <= 1e-12 * max(np.abs(A).max(), 1.0)
 
 
+def test_calcderiv_spa_is_the_sparse_apply_and_the_dense_derivative():
+    """calcderiv_spa_omp at fixed H is the sparse apply without its trace
+    element; on the full pattern with a lead shift it is the dense
+    derivative at that shift, element for element."""
+    import dataclasses
+    o = O.Model(O.siam_params(nk=2, ntop=2, V=0.3, U=1.5))
+    rng = np.random.default_rng(8)
+    mask = rng.random((o.N, o.d, o.d)) < 0.7
+    mask[0] |= np.eye(o.d, dtype=bool)
+    pat = O.spa_pattern_from_mask(mask)
+    xs = rng.standard_normal(len(pat[2])) + 1j * rng.standard_normal(len(pat[2]))
+    assert np.array_equal(O.calcderiv_spa(o, pat, xs), O.apply_spa(o, pat, xs)[:-1])
+    ot = O.Model(dataclasses.replace(O.siam_params(nk=2, ntop=2, V=0.3, U=1.5), steady=False))
+    full = O.spa_pattern_from_mask(np.ones((ot.N, ot.d, ot.d), bool))
+    x = O.seeded_ado_vector(ot, 22)
+    a_of = np.repeat(np.arange(ot.N), full[0])
+    xf = x[a_of, full[2] - 1, full[3] - 1]
+    g = O.gdraw_at(ot, O.engy_shift(ot, 0.8, 0, np.array([[2.0, 2.0], [3.0, 3.0]])))
+    yd = O.apply(ot, x, trace_row=None, gdraw=g)
+    y = O.calcderiv_spa(ot, full, xf, gdraw=g)
+    assert np.array_equal(y, yd[a_of, full[2] - 1, full[3] - 1])
+
+
 # ---- time-dependent lead shift (getengyshift, calcderiv_omp(tt), intstp) ----
 def _td_pair():
     import dataclasses
