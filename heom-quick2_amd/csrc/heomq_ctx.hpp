@@ -138,6 +138,16 @@ struct heomq_ctx {
   double field_toff = 1e99;
   bool steady_model = true;
   DevBuf td_cnt, td_gam;
+  // time-dependent system Hamiltonian of the propagators (main/calcdhs.f90):
+  // H(t) = hs + sum_k f_k(t) D_k with fixed Hermitian D_k (eV) and the time
+  // functions of calcdhs's options (sys_kind, 4 parameters each); the row
+  // pattern of H(t) is the union of hs's and the D_k's (sys_hnnz, sys_hcol,
+  // d slots per row), sys_base holds hs and sys_terms the D_k / hbar on it,
+  // sys_hval the values at the last evaluated t
+  int sys_nterm = 0;
+  std::vector<int> sys_kind;
+  std::vector<double> sys_par;
+  DevBuf sys_hnnz, sys_hcol, sys_base, sys_terms, sys_hval;
   // sparse-storage drop-in (calc_ax_tfqmr_spa_omp): COO element -> dense
   // index, positions of the tier-0 diagonal, dense scratch; spa_lunk < 0: no pattern
   DevBuf spa_map, spa_diag, spa_dense, spa_io;

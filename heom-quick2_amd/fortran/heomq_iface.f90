@@ -133,6 +133,15 @@ module heomq_iface
       real(c_double), value :: tt
     end function
 
+    integer(c_int) function c_set_system_field(ctx, nterm, ops, kinds, pars) bind(C, name='heomq_ctx_set_system_field')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nterm
+      complex(c_double), intent(in) :: ops(*)
+      integer(c_int32_t), intent(in) :: kinds(*)
+      real(c_double), intent(in) :: pars(*)
+    end function
+
     integer(c_int) function c_calcderiv_spa_tt(ctx, tt, x, y) bind(C, name='heomq_calcderiv_spa_tt')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, x, y
@@ -376,6 +385,22 @@ subroutine heomq_register_spa_pattern(lunk_spa, nnz_spa, ind_spa, irow_spa, icol
   rc = c_spa_pattern(heomq_ctx_h, int(lunk_spa, c_int64_t), nnz_spa, ind_spa, irow_spa, icol_spa)
   if (rc /= 0) call heomq_fail('heomq_register_spa_pattern', rc)
 end subroutine heomq_register_spa_pattern
+
+! The time-dependent system Hamiltonian of main/calcdhs.f90 for
+! calcderiv_omp(tt): dhs(t) = sum_k f_k(t) ops(:,:,k), ops Hermitian in eV,
+! kinds(k) and pars(4,k) the time function of each term (heomq.h lists them);
+! nterm = 0 removes it.
+subroutine heomq_fortran_set_system_field(nterm, ops, kinds, pars)
+  use heomq_iface
+  implicit none
+  integer, intent(in) :: nterm
+  complex*16, intent(in) :: ops(*)
+  integer, intent(in) :: kinds(*)
+  real*8, intent(in) :: pars(*)
+  integer(c_int) :: rc
+  rc = c_set_system_field(heomq_ctx_h, int(nterm, c_int), ops, kinds, pars)
+  if (rc /= 0) call heomq_fail('heomq_fortran_set_system_field', rc)
+end subroutine heomq_fortran_set_system_field
 
 ! td/calcderiv_spa_omp.f90 reads module array rhotmp_spa and writes
 ! rhorhs_spa (module sparsemod); the caller registers their addresses once.

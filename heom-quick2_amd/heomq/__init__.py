@@ -241,6 +241,13 @@ class Model:
         A = ams.reshape(nops, self.d, self.d).transpose(0, 2, 1).copy()   # A[m*nspin+s][i, j]
         return H, A
 
+    def number_operator(self, orb: int, spin: int) -> np.ndarray:
+        """n = c^+ c of one orbital and spin (math-indexed d x d), the
+        operators calcdhs builds its dot fields from."""
+        _, A = self.operators()
+        c = A[orb * 2 + spin]
+        return c.conj().T @ c
+
     def tier(self, n: int) -> np.ndarray:
         cnt = int(self.first[n + 1] - self.first[n])
         out = np.zeros((cnt, n), np.int32)
@@ -564,6 +571,26 @@ class Context:
         check(lib().heomq_rk4_at(self._h, self._check_vec(x, self.torch.complex128), float(t0), dt, nstep,
                                  self._stream()))
         return x
+
+    def set_system_field(self, terms=()):
+        """Time-dependent system Hamiltonian hs + dhs(t) of the propagators
+        (main/calcdhs.f90): terms = [(D, kind, params), ...] with D a Hermitian
+        d x d operator (math-indexed D[i, j]), kind and up to four params of
+        the time function (heomq.h lists them; amplitudes in eV, times in fs).
+        No terms removes it."""
+        terms = list(terms)
+        n = len(terms)
+        d = self.d
+        ops = np.zeros((max(n, 1), d, d), np.complex128)
+        kinds = np.zeros(max(n, 1), np.int32)
+        pars = np.zeros((max(n, 1), 4))
+        for k, (D, kind, par) in enumerate(terms):
+            ops[k] = np.asarray(D, np.complex128).T   # memory column-major
+            kinds[k] = int(kind)
+            par = tuple(par)
+            pars[k, : len(par)] = par
+        check(lib().heomq_ctx_set_system_field(self._h, n, _dp(ops.view(np.float64)),
+                                               kinds.ctypes.data_as(_lib.I32), _dp(pars)))
 
     def set_field(self, fieldtype, tchar=None, amps_dc=None, t_off=1e99):
         """Time-dependent lead shift eshift(alpha, s; t) (main/getengyshift.f90): fieldtype 0

@@ -265,6 +265,20 @@ int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxit
  * HEOMQ_UNSUPPORTED.
  */
 int heomq_ctx_set_field(heomq_ctx* c, int fieldtype, const double* tchar, const double* amps_dc, double t_off);
+/*
+ * Time-dependent system Hamiltonian of the propagators (main/calcdhs.f90;
+ * td/calcderiv_omp.f90:60-68 uses hs + dhs(tt)): dhs(t) = sum_k f_k(t) D_k
+ * with nterm <= 16 fixed Hermitian operators D_k (ops: nterm * d * d complex,
+ * column-major, the energy unit eV) and f_k from calcdhs's options, kinds[k]
+ * with pars[4k..4k+3] (p0 in eV, times in fs, angular frequencies in 1/fs):
+ * 0 p0 (1 - e^(-t/p1)); 1 p0 sin(p1 t); 2 p0 cos(p1 t); 3 p0 on p1 < t < p2;
+ * 4 p0 on p1 <= t < p2; 5 p0 + p1 cos(p2 t + p3); 6 p0 for t < p1; 7 the
+ * Hubbard dot field p0 e^(-p1 t) | -p0 e^(-p1 (t - p2)) | p0 e^(-p1 (t - p3))
+ * split at p2 and p3; 8 p0 for t > p1.  nterm = 0 removes it.  The options
+ * that feed back the state (pha) or project with cproj are not covered.
+ * Non-Hermitian operators are HEOMQ_BAD_ARG.
+ */
+int heomq_ctx_set_system_field(heomq_ctx* c, int nterm, const double* ops, const int32_t* kinds, const double* pars);
 /* y = L(tt) x (dense device vectors), the derivative calcderiv_omp(tt) forms. */
 int heomq_calcderiv_at(heomq_ctx* c, double tt, const double* x, double* y, void* stream);
 /* calcderiv_omp(tt) on host arrays (the Fortran drop-in's backend). */

@@ -974,10 +974,54 @@ def gdraw_at(model: Model, eshift):
     return model.cgamma + 1j * model.dipm * eshift[model.alf, model.spin]
 
 
-def rk4_field(model: Model, x, t0, dt, nstep, field):
+def dhs_coef(kind, p, t):
+    """The scalar time functions of main/calcdhs.f90's options (p[0] in eV):
+    0 dedot (1 - exp(-t/aD)) (:93-94); 1 dedot sin(wD t) (:95-96); 2 dedot
+    cos(w t) (:160-161); 3 egate on ton_gate < t < toff_gate (:112-115); 4 on
+    tson <= t < tsoff (:267-270); 5 gpara1 + gpara2 cos(gpara3 t + gpara4)
+    (:120-128); 6 dedot until t = wdot (:140-142); 7 the Hubbard dot field of
+    :46-58; 8 a term present for t > t_on (tdot12, B fields)."""
+    if kind == 0:
+        return p[0] * (1.0 - np.exp(-t / p[1]))
+    if kind == 1:
+        return p[0] * np.sin(p[1] * t)
+    if kind == 2:
+        return p[0] * np.cos(p[1] * t)
+    if kind == 3:
+        return p[0] if p[1] < t < p[2] else 0.0
+    if kind == 4:
+        return p[0] if p[1] <= t < p[2] else 0.0
+    if kind == 5:
+        return p[0] + p[1] * np.cos(p[2] * t + p[3])
+    if kind == 6:
+        return p[0] if t < p[1] else 0.0
+    if kind == 7:
+        if t < p[2]:
+            return p[0] * np.exp(-p[1] * t)
+        if t < p[3]:
+            return -p[0] * np.exp(-p[1] * (t - p[2]))
+        return p[0] * np.exp(-p[1] * (t - p[3]))
+    if kind == 8:
+        return p[0] if t > p[1] else 0.0
+    raise ValueError("kind")
+
+
+def h_at(model: Model, t, terms):
+    """hs + dhs(t) (td/calcderiv_omp.f90:60-68), dhs = sum f_k(t) D_k / hbar
+    for terms [(D math-indexed, kind, params)]."""
+    H = model.H.astype(complex).copy()
+    for D, kind, p in terms:
+        p = list(p) + [0.0] * (4 - len(p))
+        H = H + dhs_coef(kind, p, t) * np.asarray(D, complex) / HBAR
+    return H
+
+
+def rk4_field(model: Model, x, t0, dt, nstep, field, terms=()):
     """td/intstp.f90 with the derivative calcderiv_omp(tt) at t, t + dt/2 (twice)
-    and t + dt; field = (fieldtype, tchar, amps_dc, t_off)."""
-    f = lambda t, v: apply(model, v, trace_row=None, gdraw=gdraw_at(model, engy_shift(model, t, *field)))
+    and t + dt; field = (fieldtype, tchar, amps_dc, t_off), terms the system
+    field of h_at."""
+    f = lambda t, v: apply(model, v, trace_row=None, gdraw=gdraw_at(model, engy_shift(model, t, *field)),
+                           H=h_at(model, t, terms) if terms else None)
     x = x.copy()
     for step in range(nstep):
         t = t0 + step * dt

@@ -110,3 +110,53 @@ def test_calcderiv_spa_at_time(name):
             assert rel(y, yo) <= 1e-12
     finally:
         ctx.set_field(-1)
+
+
+def _sys_terms(m, o):
+    """calcdhs-style terms: a gated level shift of dot 1 (egate window), an
+    ac drive on the last orbital, and for two dots a switched-on hopping
+    t12 (c1^+ c2 + h.c. per spin); operators from the device model,
+    checked against the oracle's."""
+    n = lambda orb: m.number_operator(orb, 0) + m.number_operator(orb, 1)
+    terms = [(n(0), 3, (0.25, 0.2, 1.5)), (n(m.params.norbs - 1), 5, (0.05, 0.1, 2.0, 0.3))]
+    if m.params.norbs == 2:
+        _, A = m.operators()
+        hop = sum(A[0 * 2 + s].conj().T @ A[1 * 2 + s] for s in range(2))
+        terms.append((hop + hop.conj().T, 8, (0.15, 0.05)))
+    no = sum(o.ams[0][s].T @ o.ams[0][s] for s in range(2))
+    assert np.abs(no - n(0)).max() == 0.0
+    return terms
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_calcderiv_with_system_field(name):
+    m, ctx, o, torch = setup(name)
+    terms = _sys_terms(m, o)
+    ft, tc, dc, toff = FIELDS["switch_on"]
+    ctx.set_system_field(terms)
+    ctx.set_field(ft, tc, dc, toff)
+    try:
+        x = O.seeded_ado_vector(o, 19)
+        for tt in (0.0, 0.1, 0.9, 2.0):
+            print(f"case {name} system field tt {tt}", flush=True)
+            y = from_dev(ctx.calcderiv(to_dev(torch, x), tt=tt))
+            yo = O.apply(o, x, trace_row=None, gdraw=O.gdraw_at(o, O.engy_shift(o, tt, ft, tc, dc, toff)),
+                         H=O.h_at(o, tt, terms))
+            assert rel(y, yo) <= 1e-12
+        x0 = O.physical_projection(o, O.seeded_ado_vector(o, 20))
+        xr = from_dev(ctx.rk4(to_dev(torch, x0), 0.02, 8, t0=0.1))
+        xo = O.rk4_field(o, x0, 0.1, 0.02, 8, (ft, tc, dc, toff), terms)
+        assert rel(xr, xo) <= 1e-12
+    finally:
+        ctx.set_system_field([])
+        ctx.set_field(-1)
+
+
+def test_system_field_rejects_non_hermitian():
+    m, ctx, o, torch = setup("siam_nk2_L2")
+    bad = np.zeros((m.d, m.d), complex)
+    bad[0, 1] = 1.0
+    with pytest.raises(heomq.HeomqError):
+        ctx.set_system_field([(bad, 2, (0.1, 1.0))])
+    with pytest.raises(heomq.HeomqError):
+        ctx.set_system_field([(np.eye(m.d), 9, (0.1,))])

@@ -38,7 +38,7 @@ def test_fortran_shim_exports_reference_names():
     for name in ("calc_ax_tfqmr_omp_", "calc_ax_bicg_omp_", "calc_atrx_bicg_omp_", "calcderiv_omp_",
                  "heomq_fortran_init_", "solve_tfqmr_gpu_", "solve_bicg_gpu_", "calc_ax_jacobi_omp_",
                  "solve_jacobi_gpu_", "calc_ax_tfqmr_spa_omp_", "heomq_register_spa_pattern_",
-                 "calcderiv_spa_omp_", "heomq_register_td_spa_arrays_"):
+                 "calcderiv_spa_omp_", "heomq_register_td_spa_arrays_", "heomq_fortran_set_system_field_"):
         assert hasattr(h, name), name
 
 

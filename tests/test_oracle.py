@@ -489,6 +489,29 @@ def test_calcderiv_spa_is_the_sparse_apply_and_the_dense_derivative():
     assert np.array_equal(y, yd[a_of, full[2] - 1, full[3] - 1])
 
 
+def test_system_field_limit_is_the_shifted_level():
+    """dhs(t) of calcdhs's dedot (1 - exp(-t/aD)) on n_up + n_dn: long after
+    the switch-on, hs + dhs is the Hamiltonian of the model whose level sits
+    dedot higher (built independently by calchs), and so is its derivative."""
+    import dataclasses
+    p = dataclasses.replace(O.siam_params(nk=1, ntop=2, V=0.6, U=1.5, T=0.2), steady=False)
+    o = O.Model(p)
+    p2 = dataclasses.replace(O.siam_params(nk=1, ntop=2, V=0.6, U=1.5, T=0.2, eps=-1.0 + 0.3), steady=False)
+    o2 = O.Model(p2)
+    n = sum(o.ams[0][s].T @ o.ams[0][s] for s in range(2))
+    terms = [(n, 0, (0.3, 2.0))]
+    H = O.h_at(o, 1e4, terms)
+    assert np.abs(H - o2.H).max() <= 1e-12 * np.abs(o2.H).max()
+    assert np.abs(O.h_at(o, 0.0, terms) - o.H).max() == 0.0
+    x = O.seeded_ado_vector(o, 23)
+    y = O.apply(o, x, trace_row=None, H=H)
+    y2 = O.apply(o2, x, trace_row=None)
+    assert np.abs(y - y2).max() <= 1e-12 * np.abs(y2).max()
+    # window and gate time functions at their edges (calcdhs.f90:112, :267)
+    assert O.dhs_coef(3, [1.0, 1.0, 2.0], 1.0) == 0.0 and O.dhs_coef(4, [1.0, 1.0, 2.0], 1.0) == 1.0
+    assert O.dhs_coef(3, [1.0, 1.0, 2.0], 2.0) == 0.0 and O.dhs_coef(4, [1.0, 1.0, 2.0], 2.0) == 0.0
+
+
 # ---- time-dependent lead shift (getengyshift, calcderiv_omp(tt), intstp) ----
 def _td_pair():
     import dataclasses
