@@ -22,6 +22,13 @@ main/initialize2.f90:43-82 and fermion/initialize2_spa.f90:24-104 read them):
 ``indextable.sav``
     ``form='binary'``: ntable as int64, then indextable(1:ntable) as int32.
 
+``TAPE_<n>.resume`` / ``TAPE.resume`` (td/resumejob.f90:15-103, the
+propagators' checkpoints)
+    ``form='binary'``: tt as real*8; (ntier, norbs, nspin, ncor, nalf,
+    numfff) as int32; nunk as int64; lhb as a 4-byte logical (then four
+    int32 of the heat bath when set); then either lunk_spa as int64 and
+    rho_spa(1:lunk_spa), or the nunk dense ADO blocks, complex*16.
+
 ADO order is the reference's (tier-major, drawer tuples sorted), the order
 of the dense vectors of this package: x[a] is ADO a, its memory the
 column-major d x d block.
@@ -286,6 +293,90 @@ def read_indextable(path) -> np.ndarray:
     if n < 0 or len(data) != 8 + 4 * n:
         raise FileFormatError(f"{path}: {len(data)} bytes do not hold {n} int32 entries")
     return np.frombuffer(data, _I4, n, 8).copy()
+
+
+class ResumeHeader(NamedTuple):
+    t: float
+    ntier: int
+    norbs: int
+    nspin: int
+    ncor: int
+    nalf: int
+    numfff: int
+    nunk: int
+    lhb: bool
+    hb: tuple = ()          # (npade_hb, ndrude_hb, nmode_hb, itype_hb) when lhb
+    lunk_spa: int = -1      # sparse checkpoints: the COO length; -1 dense
+
+
+def resume_name(t: float, dt: float, nresume: int, nfbase: int = 0) -> str:
+    """The checkpoint file name td/resumejob.f90:81-84 forms at time t."""
+    n = nfbase + int((t + 1e-9) / dt) // nresume
+    return f"TAPE_{n}.resume"
+
+
+def write_resume(path, t: float, x, model=None, header: StateHeader | None = None, numfff: int = 0,
+                 sparse: bool = False) -> None:
+    """A propagation checkpoint as td/resumejob.f90:85-101 writes it: x is the
+    dense state (nunk, d, d) or, with sparse=True, rho_spa (lunk_spa,).
+    The header comes from `model` (or `header`); lhb is false."""
+    h = header_of(model) if header is None else header
+    xs = _host_c16(x)
+    with open(path, "wb") as f:
+        f.write(np.array([t], "<f8").tobytes())
+        f.write(np.array([h.ntier, h.norbs, h.nspin, h.ncor, h.nalf, numfff], _I4).tobytes())
+        f.write(np.array([h.nunk], _I8).tobytes())
+        f.write(np.array([0], _I4).tobytes())   # lhb = .false.
+        if sparse:
+            v = xs.ravel()
+            f.write(np.array([v.size], _I8).tobytes())
+            f.write(v.tobytes())
+        else:
+            if xs.ndim != 3 or xs.shape[0] != h.nunk:
+                raise FileFormatError(f"x must be (nunk={h.nunk}, d, d), got {xs.shape}")
+            f.write(xs.tobytes())
+
+
+def read_resume(path, model=None, sparse: bool = False, d: int | None = None, lunk_spa: int | None = None):
+    """Read a checkpoint (td/resumejob.f90:17-75); returns (ResumeHeader, x).
+    With `model` the checks of :26-47 apply: same ntier, norbs, nspin, ncor,
+    nalf and nunk (numfff must be 0 here), and for sparse files lunk_spa
+    when given.  Dense files need the block size d (from `model` or `d`)."""
+    path = Path(path)
+    data = path.read_bytes()
+    if len(data) < 8 + 24 + 8 + 4:
+        raise FileFormatError(f"{path}: shorter than its header")
+    t = float(np.frombuffer(data, "<f8", 1, 0)[0])
+    ntier, norbs, nspin, ncor, nalf, numfff = (int(v) for v in np.frombuffer(data, _I4, 6, 8))
+    nunk = int(np.frombuffer(data, _I8, 1, 32)[0])
+    lhb = int(np.frombuffer(data, _I4, 1, 40)[0]) != 0
+    pos, hb = 44, ()
+    if lhb:
+        hb = tuple(int(v) for v in np.frombuffer(data, _I4, 4, pos))
+        pos += 16
+    if model is not None:
+        m = header_of(model)
+        if (ntier, norbs, nspin, ncor, nalf, numfff) != (m.ntier, m.norbs, m.nspin, m.ncor, m.nalf, 0) or lhb:
+            raise FileFormatError(f"bad {path.name} to read 1: {(ntier, norbs, nspin, ncor, nalf, numfff)}")
+        if nunk != m.nunk:
+            raise FileFormatError(f"bad {path.name} to read 2: nunk {nunk}, model {m.nunk}")
+        d = model.d
+    lsp = -1
+    if sparse:
+        lsp = int(np.frombuffer(data, _I8, 1, pos)[0])
+        pos += 8
+        if lunk_spa is not None and lsp != lunk_spa:
+            raise FileFormatError(f"bad {path.name} to read 3: lunk_spa {lsp}, expected {lunk_spa}")
+        if len(data) != pos + 16 * lsp:
+            raise FileFormatError(f"{path}: {len(data) - pos} bytes do not hold {lsp} complex values")
+        x = np.frombuffer(data, _C16, lsp, pos).copy()
+    else:
+        if d is None:
+            raise FileFormatError("dense checkpoints need the block size d (or a model)")
+        if len(data) != pos + 16 * nunk * d * d:
+            raise FileFormatError(f"{path}: {len(data) - pos} bytes do not hold {nunk} blocks of {d}x{d}")
+        x = np.frombuffer(data, _C16, nunk * d * d, pos).reshape(nunk, d, d).copy()
+    return ResumeHeader(t, ntier, norbs, nspin, ncor, nalf, numfff, nunk, lhb, hb, lsp), x
 
 
 def _host_c16(x) -> np.ndarray:

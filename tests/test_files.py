@@ -217,3 +217,95 @@ def test_layout_matches_a_fortran_compiler(tmp_path):
     assert h.nunk == 4
     assert np.array_equal(xsp[:3], rho[0].reshape(-1)[0:3])
     assert np.array_equal(xsp[3:5], rho[2].reshape(-1)[4:6])
+
+
+def test_resume_round_trip_and_checks(tmp_path):
+    m = heomq.Model(heomq.siam(nk=2, ntop=2))
+    x = _state(m.N, m.d, 13)
+    p = tmp_path / F.resume_name(2.5, 0.01, 50)
+    assert p.name == "TAPE_5.resume"   # int((2.5 + 1e-9) / 0.01) / 50 = 250 / 50
+    F.write_resume(p, 2.5, x, model=m)
+    raw = p.read_bytes()
+    assert len(raw) == 8 + 24 + 8 + 4 + 16 * m.N * m.d * m.d
+    h, x2 = F.read_resume(p, model=m)
+    assert h.t == 2.5 and h.nunk == m.N and not h.lhb and np.array_equal(x2, x)
+    other = heomq.Model(heomq.siam(nk=2, ntop=3))
+    with pytest.raises(F.FileFormatError):
+        F.read_resume(p, model=other)
+    xs = x.reshape(-1)[:37]
+    F.write_resume(p, 0.75, xs, model=m, sparse=True)
+    h, xr = F.read_resume(p, model=m, sparse=True, lunk_spa=37)
+    assert h.lunk_spa == 37 and h.t == 0.75 and np.array_equal(xr, xs)
+    with pytest.raises(F.FileFormatError):
+        F.read_resume(p, model=m, sparse=True, lunk_spa=36)
+
+
+FORTRAN_RESUME = """
+program resume_check
+implicit none
+integer :: ntier, norbs, nspin, ncor, nalf, numfff, ni, nj
+integer*8 :: nunk, lni
+logical :: lhb
+real*8 :: tt
+complex*16 :: rho(2,2,3), acc
+! the checkpoint as td/resumejob.f90 writes it
+tt = 1.25d0; ntier = 3; norbs = 1; nspin = 2; ncor = 4; nalf = 2; numfff = 0; nunk = 3; lhb = .false.
+do lni = 1, nunk
+  do nj = 1, 2
+    do ni = 1, 2
+      rho(ni,nj,lni) = dcmplx(dble(ni + 2*nj), -dble(lni))
+    end do
+  end do
+end do
+open(unit=23, file='TAPE_7.resume', form='binary', status='unknown')
+rewind(23)
+write(23)tt
+write(23)ntier, norbs, nspin, ncor, nalf, numfff
+write(23)nunk
+write(23)lhb
+do lni=1,nunk
+  write(23)((rho(ni,nj,lni), ni=1,2), nj=1,2)
+end do
+close(23)
+! and read as it reads TAPE.resume (written here by Python)
+open(unit=23, file='TAPE.resume', form='binary', status='old')
+read(23)tt
+read(23)ntier, norbs, nspin, ncor, nalf, numfff
+read(23)nunk
+read(23)lhb
+acc = (0.d0, 0.d0)
+do lni=1,nunk
+  read(23)((rho(ni,nj,lni), ni=1,2), nj=1,2)
+  do nj = 1, 2
+    do ni = 1, 2
+      acc = acc + rho(ni,nj,lni) * dble(ni + 2*(nj-1) + 4*(lni-1))
+    end do
+  end do
+end do
+close(23)
+write(6,'(f10.4,6i4,i6,l2)') tt, ntier, norbs, nspin, ncor, nalf, numfff, nunk, lhb
+write(6,'(2es25.16)') dble(acc), dimag(acc)
+end program
+"""
+
+
+@pytest.mark.skipif(_flang() is None, reason="no Fortran compiler in this toolchain")
+def test_resume_layout_matches_a_fortran_compiler(tmp_path):
+    xs = _state(3, 2, 14)
+    F.write_resume(tmp_path / "TAPE.resume", 0.5, xs, header=F.StateHeader(1, 2, 4, 3, 2, 3))
+    (tmp_path / "resume_check.f90").write_text(FORTRAN_RESUME)
+    subprocess.run([_flang(), "-O0", "resume_check.f90", "-o", "resume_check"], cwd=tmp_path, check=True,
+                   capture_output=True)
+    out = subprocess.run(["./resume_check"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout.split()
+    assert float(out[0]) == 0.5 and [int(v) for v in out[1:8]] == [3, 1, 2, 4, 2, 0, 3] and out[8] == "F"
+    acc = np.sum(xs.reshape(3, 4) * (np.arange(12).reshape(3, 4) + 1))
+    tol = 1e-12 * np.abs(xs).sum() * 12
+    assert abs(float(out[9]) - acc.real) <= tol and abs(float(out[10]) - acc.imag) <= tol
+    h, x = F.read_resume(tmp_path / "TAPE_7.resume", d=2)
+    assert (h.t, h.ntier, h.norbs, h.nspin, h.ncor, h.nalf, h.numfff, h.nunk, h.lhb) == (1.25, 3, 1, 2, 4, 2, 0, 3, False)
+    want = np.zeros((3, 2, 2), complex)
+    for lni in range(1, 4):
+        for nj in range(1, 3):
+            for ni in range(1, 3):
+                want[lni - 1].reshape(-1)[(ni - 1) + 2 * (nj - 1)] = complex(ni + 2 * nj, -lni)
+    assert np.array_equal(x, want)
