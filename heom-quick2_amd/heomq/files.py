@@ -22,6 +22,16 @@ main/initialize2.f90:43-82 and fermion/initialize2_spa.f90:24-104 read them):
 ``indextable.sav``
     ``form='binary'``: ntable as int64, then indextable(1:ntable) as int32.
 
+``coefindex.data`` and ``auxindex.data`` (main/buildcoefindex.f90:683-709
+write them, :44-140 read them back instead of rebuilding)
+    ``form='binary'``: (ntier, ncor, norbs, nspin, nalf, numfff, ntier0,
+    ndrawer_slow, ncor_slow) as int32; (lad, lad_fast, lset_fast) as 4-byte
+    logicals, idegen_fast and ndegen_fast as int32, dratio_fast as real*8;
+    lfilter, nfilter_count, nfilter_long; ltrun_der; lsimple, lwalf; lscreen;
+    lomit, dfact_omit; lall as int64; then per coupling indexcoef (int64),
+    itypecoef, ifactfcoef, ifactrcoef (int8).  auxindex.data: nunk as int64,
+    then index_coef_ref(1:nunk) as int64.
+
 ``TAPE_<n>.resume`` / ``TAPE.resume`` (td/resumejob.f90:15-103, the
 propagators' checkpoints)
     ``form='binary'``: tt as real*8; (ntier, norbs, nspin, ncor, nalf,
@@ -293,6 +303,83 @@ def read_indextable(path) -> np.ndarray:
     if n < 0 or len(data) != 8 + 4 * n:
         raise FileFormatError(f"{path}: {len(data)} bytes do not hold {n} int32 entries")
     return np.frombuffer(data, _I4, n, 8).copy()
+
+
+_COEF_DT = np.dtype([("indexcoef", "<i8"), ("itypecoef", "i1"), ("ifactfcoef", "i1"), ("ifactrcoef", "i1")])
+
+
+def write_coef_tables(coefindex_path, auxindex_path, tables: dict, header: StateHeader, numfff: int = 0) -> None:
+    """coefindex.data and auxindex.data for the default flags (no adiabatic
+    modes, filter, terminator, simple or screened hierarchy, omission): the
+    reference reads them back when the header matches.  tables: indexcoef,
+    itypecoef, ifactfcoef, ifactrcoef (lall each) and index_coef_ref (nunk),
+    the module arrays of module/matmod.f90:7-9."""
+    ic = np.asarray(tables["indexcoef"], np.int64)
+    rec = np.empty(ic.size, _COEF_DT)
+    rec["indexcoef"] = ic
+    for k in ("itypecoef", "ifactfcoef", "ifactrcoef"):
+        v = np.asarray(tables[k])
+        if v.size != ic.size:
+            raise FileFormatError(f"{k} needs lall = {ic.size} entries")
+        rec[k] = v.astype(np.int8)
+    ref = np.asarray(tables["index_coef_ref"], np.int64)
+    if ref.size != header.nunk:
+        raise FileFormatError(f"index_coef_ref needs nunk = {header.nunk} entries")
+    f4 = lambda *v: np.array(v, _I4).tobytes()
+    with open(coefindex_path, "wb") as f:
+        f.write(f4(header.ntier, header.ncor, header.norbs, header.nspin, header.nalf, numfff, header.ntier, 0, 0))
+        f.write(f4(0, 0, 0, 0, 0) + np.array([0.0], "<f8").tobytes())   # lad, lad_fast, lset_fast, idegen, ndegen, dratio
+        f.write(f4(0, 0, 0))                                          # lfilter, nfilter_count, nfilter_long
+        f.write(f4(0))                                                # ltrun_der
+        f.write(f4(0, 0))                                             # lsimple, lwalf
+        f.write(f4(0))                                                # lscreen
+        f.write(f4(0) + np.array([0.0], "<f8").tobytes())            # lomit, dfact_omit
+        f.write(np.array([ic.size], _I8).tobytes())
+        f.write(rec.tobytes())
+    with open(auxindex_path, "wb") as f:
+        f.write(np.array([ref.size], _I8).tobytes())
+        f.write(ref.tobytes())
+
+
+def read_coef_tables(coefindex_path, auxindex_path, model=None):
+    """Read coefindex.data and auxindex.data; returns (header dict, tables
+    dict).  Files written with any of the non-default flags are refused (the
+    loader takes the default-flag tables only).  With `model` the header
+    checks of main/buildcoefindex.f90:56-100 and the nunk check of auxindex
+    apply."""
+    data = Path(coefindex_path).read_bytes()
+    if len(data) < 36 + 28 + 12 + 4 + 8 + 4 + 12 + 8:
+        raise FileFormatError(f"{coefindex_path}: shorter than its header")
+    h1 = [int(v) for v in np.frombuffer(data, _I4, 9, 0)]
+    hdr = dict(zip(("ntier", "ncor", "norbs", "nspin", "nalf", "numfff", "ntier0", "ndrawer_slow", "ncor_slow"), h1))
+    flags = dict(lad=np.frombuffer(data, _I4, 1, 36)[0], lfilter=np.frombuffer(data, _I4, 1, 64)[0],
+                 ltrun_der=np.frombuffer(data, _I4, 1, 76)[0], lsimple=np.frombuffer(data, _I4, 1, 80)[0],
+                 lwalf=np.frombuffer(data, _I4, 1, 84)[0], lscreen=np.frombuffer(data, _I4, 1, 88)[0],
+                 lomit=np.frombuffer(data, _I4, 1, 92)[0])
+    on = [k for k, v in flags.items() if v != 0]
+    if on:
+        raise FileFormatError(f"{coefindex_path}: written with {', '.join(on)}; the default-flag tables only")
+    if model is not None:
+        m = header_of(model)
+        want = dict(ntier=m.ntier, ncor=m.ncor, norbs=m.norbs, nspin=m.nspin, nalf=m.nalf, numfff=0)
+        bad = {k: hdr[k] for k in want if hdr[k] != want[k]}
+        if bad or hdr["ntier0"] != m.ntier:
+            raise FileFormatError(f"{coefindex_path}: header {hdr} does not match the model {want}")
+    lall = int(np.frombuffer(data, _I8, 1, 104)[0])
+    if len(data) != 112 + 11 * lall:
+        raise FileFormatError(f"{coefindex_path}: {len(data) - 112} bytes do not hold {lall} 11-byte entries")
+    rec = np.frombuffer(data, _COEF_DT, lall, 112)
+    aux = Path(auxindex_path).read_bytes()
+    nunk = int(np.frombuffer(aux, _I8, 1, 0)[0])
+    if len(aux) != 8 + 8 * nunk:
+        raise FileFormatError(f"{auxindex_path}: {len(aux) - 8} bytes do not hold {nunk} int64 entries")
+    if model is not None and nunk != model.N:
+        raise FileFormatError(f"{auxindex_path}: nunk {nunk}, the model has {model.N}")
+    hdr["lall"], hdr["nunk"] = lall, nunk
+    tables = dict(indexcoef=rec["indexcoef"].copy(), itypecoef=rec["itypecoef"].copy(),
+                  ifactfcoef=rec["ifactfcoef"].copy(), ifactrcoef=rec["ifactrcoef"].copy(),
+                  index_coef_ref=np.frombuffer(aux, _I8, nunk, 8).copy())
+    return hdr, tables
 
 
 class ResumeHeader(NamedTuple):

@@ -309,3 +309,91 @@ def test_resume_layout_matches_a_fortran_compiler(tmp_path):
             for ni in range(1, 3):
                 want[lni - 1].reshape(-1)[(ni - 1) + 2 * (nj - 1)] = complex(ni + 2 * nj, -lni)
     assert np.array_equal(x, want)
+
+
+def test_coef_tables_round_trip_into_the_loader(tmp_path):
+    """coefindex.data / auxindex.data written from the reference's module
+    arrays, read back, and loaded: the same hierarchy and couplings as the
+    model built here."""
+    # the reference's hierarchy keeps repeated drawers (lsdraw), so every
+    # upper-tier coupling of its table has a neighbour
+    o = O.Model(O.siam_params(nk=2, ntop=2, lsdraw=True))
+    arr = reference_arrays(o)
+    m = heomq.Model(heomq.siam(nk=2, ntop=2, lsdraw=True))
+    F.write_coef_tables(tmp_path / "coefindex.data", tmp_path / "auxindex.data", arr, F.header_of(m))
+    raw = (tmp_path / "coefindex.data").read_bytes()
+    lall = len(arr["indexcoef"])
+    assert len(raw) == 112 + 11 * lall and np.frombuffer(raw[104:112], np.int64)[0] == lall
+    hdr, tab = F.read_coef_tables(tmp_path / "coefindex.data", tmp_path / "auxindex.data", model=m)
+    assert hdr["nunk"] == m.N and hdr["ntier"] == m.L + 1
+    for k in tab:
+        assert np.array_equal(tab[k], np.asarray(arr[k]).astype(tab[k].dtype)), k
+    arr2 = dict(arr, **tab)
+    m2 = heomq.Model.from_reference(**arr2)
+    lo1, up1 = m.coupling()
+    lo2, up2 = m2.coupling()
+    assert np.array_equal(lo1, lo2) and np.array_equal(up1, up2)
+    with pytest.raises(F.FileFormatError):
+        F.read_coef_tables(tmp_path / "coefindex.data", tmp_path / "auxindex.data",
+                           model=heomq.Model(heomq.siam(nk=3, ntop=2, lsdraw=True)))
+
+
+FORTRAN_COEF = """
+program coef_check
+implicit none
+integer :: ntier, ncor, norbs, nspin, nalf, numfff, ntier0, ndrawer_slow, ncor_slow
+integer :: idegen_fast, ndegen_fast, nfilter_count, nfilter_long
+logical :: lad, lad_fast, lset_fast, lfilter, ltrun_der, lsimple, lwalf, lscreen, lomit
+real*8 :: dratio_fast, dfact_omit
+integer*8 :: lall, lni, nunk
+integer*8 :: indexcoef(3), index_coef_ref(2)
+integer*1 :: itypecoef(3), ifactfcoef(3), ifactrcoef(3)
+ntier = 3; ncor = 4; norbs = 1; nspin = 2; nalf = 2; numfff = 0; ntier0 = 3; ndrawer_slow = 0; ncor_slow = 0
+lad = .false.; lad_fast = .false.; lset_fast = .false.; idegen_fast = 0; ndegen_fast = 0; dratio_fast = 0.d0
+lfilter = .false.; nfilter_count = 0; nfilter_long = 0; ltrun_der = .false.; lsimple = .false.; lwalf = .false.
+lscreen = .false.; lomit = .false.; dfact_omit = 0.d0
+lall = 3; nunk = 2
+indexcoef = (/ 7_8, 123456789012_8, 1_8 /)
+itypecoef = (/ 0_1, 1_1, 0_1 /); ifactfcoef = (/ 1_1, -1_1, 1_1 /); ifactrcoef = (/ -1_1, 1_1, 1_1 /)
+index_coef_ref = (/ 1_8, 4_8 /)
+! the writes of main/buildcoefindex.f90
+open(unit=17, file='coefindex.data', form='binary', status='unknown')
+rewind(17)
+write(17)ntier, ncor, norbs, nspin, nalf, numfff, ntier0, ndrawer_slow, ncor_slow
+write(17)lad, lad_fast, lset_fast, idegen_fast, ndegen_fast, dratio_fast
+write(17)lfilter, nfilter_count, nfilter_long
+write(17)ltrun_der
+write(17)lsimple, lwalf
+write(17)lscreen
+write(17)lomit, dfact_omit
+write(17)lall
+do lni=1,lall
+   write(17)indexcoef(lni), itypecoef(lni), ifactfcoef(lni), ifactrcoef(lni)
+end do
+close(17)
+open(unit=37, file='auxindex.data', form='binary', status='unknown')
+rewind(37)
+write(37)nunk
+do lni=1,nunk
+   write(37)index_coef_ref(lni)
+end do
+close(37)
+end program
+"""
+
+
+@pytest.mark.skipif(_flang() is None, reason="no Fortran compiler in this toolchain")
+def test_coef_tables_layout_matches_a_fortran_compiler(tmp_path):
+    (tmp_path / "coef_check.f90").write_text(FORTRAN_COEF)
+    subprocess.run([_flang(), "-O0", "coef_check.f90", "-o", "coef_check"], cwd=tmp_path, check=True,
+                   capture_output=True)
+    subprocess.run(["./coef_check"], cwd=tmp_path, check=True, capture_output=True)
+    hdr, tab = F.read_coef_tables(tmp_path / "coefindex.data", tmp_path / "auxindex.data")
+    assert (hdr["ntier"], hdr["ncor"], hdr["lall"], hdr["nunk"]) == (3, 4, 3, 2)
+    assert list(tab["indexcoef"]) == [7, 123456789012, 1]
+    assert list(tab["itypecoef"]) == [0, 1, 0] and list(tab["ifactfcoef"]) == [1, -1, 1]
+    assert list(tab["ifactrcoef"]) == [-1, 1, 1] and list(tab["index_coef_ref"]) == [1, 4]
+    # and the bytes this package writes for the same tables are the compiler's
+    F.write_coef_tables(tmp_path / "c2.data", tmp_path / "a2.data", tab, F.StateHeader(1, 2, 4, 3, 2, 2))
+    assert (tmp_path / "c2.data").read_bytes() == (tmp_path / "coefindex.data").read_bytes()
+    assert (tmp_path / "a2.data").read_bytes() == (tmp_path / "auxindex.data").read_bytes()
