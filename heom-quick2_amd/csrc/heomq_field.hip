@@ -228,6 +228,15 @@ int heomq_ctx_set_system_field(heomq_ctx* c, int nterm, const double* ops, const
   return HEOMQ_OK;
 }
 
+int heomq_ctx_field_shifts(const heomq_ctx* c, double tt, double* eshift_ev) {
+  if (!c || !eshift_ev) return fail(HEOMQ_BAD_ARG, "null argument");
+  const int k = c->nalf * c->nspin;
+  std::vector<double> es(kMaxLeadSpin, 0.0);
+  if (c->field_type >= 0) field_shifts(c, tt, es.data());
+  for (int q = 0; q < k; ++q) eshift_ev[q] = es[q] * kHbar;
+  return HEOMQ_OK;
+}
+
 int heomq_calcderiv_at(heomq_ctx* c, double tt, const double* x, double* y, void* stream) {
   if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;

@@ -600,6 +600,19 @@ class Context:
         dc = None if amps_dc is None else np.ascontiguousarray(amps_dc, np.float64).ravel()
         check(lib().heomq_ctx_set_field(self._h, int(fieldtype), None if tc is None else _dp(tc),
                                         None if dc is None else _dp(dc), float(t_off)))
+        self._field_type = int(fieldtype)
+
+    def field_type(self) -> int:
+        """The lead field's fieldtype (-1: none)."""
+        return getattr(self, "_field_type", -1)
+
+    def field_shifts(self, tt: float) -> np.ndarray:
+        """eshift(alpha, s; tt) of the lead field in eV, (nalf, nspin); zeros without one."""
+        p = self.model.params
+        nalf = p.nalf if p else 2
+        es = np.zeros(nalf * 2)
+        check(lib().heomq_ctx_field_shifts(self._h, float(tt), _dp(es)))
+        return es.reshape(nalf, 2)
 
     def calcderiv(self, x, tt=0.0, y=None):
         """dx/dt at time tt (calcderiv_omp(tt)): the apply with no trace row and gamma_K(tt)."""

@@ -279,6 +279,10 @@ int heomq_ctx_set_field(heomq_ctx* c, int fieldtype, const double* tchar, const 
  * Non-Hermitian operators are HEOMQ_BAD_ARG.
  */
 int heomq_ctx_set_system_field(heomq_ctx* c, int nterm, const double* ops, const int32_t* kinds, const double* pars);
+/* eshift(alpha, s; tt) of the lead field in eV, nalf * nspin values
+ * [alpha * nspin + s] (main/getengyshift.f90 as main/main.f90:1453-1458
+ * prints it); zeros when no field is set.  Host only. */
+int heomq_ctx_field_shifts(const heomq_ctx* c, double tt, double* eshift_ev);
 /* y = L(tt) x (dense device vectors), the derivative calcderiv_omp(tt) forms. */
 int heomq_calcderiv_at(heomq_ctx* c, double tt, const double* x, double* y, void* stream);
 /* calcderiv_omp(tt) on host arrays (the Fortran drop-in's backend). */

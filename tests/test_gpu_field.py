@@ -160,3 +160,37 @@ def test_system_field_rejects_non_hermitian():
         ctx.set_system_field([(bad, 2, (0.1, 1.0))])
     with pytest.raises(heomq.HeomqError):
         ctx.set_system_field([(np.eye(m.d), 9, (0.1,))])
+
+
+def test_propagate_writes_reference_records(tmp_path):
+    """heomq.propagate: RK4 steps with the lead field, one curr.data and
+    popu.data line per step in the reference's column layout (nalf = 2)."""
+    from heomq.propagate import propagate
+    m, ctx, o, torch = setup("dqd_nk1_L2")
+    ft, tc, dc, toff = FIELDS["switch_on"]
+    ctx.set_field(ft, tc, dc, toff)
+    try:
+        x0 = O.physical_projection(o, O.seeded_ado_vector(o, 21))
+        x = to_dev(torch, x0)
+        xr = x.clone()
+        t_end = propagate(ctx, x, 0.1, 0.02, 3, curr=tmp_path / "curr.data", popu=tmp_path / "popu.data")
+        assert abs(t_end - 0.16) < 1e-12
+        curr = [list(map(float, l.split())) for l in (tmp_path / "curr.data").read_text().splitlines()]
+        popu = [list(map(float, l.split())) for l in (tmp_path / "popu.data").read_text().splitlines()]
+        assert len(curr) == 3 and len(popu) == 3 and len(curr[0]) == 9 and len(popu[0]) == 6
+        for k in range(3):
+            t = 0.1 + 0.02 * (k + 1)
+            ctx.rk4(xr, 0.02, 1, t0=t - 0.02)
+            cur, occ = ctx.observables(xr)
+            es = O.engy_shift(o, t, ft, tc, dc, toff) * heomq.HBAR
+            want_c = [t, es[0, 0], es[1, 0], cur[0].sum(), cur[1].sum(), cur.sum(), occ[:, 0].sum(),
+                      occ[:, 1].sum(), occ.sum()]
+            want_p = [t, es[0, 0], es[1, 0], occ[:, 0].sum(), occ[:, 1].sum(), occ.sum()]
+            for got, want in ((curr[k], want_c), (popu[k], want_p)):
+                assert abs(got[0] - want[0]) <= 1e-4
+                for g, w in zip(got[1:], want[1:]):
+                    # e15.6e3 keeps six significant digits: half a unit of the last one
+                    ulp = 10.0 ** (np.floor(np.log10(abs(w))) + 1 - 6) if w != 0 else 1e-300
+                    assert abs(g - w) <= 0.5 * ulp * (1 + 1e-9) + 1e-12 * abs(w)
+    finally:
+        ctx.set_field(-1)
