@@ -144,6 +144,7 @@ struct heomq_ctx {
   // pattern of H(t) is the union of hs's and the D_k's (sys_hnnz, sys_hcol,
   // d slots per row), sys_base holds hs and sys_terms the D_k / hbar on it,
   // sys_hval the values at the last evaluated t
+  int presmooth = 1;        // block-Jacobi sweeps before preconditioned steady solves (heomq_ctx_set_presmooth)
   int sys_nterm = 0;
   std::vector<int> sys_kind;
   std::vector<double> sys_par;
@@ -187,6 +188,10 @@ void build_gamma(const Model& m, const std::vector<cplx>& gdraw, std::vector<dou
 // the default shift of jacobi/prelude_jacobi.f90; run whenever H or the bath
 // coefficients change.
 int build_jacobi(heomq_ctx* c, const Model& m);
+int jacobi_presmooth(heomq_ctx* c, double* x, int nsweep, double* r, double* t, hipStream_t st);
+// block-Jacobi sweeps (default shift) applied to the start of a
+// block-Jacobi-preconditioned steady solve with b = e_trace
+constexpr int kPresmoothSweeps = 2;
 // default block-Jacobi preconditioner shift, as a multiple of the iteration's ejac
 constexpr double kPrecShiftFactor = 0.05;
 // The forward operator at time t (the context's own when no time-dependent

@@ -235,6 +235,17 @@ int residual(heomq_ctx* c, const double* x, double* r, const double* zero, hipSt
 
 namespace heomq {
 
+// nsweep block-Jacobi sweeps of x (packed) with the reference's default
+// shift: the convergent iteration of solve_jacobi.f90, used to smooth a
+// start before a Krylov solve.  r, t: packed scratch.
+int jacobi_presmooth(heomq_ctx* c, double* x, int nsweep, double* r, double* t, hipStream_t st) {
+  DevJacobi jc = c->pjac;
+  jc.ejac = c->ejac_default;
+  for (int k = 0; k < nsweep; ++k)
+    if (int rc = sweep(c, jc, x, r, t, st)) return rc;
+  return HEOMQ_OK;
+}
+
 int build_jacobi(heomq_ctx* c, const Model& m) {
   JacobiHost jh;
   if (int rc = build_jacobi_host(m, jh)) return rc;

@@ -357,8 +357,19 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
     W.pt = v[kWorkVecs + 2];
     W.bp = v[kWorkVecs + 3];
   }
+  // The steady problem (b = e_trace): a few sweeps of the reference's own
+  // convergent block-Jacobi iteration (trace kept) before TFQMR.  They damp
+  // the start's tier-1+ residual (3e3 x ||b|| on the two-impurity model), so
+  // the recurrences lose less accuracy: 16 shadow seeds took 150 iterations
+  // on average instead of 225, the slowest 191 instead of 825.
+  int pre = 0;
+  if (prec == 2 && !b && !transpose && c->presmooth) {
+    if (int rc = jacobi_presmooth(c, xs, kPresmoothSweeps, W.ax, W.pt, st)) return rc;
+    pre = kPresmoothSweeps;
+  }
   const int rc = tfqmr_run<double>(c, xs, bb, W, n, transpose != 0, tpos_of(c, trace), tol, maxiter, st, rs,
                                    packed ? 1 : 0, prec);
+  rs.applies += pre;
   if (packed) HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
@@ -556,6 +567,12 @@ int heomq_ctx_set_precond_shift(heomq_ctx* c, double shift) {
 int heomq_ctx_set_shadow_seed(heomq_ctx* c, uint64_t seed) {
   if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
   c->shadow_seed = seed;
+  return HEOMQ_OK;
+}
+
+int heomq_ctx_set_presmooth(heomq_ctx* c, int enable) {
+  if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
+  c->presmooth = enable ? 1 : 0;
   return HEOMQ_OK;
 }
 

@@ -337,6 +337,11 @@ class Context:
         """Seed of TFQMR's +-1 shadow vector at each solve's first start."""
         check(lib().heomq_ctx_set_shadow_seed(self._h, int(seed)))
 
+    def set_presmooth(self, enable: bool):
+        """Two block-Jacobi sweeps on the start of block-Jacobi-preconditioned
+        steady solves (default on)."""
+        check(lib().heomq_ctx_set_presmooth(self._h, int(enable)))
+
     def set_graphs(self, enable: bool):
         """Replay solver iterations from captured graphs (default) or launch eagerly."""
         check(lib().heomq_ctx_set_graphs(self._h, int(enable)))

@@ -108,6 +108,7 @@ SIGNATURES = {
     "heomq_calcderiv_tt": (C.c_int, [P, C.c_double, D, D]),
     "heomq_ctx_set_system_field": (C.c_int, [P, C.c_int, D, I32, D]),
     "heomq_ctx_field_shifts": (C.c_int, [P, C.c_double, D]),
+    "heomq_ctx_set_presmooth": (C.c_int, [P, C.c_int]),
     "heomq_chebyshev": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_observables": (C.c_int, [P, P, D, D, P]),
 }

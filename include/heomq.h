@@ -146,6 +146,12 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* m);
  * eight TFQMR iterations; solves requested on the null stream run on a
  * context-owned stream).  enable = 0 launches the same kernels one by one. */
 int heomq_ctx_set_graphs(heomq_ctx* c, int enable);
+/* Block-Jacobi-preconditioned steady solves with b = e_trace start with two
+ * sweeps of the reference's block-Jacobi iteration (jacobi/solve_jacobi.f90,
+ * its default shift; the trace is kept) applied to the initial guess
+ * (enable = 1, the default); 0 starts TFQMR from the guess as given.  No
+ * counterpart in the reference. */
+int heomq_ctx_set_presmooth(heomq_ctx* c, int enable);
 /* Left preconditioning of the forward steady-state TFQMR on the packed
  * layout: 0 none; 1 Jacobi (diagonal) scaling, default right-hand side only;
  * 2 block Jacobi, (shift - S_K)^-1 per ADO with S_K the same-tier operator
