@@ -124,6 +124,34 @@ def test_block_jacobi_preconditioned_tfqmr(name, shift):
 
 
 @pytest.mark.parametrize("name", list(CASES))
+def test_smoothed_start_reaches_the_same_state(name):
+    """The block-Jacobi sweeps on the start (default on) change the path, not
+    the steady state: with and without them the oracle's state within 1e-10,
+    and the sweeps keep the trace of rho0."""
+    m, ctx, o, torch = setup(name)
+    ref, _, _ = O.steady_state(o, tol=1e-13)
+    out = {}
+    try:
+        ctx.set_precond(2)
+        for pre in (True, False):
+            ctx.set_presmooth(pre)
+            x, info = ctx.solve_steady(tol=1e-12)
+            print(f"case {name} presmooth {pre}", info, flush=True)
+            assert info["relres"] <= 1e-12
+            out[pre] = from_dev(x)
+    finally:
+        ctx.set_presmooth(True)
+        ctx.set_precond(-1, 0.0)
+    for xs in out.values():
+        assert np.abs(xs[0] - ref[0]).max() <= 1e-10
+        assert abs(np.trace(xs[0]) - 1.0) <= 1e-12
+    # two sweeps of the reference's iteration on the trace-one start keep tr rho0 = 1
+    x0 = ctx.initial_guess()
+    y = ctx.jacobi_step(ctx.jacobi_step(x0))
+    assert abs(np.trace(from_dev(y)[0]) - 1.0) <= 1e-13
+
+
+@pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("form", [1, 2, 3])
 def test_preconditioned_apply_matches_oracle(name, form):
     """M^-1 (Pi A x) in every device form (the solver's one-kernel inverse on
