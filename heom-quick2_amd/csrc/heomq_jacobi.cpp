@@ -384,6 +384,9 @@ int build_jacobi(heomq_ctx* c, const Model& m) {
       c->pjac.blk = c->j_blk.as<int64_t>();
       c->pjac.desc = c->j_desc.as<JacDesc>();
       c->pjac.ado_blocks = nb;
+      bool real = true;
+      for (size_t e = 1; e < jh.u.size(); e += 2) real = real && jh.u[e] == 0.0;
+      c->pjac.u_real = real ? 1 : 0;
     } else {
       c->pjac.ado_blocks = 0;
     }

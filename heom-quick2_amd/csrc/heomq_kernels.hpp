@@ -217,6 +217,7 @@ struct DevJacobi {
   // ADOs [blk[2b], blk[2b+2]), whose packed elements [blk[2b+1], blk[2b+3])
   // number at most 256; 0 blocks when the layout does not allow it
   int64_t ado_blocks = 0;
+  int u_real = 0;                   // 1: every U_ki is real (a real H), the kernel reads U as doubles
   const int64_t* blk = nullptr;     // [ado_blocks + 1][2]: first ADO, first packed element
   const JacDesc* desc = nullptr;    // [S][d*d]
 };

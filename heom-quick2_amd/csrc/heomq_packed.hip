@@ -684,12 +684,31 @@ __device__ inline double2 cmulc(double2 a, double2 b) {   // conj(a) b
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
+// U_ki times a complex value (conj: conj(U_ki) times it); real U reads one double
+template <bool RU>
+struct UStore;
+template <>
+struct UStore<false> {
+  double2 v[256];
+  __device__ void put(int k, double2 u) { v[k] = u; }
+  __device__ double2 mul(int k, double2 x) const { return cmul(v[k], x); }
+  __device__ double2 mulc(int k, double2 x) const { return cmulc(v[k], x); }
+};
+template <>
+struct UStore<true> {
+  double v[256];
+  __device__ void put(int k, double2 u) { v[k] = u.x; }
+  __device__ double2 mul(int k, double2 x) const { return make_double2(v[k] * x.x, v[k] * x.y); }
+  __device__ double2 mulc(int k, double2 x) const { return make_double2(v[k] * x.x, v[k] * x.y); }
+};
+
+template <bool RU>
 __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const DevJacobi jc,
                                                     const double* __restrict__ in, double* __restrict__ out,
                                                     int accumulate) {
   __shared__ double2 xw[256];
   __shared__ double2 tw[256];
-  __shared__ double2 us[256];        // U, u[k + i d] = U_ki (d <= 16)
+  __shared__ UStore<RU> us;          // U, u[k + i d] = U_ki (d <= 16)
   __shared__ double ew[16];
   __shared__ int32_t offs[257];      // ADO starts relative to the block's first element
   __shared__ double2 gamw[256];
@@ -709,7 +728,7 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
       secw[k] = op.sec[a_lo + k];
     }
   }
-  if (tid < DD) us[tid] = reinterpret_cast<const double2*>(jc.u)[tid];
+  if (tid < DD) us.put(tid, reinterpret_cast<const double2*>(jc.u)[tid]);
   if (tid < d) ew[tid] = jc.e[tid];
   __syncthreads();
   // the ADO of this element: the last ia with offs[ia] <= tid
@@ -733,7 +752,7 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
 #pragma unroll
     for (int q = 0; q < 4; ++q)
       if (q < ds.nq) {
-        const double2 p = cmul(us[ds.c2[q] + ds.c * d], xw[base + ds.cslot[q]]);
+        const double2 p = us.mul(ds.c2[q] + ds.c * d, xw[base + ds.cslot[q]]);
         acc.x += p.x;
         acc.y += p.y;
       }
@@ -746,7 +765,7 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
 #pragma unroll
     for (int p = 0; p < 4; ++p)
       if (p < ds.np) {
-        const double2 v = cmulc(us[ds.r2[p] + ds.r * d], tw[base + ds.rslot[p]]);
+        const double2 v = us.mulc(ds.r2[p] + ds.r * d, tw[base + ds.rslot[p]]);
         q.x += v.x;
         q.y += v.y;
       }
@@ -762,7 +781,7 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
 #pragma unroll
     for (int q = 0; q < 4; ++q)
       if (q < ds.nq) {
-        const double2 p = cmulc(us[ds.c + ds.c2[q] * d], xw[base + ds.cslot[q]]);
+        const double2 p = us.mulc(ds.c + ds.c2[q] * d, xw[base + ds.cslot[q]]);
         acc.x += p.x;
         acc.y += p.y;
       }
@@ -775,7 +794,7 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
 #pragma unroll
     for (int p = 0; p < 4; ++p)
       if (p < ds.np) {
-        const double2 v = cmul(us[ds.r + ds.r2[p] * d], tw[base + ds.rslot[p]]);
+        const double2 v = us.mul(ds.r + ds.r2[p] * d, tw[base + ds.rslot[p]]);
         z.x += v.x;
         z.y += v.y;
       }
@@ -898,7 +917,12 @@ hipError_t launch_jacobi_win(const DevPacked& op, const DevJacobi& jc, const dou
 hipError_t launch_jacobi_ado(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
                              hipStream_t s) {
   if (jc.ado_blocks <= 0 || in == out || op.nprob != 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_jacobi_ado, dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in, out, accumulate ? 1 : 0);
+  if (jc.u_real)
+    hipLaunchKernelGGL(k_jacobi_ado<true>, dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in, out,
+                       accumulate ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_jacobi_ado<false>, dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in, out,
+                       accumulate ? 1 : 0);
   return hipGetLastError();
 }
 
