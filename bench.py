@@ -41,7 +41,7 @@ def workload(name):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="two_impurity", choices=["two_impurity", "siam", "sweep"])
     ap.add_argument("--points", type=int, default=256, help="sweep: number of parameter points")
