@@ -54,7 +54,10 @@ def main():
     ap.add_argument("--no-ref", action="store_true",
                     help="skip the long reference solve (rdo_err is then not measured); for profiling runs")
     ap.add_argument("--apply-only", action="store_true",
-                    help="profiling: a 64 MiB calibration copy, then --apply-reps packed applies; no solve, no JSON")
+                    help="profiling: a 64 MiB calibration copy, then --apply-reps applies; no solve, no JSON")
+    ap.add_argument("--apply-kernel", default="packed", choices=["packed", "dense", "precond"],
+                    help="--apply-only: the packed apply, the dense drop-in apply (reference vector layout), "
+                         "or the packed apply followed by the block-Jacobi inverse")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the batched and single-hierarchy "
@@ -70,7 +73,7 @@ def main():
         return sweep(args)
     params, wname = workload(args.workload)
     if args.apply_only:
-        return apply_only(params, args.apply_reps)
+        return apply_only(params, args.apply_reps, args.apply_kernel)
     if args.tol is None:
         # the two-impurity TFQMR residual floors near 3e-8; 1e-7 already gives
         # an RDO error far below 1e-10 against the long reference solve below
@@ -290,10 +293,12 @@ def heomq_rdo(x):
     return x[0].cpu().numpy().T
 
 
-def apply_only(params, reps):
+def apply_only(params, reps, kernel="packed"):
     """Profiling driver: one device copy of 64 MiB (the counter calibration:
-    FETCH_SIZE / WRITE_SIZE of a kernel whose traffic is known), then packed
-    applies on a trace-one state."""
+    FETCH_SIZE / WRITE_SIZE of a kernel whose traffic is known), then applies
+    on a trace-one state: packed (the solver's operator), dense (the
+    calc_ax_tfqmr_omp drop-in on the reference's vector layout) or precond
+    (packed apply + block-Jacobi inverse, the preconditioned operator)."""
     import torch
     import heomq
     src = torch.ones(1 << 23, dtype=torch.float64, device="cuda:0")
@@ -302,12 +307,25 @@ def apply_only(params, reps):
     evict.fill_(0.0)   # 256 MiB written after src: src is no longer in L2 when it is copied
     dst.copy_(src)
     ctx = heomq.Context(heomq.Model(params))
-    xp = ctx.pack(ctx.initial_guess())
+    x0 = ctx.initial_guess()
+    if kernel == "dense":
+        y = ctx.zeros()
+        for _ in range(reps):
+            ctx.apply(x0, y, trace=heomq.TRACE_LAST)
+        torch.cuda.synchronize()
+        print(f"apply-only: {reps} dense applies, {x0.numel()} dense elements", flush=True)
+        return
+    xp = ctx.pack(x0)
     yp = ctx.packed_zeros()
-    for _ in range(reps):
-        ctx.apply_packed_raw(xp, yp, trace=heomq.TRACE_RDO)
+    if kernel == "precond":
+        ctx.set_precond(2)
+        for _ in range(reps):
+            ctx.apply_packed_precond(xp, yp, trace=heomq.TRACE_RDO, form=1)
+    else:
+        for _ in range(reps):
+            ctx.apply_packed_raw(xp, yp, trace=heomq.TRACE_RDO)
     torch.cuda.synchronize()
-    print(f"apply-only: {reps} packed applies, {ctx.packed_info()['packed']} packed elements", flush=True)
+    print(f"apply-only: {reps} {kernel} applies, {ctx.packed_info()['packed']} packed elements", flush=True)
 
 
 def cpu_baseline(params, wname, napply, applies_per_solve):
