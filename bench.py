@@ -134,7 +134,7 @@ def main():
     dense_bytes = 2 * 16 * d * d * N        # the same state in the dense d x d layout
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
-                "kernel": f"k_apply_packed<{d},double,false>", "t_apply_us": round(t_apply * 1e6, 2),
+                "kernel": "k_apply_packed<double,false,2>", "t_apply_us": round(t_apply * 1e6, 2),
                 "bytes_alg": bytes_alg, "packed_elems": P, "dense_elems": info["dense"],
                 "dense_equiv_GBs": round(dense_bytes / t_apply / 1e9, 1)}
 
