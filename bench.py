@@ -59,10 +59,12 @@ def main():
                     help="skip the long reference solve (rdo_err is then not measured); for profiling runs")
     ap.add_argument("--apply-only", action="store_true",
                     help="profiling: a 64 MiB calibration copy, then --apply-reps applies; no solve, no JSON")
-    ap.add_argument("--apply-kernel", default="packed", choices=["packed", "dense", "precond", "f32", "transpose"],
+    ap.add_argument("--apply-kernel", default="packed",
+                    choices=["packed", "dense", "precond", "f32", "transpose", "spa", "rk4", "chebyshev", "bicg"],
                     help="--apply-only: the packed apply, the dense drop-in apply (reference vector layout), "
                          "the packed apply followed by the block-Jacobi inverse, the FP32 packed apply, "
-                         "or the transposed packed apply")
+                         "the transposed packed apply, the sparse (COO) apply on the full pattern, "
+                         "--apply-reps RK4 or Chebyshev steps, or one BiCG steady-state solve")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the batched and single-hierarchy "
@@ -304,8 +306,11 @@ def apply_only(params, reps, kernel="packed"):
     on a trace-one state: packed (the solver's operator), dense (the
     calc_ax_tfqmr_omp drop-in on the reference's vector layout) or precond
     (packed apply + block-Jacobi inverse, the preconditioned operator), f32
-    (the FP32 packed apply of mixed-precision solves) or transpose (the
-    transposed packed apply of adjoint solves)."""
+    (the FP32 packed apply of mixed-precision solves), transpose (the
+    transposed packed apply of adjoint solves), spa (calc_ax_tfqmr_spa_omp on
+    the full COO pattern), rk4 / chebyshev (reps propagation steps of
+    dt = 0.02) or bicg (one solvebicg steady-state solve to 1e-10)."""
+    import numpy as np
     import torch
     import heomq
     src = torch.ones(1 << 23, dtype=torch.float64, device="cuda:0")
@@ -315,6 +320,27 @@ def apply_only(params, reps, kernel="packed"):
     dst.copy_(src)
     ctx = heomq.Context(heomq.Model(params))
     x0 = ctx.initial_guess()
+    if kernel in ("rk4", "chebyshev", "bicg", "spa"):
+        if kernel == "rk4":
+            ctx.rk4(x0, 0.02, reps)
+        elif kernel == "chebyshev":
+            ctx.chebyshev(x0, 0.02, reps)
+        elif kernel == "bicg":
+            _, info = ctx.solve_bicg(x0, crit=1e-10, maxiter=20000, raise_on_fail=False)
+            print(f"apply-only: bicg {info['iters']} iterations, resid {info['resid']:.3e}", flush=True)
+        else:
+            dd = ctx.model.d * ctx.model.d
+            N = ctx.model.N
+            e = np.arange(dd, dtype=np.int32)
+            ctx.set_spa_pattern(np.full(N, dd, np.int32), 1 + dd * np.arange(N, dtype=np.int64),
+                                np.tile(1 + e % ctx.model.d, N), np.tile(1 + e // ctx.model.d, N))
+            xs = x0.reshape(-1).clone()
+            ys = torch.zeros(xs.numel() + 1, dtype=torch.complex128, device=xs.device)
+            for _ in range(reps):
+                ctx.apply_spa(xs, ys)
+        torch.cuda.synchronize()
+        print(f"apply-only: {kernel}, {reps} reps, N = {ctx.model.N}", flush=True)
+        return
     if kernel == "dense":
         y = ctx.zeros()
         for _ in range(reps):

@@ -82,10 +82,24 @@ k_apply(const DevOperator op, const T* __restrict__ x, T* __restrict__ y, int64_
     ai[k] = gr * xe.y + gi * xe.x + hs * cr;
   }
 
+  // couplings: the entry two ahead and the record one ahead are loaded while
+  // the current neighbour is gathered
   const int64_t p0 = op.rowptr[a], p1 = op.rowptr[a + 1];
+  DevEntry en_next{}, en_after{};
+  DevRec rc_next{};
+  if (p0 < p1) {
+    en_next = op.ent[p0];
+    rc_next = op.rec[en_next.rec];
+  }
+  if (p0 + 1 < p1) en_after = op.ent[p0 + 1];
   for (int64_t p = p0; p < p1; ++p) {
-    const DevEntry en = op.ent[p];
-    const DevRec rc = op.rec[en.rec];
+    const DevEntry en = en_next;
+    const DevRec rc = rc_next;
+    if (p + 1 < p1) {
+      en_next = en_after;
+      rc_next = op.rec[en_next.rec];
+      if (p + 2 < p1) en_after = op.ent[p + 2];
+    }
     const V* __restrict__ xn = xv + (int64_t)en.nbr * E;
     const bool dag = en.flags & 1;
     const T cLr = (en.flags & 2) ? -(T)rc.cLr : (T)rc.cLr;
