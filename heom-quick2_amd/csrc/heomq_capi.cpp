@@ -672,8 +672,8 @@ int heomq_model_set_params(heomq_model* h, const heomq_params* p) {
 
 int heomq_model_from_reference(
     int32_t nrho, int64_t nunk, int32_t ntier, int32_t norbs, int32_t nspin, int32_t nalf,
-    int32_t ncor, int32_t igroundsteady, const int32_t* indextable, const int64_t* nfirst,
-    const int64_t* nlast, const int64_t* ifirst, const int64_t* indexcoef, const int8_t* itypecoef,
+    int32_t ncor, int32_t igroundsteady, int64_t ntable, int64_t lall, const int32_t* indextable,
+    const int64_t* nfirst, const int64_t* nlast, const int64_t* ifirst, const int64_t* indexcoef, const int8_t* itypecoef,
     const int8_t* ifactfcoef, const int8_t* ifactrcoef, const int64_t* index_coef_ref,
     const int32_t* indexdraw, const double* cb, const double* cd, const double* cgamma,
     const double* dipm, const int32_t* ilead, const int32_t* mspin, const int32_t* morbs,
@@ -681,8 +681,8 @@ int heomq_model_from_reference(
     heomq_model** out) {
   if (!out) return fail(HEOMQ_BAD_ARG, "null output");
   auto* h = new heomq_model;
-  const int rc = model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady,
-                                      indextable, nfirst, nlast, ifirst, indexcoef, itypecoef,
+  const int rc = model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady, ntable,
+                                      lall, indextable, nfirst, nlast, ifirst, indexcoef, itypecoef,
                                       ifactfcoef, ifactrcoef, index_coef_ref, indexdraw, cb, cd, cgamma,
                                       dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, h->m);
   if (rc) { delete h; return rc; }

@@ -393,11 +393,23 @@ int build_jacobi(heomq_ctx* c, const Model& m) {
   }
   c->pjac.ejac = jh.ejac;
   c->ejac_default = jh.ejac;
-  c->prec_shift = c->prec_shift_user > 0.0 ? c->prec_shift_user : kPrecShiftFactor * jh.ejac;
   c->ejac_gap = jh.gap;
   c->ejac_cb = jh.cbterm;
   c->jac_nnz = jh.nnz;
   build_gamma(m, m.gdraw, c->gam_h, false);
+  // The preconditioner's denominators are shift - gamma_K + i dE: re-check
+  // the shift against the rates just rebuilt.  A user shift that the new
+  // parameters make singular falls back to the default; a singular default
+  // is refused, as the reference refuses its 1/(i lambda) branch.
+  auto singular = [&](double v) {
+    for (size_t a = 0; 2 * a + 1 < c->gam_h.size(); ++a)
+      if (std::abs(cplx(v - c->gam_h[2 * a], -c->gam_h[2 * a + 1])) <= 1e-12) return true;
+    return false;
+  };
+  if (c->prec_shift_user > 0.0 && singular(c->prec_shift_user)) c->prec_shift_user = 0.0;
+  c->prec_shift = c->prec_shift_user > 0.0 ? c->prec_shift_user : kPrecShiftFactor * jh.ejac;
+  if (singular(c->prec_shift))
+    return fail(HEOMQ_UNSUPPORTED, "block-Jacobi preconditioner: the default shift equals an ADO's gamma_K");
   return HEOMQ_OK;
 }
 
@@ -506,6 +518,7 @@ int heomq_solve_jacobi(heomq_ctx* c, double* x, double crit, int maxiter, double
   HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
+    std::fill(stats, stats + HEOMQ_STATS_LEN, 0.0);
     stats[0] = iter;
     stats[1] = applies;
     stats[2] = res;

@@ -235,6 +235,10 @@ hipError_t launch_jacobi_win(const DevPacked& op, const DevJacobi& jc, const dou
 // The same with one block per run of whole ADOs (jc.ado_blocks > 0); in and out must differ.
 hipError_t launch_jacobi_ado(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
                              hipStream_t s);
+// The ADO-aligned form on FP32 vectors (the transform in FP64): the
+// preconditioner of the FP32 inner solves of mixed precision.
+hipError_t launch_jacobi_ado_f32(const DevPacked& op, const DevJacobi& jc, const float* in, float* out,
+                                 hipStream_t s);
 // The one-kernel form the layout allows (ADO-aligned, else the window form).
 inline bool jacobi_one_kernel(const DevJacobi& jc) { return jc.ado_blocks > 0 || jc.windowed; }
 hipError_t launch_jacobi_1k(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,

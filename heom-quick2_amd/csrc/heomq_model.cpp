@@ -489,7 +489,8 @@ int set_params(Model& m, const heomq_params& p) {
 // ---------------------------------------------------------------------------
 int model_from_reference(
     int32_t nrho, int64_t nunk, int32_t ntier, int32_t norbs, int32_t nspin, int32_t nalf,
-    int32_t ncor, int32_t igroundsteady, const int32_t* indextable, const int64_t* nfirst,
+    int32_t ncor, int32_t igroundsteady, int64_t ntable, int64_t lall, const int32_t* indextable,
+    const int64_t* nfirst,
     const int64_t* nlast, const int64_t* ifirst, const int64_t* indexcoef,
     const int8_t* itypecoef, const int8_t* ifactfcoef, const int8_t* ifactrcoef,
     const int64_t* index_coef_ref, const int32_t* indexdraw, const double* cb,
@@ -500,6 +501,7 @@ int model_from_reference(
   int dd = 1;
   for (int t = 0; t < norbs; ++t) dd *= 4;
   if (nrho != dd) return fail(HEOMQ_BAD_DIM, "nrho != (2*nspin)**norbs");
+  if (ntable < 1 || lall < 0) return fail(HEOMQ_BAD_ARG, "ntable < 1 or lall < 0");
   m = Model{};
   m.norbs = norbs; m.nspin = nspin; m.nalf = nalf; m.ncor = ncor;
   m.nopr = 2 * norbs * nspin * nalf;
@@ -554,6 +556,9 @@ int model_from_reference(
     if (cnt < (n == 0 ? 1 : 0) || m.first[n] != (n == 0 ? 0 : nlast[it - 2]) || nlast[it - 1] > nunk)
       return fail(HEOMQ_BAD_ARG, "nfirst/nlast do not tile 1..nunk tier by tier");
     if (n > 0) {
+      // tier n's records occupy indextable(ifirst(it) .. ifirst(it) + cnt*n - 1)
+      if (ifirst[it - 1] < 1 || ifirst[it - 1] - 1 + cnt * n > ntable)
+        return fail(HEOMQ_BAD_ARG, "ifirst: tier " + std::to_string(n) + " records run past indextable(ntable)");
       m.tiers[n].resize((size_t)cnt * n);
       for (int64_t a = 0; a < cnt; ++a)
         for (int r = 0; r < n; ++r) {
@@ -590,6 +595,10 @@ int model_from_reference(
       m.lower_off[g] = (int64_t)m.lower.size();
       int64_t lnl = index_coef_ref[g] - 1;   // 0-based position into the coupling stream
       if (lnl < 0) return fail(HEOMQ_BAD_ARG, "index_coef_ref < 1");
+      // n lower entries, then M upper ones below the top tier
+      if (lnl + n + (n < m.L ? M : 0) > lall)
+        return fail(HEOMQ_BAD_ARG, "index_coef_ref: ADO " + std::to_string(g + 1) +
+                                       "'s couplings run past indexcoef(lall)");
       for (int r = 0; r < n; ++r, ++lnl) {
         const int32_t nb = (int32_t)(indexcoef[lnl] - 1);
         if (bad_entry(nb, itypecoef[lnl], false)) return fail(HEOMQ_BAD_ARG, "indexcoef: bad lower-tier entry");

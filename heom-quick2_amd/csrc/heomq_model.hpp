@@ -62,7 +62,8 @@ int build_model(const heomq_params& p, Model& m);
 int set_params(Model& m, const heomq_params& p);   // coefficients only
 int model_from_reference(
     int32_t nrho, int64_t nunk, int32_t ntier, int32_t norbs, int32_t nspin,
-    int32_t nalf, int32_t ncor, int32_t igroundsteady, const int32_t* indextable,
+    int32_t nalf, int32_t ncor, int32_t igroundsteady, int64_t ntable, int64_t lall,
+    const int32_t* indextable,
     const int64_t* nfirst, const int64_t* nlast, const int64_t* ifirst,
     const int64_t* indexcoef, const int8_t* itypecoef, const int8_t* ifactfcoef,
     const int8_t* ifactrcoef, const int64_t* index_coef_ref, const int32_t* indexdraw,

@@ -702,10 +702,13 @@ struct UStore<true> {
   __device__ double2 mulc(int k, double2 x) const { return make_double2(v[k] * x.x, v[k] * x.y); }
 };
 
-template <bool RU>
+// TIO: element type of in / out (double, or float for the FP32 inner solves of
+// mixed precision); the transform itself runs in FP64 either way.
+template <bool RU, typename TIO>
 __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const DevJacobi jc,
-                                                    const double* __restrict__ in, double* __restrict__ out,
+                                                    const TIO* __restrict__ in, TIO* __restrict__ out,
                                                     int accumulate) {
+  using VIO = typename P2<TIO>::type;
   __shared__ double2 xw[256];
   __shared__ double2 tw[256];
   __shared__ UStore<RU> us;          // U, u[k + i d] = U_ki (d <= 16)
@@ -717,10 +720,13 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
   const int64_t a_lo = jc.blk[2 * blockIdx.x], w0 = jc.blk[2 * blockIdx.x + 1];
   const int nado = (int)(jc.blk[2 * blockIdx.x + 2] - a_lo);   // 1..256, checked on the host
   const int nw = (int)(jc.blk[2 * blockIdx.x + 3] - w0);       // <= 256
-  const double2* xin = reinterpret_cast<const double2*>(in);
+  const VIO* xin = reinterpret_cast<const VIO*>(in);
   const bool act = tid < nw;
   double2 xv = make_double2(0.0, 0.0);
-  if (act) xv = xin[w0 + tid];
+  if (act) {
+    const VIO v = xin[w0 + tid];
+    xv = make_double2((double)v.x, (double)v.y);
+  }
   for (int k = tid; k <= nado; k += 256) {
     offs[k] = (int32_t)(op.off[a_lo + k] - w0);
     if (k < nado) {
@@ -810,13 +816,16 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
     }
   }
   if (!act) return;
-  double2* xo = reinterpret_cast<double2*>(out);
+  VIO* xo = reinterpret_cast<VIO*>(out);
   if (accumulate) {
-    const double2 w = xo[w0 + tid];
-    z.x += w.x;
-    z.y += w.y;
+    const VIO w = xo[w0 + tid];
+    z.x += (double)w.x;
+    z.y += (double)w.y;
   }
-  xo[w0 + tid] = z;
+  VIO o;
+  o.x = (TIO)z.x;
+  o.y = (TIO)z.y;
+  xo[w0 + tid] = o;
 }
 
 template <typename TP, typename TD>
@@ -918,11 +927,23 @@ hipError_t launch_jacobi_ado(const DevPacked& op, const DevJacobi& jc, const dou
                              hipStream_t s) {
   if (jc.ado_blocks <= 0 || in == out || op.nprob != 1) return hipErrorInvalidValue;
   if (jc.u_real)
-    hipLaunchKernelGGL(k_jacobi_ado<true>, dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in, out,
+    hipLaunchKernelGGL((k_jacobi_ado<true, double>), dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in, out,
                        accumulate ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_jacobi_ado<false>, dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in, out,
-                       accumulate ? 1 : 0);
+    hipLaunchKernelGGL((k_jacobi_ado<false, double>), dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in,
+                       out, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_ado_f32(const DevPacked& op, const DevJacobi& jc, const float* in, float* out,
+                                 hipStream_t s) {
+  if (jc.ado_blocks <= 0 || in == out || op.nprob != 1) return hipErrorInvalidValue;
+  if (jc.u_real)
+    hipLaunchKernelGGL((k_jacobi_ado<true, float>), dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in, out,
+                       0);
+  else
+    hipLaunchKernelGGL((k_jacobi_ado<false, float>), dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in,
+                       out, 0);
   return hipGetLastError();
 }
 

@@ -46,10 +46,29 @@ hipError_t apply_packed_T<double>(heomq_ctx* c, bool trans, const double* x, dou
 }
 template <>
 hipError_t apply_packed_T<float>(heomq_ctx* c, bool trans, const float* x, float* y, bool rdo, hipStream_t s,
-                                 int prec, float*) {
+                                 int prec, float* scratch) {
   DevPacked op = trans ? c->ptrn : c->pfwd;
   op.precond = prec == 1 ? 1 : 0;
-  return launch_apply_packed_f32(op, x, y, rdo, s);
+  if (prec != 2) return launch_apply_packed_f32(op, x, y, rdo, s);
+  DevJacobi jc = c->pjac;   // FP32 vectors: the ADO-aligned inverse only (checked by the caller)
+  jc.ejac = c->prec_shift;
+  if (hipError_t e = launch_apply_packed_f32(op, x, scratch, rdo, s)) return e;
+  return launch_jacobi_ado_f32(op, jc, scratch, y, s);
+}
+
+// y = M^-1 b with the block-Jacobi preconditioner (the right-hand side of a
+// left-preconditioned solve); t is scratch for the two-kernel form.
+hipError_t precond_rhs(heomq_ctx* c, const double* b, double* y, double* t, hipStream_t st) {
+  DevJacobi jc = c->pjac;
+  jc.ejac = c->prec_shift;
+  if (jacobi_one_kernel(jc)) return launch_jacobi_1k(c->pfwd, jc, b, y, false, st);
+  if (hipError_t e = launch_jacobi_fwd(c->pfwd, jc, b, t, st)) return e;
+  return launch_jacobi_back(c->pfwd, jc, t, y, false, st);
+}
+hipError_t precond_rhs(heomq_ctx* c, const float* b, float* y, float*, hipStream_t st) {
+  DevJacobi jc = c->pjac;
+  jc.ejac = c->prec_shift;
+  return launch_jacobi_ado_f32(c->pfwd, jc, b, y, st);
 }
 // Solvers iterate with the projected operator Pi*A (Pi: parity sectors +
 // Hermitian tier 0).  A maps range(Pi) into itself and the physical steady
@@ -169,18 +188,10 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
                      : apply_T<T>(c, trans, in, out, tpos, st, false);
   };
   if (prec == 2) {
-    if (mode != 1 || trans || !W.pt || !W.bp || sizeof(T) != sizeof(double))
-      return fail(HEOMQ_BAD_ARG, "block-Jacobi preconditioning needs the forward FP64 packed operator");
-    DevJacobi jc = c->pjac;
-    jc.ejac = c->prec_shift;
-    if (jacobi_one_kernel(jc)) {
-      HIPCHECK(launch_jacobi_1k(c->pfwd, jc, reinterpret_cast<const double*>(b), reinterpret_cast<double*>(W.bp),
-                                false, st));
-    } else {
-      HIPCHECK(launch_jacobi_fwd(c->pfwd, jc, reinterpret_cast<const double*>(b), reinterpret_cast<double*>(W.pt), st));
-      HIPCHECK(launch_jacobi_back(c->pfwd, jc, reinterpret_cast<const double*>(W.pt),
-                                  reinterpret_cast<double*>(W.bp), false, st));
-    }
+    if (mode != 1 || trans || !W.pt || !W.bp || (sizeof(T) != sizeof(double) && c->pjac.ado_blocks <= 0))
+      return fail(HEOMQ_BAD_ARG, "block-Jacobi preconditioning needs the forward packed operator (FP32: the "
+                                 "ADO-aligned inverse)");
+    HIPCHECK(precond_rhs(c, b, W.bp, W.pt, st));
   }
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   double* part = c->partial.as<double>();
@@ -373,6 +384,7 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
   if (packed) HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
+    std::fill(stats, stats + HEOMQ_STATS_LEN, 0.0);
     stats[0] = rs.iters;
     stats[1] = rs.applies;
     stats[2] = rs.relres;
@@ -393,13 +405,26 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   }
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t n = 2 * c->ptotal;
+  // Preconditioning as the FP64 solver chooses it (automatic: Jacobi scaling
+  // for diagonal H, block Jacobi otherwise).  The FP32 inner solves take the
+  // block-Jacobi inverse (ADO-aligned form, FP32 vectors, FP64 transform); the
+  // Jacobi scaling keeps only b = e_trace, so residual corrections of a
+  // diagonal-H model iterate unscaled.
+  const int want = c->precond >= 0 ? c->precond : (c->h_diagonal ? 1 : 2);
+  const int prec32 = (want == 2 && c->pjac.ado_blocks > 0) ? 2 : 0;
+  const int prec64 = want == 2 ? 2 : 0;
   std::vector<double*> v64;
   if (int rc = carve<double>(c->ws64, n, 3, v64)) return rc;
   std::vector<float*> v32;
-  if (int rc = carve<float>(c->ws32, n, kWorkVecs + 2, v32)) return rc;
+  if (int rc = carve<float>(c->ws32, n, kWorkVecs + 4, v32)) return rc;
+  if (stats) std::fill(stats, stats + HEOMQ_STATS_MIXED_LEN, 0.0);
   Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6], v32[7], v32[8]};
   float* x32 = v32[kWorkVecs];
   float* r32 = v32[kWorkVecs + 1];
+  if (prec32 == 2) {
+    W.pt = v32[kWorkVecs + 2];
+    W.bp = v32[kWorkVecs + 3];
+  }
   double* ax = v64[0];
   double* bb = v64[1];
   double* xs = v64[2];
@@ -412,6 +437,10 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   }
   double* red = c->red.as<double>();
   double* part = c->partial.as<double>();
+  // an inner solve stops at inner_tol or after this many FP32 iterations: the
+  // preconditioned recurrences need ~100 per five digits on the two-impurity
+  // model, the unpreconditioned ones stall earlier and are cut at 64
+  const int inner_cap = std::getenv("HEOMQ_INNER_MAXIT") ? inner_maxit() : (prec32 == 2 ? 400 : 64);
   int outer = 0, inner_total = 0, applies = 0, fp64_iters = 0;
   double relres = 0, prev = 0;
   int rc = HEOMQ_NOT_CONVERGED;
@@ -425,7 +454,7 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
     HIPCHECK(hipMemcpyAsync(hr, red, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     relres = std::sqrt(hr[0] / hr[1]);
-    if (stats && outer < 40) stats[8 + outer] = relres;   // refinement history
+    if (stats && 8 + outer < 48) stats[8 + outer] = relres;   // refinement history, slots 8..47
     if (hr[0] <= tol * tol * hr[1]) { rc = HEOMQ_OK; break; }
     if (inner_total >= maxiter) break;
     // FP32 corrections stop contracting once eps32 * ||A|| ||dx|| reaches the
@@ -436,23 +465,24 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
     HIPCHECK(vec_resid_to_f32(st, n, bb, ax, r32, 1.0 / rn));
     HIPCHECK(vec_fill<float>(st, n, x32, 0.0f));
     RunStats rs;
-    // Each FP32 inner solve is capped: FP32 TFQMR recurrences reach an
-    // attainable-accuracy floor, and the FP64 residual of the next refinement
-    // step is the better restart.
-    const int cap = std::min(inner_maxit(), maxiter - inner_total);
-    const int irc = tfqmr_run<float>(c, x32, r32, W, n, false, 0, inner_tol, cap, st, rs, 1);
+    const int cap = std::min(inner_cap, maxiter - inner_total);
+    const int irc = tfqmr_run<float>(c, x32, r32, W, n, false, 0, inner_tol, cap, st, rs, 1, prec32);
     if (irc != HEOMQ_OK && irc != HEOMQ_NOT_CONVERGED) return irc;
     inner_total += rs.iters;
     applies += rs.applies;
-    if (stats && outer < 16) stats[48 + outer] = rs.iters;
+    if (stats && 48 + outer < HEOMQ_STATS_MIXED_LEN) stats[48 + outer] = rs.iters;   // slots 48..63
     HIPCHECK(vec_axpy_f32_to_f64(st, n, xs, x32, rn));   // FP64 update
   }
   if (finish64) {
     std::vector<double*> w64;
-    if (int e = carve<double>(c->ws64b, n, kWorkVecs, w64)) return e;
+    if (int e = carve<double>(c->ws64b, n, kWorkVecs + 2, w64)) return e;
     Work<double> W64{w64[0], w64[1], w64[2], w64[3], w64[4], w64[5], w64[6], w64[7], w64[8]};
+    if (prec64 == 2) {
+      W64.pt = w64[kWorkVecs];
+      W64.bp = w64[kWorkVecs + 1];
+    }
     RunStats rs;
-    rc = tfqmr_run<double>(c, xs, bb, W64, n, false, 0, tol, std::max(maxiter - inner_total, 1), st, rs, 1);
+    rc = tfqmr_run<double>(c, xs, bb, W64, n, false, 0, tol, std::max(maxiter - inner_total, 1), st, rs, 1, prec64);
     fp64_iters = rs.iters;
     applies += rs.applies;
     relres = rs.relres;
@@ -523,6 +553,7 @@ int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* 
   HIPCHECK(launch_herm_from_lower_f64(c->d, x, st));
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
+    std::fill(stats, stats + HEOMQ_STATS_LEN, 0.0);
     stats[0] = iters;
     stats[1] = applies;
     stats[2] = std::sqrt(hs.rr);

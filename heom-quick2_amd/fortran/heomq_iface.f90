@@ -45,12 +45,12 @@ module heomq_iface
     end function
 
     integer(c_int) function c_model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igs, &
-        indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
+        ntable, lall, indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
         indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, out) &
         bind(C, name='heomq_model_from_reference')
       import :: c_int, c_int32_t, c_int64_t, c_int8_t, c_double, c_double_complex, c_ptr
       integer(c_int32_t), value :: nrho, ntier, norbs, nspin, nalf, ncor, igs
-      integer(c_int64_t), value :: nunk
+      integer(c_int64_t), value :: nunk, ntable, lall
       integer(c_int32_t) :: indextable(*), indexdraw(*), ilead(*), mspin(*), morbs(*), mpm(*)
       integer(c_int64_t) :: nfirst(*), nlast(*), ifirst(*), indexcoef(*), index_coef_ref(*)
       integer(c_int8_t) :: itypecoef(*), ifactfcoef(*), ifactrcoef(*)
@@ -196,12 +196,12 @@ end module heomq_iface
 
 ! One-time upload of the reference's module arrays (module/matmod.f90).
 subroutine heomq_fortran_init(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady, &
-    indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
+    ntable, lall, indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
     indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, device)
   use heomq_iface
   implicit none
   integer(c_int32_t), intent(in) :: nrho, ntier, norbs, nspin, nalf, ncor, igroundsteady, device
-  integer(c_int64_t), intent(in) :: nunk
+  integer(c_int64_t), intent(in) :: nunk, ntable, lall
   integer(c_int32_t), intent(in) :: indextable(*), indexdraw(*), ilead(*), mspin(*), morbs(*), mpm(*)
   integer(c_int64_t), intent(in) :: nfirst(*), nlast(*), ifirst(*), indexcoef(*), index_coef_ref(*)
   integer(c_int8_t), intent(in) :: itypecoef(*), ifactfcoef(*), ifactrcoef(*)
@@ -209,7 +209,7 @@ subroutine heomq_fortran_init(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igrou
   real(c_double), intent(in) :: dipm(*), eleadinfty(*), amsall(*)
   integer(c_int) :: rc
   rc = c_model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady, &
-       indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
+       ntable, lall, indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
        indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, heomq_model_h)
   if (rc /= 0) call heomq_fail('heomq_fortran_init (tables)', rc)
   rc = c_ctx_create(heomq_model_h, int(device, c_int), heomq_ctx_h)

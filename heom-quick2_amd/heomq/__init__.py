@@ -193,7 +193,11 @@ class Model:
 
     @classmethod
     def from_reference(cls, **arrays) -> "Model":
-        """Build from the reference's own module arrays (see include/heomq.h)."""
+        """Build from the reference's own module arrays (see include/heomq.h).
+
+        ``ntable`` and ``lall`` (the lengths the reference keeps in its common
+        block info3) default to the lengths of ``indextable`` and ``indexcoef``.
+        """
         a = arrays
         h = C.c_void_p()
         keep = []
@@ -206,6 +210,7 @@ class Model:
         check(lib().heomq_model_from_reference(
             int(a["nrho"]), int(a["nunk"]), int(a["ntier"]), int(a["norbs"]), int(a["nspin"]),
             int(a["nalf"]), int(a["ncor"]), int(a["igroundsteady"]),
+            int(a.get("ntable", np.size(a["indextable"]))), int(a.get("lall", np.size(a["indexcoef"]))),
             ptr("indextable", np.int32, _lib.I32), ptr("nfirst", np.int64, _lib.I64),
             ptr("nlast", np.int64, _lib.I64), ptr("ifirst", np.int64, _lib.I64),
             ptr("indexcoef", np.int64, _lib.I64), ptr("itypecoef", np.int8, _lib.I8),
@@ -460,7 +465,7 @@ class Context:
         """Steady state by BiCG (solvebicg.f90 convention: absolute ||r|| <= crit)."""
         t = self.torch
         x = self.initial_guess() if x is None else x
-        stats = np.zeros(8)
+        stats = np.zeros(_lib.STATS_LEN)
         rc = lib().heomq_solve_bicg(self._h, self._check_vec(x, t.complex128), crit, maxiter, _dp(stats),
                                     self._stream())
         info = dict(iters=int(stats[0]), applies=int(stats[1]), resid=stats[2], seconds=stats[3], status=rc)
@@ -525,7 +530,7 @@ class Context:
         trace-row residual <= crit, checked every 50 sweeps)."""
         t = self.torch
         x = self.initial_guess() if x is None else x
-        stats = np.zeros(8)
+        stats = np.zeros(_lib.STATS_LEN)
         rc = lib().heomq_solve_jacobi(self._h, self._check_vec(x, t.complex128), crit, maxiter, float(ejac),
                                       _dp(stats), self._stream())
         info = dict(iters=int(stats[0]), applies=int(stats[1]), resid=stats[2], seconds=stats[3], ejac=stats[4],
@@ -545,7 +550,7 @@ class Context:
         """TFQMR steady state on the device; returns (x, stats dict)."""
         t = self.torch
         x = self.initial_guess() if x is None else x
-        stats = np.zeros(8)
+        stats = np.zeros(_lib.STATS_LEN)
         bp = self._check_vec(b, t.complex128) if b is not None else None
         rc = lib().heomq_solve_steady(self._h, self._check_vec(x, t.complex128), bp, tol, maxiter, trace,
                                       int(transpose), _dp(stats), self._stream())
@@ -559,7 +564,7 @@ class Context:
         """FP32 TFQMR inner solves inside FP64 iterative refinement."""
         t = self.torch
         x = self.initial_guess() if x is None else x
-        stats = np.zeros(64)
+        stats = np.zeros(_lib.STATS_MIXED_LEN)
         bp = self._check_vec(b, t.complex128) if b is not None else None
         rc = lib().heomq_solve_steady_mixed(self._h, self._check_vec(x, t.complex128), bp, tol, inner_tol,
                                             maxiter, trace, _dp(stats), self._stream())
@@ -628,7 +633,7 @@ class Context:
         return y
 
     def chebyshev(self, x, dt, nstep):
-        stats = np.zeros(4)
+        stats = np.zeros(_lib.STATS_CHEBY_LEN)
         check(lib().heomq_chebyshev(self._h, self._check_vec(x, self.torch.complex128), dt, nstep, _dp(stats),
                                     self._stream()))
         return x, dict(ncheby=int(stats[0]), delta=stats[1])

@@ -14,6 +14,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libheomq.so"
 OK, BAD_DIM, UNSUPPORTED, HIP_ERROR, NOT_CONVERGED, BREAKDOWN, STOPPED, BAD_ARG = 0, 1, 2, 3, 4, 8, 16, 32
 TRACE_NONE, TRACE_LAST, TRACE_RDO = 0, 1, 2
 HAM_SIAM, HAM_DQD = 0, 1
+# doubles every solver writes into its stats array (heomq.h HEOMQ_STATS_*)
+STATS_LEN, STATS_MIXED_LEN, STATS_CHEBY_LEN = 8, 64, 2
 
 
 class HeomqError(RuntimeError):
@@ -52,7 +54,7 @@ SIGNATURES = {
     "heomq_model_packed_layout": (C.c_int, [P, I64]),
     "heomq_model_from_reference": (C.c_int, [
         C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-        I32, I64, I64, I64, I64, I8, I8, I8, I64, I32, D, D, D, D, I32, I32, I32, I32, D, D, D,
+        C.c_int64, C.c_int64, I32, I64, I64, I64, I64, I8, I8, I8, I64, I32, D, D, D, D, I32, I32, I32, I32, D, D, D,
         C.POINTER(P)]),
     "heomq_ctx_create": (C.c_int, [P, C.c_int, C.POINTER(P)]),
     "heomq_ctx_free": (None, [P]),

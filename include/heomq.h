@@ -40,6 +40,15 @@ extern "C" {
 #define HEOMQ_STOPPED       16
 #define HEOMQ_BAD_ARG       32
 
+/* Length (doubles) of the stats array every solver entry point fills.  The
+ * solvers write exactly these slots (unused ones are set to 0) and never
+ * beyond them: heomq_solve_steady, heomq_solve_bicg, heomq_solve_jacobi and
+ * their _host variants take HEOMQ_STATS_LEN; heomq_solve_steady_mixed takes
+ * HEOMQ_STATS_MIXED_LEN; heomq_chebyshev takes HEOMQ_STATS_CHEBY_LEN. */
+#define HEOMQ_STATS_LEN        8
+#define HEOMQ_STATS_MIXED_LEN 64
+#define HEOMQ_STATS_CHEBY_LEN  2
+
 /* Trace (normalisation) row placement of the steady-state operator. */
 #define HEOMQ_TRACE_NONE  0   /* plain L x (td/calcderiv_omp.f90)                         */
 #define HEOMQ_TRACE_LAST  1   /* y(d,d,last ADO) = tr x0 (calc_ax_tfqmr_omp.f90:492-495)  */
@@ -121,10 +130,15 @@ int heomq_model_set_params(heomq_model* m, const heomq_params* p);
  * nspin).  Only the default-flag subset is accepted (psdjob, no lscale,
  * offcor, lscreen, lsimple, lfilter, ltrun_der, lhf, lhb): the shim rejects
  * the rest with HEOMQ_UNSUPPORTED as the reference would need them.
+ * ntable and lall are the lengths of indextable and of the coupling stream
+ * (indexcoef, itypecoef, ifactfcoef, ifactrcoef), common block info3 of
+ * include/common:41-47 (indextable(ntable), main/main.f90:537;
+ * indexcoef(lall), main/buildcoefindex.f90:608): every position the tables
+ * name is checked against them, and one outside is HEOMQ_BAD_ARG.
  * ------------------------------------------------------------------------- */
 int heomq_model_from_reference(
     int32_t nrho, int64_t nunk, int32_t ntier, int32_t norbs, int32_t nspin,
-    int32_t nalf, int32_t ncor, int32_t igroundsteady,
+    int32_t nalf, int32_t ncor, int32_t igroundsteady, int64_t ntable, int64_t lall,
     const int32_t* indextable, const int64_t* nfirst, const int64_t* nlast,
     const int64_t* ifirst, const int64_t* indexcoef, const int8_t* itypecoef,
     const int8_t* ifactfcoef, const int8_t* ifactrcoef,
@@ -241,21 +255,28 @@ int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host
  * x (device, in/out): initial guess in, solution out.  Converged when
  * ||b - A x||_2 <= tol * ||b||_2 on the (Re, Im) vector.  transpose = 1 solves
  * with A^T (the adjoint solve).  b may be NULL (then b = e_trace).
- * stats[0] = iterations, stats[1] = applies, stats[2] = final relative
- * residual, stats[3] = seconds.  Returns HEOMQ_NOT_CONVERGED if maxiter hit.
+ * stats (HEOMQ_STATS_LEN): [0] iterations, [1] applies, [2] final relative
+ * residual, [3] seconds, [4] 0, [5] length of the real vector iterated on,
+ * [6..7] 0.  Returns HEOMQ_NOT_CONVERGED if maxiter hit.
  * ------------------------------------------------------------------------- */
 int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol,
                        int maxiter, int trace, int transpose, double* stats,
                        void* stream);
 /* Mixed precision: FP32 TFQMR inner solves (tolerance inner_tol) inside FP64
- * iterative refinement; residuals, updates, dots and norms in FP64. */
+ * iterative refinement; residuals, updates, dots and norms in FP64.
+ * stats (HEOMQ_STATS_MIXED_LEN): [0] FP32 inner iterations, [1] applies,
+ * [2] final relative residual, [3] seconds, [4] refinement steps, [5] 0,
+ * [6] iterations of the FP64 finish, [7] 0, [8..47] relative FP64 residual
+ * at the start of refinement steps 0..39, [48..63] inner iterations of
+ * refinement steps 0..15 (0 past the steps taken). */
 int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double tol,
                              double inner_tol, int maxiter, int trace,
                              double* stats, void* stream);
 /* Steady state by BiCG in the reference's convention (bicg/solvebicg.f90):
  * x (device, dense, in/out) holds the initial guess, b = e_(1,1) of tier 0,
  * stop when ||b - A_B x||_2 <= crit (absolute, as solvebicg.f90:124-126);
- * x's rho0 is returned Hermitian.  stats as heomq_solve_steady. */
+ * x's rho0 is returned Hermitian.  stats (HEOMQ_STATS_LEN) as
+ * heomq_solve_steady, [2] the absolute residual. */
 int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* stats, void* stream);
 /* The same on a host array (the solvebicg drop-in of the Fortran layer). */
 int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats);
@@ -348,8 +369,8 @@ int heomq_calc_ax_jacobi(heomq_ctx* c, int32_t dim0, const double* x_host, doubl
  * solve_jacobi.f90:70-85 stops); sweeps until ||A x - e||_2 (the
  * calc_ax_tfqmr_omp trace row, last ADO), checked after sweeps 1, 51, 101, ...
  * (:183-229), is <= crit or maxiter sweeps are done.  x returns the last
- * iterate.  stats (6): sweeps, applies, final residual, seconds, ejac used,
- * smallest residual seen. */
+ * iterate.  stats (HEOMQ_STATS_LEN): sweeps, applies, final residual, seconds, ejac used,
+ * smallest residual seen, [6..7] 0. */
 int heomq_solve_jacobi(heomq_ctx* c, double* x, double crit, int maxiter, double ejac, double* stats,
                        void* stream);
 /* The same on a host array (solve_jacobi_gpu of the Fortran layer). */
@@ -358,8 +379,8 @@ int heomq_solve_jacobi_host(heomq_ctx* c, double* rho_host, double crit, int max
  * rho_host (d, d, N) complex holds the initial guess on entry (a trace-one
  * tier-0 block, Hermitian as tfqmr_double/solve_tfqmr.f90:97-103 makes it)
  * and the solution on exit;
- * solved with the tier-0 trace row on the symmetry-packed layout.  stats as
- * heomq_solve_steady (at least 6 doubles). */
+ * solved with the tier-0 trace row on the symmetry-packed layout.  stats
+ * (HEOMQ_STATS_LEN) as heomq_solve_steady. */
 int heomq_solve_steady_host(heomq_ctx* c, double* rho_host, double tol, int maxiter,
                             double* stats);
 
@@ -390,7 +411,8 @@ int heomq_batch_solve(heomq_batch* b, double tol, int maxiter, double* rdo, doub
 
 /* ---------------------------------------------------------------------------
  * Propagation (td/intstp.f90 RK4 and td/chebyshev.f90 Chebyshev), fixed H.
- * x (device) is advanced in place by nstep steps of dt.
+ * x (device) is advanced in place by nstep steps of dt.  Chebyshev's stats
+ * (HEOMQ_STATS_CHEBY_LEN, may be NULL): [0] terms per step, [1] delta.
  * ------------------------------------------------------------------------- */
 int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream);
 int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats,

@@ -1292,3 +1292,57 @@ def landauer_siam(p: Params, npts=400001):
     i_l = np.sum(wts * jac * trans * (F[0] - F[1])) / (2 * np.pi) / HBAR
     n_occ = np.sum(wts * jac * (G[0] * F[0] + G[1] * F[1]) / den) / (2 * np.pi)
     return i_l * JCONST, n_occ
+
+
+def landauer_dqd(p: Params, npts=400001):
+    """Noninteracting two-orbital system ($para4 with uu1 = uu2 = u12 = 0):
+    current per spin into the dots from each lead and occupation per orbital
+    and spin, from the 2 x 2 retarded Green's function.
+
+    Each (lead alpha, orbital m) pair is its own bath (the drawers carry one
+    orbital index, evaluatepara.f90:1701-1760, and the correlation functions
+    are diagonal in it), so the self-energy of lead alpha is
+    diag_m(Gamma_am W^2 / 2 / (w - V_a + i W)) with the Lorentzian width of
+    :func:`landauer_siam`.  One-body h per spin: [[e1, t12], [t12, e2]]
+    (main/calchs.f90:514-567).  I_L = int tr(G_L G G_R G^H)(f_L - f_R) / 2 pi,
+    n_m = int [G (G_L f_L + G_R f_R) G^H]_mm / 2 pi (Caroli / Keldysh).
+    Returns (I[alpha, s] in the units of :func:`current`, occ[m, s]).
+    """
+    assert p.norbs == 2 and p.nalf == 2
+    eta, xi = pade_psd(p.nmats)
+    hp = p.hpar
+    th = np.linspace(-np.pi / 2, np.pi / 2, npts)[1:-1]
+    sc = max(p.bandwidth) * 0.5
+    w = sc * np.tan(th)
+    jac = sc / np.cos(th) ** 2
+    h = th[1] - th[0]
+    wts = np.ones_like(th)
+    wts[1:-1:2] = 4
+    wts[2:-1:2] = 2
+    wts *= h / 3 * jac / (2 * np.pi)
+    cur = np.zeros((2, 2))
+    occ = np.zeros((2, 2))
+    eps = {0: (hp["e1up"], hp["e2up"]), 1: (hp["e1down"], hp["e2down"])}
+    for s in range(2):
+        hs = np.array([[eps[s][0], hp["t12"]], [hp["t12"], eps[s][1]]], dtype=complex)
+        Ginv = w[:, None, None] * np.eye(2)[None] - hs[None]
+        Gam, F = [], []
+        for a in range(2):
+            W = p.bandwidth[a]
+            V = p.engyshift[a][s]
+            g = np.zeros((len(w), 2, 2))
+            for m in range(2):
+                ga = p.linewidth[a][m]
+                Ginv[:, m, m] -= 0.5 * ga * W / (w - V + 1j * W)
+                g[:, m, m] = ga * W ** 2 / ((w - V) ** 2 + W ** 2)
+            Gam.append(g)
+            F.append(np.real(fermi_pade((w - V) / p.temperature[a], eta, xi)))
+        G = np.linalg.inv(Ginv)
+        GH = np.conj(np.transpose(G, (0, 2, 1)))
+        trans = np.real(np.einsum("wij,wjk,wkl,wli->w", Gam[0], G, Gam[1], GH))
+        cur[0, s] = np.sum(wts * trans * (F[0] - F[1])) / HBAR
+        cur[1, s] = -cur[0, s]
+        lesser = G @ (Gam[0] * F[0][:, None, None] + Gam[1] * F[1][:, None, None]) @ GH
+        for m in range(2):
+            occ[m, s] = np.sum(wts * np.real(lesser[:, m, m]))
+    return cur * JCONST, occ

@@ -1,0 +1,273 @@
+"""The benchmark workloads on their own models, at full size, on the GPU.
+
+* single-impurity Anderson model, Pade Nk = 8 (M = 72), L = 3, V = 0: the
+  dense, transposed and packed applies against the numpy oracle at full size;
+  the steady state against the committed oracle fixture;
+* two-impurity model, d = 16, Nk = 6 (M = 112), L = 3: the applies against
+  the OpenMP C port of calc_ax_tfqmr_omp at full size; the steady state at
+  the reference's own stopping rule (||b - A x|| <= 1e-8 with ||b|| = 1,
+  main/main.f90:976, solve_tfqmr.f90:213-216) against the fixture;
+* mixed precision (FP32 inner solves inside FP64 refinement) on the
+  two-impurity model, against the same fixture;
+* the adjoint bias sweep on the single-impurity model: bias points of the
+  33-point grid in [-1, 1] against the fixtures, gradients against central
+  differences (h = 1e-5);
+* the sweep's extreme hierarchy shapes (Nk = 4, L = 2 and Nk = 8, L = 4).
+
+The fixtures (tests/golden/steady_*.json) come from an independent solver
+(restarted GMRES on the oracle's operator, tests/golden/make_steady_fixtures.py)
+at a true residual near 1e-14.  RDO and occupation are compared absolutely,
+currents relative to the workload's current scale: all within 1e-10.
+"""
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import heomq
+from heomq import _lib
+from heomq import adjoint as ADJ
+from oracle import heom_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+SIAM = dict(eps=-1.0, U=2.0, gamma=0.1, W=10.0, T=0.05, V=0.0, nk=8, ntop=3)
+DQD = dict(eps=-1.5, U=3.0, t=0.5, gamma=0.2, W=10.0, T=0.1, V=0.4, nk=6, ntop=3)
+
+_cache = {}
+
+
+def fixture(name):
+    f = json.loads((GOLDEN / f"steady_{name}.json").read_text())
+    f["rdo"] = np.array(f["rdo_re"]) + 1j * np.array(f["rdo_im"])
+    f["current"] = np.array(f["current"])
+    f["occupation"] = np.array(f["occupation"])
+    return f
+
+
+def setup(kind):
+    if kind not in _cache:
+        import torch
+        if kind == "siam":
+            m = heomq.Model(heomq.siam(**SIAM))
+            o = O.Model(O.siam_params(**SIAM))
+        else:
+            m = heomq.Model(heomq.two_impurity(**DQD))
+            o = O.Model(O.dqd_params(**DQD))
+        _cache[kind] = (m, heomq.Context(m), o, torch)
+    return _cache[kind]
+
+
+def port(kind):
+    key = "port_" + kind
+    if key not in _cache:
+        from oracle.heom_port import Port
+        _cache[key] = Port(setup(kind)[2])
+    return _cache[key]
+
+
+def to_dev(torch, x):
+    return torch.from_numpy(O.to_fortran(x)).to("cuda:0")
+
+
+def from_dev(t):
+    return O.from_fortran(t.cpu().numpy())
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def check_state(ctx, xd, fix, label):
+    x0 = from_dev(xd[:1])[0]
+    cur, occ = ctx.observables(xd)
+    e_rdo = np.abs(x0 - fix["rdo"]).max()
+    e_occ = np.abs(occ - fix["occupation"]).max()
+    e_cur = np.abs(cur - fix["current"]).max() / max(np.abs(fix["current"]).max(), heomq.JCONST)
+    print(f"{label}: RDO err {e_rdo:.2e}, occupation err {e_occ:.2e}, current err {e_cur:.2e} "
+          f"(fixture relres {fix['relres']:.1e})", flush=True)
+    assert e_rdo <= 1e-10 and e_occ <= 1e-10 and e_cur <= 1e-10
+    assert abs(np.trace(x0) - 1) <= 1e-12 and np.abs(x0 - x0.conj().T).max() <= 1e-12
+
+
+# ---------------------------------------------------------------- single-impurity workload
+@pytest.mark.parametrize("trace", ["last", None])
+def test_siam_workload_apply_full_size(trace):
+    m, ctx, o, torch = setup("siam")
+    print(f"siam workload N = {o.N}, trace {trace}", flush=True)
+    x = O.seeded_ado_vector(o, 0x5EED0000)
+    tr = heomq.TRACE_LAST if trace else heomq.TRACE_NONE
+    assert rel(from_dev(ctx.apply(to_dev(torch, x), trace=tr)), O.apply(o, x, trace_row=trace)) <= 1e-12
+    assert rel(from_dev(ctx.apply_transpose(to_dev(torch, x), trace=tr)),
+               O.apply_transpose(o, x, trace_row=trace)) <= 1e-12
+
+
+def test_siam_workload_packed_apply_full_size():
+    m, ctx, o, torch = setup("siam")
+    x = O.physical_projection(o, O.seeded_ado_vector(o, 0x5EED0001))
+    y = from_dev(ctx.apply_packed(to_dev(torch, x), trace=heomq.TRACE_RDO))
+    assert rel(y, O.physical_projection(o, O.apply(o, x, trace_row="rdo"))) <= 1e-12
+
+
+def test_siam_workload_steady_state():
+    m, ctx, o, torch = setup("siam")
+    xd, st = ctx.solve_steady(tol=1e-12)
+    print("siam workload solve", st, flush=True)
+    check_state(ctx, xd, fixture("siam_nk8_L3"), "siam workload")
+    cur, occ = ctx.observables(xd)
+    assert np.abs(cur).max() <= 1e-10 * heomq.JCONST          # V = 0
+    assert abs(occ.sum() - 1.0) <= 1e-10                     # eps = -U/2
+
+
+# ---------------------------------------------------------------- two-impurity workload
+@pytest.mark.parametrize("trace", ["last", "rdo", None])
+def test_dqd_workload_apply_matches_c_port(trace):
+    m, ctx, o, torch = setup("dqd")
+    print(f"two-impurity workload N = {o.N}, trace {trace}", flush=True)
+    x = O.seeded_ado_vector(o, 0x5EED0010)
+    tr = {"last": heomq.TRACE_LAST, "rdo": heomq.TRACE_RDO, None: heomq.TRACE_NONE}[trace]
+    y = from_dev(ctx.apply(to_dev(torch, x), trace=tr))
+    assert rel(y, port("dqd").apply(x, trace_row=trace)) <= 1e-12
+
+
+def test_dqd_workload_packed_apply_matches_c_port():
+    m, ctx, o, torch = setup("dqd")
+    x = O.physical_projection(o, O.seeded_ado_vector(o, 0x5EED0011))
+    y = from_dev(ctx.apply_packed(to_dev(torch, x), trace=heomq.TRACE_RDO))
+    assert rel(y, O.physical_projection(o, port("dqd").apply(x, trace_row="rdo"))) <= 1e-12
+
+
+def test_dqd_workload_steady_state_at_reference_criterion():
+    m, ctx, o, torch = setup("dqd")
+    ctx.set_shadow_seed(0)
+    xd, st = ctx.solve_steady(tol=1e-8)
+    print("two-impurity workload solve", st, flush=True)
+    assert st["relres"] <= 1e-8
+    check_state(ctx, xd, fixture("dqd_nk6_L3"), "two-impurity workload")
+
+
+def test_dqd_workload_mixed_precision():
+    m, ctx, o, torch = setup("dqd")
+    xd, st = ctx.solve_steady_mixed(tol=1e-8, inner_tol=1e-5, raise_on_fail=False)
+    print("two-impurity workload, mixed precision", st, flush=True)
+    assert st["status"] == 0 and st["relres"] <= 1e-8
+    check_state(ctx, xd, fixture("dqd_nk6_L3"), "two-impurity workload, mixed")
+
+
+# ---------------------------------------------------------------- stats contract
+def test_solvers_write_only_their_documented_stats():
+    # every solver writes exactly HEOMQ_STATS_*_LEN doubles (canaries after them stay)
+    m = heomq.Model(heomq.siam(nk=2, ntop=2, V=0.2))
+    ctx = heomq.Context(m)
+    L = _lib.lib()
+    canary = 16
+
+    def buf(n):
+        a = np.full(n + canary, 12345.0)
+        return a, a.ctypes.data_as(_lib.D)
+
+    st = C.c_void_p(ctx.torch.cuda.current_stream().cuda_stream)
+    vec = lambda x: ctx._check_vec(x, ctx.torch.complex128)
+    x = ctx.initial_guess()        # kept alive for every call below
+    a, p = buf(_lib.STATS_LEN)
+    _lib.check(L.heomq_solve_steady(ctx._h, vec(x), None, 1e-10, 5000, heomq.TRACE_RDO, 0, p, st))
+    assert np.all(a[_lib.STATS_LEN:] == 12345.0) and a[0] > 0
+    x = ctx.initial_guess()
+    a, p = buf(_lib.STATS_MIXED_LEN)
+    _lib.check(L.heomq_solve_steady_mixed(ctx._h, vec(x), None, 1e-10, 1e-5, 5000, heomq.TRACE_RDO, p, st))
+    assert np.all(a[_lib.STATS_MIXED_LEN:] == 12345.0) and a[4] >= 1
+    x = ctx.initial_guess()
+    a, p = buf(_lib.STATS_LEN)
+    _lib.check(L.heomq_solve_bicg(ctx._h, vec(x), 1e-10, 5000, p, st))
+    assert np.all(a[_lib.STATS_LEN:] == 12345.0)
+    x = ctx.initial_guess()
+    a, p = buf(_lib.STATS_LEN)
+    _lib.check(L.heomq_solve_jacobi(ctx._h, vec(x), 1e-8, 20000, 0.0, p, st))
+    assert np.all(a[_lib.STATS_LEN:] == 12345.0)
+    x = ctx.initial_guess()
+    a, p = buf(_lib.STATS_CHEBY_LEN)
+    _lib.check(L.heomq_chebyshev(ctx._h, vec(x), 0.01, 1, p, st))
+    assert np.all(a[_lib.STATS_CHEBY_LEN:] == 12345.0) and a[0] > 0
+    ctx.torch.cuda.synchronize()
+    ctx.close()
+
+
+# ---------------------------------------------------------------- adjoint bias sweep
+BIAS_GRID = np.linspace(-1.0, 1.0, 33)
+
+
+def sweep_problem():
+    if "sweep" not in _cache:
+        th = np.array([SIAM["eps"], SIAM["U"], SIAM["gamma"], SIAM["gamma"], 0.0])
+        _cache["sweep"] = ADJ.SteadyStateProblem(th, tol=1e-13, nk=SIAM["nk"], ntop=SIAM["ntop"], T=SIAM["T"],
+                                                 W=SIAM["W"])
+    return _cache["sweep"]
+
+
+@pytest.mark.parametrize("name,V", [("siam_nk8_L3_V-1", -1.0), ("siam_nk8_L3_V0.5", 0.5)])
+def test_bias_sweep_points_match_fixtures(name, V):
+    assert np.isclose(BIAS_GRID, V).any()
+    pr = sweep_problem()
+    th = pr.theta.copy()
+    th[4] = V
+    x, I = pr.solve(th)
+    fix = fixture(name)
+    print(f"bias point V = {V}: I_L = {I:.10e}, fixture {fix['current'][0].sum():.10e}", flush=True)
+    check_state(pr.ctx, x, fix, f"bias point V = {V}")
+    assert abs(I - fix["current"][0].sum()) <= 1e-10 * abs(fix["current"][0].sum())
+    if "dIdV" in fix:
+        g = pr.gradient(x)
+        print(f"  dI/dV adjoint {g[4]:.10e}, fixture {fix['dIdV']:.10e}", flush=True)
+        assert abs(g[4] - fix["dIdV"]) <= 1e-6 * abs(fix["dIdV"])
+
+
+def test_bias_sweep_gradient_matches_central_differences():
+    pr = sweep_problem()
+    th = pr.theta.copy()
+    th[4] = BIAS_GRID[20]
+    x, I = pr.solve(th)
+    g = pr.gradient(x)
+    gf = pr.fd_gradient(h=1e-5)
+    print(f"V = {th[4]}: adjoint {g}, central differences {gf}", flush=True)
+    assert np.abs(g - gf).max() <= 1e-6 * np.abs(gf).max()
+
+
+def test_bias_sweep_conductance_curve():
+    # dI/dV over a few grid points from the adjoint: odd in V about 0 at the
+    # particle-hole point (I(-V) = -I(V)), so dI/dV is even
+    pr = sweep_problem()
+    Vs = BIAS_GRID[[4, 12, 20, 28]]
+    cur, grads = ADJ.bias_sweep(pr, Vs, theta=pr.theta)
+    print("I", cur, "dI/dV", grads[:, 4], flush=True)
+    assert abs(cur[0] + cur[3]) <= 1e-9 * abs(cur[0]) and abs(cur[1] + cur[2]) <= 1e-9 * abs(cur[1])
+    assert abs(grads[0, 4] - grads[3, 4]) <= 1e-8 * abs(grads[0, 4])
+
+
+# ---------------------------------------------------------------- the sweep's extreme shapes
+@pytest.mark.parametrize("nk,ntop", [(4, 2), (8, 4)])
+def test_sweep_extreme_shapes(nk, ntop):
+    points = [dict(V=-1.7, eps=-1.8, T=0.03), dict(V=0.9, eps=-0.4, T=0.18)]
+    pl = [heomq.siam(nk=nk, ntop=ntop, U=2.0, gamma=0.1, **pt) for pt in points]
+    b = heomq.Batch(pl)
+    print(f"shape Nk = {nk}, L = {ntop}: {b.N} ADOs, {b.packed} packed elements per problem", flush=True)
+    rdo, cur, occ, info = b.solve(tol=1e-10)
+    print(info, flush=True)
+    assert (info["status"] == 0).all()
+    for p, pp in enumerate(pl):
+        r = rdo[p]
+        assert abs(np.trace(r) - 1) <= 1e-12 and np.abs(r - r.conj().T).max() <= 1e-12
+        assert np.abs(cur[p, 0] + cur[p, 1]).max() <= 1e-8 * np.abs(cur[p]).max()    # I_L = -I_R
+        ctx = heomq.Context(heomq.Model(pp))
+        xd, st = ctx.solve_steady(tol=1e-10)
+        x0 = from_dev(xd[:1])[0]
+        assert np.abs(x0 - r).max() <= 1e-9
+        if nk == 4:   # small enough for the oracle
+            o = O.Model(O.siam_params(nk=nk, ntop=ntop, U=2.0, gamma=0.1, **points[p]))
+            xo, _, _ = O.steady_state(o, tol=1e-12)
+            assert np.abs(r - xo[0]).max() <= 1e-9
+            assert np.abs(cur[p] - O.current(o, xo)).max() <= 1e-9 * np.abs(cur[p]).max()
+        ctx.close()
+    b.close()

@@ -46,6 +46,7 @@ def shim():
         args = [ref_scalar(arr["nrho"], i32), ref_scalar(arr["nunk"], i64), ref_scalar(arr["ntier"], i32),
                 ref_scalar(arr["norbs"], i32), ref_scalar(arr["nspin"], i32), ref_scalar(arr["nalf"], i32),
                 ref_scalar(arr["ncor"], i32), ref_scalar(arr["igroundsteady"], i32),
+                ref_scalar(np.size(arr["indextable"]), i64), ref_scalar(np.size(arr["indexcoef"]), i64),
                 ref_array(arr["indextable"], np.int32), ref_array(arr["nfirst"], np.int64),
                 ref_array(arr["nlast"], np.int64), ref_array(arr["ifirst"], np.int64),
                 ref_array(arr["indexcoef"], np.int64), ref_array(arr["itypecoef"], np.int8),

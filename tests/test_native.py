@@ -30,6 +30,20 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
 
 
+def test_header_constants_match_the_binding():
+    # status codes, trace rows, Hamiltonian kinds and stats lengths of heomq.h
+    text = (ROOT / "include" / "heomq.h").read_text()
+    defs = {k: int(v) for k, v in re.findall(r"#define\s+HEOMQ_([A-Z0-9_]+)\s+(-?\d+)", text)}
+    want = dict(OK=_lib.OK, BAD_DIM=_lib.BAD_DIM, UNSUPPORTED=_lib.UNSUPPORTED, HIP_ERROR=_lib.HIP_ERROR,
+                NOT_CONVERGED=_lib.NOT_CONVERGED, BREAKDOWN=_lib.BREAKDOWN, STOPPED=_lib.STOPPED,
+                BAD_ARG=_lib.BAD_ARG, TRACE_NONE=_lib.TRACE_NONE, TRACE_LAST=_lib.TRACE_LAST,
+                TRACE_RDO=_lib.TRACE_RDO, HAM_SIAM=_lib.HAM_SIAM, HAM_DQD=_lib.HAM_DQD,
+                STATS_LEN=_lib.STATS_LEN, STATS_MIXED_LEN=_lib.STATS_MIXED_LEN,
+                STATS_CHEBY_LEN=_lib.STATS_CHEBY_LEN)
+    for k, v in want.items():
+        assert defs[k] == v, k
+
+
 def test_fortran_shim_exports_reference_names():
     path = ROOT / "heom-quick2_amd" / "heomq" / "libheomq_fortran.so"
     if not path.exists():
@@ -113,6 +127,22 @@ def test_reference_loader_rejects_shifted_indextable():
     with pytest.raises(heomq.HeomqError) as e:
         heomq.Model.from_reference(**arr)
     assert e.value.code == _lib.BAD_ARG
+
+
+@pytest.mark.parametrize("short", ["indextable", "indexcoef"])
+def test_reference_loader_checks_table_lengths(short):
+    # a coupling stream or index table shorter than the positions the other
+    # tables name (lall / ntable of the reference's common block info3) is
+    # refused before anything past its end is read
+    o = O.Model(O.siam_params(nk=1, ntop=2, lsdraw=True))
+    arr = reference_arrays(o)
+    key = "ntable" if short == "indextable" else "lall"
+    arr[key] = np.size(arr[short]) - 1
+    with pytest.raises(heomq.HeomqError) as e:
+        heomq.Model.from_reference(**arr)
+    assert e.value.code == _lib.BAD_ARG
+    arr[key] = np.size(arr[short])
+    heomq.Model.from_reference(**arr).close()
 
 
 def test_reference_loader_rejects_bad_dims():

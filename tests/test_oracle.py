@@ -17,6 +17,10 @@ plausible mistake would break:
                               noninteracting HEOM is exact from tier 2 on).
 * steady state (U > 0)     -> tr rho = 1, Hermitian, I_L = -I_R, I(V=0) = 0,
                               <n> = 1 at particle-hole symmetry.
+* two-impurity ($para4)    -> U = 0: Landauer with the 2 x 2 retarded Green's
+                              function (symmetric and asymmetric dots, leads
+                              and spins); H's spectrum in closed form; <N> = 2
+                              at the particle-hole point eps = -U/2 - u12.
 """
 import itertools
 import math
@@ -101,8 +105,9 @@ def test_hierarchy_counts_bruteforce(nk, ntop, lsdraw):
             assert len(allk) == math.comb(m.M, n)
 
 
-def test_coupling_signs_bruteforce():
-    m = O.Model(O.siam_params(nk=2, ntop=3))
+@pytest.mark.parametrize("p", [O.siam_params(nk=2, ntop=3), O.dqd_params(nk=1, ntop=3)], ids=["siam", "dqd"])
+def test_coupling_signs_bruteforce(p):
+    m = O.Model(p)
 
     def perm_sign(seq):
         s = 1
@@ -209,6 +214,70 @@ def test_interacting_invariants():
     x2, _, _ = O.steady_state(m2, tol=1e-12)
     I = O.current(m2, x2)
     assert np.abs(I[0] + I[1]).max() < 1e-8 * np.abs(I).max()  # current conservation
+
+
+def _dqd_asym(nk=1, ntop=2):
+    # every orbital, spin and lead distinct: e1 != e2, up != down, Gamma per
+    # (lead, orbital) all different, an asymmetric bias split
+    return O.Params(norbs=2, nspin=2, nalf=2, nmats=nk, bandwidth=(8.0, 12.0),
+                    linewidth=((0.3, 0.12), (0.18, 0.25)), temperature=(0.2, 0.15),
+                    engyshift=((0.35, 0.3), (-0.2, -0.25)), ham="dqd",
+                    hpar=dict(e1up=-0.3, e1down=-0.1, e2up=0.2, e2down=0.05, uu1=0.0, uu2=0.0,
+                              u12=0.0, t12=0.4), ntop=ntop)
+
+
+@pytest.mark.parametrize("case", ["sym_t", "sym_t0", "asym", "asym_nk2"])
+def test_two_impurity_noninteracting_landauer(case):
+    # the noninteracting HEOM is exact from tier 2 on, for any number of
+    # orbitals: currents and occupations equal the 2 x 2 Green's-function result
+    if case == "sym_t":
+        p = O.dqd_params(nk=1, ntop=2, U=0.0, eps=-0.3, t=0.4, V=0.5, gamma=0.3, T=0.2)
+    elif case == "sym_t0":
+        p = O.dqd_params(nk=1, ntop=2, U=0.0, eps=-0.3, t=0.0, V=0.5, gamma=0.3, T=0.2)
+    else:
+        p = _dqd_asym(nk=2 if case == "asym_nk2" else 1)
+    m = O.Model(p)
+    x, it, rel = O.steady_state(m, tol=1e-12)
+    assert rel <= 1e-12
+    I, n = O.landauer_dqd(p)
+    Io = O.current(m, x)
+    assert np.abs(Io - I).max() <= 1e-9 * np.abs(I).max()
+    assert np.abs(O.occupation(m, x) - n).max() <= 1e-9
+
+
+def test_two_impurity_hamiltonian_spectrum_closed_form():
+    # equal levels eps, on-dot U, inter-dot u12, hopping t: the Fock spectrum
+    # sector by sector (N = 0..4), from second quantisation by hand
+    eps, U, u12, t = -0.7, 2.3, 0.6, 0.45
+    p = O.dqd_params(eps=eps, U=U, t=t, u12=u12)
+    m = O.Model(p)
+    ev = np.sort(np.linalg.eigvalsh(m.H * O.HBAR))
+    r = math.sqrt(((U - u12) / 2) ** 2 + 4 * t * t)
+    ref = [0.0]
+    ref += [eps - t, eps + t] * 2                                        # N = 1
+    ref += [2 * eps + u12] * 3 + [2 * eps + U]                           # N = 2: triplet, odd singlet
+    ref += [2 * eps + (U + u12) / 2 - r, 2 * eps + (U + u12) / 2 + r]    # N = 2: even singlets
+    ref += [3 * eps + U + 2 * u12 - t, 3 * eps + U + 2 * u12 + t] * 2    # N = 3
+    ref += [4 * eps + 2 * U + 4 * u12]                                   # N = 4
+    assert np.abs(ev - np.sort(ref)).max() < 1e-12
+
+
+def test_two_impurity_particle_hole_point():
+    # eps = -U/2 - u12 with symmetric leads and bias: <N> = 2 exactly, both
+    # dots and spins equal, I_L = -I_R
+    U, u12 = 2.0, 0.5
+    p = O.dqd_params(nk=1, ntop=2, U=U, u12=u12, eps=-U / 2 - u12, t=0.3, V=0.4, gamma=0.2, T=0.2)
+    m = O.Model(p)
+    x, _, rel = O.steady_state(m, tol=1e-12)
+    n = O.occupation(m, x)
+    assert abs(n.sum() - 2.0) < 1e-10
+    assert np.abs(n - n[0, 0]).max() < 1e-10
+    I = O.current(m, x)
+    assert np.abs(I[0] + I[1]).max() < 1e-8 * np.abs(I).max()
+    p2 = O.dqd_params(nk=1, ntop=2, U=U, u12=u12, eps=-U / 2, t=0.3, V=0.4, gamma=0.2, T=0.2)
+    m2 = O.Model(p2)
+    x2, _, _ = O.steady_state(m2, tol=1e-12)
+    assert O.occupation(m2, x2).sum() < 2.0 - 1e-3          # off the point it moves
 
 
 def test_trace_row_placements_agree_for_odd_top_tier():
