@@ -105,6 +105,8 @@ extern "C" {
 
 int heomq_batch_create(const heomq_params* params, int nprob, int device, heomq_batch** out) {
   if (!params || nprob < 1 || !out) return fail(HEOMQ_BAD_ARG, "null argument or no problems");
+  for (int p = 0; p < nprob; ++p)   // the scaled representation differs problem by problem
+    if (params[p].lscale) return fail(HEOMQ_UNSUPPORTED, "batches take unscaled (lscale = 0) problems");
   auto B = std::make_unique<heomq_batch>();
   B->nprob = nprob;
   if (int rc = heomq_model_build(&params[0], &B->shape)) return rc;

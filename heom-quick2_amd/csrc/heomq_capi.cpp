@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstdint>
 #include <chrono>
 #include <cmath>
@@ -35,8 +36,10 @@ void build_records(const Model& m, std::vector<DevRec>& rec) {
   for (int k = 0; k < M; ++k) {
     // forward lower (calc_ax_tfqmr_omp.f90:247-258): -i*ff*cb op(c) X + i*fr*cd X op(c)
     set(rec[k], -I * m.cb[k], I * m.cd[k], op_low(m, k), op_low(m, k));
-    // forward upper (:278-279, :342-353): -i*ff op'(c) X + i*fr X op'(c)
-    set(rec[M + k], -I, I, op_up(m, k), op_up(m, k));
+    // forward upper (:278-279, :342-353): -i*ff op'(c) X + i*fr X op'(c), times
+    // |cb_k|^(1/2) with lscale (:325-328)
+    const double sc = m.dbsqrt.empty() ? 1.0 : m.dbsqrt[k];
+    set(rec[M + k], -I * sc, I * sc, op_up(m, k), op_up(m, k));
     for (int kind = 0; kind < 2; ++kind) {
       const DevRec& f = rec[kind * M + k];
       const std::complex<double> cL(f.cLr, f.cLi), cR(f.cRr, f.cRi);
@@ -303,7 +306,47 @@ struct PackedHost {
   std::vector<int32_t> maps;
   int tslot = -1;
   int max_light[2] = {0, 0};
+  // grouped heavy rows (per operator) and sector-batched top-tier blocks
+  std::vector<int32_t> hg_off[2];
+  std::vector<int4> hg[2];
+  std::vector<int4> lt_blk;
+  std::vector<int32_t> lt_ados, lt_off;
+  std::vector<DevEntryP> lt_ent[2];
+  int lt_u = 0;
 };
+
+// Sort each heavy ADO's packed entries into runs of one gather map (and one
+// dagger flag) and list the runs; an ADO whose row or accumulator pairs do
+// not fit the kernel's LDS keeps the sliced form (no runs listed).
+void group_heavy_rows(const std::vector<int64_t>& rows, std::vector<DevEntryP>& ent, const std::vector<int>& nnz,
+                      int64_t n_heavy, std::vector<int32_t>& off, std::vector<int4>& grp) {
+  off.assign((size_t)n_heavy + 1, 0);
+  grp.clear();
+  for (int64_t g = 0; g < n_heavy; ++g) {
+    const int64_t p0 = rows[g], p1 = rows[g + 1];
+    auto key = [](const DevEntryP& e) { return std::make_pair((uint32_t)e.map, e.flags & 1); };
+    std::stable_sort(ent.begin() + p0, ent.begin() + p1,
+                     [&](const DevEntryP& a, const DevEntryP& b) { return key(a) < key(b); });
+    std::vector<int4> mine;
+    int P = 0;
+    for (int64_t p = p0; p < p1;) {
+      int64_t q = p;
+      while (q < p1 && key(ent[q]) == key(ent[p])) ++q;
+      if (ent[p].map >= 0) {
+        mine.push_back(make_int4((int)p, (int)q, ent[p].map, P | ((ent[p].flags & 1) ? (int)0x80000000 : 0)));
+        P += nnz[ent[p].nsec];
+      }
+      p = q;
+    }
+    const bool fits = p1 - p0 <= kHeavyStage && P <= kHeavyPairs && (int)mine.size() <= kHeavyGroups && !mine.empty();
+    if (fits) {
+      grp.insert(grp.end(), mine.begin(), mine.end());
+      grp.push_back(make_int4(0, 0, -1, P));
+    }
+    off[g + 1] = (int32_t)grp.size();
+  }
+  if (grp.empty()) grp.push_back(make_int4(0, 0, -1, 0));
+}
 
 int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
                       const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt, PackedHost& ph) {
@@ -466,6 +509,8 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
   };
   ph.pentf = conv(rowf, entf);
   ph.pentt = conv(rowt, entt);
+  group_heavy_rows(rowf, ph.pentf, nnz, n_heavy, ph.hg_off[0], ph.hg[0]);
+  group_heavy_rows(rowt, ph.pentt, nnz, n_heavy, ph.hg_off[1], ph.hg[1]);
   if (maps.empty()) maps.push_back(0);
   ph.fN = std::move(fN);
   ph.fS = std::move(fS);
@@ -489,8 +534,42 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
     for (int64_t g = n_heavy; g < m.N; ++g) mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
     ph.max_light[w] = mx;
   }
+  // sector-batched top-tier blocks: the non-empty top-tier ADOs of one sector
+  // (in ADO order), 256 / nnz of them per block; each with its first packed
+  // element and its couplings padded to lt_u entries (map -1)
+  ph.lt_u = std::max(1, std::max(ph.max_light[0], ph.max_light[1]));
+  bool lt_ok = ph.lt_u <= kLightMaxU && n_heavy < m.N;
+  for (int s = 0; s < S && lt_ok; ++s)
+    if (ph.nnz[s] > 256) lt_ok = false;
+  if (lt_ok) {
+    for (int s = 0; s < S; ++s) {
+      if (ph.nnz[s] == 0) continue;
+      const int chunk = 256 / ph.nnz[s];
+      int in_block = 0;
+      for (int64_t a = n_heavy; a < m.N; ++a) {
+        if (ph.sec[a] != s) continue;
+        if (in_block == 0) ph.lt_blk.push_back(make_int4((int)ph.lt_ados.size(), 0, ph.nnz[s], s));
+        ph.lt_ados.push_back((int32_t)a);
+        ph.lt_off.push_back((int32_t)ph.off[a]);
+        ph.lt_blk.back().y += 1;
+        if (++in_block == chunk) in_block = 0;
+      }
+    }
+    for (int w = 0; w < 2; ++w) {
+      const auto& rows = w == 0 ? rowf : rowt;
+      const auto& pent = w == 0 ? ph.pentf : ph.pentt;
+      auto& out = ph.lt_ent[w];
+      out.assign(ph.lt_ados.size() * (size_t)ph.lt_u, DevEntryP{0, -1, -1, 0, 0, 0});
+      for (size_t i = 0; i < ph.lt_ados.size(); ++i) {
+        const int64_t a = ph.lt_ados[i];
+        for (int64_t p = rows[a]; p < rows[a + 1]; ++p) out[i * ph.lt_u + (p - rows[a])] = pent[p];
+      }
+    }
+  }
   return HEOMQ_OK;
 }
+
+void set_apply_form(heomq_ctx* c, int w, int form);
 
 int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
                  const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt) {
@@ -515,6 +594,17 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
   HIPCHECK(upload(c->p_entf, ph.pentf));
   HIPCHECK(upload(c->p_entt, ph.pentt));
   HIPCHECK(upload(c->p_maps, ph.maps));
+  HIPCHECK(upload(c->p_hgoff_f, ph.hg_off[0]));
+  HIPCHECK(upload(c->p_hgoff_t, ph.hg_off[1]));
+  HIPCHECK(upload(c->p_hg_f, ph.hg[0]));
+  HIPCHECK(upload(c->p_hg_t, ph.hg[1]));
+  if (!ph.lt_blk.empty()) {
+    HIPCHECK(upload(c->p_ltblk, ph.lt_blk));
+    HIPCHECK(upload(c->p_ltados, ph.lt_ados));
+    HIPCHECK(upload(c->p_ltoff, ph.lt_off));
+    HIPCHECK(upload(c->p_ltent_f, ph.lt_ent[0]));
+    HIPCHECK(upload(c->p_ltent_t, ph.lt_ent[1]));
+  }
   c->nsec = ph.S;
   if (int rc = build_packed_h(c, m)) return rc;
   c->ptotal = ph.total;
@@ -536,8 +626,32 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     p.light_block_ado = c->p_lblock.as<int32_t>();
     p.light_nz = c->p_lnz.as<int32_t>();
     p.max_light_couplings = ph.max_light[w];
+    p.n_lt_blocks = (int)ph.lt_blk.size();
+    p.lt_u = ph.lt_u;
+    // the default form: the sector-batched top tier with the sliced heavy
+    // rows (the fastest on both test workloads' forward applies); the
+    // autotuner (heomq_ctx_autotune) picks per operator, HEOMQ_APPLY_FORM
+    // forces one
+    const char* form = std::getenv("HEOMQ_APPLY_FORM");
+    set_apply_form(c, w, form ? std::atoi(form) : 2);
   }
   return HEOMQ_OK;
+}
+
+// Form of the packed apply of operator w (0 forward, 1 transposed): bit 0
+// the grouped heavy rows, bit 1 the sector-batched top tier (0: the sliced
+// heavy and packed-order light forms).  Every form computes the same operator
+// (to rounding); a form whose tables were not built falls back per half.
+void set_apply_form(heomq_ctx* c, int w, int form) {
+  DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
+  const bool grouped = (form & 1) != 0, sector = (form & 2) != 0 && c->p_ltblk.p != nullptr;
+  p.hg_off = grouped ? (w == 0 ? c->p_hgoff_f : c->p_hgoff_t).as<int32_t>() : nullptr;
+  p.hg = grouped ? (w == 0 ? c->p_hg_f : c->p_hg_t).as<int4>() : nullptr;
+  p.lt_blk = sector ? c->p_ltblk.as<int4>() : nullptr;
+  p.lt_off = sector ? c->p_ltoff.as<int32_t>() : nullptr;
+  p.lt_id = sector ? c->p_ltados.as<int32_t>() : nullptr;
+  p.lt_ent = sector ? (w == 0 ? c->p_ltent_f : c->p_ltent_t).as<DevEntryP>() : nullptr;
+  c->apply_form[w] = (grouped ? 1 : 0) | (sector ? 2 : 0);
 }
 
 // coefficient pointers of the packed operators follow the dense ones
@@ -677,14 +791,15 @@ int heomq_model_from_reference(
     const int8_t* ifactfcoef, const int8_t* ifactrcoef, const int64_t* index_coef_ref,
     const int32_t* indexdraw, const double* cb, const double* cd, const double* cgamma,
     const double* dipm, const int32_t* ilead, const int32_t* mspin, const int32_t* morbs,
-    const int32_t* mpm, const double* eleadinfty, const double* hs, const double* amsall,
+    const int32_t* mpm, const double* eleadinfty, const double* hs, const double* amsall, const double* dbsqrt,
     heomq_model** out) {
   if (!out) return fail(HEOMQ_BAD_ARG, "null output");
   auto* h = new heomq_model;
   const int rc = model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady, ntable,
                                       lall, indextable, nfirst, nlast, ifirst, indexcoef, itypecoef,
                                       ifactfcoef, ifactrcoef, index_coef_ref, indexdraw, cb, cd, cgamma,
-                                      dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, h->m);
+                                      dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, dbsqrt,
+                                      h->m);
   if (rc) { delete h; return rc; }
   *out = h;
   return HEOMQ_OK;
@@ -702,6 +817,8 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
   c->first = m.first;
   c->sgn = m.sgn; c->spin = m.spin; c->orb = m.orb; c->alf = m.alf;
   c->ams = m.ams;
+  c->dbsqrt = m.dbsqrt.empty() ? std::vector<double>(m.M, 1.0) : m.dbsqrt;
+  c->lscale = m.lscale;
   c->tiers = m.tiers;
   c->steady_model = m.steady;
   c->field_amp = m.eleadinfty;
@@ -759,6 +876,55 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
 
 void heomq_ctx_free(heomq_ctx* c) { delete c; }
 
+// The autotuner of the packed apply: every form of each operator is timed
+// with HIP events over `reps` back-to-back launches on a trace-one state (the
+// operator's own tables, this GPU) and the fastest is kept.
+int heomq_ctx_autotune(heomq_ctx* c, int reps, int32_t* forms, double* times_us) {
+  if (!c || reps < 1) return fail(HEOMQ_BAD_ARG, "null context or reps < 1");
+  if (c->ptotal <= 0) return HEOMQ_OK;
+  HIPCHECK(hipSetDevice(c->device));
+  c->drop_graphs();   // captured chunks hold the operator's tables by value
+  DevBuf xb, yb;
+  const size_t bytes = sizeof(double) * 2 * (size_t)c->ptotal;
+  HIPCHECK(xb.alloc(bytes));
+  HIPCHECK(yb.alloc(bytes));
+  hipStream_t st = c->solve_stream;
+  HIPCHECK(vec_fill<double>(st, 2 * c->ptotal, xb.as<double>(), 1.0));
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  int rc = HEOMQ_OK;
+  for (int w = 0; w < 2 && rc == HEOMQ_OK; ++w) {
+    double best = 1e300;
+    int best_f = 2;
+    for (int f = 0; f < 4; ++f) {
+      set_apply_form(c, w, f);
+      const DevPacked& op = w == 0 ? c->pfwd : c->ptrn;
+      hipError_t e = hipSuccess;
+      for (int k = 0; k < 2 && e == hipSuccess; ++k) e = launch_apply_packed_f64(op, xb.as<double>(), yb.as<double>(), true, st);
+      if (e == hipSuccess) e = hipEventRecord(e0, st);
+      for (int k = 0; k < reps && e == hipSuccess; ++k)
+        e = launch_apply_packed_f64(op, xb.as<double>(), yb.as<double>(), true, st);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(HEOMQ_HIP_ERROR, std::string("autotune: ") + hipGetErrorString(e));
+        break;
+      }
+      const double us = 1e3 * ms / reps;
+      if (times_us) times_us[4 * w + f] = us;
+      if (us < best) { best = us; best_f = f; }
+    }
+    set_apply_form(c, w, best_f);
+    if (forms) forms[w] = c->apply_form[w];
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
 int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
   if (!c || !h) return fail(HEOMQ_BAD_ARG, "null argument");
   if (h->m.N != c->N || h->m.M != c->M) return fail(HEOMQ_BAD_ARG, "model shape differs from context");
@@ -799,6 +965,8 @@ int heomq_apply_f32(heomq_ctx* c, const float* x, float* y, int trace, void* str
 int heomq_apply_dparam(heomq_ctx* c, const double* x, double* y, const double* dH, const double* dcb,
                        const double* dcd, const double* dgam, void* stream) {
   if (!c || !x || !y || !dH || !dcb || !dcd || !dgam) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c->lscale)   // the scaling factors |cb|^(1/2) are not affine in the parameters
+    return fail(HEOMQ_UNSUPPORTED, "derivative apply: lscale models are not supported");
   const int M = c->M, d = c->d;
   // records: lower-tier kind with derivative coefficients; upper-tier terms
   // (coefficients -i, +i) do not depend on parameters and are switched off

@@ -74,6 +74,8 @@ struct heomq_ctx {
   std::vector<int> tier1_dag;        // 1 if stored as its Hermitian partner
   std::vector<int> tier1_fact;
   std::vector<double> ams;           // host copy for observables
+  std::vector<double> dbsqrt;        // lscale: |cb_k|^(1/2) (the stored tier-1 ADO is rho_k / dbsqrt_k); else 1
+  bool lscale = false;
   std::vector<std::vector<int32_t>> tiers;   // drawer tuples (for per-ADO rates)
   // shared device tables
   DevBuf rec, rec_d, opmap, hnnz, hcol, hval, hnnz_d, hcol_d, hval_d;
@@ -92,7 +94,9 @@ struct heomq_ctx {
   // symmetry-packed layout (solver path)
   DevBuf p_off, p_sec, p_elems, p_pos, p_block, p_lblock, p_entf, p_entt, p_hrow, p_hq, p_hc, p_maps;
   DevBuf p_hq_ell, p_hc_ell, p_lnz;
+  DevBuf p_hgoff_f, p_hgoff_t, p_hg_f, p_hg_t, p_ltblk, p_ltados, p_ltoff, p_ltent_f, p_ltent_t;   // restructured apply
   heomq::DevPacked pfwd, ptrn;
+  int apply_form[2] = {2, 2};        // packed apply form of the forward / transposed operator
   int64_t ptotal = 0;
   int nsec = 0;
   bool cons_n = true, cons_s = true;

@@ -70,6 +70,12 @@ struct DevEntryP {       // packed coupling (16 bytes)
   uint8_t nsec;
 };
 
+// LDS limits of the restructured packed apply (checked when its tables are built)
+constexpr int kHeavyStage = 256;   // entries of one heavy row staged at once
+constexpr int kHeavyPairs = 296;   // (group, neighbour slot) accumulator pairs of one heavy ADO
+constexpr int kHeavyGroups = 47;   // entry runs of one heavy ADO
+constexpr int kLightMaxU = 8;      // couplings of one top-tier ADO
+
 struct DevPacked {
   int64_t n_ado = 0, total = 0;      // ADOs, packed complex elements
   const int64_t* off = nullptr;      // packed offset of each ADO (n_ado + 1)
@@ -100,6 +106,28 @@ struct DevPacked {
                                              // first element (+ one entry for the end); a block spans at most
                                              // 257 of these ADOs
   int max_light_couplings = 0;       // largest coupling count of a top-tier ADO (<= 8)
+  // Grouped form of the ADOs below the top tier (the reference's
+  // accumulate-then-apply order, calc_ax_tfqmr_omp.f90:289-353): the row of
+  // heavy ADO a is sorted so that entries sharing a gather map (one c
+  // operator, one neighbour sector, one dagger flag) are contiguous runs;
+  // hg[hg_off[a] .. hg_off[a + 1] - 1] lists them as {first entry, end entry,
+  // map, first accumulator pair | dagger << 31}, closed by a sentinel whose
+  // last field is the ADO's accumulator pair count.  hg_off[a + 1] ==
+  // hg_off[a]: the ADO takes the sliced form.  Null: no grouped form.
+  const int32_t* hg_off = nullptr;
+  const int4* hg = nullptr;
+  // Sector-batched top tier: light block b takes the top-tier ADOs i =
+  // lt_blk[b].x .. + lt_blk[b].y - 1 of a list sorted by sector, all of
+  // sector lt_blk[b].w with lt_blk[b].z elements each; lt_off[i] is ADO i's
+  // first packed element, lt_id[i] the ADO, lt_ent[i * lt_u + r] its
+  // coupling r (map -1 past its last), lt_u the most couplings of a top-tier
+  // ADO.  Null: the top tier runs in packed-element order (light_block_ado).
+  const int4* lt_blk = nullptr;
+  const int32_t* lt_off = nullptr;
+  const int32_t* lt_id = nullptr;
+  const DevEntryP* lt_ent = nullptr;
+  int n_lt_blocks = 0;
+  int lt_u = 0;
   // slot-space tables: i(HX - XH) per sector as CSR over packed slots, and the
   // gather maps (value (q + 1) * sign, 0 = no term) referenced by entries
   // the same rows padded to a fixed width hell_w (slot -1 / coefficient 0 as

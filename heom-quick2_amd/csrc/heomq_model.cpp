@@ -234,6 +234,21 @@ static void build_hamiltonian(Model& m) {
 // Bath coefficients (evaluatepara.f90:1897-1982; cgamma :2210-2227; bias
 // eleadinfty :2431 with getengyshift(t -> inf) = amplitude).
 // ---------------------------------------------------------------------------
+// lscale (main/evaluatepara.f90:2927-2990): dbsqrt_k = |cb_k|^(1/2) of the
+// unscaled cb, then cb and cd divided by it (by nothing where |cb| <= dpico).
+static void apply_lscale(Model& m) {
+  m.lscale = m.p.lscale != 0;
+  m.dbsqrt.assign(m.M, 1.0);
+  if (!m.lscale) return;
+  for (int k = 0; k < m.M; ++k) {
+    const double a = std::abs(m.cb[k]);
+    m.dbsqrt[k] = std::sqrt(a);
+    const double inv = a > 1e-12 ? 1.0 / m.dbsqrt[k] : 0.0;
+    m.cb[k] *= inv;
+    m.cd[k] *= inv;
+  }
+}
+
 static void build_bath(Model& m) {
   std::vector<double> eta, xi;
   pade_psd(m.p.nmats, eta, xi);
@@ -270,6 +285,7 @@ static void build_bath(Model& m) {
     m.gdraw[k] = m.cgamma[k];
     if (m.steady) m.gdraw[k] += I * dipm * m.eleadinfty[a * m.nspin + m.spin[k]];
   }
+  apply_lscale(m);
 }
 
 // ---------------------------------------------------------------------------
@@ -446,6 +462,7 @@ static int check_params(const heomq_params& p) {
   if (p.ham == HEOMQ_HAM_DQD && p.norbs != 2) return fail(HEOMQ_BAD_ARG, "two-impurity model needs norbs = 2");
   if (p.ham != HEOMQ_HAM_SIAM && p.ham != HEOMQ_HAM_DQD) return fail(HEOMQ_BAD_ARG, "unknown Hamiltonian kind");
   if (p.ham == HEOMQ_HAM_SIAM && p.norbs != 1) return fail(HEOMQ_BAD_ARG, "SIAM needs norbs = 1");
+  if (p.lscale != 0 && p.lscale != 1) return fail(HEOMQ_BAD_ARG, "lscale must be 0 or 1");
   for (int a = 0; a < p.nalf; ++a)
     if (!(p.bandwidth[a] > 0) || !(p.temperature[a] > 0)) return fail(HEOMQ_BAD_ARG, "bandwidth and temperature must be > 0");
   return HEOMQ_OK;
@@ -475,8 +492,8 @@ int build_model(const heomq_params& p, Model& m) {
 int set_params(Model& m, const heomq_params& p) {
   if (int rc = check_params(p)) return rc;
   if (p.norbs != m.p.norbs || p.nalf != m.p.nalf || p.nmats != m.p.nmats || p.ntop != m.p.ntop ||
-      p.lsdraw != m.p.lsdraw || p.ham != m.p.ham)
-    return fail(HEOMQ_BAD_ARG, "set_params: hierarchy shape differs");
+      p.lsdraw != m.p.lsdraw || p.ham != m.p.ham || p.lscale != m.p.lscale)
+    return fail(HEOMQ_BAD_ARG, "set_params: hierarchy shape (or the lscale representation) differs");
   m.p = p;
   m.steady = p.steady != 0;
   build_bath(m);
@@ -496,7 +513,7 @@ int model_from_reference(
     const int64_t* index_coef_ref, const int32_t* indexdraw, const double* cb,
     const double* cd, const double* cgamma, const double* dipm, const int32_t* ilead,
     const int32_t* mspin, const int32_t* morbs, const int32_t* mpm, const double* eleadinfty,
-    const double* hs, const double* amsall, Model& m) {
+    const double* hs, const double* amsall, const double* dbsqrt, Model& m) {
   if (nspin != 2) return fail(HEOMQ_UNSUPPORTED, "only nspin = 2 is supported");
   int dd = 1;
   for (int t = 0; t < norbs; ++t) dd *= 4;
@@ -521,6 +538,9 @@ int model_from_reference(
       return fail(HEOMQ_BAD_ARG, "drawer attributes do not follow the reference ordering");
   }
   m.cb.resize(M); m.cd.resize(M); m.cgamma.resize(M); m.gdraw.resize(M);
+  m.dbsqrt.assign(M, 1.0);
+  m.lscale = dbsqrt != nullptr;
+  m.p.lscale = m.lscale ? 1 : 0;
   m.eleadinfty.assign((size_t)nalf * nspin, 0.0);
   for (int a = 0; a < nalf; ++a)
     for (int s = 0; s < nspin; ++s) m.eleadinfty[a * nspin + s] = eleadinfty[a + nalf * s];
@@ -531,6 +551,10 @@ int model_from_reference(
     m.cgamma[k] = cplx(cgamma[2 * k], cgamma[2 * k + 1]);
     m.gdraw[k] = m.cgamma[k];
     if (m.steady) m.gdraw[k] += cplx(0, 1) * dipm[k] * m.eleadinfty[m.alf[k] * nspin + m.spin[k]];
+    // lscale: the reference has already divided cb and cd by dbsqrt
+    // (evaluatepara.f90:2976-2990); dbsqrt scales the upper couplings
+    m.dbsqrt[k] = dbsqrt ? dbsqrt[idx] : 1.0;
+    if (dbsqrt && !(m.dbsqrt[k] >= 0.0)) return fail(HEOMQ_BAD_ARG, "dbsqrt must be >= 0");
   }
   const int d = nrho;
   m.H.assign((size_t)d * d, 0);
@@ -608,6 +632,7 @@ int model_from_reference(
       if (n < m.L)
         for (int e = 0; e < M; ++e, ++lnl) {
           const int32_t nb = (int32_t)(indexcoef[lnl] - 1);
+          if (nb == -1) continue;   // absent (indexcoef = 0): its type and signs are not read
           if (bad_entry(nb, itypecoef[lnl], true)) return fail(HEOMQ_BAD_ARG, "indexcoef: bad upper-tier entry");
           m.upper[(size_t)g * M + e] = UpperEntry{nb, itypecoef[lnl], ifactfcoef[lnl], ifactrcoef[lnl]};
         }

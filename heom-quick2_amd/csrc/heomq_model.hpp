@@ -37,6 +37,10 @@ struct Model {
   // drawer attributes (evaluatepara.f90:1701-1772)
   std::vector<int> sgn, spin, orb, alf, cor, bar;
   std::vector<cplx> cb, cd, cgamma, gdraw;   // gdraw = cgamma + i*dipm*eleadinfty
+  // lscale: |cb_k|^(1/2) of the unscaled cb (1 without lscale); cb, cd are
+  // then the scaled ones and every upper coupling carries this factor
+  std::vector<double> dbsqrt;
+  bool lscale = false;
   std::vector<double> eleadinfty;           // [alf*nspin + s]
   std::vector<cplx> H;                       // d*d column-major, already / hbar
   std::vector<double> ams;                   // [(m*nspin + s)*d*d + i + j*d]
@@ -69,6 +73,6 @@ int model_from_reference(
     const int8_t* ifactrcoef, const int64_t* index_coef_ref, const int32_t* indexdraw,
     const double* cb, const double* cd, const double* cgamma, const double* dipm,
     const int32_t* ilead, const int32_t* mspin, const int32_t* morbs, const int32_t* mpm,
-    const double* eleadinfty, const double* hs, const double* amsall, Model& m);
+    const double* eleadinfty, const double* hs, const double* amsall, const double* dbsqrt, Model& m);
 
 }  // namespace heomq

@@ -229,7 +229,7 @@ constexpr size_t kLightOffP0 = kAlign16(257 * 8), kLightOffP1 = kLightOffP0 + kA
                  kLightOffX = kLightOffGam + kAlign16(257 * 16), kLightSmem = kLightOffX + 256 * 16;
 constexpr size_t kHeavyOffRi = 256 * 8, kHeavyOffEn = 2 * 256 * 8, kHeavyOffCf = kHeavyOffEn + 256 * sizeof(DevEntryP),
                  kHeavyOffX = kHeavyOffCf + 256 * sizeof(CoefLR), kHeavySmem = kHeavyOffX + 256 * 16;
-constexpr size_t kApplySmem = kLightSmem > kHeavySmem ? kLightSmem : kHeavySmem;
+constexpr size_t kApplySmem0 = kLightSmem > kHeavySmem ? kLightSmem : kHeavySmem;
 
 template <typename T, bool TRANS, int U>
 __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
@@ -388,11 +388,269 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
   if (a == 0) tier0_fixup_block<T>(op, x, y, rdo);   // block-uniform branch
 }
 
+// ---------------------------------------------------------------------------
+// Restructured apply.
+//
+// Heavy ADOs, grouped form.  The upper couplings of an ADO below the top tier
+// come in runs that share one gather map (one c operator, one neighbour
+// sector, one dagger flag; about 14 drawers each: leads x exponents).  As the
+// reference does (calc_ax_tfqmr_omp.f90:289-353: accumulate the neighbours,
+// then apply c once), each run is first summed in the neighbour's slot space,
+// B_L[u] = sum_e wL_e x_e[u] and B_R[u] = sum_e wR_e x_e[u], one lane per
+// (run, neighbour slot) pair: the lane's loads are independent and lanes read
+// consecutive elements of one neighbour.  Each output slot then takes two LDS
+// reads per run through the run's map.  Weights: w = s * c from the entry's
+// sign flags and record, conjugated for X^H runs (sum w conj(x) = conj(sum
+// conj(w) x)).
+template <typename T>
+struct HWgt {
+  int32_t base, pad;
+  T lr, li, rr, ri;
+};
+constexpr size_t kGOffB = 256 * sizeof(HWgt<double>);                              // accumulator pairs
+constexpr size_t kGOffG = kGOffB + 2 * 16 * (size_t)kHeavyPairs;                   // run table
+constexpr size_t kGSmem = kGOffG + 16 * (size_t)(kHeavyGroups + 1);
+
+template <typename T, bool TRANS>
+__device__ inline void heavy_grouped(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x,
+                                     T* __restrict__ y, int64_t zero_at, int rdo, int bid, char* smem, bool scaled) {
+  using V = typename P2<T>::type;
+  HWgt<T>* s_w = reinterpret_cast<HWgt<T>*>(smem);
+  V* s_B = reinterpret_cast<V*>(smem + kGOffB);
+  int4* s_g = reinterpret_cast<int4*>(smem + kGOffG);
+  T* red_r = reinterpret_cast<T*>(smem);                       // phase 2: over the staged weights
+  T* red_i = reinterpret_cast<T*>(smem + kHeavyOffRi);
+  const int64_t a = bid;
+  const int64_t base = op.off[a];
+  const int nnz = (int)(op.off[a + 1] - base);
+  const V* __restrict__ xv = reinterpret_cast<const V*>(x);
+  const int s = op.sec[a];
+  const T gr = (T)cf_p.gam[2 * a], gi = (T)cf_p.gam[2 * a + 1];
+  const int64_t p0 = op.rowptr[a];
+  const int nent = (int)(op.rowptr[a + 1] - p0);               // <= kHeavyStage (host check)
+  const int g0 = op.hg_off[a];
+  const int ng = op.hg_off[a + 1] - g0 - 1;                    // runs; the sentinel follows
+  for (int k = threadIdx.x; k <= ng; k += 256) s_g[k] = op.hg[g0 + k];
+  for (int k = threadIdx.x; k < nent; k += 256) {
+    const DevEntryP e = op.ent[p0 + k];
+    const CoefLR c = load_coef(cf_p.rec, e.rec);
+    const T fl = (e.flags & 2) ? (T)-1 : (T)1, fr = (e.flags & 4) ? (T)-1 : (T)1;
+    const T cj = (e.flags & 1) ? (T)-1 : (T)1;
+    HWgt<T> w;
+    w.base = e.base;
+    w.pad = 0;
+    w.lr = fl * (T)c.l.x;
+    w.li = cj * fl * (T)c.l.y;
+    w.rr = fr * (T)c.r.x;
+    w.ri = cj * fr * (T)c.r.y;
+    s_w[k] = w;
+  }
+  __syncthreads();
+  // phase 1: the runs' accumulators in neighbour slot space
+  const int P = s_g[ng].w;
+  for (int l = threadIdx.x; l < P; l += 256) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if ((s_g[mid].w & 0x7fffffff) <= l) lo = mid; else hi = mid - 1;
+    }
+    const int4 G = s_g[lo];
+    const int u = l - (G.w & 0x7fffffff);
+    T lr = 0, li = 0, rr = 0, ri = 0;
+    int e = G.x;
+    for (; e + 4 <= G.y; e += 4) {   // four independent loads in flight
+      V v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = ld<T, TRANS>(xv, (int64_t)s_w[e + q - p0].base + u, zero_at);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const HWgt<T>& w = s_w[e + q - p0];
+        lr += w.lr * v[q].x - w.li * v[q].y;
+        li += w.lr * v[q].y + w.li * v[q].x;
+        rr += w.rr * v[q].x - w.ri * v[q].y;
+        ri += w.rr * v[q].y + w.ri * v[q].x;
+      }
+    }
+    for (; e < G.y; ++e) {
+      const V v = ld<T, TRANS>(xv, (int64_t)s_w[e - p0].base + u, zero_at);
+      const HWgt<T>& w = s_w[e - p0];
+      lr += w.lr * v.x - w.li * v.y;
+      li += w.lr * v.y + w.li * v.x;
+      rr += w.rr * v.x - w.ri * v.y;
+      ri += w.rr * v.y + w.ri * v.x;
+    }
+    V bl, br;
+    bl.x = lr; bl.y = li;
+    br.x = rr; br.y = ri;
+    s_B[2 * l] = bl;
+    s_B[2 * l + 1] = br;
+  }
+  __syncthreads();
+  // phase 2: output slots, the runs sliced over the lanes, slices summed as
+  // in the sliced form (fixed pairing)
+  for (int t0 = 0; t0 < nnz; t0 += 256) {
+    const int w = nnz - t0 < 256 ? nnz - t0 : 256;
+    const int S = 256 / w;
+    const int lane = threadIdx.x % w, slice = threadIdx.x / w;
+    const int t = t0 + lane;
+    T ar = 0, ai = 0, csr = 0, csi = 0;
+    if (slice < S) {
+      if (slice == 0)
+        diag_term<T, TRANS>(op, cf_p.hc_ell, xv, gr, gi, base, s, t, zero_at, ar, ai, nullptr, 0, 0, csr, csi);
+      for (int g = slice; g < ng; g += S) {
+        const int4 G = s_g[g];
+        const int ub = G.w & 0x7fffffff;
+        const T cj = G.w < 0 ? (T)-1 : (T)1;   // X^H run: conj of the accumulator
+        const int mv = op.maps[G.z + t];
+        const int mL = (int)(int16_t)(mv & 0xffff), mR = (int)(int16_t)((uint32_t)mv >> 16);
+        if (mL != 0) {
+          const V b = s_B[2 * (ub + (mL > 0 ? mL : -mL) - 1)];
+          const T sg = mL > 0 ? (T)1 : (T)-1;
+          ar += sg * b.x;
+          ai += sg * cj * b.y;
+        }
+        if (mR != 0) {
+          const V b = s_B[2 * (ub + (mR > 0 ? mR : -mR) - 1) + 1];
+          const T sg = mR > 0 ? (T)1 : (T)-1;
+          ar += sg * b.x;
+          ai += sg * cj * b.y;
+        }
+      }
+    }
+    red_r[threadIdx.x] = ar;
+    red_i[threadIdx.x] = ai;
+    __syncthreads();
+    for (int act = S; act > 1;) {
+      const int half = (act + 1) >> 1;
+      if (slice < act - half) {
+        red_r[threadIdx.x] += red_r[threadIdx.x + half * w];
+        red_i[threadIdx.x] += red_i[threadIdx.x + half * w];
+      }
+      __syncthreads();
+      act = half;
+    }
+    if (threadIdx.x < w) {
+      T sr = red_r[threadIdx.x], si = red_i[threadIdx.x];
+      if (scaled) jacobi_scale<T>(op, gr, gi, csr, csi, sr, si);
+      V o;
+      o.x = sr;
+      o.y = si;
+      reinterpret_cast<V*>(y)[base + t] = o;
+    }
+    __syncthreads();
+  }
+  if (a == 0) tier0_fixup_block<T>(op, x, y, rdo);   // block-uniform branch
+}
+
+// Top tier, sector-batched: a block takes 256 / nnz top-tier ADOs of one
+// sector, one lane per (ADO, slot).  Each lane loads its ADO's first element,
+// rate and couplings from the block's own tables (one level below the block
+// descriptor), then every coupling's map value and record, then the lower-tier
+// elements, and sums the couplings before any barrier; only its own element
+// of x goes through LDS, where the sector's commutator rows (staged once per
+// block) read their neighbours.  Four levels of dependent loads in all.
+constexpr size_t kLOffHc = 256 * 2, kLOffX = kLOffHc + 256 * 16, kLSmem = kLOffX + 256 * 16;
+
+template <typename T, bool TRANS>
+__device__ inline void light_sector(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x,
+                                    T* __restrict__ y, int64_t zero_at, int bid, char* smem, bool scaled) {
+  using V = typename P2<T>::type;
+  int16_t* s_hq = reinterpret_cast<int16_t*>(smem);
+  double2* s_hc = reinterpret_cast<double2*>(smem + kLOffHc);
+  V* s_x = reinterpret_cast<V*>(smem + kLOffX);
+  const V* __restrict__ xv = reinterpret_cast<const V*>(x);
+  const int4 bd = op.lt_blk[bid];
+  const int cnt = bd.y, nnz = bd.z, s = bd.w, U = op.lt_u;
+  const int dd = op.d * op.d;
+  const int W = op.hell_w;
+  const bool hst = W > 0 && nnz * W <= 256;   // the sector's commutator rows fit the LDS table
+  if (hst)
+    for (int k = threadIdx.x; k < nnz * W; k += 256) {
+      const int row = k / W, kk = k - row * W;
+      const int64_t r0 = (int64_t)s * dd + row;
+      s_hq[k] = op.hq_ell[r0 * op.hell_wp + kk];
+      s_hc[k] = reinterpret_cast<const double2*>(cf_p.hc_ell)[r0 * W + kk];
+    }
+  const int j = threadIdx.x / nnz, t = threadIdx.x - j * nnz;   // nnz >= 1 (host check)
+  const bool act = j < cnt;
+  const int64_t i = (int64_t)bd.x + j;
+  int64_t base = 0;
+  T ar = 0, ai = 0, gr = 0, gi = 0;
+  if (act) {
+    base = op.lt_off[i];
+    const int64_t a = op.lt_id[i];
+    s_x[threadIdx.x] = ld<T, TRANS>(xv, base + t, zero_at);
+    gr = (T)cf_p.gam[2 * a];
+    gi = (T)cf_p.gam[2 * a + 1];
+    // the lower-tier couplings, two at a time: entries, then map values and
+    // records, then the four elements, then the sums
+    const DevEntryP* en = op.lt_ent + i * U;
+    for (int r0 = 0; r0 < U; r0 += 2) {
+      DevEntryP e[2];
+      int mv[2];
+      CoefLR cf[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) e[q] = r0 + q < U ? en[r0 + q] : DevEntryP{0, -1, -1, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        mv[q] = e[q].map >= 0 ? op.maps[e[q].map + t] : 0;
+        cf[q] = e[q].map >= 0 ? load_coef(cf_p.rec, e[q].rec) : CoefLR{};
+      }
+      V vl[2], vr[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int mL = (int)(int16_t)(mv[q] & 0xffff), mR = (int)(int16_t)((uint32_t)mv[q] >> 16);
+        vl[q] = mL != 0 ? ld<T, TRANS>(xv, (int64_t)e[q].base + (mL > 0 ? mL : -mL) - 1, zero_at) : V{0, 0};
+        vr[q] = mR != 0 ? ld<T, TRANS>(xv, (int64_t)e[q].base + (mR > 0 ? mR : -mR) - 1, zero_at) : V{0, 0};
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int mL = (int)(int16_t)(mv[q] & 0xffff), mR = (int)(int16_t)((uint32_t)mv[q] >> 16);
+        coupling_accum<T>(mL, mR, vl[q], vr[q], e[q].flags, cf[q], ar, ai);
+      }
+    }
+  }
+  __syncthreads();
+  if (!act) return;   // after the only barrier
+  T csr = 0, csi = 0;
+  const V* xw = s_x + j * nnz;   // this ADO's elements
+  if (hst) {
+    const V xe = xw[t];
+    ar += gr * xe.x - gi * xe.y;
+    ai += gr * xe.y + gi * xe.x;
+    T cr = 0, ci = 0;
+    for (int k = 0; k < W; ++k) {
+      const int q = s_hq[t * W + k];
+      const double2 h = s_hc[t * W + k];
+      const V v = xw[q];
+      cr += (T)h.x * v.x - (T)h.y * v.y;
+      ci += (T)h.x * v.y + (T)h.y * v.x;
+      if (q == t) { csr += (T)h.x; csi += (T)h.y; }
+    }
+    const T hs = (T)op.hsign;
+    ar += hs * cr;
+    ai += hs * ci;
+  } else {
+    T dr, di;
+    diag_term<T, TRANS>(op, cf_p.hc_ell, xv, gr, gi, base, s, t, zero_at, dr, di, xw, base, nnz, csr, csi);
+    ar += dr;
+    ai += di;
+  }
+  if (scaled) jacobi_scale<T>(op, gr, gi, csr, csi, ar, ai);
+  V o;
+  o.x = ar;
+  o.y = ai;
+  reinterpret_cast<V*>(y)[base + t] = o;
+}
+
 // One launch for the whole apply: blocks [0, n_heavy) take the ADOs below the
 // top tier (dispatched first: their work per block is the longest), the rest
 // the top tier's elements, so the two phases overlap on the device.  The LDS
 // buffer allows 7 blocks per CU; the register budget is held to the same 7
 // waves per SIMD (72 VGPRs).
+constexpr size_t kApplySmem1 = kGSmem > kLSmem ? kGSmem : kLSmem;
+constexpr size_t kApplySmem = kApplySmem0 > kApplySmem1 ? kApplySmem0 : kApplySmem1;
+
 template <typename T, bool TRANS, int U>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
 k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
@@ -403,10 +661,17 @@ k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, i
   x += 2 * p * op.vstride;
   y += 2 * p * op.vstride;
   const Coefs cf{op.rec + p * op.rec_stride, op.gam + 2 * p * op.gam_stride, op.hc_ell + 2 * p * op.hc_stride};
-  if ((int64_t)blockIdx.x < op.n_heavy)   // block-uniform
-    heavy_body<T, TRANS>(op, cf, x, y, zero_at, rdo, (int)blockIdx.x, smem, scaled);
-  else
+  if ((int64_t)blockIdx.x < op.n_heavy) {   // block-uniform
+    const int a = (int)blockIdx.x;
+    if (op.hg_off && op.hg_off[a + 1] > op.hg_off[a])
+      heavy_grouped<T, TRANS>(op, cf, x, y, zero_at, rdo, a, smem, scaled);
+    else
+      heavy_body<T, TRANS>(op, cf, x, y, zero_at, rdo, a, smem, scaled);
+  } else if (op.lt_blk) {
+    light_sector<T, TRANS>(op, cf, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem, scaled);
+  } else {
     light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem, scaled);
+  }
 }
 
 // Tier-0 fix-ups (one block): trace row (forward: y(d,d) = tr x0; transposed:
@@ -852,7 +1117,7 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
   if (op.total <= 0) return hipSuccess;
   const int64_t zero_at = (op.transposed && rdo) ? op.trace_slot : -1;   // ADO 0 starts at packed offset 0
   const int64_t light = op.total - op.light_begin;
-  const int64_t blocks = op.n_heavy + (light > 0 ? (light + 255) / 256 : 0);
+  const int64_t blocks = op.n_heavy + (op.lt_blk ? op.n_lt_blocks : (light > 0 ? (light + 255) / 256 : 0));
   if (blocks > 0) {
     const dim3 grid((unsigned)blocks, (unsigned)op.nprob);
     const int u = op.max_light_couplings;

@@ -817,7 +817,10 @@ void observables_host(const heomq_ctx* c, const std::vector<double>& h, double* 
           const C xv = c->tier1_dag[k] ? (double)c->tier1_fact[k] * std::conj(X(a, i, kk)) : X(a, kk, i);
           tr += cik * xv;
         }
-      current[c->alf[k] * c->nspin + c->spin[k]] += -2.0 * tr.imag() * kJconst;   // td/calcj.f90:28-37,68
+      // lscale: the stored tier-1 ADO is rho / |cb|^(1/2) of its own drawer
+      // (k, or its sign-flipped partner when stored as the Hermitian partner)
+      const int ks = c->tier1_dag[k] ? (k < c->M / 2 ? k + c->M / 2 : k - c->M / 2) : k;
+      current[c->alf[k] * c->nspin + c->spin[k]] += -2.0 * tr.imag() * kJconst * c->dbsqrt[ks];   // td/calcj.f90:28-37,68
     }
   }
   if (occ)

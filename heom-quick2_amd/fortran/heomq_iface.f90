@@ -46,7 +46,7 @@ module heomq_iface
 
     integer(c_int) function c_model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igs, &
         ntable, lall, indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
-        indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, out) &
+        indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, dbsqrt, out) &
         bind(C, name='heomq_model_from_reference')
       import :: c_int, c_int32_t, c_int64_t, c_int8_t, c_double, c_double_complex, c_ptr
       integer(c_int32_t), value :: nrho, ntier, norbs, nspin, nalf, ncor, igs
@@ -56,6 +56,7 @@ module heomq_iface
       integer(c_int8_t) :: itypecoef(*), ifactfcoef(*), ifactrcoef(*)
       complex(c_double_complex) :: cb(*), cd(*), cgamma(*), hs(*)
       real(c_double) :: dipm(*), eleadinfty(*), amsall(*)
+      type(c_ptr), value :: dbsqrt
       type(c_ptr) :: out
     end function
 
@@ -194,10 +195,12 @@ contains
 
 end module heomq_iface
 
-! One-time upload of the reference's module arrays (module/matmod.f90).
+! One-time upload of the reference's module arrays (module/matmod.f90); with
+! lscale /= 0, dbsqrt(norbs,nspin,ncor,nalf,nsgn) of main/evaluatepara.f90:2931-2960
+! (cb and cd then the scaled ones), else dbsqrt is not read.
 subroutine heomq_fortran_init(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady, &
     ntable, lall, indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
-    indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, device)
+    indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, lscale, dbsqrt, device)
   use heomq_iface
   implicit none
   integer(c_int32_t), intent(in) :: nrho, ntier, norbs, nspin, nalf, ncor, igroundsteady, device
@@ -207,10 +210,15 @@ subroutine heomq_fortran_init(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igrou
   integer(c_int8_t), intent(in) :: itypecoef(*), ifactfcoef(*), ifactrcoef(*)
   complex(c_double_complex), intent(in) :: cb(*), cd(*), cgamma(*), hs(*)
   real(c_double), intent(in) :: dipm(*), eleadinfty(*), amsall(*)
+  integer(c_int32_t), intent(in) :: lscale
+  real(c_double), intent(in), target :: dbsqrt(*)
   integer(c_int) :: rc
+  type(c_ptr) :: pdb
+  pdb = c_null_ptr
+  if (lscale /= 0) pdb = c_loc(dbsqrt)
   rc = c_model_from_reference(nrho, nunk, ntier, norbs, nspin, nalf, ncor, igroundsteady, &
        ntable, lall, indextable, nfirst, nlast, ifirst, indexcoef, itypecoef, ifactfcoef, ifactrcoef, index_coef_ref, &
-       indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, heomq_model_h)
+       indexdraw, cb, cd, cgamma, dipm, ilead, mspin, morbs, mpm, eleadinfty, hs, amsall, pdb, heomq_model_h)
   if (rc /= 0) call heomq_fail('heomq_fortran_init (tables)', rc)
   rc = c_ctx_create(heomq_model_h, int(device, c_int), heomq_ctx_h)
   if (rc /= 0) call heomq_fail('heomq_fortran_init (device)', rc)

@@ -41,11 +41,13 @@ class Params:
     hpar: tuple = ()
     lsdraw: bool = False
     steady: bool = True
+    lscale: bool = False        # ADOs stored scaled by prod_k |cb_k|^(-1/2) (the reference's lscale)
 
     def to_c(self) -> _lib.CParams:
         p = _lib.CParams()
         p.norbs, p.nspin, p.nalf, p.nmats, p.ntop = self.norbs, 2, self.nalf, self.nmats, self.ntop
         p.lsdraw, p.ham, p.steady = int(self.lsdraw), int(self.ham), int(self.steady)
+        p.lscale = int(self.lscale)
         for a in range(min(self.nalf, 4)):
             p.bandwidth[a] = self.bandwidth[a]
             p.temperature[a] = self.temperature[a]
@@ -59,22 +61,22 @@ class Params:
 
 
 def siam(eps=-1.0, U=2.0, gamma=0.1, W=10.0, T=0.05, V=0.0, nk=8, ntop=3, gl=None, gr=None,
-         lsdraw=False) -> Params:
+         lsdraw=False, lscale=False) -> Params:
     """Single-impurity Anderson model; bias split +V/2 on the left lead, -V/2 on the right."""
     gl = gamma if gl is None else gl
     gr = gamma if gr is None else gr
     return Params(norbs=1, nalf=2, nmats=nk, ntop=ntop, bandwidth=(W, W), linewidth=((gl,), (gr,)),
                   temperature=(T, T), engyshift=((V / 2, V / 2), (-V / 2, -V / 2)), ham=HAM_SIAM,
-                  hpar=(eps, eps, U, 0.0), lsdraw=lsdraw)
+                  hpar=(eps, eps, U, 0.0), lsdraw=lsdraw, lscale=lscale)
 
 
 def two_impurity(eps=-1.5, U=3.0, t=0.5, u12=0.0, gamma=0.2, W=10.0, T=0.1, V=0.4, nk=6, ntop=3,
-                 lsdraw=False) -> Params:
+                 lsdraw=False, lscale=False) -> Params:
     """Two-impurity model ($para4): both dots coupled to both leads with the same Gamma."""
     return Params(norbs=2, nalf=2, nmats=nk, ntop=ntop, bandwidth=(W, W),
                   linewidth=((gamma, gamma), (gamma, gamma)), temperature=(T, T),
                   engyshift=((V / 2, V / 2), (-V / 2, -V / 2)), ham=HAM_DQD,
-                  hpar=(eps, eps, eps, eps, U, U, u12, t), lsdraw=lsdraw)
+                  hpar=(eps, eps, eps, eps, U, U, u12, t), lsdraw=lsdraw, lscale=lscale)
 
 
 class Batch:
@@ -196,7 +198,8 @@ class Model:
         """Build from the reference's own module arrays (see include/heomq.h).
 
         ``ntable`` and ``lall`` (the lengths the reference keeps in its common
-        block info3) default to the lengths of ``indextable`` and ``indexcoef``.
+        block info3) default to the lengths of ``indextable`` and ``indexcoef``;
+        ``dbsqrt`` (the reference's lscale factors) is optional.
         """
         a = arrays
         h = C.c_void_p()
@@ -221,7 +224,8 @@ class Model:
             ptr("ilead", np.int32, _lib.I32), ptr("mspin", np.int32, _lib.I32),
             ptr("morbs", np.int32, _lib.I32), ptr("mpm", np.int32, _lib.I32),
             ptr("eleadinfty", np.float64, _lib.D), ptr("hs", np.complex128, _lib.D),
-            ptr("amsall", np.float64, _lib.D), C.byref(h)))
+            ptr("amsall", np.float64, _lib.D),
+            ptr("dbsqrt", np.float64, _lib.D) if a.get("dbsqrt") is not None else None, C.byref(h)))
         return cls(None, _handle=h)
 
     def set_params(self, params: Params):
@@ -346,6 +350,15 @@ class Context:
         """Two block-Jacobi sweeps on the start of block-Jacobi-preconditioned
         steady solves (default on)."""
         check(lib().heomq_ctx_set_presmooth(self._h, int(enable)))
+
+    def autotune(self, reps: int = 20) -> dict:
+        """Time the packed apply's four forms on this GPU and keep the fastest
+        per operator; returns the choices and times (us)."""
+        forms = np.zeros(2, np.int32)
+        times = np.zeros(8)
+        check(lib().heomq_ctx_autotune(self._h, int(reps), forms.ctypes.data_as(_lib.I32), _dp(times)))
+        return dict(forward=int(forms[0]), transposed=int(forms[1]),
+                    times_us={"forward": times[:4].tolist(), "transposed": times[4:].tolist()})
 
     def set_graphs(self, enable: bool):
         """Replay solver iterations from captured graphs (default) or launch eagerly."""

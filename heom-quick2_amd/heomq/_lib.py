@@ -28,6 +28,7 @@ class CParams(C.Structure):
     _fields_ = [
         ("norbs", C.c_int32), ("nspin", C.c_int32), ("nalf", C.c_int32), ("nmats", C.c_int32),
         ("ntop", C.c_int32), ("lsdraw", C.c_int32), ("ham", C.c_int32), ("steady", C.c_int32),
+        ("lscale", C.c_int32), ("reserved", C.c_int32),
         ("bandwidth", C.c_double * 4), ("linewidth", (C.c_double * 4) * 4),
         ("temperature", C.c_double * 4), ("engyshift", (C.c_double * 2) * 4), ("hpar", C.c_double * 16),
     ]
@@ -55,12 +56,13 @@ SIGNATURES = {
     "heomq_model_from_reference": (C.c_int, [
         C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
         C.c_int64, C.c_int64, I32, I64, I64, I64, I64, I8, I8, I8, I64, I32, D, D, D, D, I32, I32, I32, I32, D, D, D,
-        C.POINTER(P)]),
+        D, C.POINTER(P)]),
     "heomq_ctx_create": (C.c_int, [P, C.c_int, C.POINTER(P)]),
     "heomq_ctx_free": (None, [P]),
     "heomq_ctx_refresh": (C.c_int, [P, P]),
     "heomq_ctx_vector_len": (C.c_int64, [P]),
     "heomq_ctx_set_graphs": (C.c_int, [P, C.c_int]),
+    "heomq_ctx_autotune": (C.c_int, [P, C.c_int, I32, D]),
     "heomq_ctx_set_precond": (C.c_int, [P, C.c_int]),
     "heomq_ctx_set_precond_shift": (C.c_int, [P, C.c_double]),
     "heomq_ctx_set_shadow_seed": (C.c_int, [P, C.c_uint64]),

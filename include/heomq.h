@@ -66,6 +66,11 @@ typedef struct heomq_params {
   int32_t lsdraw;               /* 1: keep ADOs with repeated drawers     */
   int32_t ham;                  /* HEOMQ_HAM_*                            */
   int32_t steady;               /* 1: bias enters gamma (igroundsteady=1) */
+  int32_t lscale;               /* 1: ADOs stored scaled by prod_k |cb_k|^(-1/2) ($jobinfo
+                                   lscale, main/evaluatepara.f90:2927-2990): cb, cd divided by
+                                   |cb|^(1/2), every upper coupling times |cb_e|^(1/2); the
+                                   vectors of every entry point are then the scaled ones      */
+  int32_t reserved;             /* 0                                       */
   double bandwidth[4];          /* W per lead                              */
   double linewidth[4][4];       /* Gamma[lead][orbital]                    */
   double temperature[4];        /* k_B T per lead                          */
@@ -127,7 +132,7 @@ int heomq_model_set_params(heomq_model* m, const heomq_params* p);
  * indexdraw (position of each drawer in tier 1), cb/cd (complex,
  * (norbs,nspin,ncor,nalf,nsgn)), cgamma(nvar), dipm(nvar), ilead/mspin/morbs/
  * mpm (nvar), eleadinfty(nalf,nspin), hs(nrho,nrho), amsall(nrho,nrho,norbs,
- * nspin).  Only the default-flag subset is accepted (psdjob, no lscale,
+ * nspin).  Only the default-flag subset plus lscale is accepted (psdjob, no
  * offcor, lscreen, lsimple, lfilter, ltrun_der, lhf, lhb): the shim rejects
  * the rest with HEOMQ_UNSUPPORTED as the reference would need them.
  * ntable and lall are the lengths of indextable and of the coupling stream
@@ -135,6 +140,9 @@ int heomq_model_set_params(heomq_model* m, const heomq_params* p);
  * include/common:41-47 (indextable(ntable), main/main.f90:537;
  * indexcoef(lall), main/buildcoefindex.f90:608): every position the tables
  * name is checked against them, and one outside is HEOMQ_BAD_ARG.
+ * dbsqrt: with the reference's lscale, its dbsqrt(norbs,nspin,ncor,nalf,nsgn)
+ * (main/evaluatepara.f90:2931-2960; cb and cd are then the scaled ones it
+ * holds), which scales every upper coupling; NULL without lscale.
  * ------------------------------------------------------------------------- */
 int heomq_model_from_reference(
     int32_t nrho, int64_t nunk, int32_t ntier, int32_t norbs, int32_t nspin,
@@ -146,7 +154,7 @@ int heomq_model_from_reference(
     const double* cb, const double* cd, const double* cgamma,
     const double* dipm, const int32_t* ilead, const int32_t* mspin,
     const int32_t* morbs, const int32_t* mpm, const double* eleadinfty,
-    const double* hs, const double* amsall, heomq_model** out);
+    const double* hs, const double* amsall, const double* dbsqrt, heomq_model** out);
 
 /* ---------------------------------------------------------------------------
  * Device context: uploads the tables once (heomq_ctx_load in the reference's
@@ -182,6 +190,16 @@ int heomq_ctx_set_precond_shift(heomq_ctx* c, double shift);
  * (restarts take the following seeds); default 0.  The reference draws its
  * shadow vector with DRANDN, seed 1 (tfqmr_double/dutfx.f90:178). */
 int heomq_ctx_set_shadow_seed(heomq_ctx* c, uint64_t seed);
+/* Autotuner of the packed apply: times each of its four forms (bit 0: ADOs
+ * below the top tier with their couplings grouped by gather map, summed in the
+ * neighbour's slot space before the map; bit 1: top-tier blocks batched by
+ * symmetry sector; 0: one workgroup per ADO with the couplings sliced over the
+ * lanes, top tier in packed-element order) over `reps` launches on this GPU,
+ * for the forward and the transposed operator, and keeps the fastest.
+ * forms[0..1] (may be NULL) receive the choices, times_us[0..7] (may be
+ * NULL) the times, form f of operator w at [4 w + f].  Every form computes
+ * the same operator.  No counterpart in the reference. */
+int heomq_ctx_autotune(heomq_ctx* c, int reps, int32_t* forms, double* times_us);
 /* Number of complex elements N*d*d of one ADO vector. */
 int64_t heomq_ctx_vector_len(const heomq_ctx* c);
 

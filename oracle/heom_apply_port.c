@@ -22,6 +22,7 @@
 #include <complex.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <math.h>
 #include <string.h>
 
 typedef double _Complex cplx;
@@ -46,6 +47,8 @@ typedef struct {
   const int32_t* sgn;             /* per drawer */
   const int32_t* spin;
   const int32_t* orb;
+  const double* dbsqrt;           /* lscale: |cb_k|^(1/2) per drawer on every upper coupling (:120-122,
+                                     :325-328); null without lscale */
 } heom_port_tables;
 
 /* one c operator as its +-1 entries (row, column, sign) */
@@ -137,13 +140,16 @@ void heom_port_apply(const heom_port_tables* t, const double* xr, double* yr, in
         for (int e = 0; e < M; ++e, ++p) {   /* upper tier: drawer e added */
           if (t->indexcoef[p] <= 0) continue;
           const int64_t nb = t->indexcoef[p] - 1;
-          couple(d, &uop[e], x + nb * d * d, t->itypecoef[p], -I * t->ifactfcoef[p], I * t->ifactrcoef[p], Y);
+          const double sc = t->dbsqrt ? t->dbsqrt[e] : 1.0;
+          couple(d, &uop[e], x + nb * d * d, t->itypecoef[p], -I * sc * t->ifactfcoef[p], I * sc * t->ifactrcoef[p],
+                 Y);
         }
     } else if (L > 0) {
       for (int e = 0; e < M; ++e) {   /* tier 0: the tier-1 ADO of drawer e, or of its partner */
         const int pe = e < half ? e : e - half;
         const int64_t nb = (t->nfirst[1] - 1) + t->indexdraw[pe] - 1;
-        couple(d, &uop[e], x + nb * d * d, e < half ? 0 : 1, -I, I, Y);
+        const double sc = t->dbsqrt ? t->dbsqrt[e] : 1.0;
+        couple(d, &uop[e], x + nb * d * d, e < half ? 0 : 1, -I * sc, I * sc, Y);
       }
     }
   }
@@ -166,3 +172,87 @@ void heom_port_set_threads(int n) { omp_set_num_threads(n); }
 int heom_port_threads(void) { return 1; }
 void heom_port_set_threads(int n) { (void)n; }
 #endif
+
+/*
+ * TFQMR iterations (Freund, SIAM J. Sci. Comput. 14 (1993) 470) on A x = b
+ * over the real (Re, Im) vector of n = 2 N d^2 doubles, with the work the
+ * reference does per iteration of its reverse-communication loop
+ * (tfqmr_double/solve_tfqmr.f90:194-280 around dutfx.f90:196-357): two
+ * applies, nine axpby passes, two dot products and two norms.  par = 0 runs
+ * the vector passes on one thread, as dutfx.f90, daxpby.f90, ddot_long.f90
+ * and dnrm2_long.f90 do (they have no OpenMP); par = 1 spreads them over the
+ * OpenMP threads.  init = 1 starts the recurrences (x = 0, the shadow
+ * vector +-1 from a fixed hash -- the reference draws it with DRANDN once per
+ * solve -- and one apply); init = 0 resumes them from work and state.  Then
+ * nit iterations advance x.  work holds 8 vectors of n doubles, state the
+ * scalars (tau, theta, eta, rho, m).  Returns the quasi-residual bound
+ * tau sqrt(2 m + 1) after the last iteration.
+ */
+static double vdot(int64_t n, const double* a, const double* b, int par) {
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) if (par)
+  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+/* z = a x + b y (daxpby) */
+static void vaxpby(int64_t n, double* z, double a, const double* x, double b, const double* y, int par) {
+#pragma omp parallel for if (par)
+  for (int64_t i = 0; i < n; ++i) z[i] = a * x[i] + b * y[i];
+}
+
+double heom_port_tfqmr(const heom_port_tables* t, const double* b, double* x, double* work, double* state,
+                       int nit, int par, int trace_mode, int init) {
+  const int64_t n = 2 * t->nunk * (int64_t)t->nrho * t->nrho;
+  double *w = work, *y1 = work + n, *y2 = work + 2 * n, *u1 = work + 3 * n, *u2 = work + 4 * n;
+  double *v = work + 5 * n, *dv = work + 6 * n, *rt = work + 7 * n;
+  if (init) {
+    for (int64_t i = 0; i < n; ++i) {
+      uint64_t z = 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+      z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
+      rt[i] = ((z >> 40) & 1) ? 1.0 : -1.0;
+      x[i] = 0.0;
+      w[i] = b[i];
+      y1[i] = b[i];
+      dv[i] = 0.0;
+    }
+    heom_port_apply(t, y1, v, trace_mode);
+    memcpy(u1, v, sizeof(double) * (size_t)n);
+    state[0] = sqrt(vdot(n, w, w, par));
+    state[1] = state[2] = 0.0;
+    state[3] = vdot(n, rt, w, par);
+    state[4] = 0.0;
+  }
+  double tau = state[0], theta = state[1], eta = state[2], rho = state[3];
+  int m = (int)state[4];
+  for (int it = 0; it < nit; ++it) {
+    const double sigma = vdot(n, rt, v, par);
+    const double alpha = rho / sigma;
+    vaxpby(n, y2, 1.0, y1, -alpha, v, par);
+    heom_port_apply(t, y2, u2, trace_mode);
+    for (int j = 0; j < 2; ++j) {
+      const double* yj = j ? y2 : y1;
+      const double* uj = j ? u2 : u1;
+      vaxpby(n, w, 1.0, w, -alpha, uj, par);
+      vaxpby(n, dv, 1.0, yj, theta * theta * eta / alpha, dv, par);
+      theta = sqrt(vdot(n, w, w, par)) / tau;
+      const double c = 1.0 / sqrt(1.0 + theta * theta);
+      tau = tau * theta * c;
+      eta = c * c * alpha;
+      vaxpby(n, x, 1.0, x, eta, dv, par);
+    }
+    ++m;
+    const double rho_n = vdot(n, rt, w, par);
+    const double beta = rho_n / rho;
+    rho = rho_n;
+    vaxpby(n, y1, 1.0, w, beta, y2, par);
+    heom_port_apply(t, y1, u1, trace_mode);
+#pragma omp parallel for if (par)
+    for (int64_t i = 0; i < n; ++i) v[i] = u1[i] + beta * (u2[i] + beta * v[i]);
+  }
+  state[0] = tau;
+  state[1] = theta;
+  state[2] = eta;
+  state[3] = rho;
+  state[4] = m;
+  return tau * sqrt(2.0 * m + 1.0);
+}

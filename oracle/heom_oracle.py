@@ -97,6 +97,7 @@ class Params:
     ntop: int = 3                       # top tier L (reference ntier = L+1)
     lsdraw: bool = False                # allow repeated drawers (reference default True)
     steady: bool = True                 # igroundsteady=1: bias enters gamma
+    lscale: bool = False                # ADOs stored scaled by prod_k |cb_k|^(-1/2) ($jobinfo lscale)
 
 
 def siam_params(eps=-1.0, U=2.0, gamma=0.1, W=10.0, T=0.05, V=0.0, nk=8, ntop=3,
@@ -312,6 +313,17 @@ class Model:
                 cg[k] = dpm * 1j * T * pole / HBAR
                 cb[k] = 1j * 2.0 * coef * zxip * T / ((pole * T - 0.0) ** 2 + W ** 2) / HBAR ** 2
                 cd[k] = np.conj(1j * 2.0 * coef * zxip * T) / np.conj((pole_b * T - 0.0) ** 2 + W ** 2) / HBAR ** 2
+        # lscale (evaluatepara.f90:2927-2990): every ADO is stored as
+        # rho_K / prod_{k in K} |cb_k|^(1/2); dbsqrt_k = |cb_k|^(1/2) of the
+        # unscaled cb, cb and cd divided by it (1/dbsqrt = 0 where |cb| <=
+        # dpico = 1e-12), and every upper coupling multiplied by it
+        # (calc_ax_tfqmr_omp.f90:120-122,325-328,388-390)
+        self.dbsqrt = np.ones(self.M)
+        if p.lscale:
+            self.dbsqrt = np.sqrt(np.abs(cb))
+            inv = np.where(np.abs(cb) > 1e-12, 1.0 / np.where(self.dbsqrt > 0, self.dbsqrt, 1.0), 0.0)
+            cb = cb * inv
+            cd = cd * inv
         self.cb, self.cd, self.cgamma = cb, cd, cg
         self.eleadinfty = np.array(p.engyshift, dtype=float) / HBAR   # fieldtype 0, t -> inf
         gk = cg.copy()
@@ -476,8 +488,8 @@ def apply(model: Model, x, trace_row="last", H=None, gdraw=None, cb=None, cd=Non
                 Xn[m] = np.conj(np.transpose(Xn[m], (0, 2, 1)))
                 c = model.ams[model.orb[e]][model.spin[e]]
                 op = c if model.sgn[e] == 0 else c.T                  # :278-279
-                ff = upf[:, e, 0][:, None, None]
-                fr = upf[:, e, 1][:, None, None]
+                ff = upf[:, e, 0][:, None, None] * model.dbsqrt[e]   # lscale, :325-328
+                fr = upf[:, e, 1][:, None, None] * model.dbsqrt[e]
                 yt += -1j * ff * (op @ Xn) + 1j * fr * (Xn @ op)
         y[s0:s1] = yt
     if trace_row == "last":
@@ -537,8 +549,8 @@ def apply_transpose(model: Model, w, trace_row="last"):
                     continue
                 c = model.ams[model.orb[e]][model.spin[e]]
                 op = c if model.sgn[e] == 0 else c.T
-                ff = upf[:, e, 0][:, None, None]
-                fr = upf[:, e, 1][:, None, None]
+                ff = upf[:, e, 0][:, None, None] * model.dbsqrt[e]
+                fr = upf[:, e, 1][:, None, None] * model.dbsqrt[e]
                 a, b = -1j * ff, 1j * fr
                 direct = np.conj(a) * (op.T @ W) + np.conj(b) * (W @ op.T)
                 dag = a * (Wh @ op) + b * (op @ Wh)
@@ -587,7 +599,9 @@ def current(model: Model, x):
         nb, it, fact = model.lookup((k,))
         X = x[nb] if it == 0 else fact * x[nb].conj().T
         c = model.ams[model.orb[k]][model.spin[k]]
-        I[model.alf[k], model.spin[k]] += -2.0 * np.imag(np.trace(c @ X))
+        # lscale: the stored tier-1 ADO is rho / |cb|^(1/2) of its own drawer
+        ks = k if it == 0 else int(model.bar[k])
+        I[model.alf[k], model.spin[k]] += -2.0 * np.imag(np.trace(c @ X)) * model.dbsqrt[ks]
     return I * JCONST
 
 
@@ -1346,3 +1360,11 @@ def landauer_dqd(p: Params, npts=400001):
         for m in range(2):
             occ[m, s] = np.sum(wts * np.real(lesser[:, m, m]))
     return cur * JCONST, occ
+
+
+def ado_scale(model: Model):
+    """prod_{k in K} |cb_k|^(1/2) per ADO (1 without lscale): rho_K = s_K * stored_K."""
+    s = np.ones(model.N)
+    for n in range(1, model.L + 1):
+        s[model.first[n]:model.first[n + 1]] = np.prod(model.dbsqrt[model.tiers[n]], axis=1)
+    return s

@@ -40,7 +40,7 @@ class Tables(C.Structure):
                 ("nspin", C.c_int32), ("nunk", C.c_int64)] + \
                [(nm, C.c_void_p) for nm in ("indextable", "nfirst", "ifirst", "indexcoef", "itypecoef",
                                             "ifactfcoef", "ifactrcoef", "index_coef_ref", "indexdraw", "cb",
-                                            "cd", "gdraw", "hs", "ams", "sgn", "spin", "orb")]
+                                            "cd", "gdraw", "hs", "ams", "sgn", "spin", "orb", "dbsqrt")]
 
 
 class Port:
@@ -53,6 +53,9 @@ class Port:
         self.lib.heom_port_apply.argtypes = [C.POINTER(Tables), C.c_void_p, C.c_void_p, C.c_int]
         self.lib.heom_port_apply.restype = None
         self.lib.heom_port_threads.restype = C.c_int
+        self.lib.heom_port_tfqmr.argtypes = [C.POINTER(Tables), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_int, C.c_int, C.c_int, C.c_int]
+        self.lib.heom_port_tfqmr.restype = C.c_double
         self.model = model
         arr = reference_arrays(model)
         keep = {}
@@ -81,6 +84,7 @@ class Port:
         t.sgn = ptr("sgn", model.sgn, np.int32)
         t.spin = ptr("spin", model.spin, np.int32)
         t.orb = ptr("orb", model.orb, np.int32)
+        t.dbsqrt = ptr("dbsqrt", model.dbsqrt, np.float64) if model.p.lscale else None
         self._keep = keep
         self.tables = t
 
@@ -99,3 +103,29 @@ class Port:
         """The same on arrays already in the reference layout (no conversion; for timing)."""
         mode = {None: 0, "last": 1, "rdo": 2}[trace_row]
         self.lib.heom_port_apply(C.byref(self.tables), xf.ctypes.data, yf.ctypes.data, mode)
+
+    def steady_rhs(self, trace_row="last"):
+        """(x0, b) of solve_tfqmr.f90:97-179 in the reference layout as real
+        vectors: x0 = identity / d on tier 0, b = e_trace - A x0."""
+        m = self.model
+        d = m.d
+        x0 = np.zeros((m.N, d, d), complex)
+        x0[0] = np.eye(d) / d
+        e = np.zeros_like(x0)
+        e[-1 if trace_row == "last" else 0, d - 1, d - 1] = 1.0
+        b = O.to_fortran(e - self.apply(x0, trace_row=trace_row))
+        return O.to_fortran(x0).view(np.float64).ravel(), np.ascontiguousarray(b).view(np.float64).ravel()
+
+    def tfqmr(self, b, nit, parallel=False, trace_row="last", x=None, work=None, state=None, resume=False):
+        """nit TFQMR iterations on A dx = b (real vectors in the reference
+        layout), the vector passes serial (the reference's DUTFX) or
+        OpenMP-parallel; from dx = 0, or (resume) continuing the recurrences
+        kept in x, work and state.  Returns (dx, quasi-residual bound)."""
+        mode = {None: 0, "last": 1, "rdo": 2}[trace_row]
+        n = b.size
+        x = np.zeros(n) if x is None else x
+        work = np.zeros(8 * n) if work is None else work
+        state = np.zeros(5) if state is None else state
+        bound = self.lib.heom_port_tfqmr(C.byref(self.tables), b.ctypes.data, x.ctypes.data, work.ctypes.data,
+                                         state.ctypes.data, int(nit), int(bool(parallel)), mode, int(not resume))
+        return x, bound

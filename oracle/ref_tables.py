@@ -31,7 +31,7 @@ def reference_arrays(o):
                 ff.append(lof[a, r, 0]); fr.append(lof[a, r, 1])
             if up is not None:
                 for e in range(M):
-                    indexcoef.append(up[0][a, e, 0] + 1); itype.append(up[0][a, e, 1])
+                    indexcoef.append(up[0][a, e, 0] + 1); itype.append(max(up[0][a, e, 1], 0))
                     ff.append(up[1][a, e, 0]); fr.append(up[1][a, e, 1])
     half = M // 2
     indexdraw = np.zeros(half, np.int32)
@@ -42,9 +42,11 @@ def reference_arrays(o):
     norbs, nspin, ncor, nalf = o.norbs, 2, o.ncor, o.nalf
     cb = np.zeros((2, nalf, ncor, nspin, norbs), complex)
     cd = np.zeros_like(cb)
+    dbs = np.zeros((2, nalf, ncor, nspin, norbs))
     for k in range(M):
         cb[o.sgn[k], o.alf[k], o.cor[k], o.spin[k], o.orb[k]] = o.cb[k]
         cd[o.sgn[k], o.alf[k], o.cor[k], o.spin[k], o.orb[k]] = o.cd[k]
+        dbs[o.sgn[k], o.alf[k], o.cor[k], o.spin[k], o.orb[k]] = o.dbsqrt[k]
     ams = np.zeros((nspin, norbs, d, d))
     for m in range(norbs):
         for s in range(nspin):
@@ -58,4 +60,5 @@ def reference_arrays(o):
         cb=cb.ravel(), cd=cd.ravel(), cgamma=o.cgamma, dipm=o.dipm,
         ilead=(o.alf + 1).astype(np.int32), mspin=(o.spin + 1).astype(np.int32),
         morbs=(o.orb + 1).astype(np.int32), mpm=(o.sgn + 1).astype(np.int32),
-        eleadinfty=o.eleadinfty.T.ravel(), hs=o.H.T.ravel(), amsall=ams.ravel())
+        eleadinfty=o.eleadinfty.T.ravel(), hs=o.H.T.ravel(), amsall=ams.ravel(),
+        dbsqrt=dbs.ravel() if o.p.lscale else None)

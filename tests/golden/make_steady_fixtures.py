@@ -8,9 +8,15 @@ oracle by tests/test_oracle.py) -- by restarted GMRES (Saad & Schultz 1986;
 Givens rotations, modified Gram-Schmidt with one reorthogonalisation), right-
 preconditioned by the per-ADO inverse (shift - S_K)^-1 applied in the
 eigenbasis of H.  GMRES shares no code and no failure mode with the device
-TFQMR it checks.  The fixture holds the tier-0 block (RDO), the lead currents
-(td/calcj.f90 units) and the occupations (main/calcocc.f90), plus the true
-residual ||b - A x|| / ||b|| the solve reached.
+TFQMR it checks.  It stops at the tolerance or when a restart cycle no longer
+halves the true residual: the FP64 floor of the system, where the residual
+of the computed x is as large as the rounding error of forming A x.  The
+fixture holds the tier-0 block (RDO), the lead currents (td/calcj.f90 units)
+and the occupations (main/calcocc.f90), plus the true residual
+||b - A x|| / ||b|| the solve reached, the change of the RDO over the last
+restart cycle (its error estimate), and the rounding noise of the residual:
+||r_port - r_numpy|| / ||b|| for the same x, the residual formed by the C port
+and by the numpy oracle's apply (two summation orders of one operator).
 
     python tests/golden/make_steady_fixtures.py [name ...]
 """
@@ -37,15 +43,21 @@ from oracle.heom_port import Port  # noqa: E402
 WORKLOADS = {
     "siam_nk8_L3": dict(kind="siam", eps=-1.0, U=2.0, gamma=0.1, W=10.0, T=0.05, V=0.0, nk=8, ntop=3),
     "dqd_nk6_L3": dict(kind="dqd", eps=-1.5, U=3.0, t=0.5, gamma=0.2, W=10.0, T=0.1, V=0.4, nk=6, ntop=3),
+    # the same system with the ADOs stored scaled (the reference's lscale)
+    "dqd_nk6_L3_lscale": dict(kind="dqd", eps=-1.5, U=3.0, t=0.5, gamma=0.2, W=10.0, T=0.1, V=0.4, nk=6, ntop=3,
+                              lscale=True),
     "siam_nk8_L3_V-1": dict(kind="siam", eps=-1.0, U=2.0, gamma=0.1, W=10.0, T=0.05, V=-1.0, nk=8, ntop=3),
     "siam_nk8_L3_V0.5": dict(kind="siam", eps=-1.0, U=2.0, gamma=0.1, W=10.0, T=0.05, V=0.5, nk=8, ntop=3),
 }
 
 
 def params_of(w):
+    import dataclasses
     w = dict(w)
     kind = w.pop("kind")
-    return O.siam_params(**w) if kind == "siam" else O.dqd_params(**w)
+    lscale = w.pop("lscale", False)
+    p = O.siam_params(**w) if kind == "siam" else O.dqd_params(**w)
+    return dataclasses.replace(p, lscale=lscale)
 
 
 class Operator:
@@ -93,16 +105,22 @@ class Operator:
 
 
 def gmres(op, b, x0, tol, restart=60, maxit=3000, log=print):
-    """Right-preconditioned restarted GMRES on real vectors; stops on the true residual."""
+    """Right-preconditioned restarted GMRES on real vectors; stops on the true
+    residual (tolerance, or a cycle that does not halve it).  Returns (x, matvecs,
+    relres, x of the previous cycle)."""
     x = x0.copy()
+    xprev = x0.copy()
     nb = np.linalg.norm(b)
     total = 0
+    last = np.inf
     while True:
         r = b - op.matvec(x)
         beta = np.linalg.norm(r)
         log(f"  restart at {total} matvecs: true relres {beta / nb:.3e}")
-        if beta <= tol * nb or total >= maxit:
-            return x, total, beta / nb
+        if beta <= tol * nb or total >= maxit or beta > 0.5 * last:
+            return x, total, beta / nb, xprev
+        last = beta
+        xprev = x.copy()
         V = [r / beta]
         Z = []
         Hm = np.zeros((restart + 1, restart))
@@ -153,14 +171,20 @@ def solve(name, tol=1e-13, log=print):
     e = np.zeros_like(x0)
     e[0, d - 1, d - 1] = 1.0
     b = op.pack(e)
-    x, it, rel = gmres(op, b, op.pack(x0), tol, log=log)
+    x, it, rel, xprev = gmres(op, b, op.pack(x0), tol, log=log)
     xs = op.unpack(x)
     r = np.linalg.norm(b - op.matvec(x)) / np.linalg.norm(b)
+    rdo_change = float(np.abs(xs[0] - op.unpack(xprev)[0]).max())
+    # the same residual through the numpy oracle's apply (another summation order)
+    r_np = op.pack(O.apply(m, xs, trace_row="rdo"))
+    noise = float(np.linalg.norm(op.matvec(x) - r_np) / np.linalg.norm(b))
+    log(f"  RDO change over the last cycle {rdo_change:.2e}, residual rounding noise {noise:.2e}")
     cur = O.current(m, xs)
     occ = O.occupation(m, xs)
     out = dict(workload=name, params=w, N=int(m.N), d=int(d), packed=int(op.n),
                rdo_re=xs[0].real.tolist(), rdo_im=xs[0].imag.tolist(), current=cur.tolist(),
                occupation=occ.tolist(), relres=float(r), matvecs=int(it),
+               rdo_change_last_cycle=rdo_change, residual_rounding_noise=noise,
                trace=[float(np.trace(xs[0]).real), float(np.trace(xs[0]).imag)],
                seconds=round(time.time() - t0, 1),
                solver="right-preconditioned GMRES(60), C-port operator Pi A Pi, trace row on tier 0")

@@ -57,7 +57,8 @@ def shim():
                 ref_array(arr["ilead"], np.int32), ref_array(arr["mspin"], np.int32),
                 ref_array(arr["morbs"], np.int32), ref_array(arr["mpm"], np.int32),
                 ref_array(arr["eleadinfty"], np.float64), ref_array(arr["hs"], np.complex128),
-                ref_array(arr["amsall"], np.float64), ref_scalar(0, i32)]
+                ref_array(arr["amsall"], np.float64), ref_scalar(0, i32), ref_array(np.ones(1), np.float64),
+                ref_scalar(0, i32)]
         print("heomq_fortran_init_ N =", o.N, flush=True)
         h.heomq_fortran_init_(*args)
         _state.update(h=h, o=o, keep=keep)

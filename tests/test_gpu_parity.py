@@ -17,7 +17,15 @@ CASES = {
     "siam_nk3_L3": (lambda: heomq.siam(nk=3, ntop=3, V=0.3, U=1.5), lambda: O.siam_params(nk=3, ntop=3, V=0.3, U=1.5)),
     "siam_nk2_L2_rep": (lambda: heomq.siam(nk=2, ntop=2, V=-0.4, lsdraw=True), lambda: O.siam_params(nk=2, ntop=2, V=-0.4, lsdraw=True)),
     "dqd_nk2_L2": (lambda: heomq.two_impurity(nk=2, ntop=2), lambda: O.dqd_params(nk=2, ntop=2)),
+    # the reference's lscale: ADOs stored scaled by prod |cb_k|^(-1/2)
+    "dqd_nk2_L2_lscale": (lambda: heomq.two_impurity(nk=2, ntop=2, lscale=True),
+                          lambda: _lscale(O.dqd_params(nk=2, ntop=2))),
 }
+
+
+def _lscale(p):
+    import dataclasses
+    return dataclasses.replace(p, lscale=True)
 
 _cache = {}
 
@@ -95,7 +103,7 @@ def test_apply_fp32(name):
     assert rel(y, yo) <= 2e-6
 
 
-@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2", "dqd_nk2_L2_lscale"])
 def test_steady_state_fp64(name):
     m, ctx, o, torch = setup(name)
     xs, st = ctx.solve_steady(tol=1e-12, raise_on_fail=False)
@@ -108,6 +116,21 @@ def test_steady_state_fp64(name):
     cur, occ = ctx.observables(to_dev(torch, xs))
     assert np.abs(cur - O.current(o, xo)).max() <= 1e-10 * max(1.0, np.abs(O.current(o, xo)).max())
     assert np.abs(occ - O.occupation(o, xo)).max() <= 1e-10
+
+
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
+def test_autotuned_apply_matches_oracle(name):
+    # every form the autotuner may keep computes the same operator
+    m, ctx, o, torch = setup(name)
+    res = ctx.autotune(reps=5)
+    print(name, res, flush=True)
+    assert res["forward"] in (0, 1, 2, 3) and res["transposed"] in (0, 1, 2, 3)
+    assert min(res["times_us"]["forward"]) > 0
+    x = O.physical_projection(o, O.seeded_ado_vector(o, 17))
+    for tr in (False, True):
+        y = from_dev(ctx.apply_packed(to_dev(torch, x), trace=heomq.TRACE_RDO, transpose=tr))
+        f = O.apply_transpose if tr else O.apply
+        assert rel(y, O.physical_projection(o, f(o, x, trace_row="rdo"))) <= 1e-12
 
 
 def test_steady_state_mixed():

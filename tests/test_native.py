@@ -90,6 +90,36 @@ def test_native_tables_match_oracle(pp, op):
     assert np.array_equal(up[mask], oup[mask]) and (up[~mask, 0] == -1).all()
 
 
+def test_native_lscale_coefficients_match_oracle():
+    # lscale: cb and cd divided by |cb|^(1/2) of the unscaled cb, the rates unchanged
+    import dataclasses
+    for pp, op in ((heomq.two_impurity(nk=6, ntop=1, lscale=True), dataclasses.replace(O.dqd_params(nk=6, ntop=1), lscale=True)),
+                   (heomq.siam(nk=3, ntop=2, lscale=True), dataclasses.replace(O.siam_params(nk=3, ntop=2), lscale=True))):
+        m, o = heomq.Model(pp), O.Model(op)
+        cb, cd, g = m.coefs()
+        assert np.abs(cb - o.cb).max() <= 1e-12 * np.abs(o.cb).max()
+        assert np.abs(cd - o.cd).max() <= 1e-12 * np.abs(o.cd).max()
+        assert np.abs(np.abs(cb) - o.dbsqrt).max() <= 1e-12 * o.dbsqrt.max()   # |cb / |cb|^(1/2)| = |cb|^(1/2)
+        assert np.abs(g - o.gdraw).max() <= 1e-12 * np.abs(o.gdraw).max()
+        with pytest.raises(heomq.HeomqError):   # the representation is part of the shape
+            m.set_params(dataclasses.replace(pp, lscale=False))
+
+
+def test_reference_loader_takes_dbsqrt_and_unique_index_tables():
+    # the reference's lscale factors, and a hierarchy without repeated drawers
+    # (absent upper couplings: indexcoef = 0)
+    import dataclasses
+    o = O.Model(dataclasses.replace(O.dqd_params(nk=1, ntop=2), lscale=True))
+    arr = reference_arrays(o)
+    assert arr["dbsqrt"] is not None and (arr["indexcoef"] == 0).any()
+    m = heomq.Model.from_reference(**arr)
+    cb, cd, g = m.coefs()
+    assert np.abs(cb - o.cb).max() == 0.0 and np.abs(cd - o.cd).max() == 0.0
+    arr["dbsqrt"] = -arr["dbsqrt"]
+    with pytest.raises(heomq.HeomqError):
+        heomq.Model.from_reference(**arr)
+
+
 def test_pade_native_matches_numpy_eigensolver():
     # the native Sturm-bisection eigenvalues reproduce LAPACK's through cb/cgamma
     for nk in (1, 4, 8, 16, 30):

@@ -280,6 +280,41 @@ def test_two_impurity_particle_hole_point():
     assert O.occupation(m2, x2).sum() < 2.0 - 1e-3          # off the point it moves
 
 
+def _scaled(p):
+    import dataclasses
+    return dataclasses.replace(p, lscale=True)
+
+
+@pytest.mark.parametrize("p", [O.siam_params(nk=2, ntop=3, V=0.3, U=1.5), O.dqd_params(nk=1, ntop=2)],
+                         ids=["siam", "dqd"])
+def test_lscale_is_the_similarity_of_the_ado_scaling(p):
+    # with lscale the operator acting on stored ADOs rho_K / s_K is exactly
+    # S^-1 L S (S = diag(s_K), s_K = prod |cb_k|^(1/2)): coefficients, not the
+    # vector, carry the scaling, so this pins the rescaled cb, cd and the
+    # sqrt|cb| factor on every upper coupling
+    m, ms = O.Model(p), O.Model(_scaled(p))
+    s = O.ado_scale(ms)[:, None, None]
+    assert np.abs(s - 1).max() > 0.1
+    x = O.seeded_ado_vector(m, 4)
+    for tr in (None, "rdo"):
+        y = O.apply(m, x, trace_row=tr) / s
+        ys = O.apply(ms, x / s, trace_row=tr)
+        assert np.abs(ys - y).max() <= 1e-13 * np.abs(y).max()
+        z = O.apply_transpose(m, x, trace_row=tr) * s
+        zs = O.apply_transpose(ms, x * s, trace_row=tr)
+        assert np.abs(zs - z).max() <= 1e-13 * np.abs(z).max()
+
+
+def test_lscale_steady_state_is_the_unscaled_one():
+    p = O.dqd_params(nk=1, ntop=2, U=2.0, t=0.3, V=0.3, T=0.2)
+    m, ms = O.Model(p), O.Model(_scaled(p))
+    x, _, _ = O.steady_state(m, tol=1e-12)
+    xs, _, _ = O.steady_state(ms, tol=1e-12)
+    assert np.abs(xs[0] - x[0]).max() <= 1e-10
+    assert np.abs(xs * O.ado_scale(ms)[:, None, None] - x).max() <= 1e-9
+    assert np.abs(O.current(ms, xs) - O.current(m, x)).max() <= 1e-10 * np.abs(O.current(m, x)).max()
+
+
 def test_trace_row_placements_agree_for_odd_top_tier():
     m = O.Model(O.siam_params(nk=2, ntop=3, V=0.3, U=1.5))
     xr, _, _ = O.steady_state(m, tol=1e-12, trace_row="rdo")
@@ -430,6 +465,25 @@ def test_c_port_matches_numpy_oracle(p):
     for tr in ("last", "rdo", None):
         y, yo = port.apply(x, tr), O.apply(o, x, trace_row=tr)
         assert np.abs(y - yo).max() <= 1e-14 * np.abs(yo).max()
+
+
+def test_c_port_tfqmr_solves_the_steady_state():
+    # the CPU baseline's TFQMR loop (serial and parallel vector passes) drives
+    # the C-port apply to the steady state: residual of A (x0 + dx) = e_trace
+    # and the RDO of the oracle's own solve (odd top tier: the reference's
+    # last-ADO trace row is a consistent normalisation there)
+    from oracle.heom_port import Port
+    o = O.Model(O.siam_params(nk=1, ntop=3, V=0.3, U=1.5, T=0.2))
+    port = Port(o)
+    x0, b = port.steady_rhs("last")
+    for par in (False, True):
+        dx, bound = port.tfqmr(b, 200, parallel=par)
+        x = O.from_fortran((x0 + dx).view(np.complex128).reshape(o.N, o.d, o.d))
+        r = O.apply(o, x, trace_row="last")
+        r[-1, o.d - 1, o.d - 1] -= 1.0
+        assert np.linalg.norm(r) <= 1e-9 and bound <= 1e-8
+        xo, _, _ = O.steady_state(o, tol=1e-12)
+        assert np.abs(x[0] - xo[0]).max() <= 1e-9
 
 
 # ---- block-Jacobi iteration (jacobi/) ----
