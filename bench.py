@@ -38,27 +38,31 @@ sys.path.insert(0, str(ROOT / "heom-quick2_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def workload(name):
+def workload(name, lscale):
     import heomq
+    tag = ", ADOs scaled (lscale)" if lscale else ""
     if name == "two_impurity":
-        return heomq.two_impurity(), "two-impurity d=16 Nk=6 M=112 L=3 TFQMR steady state"
+        return heomq.two_impurity(lscale=lscale), f"two-impurity d=16 Nk=6 M=112 L=3 TFQMR steady state{tag}"
     if name == "siam":
-        return heomq.siam(), "SIAM d=4 Nk=8 M=72 L=3 TFQMR steady state"
+        return heomq.siam(lscale=lscale), f"SIAM d=4 Nk=8 M=72 L=3 TFQMR steady state{tag}"
     raise SystemExit(f"unknown workload {name}")
 
 
-FIXTURES = {"two_impurity": "steady_dqd_nk6_L3.json", "siam": "steady_siam_nk8_L3.json"}
+FIXTURES = {"two_impurity": ["steady_dqd_nk6_L3_lscale.json", "steady_dqd_nk6_L3.json"],
+            "siam": ["steady_siam_nk8_L3.json"]}
 
 
 def fixture(name):
-    """The committed steady state of a workload: (RDO [i, j], current, occupation, relres)."""
+    """The committed steady state of a workload (the most accurate one present):
+    (RDO [i, j], current, occupation, relres, file name).  RDO, current and
+    occupation do not depend on the ADO scaling."""
     import numpy as np
-    f = ROOT / "tests" / "golden" / FIXTURES[name]
-    if not f.exists():
+    f = next((ROOT / "tests" / "golden" / n for n in FIXTURES[name] if (ROOT / "tests" / "golden" / n).exists()), None)
+    if f is None:
         return None
     d = json.loads(f.read_text())
     return (np.array(d["rdo_re"]) + 1j * np.array(d["rdo_im"]), np.array(d["current"]),
-            np.array(d["occupation"]), d["relres"])
+            np.array(d["occupation"]), d["relres"], f.name)
 
 
 def main():
@@ -70,8 +74,11 @@ def main():
     ap.add_argument("--points", type=int, default=256, help="sweep: number of parameter points")
     ap.add_argument("--maxiter", type=int, default=3000, help="sweep: TFQMR iteration limit per point")
     ap.add_argument("--tol", type=float, default=None,
-                    help="residual target ||b - A x|| / ||b|| (||b|| = 1; default 1e-8, the reference's crit, "
-                         "for the two-impurity workload, 1e-12 for SIAM)")
+                    help="residual target ||b - A x|| / ||b|| (||b|| = 1; default 3e-11 for the scaled "
+                         "two-impurity workload, 2e-8 unscaled, 1e-12 for SIAM)")
+    ap.add_argument("--lscale", type=int, default=None,
+                    help="1: ADOs scaled by prod |cb_k|^(-1/2), the reference's lscale (default for the "
+                         "two-impurity workload); 0: unscaled")
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-samples", type=int, default=5, help="timed CPU TFQMR iterations (median)")
@@ -96,18 +103,24 @@ def main():
 
     if args.workload == "sweep":
         return sweep(args)
-    params, wname = workload(args.workload)
+    if args.lscale is None:
+        args.lscale = 1 if args.workload == "two_impurity" else 0
+    params, wname = workload(args.workload, bool(args.lscale))
     if args.apply_only:
         return apply_only(params, args.apply_reps, args.apply_kernel)
     if args.tol is None:
-        args.tol = 1e-8 if args.workload == "two_impurity" else 1e-12
+        # scaled: 3e-11 on the scaled residual holds every step's RDO error under 1e-10;
+        # unscaled, the FP64 floor of this bath (the last Pade pole meets the
+        # band pole) is ~1.2e-8, and 2e-8 is the nearest target every seed meets
+        args.tol = (3e-11 if args.lscale else 2e-8) if args.workload == "two_impurity" else 1e-12
     model = heomq.Model(params)
     ctx = heomq.Context(model)
+    tuned = ctx.autotune(reps=20)
     d, N = model.d, model.N
     fix = fixture(args.workload)
 
     for _ in range(args.warmup):
-        ctx.solve_steady(tol=args.tol)
+        ctx.solve_steady(tol=args.tol, raise_on_fail=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats, rdos = [], []
@@ -115,7 +128,7 @@ def main():
         # a different shadow vector per step: TFQMR's iteration count depends
         # on it, and the mean over seeds is the representative figure
         ctx.set_shadow_seed(k)
-        x, st = ctx.solve_steady(tol=args.tol)
+        x, st = ctx.solve_steady(tol=args.tol, raise_on_fail=False)
         stats.append(st)
         rdos.append(x[0].clone())
     torch.cuda.synchronize()
@@ -126,12 +139,12 @@ def main():
     # outside the timed region: every step's RDO against the fixture
     accuracy = None
     if fix is not None:
-        rho_fix, cur_fix, occ_fix, fix_relres = fix
+        rho_fix, cur_fix, occ_fix, fix_relres, fix_name = fix
         errs = [float(np.abs(r.cpu().numpy().T - rho_fix).max()) for r in rdos]
         accuracy = {"rdo_err_max": max(errs), "rdo_err_by_step": errs,
                     "current_rel_err": float(np.abs(cur - cur_fix).max() / np.abs(cur_fix).max()),
                     "occupation_err": float(np.abs(occ - occ_fix).max()),
-                    "reference": f"tests/golden/{FIXTURES[args.workload]} (GMRES on the oracle operator, "
+                    "reference": f"tests/golden/{fix_name} (GMRES on the oracle operator, "
                                  f"residual {fix_relres:.1e})"}
 
     # ---- roofline of the dominant kernel: the packed (symmetry-sector) apply ----
@@ -155,6 +168,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                 "kernel": "k_apply_packed<double,false,2>", "t_apply_us": round(t_apply * 1e6, 2),
+                "form": tuned["forward"], "autotune_us": tuned["times_us"],
                 "bytes_alg": bytes_alg, "packed_elems": P, "dense_elems": info["dense"],
                 "dense_equiv_GBs": round(dense_bytes / t_apply / 1e9, 1)}
 
@@ -176,7 +190,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.workload, wname, args.cpu_samples, stpl["iters"], stpl["status"] == 0)
+        cpu = cpu_baseline(args.workload, wname, args.cpu_samples, stpl["iters"], stpl["status"] == 0,
+                           bool(args.lscale))
 
     line = {
         "metric": "steady-state solves/s at 1e-10 RDO err; ADO-apply GB/s vs HBM peak",
@@ -192,11 +207,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": wname, "d": d, "M": model.M, "L": model.L, "N_ado": N, "tol": args.tol,
+                   "lscale": bool(args.lscale),
                    "solver": "TFQMR (device-resident, FP64), left-preconditioned: block Jacobi "
                              "(shift - S_K)^-1 when H couples Fock states, Jacobi scaling when H is diagonal"},
         "rdo_err": accuracy["rdo_err_max"] if accuracy else None,
         "accuracy": accuracy,
         "relres_by_step": [st["relres"] for st in stats],
+        "unconverged_steps": sum(1 for st in stats if st["status"] != 0),
         "iters_per_solve": iters,
         "iters_by_shadow_seed": [st["iters"] for st in stats],
         "applies_per_solve": applies,
@@ -338,6 +355,8 @@ def apply_only(params, reps, kernel="packed"):
     evict.fill_(0.0)   # 256 MiB written after src: src is no longer in L2 when it is copied
     dst.copy_(src)
     ctx = heomq.Context(heomq.Model(params))
+    if kernel in ("packed", "precond", "f32", "transpose"):
+        print(f"apply-only: autotuned forms {ctx.autotune(reps=20)}", flush=True)
     x0 = ctx.initial_guess()
     if kernel in ("rk4", "chebyshev", "bicg", "spa"):
         if kernel == "rk4":
@@ -400,7 +419,7 @@ def host_info():
     return os.cpu_count(), model
 
 
-def cpu_baseline(wname_key, wname, samples, iters_per_solve, converged):
+def cpu_baseline(wname_key, wname, samples, iters_per_solve, converged, lscale=False):
     """TFQMR iterations of the oracle's C port on the host cores: each is two
     OpenMP applies of the reference's calc_ax_tfqmr_omp on its own tables plus
     the nine axpby, two dot and two norm passes of dutfx.f90:196-357 over the
@@ -411,8 +430,9 @@ def cpu_baseline(wname_key, wname, samples, iters_per_solve, converged):
     import numpy as np
     from oracle import heom_oracle as O
     from oracle import heom_port as P
+    import dataclasses
     op = O.dqd_params() if wname_key == "two_impurity" else O.siam_params()
-    om = O.Model(op)
+    om = O.Model(dataclasses.replace(op, lscale=lscale))
     port = P.Port(om)
     threads = port.threads()
     x0, b = port.steady_rhs("rdo")
