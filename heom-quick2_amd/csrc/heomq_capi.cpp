@@ -864,6 +864,7 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
   if (int rc = build_jacobi(c.get(), m)) return rc;
   HIPCHECK(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
   HIPCHECK(hipStreamCreateWithFlags(&c->solve_stream, hipStreamNonBlocking));
+  if (int rc = scratch_address(device, &c->scratch_dev)) return rc;   // with `device` current
   HIPCHECK(c->scal.alloc(sizeof(TfqmrScal) * 2));
   HIPCHECK(hipMemset(c->scal.p, 0, sizeof(TfqmrScal) * 2));
   HIPCHECK(c->partial.alloc(sizeof(double) * 2 * kMaxRedBlocks));

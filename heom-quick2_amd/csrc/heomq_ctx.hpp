@@ -101,6 +101,7 @@ struct heomq_ctx {
   int nsec = 0;
   bool cons_n = true, cons_s = true;
   void* pinned = nullptr;            // small pinned host buffer for scalar read-back
+  double* scratch_dev = nullptr;     // this device's address of the static scratch (no-alloc entry point)
   heomq::TierBounds tb{};
   // Solves asked for on the null stream run on this non-blocking stream (the
   // null stream cannot be captured into a graph).
@@ -198,6 +199,11 @@ int jacobi_presmooth(heomq_ctx* c, double* x, int nsweep, double* r, double* t, 
 constexpr int kPresmoothSweeps = 2;
 // default block-Jacobi preconditioner shift, as a multiple of the iteration's ejac
 constexpr double kPrecShiftFactor = 0.05;
+// the static device scratch of heomq_apply_packed_precond_noalloc: complex FP64
+// elements (32 MiB) and the devices whose addresses are cached
+constexpr int64_t kNoAllocElems = int64_t(1) << 21;
+constexpr int kMaxDevices = 64;
+int scratch_address(int device, double** out);
 // The forward operator at time t (the context's own when no time-dependent
 // field is set; else gamma_K(t) written into the context's td_gam on st).
 int operator_at(heomq_ctx* c, double t, hipStream_t st, DevOperator& op);

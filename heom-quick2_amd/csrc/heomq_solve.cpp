@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "heomq_ctx.hpp"
@@ -310,6 +311,33 @@ int inner_maxit() {
   }();
   return v;
 }
+
+}  // namespace
+
+namespace heomq {
+// Device scratch of heomq_apply_packed_precond_noalloc (one per GPU).
+__device__ __align__(256) double scratch[2 * kNoAllocElems];
+
+// Address of `scratch` on `device` (made current by the caller), resolved
+// with hipGetSymbolAddress once per device and cached; host code passes this
+// address, never the array's name, to kernels.
+int scratch_address(int device, double** out) {
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return fail(HEOMQ_BAD_ARG, "device id out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(HEOMQ_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    addr[device] = static_cast<double*>(p);
+  }
+  *out = addr[device];
+  return HEOMQ_OK;
+}
+}  // namespace heomq
+
+namespace {
 
 int e_trace_into(heomq_ctx* c, double* b, int64_t n, int trace, hipStream_t st) {
   HIPCHECK(vec_fill<double>(st, n, b, 0.0));
@@ -663,6 +691,33 @@ int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int t
     HIPCHECK(launch_jacobi_fwd(op, jc, yp, w[0], st));
     HIPCHECK(launch_jacobi_back(op, jc, w[0], yp, false, st));
   }
+  return HEOMQ_OK;
+}
+
+// The preconditioned operator without allocating: A x goes to the file-scope
+// device array `scratch` (kNoAllocElems complex FP64 elements, 32 MiB, held by
+// every GPU the library runs on until the process ends) instead of a
+// context-owned workspace.  Its address on each device is resolved once, at
+// context creation (scratch_address below), and only that address is passed
+// to kernels.  Every call writes the whole of A x before reading it, so
+// nothing carries over between calls; calls that overlap on one GPU share the
+// array and race.
+int heomq_apply_packed_precond_noalloc(heomq_ctx* c, const double* xp, double* yp, int trace, void* stream) {
+  if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
+  if (xp == yp) return fail(HEOMQ_BAD_ARG, "x and y must differ");
+  if (c->ptotal > kNoAllocElems)
+    return fail(HEOMQ_BAD_DIM, "packed vector of " + std::to_string(c->ptotal) +
+                                   " complex elements exceeds the static scratch (" + std::to_string(kNoAllocElems) + ")");
+  if (!c->scratch_dev) return fail(HEOMQ_HIP_ERROR, "static scratch address not resolved for this device");
+  DevPacked op = c->pfwd;
+  op.precond = 0;
+  DevJacobi jc = c->pjac;
+  jc.ejac = c->prec_shift;
+  if (!jacobi_one_kernel(jc)) return fail(HEOMQ_UNSUPPORTED, "the one-kernel preconditioner does not apply here");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHECK(launch_apply_packed_f64(op, xp, c->scratch_dev, trace == HEOMQ_TRACE_RDO, st));
+  HIPCHECK(launch_jacobi_1k(op, jc, c->scratch_dev, yp, false, st));
   return HEOMQ_OK;
 }
 

@@ -72,6 +72,7 @@ SIGNATURES = {
     "heomq_apply_dparam": (C.c_int, [P, P, P, D, D, D, D, P]),
     "heomq_apply_packed": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),
     "heomq_packed_info": (C.c_int, [P, I64]),
+    "heomq_apply_packed_precond_noalloc": (C.c_int, [P, P, P, C.c_int, P]),
     "heomq_apply_packed_raw": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),
     "heomq_apply_packed_raw_f32": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),
     "heomq_apply_packed_precond": (C.c_int, [P, P, P, C.c_int, C.c_int, P]),

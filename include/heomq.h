@@ -239,6 +239,14 @@ int heomq_apply_packed_raw_f32(heomq_ctx* c, const float* xp, float* yp, int tra
  * (when every ADO range a 256-element block touches fits its LDS).  All forms
  * compute the same values.  No counterpart in the reference. */
 int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int trace, int form, void* stream);
+/* The same operator as heomq_apply_packed_precond form 1, with no allocation:
+ * A x is staged in a static device array of 2^21 complex FP64 elements
+ * (32 MiB per GPU, held until the process ends) instead of a context
+ * workspace; a packed vector longer than that is HEOMQ_BAD_DIM.  Calls that
+ * overlap on one GPU share the array and race.  No counterpart in the
+ * reference: for callers that cannot allocate (HIP graph capture, serving
+ * loops). */
+int heomq_apply_packed_precond_noalloc(heomq_ctx* c, const double* xp, double* yp, int trace, void* stream);
 int heomq_pack(heomq_ctx* c, const double* dense, double* packed, void* stream);
 int heomq_unpack(heomq_ctx* c, const double* packed, double* dense, void* stream);
 /* info[0] = packed complex elements, [1] = sectors, [2] = N conserved,
