@@ -356,7 +356,10 @@ def apply_only(params, reps, kernel="packed"):
     dst.copy_(src)
     ctx = heomq.Context(heomq.Model(params))
     if kernel in ("packed", "precond", "f32", "transpose"):
-        print(f"apply-only: autotuned forms {ctx.autotune(reps=20)}", flush=True)
+        if os.environ.get("HEOMQ_APPLY_FORM"):   # a forced form: every profiled launch is that one
+            print(f"apply-only: form {os.environ['HEOMQ_APPLY_FORM']} (forced)", flush=True)
+        else:
+            print(f"apply-only: autotuned forms {ctx.autotune(reps=20)}", flush=True)
     x0 = ctx.initial_guess()
     if kernel in ("rk4", "chebyshev", "bicg", "spa"):
         if kernel == "rk4":
