@@ -80,6 +80,9 @@ def main():
                     help="1: ADOs scaled by prod |cb_k|^(-1/2), the reference's lscale (default for the "
                          "two-impurity workload); 0: unscaled")
     ap.add_argument("--apply-reps", type=int, default=50)
+    ap.add_argument("--prop-steps", type=int, default=None,
+                    help="Chebyshev propagation steps timed after the solves (two_impurity default 2000; 0 skips)")
+    ap.add_argument("--prop-dt", type=float, default=0.02, help="propagation time step (hbar / eV)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-samples", type=int, default=5, help="timed CPU TFQMR iterations (median)")
     ap.add_argument("--apply-only", action="store_true",
@@ -188,6 +191,33 @@ def main():
              "rdo_err": float(np.abs(xpl[0].cpu().numpy().T - fix[0]).max()) if fix is not None else None}
     ctx.set_precond(-1, 0.0)
 
+    # ---- propagation: Chebyshev steps (td/chebyshev.f90) from a seeded random
+    # tier-0 density matrix, untimed by the headline; an in-sector state runs
+    # on the packed layout ----
+    prop = None
+    if args.prop_steps is None:
+        args.prop_steps = 2000 if args.workload == "two_impurity" else 0
+    if args.prop_steps > 0:
+        xs = ctx.random_state(7)
+        r_start = xs[0].cpu().numpy().T
+        ctx.chebyshev(xs.clone(), args.prop_dt, 2)
+        torch.cuda.synchronize()
+        tq = time.perf_counter()
+        _, cinfo = ctx.chebyshev(xs, args.prop_dt, args.prop_steps)
+        torch.cuda.synchronize()
+        tprop = time.perf_counter() - tq
+        rho = xs[0].cpu().numpy().T
+        prop = {"method": "Chebyshev (td/chebyshev.f90 recurrence, weights J_n(delta dt))",
+                "start": "seeded random tier-0 density matrix (seed 7) in the (N, S_z) blocks, higher tiers 0",
+                "steps": args.prop_steps, "dt": args.prop_dt, "terms_per_step": cinfo["ncheby"],
+                "delta": cinfo["delta"], "layout": "packed" if cinfo["packed"] else "dense",
+                "seconds": round(tprop, 4), "steps_per_s": round(args.prop_steps / tprop, 2),
+                "applies_per_s": round(args.prop_steps * cinfo["ncheby"] / tprop, 1),
+                "trace_err": float(abs(np.trace(rho) - 1.0)),
+                "hermiticity_err": float(np.abs(rho - rho.conj().T).max()),
+                "rho0_change": float(np.abs(rho - r_start).max()),
+                "current_L": float(ctx.observables(xs)[0][0].sum())}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.workload, wname, args.cpu_samples, stpl["iters"], stpl["status"] == 0,
@@ -221,6 +251,7 @@ def main():
         "current_L": float(cur[0].sum()),
         "occupation": float(occ.sum()),
         "roofline": roofline,
+        "propagation": prop,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

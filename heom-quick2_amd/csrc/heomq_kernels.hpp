@@ -157,10 +157,13 @@ struct DevPacked {
   // elements whose diagonal vanishes (tier-0 populations) and the trace row
   // are left unscaled
   int precond = 0;
+  // 1: the tier-0 output block is Hermitianised (the solvers' projected
+  // operator); 0: the plain restriction of A to the sector (propagation)
+  int herm0 = 1;
 };
 
 // Projected apply on packed vectors: y = Pi A x with the trace row on the
-// tier-0 block (rdo = true) or none, tier-0 output Hermitianised.
+// tier-0 block (rdo = true) or none, tier-0 output Hermitianised (herm0).
 hipError_t launch_apply_packed_f64(const DevPacked& op, const double* x, double* y, bool rdo, hipStream_t s);
 hipError_t launch_apply_packed_f32(const DevPacked& op, const float* x, float* y, bool rdo, hipStream_t s);
 // dense (N*d*d complex) <-> packed conversions; unpack writes zeros off-sector.
@@ -168,6 +171,9 @@ template <typename TD, typename TP>
 hipError_t launch_pack(const DevPacked& op, const TD* dense, TP* packed, hipStream_t s);
 template <typename TP, typename TD>
 hipError_t launch_unpack(const DevPacked& op, const TP* packed, TD* dense, hipStream_t s);
+// flag = 1 when a dense (N*d*d complex) vector holds a nonzero element outside
+// the packed sectors, else 0 (flag is written first).
+hipError_t launch_offsector_any(const DevPacked& op, const double* dense, int32_t* flag, hipStream_t s);
 // Hermitianise the tier-0 block of a packed vector in place.
 template <typename T>
 hipError_t launch_herm_packed(const DevPacked& op, T* x, hipStream_t s);

@@ -10,6 +10,8 @@
 // summed through LDS; top-tier ADOs (L couplings) run one thread per packed
 // element.  The output is the projected operator Pi*A: off-sector elements do
 // not exist and the tier-0 block is Hermitianised by a one-block fix-up.
+#include <algorithm>
+
 #include "heomq_kernels.hpp"
 
 namespace heomq {
@@ -725,7 +727,7 @@ __device__ void tier0_fixup_block(const DevPacked& op, const T* __restrict__ x, 
       }
     __syncthreads();
   }
-  herm0<T>(op, y);
+  if (op.herm0) herm0<T>(op, y);
 }
 
 template <typename T>
@@ -1105,6 +1107,20 @@ __global__ void k_unpack(const DevPacked op, const TP* __restrict__ packed, TD* 
   dense[2 * dst + 1] = (TD)packed[2 * t + 1];
 }
 
+__global__ void k_flag_clear(int32_t* flag) { *flag = 0; }
+
+__global__ void k_offsector_any(const DevPacked op, const double2* __restrict__ x, int64_t nado, int32_t* flag) {
+  const int dd = op.d * op.d;
+  for (int64_t a = blockIdx.x; a < nado; a += gridDim.x) {
+    const int16_t* pos = op.sec_pos + (int64_t)op.sec[a] * dd;
+    for (int e = threadIdx.x; e < dd; e += blockDim.x) {
+      if (pos[e] >= 0) continue;
+      const double2 v = x[a * dd + e];
+      if (v.x != 0.0 || v.y != 0.0) atomicOr(flag, 1);
+    }
+  }
+}
+
 template <typename TD>
 __global__ void k_zero(int64_t n, TD* __restrict__ v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1146,6 +1162,16 @@ hipError_t launch_apply_packed_f64(const DevPacked& op, const double* x, double*
 }
 hipError_t launch_apply_packed_f32(const DevPacked& op, const float* x, float* y, bool rdo, hipStream_t s) {
   return launch_p<float>(op, x, y, rdo, s);
+}
+
+hipError_t launch_offsector_any(const DevPacked& op, const double* dense, int32_t* flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, flag);
+  if (hipError_t e = hipGetLastError()) return e;
+  if (op.n_ado <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<int64_t>(op.n_ado, 8192);
+  hipLaunchKernelGGL(k_offsector_any, dim3(grid), dim3(256), 0, s, op, reinterpret_cast<const double2*>(dense),
+                     op.n_ado, flag);
+  return hipGetLastError();
 }
 
 template <typename TD, typename TP>

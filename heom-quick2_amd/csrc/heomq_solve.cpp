@@ -751,22 +751,54 @@ int heomq_packed_info(const heomq_ctx* c, int64_t* info) {
   return HEOMQ_OK;
 }
 
-int heomq_rk4_at(heomq_ctx* c, double* x, double t0, double dt, int nstep, void* stream) {
-  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
-  hipStream_t st = (hipStream_t)stream;
-  const int64_t n = 2 * c->N * c->d * c->d;
-  std::vector<double*> v;
-  if (int rc = carve<double>(c->ws64, n, 3, v)) return rc;
-  double *rho0 = v[0], *tmp = v[1], *k = v[2];
+}  // extern "C"
+
+namespace {
+
+// The operator of packed propagation: the forward packed apply with neither
+// the trace row, the Jacobi scaling nor the tier-0 Hermitianisation, i.e. A
+// restricted to the sector that holds every physical state; the rates are
+// gamma_K(t) when a lead field is set.  A time-dependent system Hamiltonian
+// is not in the packed tables.
+int packed_prop_op(heomq_ctx* c, double t, hipStream_t st, DevPacked& op) {
+  if (c->ptotal <= 0) return fail(HEOMQ_UNSUPPORTED, "packed propagation: no packed layout");
+  if (c->sys_nterm > 0)
+    return fail(HEOMQ_UNSUPPORTED, "packed propagation: a time-dependent system Hamiltonian needs the dense path");
+  op = c->pfwd;
+  op.precond = 0;
+  op.herm0 = 0;
+  if (c->field_type >= 0) {
+    DevOperator dop;
+    if (int rc = operator_at(c, t, st, dop)) return rc;
+    op.gam = dop.gam;
+  }
+  return HEOMQ_OK;
+}
+
+// Whether a dense state can be propagated on the packed layout: no
+// time-dependent system Hamiltonian, and no nonzero element outside the
+// sectors (that part is zero and stays zero: A maps the sector into itself and
+// the rest into the rest, so the packed result, unpacked, is the dense one).
+int dense_in_sector(heomq_ctx* c, const double* x, hipStream_t st, bool& yes) {
+  yes = false;
+  if (c->ptotal <= 0 || c->sys_nterm > 0) return HEOMQ_OK;
+  HIPCHECK(c->prop_flag.alloc(sizeof(int32_t)));
+  HIPCHECK(launch_offsector_any(c->pfwd, x, c->prop_flag.as<int32_t>(), st));
+  int32_t* h = reinterpret_cast<int32_t*>(static_cast<char*>(c->pinned) + 3072);
+  HIPCHECK(hipMemcpyAsync(h, c->prop_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  yes = *h == 0;
+  return HEOMQ_OK;
+}
+
+// nstep RK4 steps of td/intstp.f90:20-100 on n reals; deriv(t, in, out) is
+// the derivative at stage time t (calcderiv_omp(tt) at tt, tt + dt/2 twice,
+// tt + dt).
+template <typename Deriv>
+int rk4_steps(double* x, int64_t n, double* rho0, double* tmp, double* k, double t0, double dt, int nstep,
+              hipStream_t st, Deriv&& deriv) {
   const double dt2 = dt / 2, dt3 = dt / 3, dt6 = dt / 6;
-  // the derivative at time t (td/intstp.f90 calls calcderiv_omp(tt), (tt + dt2) twice, (tt + dt))
-  auto deriv = [&](double t, const double* in, double* out) -> int {
-    DevOperator op;
-    if (int rc = operator_at(c, t, st, op)) return rc;
-    HIPCHECK(launch_apply_f64(op, in, out, -1, st));
-    return HEOMQ_OK;
-  };
-  for (int step = 0; step < nstep; ++step) {   // td/intstp.f90:20-100
+  for (int step = 0; step < nstep; ++step) {
     const double t = t0 + step * dt;
     HIPCHECK(hipMemcpyAsync(rho0, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     if (int rc = deriv(t, rho0, k)) return rc;
@@ -784,18 +816,10 @@ int heomq_rk4_at(heomq_ctx* c, double* x, double t0, double dt, int nstep, void*
   return HEOMQ_OK;
 }
 
-int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream) {
-  return heomq_rk4_at(c, x, 0.0, dt, nstep, stream);
-}
-
-// Chebyshev propagation with the reference's recurrence P_n = P_{n-2} +
-// (2/delta) L P_{n-1} and weights J_n(delta*dt) (td/chebyshev.f90:126-200);
-// delta is the spread of Re(gamma_K) over the hierarchy and the expansion is
-// cut where |J_n| / sum |J_m| < 1e-21 (td/prechebyshev.f90:71-134).
-int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats, void* stream) {
-  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
-  hipStream_t st = (hipStream_t)stream;
-  const int64_t n = 2 * c->N * c->d * c->d;
+// The expansion of td/prechebyshev.f90:71-134: delta is the spread of
+// Re(gamma_K) over the hierarchy, the weights are J_n(delta dt), cut where
+// |J_n| / sum |J_m| < 1e-21 (the term that meets the threshold is kept).
+int cheby_setup(heomq_ctx* c, double dt, std::vector<double>& J, double& delta) {
   std::vector<double> gam(2 * (size_t)c->N);
   HIPCHECK(hipMemcpy(gam.data(), c->fwd.gam.p, sizeof(double) * gam.size(), hipMemcpyDeviceToHost));
   double rmax = 0.0, rmin = 0.0;
@@ -803,38 +827,159 @@ int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats
     rmax = std::max(rmax, gam[2 * a]);
     rmin = std::min(rmin, gam[2 * a]);
   }
-  const double delta = rmax - rmin;
+  delta = rmax - rmin;
   if (!(delta > 0)) return fail(HEOMQ_BAD_ARG, "Chebyshev: zero spectral width");
   const double arg = delta * dt;
-  std::vector<double> J;
+  J.clear();
   double sum = 0.0;
   for (int k = 0; k < 100000; ++k) {
     const double j = std::cyl_bessel_j((double)k, arg);
     J.push_back(j);
     sum += std::fabs(j);
-    if (std::fabs(j) / sum < 1e-21) break;   // the term that meets the threshold is kept
+    if (std::fabs(j) / sum < 1e-21) break;
   }
+  return HEOMQ_OK;
+}
+
+// nstep Chebyshev steps of td/chebyshev.f90:126-200 on n reals: P_0 = x,
+// P_1 = L x / delta, P_k = P_{k-2} + (2 / delta) L P_{k-1}, x = sum_k c_k J_k P_k
+// (c_0 = 1, else 2); apply(in, out) is L.
+template <typename Apply>
+int cheby_steps(double* x, int64_t n, double* pm2, double* pm1, double* lp, const std::vector<double>& J,
+                double delta, int nstep, hipStream_t st, Apply&& apply) {
   const int ncheby = (int)J.size();
-  std::vector<double*> v;
-  if (int rc = carve<double>(c->ws64, n, 3, v)) return rc;
-  double *pm2 = v[0], *pm1 = v[1], *lp = v[2];
   for (int step = 0; step < nstep; ++step) {
     HIPCHECK(hipMemcpyAsync(pm2, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));   // P0
     HIPCHECK(vec_lincomb3(st, n, x, J[0], pm2, 0.0, nullptr, 0.0, nullptr));
-    HIPCHECK(launch_apply_f64(c->fwd.op, pm2, lp, -1, st));
+    if (int rc = apply(pm2, lp)) return rc;
     HIPCHECK(vec_lincomb3(st, n, pm1, 1.0 / delta, lp, 0.0, nullptr, 0.0, nullptr));        // P1
     if (ncheby > 1) HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, 2.0 * J[1], pm1, 0.0, nullptr));
     for (int k = 2; k < ncheby; ++k) {
-      HIPCHECK(launch_apply_f64(c->fwd.op, pm1, lp, -1, st));
+      if (int rc = apply(pm1, lp)) return rc;
       // P_k = P_{k-2} + (2/delta) L P_{k-1}, written over P_{k-2}
       HIPCHECK(vec_lincomb3(st, n, pm2, 1.0, pm2, 2.0 / delta, lp, 0.0, nullptr));
       HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, 2.0 * J[k], pm2, 0.0, nullptr));
       std::swap(pm1, pm2);
     }
   }
+  return HEOMQ_OK;
+}
+
+int rk4_packed_core(heomq_ctx* c, double* xp, double t0, double dt, int nstep, hipStream_t st,
+                    const std::vector<double*>& v) {
+  auto deriv = [&](double t, const double* in, double* out) -> int {
+    DevPacked op;
+    if (int rc = packed_prop_op(c, t, st, op)) return rc;
+    HIPCHECK(launch_apply_packed_f64(op, in, out, false, st));
+    return HEOMQ_OK;
+  };
+  return rk4_steps(xp, 2 * c->ptotal, v[0], v[1], v[2], t0, dt, nstep, st, deriv);
+}
+
+int cheby_packed_core(heomq_ctx* c, double* xp, const std::vector<double>& J, double delta, int nstep,
+                      hipStream_t st, const std::vector<double*>& v) {
+  DevPacked op;
+  if (int rc = packed_prop_op(c, 0.0, st, op)) return rc;
+  auto apply = [&](const double* in, double* out) -> int {
+    HIPCHECK(launch_apply_packed_f64(op, in, out, false, st));
+    return HEOMQ_OK;
+  };
+  return cheby_steps(xp, 2 * c->ptotal, v[0], v[1], v[2], J, delta, nstep, st, apply);
+}
+
+}  // namespace
+
+extern "C" {
+
+// RK4 on the dense layout (the drop-in); a state with nothing outside the
+// packed sectors is propagated on the packed layout instead (the same result
+// to rounding, d^2 / (packed share) fewer bytes per apply).
+int heomq_rk4_at(heomq_ctx* c, double* x, double t0, double dt, int nstep, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (nstep < 0) return fail(HEOMQ_BAD_ARG, "negative step count");
+  hipStream_t st = (hipStream_t)stream;
+  bool packed = false;
+  if (int rc = dense_in_sector(c, x, st, packed)) return rc;
+  std::vector<double*> v;
+  if (packed) {
+    if (int rc = carve<double>(c->ws64, 2 * c->ptotal, 4, v)) return rc;
+    HIPCHECK((launch_pack<double, double>(c->pfwd, x, v[3], st)));
+    if (int rc = rk4_packed_core(c, v[3], t0, dt, nstep, st, v)) return rc;
+    HIPCHECK((launch_unpack<double, double>(c->pfwd, v[3], x, st)));
+    return HEOMQ_OK;
+  }
+  const int64_t n = 2 * c->N * c->d * c->d;
+  if (int rc = carve<double>(c->ws64, n, 3, v)) return rc;
+  auto deriv = [&](double t, const double* in, double* out) -> int {
+    DevOperator op;
+    if (int rc = operator_at(c, t, st, op)) return rc;
+    HIPCHECK(launch_apply_f64(op, in, out, -1, st));
+    return HEOMQ_OK;
+  };
+  return rk4_steps(x, n, v[0], v[1], v[2], t0, dt, nstep, st, deriv);
+}
+
+int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream) {
+  return heomq_rk4_at(c, x, 0.0, dt, nstep, stream);
+}
+
+int heomq_rk4_packed(heomq_ctx* c, double* xp, double t0, double dt, int nstep, void* stream) {
+  if (!c || !xp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (nstep < 0) return fail(HEOMQ_BAD_ARG, "negative step count");
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64, 2 * c->ptotal, 3, v)) return rc;
+  return rk4_packed_core(c, xp, t0, dt, nstep, (hipStream_t)stream, v);
+}
+
+// Chebyshev propagation with the reference's recurrence and weights (see
+// cheby_steps and cheby_setup); like heomq_rk4_at, a state with nothing
+// outside the packed sectors runs on the packed layout.
+int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (nstep < 0) return fail(HEOMQ_BAD_ARG, "negative step count");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<double> J;
+  double delta = 0.0;
+  if (int rc = cheby_setup(c, dt, J, delta)) return rc;
+  bool packed = false;
+  if (int rc = dense_in_sector(c, x, st, packed)) return rc;
+  std::vector<double*> v;
+  if (packed) {
+    if (int rc = carve<double>(c->ws64, 2 * c->ptotal, 4, v)) return rc;
+    HIPCHECK((launch_pack<double, double>(c->pfwd, x, v[3], st)));
+    if (int rc = cheby_packed_core(c, v[3], J, delta, nstep, st, v)) return rc;
+    HIPCHECK((launch_unpack<double, double>(c->pfwd, v[3], x, st)));
+  } else {
+    const int64_t n = 2 * c->N * c->d * c->d;
+    if (int rc = carve<double>(c->ws64, n, 3, v)) return rc;
+    auto apply = [&](const double* in, double* out) -> int {
+      HIPCHECK(launch_apply_f64(c->fwd.op, in, out, -1, st));
+      return HEOMQ_OK;
+    };
+    if (int rc = cheby_steps(x, n, v[0], v[1], v[2], J, delta, nstep, st, apply)) return rc;
+  }
   if (stats) {
-    stats[0] = ncheby;
+    stats[0] = (double)J.size();
     stats[1] = delta;
+    stats[2] = packed ? 1.0 : 0.0;
+  }
+  return HEOMQ_OK;
+}
+
+int heomq_chebyshev_packed(heomq_ctx* c, double* xp, double dt, int nstep, double* stats, void* stream) {
+  if (!c || !xp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (nstep < 0) return fail(HEOMQ_BAD_ARG, "negative step count");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<double> J;
+  double delta = 0.0;
+  if (int rc = cheby_setup(c, dt, J, delta)) return rc;
+  std::vector<double*> v;
+  if (int rc = carve<double>(c->ws64, 2 * c->ptotal, 3, v)) return rc;
+  if (int rc = cheby_packed_core(c, xp, J, delta, nstep, st, v)) return rc;
+  if (stats) {
+    stats[0] = (double)J.size();
+    stats[1] = delta;
+    stats[2] = 1.0;
   }
   return HEOMQ_OK;
 }

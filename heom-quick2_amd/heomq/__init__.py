@@ -553,6 +553,21 @@ class Context:
         return x, info
 
     # ---- solvers ----
+    def random_state(self, seed):
+        """Dense state whose tier-0 block is a seeded random density matrix,
+        G G^H with G complex normal, reduced to the elements the symmetry
+        sectors allow (its (N, S_z) blocks, so it stays positive) and
+        normalised to trace 1; every higher tier is zero."""
+        t = self.torch
+        rng = np.random.default_rng(seed)
+        g = rng.standard_normal((self.d, self.d)) + 1j * rng.standard_normal((self.d, self.d))
+        r = g @ g.conj().T
+        x = self.zeros()
+        x[0] = t.from_numpy(np.ascontiguousarray(r.T)).to(x.device)   # block (i, j) stored at [j, i]
+        x = self.unpack(self.pack(x))
+        x[0] /= t.diagonal(x[0]).sum()
+        return x
+
     def initial_guess(self):
         x = self.zeros()
         x[0] = self.torch.eye(self.d, dtype=self.torch.complex128, device=x.device) / self.d
@@ -646,10 +661,24 @@ class Context:
         return y
 
     def chebyshev(self, x, dt, nstep):
+        """Chebyshev steps (td/chebyshev.f90) on a dense state, in place."""
         stats = np.zeros(_lib.STATS_CHEBY_LEN)
         check(lib().heomq_chebyshev(self._h, self._check_vec(x, self.torch.complex128), dt, nstep, _dp(stats),
                                     self._stream()))
-        return x, dict(ncheby=int(stats[0]), delta=stats[1])
+        return x, dict(ncheby=int(stats[0]), delta=stats[1], packed=bool(stats[2]))
+
+    def rk4_packed(self, xp, dt, nstep, t0=0.0):
+        """RK4 (td/intstp.f90) on a symmetry-packed state, in place."""
+        check(lib().heomq_rk4_packed(self._h, self._check_packed(xp, self.torch.complex128), float(t0), dt, nstep,
+                                     self._stream()))
+        return xp
+
+    def chebyshev_packed(self, xp, dt, nstep):
+        """Chebyshev steps (td/chebyshev.f90) on a symmetry-packed state, in place."""
+        stats = np.zeros(_lib.STATS_CHEBY_LEN)
+        check(lib().heomq_chebyshev_packed(self._h, self._check_packed(xp, self.torch.complex128), dt, nstep,
+                                           _dp(stats), self._stream()))
+        return xp, dict(ncheby=int(stats[0]), delta=stats[1], packed=True)
 
     def observables(self, x):
         p = self.model.params

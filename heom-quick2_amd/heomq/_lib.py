@@ -15,7 +15,7 @@ OK, BAD_DIM, UNSUPPORTED, HIP_ERROR, NOT_CONVERGED, BREAKDOWN, STOPPED, BAD_ARG 
 TRACE_NONE, TRACE_LAST, TRACE_RDO = 0, 1, 2
 HAM_SIAM, HAM_DQD = 0, 1
 # doubles every solver writes into its stats array (heomq.h HEOMQ_STATS_*)
-STATS_LEN, STATS_MIXED_LEN, STATS_CHEBY_LEN = 8, 64, 2
+STATS_LEN, STATS_MIXED_LEN, STATS_CHEBY_LEN = 8, 64, 3
 
 
 class HeomqError(RuntimeError):
@@ -115,6 +115,8 @@ SIGNATURES = {
     "heomq_ctx_field_shifts": (C.c_int, [P, C.c_double, D]),
     "heomq_ctx_set_presmooth": (C.c_int, [P, C.c_int]),
     "heomq_chebyshev": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
+    "heomq_rk4_packed": (C.c_int, [P, P, C.c_double, C.c_double, C.c_int, P]),
+    "heomq_chebyshev_packed": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_observables": (C.c_int, [P, P, D, D, P]),
 }
 

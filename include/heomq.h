@@ -47,7 +47,7 @@ extern "C" {
  * HEOMQ_STATS_MIXED_LEN; heomq_chebyshev takes HEOMQ_STATS_CHEBY_LEN. */
 #define HEOMQ_STATS_LEN        8
 #define HEOMQ_STATS_MIXED_LEN 64
-#define HEOMQ_STATS_CHEBY_LEN  2
+#define HEOMQ_STATS_CHEBY_LEN  3
 
 /* Trace (normalisation) row placement of the steady-state operator. */
 #define HEOMQ_TRACE_NONE  0   /* plain L x (td/calcderiv_omp.f90)                         */
@@ -438,11 +438,24 @@ int heomq_batch_solve(heomq_batch* b, double tol, int maxiter, double* rdo, doub
 /* ---------------------------------------------------------------------------
  * Propagation (td/intstp.f90 RK4 and td/chebyshev.f90 Chebyshev), fixed H.
  * x (device) is advanced in place by nstep steps of dt.  Chebyshev's stats
- * (HEOMQ_STATS_CHEBY_LEN, may be NULL): [0] terms per step, [1] delta.
+ * (HEOMQ_STATS_CHEBY_LEN, may be NULL): [0] terms per step, [1] delta, [2] 1 when
+ * the steps ran on the packed layout, 0 on the dense one.
+ * A dense x with no nonzero element outside the symmetry-packed sectors (every
+ * physical state) is propagated on the packed layout and unpacked (heomq_rk4_at
+ * likewise, unless a time-dependent system Hamiltonian is set); any other x
+ * takes the dense apply.
  * ------------------------------------------------------------------------- */
 int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream);
 int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats,
                     void* stream);
+/* The same on a symmetry-packed vector xp (heomq_packed_info()[0] complex
+ * elements, device), with the operator A restricted to the sectors (no trace
+ * row, no projection): td/intstp.f90:20-100 from time t0 (lead-field rates
+ * gamma_K(t) at every stage), td/chebyshev.f90:126-200.  HEOMQ_UNSUPPORTED
+ * when a time-dependent system Hamiltonian is set. */
+int heomq_rk4_packed(heomq_ctx* c, double* xp, double t0, double dt, int nstep, void* stream);
+int heomq_chebyshev_packed(heomq_ctx* c, double* xp, double dt, int nstep, double* stats,
+                           void* stream);
 
 /* Observables from a device state: I[alpha*nspin + s] (td/calcj.f90, times
  * jconst) and occupation occ[m*nspin + s] (main/calcocc.f90). */

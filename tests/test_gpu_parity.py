@@ -171,6 +171,62 @@ def test_chebyshev_device(name):
     assert rel(from_dev(xd), xo) <= 1e-12
 
 
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2", "dqd_nk2_L2_lscale"])
+def test_rk4_in_sector_takes_the_packed_layout(name):
+    """A state with nothing outside the symmetry sectors is propagated on the
+    packed layout (heomq_rk4_at routes it; heomq_rk4_packed takes it packed):
+    both equal the oracle's dense RK4 of td/intstp.f90."""
+    m, ctx, o, torch = setup(name)
+    x = O.physical_projection(o, O.seeded_ado_vector(o, 43), hermitian=False)
+    xo = O.rk4(o, x, 0.02, 3)
+    assert rel(from_dev(ctx.rk4(to_dev(torch, x), 0.02, 3)), xo) <= 1e-12
+    xp = ctx.pack(to_dev(torch, x))
+    ctx.rk4_packed(xp, 0.02, 3)
+    assert rel(from_dev(ctx.unpack(xp)), xo) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2", "dqd_nk2_L2_lscale"])
+def test_chebyshev_in_sector_takes_the_packed_layout(name):
+    """td/chebyshev.f90 steps on an in-sector state: the dense entry point
+    reports the packed layout, and it and the packed entry point equal the
+    oracle's recurrence; a state with an off-sector element stays dense."""
+    m, ctx, o, torch = setup(name)
+    x = O.physical_projection(o, O.seeded_ado_vector(o, 44))
+    xo, ncheby, delta = O.chebyshev(o, x, 0.02, 2)
+    xd, info = ctx.chebyshev(to_dev(torch, x), 0.02, 2)
+    assert info["packed"] and info["ncheby"] == ncheby
+    assert rel(from_dev(xd), xo) <= 1e-12
+    xp, info = ctx.chebyshev_packed(ctx.pack(to_dev(torch, x)), 0.02, 2)
+    assert rel(from_dev(ctx.unpack(xp)), xo) <= 1e-12
+    y = O.seeded_ado_vector(o, 45)
+    _, info = ctx.chebyshev(to_dev(torch, y), 0.02, 1)
+    assert not info["packed"]
+
+
+def test_packed_propagation_conserves_trace_and_hermiticity():
+    """Size-independent properties of the propagation at the two-impurity
+    workload's full size (d=16, Nk=6, L=3, 117,321 ADOs) from a seeded random
+    tier-0 density matrix with (N, S_z) block structure: tr rho0 stays 1 and
+    rho0 Hermitian over 20 Chebyshev steps and 20 RK4 steps."""
+    ctx = heomq.Context(heomq.Model(heomq.two_impurity(lscale=True)))
+    x = ctx.random_state(7)
+    r0 = x[0].cpu().numpy().T
+    assert abs(np.trace(r0) - 1.0) <= 1e-14 and np.linalg.eigvalsh(r0).min() > 0
+    dt = 0.02
+    for step in ("chebyshev", "rk4"):
+        y = x.clone()
+        if step == "chebyshev":
+            _, info = ctx.chebyshev(y, dt, 20)
+            assert info["packed"]
+        else:
+            ctx.rk4(y, dt, 20)
+        rho = y[0].cpu().numpy().T
+        assert abs(np.trace(rho) - 1.0) <= 1e-12
+        assert np.abs(rho - rho.conj().T).max() <= 1e-12 * np.abs(rho).max()
+        assert np.abs(rho - r0).max() > 1e-6   # it moved
+    ctx.close()
+
+
 def test_graph_replays_match_eager():
     """The solver's captured chunk graphs, replayed many times with host
     synchronisations between replays, give the same iterates as launching the
