@@ -174,6 +174,12 @@ hipError_t launch_unpack(const DevPacked& op, const TP* packed, TD* dense, hipSt
 // flag = 1 when a dense (N*d*d complex) vector holds a nonzero element outside
 // the packed sectors, else 0 (flag is written first).
 hipError_t launch_offsector_any(const DevPacked& op, const double* dense, int32_t* flag, hipStream_t s);
+// The BiCG convention on the tier-0 block of a packed vector (ADO 0's slots),
+// one block: mode 0 rebuilds the upper triangle from the lower one with a real
+// diagonal (calc_ax_bicg_omp.f90:70-77), mode 1 zeroes the upper triangle and
+// the diagonal's imaginary part (:330-337), mode 2 folds the upper triangle
+// onto the lower one (lower += conj(upper)^T, upper 0, diagonal real).
+hipError_t launch_bicg_tier0_packed(const DevPacked& op, double* x, int mode, hipStream_t s);
 // Hermitianise the tier-0 block of a packed vector in place.
 template <typename T>
 hipError_t launch_herm_packed(const DevPacked& op, T* x, hipStream_t s);

@@ -1107,6 +1107,38 @@ __global__ void k_unpack(const DevPacked op, const TP* __restrict__ packed, TD* 
   dense[2 * dst + 1] = (TD)packed[2 * t + 1];
 }
 
+// ADO 0's sector holds (j, i) whenever it holds (i, j) (N_i = N_j and
+// S_z,i = S_z,j are symmetric), so every off-diagonal slot has its mirror.
+// Upper slots are written only by their lower mirror's thread (mode 2) or
+// read only from lower slots no thread writes (mode 0).
+__global__ void k_bicg_tier0_p(const DevPacked op, double* __restrict__ x, int mode) {
+  const int D = op.d, DD = D * D;
+  const int n0 = (int)(op.off[1] - op.off[0]);
+  const int16_t* pos = op.sec_pos + op.sec[0] * DD;
+  const int16_t* el = op.sec_elems + op.sec[0] * DD;
+  for (int t = threadIdx.x; t < n0; t += blockDim.x) {
+    const int e = el[t];
+    const int i = e % D, j = e / D;
+    if (i == j) {
+      x[2 * t + 1] = 0.0;
+      continue;
+    }
+    const int q = pos[j + i * D];
+    if (mode == 0 && i < j) {
+      x[2 * t] = q >= 0 ? x[2 * q] : 0.0;
+      x[2 * t + 1] = q >= 0 ? -x[2 * q + 1] : 0.0;
+    } else if (mode == 1 && i < j) {
+      x[2 * t] = 0.0;
+      x[2 * t + 1] = 0.0;
+    } else if (mode == 2 && i > j && q >= 0) {
+      x[2 * t] += x[2 * q];
+      x[2 * t + 1] -= x[2 * q + 1];
+      x[2 * q] = 0.0;
+      x[2 * q + 1] = 0.0;
+    }
+  }
+}
+
 __global__ void k_flag_clear(int32_t* flag) { *flag = 0; }
 
 __global__ void k_offsector_any(const DevPacked op, const double2* __restrict__ x, int64_t nado, int32_t* flag) {
@@ -1162,6 +1194,12 @@ hipError_t launch_apply_packed_f64(const DevPacked& op, const double* x, double*
 }
 hipError_t launch_apply_packed_f32(const DevPacked& op, const float* x, float* y, bool rdo, hipStream_t s) {
   return launch_p<float>(op, x, y, rdo, s);
+}
+
+hipError_t launch_bicg_tier0_packed(const DevPacked& op, double* x, int mode, hipStream_t s) {
+  if (op.n_ado <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bicg_tier0_p, dim3(1), dim3(256), 0, s, op, x, mode);
+  return hipGetLastError();
 }
 
 hipError_t launch_offsector_any(const DevPacked& op, const double* dense, int32_t* flag, hipStream_t s) {

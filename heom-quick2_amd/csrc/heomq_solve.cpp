@@ -545,20 +545,28 @@ int heomq_apply_transpose_bicg(heomq_ctx* c, double* w, double* z, void* stream)
 // solvebicg.f90 on the device: x (dense, in/out) starts from its tier-0 lower
 // triangle (:52-58), b = e_(1,1) (:60-61), stop when ||r|| <= crit (absolute,
 // :124-126); x's rho0 is returned Hermitian (:108-113).
-int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* stats, void* stream) {
-  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
-  hipStream_t st = (hipStream_t)stream;
-  const auto t0 = std::chrono::steady_clock::now();
-  const int64_t n = 2 * c->N * c->d * c->d;
-  std::vector<double*> v;
-  if (int rc = carve<double>(c->ws64, n, 6, v)) return rc;
+}  // extern "C"
+
+namespace {
+
+int dense_in_sector(heomq_ctx* c, const double* x, hipStream_t st, bool& yes);
+
+// The BiCG iteration of bicg/solvebicg.f90 on vectors of n reals: fwd(in, out)
+// is A_B (in's tier 0 rebuilt from its lower triangle first), trn(w, z) its
+// transpose, e_off the real offset of b = e_(1,1); x holds the initial guess
+// with its tier-0 upper triangle zeroed.  The device scalars are read back
+// every eight iterations.
+template <typename Fwd, typename Trn>
+int bicg_core(heomq_ctx* c, double* x, int64_t n, int64_t e_off, const std::vector<double*>& v, double crit,
+              int maxiter, hipStream_t st, Fwd&& fwd, Trn&& trn, int& iters, int& applies, int& flag,
+              double& rr) {
   double *r = v[0], *sv = v[1], *dv = v[2], *f = v[3], *q = v[4], *qt = v[5];
   BicgScal* sc = reinterpret_cast<BicgScal*>(c->scal.p);
   double* part = c->partial.as<double>();
-  HIPCHECK(launch_lower_tier0_f64(c->d, x, st));
-  HIPCHECK(launch_apply_bicg_f64(c->fwd.op, x, q, st));
-  HIPCHECK(bicg_init(st, n, q, 0, r, sv, dv, f, sc, crit, part));
-  int applies = 1, iters = 0, flag = 0;
+  if (int rc = fwd(x, q)) return rc;
+  HIPCHECK(bicg_init(st, n, q, e_off, r, sv, dv, f, sc, crit, part));
+  applies = 1;
+  iters = 0;
   BicgScal& hs = *reinterpret_cast<BicgScal*>(static_cast<char*>(c->pinned) + 2048);
   HIPCHECK(hipMemcpyAsync(&hs, sc, sizeof(BicgScal), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
@@ -566,9 +574,9 @@ int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* 
   const int check_every = 8;
   while (flag == 0 && iters < maxiter) {
     for (int k = 0; k < check_every; ++k) {
-      HIPCHECK(launch_apply_bicg_f64(c->fwd.op, dv, q, st));
+      if (int rc = fwd(dv, q)) return rc;
       HIPCHECK(bicg_alpha(st, n, f, q, sc, part));
-      HIPCHECK(launch_apply_transpose_bicg_f64(c->trn.op, f, qt, st));
+      if (int rc = trn(f, qt)) return rc;
       HIPCHECK(bicg_update(st, n, x, dv, r, q, sv, qt, sc, part));
       HIPCHECK(bicg_dirs(st, n, r, sv, dv, f, sc));
     }
@@ -578,14 +586,79 @@ int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* 
     flag = hs.flag;
     applies = 1 + 2 * iters;
   }
-  HIPCHECK(launch_herm_from_lower_f64(c->d, x, st));
+  rr = hs.rr;
+  return HEOMQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// A state with nothing outside the symmetry sectors (every physical guess)
+// iterates on the packed layout: b = e_(1,1) and A_B map the sectors into
+// themselves, so every BiCG vector stays there and the packed iterates are the
+// dense ones to rounding.  The packed operators keep the BiCG trace row at
+// element (1,1) of tier 0 and skip the tier-0 Hermitianisation.
+int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* stats, void* stream) {
+  if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  bool packed = false;
+  if (int rc = dense_in_sector(c, x, st, packed)) return rc;
+  std::vector<double*> v;
+  int iters = 0, applies = 0, flag = 0;
+  double rr = 0.0;
+  if (packed) {
+    const int64_t n = 2 * c->ptotal;
+    if (int rc = carve<double>(c->ws64, n, 7, v)) return rc;
+    double* xp = v[6];
+    DevPacked fo = c->pfwd, to = c->ptrn;
+    fo.precond = to.precond = 0;
+    fo.herm0 = to.herm0 = 0;
+    const int slot = c->sec_pos_h[(size_t)c->psec_h[0] * c->d * c->d];   // element (1,1) of tier 0
+    if (slot < 0) return fail(HEOMQ_UNSUPPORTED, "BiCG: element (1,1) of tier 0 is not in its sector");
+    fo.trace_slot = to.trace_slot = slot;
+    auto fwd = [&](double* in, double* out) -> int {
+      HIPCHECK(launch_bicg_tier0_packed(fo, in, 0, st));
+      HIPCHECK(launch_apply_packed_f64(fo, in, out, true, st));   // y(1,1) of tier 0 = tr rho0
+      HIPCHECK(launch_bicg_tier0_packed(fo, out, 1, st));
+      return HEOMQ_OK;
+    };
+    auto trn = [&](double* w, double* z) -> int {
+      HIPCHECK(launch_bicg_tier0_packed(to, w, 1, st));
+      HIPCHECK(launch_apply_packed_f64(to, w, z, true, st));      // w(1,1) moved onto the diagonal of z0
+      HIPCHECK(launch_bicg_tier0_packed(to, z, 2, st));
+      return HEOMQ_OK;
+    };
+    HIPCHECK((launch_pack<double, double>(c->pfwd, x, xp, st)));
+    HIPCHECK(launch_bicg_tier0_packed(fo, xp, 1, st));
+    if (int rc = bicg_core(c, xp, n, 2 * (int64_t)slot, v, crit, maxiter, st, fwd, trn, iters, applies, flag, rr))
+      return rc;
+    HIPCHECK(launch_bicg_tier0_packed(fo, xp, 0, st));
+    HIPCHECK((launch_unpack<double, double>(c->pfwd, xp, x, st)));
+  } else {
+    const int64_t n = 2 * c->N * c->d * c->d;
+    if (int rc = carve<double>(c->ws64, n, 6, v)) return rc;
+    auto fwd = [&](double* in, double* out) -> int {
+      HIPCHECK(launch_apply_bicg_f64(c->fwd.op, in, out, st));
+      return HEOMQ_OK;
+    };
+    auto trn = [&](double* w, double* z) -> int {
+      HIPCHECK(launch_apply_transpose_bicg_f64(c->trn.op, w, z, st));
+      return HEOMQ_OK;
+    };
+    HIPCHECK(launch_lower_tier0_f64(c->d, x, st));
+    if (int rc = bicg_core(c, x, n, 0, v, crit, maxiter, st, fwd, trn, iters, applies, flag, rr)) return rc;
+    HIPCHECK(launch_herm_from_lower_f64(c->d, x, st));
+  }
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
     std::fill(stats, stats + HEOMQ_STATS_LEN, 0.0);
     stats[0] = iters;
     stats[1] = applies;
-    stats[2] = std::sqrt(hs.rr);
+    stats[2] = std::sqrt(rr);
     stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats[4] = packed ? 1.0 : 0.0;
   }
   if (flag == 2) return fail(HEOMQ_BREAKDOWN, "BiCG breakdown");
   if (flag != 1) return fail(HEOMQ_NOT_CONVERGED, "BiCG did not reach the criterion");

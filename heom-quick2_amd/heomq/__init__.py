@@ -481,7 +481,8 @@ class Context:
         stats = np.zeros(_lib.STATS_LEN)
         rc = lib().heomq_solve_bicg(self._h, self._check_vec(x, t.complex128), crit, maxiter, _dp(stats),
                                     self._stream())
-        info = dict(iters=int(stats[0]), applies=int(stats[1]), resid=stats[2], seconds=stats[3], status=rc)
+        info = dict(iters=int(stats[0]), applies=int(stats[1]), resid=stats[2], seconds=stats[3], status=rc,
+                    packed=bool(stats[4]))
         if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
             check(rc)
         return x, info

@@ -302,7 +302,9 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
  * x (device, dense, in/out) holds the initial guess, b = e_(1,1) of tier 0,
  * stop when ||b - A_B x||_2 <= crit (absolute, as solvebicg.f90:124-126);
  * x's rho0 is returned Hermitian.  stats (HEOMQ_STATS_LEN) as
- * heomq_solve_steady, [2] the absolute residual. */
+ * heomq_solve_steady, [2] the absolute residual, [4] 1 when the iteration ran
+ * on the symmetry-packed layout (an x with nothing outside the sectors; the
+ * iterates are the dense ones to rounding), 0 on the dense one. */
 int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* stats, void* stream);
 /* The same on a host array (the solvebicg drop-in of the Fortran layer). */
 int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats);

@@ -272,10 +272,19 @@ def test_bicg_steady_state(name):
     x, st = ctx.solve_bicg(crit=1e-10, maxiter=5000)
     print(name, st, flush=True)
     assert st["status"] == 0 and st["resid"] <= 1e-10
+    assert st["packed"]   # the identity guess is in the sectors: the packed iteration
     xb = from_dev(x)
     assert np.abs(xb[0] - xb[0].conj().T).max() == 0.0
     xt, _, _ = O.steady_state(o, tol=1e-13)
     assert np.abs(xb[0] - xt[0]).max() <= 1e-9
+    # a guess with an element outside the sectors iterates on the dense layout
+    # and reaches the same state
+    g = ctx.initial_guess()
+    g[1, 0, 0] = 1e-3   # element (1,1) of the first tier-1 ADO, whose sector excludes it
+    assert not ctx.unpack(ctx.pack(g)).equal(g)
+    x2, st2 = ctx.solve_bicg(g, crit=1e-10, maxiter=5000)
+    assert st2["status"] == 0 and not st2["packed"]
+    assert np.abs(from_dev(x2)[0] - xt[0]).max() <= 1e-9
 
 
 @pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
