@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="two_impurity", choices=["two_impurity", "siam", "sweep"])
+    ap.add_argument("--workload", default="two_impurity", choices=["two_impurity", "siam", "sweep", "didv"])
     ap.add_argument("--points", type=int, default=256, help="sweep: number of parameter points")
     ap.add_argument("--maxiter", type=int, default=3000, help="sweep: TFQMR iteration limit per point")
     ap.add_argument("--tol", type=float, default=None,
@@ -106,6 +106,8 @@ def main():
 
     if args.workload == "sweep":
         return sweep(args)
+    if args.workload == "didv":
+        return didv(args)
     if args.lscale is None:
         args.lscale = 1 if args.workload == "two_impurity" else 0
     params, wname = workload(args.workload, bool(args.lscale))
@@ -266,6 +268,52 @@ def measured_traffic(wname):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", f.read_text())
     return int(m.group(1)) if m else None
+
+
+def didv(args):
+    """Adjoint bias sweep on the single-impurity model (eps_d = -U/2, U = 2,
+    Gamma_L = Gamma_R = 0.1, W = 10, T = 0.05, Pade Nk = 8, L = 3): 33 bias
+    points on [-1, 1], each a steady-state solve (warm-started from the
+    previous point) and one transposed solve giving dI_L/dtheta for theta =
+    (eps_d, U, Gamma_L, Gamma_R, V); dI/dV is the differential conductance.
+    Outside the timed region: central differences (h = 1e-5) at one point and
+    the particle-hole symmetry of the curve."""
+    import numpy as np
+    import torch
+    from heomq import adjoint as ADJ
+    th = np.array([-1.0, 2.0, 0.1, 0.1, 0.0])
+    pr = ADJ.SteadyStateProblem(th, tol=1e-12, nk=8, ntop=3, T=0.05, W=10.0)
+    Vs = np.linspace(-1.0, 1.0, 33)
+    for _ in range(args.warmup):
+        x, _ = pr.solve(th)
+        pr.gradient(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cur, grads = ADJ.bias_sweep(pr, Vs, theta=th)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    k = 20
+    thk = th.copy()
+    thk[4] = Vs[k]
+    x, _ = pr.solve(thk)
+    g = pr.gradient(x)
+    gf = pr.fd_gradient(h=1e-5)
+    line = {
+        "metric": "adjoint bias-sweep points/s (steady state + dI_L/dtheta, 5 parameters)",
+        "value": round(len(Vs) / elapsed, 3), "unit": "points/s", "n_gpus": 1, "steps": len(Vs),
+        "warmup": args.warmup, "ms_per_step": round(elapsed / len(Vs) * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "single-impurity d=4 Nk=8 M=72 L=3, 33 bias points on [-1, 1], "
+                               "TFQMR forward + transposed solves (tol 1e-12)", "N_ado": pr.model.N},
+        "curve": {"V": Vs.tolist(), "I_L": cur.tolist(), "dIdV": grads[:, 4].tolist()},
+        "gradients": {"names": list(ADJ.THETA_NAMES), "at_V": float(Vs[k]), "adjoint": g.tolist(),
+                      "central_differences": gf.tolist(),
+                      "max_rel_err": float(np.abs(g - gf).max() / np.abs(gf).max())},
+        "particle_hole": {"I_odd_err": float(np.abs(cur + cur[::-1]).max() / np.abs(cur).max()),
+                          "dIdV_even_err": float(np.abs(grads[:, 4] - grads[::-1, 4]).max()
+                                                 / np.abs(grads[:, 4]).max())},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def sweep_points(npts, seed=0x5EED0004):
