@@ -86,5 +86,51 @@ def build_fortran() -> Path | None:
     return FLIB_PATH
 
 
+SAN_DIR = PKG_DIR.parent / "_build" / "san"
+SAN_LIB = SAN_DIR / "libheomq_san.so"
+
+
+def asan_runtime() -> Path:
+    """The compiler's AddressSanitizer runtime, preloaded into the process that
+    loads the sanitized library."""
+    r = subprocess.run([_hipcc(), "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True, text=True)
+    p = Path(r.stdout.strip())
+    if r.returncode != 0 or not p.is_absolute() or not p.exists():
+        raise RuntimeError("the compiler's ASan runtime (libclang_rt.asan-x86_64.so) was not found")
+    return p
+
+
+def build_sanitized(jobs: int = 8) -> Path:
+    """libheomq with its host code under AddressSanitizer and UBSan (device
+    code unchanged: the flags reach the host pass only), for the CPU checks of
+    the host logic: table builders, the reference loader, layout and
+    eigensystem builders.  Load it with HEOMQ_LIBRARY=<path> and the runtime of
+    asan_runtime() in LD_PRELOAD."""
+    hipcc = _hipcc()
+    SAN_DIR.mkdir(parents=True, exist_ok=True)
+    common = ["-O1", "-g", "-std=c++17", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}",
+              "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+              "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined"]
+    procs, objs = [], []
+    for src in SOURCES:
+        obj = SAN_DIR / (src.rsplit(".", 1)[0] + ".o")
+        cmd = [hipcc, *common, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
+        objs.append(obj)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        while len([p for _, p in procs if p.poll() is None]) >= jobs:
+            next(p for _, p in procs if p.poll() is None).wait()
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{out}")
+    link = [hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", *map(str, objs), "-o", str(SAN_LIB)]
+    r = subprocess.run(link, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(link)}\n{r.stdout}{r.stderr}")
+    return SAN_LIB
+
+
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import sys
+    print(build_sanitized() if "--sanitize" in sys.argv else build(verbose=True))

@@ -1,11 +1,12 @@
 """ctypes binding of libheomq.so (the C ABI declared in include/heomq.h).
 
-The library is loaded from this package directory only; a missing library is
-an error, never a fallback.
+The library is loaded from this package directory (or the path HEOMQ_LIBRARY
+names); a missing library is an error, never a fallback.
 """
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 _LIB = None
@@ -122,12 +123,15 @@ SIGNATURES = {
 
 
 def lib():
-    """Load libheomq.so once; raise if it was not built."""
+    """Load libheomq.so once; raise if it was not built.  HEOMQ_LIBRARY names
+    another build of the same library (the host-sanitizer build of the CPU
+    checks)."""
     global _LIB
     if _LIB is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python __graft_entry__.py build)")
-        h = C.CDLL(str(LIB_PATH))
+        path = Path(os.environ.get("HEOMQ_LIBRARY") or LIB_PATH)
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it first (python __graft_entry__.py build)")
+        h = C.CDLL(str(path))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(h, name)
             fn.restype = res

@@ -238,8 +238,12 @@ def test_solvers_write_only_their_documented_stats():
     assert np.all(a[_lib.STATS_MIXED_LEN:] == 12345.0) and a[4] >= 1
     x = ctx.initial_guess()
     a, p = buf(_lib.STATS_LEN)
-    _lib.check(L.heomq_solve_bicg(ctx._h, vec(x), 1e-10, 5000, p, st))
-    assert np.all(a[_lib.STATS_LEN:] == 12345.0)
+    # the reference's BiCG breaks down at its first step on this model
+    # (<f, A_B f> = 0 exactly for f = r0; oracle.bicg raises there too): the
+    # status is the breakdown code, and the stats are written all the same
+    rc = L.heomq_solve_bicg(ctx._h, vec(x), 1e-10, 5000, p, st)
+    assert rc in (_lib.OK, _lib.BREAKDOWN)
+    assert np.all(a[_lib.STATS_LEN:] == 12345.0) and a[1] >= 1
     x = ctx.initial_guess()
     a, p = buf(_lib.STATS_LEN)
     _lib.check(L.heomq_solve_jacobi(ctx._h, vec(x), 1e-8, 20000, 0.0, p, st))
