@@ -193,31 +193,29 @@ void fill_op(heomq_ctx* c, OperatorTables& t, const DevBuf& rec, bool transposed
   o.trace_elem = c->d * c->d - 1;
 }
 
-// i(HX - XH) per symmetry sector as a CSR over packed slots (row t of sector
-// s: the slots q and coefficients v with (i(HX - XH))_t = sum v * X_q).  Built
-// at context creation and again whenever H changes; H must keep the N / S_z
-// conservation the sector layout was built for.
-int build_packed_h(heomq_ctx* c, const Model& m) {
-  const int d = m.d, dd = d * d, S = c->nsec;
-  c->h_diagonal = true;
-  for (int i = 0; i < d; ++i)
-    for (int k = 0; k < d; ++k)
-      if (i != k && m.H[i + (size_t)k * d] != cplx(0, 0)) c->h_diagonal = false;
-  for (int i = 0; i < d; ++i)
-    for (int k = 0; k < d; ++k)
-      if (m.H[i + (size_t)k * d] != cplx(0, 0)) {
-        if ((c->cons_n && c->fock_n[i] != c->fock_n[k]) || (c->cons_s && c->fock_s[i] != c->fock_s[k]))
-          return fail(HEOMQ_UNSUPPORTED,
-                      "H no longer conserves the quantum numbers the packed layout uses: create a new context");
-      }
-  const auto& elems = c->sec_elems_h;
-  const auto& pos = c->sec_pos_h;
-  std::vector<int32_t> hrow((size_t)S * (dd + 1), 0);
-  std::vector<int16_t> hq;
-  std::vector<double> hc;
+// i(HX - XH) per sector as a CSR over its slots (row t of sector s: the slots
+// q and coefficients v with (i(HX - XH))_t = sum v * X_q, elements outside the
+// sector dropped), and the same rows padded to a fixed width when every row
+// is short and the padded table stays under ell_cap bytes.
+struct HTables {
+  std::vector<int32_t> hrow;
+  std::vector<int16_t> hq, hq_ell;
+  std::vector<double> hc, hc_ell;
+  int ew = 0, ewp = 0;
+};
+
+void build_h_tables(const std::vector<cplx>& H, int d, int S, const std::vector<int16_t>& elems,
+                    const std::vector<int16_t>& pos, const std::vector<int>& nnz, size_t ell_cap, HTables& ht) {
+  const int dd = d * d;
+  auto& hrow = ht.hrow;
+  auto& hq = ht.hq;
+  auto& hc = ht.hc;
+  hrow.assign((size_t)S * (dd + 1), 0);
+  hq.clear();
+  hc.clear();
   for (int s = 0; s < S; ++s) {
-    const int nnz = c->sec_nnz_h[s];
-    for (int t = 0; t < nnz; ++t) {
+    const int nz = nnz[s];
+    for (int t = 0; t < nz; ++t) {
       hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
       const int e = elems[(size_t)s * dd + t];
       const int i = e % d, j = e / d;
@@ -229,7 +227,7 @@ int build_packed_h(heomq_ctx* c, const Model& m) {
         terms.push_back({q, v});
       };
       for (int k = 0; k < d; ++k) {
-        const cplx hik = m.H[i + (size_t)k * d], hkj = m.H[k + (size_t)j * d];
+        const cplx hik = H[i + (size_t)k * d], hkj = H[k + (size_t)j * d];
         if (hik != cplx(0, 0)) addterm(pos[(size_t)s * dd + k + j * d], cplx(0, 1) * hik);
         if (hkj != cplx(0, 0)) addterm(pos[(size_t)s * dd + i + k * d], -cplx(0, 1) * hkj);
       }
@@ -240,7 +238,7 @@ int build_packed_h(heomq_ctx* c, const Model& m) {
         hc.push_back(tm.second.imag());
       }
     }
-    for (int t = nnz; t <= dd; ++t) hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
+    for (int t = nz; t <= dd; ++t) hrow[(size_t)s * (dd + 1) + t] = (int32_t)hq.size();
   }
   // fixed-width copy when all rows are short
   int width = 0;
@@ -248,40 +246,61 @@ int build_packed_h(heomq_ctx* c, const Model& m) {
     for (int t = 0; t < dd; ++t)
       width = std::max(width, hrow[(size_t)s * (dd + 1) + t + 1] - hrow[(size_t)s * (dd + 1) + t]);
   const int kMaxEll = 16;
-  const int ew = width <= kMaxEll ? std::max(width, 1) : 0;
+  int ew = width <= kMaxEll ? std::max(width, 1) : 0;
+  if (ew > 0 && (size_t)S * dd * ew * 16 > ell_cap) ew = 0;
   const int ewp = (ew + 7) / 8 * 8;
-  std::vector<int16_t> hq_ell;
-  std::vector<double> hc_ell;
   if (ew > 0) {
-    hq_ell.assign((size_t)S * dd * ewp, 0);
-    hc_ell.assign((size_t)S * dd * ew * 2, 0.0);
+    ht.hq_ell.assign((size_t)S * dd * ewp, 0);
+    ht.hc_ell.assign((size_t)S * dd * ew * 2, 0.0);
     for (int s = 0; s < S; ++s)
       for (int t = 0; t < dd; ++t) {
         const int r0 = hrow[(size_t)s * (dd + 1) + t], r1 = hrow[(size_t)s * (dd + 1) + t + 1];
         for (int u = r0; u < r1; ++u) {
           const size_t at = ((size_t)s * dd + t) * ew + (u - r0);
-          hq_ell[((size_t)s * dd + t) * ewp + (u - r0)] = hq[u];
-          hc_ell[2 * at] = hc[2 * (size_t)u];
-          hc_ell[2 * at + 1] = hc[2 * (size_t)u + 1];
+          ht.hq_ell[((size_t)s * dd + t) * ewp + (u - r0)] = hq[u];
+          ht.hc_ell[2 * at] = hc[2 * (size_t)u];
+          ht.hc_ell[2 * at + 1] = hc[2 * (size_t)u + 1];
         }
       }
   } else {
-    hq_ell.assign(1, 0);
-    hc_ell.assign(2, 0.0);
+    ht.hq_ell.assign(1, 0);
+    ht.hc_ell.assign(2, 0.0);
   }
   if (hq.empty()) { hq.push_back(0); hc.push_back(0); hc.push_back(0); }
-  HIPCHECK(upload(c->p_hrow, hrow));
-  HIPCHECK(upload(c->p_hq, hq));
-  HIPCHECK(upload(c->p_hc, hc));
-  HIPCHECK(upload(c->p_hq_ell, hq_ell));
-  HIPCHECK(upload(c->p_hc_ell, hc_ell));
+  ht.ew = ew;
+  ht.ewp = ewp;
+}
+
+// The solver layout's H tables.  Built at context creation and again whenever
+// H changes; H must keep the N / S_z conservation the sector layout was built
+// for.
+int build_packed_h(heomq_ctx* c, const Model& m) {
+  const int d = m.d;
+  c->h_diagonal = true;
+  for (int i = 0; i < d; ++i)
+    for (int k = 0; k < d; ++k)
+      if (i != k && m.H[i + (size_t)k * d] != cplx(0, 0)) c->h_diagonal = false;
+  for (int i = 0; i < d; ++i)
+    for (int k = 0; k < d; ++k)
+      if (m.H[i + (size_t)k * d] != cplx(0, 0)) {
+        if ((c->cons_n && c->fock_n[i] != c->fock_n[k]) || (c->cons_s && c->fock_s[i] != c->fock_s[k]))
+          return fail(HEOMQ_UNSUPPORTED,
+                      "H no longer conserves the quantum numbers the packed layout uses: create a new context");
+      }
+  HTables ht;
+  build_h_tables(m.H, d, c->nsec, c->sec_elems_h, c->sec_pos_h, c->sec_nnz_h, SIZE_MAX, ht);
+  HIPCHECK(upload(c->p_hrow, ht.hrow));
+  HIPCHECK(upload(c->p_hq, ht.hq));
+  HIPCHECK(upload(c->p_hc, ht.hc));
+  HIPCHECK(upload(c->p_hq_ell, ht.hq_ell));
+  HIPCHECK(upload(c->p_hc_ell, ht.hc_ell));
   for (int w = 0; w < 2; ++w) {
     DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
     p.hrow = c->p_hrow.as<int32_t>();
     p.hq = c->p_hq.as<int16_t>();
     p.hc = c->p_hc.as<double>();
-    p.hell_w = ew;
-    p.hell_wp = ewp;
+    p.hell_w = ht.ew;
+    p.hell_wp = ht.ewp;
     p.hq_ell = c->p_hq_ell.as<int16_t>();
     p.hc_ell = c->p_hc_ell.as<double>();
   }
@@ -291,13 +310,16 @@ int build_packed_h(heomq_ctx* c, const Model& m) {
 // Symmetry-packed layout: sectors, packed offsets, packed coupling entries.
 // Host-side symmetry-packed layout (sectors, offsets, block tables, packed
 // entries and gather maps); built without a device, uploaded by build_packed.
+// A "sector" is an ordered list of the elements an ADO stores; the solver
+// layout takes the (N, S_z) selection rule's lists, the sparse-storage layout
+// the per-ADO lists of a COO pattern (heomq_spa_pattern).
 struct PackedHost {
   bool cons_n = true, cons_s = true;
   std::vector<int> fN, fS;           // electron number and 2 S_z of each Fock state
   int S = 0;                         // sectors
   std::vector<int16_t> elems, pos;   // [S][d*d]
   std::vector<int> nnz;              // elements per sector
-  std::vector<uint8_t> sec;          // sector of each ADO
+  std::vector<uint16_t> sec;         // sector of each ADO
   std::vector<int64_t> off;          // N + 1
   int64_t total = 0;
   std::vector<int32_t> block, lblock, lnz;
@@ -314,6 +336,8 @@ struct PackedHost {
   std::vector<DevEntryP> lt_ent[2];
   int lt_u = 0;
 };
+
+constexpr int kMaxSectors = 65535;
 
 // Sort each heavy ADO's packed entries into runs of one gather map (and one
 // dagger flag) and list the runs; an ADO whose row or accumulator pairs do
@@ -334,7 +358,7 @@ void group_heavy_rows(const std::vector<int64_t>& rows, std::vector<DevEntryP>& 
       while (q < p1 && key(ent[q]) == key(ent[p])) ++q;
       if (ent[p].map >= 0) {
         mine.push_back(make_int4((int)p, (int)q, ent[p].map, P | ((ent[p].flags & 1) ? (int)0x80000000 : 0)));
-        P += nnz[ent[p].nsec];
+        P += nnz[ent[p].ncls];
       }
       p = q;
     }
@@ -348,15 +372,12 @@ void group_heavy_rows(const std::vector<int64_t>& rows, std::vector<DevEntryP>& 
   if (grp.empty()) grp.push_back(make_int4(0, 0, -1, 0));
 }
 
-int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
-                      const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt, PackedHost& ph) {
-  std::vector<DevRec> recs;
-  build_records(m, recs);
-  std::vector<int8_t> omap;
-  build_opmap(m, omap);
-  const int d = m.d, dd = d * d;
-  std::vector<int> fN(d), fS(d);
-  for (int i = 0; i < d; ++i) {
+// Electron number and 2 S_z of every Fock state (occupation basis, 4 states
+// per orbital: empty, up, down, both).
+void fock_numbers(const Model& m, std::vector<int>& fN, std::vector<int>& fS) {
+  fN.assign(m.d, 0);
+  fS.assign(m.d, 0);
+  for (int i = 0; i < m.d; ++i) {
     int n = 0, s2 = 0, v = i;
     for (int o = 0; o < m.norbs; ++o, v /= 4) {
       const int dig = v % 4;
@@ -367,6 +388,15 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
     fN[i] = n;
     fS[i] = s2;
   }
+}
+
+// The solver layout's sectors: ADOs with equal (dN, 2 dS_z) share one list of
+// the elements the selection rule allows, column-major over (i, j).
+int symmetry_sectors(const Model& m, PackedHost& ph) {
+  const int d = m.d, dd = d * d;
+  fock_numbers(m, ph.fN, ph.fS);
+  const auto& fN = ph.fN;
+  const auto& fS = ph.fS;
   ph.cons_n = ph.cons_s = true;
   for (int i = 0; i < d; ++i)
     for (int k = 0; k < d; ++k)
@@ -390,7 +420,7 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
       key[g] = {dN, dS};
     }
   std::vector<std::pair<int, int>> keys;
-  std::vector<uint8_t> sec(m.N);
+  ph.sec.assign(m.N, 0);
   for (int64_t g = 0; g < m.N; ++g) {
     auto it = std::find(keys.begin(), keys.end(), key[g]);
     if (it == keys.end()) {
@@ -398,11 +428,13 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
       keys.push_back(key[g]);
       it = keys.end() - 1;
     }
-    sec[g] = (uint8_t)(it - keys.begin());
+    ph.sec[g] = (uint16_t)(it - keys.begin());
   }
   const int S = (int)keys.size();
-  std::vector<int16_t> elems((size_t)S * dd, -1), pos((size_t)S * dd, -1);
-  std::vector<int> nnz(S, 0);
+  ph.S = S;
+  ph.elems.assign((size_t)S * dd, -1);
+  ph.pos.assign((size_t)S * dd, -1);
+  ph.nnz.assign(S, 0);
   for (int s = 0; s < S; ++s)
     for (int j = 0; j < d; ++j)
       for (int i = 0; i < d; ++i) {
@@ -410,10 +442,30 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
         const bool okn = ph.cons_n ? dn == keys[s].first : (((dn - keys[s].first) % 2) == 0);
         const bool oks = ph.cons_s ? ds == keys[s].second : true;
         if (!okn || !oks) continue;
-        pos[(size_t)s * dd + i + j * d] = (int16_t)nnz[s];
-        elems[(size_t)s * dd + nnz[s]] = (int16_t)(i + j * d);
-        nnz[s]++;
+        ph.pos[(size_t)s * dd + i + j * d] = (int16_t)ph.nnz[s];
+        ph.elems[(size_t)s * dd + ph.nnz[s]] = (int16_t)(i + j * d);
+        ph.nnz[s]++;
       }
+  return HEOMQ_OK;
+}
+
+// Everything that follows from the sectors (ph.S, sec, elems, pos, nnz):
+// packed offsets, block tables, gather maps, packed entries, the grouped heavy
+// rows and the sector-batched top tier.  both = false builds the forward
+// operator's entries only.
+int finish_packed_host(const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
+                       const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt, PackedHost& ph,
+                       bool both) {
+  std::vector<DevRec> recs;
+  build_records(m, recs);
+  std::vector<int8_t> omap;
+  build_opmap(m, omap);
+  const int d = m.d, dd = d * d;
+  const int S = ph.S;
+  const auto& sec = ph.sec;
+  const auto& elems = ph.elems;
+  const auto& pos = ph.pos;
+  const auto& nnz = ph.nnz;
   std::vector<int64_t> off(m.N + 1, 0);
   for (int64_t g = 0; g < m.N; ++g) off[g + 1] = off[g] + nnz[sec[g]];
   const int64_t total = off[m.N];
@@ -423,7 +475,7 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
   {
     int64_t a = 0;
     for (int64_t b = 0; b <= nblk; ++b) {
-      const int64_t t = std::min<int64_t>(b * 256, total - 1);
+      const int64_t t = std::min<int64_t>(b * 256, std::max<int64_t>(total - 1, 0));
       while (a + 1 < m.N && off[a + 1] <= t) ++a;
       block[b] = (int32_t)a;
     }
@@ -492,9 +544,11 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
     }
     const int32_t res = any ? at : -1;
     if (!any) maps.resize(at);
+    if ((int64_t)maps.size() >= ((int64_t)1 << 31) - 1) return -2;
     mapidx[key] = res;
     return res;
   };
+  bool too_many = false;
   auto conv = [&](const std::vector<int64_t>& rows, const std::vector<DevEntry>& e) {
     std::vector<DevEntryP> out(e.size());
     for (int64_t g = 0; g < m.N; ++g)
@@ -502,23 +556,18 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
         const DevRec& r = recs[e[p].rec];
         const int dag = e[p].flags & 1;
         const int s = sec[g], s2 = sec[e[p].nbr];
-        out[p] = DevEntryP{(int32_t)off[e[p].nbr], getmap(s, s2, r.opL, r.opR, dag), -1, e[p].rec, e[p].flags,
-                           (uint8_t)s2};
+        int32_t mp = getmap(s, s2, r.opL, r.opR, dag);
+        if (mp == -2) { too_many = true; mp = -1; }
+        out[p] = DevEntryP{(int32_t)off[e[p].nbr], mp, s2, e[p].rec, e[p].flags, 0};
       }
     return out;
   };
   ph.pentf = conv(rowf, entf);
-  ph.pentt = conv(rowt, entt);
+  if (both) ph.pentt = conv(rowt, entt);
+  if (too_many) return fail(HEOMQ_UNSUPPORTED, "packed layout: gather maps exceed 2^31 entries");
   group_heavy_rows(rowf, ph.pentf, nnz, n_heavy, ph.hg_off[0], ph.hg[0]);
-  group_heavy_rows(rowt, ph.pentt, nnz, n_heavy, ph.hg_off[1], ph.hg[1]);
+  if (both) group_heavy_rows(rowt, ph.pentt, nnz, n_heavy, ph.hg_off[1], ph.hg[1]);
   if (maps.empty()) maps.push_back(0);
-  ph.fN = std::move(fN);
-  ph.fS = std::move(fS);
-  ph.S = S;
-  ph.elems = std::move(elems);
-  ph.pos = std::move(pos);
-  ph.nnz = std::move(nnz);
-  ph.sec = std::move(sec);
   ph.off = std::move(off);
   ph.total = total;
   ph.block = std::move(block);
@@ -528,7 +577,7 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
   ph.light_begin = light_begin;
   ph.maps = std::move(maps);
   ph.tslot = ph.pos[(size_t)ph.sec[0] * dd + (d - 1) + (d - 1) * d];
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < (both ? 2 : 1); ++w) {
     const auto& rows = w == 0 ? rowf : rowt;
     int mx = 0;
     for (int64_t g = n_heavy; g < m.N; ++g) mx = std::max<int>(mx, (int)(rows[g + 1] - rows[g]));
@@ -542,12 +591,13 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
   for (int s = 0; s < S && lt_ok; ++s)
     if (ph.nnz[s] > 256) lt_ok = false;
   if (lt_ok) {
+    std::vector<std::vector<int64_t>> by_sec(S);
+    for (int64_t a = n_heavy; a < m.N; ++a) by_sec[ph.sec[a]].push_back(a);
     for (int s = 0; s < S; ++s) {
       if (ph.nnz[s] == 0) continue;
       const int chunk = 256 / ph.nnz[s];
       int in_block = 0;
-      for (int64_t a = n_heavy; a < m.N; ++a) {
-        if (ph.sec[a] != s) continue;
+      for (int64_t a : by_sec[s]) {
         if (in_block == 0) ph.lt_blk.push_back(make_int4((int)ph.lt_ados.size(), 0, ph.nnz[s], s));
         ph.lt_ados.push_back((int32_t)a);
         ph.lt_off.push_back((int32_t)ph.off[a]);
@@ -555,7 +605,7 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
         if (++in_block == chunk) in_block = 0;
       }
     }
-    for (int w = 0; w < 2; ++w) {
+    for (int w = 0; w < (both ? 2 : 1); ++w) {
       const auto& rows = w == 0 ? rowf : rowt;
       const auto& pent = w == 0 ? ph.pentf : ph.pentt;
       auto& out = ph.lt_ent[w];
@@ -569,6 +619,11 @@ int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const st
   return HEOMQ_OK;
 }
 
+int build_packed_host(const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
+                      const std::vector<int64_t>& rowt, const std::vector<DevEntry>& entt, PackedHost& ph) {
+  if (int rc = symmetry_sectors(m, ph)) return rc;
+  return finish_packed_host(m, rowf, entf, rowt, entt, ph, true);
+}
 void set_apply_form(heomq_ctx* c, int w, int form);
 
 int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf, const std::vector<DevEntry>& entf,
@@ -613,7 +668,7 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
     p.n_ado = m.N;
     p.total = ph.total;
     p.off = c->p_off.as<int64_t>();
-    p.sec = c->p_sec.as<uint8_t>();
+    p.sec = c->p_sec.as<uint16_t>();
     p.sec_elems = c->p_elems.as<int16_t>();
     p.sec_pos = c->p_pos.as<int16_t>();
     p.block_ado = c->p_block.as<int32_t>();
@@ -679,6 +734,277 @@ int64_t trace_pos(const heomq_ctx* c, int trace) {
 }
 
 }  // namespace
+
+namespace heomq {
+
+int build_spa_h(heomq_ctx* c) {
+  SpaOperator& so = c->spa;
+  HTables ht;
+  // the fixed-width rows are kept while their table stays under 256 MiB
+  build_h_tables(c->model.H, c->d, so.nsec, so.elems_h, so.pos_h, so.nnz_h, (size_t)256 << 20, ht);
+  HIPCHECK(upload(so.hrow, ht.hrow));
+  HIPCHECK(upload(so.hq, ht.hq));
+  HIPCHECK(upload(so.hc, ht.hc));
+  HIPCHECK(upload(so.hq_ell, ht.hq_ell));
+  HIPCHECK(upload(so.hc_ell, ht.hc_ell));
+  DevPacked& p = so.op;
+  p.hrow = so.hrow.as<int32_t>();
+  p.hq = so.hq.as<int16_t>();
+  p.hc = so.hc.as<double>();
+  p.hell_w = ht.ew;
+  p.hell_wp = ht.ewp;
+  p.hq_ell = so.hq_ell.as<int16_t>();
+  p.hc_ell = so.hc_ell.as<double>();
+  return HEOMQ_OK;
+}
+
+// The pattern's sectors are its distinct per-ADO element lists (in storage
+// order, so rho_spa is the packed vector of this layout as it stands); every
+// table of the packed apply then follows as for the symmetry sectors, with
+// neighbour elements outside a neighbour's list absent from the gather maps
+// (P^T: they are zero) and output elements outside the list never formed (P).
+int build_spa_operator(heomq_ctx* c, const int32_t* nnz_spa, const std::vector<int64_t>& start,
+                       const int32_t* irow, const int32_t* icol) {
+  const Model& m = c->model;
+  const int d = m.d, dd = d * d;
+  PackedHost ph;
+  std::map<std::vector<int16_t>, int> cls;
+  std::vector<const std::vector<int16_t>*> lists;
+  ph.sec.assign(m.N, 0);
+  std::vector<int16_t> key;
+  for (int64_t a = 0; a < m.N; ++a) {
+    const int nz = nnz_spa[a];
+    key.resize(nz);
+    for (int e = 0; e < nz; ++e)
+      key[e] = (int16_t)((irow[start[a] + e] - 1) + (icol[start[a] + e] - 1) * d);
+    auto it = cls.find(key);
+    if (it == cls.end()) {
+      if ((int)lists.size() >= kMaxSectors)
+        return fail(HEOMQ_UNSUPPORTED, "sparse pattern: more than 65535 distinct per-ADO element lists");
+      it = cls.emplace(key, (int)lists.size()).first;
+      lists.push_back(&it->first);
+    }
+    ph.sec[a] = (uint16_t)it->second;
+  }
+  const int S = (int)lists.size();
+  if ((size_t)S * dd * 4 > ((size_t)1 << 30))
+    return fail(HEOMQ_UNSUPPORTED, "sparse pattern: per-list tables over 1 GiB (too many distinct element lists)");
+  ph.S = S;
+  ph.elems.assign((size_t)S * dd, -1);
+  ph.pos.assign((size_t)S * dd, -1);
+  ph.nnz.assign(S, 0);
+  for (int s = 0; s < S; ++s) {
+    const auto& l = *lists[s];
+    ph.nnz[s] = (int)l.size();
+    for (size_t t = 0; t < l.size(); ++t) {
+      ph.elems[(size_t)s * dd + t] = l[t];
+      ph.pos[(size_t)s * dd + l[t]] = (int16_t)t;
+    }
+  }
+  std::vector<int64_t> rowf, rowt;
+  std::vector<DevEntry> entf, entt;
+  if (int rc = build_entries(m, rowf, entf, rowt, entt)) return rc;
+  if (int rc = finish_packed_host(m, rowf, entf, rowt, entt, ph, false)) return rc;
+  SpaOperator& so = c->spa;
+  so.built = false;
+  HIPCHECK(upload(so.off, ph.off));
+  HIPCHECK(upload(so.sec, ph.sec));
+  HIPCHECK(upload(so.elems, ph.elems));
+  HIPCHECK(upload(so.pos, ph.pos));
+  HIPCHECK(upload(so.block, ph.block));
+  HIPCHECK(upload(so.lblock, ph.lblock));
+  HIPCHECK(upload(so.lnz, ph.lnz));
+  HIPCHECK(upload(so.ent, ph.pentf));
+  HIPCHECK(upload(so.maps, ph.maps));
+  HIPCHECK(upload(so.hgoff, ph.hg_off[0]));
+  HIPCHECK(upload(so.hg, ph.hg[0]));
+  // the top tier batched by element list when the blocks stay mostly full
+  // (a pattern with few distinct lists), else in rho_spa order
+  int64_t top = ph.total - ph.light_begin;
+  const bool batched = !ph.lt_blk.empty() && top * 2 >= (int64_t)ph.lt_blk.size() * 256;
+  if (batched) {
+    HIPCHECK(upload(so.ltblk, ph.lt_blk));
+    HIPCHECK(upload(so.ltados, ph.lt_ados));
+    HIPCHECK(upload(so.ltoff, ph.lt_off));
+    HIPCHECK(upload(so.ltent, ph.lt_ent[0]));
+  }
+  so.nsec = S;
+  so.elems_h = std::move(ph.elems);
+  so.pos_h = std::move(ph.pos);
+  so.nnz_h = std::move(ph.nnz);
+  DevPacked p = c->pfwd;
+  p.n_ado = m.N;
+  p.total = ph.total;
+  p.off = so.off.as<int64_t>();
+  p.sec = so.sec.as<uint16_t>();
+  p.sec_elems = so.elems.as<int16_t>();
+  p.sec_pos = so.pos.as<int16_t>();
+  p.block_ado = so.block.as<int32_t>();
+  p.rowptr = c->fwd.rowptr.as<int64_t>();
+  p.ent = so.ent.as<DevEntryP>();
+  p.trace_slot = -1;
+  p.maps = so.maps.as<int32_t>();
+  p.n_heavy = ph.n_heavy;
+  p.light_begin = ph.light_begin;
+  p.light_block_ado = so.lblock.as<int32_t>();
+  p.light_nz = so.lnz.as<int32_t>();
+  p.max_light_couplings = ph.max_light[0];
+  p.hg_off = so.hgoff.as<int32_t>();
+  p.hg = so.hg.as<int4>();
+  p.n_lt_blocks = batched ? (int)ph.lt_blk.size() : 0;
+  p.lt_u = ph.lt_u;
+  p.lt_blk = batched ? so.ltblk.as<int4>() : nullptr;
+  p.lt_off = batched ? so.ltoff.as<int32_t>() : nullptr;
+  p.lt_id = batched ? so.ltados.as<int32_t>() : nullptr;
+  p.lt_ent = batched ? so.ltent.as<DevEntryP>() : nullptr;
+  p.precond = 0;
+  p.herm0 = 0;
+  p.nprob = 1;
+  p.vstride = p.rec_stride = p.gam_stride = p.hc_stride = 0;
+  p.frozen_flag = nullptr;
+  so.op = p;
+  if (int rc = build_spa_h(c)) return rc;
+  so.built = true;
+  return HEOMQ_OK;
+}
+
+DevPacked spa_operator(const heomq_ctx* c, const double* gam) {
+  DevPacked p = c->spa.op;
+  const DevPacked& f = c->pfwd;
+  p.rec = f.rec;
+  p.gam = gam ? gam : f.gam;
+  p.opmap = f.opmap;
+  p.hnnz = f.hnnz;
+  p.hcol = f.hcol;
+  p.hval = f.hval;
+  p.nop = f.nop;
+  p.d = f.d;
+  p.hsign = f.hsign;
+  p.transposed = false;
+  return p;
+}
+
+}  // namespace heomq
+
+namespace heomq {
+
+// The derivative terminator's tables (calc_ax_tfqmr_omp.f90:362-478): the
+// hierarchy one tier higher, built from the same parameters.  Its ADOs of
+// tiers 0..L are the context's (same tuples, same order, same packed
+// elements, checked here), so the context's packed vectors are the prefix of
+// its packed vectors; the apply eliminates its top tier in every call.
+int build_trun(heomq_ctx* c) {
+  TrunOperator& tr = c->trun;
+  tr.built = false;
+  const Model& m = c->model;
+  if (m.L < 1) return fail(HEOMQ_UNSUPPORTED, "derivative terminator: needs a top tier L >= 1");
+  if (m.p.ntop != m.L || m.p.nmats + 1 != m.ncor || m.p.norbs != m.norbs)
+    return fail(HEOMQ_UNSUPPORTED, "derivative terminator: needs a model built from parameters (heomq_model_build)");
+  heomq_params px = m.p;
+  px.ntop = m.L + 1;
+  Model mx;
+  if (int rc = build_model(px, mx)) return rc;
+  for (int n = 0; n <= m.L; ++n)
+    if (mx.first[n] != m.first[n] || mx.tiers[n] != m.tiers[n])
+      return fail(HEOMQ_UNSUPPORTED, "derivative terminator: the extended hierarchy's lower tiers differ");
+  if (mx.first[m.L + 1] != m.N) return fail(HEOMQ_UNSUPPORTED, "derivative terminator: tier sizes differ");
+  PackedHost ph;
+  if (int rc = symmetry_sectors(mx, ph)) return rc;
+  std::vector<int64_t> rowf, rowt;
+  std::vector<DevEntry> entf, entt;
+  if (int rc = build_entries(mx, rowf, entf, rowt, entt)) return rc;
+  if (int rc = finish_packed_host(mx, rowf, entf, rowt, entt, ph, false)) return rc;
+  const int dd = m.d * m.d;
+  for (int64_t a = 0; a < m.N; ++a) {
+    const int s = ph.sec[a], s0 = c->psec_h[a];
+    if (ph.off[a] != c->poff_h[a] || ph.nnz[s] != c->sec_nnz_h[s0] ||
+        !std::equal(ph.elems.begin() + (size_t)s * dd, ph.elems.begin() + (size_t)s * dd + ph.nnz[s],
+                    c->sec_elems_h.begin() + (size_t)s0 * dd))
+      return fail(HEOMQ_UNSUPPORTED, "derivative terminator: the extended packed layout's prefix differs");
+  }
+  HIPCHECK(upload(tr.off, ph.off));
+  HIPCHECK(upload(tr.sec, ph.sec));
+  HIPCHECK(upload(tr.elems, ph.elems));
+  HIPCHECK(upload(tr.pos, ph.pos));
+  HIPCHECK(upload(tr.block, ph.block));
+  HIPCHECK(upload(tr.lblock, ph.lblock));
+  HIPCHECK(upload(tr.lnz, ph.lnz));
+  HIPCHECK(upload(tr.rowptr, rowf));
+  HIPCHECK(upload(tr.ent, ph.pentf));
+  HIPCHECK(upload(tr.maps, ph.maps));
+  HIPCHECK(upload(tr.hgoff, ph.hg_off[0]));
+  HIPCHECK(upload(tr.hg, ph.hg[0]));
+  const bool batched = !ph.lt_blk.empty();
+  if (batched) {
+    HIPCHECK(upload(tr.ltblk, ph.lt_blk));
+    HIPCHECK(upload(tr.ltados, ph.lt_ados));
+    HIPCHECK(upload(tr.ltoff, ph.lt_off));
+    HIPCHECK(upload(tr.ltent, ph.lt_ent[0]));
+  }
+  std::vector<double> gam;
+  build_gamma(mx, mx.gdraw, gam, false);
+  HIPCHECK(upload(tr.gam, gam));
+  HTables ht;
+  build_h_tables(mx.H, mx.d, ph.S, ph.elems, ph.pos, ph.nnz, SIZE_MAX, ht);
+  HIPCHECK(upload(tr.hrow, ht.hrow));
+  HIPCHECK(upload(tr.hq, ht.hq));
+  HIPCHECK(upload(tr.hc, ht.hc));
+  HIPCHECK(upload(tr.hq_ell, ht.hq_ell));
+  HIPCHECK(upload(tr.hc_ell, ht.hc_ell));
+  if (int rc = build_tier_jacobi(mx, ph.S, ph.elems, ph.pos, ph.nnz, ph.off, m.N, tr.jblk, tr.jdesc, tr.ju, tr.je,
+                                 tr.jac))
+    return rc;
+  // coefficient records: the extended hierarchy has the context's drawers and bath
+  DevPacked p = c->pfwd;
+  p.n_ado = mx.N;
+  p.total = ph.total;
+  p.off = tr.off.as<int64_t>();
+  p.sec = tr.sec.as<uint16_t>();
+  p.sec_elems = tr.elems.as<int16_t>();
+  p.sec_pos = tr.pos.as<int16_t>();
+  p.block_ado = tr.block.as<int32_t>();
+  p.rowptr = tr.rowptr.as<int64_t>();
+  p.ent = tr.ent.as<DevEntryP>();
+  p.gam = tr.gam.as<double>();
+  p.trace_slot = ph.tslot;
+  p.maps = tr.maps.as<int32_t>();
+  p.n_heavy = ph.n_heavy;
+  p.light_begin = ph.light_begin;
+  p.light_block_ado = tr.lblock.as<int32_t>();
+  p.light_nz = tr.lnz.as<int32_t>();
+  p.max_light_couplings = ph.max_light[0];
+  p.hg_off = tr.hgoff.as<int32_t>();
+  p.hg = tr.hg.as<int4>();
+  p.n_lt_blocks = batched ? (int)ph.lt_blk.size() : 0;
+  p.lt_u = ph.lt_u;
+  p.lt_blk = batched ? tr.ltblk.as<int4>() : nullptr;
+  p.lt_off = batched ? tr.ltoff.as<int32_t>() : nullptr;
+  p.lt_id = batched ? tr.ltados.as<int32_t>() : nullptr;
+  p.lt_ent = batched ? tr.ltent.as<DevEntryP>() : nullptr;
+  p.hrow = tr.hrow.as<int32_t>();
+  p.hq = tr.hq.as<int16_t>();
+  p.hc = tr.hc.as<double>();
+  p.hell_w = ht.ew;
+  p.hell_wp = ht.ewp;
+  p.hq_ell = tr.hq_ell.as<int16_t>();
+  p.hc_ell = tr.hc_ell.as<double>();
+  p.precond = 0;
+  p.herm0 = 0;
+  p.nprob = 1;
+  p.vstride = p.rec_stride = p.gam_stride = p.hc_stride = 0;
+  p.frozen_flag = nullptr;
+  if (p.trace_slot != c->pfwd.trace_slot) return fail(HEOMQ_UNSUPPORTED, "derivative terminator: trace slot differs");
+  tr.op = p;
+  HIPCHECK(tr.xext.alloc(sizeof(double) * 2 * (size_t)ph.total));
+  HIPCHECK(tr.text.alloc(sizeof(double) * 2 * (size_t)ph.total));
+  tr.total = ph.total;
+  tr.vis = c->ptotal;
+  tr.built = true;
+  return HEOMQ_OK;
+}
+
+}  // namespace heomq
+
 
 extern "C" {
 
@@ -779,6 +1105,25 @@ int heomq_model_packed_layout(const heomq_model* h, int64_t* info) {
   return HEOMQ_OK;
 }
 
+int heomq_spa_build_pattern(const heomq_model* h, double dsmall, uint64_t seed, int64_t cap, int64_t* lunk_spa,
+                            int32_t* nnz_spa, int64_t* ind_spa, int32_t* irow_spa, int32_t* icol_spa) {
+  if (!h || !lunk_spa) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (!(dsmall >= 0.0)) return fail(HEOMQ_BAD_ARG, "dsmall must be >= 0");
+  std::vector<int32_t> nnz, ir, ic;
+  if (int rc = spa_pattern_build(h->m, dsmall, seed, nnz, ir, ic)) return rc;
+  *lunk_spa = (int64_t)ir.size();
+  if (nnz_spa) std::memcpy(nnz_spa, nnz.data(), nnz.size() * sizeof(int32_t));
+  if (ind_spa) {
+    int64_t run = 1;
+    for (size_t a = 0; a < nnz.size(); ++a) { ind_spa[a] = run; run += nnz[a]; }
+  }
+  if (!irow_spa || !icol_spa) return HEOMQ_OK;
+  if (cap < (int64_t)ir.size()) return fail(HEOMQ_BAD_DIM, "sparse pattern: irow_spa / icol_spa shorter than lunk_spa");
+  std::memcpy(irow_spa, ir.data(), ir.size() * sizeof(int32_t));
+  std::memcpy(icol_spa, ic.data(), ic.size() * sizeof(int32_t));
+  return HEOMQ_OK;
+}
+
 int heomq_model_set_params(heomq_model* h, const heomq_params* p) {
   if (!h || !p) return fail(HEOMQ_BAD_ARG, "null argument");
   return set_params(h->m, *p);
@@ -812,6 +1157,7 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
   HIPCHECK(hipSetDevice(device));
   auto c = std::make_unique<heomq_ctx>();
   c->device = device;
+  c->model = m;
   c->d = m.d; c->M = m.M; c->L = m.L; c->N = m.N;
   c->nspin = m.nspin; c->norbs = m.norbs; c->nalf = m.nalf;
   c->first = m.first;
@@ -937,8 +1283,26 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
   fill_op(c, c->trn, c->rec, true, +1.0);
   sync_packed(c);
   if (int rc = build_jacobi(c, h->m)) return rc;
+  c->model = h->m;
+  if (c->spa.built)
+    if (int rc = build_spa_h(c)) return rc;
+  if (c->trun.built)
+    if (int rc = build_trun(c)) return rc;
   c->steady_model = h->m.steady;
   c->field_amp = h->m.eleadinfty;
+  HIPCHECK(hipDeviceSynchronize());
+  return HEOMQ_OK;
+}
+
+int heomq_ctx_set_trun_der(heomq_ctx* c, int enable) {
+  if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
+  HIPCHECK(hipSetDevice(c->device));
+  c->drop_graphs();   // captured chunks hold the apply's kernels
+  if (!enable) {
+    c->trun.built = false;
+    return HEOMQ_OK;
+  }
+  if (int rc = build_trun(c)) return rc;
   HIPCHECK(hipDeviceSynchronize());
   return HEOMQ_OK;
 }
@@ -947,18 +1311,21 @@ int64_t heomq_ctx_vector_len(const heomq_ctx* c) { return c ? c->N * c->d * c->d
 
 int heomq_apply(heomq_ctx* c, const double* x, double* y, int trace, void* stream) {
   if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_apply: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   HIPCHECK(launch_apply_f64(c->fwd.op, x, y, trace_pos(c, trace), (hipStream_t)stream));
   return HEOMQ_OK;
 }
 
 int heomq_apply_transpose(heomq_ctx* c, const double* x, double* y, int trace, void* stream) {
   if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_apply_transpose: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   HIPCHECK(launch_apply_f64(c->trn.op, x, y, trace_pos(c, trace), (hipStream_t)stream));
   return HEOMQ_OK;
 }
 
 int heomq_apply_f32(heomq_ctx* c, const float* x, float* y, int trace, void* stream) {
   if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_apply_f32: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   HIPCHECK(launch_apply_f32(c->fwd.op, x, y, trace_pos(c, trace), (hipStream_t)stream));
   return HEOMQ_OK;
 }
@@ -966,6 +1333,7 @@ int heomq_apply_f32(heomq_ctx* c, const float* x, float* y, int trace, void* str
 int heomq_apply_dparam(heomq_ctx* c, const double* x, double* y, const double* dH, const double* dcb,
                        const double* dcd, const double* dgam, void* stream) {
   if (!c || !x || !y || !dH || !dcb || !dcd || !dgam) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_apply_dparam: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (c->lscale)   // the scaling factors |cb|^(1/2) are not affine in the parameters
     return fail(HEOMQ_UNSUPPORTED, "derivative apply: lscale models are not supported");
   const int M = c->M, d = c->d;

@@ -63,8 +63,36 @@ struct OperatorTables {
   heomq::DevOperator op;
 };
 
+// Sparse-storage operator (calc_ax_tfqmr_spa_omp): the packed apply's tables
+// over "sectors" that are the distinct per-ADO element lists of a COO pattern,
+// so y = P A P^T x runs directly on rho_spa (no dense scratch).
+struct SpaOperator {
+  DevBuf off, sec, elems, pos, block, lblock, lnz, ent, maps, hgoff, hg, ltblk, ltados, ltoff, ltent;
+  DevBuf hrow, hq, hc, hq_ell, hc_ell;
+  heomq::DevPacked op;               // structural fields; coefficients come from the context's operator
+  int nsec = 0;
+  bool built = false;
+  std::vector<int16_t> elems_h, pos_h;   // host copies (H tables are rebuilt when H changes)
+  std::vector<int> nnz_h;
+};
+
+// Derivative terminator of the top tier (ltrun_der): the packed tables of the
+// hierarchy one tier higher.  Its tiers 0..L are the context's own ADOs at
+// the same packed offsets; its top tier L + 1 holds the ADOs the terminator
+// eliminates in every apply (their steady equation without upper couplings).
+struct TrunOperator {
+  DevBuf off, sec, elems, pos, block, lblock, lnz, rowptr, ent, maps, hgoff, hg, ltblk, ltados, ltoff, ltent;
+  DevBuf gam, hrow, hq, hc, hq_ell, hc_ell, jblk, jdesc, ju, je;
+  DevBuf xext, text;                 // the extended iterate, the eliminated tier's right-hand sides
+  heomq::DevPacked op;               // every table of the extended hierarchy (records: the context's)
+  heomq::DevJacobi jac;              // (-S_V)^-1 on the eliminated tier (shift 0)
+  int64_t total = 0, vis = 0;        // packed elements of the extended / the context's hierarchy
+  bool built = false;
+};
+
 struct heomq_ctx {
   int device = 0;
+  heomq::Model model;                // the model the tables were built from (sparse layouts are built later)
   int d = 0, M = 0, L = 0, nspin = 2, norbs = 1, nalf = 2;
   int64_t N = 0, n1first = 1, n1count = 0;
   std::vector<int64_t> first;
@@ -90,7 +118,7 @@ struct heomq_ctx {
   std::vector<int16_t> sec_elems_h, sec_pos_h;
   std::vector<int> sec_nnz_h, fock_n, fock_s;
   std::vector<int64_t> poff_h;       // packed offset of each ADO (N + 1)
-  std::vector<uint8_t> psec_h;       // sector of each ADO
+  std::vector<uint16_t> psec_h;      // sector of each ADO
   DevBuf parity;                     // electron-number parity of each Fock state
   // symmetry-packed layout (solver path)
   DevBuf p_off, p_sec, p_elems, p_pos, p_block, p_lblock, p_entf, p_entt, p_hrow, p_hq, p_hc, p_maps;
@@ -160,6 +188,8 @@ struct heomq_ctx {
   DevBuf spa_map, spa_diag, spa_dense, spa_io;
   int64_t spa_lunk = -1;
   int spa_ndiag = 0;
+  SpaOperator spa;
+  TrunOperator trun;
   heomq::DevJacobi pjac;
   double ejac_default = 0.0, ejac_gap = 0.0, ejac_cb = 0.0;
   // shift of the block-Jacobi preconditioner of TFQMR: (shift - S_K)^-1 per
@@ -205,6 +235,26 @@ constexpr double kPrecShiftFactor = 0.05;
 constexpr int64_t kNoAllocElems = int64_t(1) << 21;
 constexpr int kMaxDevices = 64;
 int scratch_address(int device, double** out);
+// The sparse-storage operator of a validated COO pattern: start[a] is ADO a's
+// first element (0-based) in rho_spa, irow / icol 1-based per element.
+int build_spa_operator(heomq_ctx* c, const int32_t* nnz_spa, const std::vector<int64_t>& start,
+                       const int32_t* irow, const int32_t* icol);
+// Its H tables again (after H changed).
+int build_spa_h(heomq_ctx* c);
+// The derivative terminator's tables from the context's model (built from
+// parameters: the hierarchy one tier higher is built the same way).
+int build_trun(heomq_ctx* c);
+// The same-tier inverse (-S_K)^-1 (shift 0) of the ADOs [a0, N) of a packed
+// layout (sectors S, elems, pos, nnz; offsets off), ADO-aligned blocks; the
+// eigensystem comes from m.H.
+int build_tier_jacobi(const Model& m, int S, const std::vector<int16_t>& elems, const std::vector<int16_t>& pos,
+                      const std::vector<int>& nnz, const std::vector<int64_t>& off, int64_t a0, DevBuf& blk_buf,
+                      DevBuf& desc_buf, DevBuf& u_buf, DevBuf& e_buf, DevJacobi& out);
+// y = (A + T) x on the context's packed layout with T the derivative
+// terminator; precond: the apply's Jacobi scaling flag; herm0 as DevPacked.
+hipError_t trun_apply(heomq_ctx* c, const double* x, double* y, bool rdo, int precond, int herm0, hipStream_t st);
+// The sparse operator with the context's current coefficients (rates gam).
+heomq::DevPacked spa_operator(const heomq_ctx* c, const double* gam);
 // The forward operator at time t (the context's own when no time-dependent
 // field is set; else gamma_K(t) written into the context's td_gam on st).
 int operator_at(heomq_ctx* c, double t, hipStream_t st, DevOperator& op);

@@ -239,6 +239,7 @@ int heomq_ctx_field_shifts(const heomq_ctx* c, double tt, double* eshift_ev) {
 
 int heomq_calcderiv_at(heomq_ctx* c, double tt, const double* x, double* y, void* stream) {
   if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calcderiv_at: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   hipStream_t st = (hipStream_t)stream;
   DevOperator op;
   if (int rc = operator_at(c, tt, st, op)) return rc;

@@ -246,6 +246,66 @@ int jacobi_presmooth(heomq_ctx* c, double* x, int nsweep, double* r, double* t, 
   return HEOMQ_OK;
 }
 
+int build_tier_jacobi(const Model& m, int S, const std::vector<int16_t>& elems, const std::vector<int16_t>& pos,
+                      const std::vector<int>& nnz, const std::vector<int64_t>& off, int64_t a0, DevBuf& blk_buf,
+                      DevBuf& desc_buf, DevBuf& u_buf, DevBuf& e_buf, DevJacobi& out) {
+  JacobiHost jh;
+  if (int rc = build_jacobi_host(m, jh)) return rc;
+  const int d = m.d, dd = d * d;
+  if (d > 16 || jh.gmax > 4)
+    return fail(HEOMQ_UNSUPPORTED, "derivative terminator: the eliminated tier's inverse needs d <= 16 and H groups "
+                                   "of at most 4 states");
+  std::vector<int64_t> blk;
+  for (int64_t a = a0; a < m.N; ++a) {
+    const int64_t sz = off[a + 1] - off[a];
+    if (sz > 256) return fail(HEOMQ_UNSUPPORTED, "derivative terminator: an eliminated ADO holds more than 256 elements");
+    if (blk.empty() || off[a + 1] - blk[blk.size() - 1] > 256 || a - blk[blk.size() - 2] >= 256) {
+      blk.push_back(a);
+      blk.push_back(off[a]);
+    }
+  }
+  std::vector<JacDesc> desc((size_t)S * dd);
+  for (int s = 0; s < S; ++s)
+    for (int t = 0; t < nnz[s]; ++t) {
+      const int e = elems[(size_t)s * dd + t], r = e % d, col = e / d;
+      JacDesc& ds = desc[(size_t)s * dd + t];
+      ds.r = (uint8_t)r;
+      ds.c = (uint8_t)col;
+      ds.nq = (uint8_t)jh.gn[col];
+      ds.np = (uint8_t)jh.gn[r];
+      for (int q = 0; q < jh.gn[col]; ++q) {
+        const int c2 = jh.grp[(size_t)col * jh.gmax + q];
+        ds.c2[q] = (uint8_t)c2;
+        ds.cslot[q] = pos[(size_t)s * dd + r + c2 * d];
+        if (ds.cslot[q] < 0) return fail(HEOMQ_UNSUPPORTED, "block-Jacobi groups do not match the packed layout");
+      }
+      for (int p = 0; p < jh.gn[r]; ++p) {
+        const int r2 = jh.grp[(size_t)r * jh.gmax + p];
+        ds.r2[p] = (uint8_t)r2;
+        ds.rslot[p] = pos[(size_t)s * dd + r2 + col * d];
+        if (ds.rslot[p] < 0) return fail(HEOMQ_UNSUPPORTED, "block-Jacobi groups do not match the packed layout");
+      }
+    }
+  out = DevJacobi{};
+  out.ado_blocks = (int64_t)blk.size() / 2;
+  blk.push_back(m.N);
+  blk.push_back(off[m.N]);
+  HIPCHECK(upload(blk_buf, blk));
+  HIPCHECK(upload(desc_buf, desc));
+  HIPCHECK(upload(u_buf, jh.u));
+  HIPCHECK(upload(e_buf, jh.e));
+  out.u = u_buf.as<double>();
+  out.e = e_buf.as<double>();
+  out.gmax = jh.gmax;
+  out.blk = blk_buf.as<int64_t>();
+  out.desc = desc_buf.as<JacDesc>();
+  bool real = true;
+  for (size_t e = 1; e < jh.u.size(); e += 2) real = real && jh.u[e] == 0.0;
+  out.u_real = real ? 1 : 0;
+  out.ejac = 0.0;   // (0 - S_K)^-1: the eliminated ADO's steady equation
+  return HEOMQ_OK;
+}
+
 int build_jacobi(heomq_ctx* c, const Model& m) {
   JacobiHost jh;
   if (int rc = build_jacobi_host(m, jh)) return rc;
@@ -443,6 +503,7 @@ int heomq_jacobi_info(const heomq_ctx* c, double* info) {
 
 int heomq_jacobi_step(heomq_ctx* c, const double* x, double* y, double ejac, void* stream) {
   if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_jacobi_step: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (ejac <= 0.0) ejac = c->ejac_default;
   if (int rc = check_shift(c, ejac)) return rc;
   hipStream_t st = (hipStream_t)stream;
@@ -458,6 +519,7 @@ int heomq_jacobi_step(heomq_ctx* c, const double* x, double* y, double ejac, voi
 
 int heomq_calc_ax_jacobi(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_host) {
   if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calc_ax_jacobi: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_ax_jacobi_omp");
   const int64_t n = 2 * c->N * c->d * c->d;
   HIPCHECK(c->ws_host.alloc(2 * sizeof(double) * (size_t)n));
@@ -476,6 +538,7 @@ int heomq_calc_ax_jacobi(heomq_ctx* c, int32_t dim0, const double* x_host, doubl
 int heomq_solve_jacobi(heomq_ctx* c, double* x, double crit, int maxiter, double ejac, double* stats,
                        void* stream) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_solve_jacobi: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (ejac <= 0.0) ejac = c->ejac_default;
   if (int rc = check_shift(c, ejac)) return rc;
   hipStream_t st = (hipStream_t)stream;
@@ -532,6 +595,7 @@ int heomq_solve_jacobi(heomq_ctx* c, double* x, double crit, int maxiter, double
 
 int heomq_solve_jacobi_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats) {
   if (!c || !rho_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_solve_jacobi_host: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   const int64_t n = 2 * c->N * c->d * c->d;
   HIPCHECK(c->ws_host.alloc(sizeof(double) * (size_t)n));
   double* x = c->ws_host.as<double>();

@@ -64,10 +64,10 @@ struct DevEntryP {       // packed coupling (16 bytes)
   int32_t base;          // neighbour's first packed element
   int32_t map;           // offset of the paired gather map (output slot -> two signed neighbour
                          // slots, left and right operator, as int16 halves); -1: no term
-  int32_t spare;
+  int32_t ncls;          // the neighbour's sector (host-side table building only)
   uint16_t rec;          // coefficient record (cL, cR)
   uint8_t flags;         // bit0 X^H, bit1 sL = -1, bit2 sR = -1
-  uint8_t nsec;
+  uint8_t pad;
 };
 
 // LDS limits of the restructured packed apply (checked when its tables are built)
@@ -79,7 +79,7 @@ constexpr int kLightMaxU = 8;      // couplings of one top-tier ADO
 struct DevPacked {
   int64_t n_ado = 0, total = 0;      // ADOs, packed complex elements
   const int64_t* off = nullptr;      // packed offset of each ADO (n_ado + 1)
-  const uint8_t* sec = nullptr;      // sector of each ADO
+  const uint16_t* sec = nullptr;     // sector of each ADO
   const int16_t* sec_elems = nullptr;   // [S][d*d]: element index i + j*d of slot t
   const int16_t* sec_pos = nullptr;     // [S][d*d]: slot of element i + j*d, -1 if forbidden
   const int32_t* block_ado = nullptr;   // ADO of packed element 256*b (one per 256-thread block, + sentinel)

@@ -643,3 +643,132 @@ int model_from_reference(
 }
 
 }  // namespace heomq
+
+namespace heomq {
+
+namespace {
+// splitmix64 and a Box-Muller normal from it: the mixing coefficients of the
+// pattern builder, a pure function of (seed, ADO, coupling, slot)
+uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double normal_of(uint64_t key) {
+  const double u1 = ((mix64(key) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((mix64(key ^ 0xA5A5A5A5A5A5A5A5ull) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+}  // namespace
+
+// The sparse-mode pattern of fermion/prelude_spa.f90:532-749.  Tier 0 holds
+// the nonzeros of E = sum_{k <= 10} (a H)^k / k!, a = 0.31415926
+// (utility/calc_exp_hmat.f90, H already / hbar; prelude_spa.f90:231-238
+// requires 10 and 11 terms to give the same sparsity), with E as its values.
+// An ADO K of tier n >= 1 sums, over its lower neighbours J (K without drawer
+// k), m1 E (op_k X) + m2 E (X op_k) with X = rho_J (or rho_J^H when J is
+// stored as K's partner's neighbour) and op_k = c^+ for sigma = +, c for
+// sigma = - (:640-686); m1, m2 are random complex normals that keep the two
+// products from cancelling (:676-683).  K's pattern is the elements of the sum
+// above dsmall in column-major order (zmat_dns2coo), and the sum is K's value
+// for the next tier.  A drawer whose lead-orbital coupling vanishes empties
+// the ADO (:650-653, :733-737).
+int spa_pattern_build(const Model& m, double dsmall, uint64_t seed, std::vector<int32_t>& nnz,
+                      std::vector<int32_t>& irow, std::vector<int32_t>& icol) {
+  const int d = m.d, dd = d * d;
+  auto expH = [&](int kmax) {
+    std::vector<cplx> out((size_t)dd, cplx(0, 0)), pw((size_t)dd, cplx(0, 0)), nx((size_t)dd);
+    for (int i = 0; i < d; ++i) out[i + (size_t)i * d] = pw[i + (size_t)i * d] = cplx(1, 0);
+    double f = 1.0;
+    for (int k = 1; k <= kmax; ++k) {
+      f /= k;
+      for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+          cplx s(0, 0);
+          for (int q = 0; q < d; ++q) s += pw[i + (size_t)q * d] * m.H[q + (size_t)j * d];
+          nx[i + (size_t)j * d] = 0.31415926 * s;
+        }
+      pw.swap(nx);
+      for (size_t t = 0; t < (size_t)dd; ++t) out[t] += f * pw[t];
+    }
+    return out;
+  };
+  const std::vector<cplx> E = expH(10), E11 = expH(11);
+  for (size_t t = 0; t < (size_t)dd; ++t)
+    if ((std::abs(E[t]) > dsmall) != (std::abs(E11[t]) > dsmall))
+      return fail(HEOMQ_UNSUPPORTED, "sparse pattern: the sparsity of exp(a H) differs between 10 and 11 terms");
+  // columns of E as (row, value) lists
+  std::vector<std::vector<std::pair<int, cplx>>> Ecol(d);
+  for (int k = 0; k < d; ++k)
+    for (int i = 0; i < d; ++i)
+      if (E[i + (size_t)k * d] != cplx(0, 0)) Ecol[k].push_back({i, E[i + (size_t)k * d]});
+  nnz.assign(m.N, 0);
+  irow.clear();
+  icol.clear();
+  std::vector<int64_t> start(m.N + 1, 0);
+  std::vector<cplx> val;   // every stored value, ADO after ADO
+  auto emit = [&](int64_t g, const std::vector<cplx>& Z, bool empty) {
+    start[g] = (int64_t)irow.size();
+    if (!empty)
+      for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+          const cplx z = Z[i + (size_t)j * d];
+          if (std::abs(z) > dsmall) {
+            irow.push_back(i + 1);
+            icol.push_back(j + 1);
+            val.push_back(z);
+          }
+        }
+    nnz[g] = (int32_t)((int64_t)irow.size() - start[g]);
+  };
+  emit(0, E, false);
+  // op_k (r, c) as row lists (for X op) and column lists (for op X)
+  auto opv = [&](int k, int r, int c) { return m.sgn[k] == 0 ? m.amsv(m.orb[k], m.spin[k], c, r)
+                                                             : m.amsv(m.orb[k], m.spin[k], r, c); };
+  std::vector<cplx> Z((size_t)dd), YL((size_t)dd), YR((size_t)dd);
+  for (int n = 1; n <= m.L; ++n)
+    for (int64_t g = m.first[n]; g < m.first[n + 1]; ++g) {
+      std::fill(Z.begin(), Z.end(), cplx(0, 0));
+      bool empty = false;
+      for (int64_t p = m.lower_off[g]; p < m.lower_off[g + 1]; ++p) {
+        const LowerEntry& le = m.lower[p];
+        const int k = le.drawer;
+        if (m.cb[k] == cplx(0, 0) && m.cd[k] == cplx(0, 0)) { empty = true; continue; }
+        std::fill(YL.begin(), YL.end(), cplx(0, 0));
+        std::fill(YR.begin(), YR.end(), cplx(0, 0));
+        const int64_t J = le.nbr;
+        for (int64_t e = start[J]; e < start[J] + nnz[J]; ++e) {
+          int a = irow[e] - 1, b = icol[e] - 1;
+          cplx x = val[e];
+          if (le.itype) { std::swap(a, b); x = std::conj(x); }   // X^H
+          for (int r = 0; r < d; ++r) {
+            const double o = opv(k, r, a);                        // (op X)(r, b) += op(r, a) X(a, b)
+            if (o != 0.0) YL[r + (size_t)b * d] += o * x;
+          }
+          for (int c = 0; c < d; ++c) {
+            const double o = opv(k, b, c);                        // (X op)(a, c) += X(a, b) op(b, c)
+            if (o != 0.0) YR[a + (size_t)c * d] += x * o;
+          }
+        }
+        const uint64_t key = seed * 0x100000001B3ull + (uint64_t)g * 64 + (uint64_t)(p - m.lower_off[g]) * 4;
+        cplx m1, m2;
+        for (uint64_t t = 0;; t += 0x10000) {
+          m1 = cplx(normal_of(key + t), normal_of(key + t + 1));
+          m2 = cplx(normal_of(key + t + 2), normal_of(key + t + 3));
+          if (std::abs(m1) > 1e-6 && std::abs(m2) > 1e-6) break;
+        }
+        for (int j = 0; j < d; ++j)
+          for (int q = 0; q < d; ++q) {
+            const cplx yl = YL[q + (size_t)j * d], yr = YR[q + (size_t)j * d];
+            if (yl == cplx(0, 0) && yr == cplx(0, 0)) continue;
+            const cplx y = m1 * yl + m2 * yr;
+            for (const auto& ev : Ecol[q]) Z[ev.first + (size_t)j * d] += ev.second * y;
+          }
+      }
+      emit(g, Z, empty);
+    }
+  return HEOMQ_OK;
+}
+
+}  // namespace heomq

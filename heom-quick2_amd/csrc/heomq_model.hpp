@@ -75,4 +75,10 @@ int model_from_reference(
     const int32_t* ilead, const int32_t* mspin, const int32_t* morbs, const int32_t* mpm,
     const double* eleadinfty, const double* hs, const double* amsall, const double* dbsqrt, Model& m);
 
+// The sparse-mode pattern of fermion/prelude_spa.f90:532-749 (module
+// sparsemod layout: nnz per ADO, 1-based row / column per element, ADO after
+// ADO); seed fixes the mixing coefficients.
+int spa_pattern_build(const Model& m, double dsmall, uint64_t seed, std::vector<int32_t>& nnz,
+                      std::vector<int32_t>& irow, std::vector<int32_t>& icol);
+
 }  // namespace heomq

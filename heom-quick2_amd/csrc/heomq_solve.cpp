@@ -29,25 +29,35 @@ hipError_t apply_T(heomq_ctx* c, bool trans, const T* x, T* y, int64_t tpos, hip
 template <typename T>
 hipError_t apply_packed_T(heomq_ctx* c, bool trans, const T* x, T* y, bool rdo, hipStream_t s, int prec = 0,
                           T* scratch = nullptr);
+// The forward packed operator of the context: the plain apply, or with the
+// derivative terminator set the apply of the hierarchy one tier higher with
+// that tier eliminated.
+hipError_t packed_forward(heomq_ctx* c, const DevPacked& op, const double* x, double* y, bool rdo, hipStream_t s) {
+  if (c->trun.built) return trun_apply(c, x, y, rdo, op.precond, op.herm0, s);
+  return launch_apply_packed_f64(op, x, y, rdo, s);
+}
+
 template <>
 hipError_t apply_packed_T<double>(heomq_ctx* c, bool trans, const double* x, double* y, bool rdo, hipStream_t s,
                                   int prec, double* scratch) {
+  if (trans && c->trun.built) return hipErrorNotSupported;
   DevPacked op = trans ? c->ptrn : c->pfwd;
   op.precond = prec == 1 ? 1 : 0;
-  if (prec != 2) return launch_apply_packed_f64(op, x, y, rdo, s);
+  if (prec != 2) return trans ? launch_apply_packed_f64(op, x, y, rdo, s) : packed_forward(c, op, x, y, rdo, s);
   DevJacobi jc = c->pjac;
   jc.ejac = c->prec_shift;
   if (jacobi_one_kernel(jc)) {   // A x into the scratch, then y = M^-1 (A x) in one kernel
-    if (hipError_t e = launch_apply_packed_f64(op, x, scratch, rdo, s)) return e;
+    if (hipError_t e = packed_forward(c, op, x, scratch, rdo, s)) return e;
     return launch_jacobi_1k(op, jc, scratch, y, false, s);
   }
-  if (hipError_t e = launch_apply_packed_f64(op, x, y, rdo, s)) return e;
+  if (hipError_t e = packed_forward(c, op, x, y, rdo, s)) return e;
   if (hipError_t e = launch_jacobi_fwd(op, jc, y, scratch, s)) return e;
   return launch_jacobi_back(op, jc, scratch, y, false, s);
 }
 template <>
 hipError_t apply_packed_T<float>(heomq_ctx* c, bool trans, const float* x, float* y, bool rdo, hipStream_t s,
                                  int prec, float* scratch) {
+  if (c->trun.built) return hipErrorNotSupported;
   DevPacked op = trans ? c->ptrn : c->pfwd;
   op.precond = prec == 1 ? 1 : 0;
   if (prec != 2) return launch_apply_packed_f32(op, x, y, rdo, s);
@@ -350,12 +360,49 @@ int e_trace_into(heomq_ctx* c, double* b, int64_t n, int trace, hipStream_t st) 
 
 }  // namespace
 
+namespace heomq {
+
+// (A + T) x: x goes into the extended iterate with the eliminated tier zeroed;
+// the extended apply's top-tier blocks give that tier's right-hand sides B_V
+// (lower couplings only, its own elements being zero); the ADO-aligned
+// inverse at shift 0 sets x_V = (-S_V)^-1 B_V, the steady solution of V's
+// equation without upper couplings; the lower-tier blocks then form rows
+// 0..L, the top ones with their upper couplings to x_V.
+hipError_t trun_apply(heomq_ctx* c, const double* x, double* y, bool rdo, int precond, int herm0, hipStream_t st) {
+  TrunOperator& tr = c->trun;
+  double* X = tr.xext.as<double>();
+  double* T = tr.text.as<double>();
+  const int64_t nv = 2 * tr.vis, nt = 2 * (tr.total - tr.vis);
+  if (hipError_t e = hipMemcpyAsync(X, x, sizeof(double) * nv, hipMemcpyDeviceToDevice, st)) return e;
+  DevPacked op = tr.op;
+  op.rec = c->pfwd.rec;
+  if (nt > 0) {
+    if (hipError_t e = vec_fill<double>(st, nt, X + nv, 0.0)) return e;
+    DevPacked lo = op;   // the eliminated tier's blocks only
+    lo.n_heavy = 0;
+    if (hipError_t e = launch_apply_packed_f64(lo, X, T, false, st)) return e;
+    if (hipError_t e = launch_jacobi_ado(op, tr.jac, T, X, false, st)) return e;
+  }
+  DevPacked hi = op;     // tiers 0..L only
+  hi.total = op.light_begin;
+  hi.lt_blk = nullptr;
+  hi.n_lt_blocks = 0;
+  hi.precond = precond;
+  hi.herm0 = herm0;
+  return launch_apply_packed_f64(hi, X, y, rdo, st);
+}
+
+}  // namespace heomq
+
 extern "C" {
 
 int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int maxiter, int trace,
                        int transpose, double* stats, void* stream) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
   if (trace != HEOMQ_TRACE_LAST && trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "trace row required");
+  if (c->trun.built && (trace != HEOMQ_TRACE_RDO || transpose))
+    return fail(HEOMQ_UNSUPPORTED, "with the derivative terminator the steady solve is the forward packed one "
+                                   "(trace row RDO, no transpose)");
   hipStream_t st = (hipStream_t)stream;
   if (st == nullptr) {   // the iteration is replayed as a graph, which the null stream cannot capture
     HIPCHECK(hipStreamSynchronize(nullptr));
@@ -425,6 +472,7 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
 int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double tol, double inner_tol,
                              int maxiter, int trace, double* stats, void* stream) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_solve_steady_mixed: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "mixed precision solves use the tier-0 trace row");
   hipStream_t st = (hipStream_t)stream;
   if (st == nullptr) {
@@ -532,12 +580,14 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
 // ---- BiCG convention (bicg/) ----
 int heomq_apply_bicg(heomq_ctx* c, double* x, double* y, void* stream) {
   if (!c || !x || !y) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_apply_bicg: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   HIPCHECK(launch_apply_bicg_f64(c->fwd.op, x, y, (hipStream_t)stream));
   return HEOMQ_OK;
 }
 
 int heomq_apply_transpose_bicg(heomq_ctx* c, double* w, double* z, void* stream) {
   if (!c || !w || !z) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_apply_transpose_bicg: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   HIPCHECK(launch_apply_transpose_bicg_f64(c->trn.op, w, z, (hipStream_t)stream));
   return HEOMQ_OK;
 }
@@ -601,6 +651,7 @@ extern "C" {
 // element (1,1) of tier 0 and skip the tier-0 Hermitianisation.
 int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* stats, void* stream) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_solve_bicg: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   hipStream_t st = (hipStream_t)stream;
   const auto t0 = std::chrono::steady_clock::now();
   bool packed = false;
@@ -667,6 +718,7 @@ int heomq_solve_bicg(heomq_ctx* c, double* x, double crit, int maxiter, double* 
 
 int heomq_solve_bicg_host(heomq_ctx* c, double* rho_host, double crit, int maxiter, double* stats) {
   if (!c || !rho_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_solve_bicg_host: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   const int64_t n = 2 * c->N * c->d * c->d;
   HIPCHECK(c->ws_host.alloc(sizeof(double) * (size_t)n));
   double* x = c->ws_host.as<double>();
@@ -722,8 +774,12 @@ int heomq_apply_packed(heomq_ctx* c, const double* x, double* y, int trace, int 
   std::vector<double*> v;
   if (int rc = carve<double>(c->ws_io, n, 2, v)) return rc;
   const DevPacked& op = transpose ? c->ptrn : c->pfwd;
+  if (transpose && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "no transposed apply with the derivative terminator");
   HIPCHECK((launch_pack<double, double>(op, x, v[0], st)));
-  HIPCHECK(launch_apply_packed_f64(op, v[0], v[1], trace == HEOMQ_TRACE_RDO, st));
+  if (transpose)
+    HIPCHECK(launch_apply_packed_f64(op, v[0], v[1], trace == HEOMQ_TRACE_RDO, st));
+  else
+    HIPCHECK(packed_forward(c, op, v[0], v[1], trace == HEOMQ_TRACE_RDO, st));
   HIPCHECK((launch_unpack<double, double>(op, v[1], y, st)));
   HIPCHECK(hipStreamSynchronize(st));
   return HEOMQ_OK;
@@ -732,8 +788,11 @@ int heomq_apply_packed(heomq_ctx* c, const double* x, double* y, int trace, int 
 int heomq_apply_packed_raw(heomq_ctx* c, const double* xp, double* yp, int trace, int transpose, void* stream) {
   if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
   if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
-  HIPCHECK(launch_apply_packed_f64(transpose ? c->ptrn : c->pfwd, xp, yp, trace == HEOMQ_TRACE_RDO,
-                                   (hipStream_t)stream));
+  if (transpose && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "no transposed apply with the derivative terminator");
+  if (transpose)
+    HIPCHECK(launch_apply_packed_f64(c->ptrn, xp, yp, trace == HEOMQ_TRACE_RDO, (hipStream_t)stream));
+  else
+    HIPCHECK(packed_forward(c, c->pfwd, xp, yp, trace == HEOMQ_TRACE_RDO, (hipStream_t)stream));
   return HEOMQ_OK;
 }
 
@@ -750,7 +809,7 @@ int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int t
   jc.ejac = c->prec_shift;
   std::vector<double*> v;
   if (int rc = carve<double>(c->ws64b, 2 * c->ptotal, 1, v)) return rc;
-  HIPCHECK(launch_apply_packed_f64(op, xp, v[0], rdo, st));
+  HIPCHECK(packed_forward(c, op, xp, v[0], rdo, st));
   if (form == 1) {
     if (!jacobi_one_kernel(jc)) return fail(HEOMQ_UNSUPPORTED, "the one-kernel preconditioner does not apply here");
     HIPCHECK(launch_jacobi_1k(op, jc, v[0], yp, false, st));
@@ -789,13 +848,14 @@ int heomq_apply_packed_precond_noalloc(heomq_ctx* c, const double* xp, double* y
   jc.ejac = c->prec_shift;
   if (!jacobi_one_kernel(jc)) return fail(HEOMQ_UNSUPPORTED, "the one-kernel preconditioner does not apply here");
   hipStream_t st = (hipStream_t)stream;
-  HIPCHECK(launch_apply_packed_f64(op, xp, c->scratch_dev, trace == HEOMQ_TRACE_RDO, st));
+  HIPCHECK(packed_forward(c, op, xp, c->scratch_dev, trace == HEOMQ_TRACE_RDO, st));
   HIPCHECK(launch_jacobi_1k(op, jc, c->scratch_dev, yp, false, st));
   return HEOMQ_OK;
 }
 
 int heomq_apply_packed_raw_f32(heomq_ctx* c, const float* xp, float* yp, int trace, int transpose, void* stream) {
   if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_apply_packed_raw_f32: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
   HIPCHECK(launch_apply_packed_f32(transpose ? c->ptrn : c->pfwd, xp, yp, trace == HEOMQ_TRACE_RDO,
                                    (hipStream_t)stream));
@@ -969,6 +1029,7 @@ extern "C" {
 // to rounding, d^2 / (packed share) fewer bytes per apply).
 int heomq_rk4_at(heomq_ctx* c, double* x, double t0, double dt, int nstep, void* stream) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_rk4_at: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (nstep < 0) return fail(HEOMQ_BAD_ARG, "negative step count");
   hipStream_t st = (hipStream_t)stream;
   bool packed = false;
@@ -993,11 +1054,13 @@ int heomq_rk4_at(heomq_ctx* c, double* x, double t0, double dt, int nstep, void*
 }
 
 int heomq_rk4(heomq_ctx* c, double* x, double dt, int nstep, void* stream) {
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_rk4: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   return heomq_rk4_at(c, x, 0.0, dt, nstep, stream);
 }
 
 int heomq_rk4_packed(heomq_ctx* c, double* xp, double t0, double dt, int nstep, void* stream) {
   if (!c || !xp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_rk4_packed: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (nstep < 0) return fail(HEOMQ_BAD_ARG, "negative step count");
   std::vector<double*> v;
   if (int rc = carve<double>(c->ws64, 2 * c->ptotal, 3, v)) return rc;
@@ -1009,6 +1072,7 @@ int heomq_rk4_packed(heomq_ctx* c, double* xp, double t0, double dt, int nstep, 
 // outside the packed sectors runs on the packed layout.
 int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats, void* stream) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_chebyshev: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (nstep < 0) return fail(HEOMQ_BAD_ARG, "negative step count");
   hipStream_t st = (hipStream_t)stream;
   std::vector<double> J;
@@ -1041,6 +1105,7 @@ int heomq_chebyshev(heomq_ctx* c, double* x, double dt, int nstep, double* stats
 
 int heomq_chebyshev_packed(heomq_ctx* c, double* xp, double dt, int nstep, double* stats, void* stream) {
   if (!c || !xp) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_chebyshev_packed: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (nstep < 0) return fail(HEOMQ_BAD_ARG, "negative step count");
   hipStream_t st = (hipStream_t)stream;
   std::vector<double> J;
@@ -1124,12 +1189,14 @@ static int host_apply(heomq_ctx* c, const double* xh, double* yh, int trace, boo
 
 int heomq_calc_ax_tfqmr(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_host) {
   if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calc_ax_tfqmr: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_ax_tfqmr_omp");
   return host_apply(c, x_host, y_host, HEOMQ_TRACE_LAST, false);
 }
 
 int heomq_calc_atrx(heomq_ctx* c, int32_t dim0, const double* x_host, double* y_host) {
   if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calc_atrx: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_atrx_bicg_omp");
   return host_apply(c, x_host, y_host, HEOMQ_TRACE_LAST, true);
 }
@@ -1150,18 +1217,21 @@ static int host_bicg(heomq_ctx* c, double* xh, double* yh, bool trans) {
 
 int heomq_calc_ax_bicg(heomq_ctx* c, int32_t dim0, double* x_host, double* y_host) {
   if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calc_ax_bicg: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_ax_bicg_omp");
   return host_bicg(c, x_host, y_host, false);
 }
 
 int heomq_calc_atrx_bicg(heomq_ctx* c, int32_t dim0, double* x_host, double* y_host) {
   if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calc_atrx_bicg: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (dim0 != c->d) return fail(HEOMQ_BAD_DIM, "error! dim0 != nrho in calc_atrx_bicg_omp");
   return host_bicg(c, x_host, y_host, true);
 }
 
 int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host) {
   if (!c || !rhotmp_host || !rhorhs_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calcderiv: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   return host_apply(c, rhotmp_host, rhorhs_host, HEOMQ_TRACE_NONE, false);
 }
 

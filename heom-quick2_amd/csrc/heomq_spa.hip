@@ -9,12 +9,17 @@
 // the selection of the pattern; ADOs with nnz = 0 hold nothing (:225).  The
 // trace of rho0 is appended as element lunk_spa + 1 (:1061-1069).
 //
-// Here the pattern becomes a map from COO element to dense index, and the
-// apply is scatter -> the dense device apply -> gather, plus the trace: the
-// arithmetic is the dense operator's, so P A P^T holds for any pattern.  The
+// Here the pattern's distinct per-ADO element lists become the "sectors" of
+// the packed apply (heomq_capi.cpp, build_spa_operator): rho_spa, stored ADO
+// after ADO as prelude_spa lays it out, is that layout's packed vector as it
+// stands, the gather maps leave out neighbour elements outside a neighbour's
+// list (P^T) and only the list's own elements are formed (P).  Device memory
+// scales with lunk_spa and the number of distinct lists, not with N d^2.  The
 // derivative of the sparse propagators (td/calcderiv_spa_omp.f90: rhotmp_spa
 // -> rhorhs_spa, H and the lead shift at time tt, :84-98) is the same
-// composition around the derivative at tt, with no trace element.
+// operator with the rates at tt; only a time-dependent system Hamiltonian
+// (whose rows are not in the list tables) goes through the dense derivative
+// between a scatter and a gather.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -97,6 +102,17 @@ int heomq_spa_pattern(heomq_ctx* c, int64_t lunk_spa, const int32_t* nnz_spa, co
   }
   for (int64_t e = 0; e < lunk_spa; ++e)
     if (map[e] < 0) return fail(HEOMQ_BAD_ARG, "sparse pattern: an element of rho_spa belongs to no ADO");
+  // ADO after ADO, as prelude_spa.f90:741-742 stores them (ind_spa(a) =
+  // ind_spa(a-1) + nnz_spa(a-1)); an empty ADO's ind_spa is not read
+  std::vector<int64_t> start(c->N, 0);
+  int64_t run = 0;
+  for (int64_t a = 0; a < c->N; ++a) {
+    if (nnz_spa[a] > 0 && ind_spa[a] - 1 != run)
+      return fail(HEOMQ_BAD_ARG, "sparse pattern: ADOs not stored in order (ind_spa(a) != ind_spa(a-1) + nnz_spa(a-1))");
+    start[a] = run;
+    run += nnz_spa[a];
+  }
+  if (int rc = build_spa_operator(c, nnz_spa, start, irow_spa, icol_spa)) return rc;
   const int ndiag = (int)diag.size();
   if (map.empty()) map.push_back(0);     // keep the device buffers non-empty
   if (diag.empty()) diag.push_back(0);
@@ -109,25 +125,11 @@ int heomq_spa_pattern(heomq_ctx* c, int64_t lunk_spa, const int32_t* nnz_spa, co
 
 int heomq_apply_spa(heomq_ctx* c, const double* x_spa, double* y_spa, void* stream) {
   if (!c || !x_spa || !y_spa) return fail(HEOMQ_BAD_ARG, "null argument");
-  if (c->spa_lunk < 0) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_apply_spa: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
+  if (c->spa_lunk < 0 || !c->spa.built) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
   hipStream_t st = (hipStream_t)stream;
-  const int64_t n = c->spa_lunk, nd = 2 * c->N * c->d * c->d;
-  HIPCHECK(c->spa_dense.alloc(2 * sizeof(double) * (size_t)nd));
-  double* xd = c->spa_dense.as<double>();
-  double* yd = xd + nd;
-  hipLaunchKernelGGL(k_spa_zero, dim3(grid_of(nd)), dim3(256), 0, st, nd, xd);
-  HIPCHECK(hipGetLastError());
-  if (n > 0) {
-    hipLaunchKernelGGL(k_spa_scatter, dim3(grid_of(n)), dim3(256), 0, st, n, c->spa_map.as<int64_t>(),
-                       reinterpret_cast<const double2*>(x_spa), reinterpret_cast<double2*>(xd));
-    HIPCHECK(hipGetLastError());
-  }
-  HIPCHECK(launch_apply_f64(c->fwd.op, xd, yd, -1, st));
-  if (n > 0) {
-    hipLaunchKernelGGL(k_spa_gather, dim3(grid_of(n)), dim3(256), 0, st, n, c->spa_map.as<int64_t>(),
-                       reinterpret_cast<const double2*>(yd), reinterpret_cast<double2*>(y_spa));
-    HIPCHECK(hipGetLastError());
-  }
+  const int64_t n = c->spa_lunk;
+  if (n > 0) HIPCHECK(launch_apply_packed_f64(spa_operator(c, nullptr), x_spa, y_spa, false, st));
   hipLaunchKernelGGL(k_spa_trace, dim3(1), dim3(256), 0, st, c->spa_ndiag, c->spa_diag.as<int64_t>(),
                      reinterpret_cast<const double2*>(x_spa), reinterpret_cast<double2*>(y_spa) + n);
   HIPCHECK(hipGetLastError());
@@ -136,30 +138,36 @@ int heomq_apply_spa(heomq_ctx* c, const double* x_spa, double* y_spa, void* stre
 
 int heomq_calcderiv_spa_at(heomq_ctx* c, double tt, const double* x_spa, double* y_spa, void* stream) {
   if (!c || !x_spa || !y_spa) return fail(HEOMQ_BAD_ARG, "null argument");
-  if (c->spa_lunk < 0) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calcderiv_spa_at: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
+  if (c->spa_lunk < 0 || !c->spa.built) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
   hipStream_t st = (hipStream_t)stream;
-  const int64_t n = c->spa_lunk, nd = 2 * c->N * c->d * c->d;
+  const int64_t n = c->spa_lunk;
+  if (n == 0) return HEOMQ_OK;
+  if (c->sys_nterm == 0) {   // rates at tt (lead shift), H fixed: the list tables hold the whole operator
+    DevOperator dop;
+    if (int rc = operator_at(c, tt, st, dop)) return rc;
+    HIPCHECK(launch_apply_packed_f64(spa_operator(c, dop.gam), x_spa, y_spa, false, st));
+    return HEOMQ_OK;
+  }
+  const int64_t nd = 2 * c->N * c->d * c->d;
   HIPCHECK(c->spa_dense.alloc(2 * sizeof(double) * (size_t)nd));
   double* xd = c->spa_dense.as<double>();
   double* yd = xd + nd;
   hipLaunchKernelGGL(k_spa_zero, dim3(grid_of(nd)), dim3(256), 0, st, nd, xd);
   HIPCHECK(hipGetLastError());
-  if (n > 0) {
-    hipLaunchKernelGGL(k_spa_scatter, dim3(grid_of(n)), dim3(256), 0, st, n, c->spa_map.as<int64_t>(),
-                       reinterpret_cast<const double2*>(x_spa), reinterpret_cast<double2*>(xd));
-    HIPCHECK(hipGetLastError());
-  }
+  hipLaunchKernelGGL(k_spa_scatter, dim3(grid_of(n)), dim3(256), 0, st, n, c->spa_map.as<int64_t>(),
+                     reinterpret_cast<const double2*>(x_spa), reinterpret_cast<double2*>(xd));
+  HIPCHECK(hipGetLastError());
   if (int rc = heomq_calcderiv_at(c, tt, xd, yd, stream)) return rc;
-  if (n > 0) {
-    hipLaunchKernelGGL(k_spa_gather, dim3(grid_of(n)), dim3(256), 0, st, n, c->spa_map.as<int64_t>(),
-                       reinterpret_cast<const double2*>(yd), reinterpret_cast<double2*>(y_spa));
-    HIPCHECK(hipGetLastError());
-  }
+  hipLaunchKernelGGL(k_spa_gather, dim3(grid_of(n)), dim3(256), 0, st, n, c->spa_map.as<int64_t>(),
+                     reinterpret_cast<const double2*>(yd), reinterpret_cast<double2*>(y_spa));
+  HIPCHECK(hipGetLastError());
   return HEOMQ_OK;
 }
 
 int heomq_calcderiv_spa_tt(heomq_ctx* c, double tt, const double* x_host, double* y_host) {
   if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calcderiv_spa_tt: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (c->spa_lunk < 0) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
   const int64_t n = c->spa_lunk;
   if (n == 0) return HEOMQ_OK;
@@ -174,6 +182,7 @@ int heomq_calcderiv_spa_tt(heomq_ctx* c, double tt, const double* x_host, double
 
 int heomq_calc_ax_tfqmr_spa(heomq_ctx* c, int64_t ldim, const double* x_host, double* y_host) {
   if (!c || !x_host || !y_host) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c && c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_calc_ax_tfqmr_spa: not with the derivative terminator (it runs in the packed forward apply and the steady TFQMR solve)");
   if (c->spa_lunk < 0) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
   if (ldim != c->spa_lunk) return fail(HEOMQ_BAD_DIM, "calc_ax_tfqmr_spa_omp: error dimension, ldim != lunk_spa");
   const int64_t n = c->spa_lunk;

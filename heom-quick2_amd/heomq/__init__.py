@@ -277,6 +277,24 @@ class Model:
                 "light_blocks", "max_ados_per_light_block", "empty_ados", "max_light_couplings", "map_entries")
         return dict(zip(keys, (int(v) for v in info)))
 
+    def spa_pattern(self, dsmall=1e-32, seed=0):
+        """The sparse-mode pattern fermion/prelude_spa.f90:532-749 builds, in the
+        reference's module sparsemod arrays (1-based): (nnz_spa, ind_spa,
+        irow_spa, icol_spa).  dsmall is the reference's dsmall_spa (1e-32; 1e-12
+        with lproj); seed fixes the mixing coefficients."""
+        n = np.int64(0)
+        lunk = np.zeros(1, np.int64)
+        nnz = np.zeros(self.N, np.int32)
+        ind = np.zeros(self.N, np.int64)
+        check(lib().heomq_spa_build_pattern(self._h, dsmall, seed, 0, lunk.ctypes.data_as(_lib.I64),
+                                            nnz.ctypes.data_as(_lib.I32), ind.ctypes.data_as(_lib.I64), None, None))
+        n = int(lunk[0])
+        ir = np.zeros(max(n, 1), np.int32)
+        ic = np.zeros(max(n, 1), np.int32)
+        check(lib().heomq_spa_build_pattern(self._h, dsmall, seed, n, lunk.ctypes.data_as(_lib.I64), None, None,
+                                            ir.ctypes.data_as(_lib.I32), ic.ctypes.data_as(_lib.I32)))
+        return nnz, ind, ir[:n], ic[:n]
+
     def jacobi_info(self):
         """(info dict, eigenvalues of H, eigenvectors of H) of the block-Jacobi
         iteration, built on the host (prelude_jacobi.f90:43-113)."""
@@ -350,6 +368,13 @@ class Context:
         """Two block-Jacobi sweeps on the start of block-Jacobi-preconditioned
         steady solves (default on)."""
         check(lib().heomq_ctx_set_presmooth(self._h, int(enable)))
+
+    def set_trun_der(self, enable: bool = True):
+        """The derivative terminator of the top tier ($jobinfo ltrun_der,
+        calc_ax_tfqmr_omp.f90:362-478): the forward packed applies and the
+        steady TFQMR solve then include it; the other solvers and propagators
+        refuse to run while it is set."""
+        check(lib().heomq_ctx_set_trun_der(self._h, int(enable)))
 
     def autotune(self, reps: int = 20) -> dict:
         """Time the packed apply's four forms on this GPU and keep the fastest

@@ -89,6 +89,7 @@ SIGNATURES = {
     "heomq_solve_bicg": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_solve_bicg_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),
     "heomq_spa_pattern": (C.c_int, [P, C.c_int64, I32, I64, I32, I32]),
+    "heomq_spa_build_pattern": (C.c_int, [P, C.c_double, C.c_uint64, C.c_int64, I64, I32, I64, I32, I32]),
     "heomq_apply_spa": (C.c_int, [P, P, P, P]),
     "heomq_calc_ax_tfqmr_spa": (C.c_int, [P, C.c_int64, D, D]),
     "heomq_calcderiv_spa_at": (C.c_int, [P, C.c_double, P, P, P]),
@@ -115,6 +116,7 @@ SIGNATURES = {
     "heomq_ctx_set_system_field": (C.c_int, [P, C.c_int, D, I32, D]),
     "heomq_ctx_field_shifts": (C.c_int, [P, C.c_double, D]),
     "heomq_ctx_set_presmooth": (C.c_int, [P, C.c_int]),
+    "heomq_ctx_set_trun_der": (C.c_int, [P, C.c_int]),
     "heomq_chebyshev": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_rk4_packed": (C.c_int, [P, P, C.c_double, C.c_double, C.c_int, P]),
     "heomq_chebyshev_packed": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),

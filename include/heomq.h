@@ -168,6 +168,20 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* m);
  * eight TFQMR iterations; solves requested on the null stream run on a
  * context-owned stream).  enable = 0 launches the same kernels one by one. */
 int heomq_ctx_set_graphs(heomq_ctx* c, int enable);
+/* Derivative terminator of the top tier ($jobinfo ltrun_der;
+ * tfqmr_double/calc_ax_tfqmr_omp.f90:362-478 with fermion/prelude_trun.f90):
+ * every top-tier ADO K receives the upper couplings of the tier-(L+1) ADOs
+ * K + j, each set to the steady solution of its own equation without upper
+ * couplings, (i [H, .] - gamma_{K+j})^-1 (its lower-coupling terms).  The
+ * context then holds the packed tables of the hierarchy one tier higher
+ * (same parameters; a model from heomq_model_from_reference is refused) and
+ * its forward packed applies (heomq_apply_packed*, the preconditioned and
+ * allocation-free ones) and the steady TFQMR solve (trace row RDO) include
+ * the terminator; every other apply, solver and propagator of the context
+ * returns HEOMQ_UNSUPPORTED while it is set.  enable 0 removes it.  The
+ * steady state with the terminator equals, on tiers 0..L, the steady state of
+ * the hierarchy one tier higher (a Schur complement). */
+int heomq_ctx_set_trun_der(heomq_ctx* c, int enable);
 /* Block-Jacobi-preconditioned steady solves with b = e_trace start with two
  * sweeps of the reference's block-Jacobi iteration (jacobi/solve_jacobi.f90,
  * its default shift; the trace is kept) applied to the initial guess
@@ -351,10 +365,28 @@ int heomq_rk4_at(heomq_ctx* c, double* x, double t0, double dt, int nstep, void*
  * fermion/prelude_spa.f90:532-749).  y = P A P^T x with P the pattern's
  * selection, and y(lunk_spa + 1) = tr rho0 (:1061-1069).
  */
+/* Build the pattern as fermion/prelude_spa.f90:532-749 does (replaces its
+ * tier loop; the reference writes the result into module sparsemod): tier 0
+ * holds the nonzeros of exp(a H) (a = 0.31415926, 10 Taylor terms,
+ * utility/calc_exp_hmat.f90); an ADO of tier n >= 1 holds the elements above
+ * dsmall of sum_J m1 E (op X_J) + m2 E (X_J op) over its lower neighbours J
+ * (random complex m1, m2 drawn from `seed`: they only keep the two products
+ * from cancelling).  dsmall: the reference's dsmall_spa (1e-32, or 1e-12 with
+ * lproj).  Two calls: with irow_spa / icol_spa NULL, or cap < the pattern's
+ * length, only *lunk_spa (and nnz_spa / ind_spa when given, N each) are set
+ * and HEOMQ_BAD_DIM is returned when arrays were given but too short; then
+ * with arrays of lunk_spa entries.  ind_spa(a) = ind_spa(a-1) + nnz_spa(a-1),
+ * 1-based, as :741-742. */
+int heomq_spa_build_pattern(const heomq_model* m, double dsmall, uint64_t seed, int64_t cap, int64_t* lunk_spa,
+                            int32_t* nnz_spa, int64_t* ind_spa, int32_t* irow_spa, int32_t* icol_spa);
 /* Set the pattern from the reference's arrays: nnz_spa and ind_spa (1-based
  * start in rho_spa) per ADO (N each), irow_spa / icol_spa (1-based, lunk_spa
- * each).  Rejected with HEOMQ_BAD_ARG: indices out of range, overlapping
- * ADOs, an element listed twice, elements of no ADO. */
+ * each); ADOs are stored in order (ind_spa(a) = ind_spa(a-1) + nnz_spa(a-1)
+ * for every non-empty ADO, as prelude_spa.f90:741-742 lays them out).
+ * Rejected with HEOMQ_BAD_ARG: indices out of range, overlapping or
+ * out-of-order ADOs, an element listed twice, elements of no ADO.  The apply
+ * then runs on rho_spa as it is: device tables scale with lunk_spa and the
+ * number of distinct per-ADO element lists (at most 65535). */
 int heomq_spa_pattern(heomq_ctx* c, int64_t lunk_spa, const int32_t* nnz_spa, const int64_t* ind_spa,
                       const int32_t* irow_spa, const int32_t* icol_spa);
 /* Device vectors: x_spa (lunk_spa complex), y_spa (lunk_spa + 1 complex). */

@@ -439,7 +439,7 @@ def _cop(model, k, left):
     return c.T if model.sgn[k] == 0 else c
 
 
-def apply(model: Model, x, trace_row="last", H=None, gdraw=None, cb=None, cd=None):
+def apply(model: Model, x, trace_row="last", H=None, gdraw=None, cb=None, cd=None, trun_der=False):
     """y = L x, the dense TFQMR-convention apply.
 
     Restates tfqmr_double/calc_ax_tfqmr_omp.f90: same tier :198-203, lower
@@ -449,7 +449,9 @@ def apply(model: Model, x, trace_row="last", H=None, gdraw=None, cb=None, cd=Non
     of the last ADO), "rdo" (element (d,d) of the tier-0 block, a row that is
     linearly dependent in L for every truncation tier), or None
     (calcderiv_omp: no normalisation row).  Optional H/gdraw/cb/cd override
-    the model's coefficients (used for parameter derivatives).
+    the model's coefficients (used for parameter derivatives).  ``trun_der``
+    adds the derivative terminator of the top tier (:362-478,
+    :func:`trun_der_term`).
     """
     H = model.H if H is None else H
     gdraw = model.gdraw if gdraw is None else gdraw
@@ -492,12 +494,93 @@ def apply(model: Model, x, trace_row="last", H=None, gdraw=None, cb=None, cd=Non
                 fr = upf[:, e, 1][:, None, None] * model.dbsqrt[e]
                 yt += -1j * ff * (op @ Xn) + 1j * fr * (Xn @ op)
         y[s0:s1] = yt
+    if trun_der:
+        y += trun_der_term(model, x, H=H, gdraw=gdraw, cb=cb, cd=cd)
     if trace_row == "last":
         y[-1, d - 1, d - 1] = np.trace(x[0])
     elif trace_row == "rdo":
         y[0, d - 1, d - 1] = np.trace(x[0])
     elif trace_row is not None:
         raise ValueError(trace_row)
+    return y
+
+
+def liouvillian_eigensystem(H):
+    """fermion/prelude_trun.f90:41-78: the eigenvalues and eigenvectors of the
+    commutator superoperator X -> H X - X H (zhil2liou 'l' minus 'r', column-
+    major vec(X)), as zheev gives them for the Hermitian d^2 x d^2 matrix."""
+    d = H.shape[0]
+    lsys = np.kron(np.eye(d), H) - np.kron(H.T, np.eye(d))
+    return np.linalg.eigh(lsys)
+
+
+def trun_der_term(model: Model, x, H=None, gdraw=None, cb=None, cd=None, eig=None):
+    """The derivative terminator (ltrun_der) of calc_ax_tfqmr_omp.f90:362-478:
+    every top-tier ADO K receives, from each drawer j, the upper coupling of
+    the tier-(L+1) ADO V = K + j that is not stored but set to the steady
+    solution of its own equation without its upper couplings,
+    rho_V = (i L_sys - gamma_V)^-1 B_V, B_V being V's lower-coupling terms
+    from the stored tier-L ADOs V - m (:397-423; findlowertier, add2list) and
+    the resolvent taken in the eigenbasis of L_sys = [H, .] (:426-447:
+    -B/gamma_V plus the nonzero-eigenvalue correction; -i/lambda only when
+    |gamma_V| <= dpico).  The coupling back to K is the upper-tier rule with
+    the signs of j's place in V (:376-385, :462-467; dbsqrt with lscale).
+    With lsdraw the reference loops over every drawer (a repeated one gives a
+    V whose lower terms cancel); without it, a drawer already in K is skipped
+    (no such V exists, and the reference's lookup of it would stop)."""
+    H = model.H if H is None else H
+    gdraw = model.gdraw if gdraw is None else gdraw
+    cb = model.cb if cb is None else cb
+    cd = model.cd if cd is None else cd
+    d, L, M = model.d, model.L, model.M
+    lam, U = liouvillian_eigensystem(H) if eig is None else eig
+    nz = np.abs(lam) > 1e-12
+    y = np.zeros_like(x)
+    if L < 1:
+        raise ValueError("the derivative terminator needs a top tier L >= 1")
+
+    def vec(A):
+        return A.reshape(-1, order="F")
+
+    for a, K in enumerate(model.tiers[L]):
+        g = int(model.first[L] + a)
+        K = [int(k) for k in K]
+        gam_k = gdraw[K].sum()
+        for e in range(M):
+            if not model.p.lsdraw and e in K:
+                continue
+            mj = next((i for i, k in enumerate(K) if k > e), len(K)) + 1    # add2list: after equal drawers
+            V = K[:mj - 1] + [e] + K[mj - 1:]
+            B = np.zeros((d, d), complex)
+            for mi in range(1, L + 2):
+                nn = V[mi - 1]
+                res = model.lookup(tuple(V[:mi - 1] + V[mi:]))
+                if res is None:
+                    raise RuntimeError("findlowertier: lower neighbour of the virtual ADO not stored")
+                lnj, it, fact = res
+                ff = (-1) ** (mi - 1)
+                fr = (-1) ** (L + 1 - mi)
+                Xn = x[lnj]
+                if it == 1:
+                    ff, fr = ff * fact, fr * fact
+                    Xn = Xn.conj().T
+                c = model.ams[model.orb[nn]][model.spin[nn]]
+                op = c.T if model.sgn[nn] == 0 else c
+                B += cb[nn] * (-1j * ff) * (op @ Xn) + cd[nn] * (1j * fr) * (Xn @ op)
+            gv = gam_k + gdraw[e]
+            w = U.conj().T @ vec(B)
+            if abs(gv) > 1e-12:
+                lam0 = np.where(nz, lam, 0.0)
+                w = w / (1j * lam0 - gv)
+            else:
+                w = np.where(nz, -1j / np.where(nz, lam, 1.0), 0.0) * w
+            rv = (U @ w).reshape(d, d, order="F")
+            c = model.ams[model.orb[e]][model.spin[e]]
+            op = c if model.sgn[e] == 0 else c.T
+            sc = model.dbsqrt[e]
+            f1 = (-1) ** (mj - 1) * sc
+            f2 = (-1) ** (L + 1 - mj) * sc
+            y[g] += -1j * f1 * (op @ rv) + 1j * f2 * (rv @ op)
     return y
 
 
@@ -694,29 +777,36 @@ def tfqmr(matvec, b, tol=1e-12, maxiter=20000, x0=None, max_restarts=3, project=
 
 
 
-def steady_state(model: Model, tol=1e-12, maxiter=20000, trace_row="rdo"):
+def steady_state(model: Model, tol=1e-12, maxiter=20000, trace_row="rdo", trun_der=False):
     """Steady state: solve A x = e_trace with A the trace-row apply.
 
     solve_tfqmr.f90:97-179 solves A*dx = e_trace - A*x0 from a physical
     trace-one guess x0 and sets rho = x0 + dx; the same is done here with
     x0 = identity/d on tier 0.  Starting from a trace-one guess keeps the
     Krylov space inside the Hermitian, parity-allowed, traceless sector.
+    ``trun_der`` solves with the derivative terminator of the top tier.
     """
     d = model.d
     nvec = model.N * d * d * 2
+    if trun_der:
+        eig = liouvillian_eigensystem(model.H)
+        base = apply
+        apply_ = lambda m, v, trace_row: base(m, v, trace_row=trace_row) + trun_der_term(m, v, eig=eig)
+    else:
+        apply_ = lambda m, v, trace_row: apply(m, v, trace_row=trace_row)
     x0 = np.zeros((model.N, d, d), complex)
     x0[0] = np.eye(d) / d
     e = np.zeros((model.N, d, d), complex)
     e[-1 if trace_row == "last" else 0, d - 1, d - 1] = 1.0
-    b = as_real(e - apply(model, x0, trace_row=trace_row))
+    b = as_real(e - apply_(model, x0, trace_row=trace_row))
     proj = lambda v: as_real(physical_projection(model, from_real(v, model)))
     if trace_row == "rdo":
         # iterate with Pi*A: A maps the physical sector into itself and the
         # steady state is its unique solution there
-        mv = lambda v: proj(as_real(apply(model, from_real(v, model), trace_row=trace_row)))
+        mv = lambda v: proj(as_real(apply_(model, from_real(v, model), trace_row=trace_row)))
         xr, it, rel = tfqmr(mv, b, tol=tol, maxiter=maxiter, project=proj)
     else:
-        mv = lambda v: as_real(apply(model, from_real(v, model), trace_row=trace_row))
+        mv = lambda v: as_real(apply_(model, from_real(v, model), trace_row=trace_row))
         xr, it, rel = tfqmr(mv, b, tol=tol, maxiter=maxiter, project=proj, restart_every=100)
     return x0 + from_real(xr, model), it, rel
 
@@ -1118,6 +1208,71 @@ def spa_pattern_from_mask(mask):
         cols.extend(jj + 1)
         pos += len(ii)
     return nnz, ind, np.array(rows, np.int32), np.array(cols, np.int32)
+
+
+def calc_exp_hmat(H, alpha, kmax):
+    """utility/calc_exp_hmat.f90: 1 + sum_{k=1}^{kmax} (alpha H)^k / k!."""
+    out = np.eye(H.shape[0], dtype=complex)
+    pw = np.eye(H.shape[0], dtype=complex)
+    f = 1.0
+    for k in range(1, kmax + 1):
+        f /= k
+        pw = alpha * (pw @ H)
+        out = out + f * pw
+    return out
+
+
+def spa_pattern_build(model: Model, dsmall=1e-32, seed=0):
+    """The sparse-mode pattern of fermion/prelude_spa.f90:532-749, in the
+    module sparsemod arrays (nnz_spa, ind_spa, irow_spa, icol_spa; 1-based,
+    ADO after ADO as :741-742).
+
+    Tier 0: the elements of E = calc_exp_hmat(H, 0.31415926, 10) above dsmall
+    (:231-238; 10 and 11 terms must agree, :233-237), E its values.  Tier n:
+    for every lower neighbour J of K (drawer k removed), Y_L = op_k X and
+    Y_R = X op_k with X = rho_J (rho_J^H for itype 1) and op_k = c^+ for
+    sigma = +, c for sigma = - (:640-686, zamsmm_coo); Z += m1 E Y_L + m2 E Y_R
+    with random complex normals m1, m2 (:676-683, mkl_zcoomm with the tier-0
+    values).  K holds the elements of Z above dsmall (zmat_dns2coo, column
+    major) and Z is its value for the next tier; a drawer whose lead-orbital
+    linewidth is below dsmall empties K (:650-653, :733-737).
+    """
+    d = model.d
+    E = calc_exp_hmat(model.H, 0.31415926, 10)
+    if not np.array_equal(np.abs(E) > dsmall, np.abs(calc_exp_hmat(model.H, 0.31415926, 11)) > dsmall):
+        raise RuntimeError("prelude_spa: sparsity analysis failed")
+    rng = np.random.default_rng(seed)
+    vals = [None] * model.N
+    masks = np.zeros((model.N, d, d), bool)
+    masks[0] = np.abs(E) > dsmall
+    vals[0] = np.where(masks[0], E, 0)
+    for n in range(1, model.L + 1):
+        lo, _ = model.lower[n]
+        for a, K in enumerate(model.tiers[n]):
+            g = model.first[n] + a
+            Z = np.zeros((d, d), complex)
+            empty = False
+            for r in range(n):
+                k = int(K[r])
+                if abs(model.p.linewidth[model.alf[k]][model.orb[k]]) < dsmall:
+                    empty = True
+                    continue
+                X = vals[lo[a, r, 0]]
+                if lo[a, r, 1] == 1:
+                    X = X.conj().T
+                c = model.ams[model.orb[k]][model.spin[k]]
+                op = c.T if model.sgn[k] == 0 else c
+                while True:
+                    m1, m2 = rng.standard_normal(2) + 1j * rng.standard_normal(2)
+                    if abs(m1) > 1e-6 and abs(m2) > 1e-6:
+                        break
+                Z += m1 * (E @ (op @ X)) + m2 * (E @ (X @ op))
+            if not empty:
+                masks[g] = np.abs(Z) > dsmall
+            vals[g] = np.where(masks[g], Z, 0)
+    nnz, ind, irow, icol = spa_pattern_from_mask(masks)
+    ind = np.concatenate([[1], 1 + np.cumsum(nnz)[:-1]]).astype(np.int64)
+    return nnz, ind, irow, icol
 
 
 def apply_spa(model: Model, pattern, x_spa):

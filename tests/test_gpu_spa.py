@@ -2,9 +2,12 @@
 
 Patterns in the reference's own module layout (nnz_spa, ind_spa, irow_spa,
 icol_spa; 1-based), built from masks: the full d x d blocks, the physical
-sector (what an exact run's pattern can hold), and a random sub-pattern of it.
-The device result is checked against the oracle's P A P^T x plus the
-appended trace.
+sector (what an exact run's pattern can hold), a random sub-pattern of it,
+and the pattern the native prelude_spa restatement builds (also for a model
+with a decoupled lead, whose left-lead ADOs are empty).  The device result is
+checked against the oracle's P A P^T x plus the appended trace; at the
+two-impurity workload's full size against the OpenMP C port of the dense
+reference apply restricted to the pattern.
 """
 import numpy as np
 import pytest
@@ -17,6 +20,8 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "siam_nk2_L3": (lambda: heomq.siam(nk=2, ntop=3, V=0.3, U=1.5), lambda: O.siam_params(nk=2, ntop=3, V=0.3, U=1.5)),
     "dqd_nk2_L2": (lambda: heomq.two_impurity(nk=2, ntop=2), lambda: O.dqd_params(nk=2, ntop=2)),
+    "siam_gl0": (lambda: heomq.siam(nk=2, ntop=3, V=0.3, U=1.5, gl=0.0),
+                 lambda: O.siam_params(nk=2, ntop=3, V=0.3, U=1.5, gl=0.0)),
 }
 
 _cache = {}
@@ -43,12 +48,12 @@ def make_mask(o, kind, seed=7):
     return mask
 
 
-@pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("kind", ["full", "physical", "sub"])
+@pytest.mark.parametrize("name", ["siam_nk2_L3", "dqd_nk2_L2"])
+@pytest.mark.parametrize("kind", ["full", "physical", "sub", "built"])
 def test_apply_spa_matches_oracle(name, kind):
     m, ctx, o, torch = setup(name)
     print(f"case {name} pattern {kind}", flush=True)
-    pat = O.spa_pattern_from_mask(make_mask(o, kind))
+    pat = m.spa_pattern() if kind == "built" else O.spa_pattern_from_mask(make_mask(o, kind))
     ctx.set_spa_pattern(*pat)
     rng = np.random.default_rng(11)
     n = len(pat[2])
@@ -76,3 +81,43 @@ def test_spa_pattern_rejects_bad_tables():
     short[last] = len(irow) + 1  # the last stored ADO's elements start past lunk_spa
     with pytest.raises(heomq.HeomqError, match="outside"):
         ctx.set_spa_pattern(nnz, short, irow, icol)
+
+
+def test_apply_spa_built_pattern_with_a_decoupled_lead():
+    m, ctx, o, torch = setup("siam_gl0")
+    pat = m.spa_pattern()
+    assert (pat[0] == 0).any()   # the left-lead ADOs hold nothing
+    ctx.set_spa_pattern(*pat)
+    rng = np.random.default_rng(12)
+    n = len(pat[2])
+    xs = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = ctx.apply_spa(torch.from_numpy(xs).to("cuda:0")).cpu().numpy()
+    yo = O.apply_spa(o, pat, xs)
+    assert np.abs(y - yo).max() <= 1e-12 * np.abs(yo).max()
+
+
+def test_apply_spa_two_impurity_workload_against_c_port():
+    # d = 16, M = 112, L = 3: the built pattern (1.8 M elements), the apply on
+    # rho_spa against the C port's dense apply (no trace row) at the pattern
+    from oracle.heom_port import Port
+    from ref_arrays import reference_arrays
+    o = O.Model(O.dqd_params())
+    m = heomq.Model.from_reference(**reference_arrays(o))
+    ctx = heomq.Context(m)
+    pat = heomq.Model(heomq.two_impurity()).spa_pattern()
+    nnz, ind, irow, icol = pat
+    ctx.set_spa_pattern(*pat)
+    x = O.seeded_ado_vector(o, 0x5EED0013)
+    a_of = np.repeat(np.arange(o.N), nnz)
+    xs = x[a_of, irow - 1, icol - 1]
+    import torch
+    y = ctx.apply_spa(torch.from_numpy(xs).to("cuda:0")).cpu().numpy()
+    xm = np.zeros_like(x)
+    xm[a_of, irow - 1, icol - 1] = xs
+    yd = Port(o).apply(xm, trace_row=None)
+    yo = yd[a_of, irow - 1, icol - 1]
+    print(f"two-impurity sparse apply: lunk_spa = {len(xs)}", flush=True)
+    assert np.linalg.norm(y[:-1] - yo) <= 1e-12 * np.linalg.norm(yo)
+    assert abs(y[-1] - np.trace(xm[0])) <= 1e-12 * abs(np.trace(xm[0]))
+    ctx.close()
+

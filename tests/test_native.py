@@ -253,3 +253,27 @@ def test_jacobi_eigensystem_host(pp, op):
     if same:
         assert abs(info["ejac"] - ej) <= 1e-12 * ej
         assert abs(info["cb_term"] - cbt) <= 1e-12 * cbt
+
+
+@pytest.mark.parametrize("pf,of", [
+    (lambda: heomq.siam(nk=2, ntop=3, V=0.3, U=1.5), lambda: O.siam_params(nk=2, ntop=3, V=0.3, U=1.5)),
+    (lambda: heomq.two_impurity(nk=2, ntop=2), lambda: O.dqd_params(nk=2, ntop=2)),
+    (lambda: heomq.siam(nk=2, ntop=3, V=0.3, U=1.5, gl=0.0), lambda: O.siam_params(nk=2, ntop=3, V=0.3, U=1.5, gl=0.0)),
+])
+def test_spa_pattern_builder_matches_oracle(pf, of):
+    # the native prelude_spa restatement against the oracle's (independent
+    # mixing coefficients: the pattern must not depend on them)
+    nat = heomq.Model(pf()).spa_pattern(seed=5)
+    ora = O.spa_pattern_build(O.Model(of()), seed=3)
+    for a, b in zip(nat, ora):
+        assert np.array_equal(np.asarray(a, np.int64), np.asarray(b, np.int64))
+
+
+def test_spa_pattern_builder_full_two_impurity_size():
+    # the two-impurity workload (d = 16, M = 112, L = 3): the built pattern has
+    # exactly the symmetry-packed layout's elements
+    m = heomq.Model(heomq.two_impurity())
+    nnz, ind, irow, icol = m.spa_pattern()
+    assert len(irow) == m.packed_layout()["packed"] == int(nnz.sum())
+    assert ind[0] == 1 and np.array_equal(np.diff(ind), nnz[:-1])
+

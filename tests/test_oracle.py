@@ -612,6 +612,101 @@ def test_calcderiv_spa_is_the_sparse_apply_and_the_dense_derivative():
     assert np.array_equal(y, yd[a_of, full[2] - 1, full[3] - 1])
 
 
+def test_spa_pattern_is_the_selection_rule_when_every_lead_couples():
+    """prelude_spa's pattern (exp(a H) at tier 0, products with the c
+    operators tier by tier) is a numerical construction; with every orbital
+    coupled to every lead it must reproduce, element for element, the (N, S_z)
+    selection rule the symmetry sectors are built from (physical_projection),
+    for a diagonal H (SIAM) and a hopping one (double dot)."""
+    for p in (O.siam_params(nk=2, ntop=3, V=0.3, U=1.5), O.dqd_params(nk=2, ntop=2)):
+        o = O.Model(p)
+        nnz, ind, irow, icol = O.spa_pattern_build(o, seed=3)
+        phys = O.physical_projection(o, np.ones((o.N, o.d, o.d), complex), hermitian=False) != 0
+        pn, _, pr, pc = O.spa_pattern_from_mask(phys)
+        assert np.array_equal(nnz, pn) and np.array_equal(irow, pr) and np.array_equal(icol, pc)
+        assert ind[0] == 1 and np.array_equal(np.diff(ind), nnz[:-1])
+
+
+def test_spa_pattern_holds_the_steady_state_and_ignores_the_mixing():
+    """Every element of the exact steady state lies in the pattern (the
+    sparse mode loses nothing), including a model whose left lead is
+    decoupled (its drawers empty their ADOs, which the steady state leaves at
+    zero); the mixing coefficients change no element of the pattern."""
+    for p in (O.siam_params(nk=2, ntop=2, V=0.3, U=1.5), O.siam_params(nk=2, ntop=2, V=0.3, U=1.5, gl=0.0)):
+        o = O.Model(p)
+        pats = [O.spa_pattern_build(o, seed=s) for s in (1, 2)]
+        assert all(np.array_equal(a, b) for a, b in zip(*pats))
+        nnz, _, irow, icol = pats[0]
+        mask = np.zeros((o.N, o.d, o.d), bool)
+        mask[np.repeat(np.arange(o.N), nnz), irow - 1, icol - 1] = True
+        x, _, _ = O.steady_state(o, tol=1e-13)
+        assert not np.any(np.abs(x[~mask]) > 1e-12 * np.abs(x).max())
+        if p.linewidth[0][0] == 0.0:
+            assert nnz.sum() < np.count_nonzero(O.physical_projection(o, np.ones_like(x), hermitian=False))
+
+
+def _trun_cases():
+    import dataclasses
+    return [O.siam_params(nk=2, ntop=1, V=0.5, U=1.0, T=0.3), O.dqd_params(nk=1, ntop=1, T=0.3),
+            O.siam_params(nk=1, ntop=2, V=0.4, U=1.5, T=0.3),
+            dataclasses.replace(O.siam_params(nk=1, ntop=1, V=0.5, U=1.0, T=0.3), lsdraw=True)]
+
+
+def test_trun_der_term_is_the_eliminated_next_tier():
+    """The terminator restated from the reference's loop (per top ADO and
+    drawer, the virtual ADO's lower couplings, the resolvent from the
+    Liouvillian eigenbasis) equals the hierarchy one tier higher with that
+    tier eliminated by dense solves: x_V = -S_V^-1 [A_{L+1}(x, 0)]_V, then
+    [A_{L+1}(x, x_V)] - A_L x on tiers 0..L.  (On the physical subspace:
+    the stored hierarchy keeps one ADO of each conjugate pair, and the pair
+    rule that lets either stand for the other holds there.)"""
+    import dataclasses
+    for p in _trun_cases():
+        o = O.Model(p)
+        o1 = O.Model(dataclasses.replace(p, ntop=p.ntop + 1))
+        d, n = o.d, o.N
+        assert all(np.array_equal(a, b) for a, b in zip(o.tiers, o1.tiers[:o.L + 1]))
+        # a vector of the physical subspace every state and iterate lives in
+        # (parity-allowed elements, and the conjugation rule of the paired
+        # storage on self-paired ADOs): a random mix of A^k applied to a
+        # trace-one tier-0 state
+        v = np.zeros((n, d, d), complex)
+        v[0] = np.diag(np.arange(1.0, d + 1))
+        x = np.zeros_like(v)
+        rng = np.random.default_rng(0x7E11)
+        for _ in range(o.L + 3):
+            v = O.apply(o, v, trace_row=None)
+            v /= np.linalg.norm(v)
+            x += rng.standard_normal() * v
+        X = np.zeros((o1.N, d, d), complex)
+        X[:n] = x
+        B = O.apply(o1, X, trace_row=None)[n:]
+        lsup = np.kron(np.eye(d), o.H) - np.kron(o.H.T, np.eye(d))
+        for v, K in enumerate(o1.tiers[o1.L]):
+            g = o1.gdraw[K].sum()
+            X[n + v] = np.linalg.solve(-1j * lsup + g * np.eye(d * d), -B[v].reshape(-1, order="F")).reshape(
+                d, d, order="F")
+        T = O.apply(o1, X, trace_row=None)[:n] - O.apply(o, x, trace_row=None)
+        Tr = O.trun_der_term(o, x)
+        assert np.abs(T[:o.first[o.L]]).max() <= 1e-13 * np.abs(T).max()   # only the top tier changes
+        assert np.linalg.norm(Tr - T) <= 1e-12 * np.linalg.norm(T)
+
+
+def test_trun_der_steady_state_is_the_next_tier():
+    """With the terminator the steady state on tiers 0..L is exactly that of
+    the hierarchy one tier higher (its top tier's equation is what the
+    terminator solves), and differs from the plain tier-L truncation."""
+    import dataclasses
+    for p in _trun_cases()[:1] + _trun_cases()[3:]:
+        o = O.Model(p)
+        o1 = O.Model(dataclasses.replace(p, ntop=p.ntop + 1))
+        xt, _, _ = O.steady_state(o, tol=1e-13, trun_der=True)
+        x1, _, _ = O.steady_state(o1, tol=1e-13)
+        xp, _, _ = O.steady_state(o, tol=1e-13)
+        assert np.abs(xt - x1[:o.N]).max() <= 1e-11
+        assert np.abs(xp[0] - x1[0]).max() >= 1e-5
+
+
 def test_system_field_limit_is_the_shifted_level():
     """dhs(t) of calcdhs's dedot (1 - exp(-t/aD)) on n_up + n_dn: long after
     the switch-on, hs + dhs is the Hamiltonian of the model whose level sits
