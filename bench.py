@@ -336,6 +336,9 @@ def sweep(args):
     import heomq
     tol = args.tol or 1e-10
     pts = sweep_points(args.points)
+    lscale = bool(args.lscale) if args.lscale is not None else True
+    for pt in pts:
+        pt["lscale"] = lscale
     groups = {}
     for i, pt in enumerate(pts):
         groups.setdefault((pt["nk"], pt["ntop"]), []).append(i)
@@ -398,6 +401,7 @@ def sweep(args):
         "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"SIAM sweep, {len(pts)} points in {len(groups)} shape batches", "tol": tol,
+                   "lscale": lscale,
                    "shapes": {f"nk{k[0]}_L{k[1]}": len(v) for k, v in sorted(groups.items())}},
         "plain_fp64_points_per_s": round(plain_rate, 3),
         "speedup_vs_plain": round(value / plain_rate, 2),

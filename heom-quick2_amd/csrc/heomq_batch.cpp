@@ -32,6 +32,7 @@ struct heomq_batch {
   heomq_ctx* ctx = nullptr;       // device tables of the shape
   DevBuf rec_all, gam_all, hq_union, hc_all, work, scal, partial, out, mask;
   DevPacked bf;                   // batched forward operator
+  std::vector<std::vector<double>> dbsqrt;   // per problem: lscale factors (1 unscaled), for the observables
   int64_t n = 0;                  // reals per problem vector (2 * packed elements)
   hipStream_t stream = nullptr;
   hipGraphExec_t chunk = nullptr;
@@ -105,8 +106,6 @@ extern "C" {
 
 int heomq_batch_create(const heomq_params* params, int nprob, int device, heomq_batch** out) {
   if (!params || nprob < 1 || !out) return fail(HEOMQ_BAD_ARG, "null argument or no problems");
-  for (int p = 0; p < nprob; ++p)   // the scaled representation differs problem by problem
-    if (params[p].lscale) return fail(HEOMQ_UNSUPPORTED, "batches take unscaled (lscale = 0) problems");
   auto B = std::make_unique<heomq_batch>();
   B->nprob = nprob;
   if (int rc = heomq_model_build(&params[0], &B->shape)) return rc;
@@ -121,8 +120,9 @@ int heomq_batch_create(const heomq_params* params, int nprob, int device, heomq_
     if (p > 0)
       if (int rc = set_params(scratch, params[p]))
         return fail(rc, "batch: problem " + std::to_string(p) + " differs in shape: " + last_error());
-    build_records(scratch, rec);
+    build_records(scratch, rec);   // lscale: every problem's upper couplings carry its own |cb|^(1/2)
     build_gamma(scratch, scratch.gdraw, gam, false);
+    B->dbsqrt.push_back(scratch.dbsqrt.empty() ? std::vector<double>(scratch.M, 1.0) : scratch.dbsqrt);
     rec_all.insert(rec_all.end(), rec.begin(), rec.end());
     gam_all.insert(gam_all.end(), gam.begin(), gam.end());
     for (int i = 0; i < c->d; ++i)   // the sector layout needs the conserved quantities of problem 0
@@ -394,6 +394,7 @@ int heomq_batch_solve(heomq_batch* B, double tol, int maxiter, double* rdo, doub
     }
     if (rdo) std::memcpy(rdo + 2 * (size_t)p * d * d, dense.data(), sizeof(double) * 2 * d * d);
     std::vector<double> cur(nal), oc(noc);
+    c->dbsqrt = B->dbsqrt[p];   // the currents read tier 1, stored scaled with lscale
     observables_host(c, dense, cur.data(), oc.data());
     if (current) std::memcpy(current + (size_t)p * nal, cur.data(), sizeof(double) * nal);
     if (occ) std::memcpy(occ + (size_t)p * noc, oc.data(), sizeof(double) * noc);

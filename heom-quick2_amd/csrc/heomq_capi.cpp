@@ -898,7 +898,7 @@ int build_trun(heomq_ctx* c) {
   tr.built = false;
   const Model& m = c->model;
   if (m.L < 1) return fail(HEOMQ_UNSUPPORTED, "derivative terminator: needs a top tier L >= 1");
-  if (m.p.ntop != m.L || m.p.nmats + 1 != m.ncor || m.p.norbs != m.norbs)
+  if (!m.from_params || m.p.ntop != m.L)
     return fail(HEOMQ_UNSUPPORTED, "derivative terminator: needs a model built from parameters (heomq_model_build)");
   heomq_params px = m.p;
   px.ntop = m.L + 1;

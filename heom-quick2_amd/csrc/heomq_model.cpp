@@ -472,6 +472,7 @@ int build_model(const heomq_params& p, Model& m) {
   if (int rc = check_params(p)) return rc;
   m = Model{};
   m.p = p;
+  m.from_params = true;
   m.norbs = p.norbs; m.nspin = p.nspin; m.nalf = p.nalf;
   m.ncor = 1 + p.nmats;
   m.nopr = 2 * p.norbs * p.nspin * p.nalf;

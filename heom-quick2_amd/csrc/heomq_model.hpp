@@ -31,6 +31,7 @@ struct UpperEntry {   // one upper-tier coupling (buildcoefindex.f90:289-362)
 
 struct Model {
   heomq_params p{};
+  bool from_params = false;                  // built by build_model (p describes it); false: loaded tables
   int d = 0, M = 0, L = 0, ncor = 0, nopr = 0, norbs = 0, nspin = 0, nalf = 0;
   int64_t N = 0;
   bool steady = true;

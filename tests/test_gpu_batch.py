@@ -63,3 +63,19 @@ def test_batch_solve_matches_oracle():
 def test_batch_rejects_mixed_shapes():
     with pytest.raises(heomq.HeomqError):
         heomq.Batch([heomq.siam(nk=3, ntop=3), heomq.siam(nk=2, ntop=3)])
+
+
+def test_batch_scaled_problems_match_oracle():
+    # the reference's lscale (ADOs stored scaled by prod |cb_k|^(-1/2), each
+    # problem with its own factors): states and currents are the unscaled ones
+    pl = [heomq.siam(**SHAPE, **pt, lscale=True) for pt in POINTS[:3]]
+    b = heomq.Batch(pl)
+    rdo, cur, occ, info = b.solve(tol=1e-12)
+    print(info, flush=True)
+    assert (info["status"] == 0).all()
+    iscale = np.abs(cur).max()
+    for p, pt in enumerate(POINTS[:3]):
+        o = O.Model(O.siam_params(**SHAPE, **pt))
+        xo, _, _ = O.steady_state(o, tol=1e-13)
+        assert np.abs(rdo[p] - xo[0]).max() <= 1e-10
+        assert np.abs(cur[p] - O.current(o, xo)).max() <= 1e-10 * iscale
