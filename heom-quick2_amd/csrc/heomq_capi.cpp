@@ -864,6 +864,14 @@ int build_spa_operator(heomq_ctx* c, const int32_t* nnz_spa, const std::vector<i
   p.frozen_flag = nullptr;
   so.op = p;
   if (int rc = build_spa_h(c)) return rc;
+  so.trace_slot = so.pos_h[(size_t)ph.sec[0] * dd + (d - 1) + (size_t)(d - 1) * d];
+  // the preconditioner of sparse steady solves, where the lists allow it
+  {
+    const std::string keep = last_error();
+    so.has_jac = build_tier_jacobi(m, S, so.elems_h, so.pos_h, so.nnz_h, ph.off, 0, so.jblk, so.jdesc, so.ju, so.je,
+                                   so.jac) == HEOMQ_OK;
+    last_error() = keep;
+  }
   so.built = true;
   return HEOMQ_OK;
 }

@@ -69,7 +69,11 @@ struct OperatorTables {
 struct SpaOperator {
   DevBuf off, sec, elems, pos, block, lblock, lnz, ent, maps, hgoff, hg, ltblk, ltados, ltoff, ltent;
   DevBuf hrow, hq, hc, hq_ell, hc_ell;
+  DevBuf jblk, jdesc, ju, je;        // block-Jacobi tables of the pattern's lists (steady solves)
   heomq::DevPacked op;               // structural fields; coefficients come from the context's operator
+  heomq::DevJacobi jac;
+  bool has_jac = false;              // every list closed under H's groups, every ADO <= 256 elements
+  int trace_slot = -1;               // slot of element (d, d) in ADO 0's list, -1 if absent
   int nsec = 0;
   bool built = false;
   std::vector<int16_t> elems_h, pos_h;   // host copies (H tables are rebuilt when H changes)

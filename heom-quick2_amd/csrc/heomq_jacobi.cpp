@@ -253,12 +253,11 @@ int build_tier_jacobi(const Model& m, int S, const std::vector<int16_t>& elems, 
   if (int rc = build_jacobi_host(m, jh)) return rc;
   const int d = m.d, dd = d * d;
   if (d > 16 || jh.gmax > 4)
-    return fail(HEOMQ_UNSUPPORTED, "derivative terminator: the eliminated tier's inverse needs d <= 16 and H groups "
-                                   "of at most 4 states");
+    return fail(HEOMQ_UNSUPPORTED, "ADO-aligned same-tier inverse: needs d <= 16 and H groups of at most 4 states");
   std::vector<int64_t> blk;
   for (int64_t a = a0; a < m.N; ++a) {
     const int64_t sz = off[a + 1] - off[a];
-    if (sz > 256) return fail(HEOMQ_UNSUPPORTED, "derivative terminator: an eliminated ADO holds more than 256 elements");
+    if (sz > 256) return fail(HEOMQ_UNSUPPORTED, "ADO-aligned same-tier inverse: an ADO holds more than 256 elements");
     if (blk.empty() || off[a + 1] - blk[blk.size() - 1] > 256 || a - blk[blk.size() - 2] >= 256) {
       blk.push_back(a);
       blk.push_back(off[a]);

@@ -396,8 +396,14 @@ hipError_t trun_apply(heomq_ctx* c, const double* x, double* y, bool rdo, int pr
 
 extern "C" {
 
-int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int maxiter, int trace,
-                       int transpose, double* stats, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// The steady TFQMR solve; x_packed: x already holds the packed iterate (the
+// context's packed layout, 2 * ptotal reals), no pack / unpack.
+int steady_solve(heomq_ctx* c, double* x, const double* b, double tol, int maxiter, int trace, int transpose,
+                 double* stats, void* stream, bool x_packed) {
   if (!c || !x) return fail(HEOMQ_BAD_ARG, "null argument");
   if (trace != HEOMQ_TRACE_LAST && trace != HEOMQ_TRACE_RDO) return fail(HEOMQ_BAD_ARG, "trace row required");
   if (c->trun.built && (trace != HEOMQ_TRACE_RDO || transpose))
@@ -416,9 +422,9 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
   if (int rc = carve<double>(c->ws64, n, kWorkVecs + (packed ? 4 : 1), v)) return rc;
   Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
   double* bb = v[kWorkVecs];
-  double* xs = packed ? v[kWorkVecs + 1] : x;
+  double* xs = packed && !x_packed ? v[kWorkVecs + 1] : x;
   if (packed) {
-    HIPCHECK((launch_pack<double, double>(c->pfwd, x, xs, st)));
+    if (!x_packed) HIPCHECK((launch_pack<double, double>(c->pfwd, x, xs, st)));
     if (b) {
       HIPCHECK((launch_pack<double, double>(c->pfwd, b, bb, st)));
     } else {
@@ -456,7 +462,7 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
   const int rc = tfqmr_run<double>(c, xs, bb, W, n, transpose != 0, tpos_of(c, trace), tol, maxiter, st, rs,
                                    packed ? 1 : 0, prec);
   rs.applies += pre;
-  if (packed) HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
+  if (packed && !x_packed) HIPCHECK((launch_unpack<double, double>(c->pfwd, xs, x, st)));
   HIPCHECK(hipStreamSynchronize(st));
   if (stats) {
     std::fill(stats, stats + HEOMQ_STATS_LEN, 0.0);
@@ -464,8 +470,56 @@ int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int
     stats[1] = rs.applies;
     stats[2] = rs.relres;
     stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats[4] = prec;
     stats[5] = (double)n;
   }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol, int maxiter, int trace,
+                       int transpose, double* stats, void* stream) {
+  return steady_solve(c, x, b, tol, maxiter, trace, transpose, stats, stream, false);
+}
+
+// The steady state on the sparse pattern (solve_tfqmr.f90 with lsparse = T):
+// TFQMR on P A P^T over rho_spa, the trace row on element (d, d) of tier 0
+// (which the pattern must hold), tier 0 Hermitianised; preconditioned as the
+// packed solve (block Jacobi when the pattern's lists are closed under H's
+// groups, Jacobi scaling for diagonal H).  The context's packed operator and
+// preconditioner tables are swapped for the pattern's for the duration.
+int heomq_spa_solve_steady(heomq_ctx* c, double* x_spa, double tol, int maxiter, double* stats, void* stream) {
+  if (!c || !x_spa) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (c->spa_lunk < 0 || !c->spa.built) return fail(HEOMQ_BAD_ARG, "no sparse pattern: call heomq_spa_pattern first");
+  if (c->trun.built) return fail(HEOMQ_UNSUPPORTED, "heomq_spa_solve_steady: not with the derivative terminator");
+  if (c->spa.trace_slot < 0)
+    return fail(HEOMQ_UNSUPPORTED, "sparse steady solve: the pattern lacks element (d, d) of tier 0 (the trace row)");
+  const int want = c->precond >= 0 ? c->precond : (c->h_diagonal ? 1 : 2);
+  if (want == 2 && !c->spa.has_jac)
+    return fail(HEOMQ_UNSUPPORTED, "sparse steady solve: block-Jacobi preconditioning needs every ADO's list closed "
+                                   "under H's groups (heomq_ctx_set_precond(c, 0) solves unpreconditioned)");
+  DevPacked op = spa_operator(c, nullptr);
+  op.trace_slot = c->spa.trace_slot;
+  op.herm0 = 1;
+  const DevPacked keep_op = c->pfwd;
+  const DevJacobi keep_jac = c->pjac;
+  const int64_t keep_total = c->ptotal;
+  c->drop_graphs();   // captured chunks hold the operator by value
+  c->pfwd = op;
+  if (c->spa.has_jac) {
+    DevJacobi jc = c->spa.jac;
+    jc.ejac = keep_jac.ejac;
+    c->pjac = jc;
+  }
+  c->ptotal = c->spa_lunk;
+  const int rc = steady_solve(c, x_spa, nullptr, tol, maxiter, HEOMQ_TRACE_RDO, 0, stats, stream, true);
+  c->drop_graphs();
+  c->pfwd = keep_op;
+  c->pjac = keep_jac;
+  c->ptotal = keep_total;
   return rc;
 }
 

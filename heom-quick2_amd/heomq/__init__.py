@@ -295,6 +295,36 @@ class Model:
                                             ir.ctypes.data_as(_lib.I32), ic.ctypes.data_as(_lib.I32)))
         return nnz, ind, ir[:n], ic[:n]
 
+    def ado_scale(self) -> np.ndarray:
+        """Per ADO, the factor between the physical ADO and the stored one:
+        prod_k |cb_k|^(1/2) over its drawers with lscale (the stored ADOs are
+        rho_K / that product), 1 without."""
+        s = np.ones(self.N)
+        if self.params is None or not self.params.lscale:
+            return s
+        cbs = np.abs(self.coefs()[0])     # scaled cb: |cb| / |cb|^(1/2) = |cb|^(1/2)
+        for n in range(1, self.L + 1):
+            s[self.first[n]:self.first[n + 1]] = np.prod(cbs[self.tier(n)], axis=1)
+        return s
+
+    def filter_pattern(self, x, thresh, pattern=None):
+        """The ADO norm filter: the sparse pattern (default: spa_pattern()) with
+        every ADO whose physical Frobenius norm in the state x is below thresh
+        left empty; tier 0 is always kept.  x is a state as the context holds
+        it ((N, d, d), Fortran element order); with lscale its ADOs are scaled
+        back by ado_scale() first.
+        Returns (nnz_spa, ind_spa, irow_spa, icol_spa, kept) with kept the
+        boolean mask of ADOs that hold elements."""
+        nnz, ind, ir, ic = self.spa_pattern() if pattern is None else pattern
+        xt = x if isinstance(x, np.ndarray) else x.reshape(self.N, -1).cpu().numpy()
+        norms = np.linalg.norm(xt.reshape(self.N, -1), axis=1) * self.ado_scale()
+        keep = norms >= thresh
+        keep[0] = True
+        sel = np.repeat(keep, nnz)
+        nnz2 = np.where(keep, nnz, 0).astype(np.int32)
+        ind2 = (1 + np.concatenate([[0], np.cumsum(nnz2)[:-1]])).astype(np.int64)
+        return nnz2, ind2, ir[sel], ic[sel], keep & (nnz > 0)
+
     def jacobi_info(self):
         """(info dict, eigenvalues of H, eigenvectors of H) of the block-Jacobi
         iteration, built on the host (prelude_jacobi.f90:43-113)."""
@@ -521,6 +551,7 @@ class Context:
         if nnz.size != self.N or ind.size != self.N or ir.size != ic.size:
             raise HeomqError(_lib.BAD_DIM, "nnz_spa / ind_spa need N entries, irow_spa / icol_spa lunk_spa each")
         self._spa_lunk = int(ir.size)
+        self._spa_pat = (nnz.copy(), ir.copy(), ic.copy())
         check(lib().heomq_spa_pattern(self._h, ir.size, nnz.ctypes.data_as(_lib.I32), ind.ctypes.data_as(_lib.I64),
                                       ir.ctypes.data_as(_lib.I32), ic.ctypes.data_as(_lib.I32)))
 
@@ -536,6 +567,37 @@ class Context:
         check(lib().heomq_apply_spa(self._h, C.c_void_p(x_spa.data_ptr()), C.c_void_p(y_spa.data_ptr()),
                                     self._stream()))
         return y_spa
+
+    def spa_initial_guess(self):
+        """rho_spa of the trace-one start: identity / d on tier 0's diagonal
+        elements of the registered pattern, every other element zero."""
+        t = self.torch
+        n = getattr(self, "_spa_lunk", -1)
+        if n < 0:
+            raise HeomqError(_lib.BAD_ARG, "no sparse pattern: call set_spa_pattern first")
+        nnz0 = int(self._spa_pat[0][0])
+        ir, ic = self._spa_pat[1][:nnz0], self._spa_pat[2][:nnz0]
+        x = np.zeros(n, np.complex128)
+        x[:nnz0][ir == ic] = 1.0 / self.model.d
+        return t.from_numpy(x).to(f"cuda:{self.device}")
+
+    def solve_steady_spa(self, x_spa=None, tol=1e-12, maxiter=20000, raise_on_fail=True):
+        """The steady state on the registered sparse pattern (solve_tfqmr with
+        lsparse = T): TFQMR on P A P^T over rho_spa, preconditioned as the
+        packed solve.  Returns (x_spa, stats dict)."""
+        t = self.torch
+        x_spa = self.spa_initial_guess() if x_spa is None else x_spa
+        n = getattr(self, "_spa_lunk", -1)
+        if x_spa.dtype != t.complex128 or not x_spa.is_cuda or not x_spa.is_contiguous() or x_spa.numel() != n:
+            raise HeomqError(_lib.BAD_DIM, "solve_steady_spa: a contiguous complex128 device vector of lunk_spa")
+        stats = np.zeros(_lib.STATS_LEN)
+        rc = lib().heomq_spa_solve_steady(self._h, C.c_void_p(x_spa.data_ptr()), tol, maxiter, _dp(stats),
+                                          self._stream())
+        info = dict(iters=int(stats[0]), applies=int(stats[1]), relres=stats[2], seconds=stats[3], status=rc,
+                    precond=int(stats[4]))
+        if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
+            check(rc)
+        return x_spa, info
 
     def calcderiv_spa(self, x_spa, tt=0.0, y_spa=None):
         """calcderiv_spa_omp(tt) on device vectors: y = P L(tt) P^T x over the

@@ -90,6 +90,7 @@ SIGNATURES = {
     "heomq_solve_bicg_host": (C.c_int, [P, D, C.c_double, C.c_int, D]),
     "heomq_spa_pattern": (C.c_int, [P, C.c_int64, I32, I64, I32, I32]),
     "heomq_spa_build_pattern": (C.c_int, [P, C.c_double, C.c_uint64, C.c_int64, I64, I32, I64, I32, I32]),
+    "heomq_spa_solve_steady": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_apply_spa": (C.c_int, [P, P, P, P]),
     "heomq_calc_ax_tfqmr_spa": (C.c_int, [P, C.c_int64, D, D]),
     "heomq_calcderiv_spa_at": (C.c_int, [P, C.c_double, P, P, P]),

@@ -389,6 +389,17 @@ int heomq_spa_build_pattern(const heomq_model* m, double dsmall, uint64_t seed, 
  * number of distinct per-ADO element lists (at most 65535). */
 int heomq_spa_pattern(heomq_ctx* c, int64_t lunk_spa, const int32_t* nnz_spa, const int64_t* ind_spa,
                       const int32_t* irow_spa, const int32_t* icol_spa);
+/* The steady state on the registered pattern (tfqmr_double/solve_tfqmr.f90
+ * with lsparse = T, which iterates calc_ax_tfqmr_spa_omp): TFQMR on P A P^T
+ * over x_spa (lunk_spa complex, in: a trace-one guess, out: the state), the
+ * trace row on element (d, d) of tier 0 (the pattern must hold it), tier 0
+ * kept Hermitian.  Preconditioned as heomq_solve_steady (block Jacobi needs
+ * every ADO's list closed under H's groups; else HEOMQ_UNSUPPORTED unless
+ * heomq_ctx_set_precond chose none or Jacobi scaling).  stats
+ * (HEOMQ_STATS_LEN): [0] iterations, [1] applies, [2] relative residual,
+ * [3] seconds, [4] preconditioner used (0 none, 1 Jacobi scaling, 2 block
+ * Jacobi), [5] reals per vector. */
+int heomq_spa_solve_steady(heomq_ctx* c, double* x_spa, double tol, int maxiter, double* stats, void* stream);
 /* Device vectors: x_spa (lunk_spa complex), y_spa (lunk_spa + 1 complex). */
 int heomq_apply_spa(heomq_ctx* c, const double* x_spa, double* y_spa, void* stream);
 /* calc_ax_tfqmr_spa_omp(ldim, rcgtmp, rcgrhs) on host arrays; ldim /= lunk_spa

@@ -121,3 +121,68 @@ def test_apply_spa_two_impurity_workload_against_c_port():
     assert abs(y[-1] - np.trace(xm[0])) <= 1e-12 * abs(np.trace(xm[0]))
     ctx.close()
 
+
+
+def _spa_to_dense(o, pat, xs):
+    nnz, _, irow, icol = pat
+    x = np.zeros((o.N, o.d, o.d), complex)
+    x[np.repeat(np.arange(o.N), nnz), irow - 1, icol - 1] = xs
+    return x
+
+
+@pytest.mark.parametrize("name", ["siam_nk2_L3", "dqd_nk2_L2"])
+def test_solve_steady_spa_matches_oracle(name):
+    # solve_tfqmr with lsparse = T on the built pattern: Jacobi scaling for
+    # the diagonal H, block Jacobi for the double dot
+    m, ctx, o, torch = setup(name)
+    pat = m.spa_pattern()
+    ctx.set_spa_pattern(*pat)
+    xs, st = ctx.solve_steady_spa(tol=1e-12)
+    print(name, st, flush=True)
+    assert st["status"] == 0 and st["precond"] == (1 if name.startswith("siam") else 2)
+    x = _spa_to_dense(o, pat, xs.cpu().numpy())
+    xo, _, _ = O.steady_state(o, tol=1e-13)
+    assert np.abs(x[0] - xo[0]).max() <= 1e-10
+    assert np.abs(O.current(o, x) - O.current(o, xo)).max() <= 1e-9 * np.abs(O.current(o, xo)).max()
+
+
+def test_solve_steady_spa_two_impurity_workload_and_norm_filter():
+    # the two-impurity workload (lscale) on its built pattern: the sparse
+    # solve reaches the fixture's state; then the ADO norm filter (physical
+    # Frobenius norm below 1e-8 dropped, from the full state) and a solve on
+    # the filtered pattern, whose RDO error against the unfiltered fixture is
+    # reported and bounded
+    import json
+    from pathlib import Path
+    import torch
+    fix = json.loads((Path(__file__).resolve().parent / "golden" / "steady_dqd_nk6_L3.json").read_text())
+    rho = np.array(fix["rdo_re"]) + 1j * np.array(fix["rdo_im"])
+    m = heomq.Model(heomq.two_impurity(lscale=True))
+    ctx = heomq.Context(m)
+    pat = m.spa_pattern()
+    ctx.set_spa_pattern(*pat)
+    xs, st = ctx.solve_steady_spa(tol=1e-10)
+    print("sparse solve, full pattern:", st, flush=True)
+    r = xs[:int(pat[0][0])].cpu().numpy()
+    nnz0, ir, ic = int(pat[0][0]), pat[2][:int(pat[0][0])] - 1, pat[3][:int(pat[0][0])] - 1
+    r0 = np.zeros((m.d, m.d), complex)
+    r0[ir, ic] = r
+    assert st["status"] == 0 and np.abs(r0 - rho).max() <= 1e-10
+    x, _ = ctx.solve_steady(tol=1e-10)
+    # at 1e-8 no ADO that holds elements is dropped (the smallest physical ADO
+    # norm of this state is above it); at 1e-5 a quarter of the elements go
+    # and the RDO moves by a few 1e-2 (measured 3.2e-2): the filter's error,
+    # reported against the unfiltered fixture
+    for thresh, bound in ((1e-8, 1e-10), (1e-5, 1e-1)):
+        fp = m.filter_pattern(x, thresh)
+        kept = fp[4]
+        ctx.set_spa_pattern(*fp[:4])
+        xf, stf = ctx.solve_steady_spa(tol=1e-10)
+        nf = int(fp[0][0])
+        rf = np.zeros((m.d, m.d), complex)
+        rf[fp[2][:nf] - 1, fp[3][:nf] - 1] = xf[:nf].cpu().numpy()
+        err = np.abs(rf - rho).max()
+        print(f"filter {thresh:.0e}: {kept.sum()} of {m.N} ADOs, {len(fp[2])} of {len(pat[2])} elements; "
+              f"RDO error vs the unfiltered fixture {err:.2e}; {stf}", flush=True)
+        assert stf["status"] == 0 and err <= bound
+    ctx.close()
