@@ -142,6 +142,8 @@ int heomq_batch_create(const heomq_params* params, int nprob, int device, heomq_
   HIPCHECK(upload(B->hq_union, hq));
   HIPCHECK(upload(B->hc_all, hc));
   const int64_t nrec = (int64_t)rec.size();
+  // the apply form (work granularity) timed on this shape and GPU
+  if (int rc = heomq_ctx_autotune(c, 10, nullptr, nullptr)) return rc;
   B->bf = c->pfwd;
   B->bf.rec = B->rec_all.as<DevRec>();
   B->bf.gam = B->gam_all.as<double>();

@@ -663,6 +663,9 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
   c->nsec = ph.S;
   if (int rc = build_packed_h(c, m)) return rc;
   c->ptotal = ph.total;
+  c->heavy_wave_ok = ph.n_heavy > 1;
+  for (int64_t a = 1; a < ph.n_heavy; ++a)
+    if (ph.off[a + 1] - ph.off[a] > kHeavyWaveMaxNnz) c->heavy_wave_ok = false;
   for (int w = 0; w < 2; ++w) {
     DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
     p.n_ado = m.N;
@@ -695,10 +698,13 @@ int build_packed(heomq_ctx* c, const Model& m, const std::vector<int64_t>& rowf,
 
 // Form of the packed apply of operator w (0 forward, 1 transposed): bit 0
 // the grouped heavy rows, bit 1 the sector-batched top tier (0: the sliced
-// heavy and packed-order light forms).  Every form computes the same operator
-// (to rounding); a form whose tables were not built falls back per half.
+// heavy and packed-order light forms), bit 2 one wave per heavy ADO (small
+// sectors only).  Every form computes the same operator (to rounding); a form
+// whose tables were not built falls back per half.
 void set_apply_form(heomq_ctx* c, int w, int form) {
   DevPacked& p = w == 0 ? c->pfwd : c->ptrn;
+  const bool wave = (form & 4) != 0 && c->heavy_wave_ok;
+  p.heavy_wave = wave ? 1 : 0;
   const bool grouped = (form & 1) != 0, sector = (form & 2) != 0 && c->p_ltblk.p != nullptr;
   p.hg_off = grouped ? (w == 0 ? c->p_hgoff_f : c->p_hgoff_t).as<int32_t>() : nullptr;
   p.hg = grouped ? (w == 0 ? c->p_hg_f : c->p_hg_t).as<int4>() : nullptr;
@@ -706,7 +712,7 @@ void set_apply_form(heomq_ctx* c, int w, int form) {
   p.lt_off = sector ? c->p_ltoff.as<int32_t>() : nullptr;
   p.lt_id = sector ? c->p_ltados.as<int32_t>() : nullptr;
   p.lt_ent = sector ? (w == 0 ? c->p_ltent_f : c->p_ltent_t).as<DevEntryP>() : nullptr;
-  c->apply_form[w] = (grouped ? 1 : 0) | (sector ? 2 : 0);
+  c->apply_form[w] = (grouped ? 1 : 0) | (sector ? 2 : 0) | (wave ? 4 : 0);
 }
 
 // coefficient pointers of the packed operators follow the dense ones
@@ -862,6 +868,9 @@ int build_spa_operator(heomq_ctx* c, const int32_t* nnz_spa, const std::vector<i
   p.nprob = 1;
   p.vstride = p.rec_stride = p.gam_stride = p.hc_stride = 0;
   p.frozen_flag = nullptr;
+  p.heavy_wave = ph.n_heavy > 1 ? 1 : 0;   // the pattern's heavy ADOs all small: one wave each
+  for (int64_t a = 1; a < ph.n_heavy && p.heavy_wave; ++a)
+    if (ph.off[a + 1] - ph.off[a] > kHeavyWaveMaxNnz) p.heavy_wave = 0;
   so.op = p;
   if (int rc = build_spa_h(c)) return rc;
   so.trace_slot = so.pos_h[(size_t)ph.sec[0] * dd + (d - 1) + (size_t)(d - 1) * d];
@@ -1252,7 +1261,8 @@ int heomq_ctx_autotune(heomq_ctx* c, int reps, int32_t* forms, double* times_us)
   for (int w = 0; w < 2 && rc == HEOMQ_OK; ++w) {
     double best = 1e300;
     int best_f = 2;
-    for (int f = 0; f < 4; ++f) {
+    for (int f = 0; f < 8; ++f) {
+      if ((f & 4) && !c->heavy_wave_ok) continue;
       set_apply_form(c, w, f);
       const DevPacked& op = w == 0 ? c->pfwd : c->ptrn;
       hipError_t e = hipSuccess;
@@ -1269,7 +1279,7 @@ int heomq_ctx_autotune(heomq_ctx* c, int reps, int32_t* forms, double* times_us)
         break;
       }
       const double us = 1e3 * ms / reps;
-      if (times_us) times_us[4 * w + f] = us;
+      if (times_us) times_us[8 * w + f] = us;
       if (us < best) { best = us; best_f = f; }
     }
     set_apply_form(c, w, best_f);
@@ -1299,6 +1309,22 @@ int heomq_ctx_refresh(heomq_ctx* c, const heomq_model* h) {
   c->steady_model = h->m.steady;
   c->field_amp = h->m.eleadinfty;
   HIPCHECK(hipDeviceSynchronize());
+  return HEOMQ_OK;
+}
+
+int heomq_ctx_set_apply_form(heomq_ctx* c, int forward, int transposed) {
+  if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
+  if (forward < 0 || forward > 7 || transposed < 0 || transposed > 7) return fail(HEOMQ_BAD_ARG, "forms are 0..7");
+  c->drop_graphs();   // captured chunks hold the operator by value
+  set_apply_form(c, 0, forward);
+  set_apply_form(c, 1, transposed);
+  return HEOMQ_OK;
+}
+
+int heomq_ctx_get_apply_form(const heomq_ctx* c, int32_t* forms) {
+  if (!c || !forms) return fail(HEOMQ_BAD_ARG, "null argument");
+  forms[0] = c->apply_form[0];
+  forms[1] = c->apply_form[1];
   return HEOMQ_OK;
 }
 

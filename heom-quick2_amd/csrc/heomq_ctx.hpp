@@ -130,6 +130,7 @@ struct heomq_ctx {
   DevBuf p_hgoff_f, p_hgoff_t, p_hg_f, p_hg_t, p_ltblk, p_ltados, p_ltoff, p_ltent_f, p_ltent_t;   // restructured apply
   heomq::DevPacked pfwd, ptrn;
   int apply_form[2] = {2, 2};        // packed apply form of the forward / transposed operator
+  bool heavy_wave_ok = false;        // every heavy ADO but ADO 0 holds at most kHeavyWaveMaxNnz elements
   int64_t ptotal = 0;
   int nsec = 0;
   bool cons_n = true, cons_s = true;

@@ -160,7 +160,18 @@ struct DevPacked {
   // 1: the tier-0 output block is Hermitianised (the solvers' projected
   // operator); 0: the plain restriction of A to the sector (propagation)
   int herm0 = 1;
+  // 1: ADOs 1 .. n_heavy - 1 run one wave each, four per workgroup (sectors
+  // of at most 16 elements; ADO 0 keeps its own workgroup for the tier-0
+  // fix-up)
+  int heavy_wave = 0;
 };
+
+// Workgroups of the heavy part of a packed apply launch.
+__host__ __device__ inline int64_t heavy_blocks(const DevPacked& op) {
+  if (!op.heavy_wave) return op.n_heavy;
+  return op.n_heavy > 0 ? 1 + (op.n_heavy - 1 + 3) / 4 : 0;
+}
+constexpr int kHeavyWaveMaxNnz = 16;
 
 // Projected apply on packed vectors: y = Pi A x with the trace row on the
 // tier-0 block (rdo = true) or none, tier-0 output Hermitianised (herm0).

@@ -544,6 +544,67 @@ __device__ inline void heavy_grouped(const DevPacked& op, const Coefs cf_p, cons
   if (a == 0) tier0_fixup_block<T>(op, x, y, rdo);   // block-uniform branch
 }
 
+// Heavy ADOs of small sectors (at most 16 elements, e.g. every ADO of the
+// single-impurity model): one wave per ADO, four ADOs per workgroup.  The
+// wave's 64 lanes are (slice, slot) with the slot count rounded up to a power
+// of two w, so 64 / w slices walk the coupling list (entries, records and
+// gather maps from global memory, two couplings in flight); the slices are
+// summed by shuffles in a fixed tree, with no LDS and no barrier.
+template <typename T, bool TRANS>
+__device__ inline void heavy_wave(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
+                                  int64_t zero_at, int64_t a, bool scaled) {
+  using V = typename P2<T>::type;
+  const int lane = threadIdx.x & 63;
+  const int64_t base = op.off[a];
+  const int nnz = (int)(op.off[a + 1] - base);
+  if (nnz == 0) return;   // wave-uniform
+  int w = 1;
+  while (w < nnz) w <<= 1;   // <= kHeavyWaveMaxNnz (host check)
+  const int S = 64 / w, slice = lane / w, t = lane - slice * w;
+  const bool act = t < nnz;
+  const V* __restrict__ xv = reinterpret_cast<const V*>(x);
+  const int s = op.sec[a];
+  const T gr = (T)cf_p.gam[2 * a], gi = (T)cf_p.gam[2 * a + 1];
+  const int64_t p0 = op.rowptr[a];
+  const int nent = (int)(op.rowptr[a + 1] - p0);
+  T ar = 0, ai = 0, csr = 0, csi = 0;
+  if (act) {
+    if (slice == 0)
+      diag_term<T, TRANS>(op, cf_p.hc_ell, xv, gr, gi, base, s, t, zero_at, ar, ai, nullptr, 0, 0, csr, csi);
+    constexpr int U = 2;
+    for (int k0 = slice; k0 < nent; k0 += S * U) {
+      DevEntryP en[U];
+      CoefLR cf[U];
+      int nv = 0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * S;
+        if (k < nent) {
+          en[u] = op.ent[p0 + k];
+          cf[u] = load_coef(cf_p.rec, en[u].rec);
+          nv = u + 1;
+        } else {
+          en[u] = DevEntryP{0, -1, -1, 0, 0, 0};
+          cf[u] = CoefLR{};
+        }
+      }
+      coupling_block<T, TRANS, U>(en, cf, nv, op.maps, xv, t, zero_at, ar, ai);
+    }
+  }
+  // lane = slice * w + t: shuffles by multiples of w fold the slices onto slice 0
+  for (int off = 32; off >= w; off >>= 1) {
+    ar += __shfl_down(ar, off, 64);
+    ai += __shfl_down(ai, off, 64);
+  }
+  if (slice == 0 && act) {
+    if (scaled) jacobi_scale<T>(op, gr, gi, csr, csi, ar, ai);
+    V o;
+    o.x = ar;
+    o.y = ai;
+    reinterpret_cast<V*>(y)[base + t] = o;
+  }
+}
+
 // Top tier, sector-batched: a block takes 256 / nnz top-tier ADOs of one
 // sector, one lane per (ADO, slot).  Each lane loads its ADO's first element,
 // rate and couplings from the block's own tables (one level below the block
@@ -663,16 +724,22 @@ k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, i
   x += 2 * p * op.vstride;
   y += 2 * p * op.vstride;
   const Coefs cf{op.rec + p * op.rec_stride, op.gam + 2 * p * op.gam_stride, op.hc_ell + 2 * p * op.hc_stride};
-  if ((int64_t)blockIdx.x < op.n_heavy) {   // block-uniform
+  const int64_t hb = heavy_blocks(op);
+  if ((int64_t)blockIdx.x < hb) {   // block-uniform
+    if (op.heavy_wave && blockIdx.x > 0) {   // four ADOs of small sectors, a wave each (no barrier)
+      const int64_t a = 1 + ((int64_t)blockIdx.x - 1) * 4 + (threadIdx.x >> 6);
+      if (a < op.n_heavy) heavy_wave<T, TRANS>(op, cf, x, y, zero_at, a, scaled);
+      return;
+    }
     const int a = (int)blockIdx.x;
     if (op.hg_off && op.hg_off[a + 1] > op.hg_off[a])
       heavy_grouped<T, TRANS>(op, cf, x, y, zero_at, rdo, a, smem, scaled);
     else
       heavy_body<T, TRANS>(op, cf, x, y, zero_at, rdo, a, smem, scaled);
   } else if (op.lt_blk) {
-    light_sector<T, TRANS>(op, cf, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem, scaled);
+    light_sector<T, TRANS>(op, cf, x, y, zero_at, (int)(blockIdx.x - hb), smem, scaled);
   } else {
-    light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(blockIdx.x - op.n_heavy), smem, scaled);
+    light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(blockIdx.x - hb), smem, scaled);
   }
 }
 
@@ -1165,7 +1232,7 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
   if (op.total <= 0) return hipSuccess;
   const int64_t zero_at = (op.transposed && rdo) ? op.trace_slot : -1;   // ADO 0 starts at packed offset 0
   const int64_t light = op.total - op.light_begin;
-  const int64_t blocks = op.n_heavy + (op.lt_blk ? op.n_lt_blocks : (light > 0 ? (light + 255) / 256 : 0));
+  const int64_t blocks = heavy_blocks(op) + (op.lt_blk ? op.n_lt_blocks : (light > 0 ? (light + 255) / 256 : 0));
   if (blocks > 0) {
     const dim3 grid((unsigned)blocks, (unsigned)op.nprob);
     const int u = op.max_light_couplings;

@@ -407,13 +407,23 @@ class Context:
         check(lib().heomq_ctx_set_trun_der(self._h, int(enable)))
 
     def autotune(self, reps: int = 20) -> dict:
-        """Time the packed apply's four forms on this GPU and keep the fastest
-        per operator; returns the choices and times (us)."""
+        """Time the packed apply's forms on this GPU and keep the fastest per
+        operator; returns the choices and times (us; None for forms the layout
+        does not take)."""
         forms = np.zeros(2, np.int32)
-        times = np.zeros(8)
+        times = np.full(16, np.nan)
         check(lib().heomq_ctx_autotune(self._h, int(reps), forms.ctypes.data_as(_lib.I32), _dp(times)))
+        clean = [None if np.isnan(v) else float(v) for v in times]
         return dict(forward=int(forms[0]), transposed=int(forms[1]),
-                    times_us={"forward": times[:4].tolist(), "transposed": times[4:].tolist()})
+                    times_us={"forward": clean[:8], "transposed": clean[8:]})
+
+    def set_apply_form(self, forward: int, transposed: int | None = None):
+        """Fix the packed apply's form (bits as autotune); returns the forms in
+        use (a form the layout does not take falls back per part)."""
+        check(lib().heomq_ctx_set_apply_form(self._h, int(forward), int(forward if transposed is None else transposed)))
+        forms = np.zeros(2, np.int32)
+        check(lib().heomq_ctx_get_apply_form(self._h, forms.ctypes.data_as(_lib.I32)))
+        return int(forms[0]), int(forms[1])
 
     def set_graphs(self, enable: bool):
         """Replay solver iterations from captured graphs (default) or launch eagerly."""

@@ -118,6 +118,8 @@ SIGNATURES = {
     "heomq_ctx_field_shifts": (C.c_int, [P, C.c_double, D]),
     "heomq_ctx_set_presmooth": (C.c_int, [P, C.c_int]),
     "heomq_ctx_set_trun_der": (C.c_int, [P, C.c_int]),
+    "heomq_ctx_set_apply_form": (C.c_int, [P, C.c_int, C.c_int]),
+    "heomq_ctx_get_apply_form": (C.c_int, [P, I32]),
     "heomq_chebyshev": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),
     "heomq_rk4_packed": (C.c_int, [P, P, C.c_double, C.c_double, C.c_int, P]),
     "heomq_chebyshev_packed": (C.c_int, [P, P, C.c_double, C.c_int, D, P]),

@@ -204,16 +204,25 @@ int heomq_ctx_set_precond_shift(heomq_ctx* c, double shift);
  * (restarts take the following seeds); default 0.  The reference draws its
  * shadow vector with DRANDN, seed 1 (tfqmr_double/dutfx.f90:178). */
 int heomq_ctx_set_shadow_seed(heomq_ctx* c, uint64_t seed);
-/* Autotuner of the packed apply: times each of its four forms (bit 0: ADOs
- * below the top tier with their couplings grouped by gather map, summed in the
+/* Autotuner of the packed apply: times each of its forms (bit 0: ADOs below
+ * the top tier with their couplings grouped by gather map, summed in the
  * neighbour's slot space before the map; bit 1: top-tier blocks batched by
- * symmetry sector; 0: one workgroup per ADO with the couplings sliced over the
+ * symmetry sector; bit 2: one wave per ADO below the top tier, four per
+ * workgroup, when every such ADO but the tier-0 one holds at most 16
+ * elements; 0: one workgroup per ADO with the couplings sliced over the
  * lanes, top tier in packed-element order) over `reps` launches on this GPU,
  * for the forward and the transposed operator, and keeps the fastest.
- * forms[0..1] (may be NULL) receive the choices, times_us[0..7] (may be
- * NULL) the times, form f of operator w at [4 w + f].  Every form computes
- * the same operator.  No counterpart in the reference. */
+ * forms[0..1] (may be NULL) receive the choices, times_us[0..15] (may be
+ * NULL) the times, form f of operator w at [8 w + f] (forms the layout does
+ * not take are left untouched).  Every form computes the same operator.  No
+ * counterpart in the reference. */
 int heomq_ctx_autotune(heomq_ctx* c, int reps, int32_t* forms, double* times_us);
+/* Set the packed apply's form per operator (bits as heomq_ctx_autotune; a
+ * form the layout does not take falls back per part), and read the forms in
+ * use (forms[0] forward, forms[1] transposed).  No counterpart in the
+ * reference. */
+int heomq_ctx_set_apply_form(heomq_ctx* c, int forward, int transposed);
+int heomq_ctx_get_apply_form(const heomq_ctx* c, int32_t* forms);
 /* Number of complex elements N*d*d of one ADO vector. */
 int64_t heomq_ctx_vector_len(const heomq_ctx* c);
 
