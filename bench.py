@@ -73,6 +73,12 @@ def main():
     ap.add_argument("--workload", default="two_impurity", choices=["two_impurity", "siam", "sweep", "didv"])
     ap.add_argument("--points", type=int, default=256, help="sweep: number of parameter points")
     ap.add_argument("--maxiter", type=int, default=3000, help="sweep: TFQMR iteration limit per point")
+    ap.add_argument("--driver", default="sweep", choices=["sweep", "batches"],
+                    help="sweep: native drivers over every shape (one launch per solver step), or one "
+                         "batch per shape on its own host thread and stream")
+    ap.add_argument("--sweeps", type=int, default=1,
+                    help="sweep driver: points dealt round-robin to this many sweeps, each on its own host "
+                         "thread and stream (their phases overlap on the device)")
     ap.add_argument("--tol", type=float, default=None,
                     help="residual target ||b - A x|| / ||b|| (||b|| = 1; default 3e-11 for the scaled "
                          "two-impurity workload, 2e-8 unscaled, 1e-12 for SIAM)")
@@ -329,8 +335,9 @@ def sweep_points(npts, seed=0x5EED0004):
 
 
 def sweep(args):
-    """Batched sweep: every point's FP64 steady state (relative residual <= tol),
-    the points of each hierarchy shape solved together as one batch."""
+    """Batched sweep: every point's FP64 steady state (relative residual <= tol).
+    Default driver: one native sweep over every hierarchy shape, one launch per
+    solver step; --driver batches: one batch per shape on its own thread."""
     import numpy as np
     import torch
     import heomq
@@ -342,35 +349,48 @@ def sweep(args):
     groups = {}
     for i, pt in enumerate(pts):
         groups.setdefault((pt["nk"], pt["ntop"]), []).append(i)
-    batches = {k: heomq.Batch([heomq.siam(**pts[i]) for i in idx]) for k, idx in groups.items()}
     cur = np.zeros((len(pts), 2, 2))
     rdo = {}
     iters = np.zeros(len(pts))
     relres = np.zeros(len(pts))
     failed = {}
-
-    from concurrent.futures import ThreadPoolExecutor
-    pool = ThreadPoolExecutor(max_workers=len(groups))   # each batch runs on its own stream
-
     shape_ms = {}
 
-    def solve_group(k):
-        t = time.perf_counter()
-        out = batches[k].solve(tol=tol, maxiter=args.maxiter, raise_on_fail=False)
-        shape_ms[f"nk{k[0]}_L{k[1]}"] = round((time.perf_counter() - t) * 1e3, 2)
-        return k, out
+    def record(idx, r, c, info):
+        cur[idx] = c
+        iters[idx] = info["iters"]
+        relres[idx] = info["relres"]
+        for j, i in enumerate(idx):
+            rdo[i] = r[j]
+            if info["status"][j] != 0:
+                failed[i] = dict(pts[i], relres=float(info["relres"][j]), iters=int(info["iters"][j]),
+                                 restarts=int(info["restarts"][j]))
 
-    def run():
-        for k, (r, c, o, info) in pool.map(solve_group, list(groups)):
-            idx = groups[k]
-            cur[idx] = c
-            iters[idx] = info["iters"]
-            relres[idx] = info["relres"]
-            for j, i in enumerate(idx):
-                rdo[i] = r[j]
-                if info["status"][j] != 0:
-                    failed[i] = dict(pts[i], relres=float(info["relres"][j]), iters=int(info["iters"][j]),
-                                     restarts=int(info["restarts"][j]))
+    if args.driver == "sweep":
+        K = max(1, args.sweeps)
+        parts = [list(range(k, len(pts), K)) for k in range(K)]
+        sws = [heomq.Sweep([heomq.siam(**pts[i]) for i in idx]) for idx in parts]
+        from concurrent.futures import ThreadPoolExecutor
+        pool = ThreadPoolExecutor(max_workers=K)
+
+        def run():
+            outs = pool.map(lambda sw: sw.solve(tol=tol, maxiter=args.maxiter, raise_on_fail=False), sws)
+            for idx, (r, c, o, info) in zip(parts, outs):
+                record(idx, r, c, info)
+    else:
+        batches = {k: heomq.Batch([heomq.siam(**pts[i]) for i in idx]) for k, idx in groups.items()}
+        from concurrent.futures import ThreadPoolExecutor
+        pool = ThreadPoolExecutor(max_workers=len(groups))   # each batch runs on its own stream
+
+        def solve_group(k):
+            t = time.perf_counter()
+            out = batches[k].solve(tol=tol, maxiter=args.maxiter, raise_on_fail=False)
+            shape_ms[f"nk{k[0]}_L{k[1]}"] = round((time.perf_counter() - t) * 1e3, 2)
+            return k, out
+
+        def run():
+            for k, (r, c, o, info) in pool.map(solve_group, list(groups)):
+                record(groups[k], r, c, info)
 
     for _ in range(args.warmup):
         run()
@@ -400,15 +420,18 @@ def sweep(args):
         "value": round(value, 3), "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"SIAM sweep, {len(pts)} points in {len(groups)} shape batches", "tol": tol,
-                   "lscale": lscale,
+        "config": {"workload": f"SIAM sweep, {len(pts)} points of {len(groups)} hierarchy shapes", "tol": tol,
+                   "lscale": lscale, "driver": args.driver,
+                   **({"sweeps": max(1, args.sweeps)} if args.driver == "sweep" else {}),
                    "shapes": {f"nk{k[0]}_L{k[1]}": len(v) for k, v in sorted(groups.items())}},
         "plain_fp64_points_per_s": round(plain_rate, 3),
         "speedup_vs_plain": round(value / plain_rate, 2),
         "rdo_err_vs_plain": max_err,
         "iters_mean": float(iters.mean()), "iters_max": float(iters.max()), "relres_max": float(relres.max()),
-        "failed_points": len(failed), "shape_ms": shape_ms,
+        "failed_points": len(failed),
     }
+    if shape_ms:
+        line["shape_ms"] = shape_ms
     for i, f in failed.items():
         print(f"sweep: point {i} not converged: {f}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)

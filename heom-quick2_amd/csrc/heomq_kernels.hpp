@@ -164,6 +164,8 @@ struct DevPacked {
   // of at most 16 elements; ADO 0 keeps its own workgroup for the tier-0
   // fix-up)
   int heavy_wave = 0;
+  // multi-shape launches: reals before this shape's first problem in x and y
+  int64_t xoff = 0;
 };
 
 // Workgroups of the heavy part of a packed apply launch.
@@ -172,6 +174,27 @@ __host__ __device__ inline int64_t heavy_blocks(const DevPacked& op) {
   return op.n_heavy > 0 ? 1 + (op.n_heavy - 1 + 3) / 4 : 0;
 }
 constexpr int kHeavyWaveMaxNnz = 16;
+
+// Forward applies of problems of up to kApplyShapesPerLaunch shapes in one
+// launch.  The shapes' operators travel in the kernel argument itself (so
+// their table pointers are known to be global memory); shape g's blocks are
+// bstart[g] .. bstart[g + 1] - 1 of the flat grid, nb[g] = apply_blocks(ops[g])
+// per covered problem, the j-th covered problem being pmap[g][j] (j when
+// null), and its vectors start ops[g].xoff reals into x and y.
+constexpr int kApplyShapesPerLaunch = 10;
+struct ApplyMulti {
+  DevPacked ops[kApplyShapesPerLaunch];
+  const int32_t* pmap[kApplyShapesPerLaunch] = {};
+  int32_t bstart[kApplyShapesPerLaunch + 1] = {};
+  int32_t nb[kApplyShapesPerLaunch] = {};
+  int ng = 0;
+};
+static_assert(sizeof(ApplyMulti) <= 4000, "the multi-shape apply's kernel argument must stay under 4 KB");
+// workgroups of one problem's packed apply
+int64_t apply_blocks(const DevPacked& op);
+// every shape needs a heavy part (top tier >= 1), whose block 0 does the
+// tier-0 fix-up, and its trace slot
+hipError_t launch_apply_packed_multi_f64(const ApplyMulti& am, const double* x, double* y, bool rdo, hipStream_t s);
 
 // Projected apply on packed vectors: y = Pi A x with the trace row on the
 // tier-0 block (rdo = true) or none, tier-0 output Hermitianised (herm0).

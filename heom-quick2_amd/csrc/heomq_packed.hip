@@ -714,33 +714,54 @@ __device__ inline void light_sector(const DevPacked& op, const Coefs cf_p, const
 constexpr size_t kApplySmem1 = kGSmem > kLSmem ? kGSmem : kLSmem;
 constexpr size_t kApplySmem = kApplySmem0 > kApplySmem1 ? kApplySmem0 : kApplySmem1;
 
+// Block bx of problem p of the apply (the kernels below supply the two).
 template <typename T, bool TRANS, int U>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
-k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
-  __shared__ __align__(16) char smem[kApplySmem];
-  const int64_t p = blockIdx.y;   // problem of a batch
+__device__ inline void apply_block(const DevPacked& op, int64_t p, int64_t bx, const T* __restrict__ x,
+                                   T* __restrict__ y, int64_t zero_at, int rdo, char* smem) {
   if (op.frozen_flag && op.frozen_flag[p * op.frozen_stride] == 3) return;   // finished: block-uniform, no barrier yet
   const bool scaled = op.precond && !(op.frozen_flag && op.frozen_flag[p * op.frozen_stride + 2]);
   x += 2 * p * op.vstride;
   y += 2 * p * op.vstride;
   const Coefs cf{op.rec + p * op.rec_stride, op.gam + 2 * p * op.gam_stride, op.hc_ell + 2 * p * op.hc_stride};
   const int64_t hb = heavy_blocks(op);
-  if ((int64_t)blockIdx.x < hb) {   // block-uniform
-    if (op.heavy_wave && blockIdx.x > 0) {   // four ADOs of small sectors, a wave each (no barrier)
-      const int64_t a = 1 + ((int64_t)blockIdx.x - 1) * 4 + (threadIdx.x >> 6);
+  if (bx < hb) {   // block-uniform
+    if (op.heavy_wave && bx > 0) {   // four ADOs of small sectors, a wave each (no barrier)
+      const int64_t a = 1 + (bx - 1) * 4 + (threadIdx.x >> 6);
       if (a < op.n_heavy) heavy_wave<T, TRANS>(op, cf, x, y, zero_at, a, scaled);
       return;
     }
-    const int a = (int)blockIdx.x;
+    const int a = (int)bx;
     if (op.hg_off && op.hg_off[a + 1] > op.hg_off[a])
       heavy_grouped<T, TRANS>(op, cf, x, y, zero_at, rdo, a, smem, scaled);
     else
       heavy_body<T, TRANS>(op, cf, x, y, zero_at, rdo, a, smem, scaled);
   } else if (op.lt_blk) {
-    light_sector<T, TRANS>(op, cf, x, y, zero_at, (int)(blockIdx.x - hb), smem, scaled);
+    light_sector<T, TRANS>(op, cf, x, y, zero_at, (int)(bx - hb), smem, scaled);
   } else {
-    light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(blockIdx.x - hb), smem, scaled);
+    light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(bx - hb), smem, scaled);
   }
+}
+
+template <typename T, bool TRANS, int U>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
+  __shared__ __align__(16) char smem[kApplySmem];
+  apply_block<T, TRANS, U>(op, blockIdx.y, blockIdx.x, x, y, zero_at, rdo, smem);   // grid y: problem of a batch
+}
+
+// Forward applies of problems of several shapes in one flat grid (the shapes
+// come largest first, so their long heavy workgroups are dispatched first).
+template <typename T, int U>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+k_apply_packed_multi(const ApplyMulti am, const T* __restrict__ x, T* __restrict__ y, int rdo) {
+  __shared__ __align__(16) char smem[kApplySmem];
+  const int64_t b = blockIdx.x;
+  int g = 0;
+  while (g + 1 < am.ng && am.bstart[g + 1] <= b) ++g;   // block-uniform
+  const int64_t loc = b - am.bstart[g], j = loc / am.nb[g];
+  const int64_t p = am.pmap[g] ? am.pmap[g][j] : j;
+  const DevPacked& op = am.ops[g];
+  apply_block<T, false, U>(op, p, loc - j * am.nb[g], x + op.xoff, y + op.xoff, -1, rdo, smem);
 }
 
 // Tier-0 fix-ups (one block): trace row (forward: y(d,d) = tr x0; transposed:
@@ -1231,8 +1252,7 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
   if (rdo && op.trace_slot < 0) return hipErrorInvalidValue;
   if (op.total <= 0) return hipSuccess;
   const int64_t zero_at = (op.transposed && rdo) ? op.trace_slot : -1;   // ADO 0 starts at packed offset 0
-  const int64_t light = op.total - op.light_begin;
-  const int64_t blocks = heavy_blocks(op) + (op.lt_blk ? op.n_lt_blocks : (light > 0 ? (light + 255) / 256 : 0));
+  const int64_t blocks = apply_blocks(op);
   if (blocks > 0) {
     const dim3 grid((unsigned)blocks, (unsigned)op.nprob);
     const int u = op.max_light_couplings;
@@ -1258,6 +1278,18 @@ static hipError_t launch_p(const DevPacked& op, const T* x, T* y, bool rdo, hipS
 
 hipError_t launch_apply_packed_f64(const DevPacked& op, const double* x, double* y, bool rdo, hipStream_t s) {
   return launch_p<double>(op, x, y, rdo, s);
+}
+
+int64_t apply_blocks(const DevPacked& op) {
+  const int64_t light = op.total - op.light_begin;
+  return heavy_blocks(op) + (op.lt_blk ? op.n_lt_blocks : (light > 0 ? (light + 255) / 256 : 0));
+}
+
+hipError_t launch_apply_packed_multi_f64(const ApplyMulti& am, const double* x, double* y, bool rdo, hipStream_t s) {
+  if (am.ng < 1 || am.ng > kApplyShapesPerLaunch || am.bstart[am.ng] <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_apply_packed_multi<double, 2>), dim3((unsigned)am.bstart[am.ng]), dim3(256), 0, s, am, x, y,
+                     rdo ? 1 : 0);
+  return hipGetLastError();
 }
 hipError_t launch_apply_packed_f32(const DevPacked& op, const float* x, float* y, bool rdo, hipStream_t s) {
   return launch_p<float>(op, x, y, rdo, s);

@@ -47,10 +47,68 @@ template <> struct W2<float> { using type = float2; };
 #define GRID_LOOP(i, n) \
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
-// Batched launches: blockIdx.y is the problem; its vectors start vs reals
-// after the previous problem's, its scalars and partial sums are its own.
-#define BATCH_PARTIAL() (partial += 2 * (int64_t)blockIdx.y * gridDim.x)
-#define BATCH_OFF() ((int64_t)blockIdx.y * vs)
+// Where a block of a TFQMR vector kernel works: its problem q (the index of
+// the problem's scalars), the problem's first real in every vector, its
+// complex-pair count, the block's rank among the problem's blocks, and the
+// slot of the block's partial sums.
+struct Span {
+  int64_t q, off, n2, blk, nblk, pidx;
+};
+
+// Problems of one size: blockIdx.y is the problem, its vectors start vs reals
+// after the previous problem's; gridDim.x blocks per problem.
+struct UniLoc {
+  int64_t n2, vs;
+  __device__ Span span() const {
+    const int64_t q = blockIdx.y;
+    return Span{q, q * vs, n2, (int64_t)blockIdx.x, (int64_t)gridDim.x, q * gridDim.x + blockIdx.x};
+  }
+  // finalize block q: problem q and its partial sums
+  __device__ int64_t fin(int64_t q, int64_t nblk_uni, int64_t& pbase, int64_t& nblk) const {
+    pbase = q * nblk_uni;
+    nblk = nblk_uni;
+    return q;
+  }
+};
+
+// Problems of several sizes in one flat grid (the shape loop is over at most
+// kMaxShapes entries of the kernel argument, block-uniform).
+struct MultiLoc {
+  Multi m;
+  __device__ int shape_of_block(int64_t b) const {
+    int g = 0;
+    while (g + 1 < m.ng && m.bstart[g + 1] <= b) ++g;
+    return g;
+  }
+  __device__ Span span() const {
+    const int64_t b = blockIdx.x;
+    const int g = shape_of_block(b);
+    const int64_t loc = b - m.bstart[g], j = loc / m.nb[g];
+    const int64_t p = m.pmap[g] ? m.pmap[g][j] : j;
+    return Span{m.q0[g] + p, m.voff[g] + 2 * p * m.n2[g], m.n2[g], loc - j * m.nb[g], m.nb[g], b};
+  }
+  // finalize block f: the covered problem's global index and partial sums
+  __device__ int64_t fin(int64_t f, int64_t, int64_t& pbase, int64_t& nblk) const {
+    int g = 0;
+    while (g + 1 < m.ng && m.qstart[g + 1] <= f) ++g;
+    const int64_t j = f - m.qstart[g];
+    pbase = m.bstart[g] + j * m.nb[g];
+    nblk = m.nb[g];
+    return m.q0[g] + (m.pmap[g] ? m.pmap[g][j] : j);
+  }
+};
+
+#define SPAN_LOOP(i, sp) \
+  for (int64_t i = (sp).blk * blockDim.x + threadIdx.x; i < (sp).n2; i += (sp).nblk * blockDim.x)
+
+// Phase 1 of a span's reduction: the block's (a, b) partial sums in its slot.
+__device__ inline void span_partial(double a, double b, double* partial, const Span& sp) {
+  block_sum2(a, b);
+  if (threadIdx.x == 0) {
+    partial[2 * sp.pidx] = a;
+    partial[2 * sp.pidx + 1] = b;
+  }
+}
 
 // Shadow vector of TFQMR, generated where it is used instead of stored: a
 // +-1 sign per real element from a splitmix64 hash of its index (the same
@@ -64,17 +122,17 @@ __device__ inline double shadow(int64_t k, uint64_t seed) {
 }
 
 // r = b - Ax (stored in w and y1), d = 0; partials of ||r||^2 and rt.r.
-template <typename T>
-__global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restrict__ ax, T* __restrict__ w,
-                         T* __restrict__ y1, T* __restrict__ d, const TfqmrScal* s, double* partial, int64_t vs) {
+template <typename T, typename Loc>
+__global__ void k_init_r(const Loc loc, const T* __restrict__ b, const T* __restrict__ ax, T* __restrict__ w,
+                         T* __restrict__ y1, T* __restrict__ d, const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
-  s += blockIdx.y;
-  BATCH_PARTIAL();
+  const Span sp = loc.span();
+  s += sp.q;
   if (s->flag != kInitRequested) return;   // partials of this problem are not read
   const uint64_t seed = s->seed;
-  { const int64_t o = BATCH_OFF(); b += o; ax += o; w += o; y1 += o; d += o; }
+  { const int64_t o = sp.off; b += o; ax += o; w += o; y1 += o; d += o; }
   double rr = 0, rrt = 0;
-  GRID_LOOP(i, n2) {
+  SPAN_LOOP(i, sp) {
     const V bb = reinterpret_cast<const V*>(b)[i], aa = reinterpret_cast<const V*>(ax)[i];
     V r;
     r.x = bb.x - aa.x;
@@ -88,43 +146,43 @@ __global__ void k_init_r(int64_t n2, const T* __restrict__ b, const T* __restric
     rr += (double)r.x * r.x + (double)r.y * r.y;
     rrt += shadow(2 * i, seed) * r.x + shadow(2 * i + 1, seed) * r.y;
   }
-  block_partial(rr, rrt, partial);
+  span_partial(rr, rrt, partial, sp);
 }
 
 // v = u1; partial of rt.v
-template <typename T>
-__global__ void k_init_v(int64_t n2, const T* __restrict__ u1, T* __restrict__ v, T* __restrict__ u1_copy,
-                         const TfqmrScal* s, double* partial, int64_t vs) {
+template <typename T, typename Loc>
+__global__ void k_init_v(const Loc loc, const T* __restrict__ u1, T* __restrict__ v, T* __restrict__ u1_copy,
+                         const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
-  s += blockIdx.y;
-  BATCH_PARTIAL();
+  const Span sp = loc.span();
+  s += sp.q;
   if (s->flag != kInitialising) return;
   const uint64_t seed = s->seed;
-  { const int64_t o = BATCH_OFF(); u1 += o; v += o; if (u1_copy) u1_copy += o; }
+  { const int64_t o = sp.off; u1 += o; v += o; if (u1_copy) u1_copy += o; }
   double sg = 0;
-  GRID_LOOP(i, n2) {
+  SPAN_LOOP(i, sp) {
     const V t = reinterpret_cast<const V*>(u1)[i];
     reinterpret_cast<V*>(v)[i] = t;
     if (u1_copy) reinterpret_cast<V*>(u1_copy)[i] = t;
     sg += shadow(2 * i, seed) * t.x + shadow(2 * i + 1, seed) * t.y;
   }
-  block_partial(sg, 0.0, partial);
+  span_partial(sg, 0.0, partial, sp);
 }
 
 // First half of an iteration, fused with the next search direction:
 // y2 = y1 - alpha v;  w -= alpha u1;  d = y1 + dcoef d;  partial ||w||^2.
-template <typename T>
-__global__ void k_step1(int64_t n2, const T* __restrict__ y1, const T* __restrict__ v, T* __restrict__ y2,
+template <typename T, typename Loc>
+__global__ void k_step1(const Loc loc, const T* __restrict__ y1, const T* __restrict__ v, T* __restrict__ y2,
                         T* __restrict__ w, const T* __restrict__ u1, T* __restrict__ d, const TfqmrScal* s,
-                        double* partial, int64_t vs) {
+                        double* partial) {
   using V = typename W2<T>::type;
-  s += blockIdx.y;
-  BATCH_PARTIAL();
+  const Span sp = loc.span();
+  s += sp.q;
   if (s->flag) return;
-  { const int64_t o = BATCH_OFF(); y1 += o; v += o; y2 += o; w += o; u1 += o; d += o; }
+  { const int64_t o = sp.off; y1 += o; v += o; y2 += o; w += o; u1 += o; d += o; }
   const T alpha = (T)s->alpha, dcoef = (T)s->dcoef;
   double ww = 0;
-  GRID_LOOP(i, n2) {
+  SPAN_LOOP(i, sp) {
     const V a = reinterpret_cast<const V*>(y1)[i], b = reinterpret_cast<const V*>(v)[i];
     V o;
     o.x = a.x - alpha * b.x;
@@ -142,24 +200,23 @@ __global__ void k_step1(int64_t n2, const T* __restrict__ y1, const T* __restric
     reinterpret_cast<V*>(d)[i] = dn;
     ww += (double)wi.x * wi.x + (double)wi.y * wi.y;
   }
-  block_partial(ww, 0.0, partial);
+  span_partial(ww, 0.0, partial, sp);
 }
 
 // Second half: x += eta d;  w -= alpha u2;  d = y2 + dcoef d;  partials of
 // ||w||^2 and rt.w (w is final for this iteration).
-template <typename T>
-__global__ void k_step2(int64_t n2, T* __restrict__ x, T* __restrict__ w, const T* __restrict__ u2,
-                        T* __restrict__ d, const T* __restrict__ y2, const TfqmrScal* s, double* partial,
-                        int64_t vs) {
+template <typename T, typename Loc>
+__global__ void k_step2(const Loc loc, T* __restrict__ x, T* __restrict__ w, const T* __restrict__ u2,
+                        T* __restrict__ d, const T* __restrict__ y2, const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
-  s += blockIdx.y;
-  BATCH_PARTIAL();
+  const Span sp = loc.span();
+  s += sp.q;
   if (s->flag) return;
   const uint64_t seed = s->seed;
-  { const int64_t o = BATCH_OFF(); x += o; w += o; u2 += o; d += o; y2 += o; }
+  { const int64_t o = sp.off; x += o; w += o; u2 += o; d += o; y2 += o; }
   const T alpha = (T)s->alpha, dcoef = (T)s->dcoef, eta = (T)s->eta;
   double ww = 0, rw = 0;
-  GRID_LOOP(i, n2) {
+  SPAN_LOOP(i, sp) {
     const V di = reinterpret_cast<const V*>(d)[i];
     V xi = reinterpret_cast<V*>(x)[i];
     xi.x += eta * di.x;
@@ -178,20 +235,21 @@ __global__ void k_step2(int64_t n2, T* __restrict__ x, T* __restrict__ w, const 
     ww += (double)wi.x * wi.x + (double)wi.y * wi.y;
     rw += shadow(2 * i, seed) * wi.x + shadow(2 * i + 1, seed) * wi.y;
   }
-  block_partial(ww, rw, partial);
+  span_partial(ww, rw, partial, sp);
 }
 
 // x += eta d;  y1 = w + beta y2 (not on the iteration that met the bound).
-template <typename T>
-__global__ void k_step3(int64_t n2, T* __restrict__ x, const T* __restrict__ d, const T* __restrict__ w,
-                        const T* __restrict__ y2, T* __restrict__ y1, const TfqmrScal* s, int64_t vs) {
+template <typename T, typename Loc>
+__global__ void k_step3(const Loc loc, T* __restrict__ x, const T* __restrict__ d, const T* __restrict__ w,
+                        const T* __restrict__ y2, T* __restrict__ y1, const TfqmrScal* s) {
   using V = typename W2<T>::type;
-  s += blockIdx.y;
+  const Span sp = loc.span();
+  s += sp.q;
   if (s->flag) return;
-  { const int64_t o = BATCH_OFF(); x += o; d += o; w += o; y2 += o; y1 += o; }
+  { const int64_t o = sp.off; x += o; d += o; w += o; y2 += o; y1 += o; }
   const T eta = (T)s->eta, beta = (T)s->beta;
   const bool next = s->stop == 0;
-  GRID_LOOP(i, n2) {
+  SPAN_LOOP(i, sp) {
     V xi = reinterpret_cast<V*>(x)[i];
     const V di = reinterpret_cast<const V*>(d)[i];
     xi.x += eta * di.x;
@@ -208,18 +266,18 @@ __global__ void k_step3(int64_t n2, T* __restrict__ x, const T* __restrict__ d, 
 }
 
 // v = u1 + beta (u2 + beta v); partial of rt.v
-template <typename T>
-__global__ void k_v(int64_t n2, const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ v,
-                    const TfqmrScal* s, double* partial, int64_t vs) {
+template <typename T, typename Loc>
+__global__ void k_v(const Loc loc, const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ v,
+                    const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
-  s += blockIdx.y;
-  BATCH_PARTIAL();
+  const Span sp = loc.span();
+  s += sp.q;
   if (s->flag || s->stop) return;
   const uint64_t seed = s->seed;
-  { const int64_t o = BATCH_OFF(); u1 += o; u2 += o; v += o; }
+  { const int64_t o = sp.off; u1 += o; u2 += o; v += o; }
   const T beta = (T)s->beta;
   double sg = 0;
-  GRID_LOOP(i, n2) {
+  SPAN_LOOP(i, sp) {
     const V a = reinterpret_cast<const V*>(u1)[i], b = reinterpret_cast<const V*>(u2)[i];
     const V c = reinterpret_cast<const V*>(v)[i];
     V t;
@@ -228,23 +286,22 @@ __global__ void k_v(int64_t n2, const T* __restrict__ u1, const T* __restrict__ 
     reinterpret_cast<V*>(v)[i] = t;
     sg += shadow(2 * i, seed) * t.x + shadow(2 * i + 1, seed) * t.y;
   }
-  block_partial(sg, 0.0, partial);
+  span_partial(sg, 0.0, partial, sp);
 }
 
 // partials of ||b - ax||^2 and ||b||^2 (FP64)
-template <typename TB, typename TA>
-__global__ void k_resid(int64_t n2, const TB* __restrict__ b, const TA* __restrict__ ax, double* partial,
-                        int64_t vs) {
-  BATCH_PARTIAL();
-  { const int64_t o = BATCH_OFF(); b += o; ax += o; }
+template <typename TB, typename TA, typename Loc>
+__global__ void k_resid(const Loc loc, const TB* __restrict__ b, const TA* __restrict__ ax, double* partial) {
+  const Span sp = loc.span();
+  { const int64_t o = sp.off; b += o; ax += o; }
   double rr = 0, bb = 0;
-  GRID_LOOP(i, n2) {
+  SPAN_LOOP(i, sp) {
     const double b0 = (double)b[2 * i], b1 = (double)b[2 * i + 1];
     const double r0 = b0 - (double)ax[2 * i], r1 = b1 - (double)ax[2 * i + 1];
     rr += r0 * r0 + r1 * r1;
     bb += b0 * b0 + b1 * b1;
   }
-  block_partial(rr, bb, partial);
+  span_partial(rr, bb, partial, sp);
 }
 
 // Phase 2: one block folds the partials in a fixed order and advances the
@@ -261,16 +318,22 @@ __device__ inline void half_step(TfqmrScal* s, double a) {
   s->dcoef = theta * theta * s->eta / s->alpha;
 }
 
-// Block p finalizes problem p of a batch (one block for a single problem).
-__global__ void k_finalize(int op, int nblk, const double* __restrict__ partial, TfqmrScal* s, double* out) {
-  partial += 2 * (int64_t)blockIdx.x * nblk;
-  if (out) out += 2 * blockIdx.x;
-  if (s) s += blockIdx.x;
+// Block q finalizes problem q (one block for a single problem; the covered
+// problems of a multi-size launch in order); nblk_uni is the blocks per
+// problem of a one-size launch.
+template <typename Loc>
+__global__ void k_finalize(const Loc loc, int op, int nblk_uni, const double* __restrict__ partial, TfqmrScal* s,
+                           double* out) {
+  int64_t pbase = 0, nblk = 0;
+  const int64_t q = loc.fin(blockIdx.x, nblk_uni, pbase, nblk);
+  partial += 2 * pbase;
+  if (out) out += 2 * q;
+  if (s) s += q;
   if (op == F_INIT_R && s->flag != kInitRequested) return;
   if (op == F_INIT_V && s->flag != kInitialising) return;
   if (op != F_INIT_R && op != F_INIT_V && op != F_RESID && s->flag) return;   // s unused for F_RESID
   double a = 0, b = 0;
-  for (int t = threadIdx.x; t < nblk; t += blockDim.x) {
+  for (int64_t t = threadIdx.x; t < nblk; t += blockDim.x) {
     a += partial[2 * t];
     b += partial[2 * t + 1];
   }
@@ -460,12 +523,14 @@ static int grid_for(int64_t n) {
 }
 
 #define L1(kern, n, ...) hipLaunchKernelGGL(kern, dim3(grid_for<T>(n)), dim3(kRedThreads), 0, st, n, __VA_ARGS__)
-#define LB(kern, n, ...) \
-  hipLaunchKernelGGL(kern, dim3(grid_for<T>(n), bt.nprob), dim3(kRedThreads), 0, st, n, __VA_ARGS__)
+#define LB(kern, n, ...)                                                                        \
+  hipLaunchKernelGGL(kern, dim3(grid_for<T>(n), bt.nprob), dim3(kRedThreads), 0, st, UniLoc{n, bt.vs}, \
+                     __VA_ARGS__)
 
 static hipError_t finalize(hipStream_t st, int op, int nblk, const double* partial, TfqmrScal* s, double* out,
                            int nprob = 1) {
-  hipLaunchKernelGGL(k_finalize, dim3(nprob), dim3(kRedThreads), 0, st, op, nblk, partial, s, out);
+  hipLaunchKernelGGL(k_finalize<UniLoc>, dim3(nprob), dim3(kRedThreads), 0, st, UniLoc{0, 0}, op, nblk, partial,
+                     s, out);
   return hipGetLastError();
 }
 
@@ -474,7 +539,7 @@ template <typename T>
 hipError_t tfq_init_r(hipStream_t st, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, TfqmrScal* s,
                       double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_init_r<T>), n2, b, ax, w, y1, d, s, partial, bt.vs);
+  LB((k_init_r<T, UniLoc>), n2, b, ax, w, y1, d, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_INIT_R, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
@@ -482,7 +547,7 @@ template <typename T>
 hipError_t tfq_init_v(hipStream_t st, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial, Batch bt,
                       T* u1_copy) {
   const int64_t n2 = n / 2;
-  LB((k_init_v<T>), n2, u1, v, u1_copy, s, partial, bt.vs);
+  LB((k_init_v<T, UniLoc>), n2, u1, v, u1_copy, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_INIT_V, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
@@ -490,7 +555,7 @@ template <typename T>
 hipError_t tfq_step1(hipStream_t st, int64_t n, const T* y1, const T* v, T* y2, T* w, const T* u1, T* d,
                      TfqmrScal* s, double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_step1<T>), n2, y1, v, y2, w, u1, d, s, partial, bt.vs);
+  LB((k_step1<T, UniLoc>), n2, y1, v, y2, w, u1, d, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_HALF, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
@@ -498,7 +563,7 @@ template <typename T>
 hipError_t tfq_step2(hipStream_t st, int64_t n, T* x, T* w, const T* u2, T* d, const T* y2, TfqmrScal* s,
                      double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_step2<T>), n2, x, w, u2, d, y2, s, partial, bt.vs);
+  LB((k_step2<T, UniLoc>), n2, x, w, u2, d, y2, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_HALF_RHO, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
@@ -506,14 +571,14 @@ template <typename T>
 hipError_t tfq_step3(hipStream_t st, int64_t n, T* x, const T* d, const T* w, const T* y2, T* y1,
                      const TfqmrScal* s, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_step3<T>), n2, x, d, w, y2, y1, s, bt.vs);
+  LB((k_step3<T, UniLoc>), n2, x, d, w, y2, y1, s);
   return hipGetLastError();
 }
 template <typename T>
 hipError_t tfq_v(hipStream_t st, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial,
                  Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_v<T>), n2, u1, u2, v, s, partial, bt.vs);
+  LB((k_v<T, UniLoc>), n2, u1, u2, v, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_V, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
@@ -531,7 +596,7 @@ template <typename TB, typename TA>
 hipError_t vec_resid(hipStream_t st, int64_t n, const TB* b, const TA* ax, double* out, double* partial, Batch bt) {
   using T = TB;
   const int64_t n2 = n / 2;
-  LB((k_resid<TB, TA>), n2, b, ax, partial, bt.vs);
+  LB((k_resid<TB, TA, UniLoc>), n2, b, ax, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_RESID, grid_for<T>(n2), partial, nullptr, out, bt.nprob);
 }
@@ -587,6 +652,58 @@ hipError_t vec_lincomb3(hipStream_t st, int64_t n, double* out, double a, const 
   using T = double;
   L1(k_lincomb3, n, out, a, x, b, y, c, z);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Multi-size launches: one flat grid over every problem of every shape
+// ---------------------------------------------------------------------------
+int vec_blocks(int64_t n2) { return grid_for<double>(n2); }
+
+#define LM(kern, ...) \
+  hipLaunchKernelGGL(kern, dim3(m.bstart[m.ng]), dim3(kRedThreads), 0, st, MultiLoc{m}, __VA_ARGS__)
+
+static hipError_t finalize_m(hipStream_t st, const Multi& m, int op, const double* partial, TfqmrScal* s,
+                             double* out) {
+  if (hipError_t e = hipGetLastError()) return e;
+  hipLaunchKernelGGL(k_finalize<MultiLoc>, dim3(m.qstart[m.ng]), dim3(kRedThreads), 0, st, MultiLoc{m}, op, 0,
+                     partial, s, out);
+  return hipGetLastError();
+}
+
+hipError_t tfq_init_r(hipStream_t st, const Multi& m, const double* b, const double* ax, double* w, double* y1,
+                      double* d, TfqmrScal* s, double* partial) {
+  LM((k_init_r<double, MultiLoc>), b, ax, w, y1, d, s, partial);
+  return finalize_m(st, m, F_INIT_R, partial, s, nullptr);
+}
+hipError_t tfq_init_v(hipStream_t st, const Multi& m, const double* u1, double* v, TfqmrScal* s, double* partial,
+                      double* u1_copy) {
+  LM((k_init_v<double, MultiLoc>), u1, v, u1_copy, s, partial);
+  return finalize_m(st, m, F_INIT_V, partial, s, nullptr);
+}
+hipError_t tfq_step1(hipStream_t st, const Multi& m, const double* y1, const double* v, double* y2, double* w,
+                     const double* u1, double* d, TfqmrScal* s, double* partial) {
+  LM((k_step1<double, MultiLoc>), y1, v, y2, w, u1, d, s, partial);
+  return finalize_m(st, m, F_HALF, partial, s, nullptr);
+}
+hipError_t tfq_step2(hipStream_t st, const Multi& m, double* x, double* w, const double* u2, double* d,
+                     const double* y2, TfqmrScal* s, double* partial) {
+  LM((k_step2<double, MultiLoc>), x, w, u2, d, y2, s, partial);
+  return finalize_m(st, m, F_HALF_RHO, partial, s, nullptr);
+}
+hipError_t tfq_step3(hipStream_t st, const Multi& m, double* x, const double* d, const double* w, const double* y2,
+                     double* y1, const TfqmrScal* s) {
+  LM((k_step3<double, MultiLoc>), x, d, w, y2, y1, s);
+  return hipGetLastError();
+}
+hipError_t tfq_v(hipStream_t st, const Multi& m, const double* u1, const double* u2, double* v, TfqmrScal* s,
+                 double* partial) {
+  LM((k_v<double, MultiLoc>), u1, u2, v, s, partial);
+  return finalize_m(st, m, F_V, partial, s, nullptr);
+}
+hipError_t vec_resid(hipStream_t st, const Multi& m, const double* b, const double* ax, double* out,
+                     double* partial) {
+  LM((k_resid<double, double, MultiLoc>), b, ax, partial);
+  return finalize_m(st, m, F_RESID, partial, nullptr, out);
 }
 
 #define INST(T)                                                                                          \

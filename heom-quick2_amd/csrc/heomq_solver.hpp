@@ -22,6 +22,30 @@ struct Batch {
   int64_t vs = 0;
 };
 
+// Problems of several sizes advanced by one launch of each vector kernel.
+// Shape g < ng holds global problems q0[g] + p, each n2[g] complex pairs long
+// and stored one after another (problem p at real voff[g] + 2 p n2[g] of every
+// vector).  The launch covers qstart[g + 1] - qstart[g] of them: p =
+// pmap[g][j] for j = 0, 1, ... (p = j when pmap[g] is null), with blocks
+// bstart[g] .. bstart[g + 1] - 1 of the flat grid, nb[g] = vec_blocks(n2[g])
+// per problem; finalize block f takes the f-th covered problem.
+constexpr int kMaxShapes = 16;
+struct Multi {
+  int ng = 0;
+  int32_t bstart[kMaxShapes + 1] = {};
+  int32_t nb[kMaxShapes] = {};
+  int32_t qstart[kMaxShapes + 1] = {};
+  int32_t q0[kMaxShapes] = {};
+  int64_t n2[kMaxShapes] = {};
+  int64_t voff[kMaxShapes] = {};
+  const int32_t* pmap[kMaxShapes] = {};
+};
+
+// Blocks per problem of a vector kernel over n2 complex pairs.  The flat
+// multi-size launch gives each problem the same count, so its reductions fold
+// the same partial sums in the same order as a launch over its size alone.
+int vec_blocks(int64_t n2);
+
 // TFQMR scalars, resident on the device (Freund 1993 notation).
 struct TfqmrScal {
   double rho, sigma, alpha, beta;
@@ -94,6 +118,21 @@ hipError_t tfq_freeze(hipStream_t, TfqmrScal* s, const int32_t* mask, int nprob)
 hipError_t tfq_set_unscaled(hipStream_t, TfqmrScal* s, const int32_t* mask, int nprob);
 // target *= sqrt(rr / rr_true[2p]) for problems in kInitialising (scaled iterations)
 hipError_t tfq_scale_targets(hipStream_t, TfqmrScal* s, const double* rr_true, int nprob);
+// The same steps over every problem of a Multi layout (FP64; partial holds
+// two doubles per flat-grid block, out two per problem).
+hipError_t tfq_init_r(hipStream_t, const Multi&, const double* b, const double* ax, double* w, double* y1, double* d,
+                      TfqmrScal* s, double* partial);
+hipError_t tfq_init_v(hipStream_t, const Multi&, const double* u1, double* v, TfqmrScal* s, double* partial,
+                      double* u1_copy);
+hipError_t tfq_step1(hipStream_t, const Multi&, const double* y1, const double* v, double* y2, double* w,
+                     const double* u1, double* d, TfqmrScal* s, double* partial);
+hipError_t tfq_step2(hipStream_t, const Multi&, double* x, double* w, const double* u2, double* d, const double* y2,
+                     TfqmrScal* s, double* partial);
+hipError_t tfq_step3(hipStream_t, const Multi&, double* x, const double* d, const double* w, const double* y2,
+                     double* y1, const TfqmrScal* s);
+hipError_t tfq_v(hipStream_t, const Multi&, const double* u1, const double* u2, double* v, TfqmrScal* s,
+                 double* partial);
+hipError_t vec_resid(hipStream_t, const Multi&, const double* b, const double* ax, double* out, double* partial);
 hipError_t vec_set_elem(hipStream_t, double* v, int64_t off, double val);
 // A few scattered values written into every problem's copy of a vector.
 struct SetPattern {

@@ -143,6 +143,78 @@ class Batch:
         return rdo, cur, occ, info
 
 
+class Sweep:
+    """Problems of several hierarchy shapes (Pade terms, top tier) solved
+    together on one GPU, one launch per solver step over all of them.
+
+    Problems must share norbs, nalf and d; up to 16 shapes.  Outputs come in
+    the order of ``params_list``.
+    """
+
+    def __init__(self, params_list, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("heomq.Sweep needs a GPU (HIP device); none is visible")
+        self.torch = torch
+        self.params = list(params_list)
+        arr = (_lib.CParams * len(self.params))(*[p.to_c() for p in self.params])
+        self._h = C.c_void_p()
+        torch.cuda.set_device(device)
+        check(lib().heomq_sweep_create(arr, len(self.params), device, C.byref(self._h)))
+        info = np.zeros(8, np.int64)
+        check(lib().heomq_sweep_info(self._h, info.ctypes.data_as(_lib.I64)))
+        self.nprob, self.nshapes, self.packed, self.d, self.apply_blocks, self.vector_blocks = (int(v) for v in info[:6])
+        P = self.nprob
+        self.shape = np.zeros(P, np.int32)
+        self.offset = np.zeros(P, np.int64)
+        self.length = np.zeros(P, np.int64)
+        check(lib().heomq_sweep_problems(self._h, self.shape.ctypes.data_as(_lib.I32),
+                                         self.offset.ctypes.data_as(_lib.I64), self.length.ctypes.data_as(_lib.I64)))
+        self.device = device
+        _LIVE_BATCH.add(self)
+
+    def close(self):
+        if self._h:
+            self.torch.cuda.synchronize(self.device)
+            lib().heomq_sweep_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply_packed(self, xp, yp):
+        """y = Pi A x for every problem on flat-layout complex128 device vectors
+        (problem p at ``offset[p]``, ``length[p]`` elements), trace row on tier 0."""
+        t = self.torch
+        for v in (xp, yp):
+            if v.dtype != t.complex128 or not v.is_cuda or not v.is_contiguous() or v.numel() != self.packed:
+                raise HeomqError(_lib.BAD_DIM, "expected flat complex128 device vectors of the sweep's size")
+        check(lib().heomq_sweep_apply_packed(self._h, C.c_void_p(xp.data_ptr()), C.c_void_p(yp.data_ptr()), TRACE_RDO,
+                                             C.c_void_p(t.cuda.current_stream(self.device).cuda_stream)))
+        return yp
+
+    def solve(self, tol=1e-10, maxiter=20000, raise_on_fail=True):
+        """Steady states of all problems, as Batch.solve."""
+        P, d = self.nprob, self.d
+        nalf = self.params[0].nalf
+        norbs = self.params[0].norbs
+        rdo = np.zeros((P, d, d), np.complex128)
+        cur = np.zeros((P, nalf, 2))
+        occ = np.zeros((P, norbs, 2))
+        stats = np.zeros((P, 4))
+        rc = lib().heomq_sweep_solve(self._h, tol, maxiter, _dp(rdo.view(np.float64)), _dp(cur), _dp(occ), _dp(stats),
+                                     C.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream))
+        if rc != _lib.OK and raise_on_fail:
+            check(rc)
+        rdo = np.ascontiguousarray(rdo.transpose(0, 2, 1))
+        info = dict(iters=stats[:, 0].astype(int), relres=stats[:, 1], status=stats[:, 2].astype(int),
+                    restarts=stats[:, 3].astype(int), rc=rc)
+        return rdo, cur, occ, info
+
+
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(_lib.D)
 

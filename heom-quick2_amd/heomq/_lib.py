@@ -109,6 +109,12 @@ SIGNATURES = {
     "heomq_batch_info": (C.c_int, [P, I64]),
     "heomq_batch_apply_packed": (C.c_int, [P, P, P, C.c_int, P]),
     "heomq_batch_solve": (C.c_int, [P, C.c_double, C.c_int, D, D, D, D, P]),
+    "heomq_sweep_create": (C.c_int, [C.POINTER(CParams), C.c_int, C.c_int, C.POINTER(P)]),
+    "heomq_sweep_free": (None, [P]),
+    "heomq_sweep_info": (C.c_int, [P, I64]),
+    "heomq_sweep_problems": (C.c_int, [P, I32, I64, I64]),
+    "heomq_sweep_apply_packed": (C.c_int, [P, P, P, C.c_int, P]),
+    "heomq_sweep_solve": (C.c_int, [P, C.c_double, C.c_int, D, D, D, D, P]),
     "heomq_rk4": (C.c_int, [P, P, C.c_double, C.c_int, P]),
     "heomq_rk4_at": (C.c_int, [P, P, C.c_double, C.c_double, C.c_int, P]),
     "heomq_ctx_set_field": (C.c_int, [P, C.c_int, D, D, C.c_double]),

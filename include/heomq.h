@@ -490,6 +490,37 @@ int heomq_batch_solve(heomq_batch* b, double tol, int maxiter, double* rdo, doub
                       double* stats, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Sweeps: nprob problems of up to 16 hierarchy shapes (Pade terms, top tier;
+ * the same orbitals, leads and d for all) solved together with ONE launch per
+ * solver step (the apply: one launch per 10 shapes).  The problems are
+ * grouped by shape (each shape's structural tables shared, as a batch), and
+ * every packed apply and every TFQMR vector kernel runs over a flat grid that
+ * covers every problem of every shape, largest shape first; each shape's
+ * apply form is the one heomq_ctx_autotune picks for it.  Each problem keeps the blocks-per-problem count of a solve
+ * on its own, so its reductions fold the same partial sums in the same order.
+ * The reference's counterpart is one process per point
+ * (tools/make_input_ivcurve/ivcurve.csh); no entry point of its own.
+ * ------------------------------------------------------------------------- */
+typedef struct heomq_sweep heomq_sweep;
+int heomq_sweep_create(const heomq_params* params, int nprob, int device, heomq_sweep** out);
+void heomq_sweep_free(heomq_sweep* w);
+/* info[0] = problems, [1] = shapes, [2] = packed complex elements of all
+ * problems, [3] = d, [4] = workgroups of one apply launch, [5] = workgroups of
+ * one vector-kernel launch. */
+int heomq_sweep_info(const heomq_sweep* w, int64_t* info);
+/* Per problem (caller's order; any output may be NULL): its shape (shapes
+ * numbered largest first), the complex offset and length of its packed vector
+ * in the sweep's flat layout. */
+int heomq_sweep_problems(const heomq_sweep* w, int32_t* shape, int64_t* off, int64_t* len);
+/* y = Pi A x for every problem at once on flat-layout device vectors (trace
+ * row HEOMQ_TRACE_RDO), on `stream`; one launch for up to 10 shapes. */
+int heomq_sweep_apply_packed(heomq_sweep* w, const double* xp, double* yp, int trace, void* stream);
+/* TFQMR steady state of every problem, as heomq_batch_solve; outputs in the
+ * caller's problem order. */
+int heomq_sweep_solve(heomq_sweep* w, double tol, int maxiter, double* rdo, double* current, double* occ,
+                      double* stats, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Propagation (td/intstp.f90 RK4 and td/chebyshev.f90 Chebyshev), fixed H.
  * x (device) is advanced in place by nstep steps of dt.  Chebyshev's stats
  * (HEOMQ_STATS_CHEBY_LEN, may be NULL): [0] terms per step, [1] delta, [2] 1 when
