@@ -94,9 +94,11 @@ def main():
     ap.add_argument("--apply-only", action="store_true",
                     help="profiling: a 64 MiB calibration copy, then --apply-reps applies; no solve, no JSON")
     ap.add_argument("--apply-kernel", default="packed",
-                    choices=["packed", "dense", "precond", "f32", "transpose", "spa", "rk4", "chebyshev", "bicg"],
+                    choices=["packed", "dense", "precond", "precond_fused", "f32", "transpose", "spa", "rk4",
+                             "chebyshev", "bicg"],
                     help="--apply-only: the packed apply, the dense drop-in apply (reference vector layout), "
-                         "the packed apply followed by the block-Jacobi inverse, the FP32 packed apply, "
+                         "the packed apply followed by the block-Jacobi inverse, the two fused in one launch "
+                         "(the steady solve's operator), the FP32 packed apply, "
                          "the transposed packed apply, the sparse (COO) apply on the full pattern, "
                          "--apply-reps RK4 or Chebyshev steps, or one BiCG steady-state solve")
     args = ap.parse_args()
@@ -146,6 +148,8 @@ def main():
     elapsed = time.perf_counter() - t0
     ctx.set_shadow_seed(0)
     solves_per_s = args.steps / elapsed
+    # 2: block Jacobi (the fused apply where the form allows it) in any step
+    precond_kind = max(st["precond"] for st in stats)
     cur, occ = ctx.observables(x)
     # outside the timed region: every step's RDO against the fixture
     accuracy = None
@@ -158,30 +162,50 @@ def main():
                     "reference": f"tests/golden/{fix_name} (GMRES on the oracle operator, "
                                  f"residual {fix_relres:.1e})"}
 
-    # ---- roofline of the dominant kernel: the packed (symmetry-sector) apply ----
+    # ---- roofline of the dominant kernel: the solve's operator, the packed
+    # (symmetry-sector) apply, fused with the block-Jacobi inverse when the
+    # solve runs block-Jacobi preconditioned; the plain apply beside it ----
     info = ctx.packed_info()
     P = info["packed"]
     xp = ctx.pack(x)
     yp = ctx.packed_zeros()
-    for _ in range(3):
-        ctx.apply_packed_raw(xp, yp, trace=heomq.TRACE_RDO)
     s = torch.cuda.current_stream()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(s)
-    for _ in range(args.apply_reps):
-        ctx.apply_packed_raw(xp, yp, trace=heomq.TRACE_RDO)
-    ev1.record(s)
-    torch.cuda.synchronize()
-    t_apply = ev0.elapsed_time(ev1) / 1e3 / args.apply_reps
     bytes_alg = 2 * 16 * P                  # one FP64-complex read of x + one write of y (packed state)
-    achieved = bytes_alg / t_apply / 1e9
     dense_bytes = 2 * 16 * d * d * N        # the same state in the dense d x d layout
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
-                "kernel": "k_apply_packed<double,false,2>", "t_apply_us": round(t_apply * 1e6, 2),
-                "form": tuned["forward"], "autotune_us": tuned["times_us"],
-                "bytes_alg": bytes_alg, "packed_elems": P, "dense_elems": info["dense"],
-                "dense_equiv_GBs": round(dense_bytes / t_apply / 1e9, 1)}
+
+    def timed(call):
+        for _ in range(3):
+            call()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(s)
+        for _ in range(args.apply_reps):
+            call()
+        ev1.record(s)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / 1e3 / args.apply_reps
+
+    t_apply = timed(lambda: ctx.apply_packed_raw(xp, yp, trace=heomq.TRACE_RDO))
+    achieved = bytes_alg / t_apply / 1e9
+    plain = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
+             "kernel": "k_apply_packed<double,false,2>", "t_apply_us": round(t_apply * 1e6, 2),
+             "form": tuned["forward"], "autotune_us": tuned["times_us"],
+             "bytes_alg": bytes_alg, "packed_elems": P, "dense_elems": info["dense"],
+             "dense_equiv_GBs": round(dense_bytes / t_apply / 1e9, 1)}
+    roofline = plain
+    if precond_kind == 2:
+        try:
+            t_fused = timed(lambda: ctx.apply_packed_precond(xp, yp, trace=heomq.TRACE_RDO, form=4))
+        except heomq.HeomqError:
+            t_fused = None   # the apply form in use is not the fusable one: the solve ran two kernels
+        if t_fused is not None:
+            a_f = bytes_alg / t_fused / 1e9
+            roofline = {"bound": "hbm", "achieved": round(a_f, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(a_f / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(args.workload, "precond_apply"),
+                        "kernel": "k_apply_packed_jac<true> (packed apply + block-Jacobi inverse, one launch)",
+                        "t_apply_us": round(t_fused * 1e6, 2), "form": tuned["forward"], "bytes_alg": bytes_alg,
+                        "packed_elems": P, "plain_apply": plain}
 
     iters = float(np.mean([st["iters"] for st in stats]))
     applies = float(np.mean([st["applies"] for st in stats]))
@@ -265,11 +289,12 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def measured_traffic(wname):
-    """HBM bytes per packed-apply launch from the committed rocprofv3 --pmc
-    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+def measured_traffic(wname, kind="packed_apply"):
+    """HBM bytes per launch of the packed apply (or the fused preconditioned
+    apply, kind "precond_apply") from the committed rocprofv3 --pmc summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
     import re
-    f = ROOT / "profiles" / f"packed_apply_{wname}_traffic.md"
+    f = ROOT / "profiles" / f"{kind}_{wname}_traffic.md"
     if not f.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", f.read_text())
@@ -461,7 +486,7 @@ def apply_only(params, reps, kernel="packed"):
     evict.fill_(0.0)   # 256 MiB written after src: src is no longer in L2 when it is copied
     dst.copy_(src)
     ctx = heomq.Context(heomq.Model(params))
-    if kernel in ("packed", "precond", "f32", "transpose"):
+    if kernel in ("packed", "precond", "precond_fused", "f32", "transpose"):
         if os.environ.get("HEOMQ_APPLY_FORM"):   # a forced form: every profiled launch is that one
             print(f"apply-only: form {os.environ['HEOMQ_APPLY_FORM']} (forced)", flush=True)
         else:
@@ -504,10 +529,10 @@ def apply_only(params, reps, kernel="packed"):
     elif kernel == "transpose":
         for _ in range(reps):
             ctx.apply_packed_raw(xp, yp, trace=heomq.TRACE_RDO, transpose=True)
-    elif kernel == "precond":
+    elif kernel in ("precond", "precond_fused"):
         ctx.set_precond(2)
         for _ in range(reps):
-            ctx.apply_packed_precond(xp, yp, trace=heomq.TRACE_RDO, form=1)
+            ctx.apply_packed_precond(xp, yp, trace=heomq.TRACE_RDO, form=4 if kernel == "precond_fused" else 1)
     else:
         for _ in range(reps):
             ctx.apply_packed_raw(xp, yp, trace=heomq.TRACE_RDO)

@@ -52,6 +52,8 @@ struct heomq_batch {
   }
 };
 
+constexpr int kTfqVecs = 10;   // x, b and the TFQMR vectors of one solve, plus A x and the best iterate
+
 struct heomq_sweep {
   int nprob = 0, ng = 0;
   std::vector<heomq_batch*> shapes;   // one per shape, largest first (structure and coefficients only)
@@ -200,8 +202,8 @@ int batch_build(const heomq_params* params, int nprob, int device, bool workspac
         if (i != k && H[i + (size_t)k * c->d] != cplx(0, 0)) diag = false;
   B->precond = diag;
   if (workspace) {
-    // x, b, w, y1, y2, u1, u2, v, d, ax, xbest for every problem
-    HIPCHECK(B->work.alloc(sizeof(double) * (size_t)B->n * nprob * 11));
+    // x, b, w, y2, u2, aw, v, d, ax, xbest for every problem
+    HIPCHECK(B->work.alloc(sizeof(double) * (size_t)B->n * nprob * kTfqVecs));
     HIPCHECK(B->scal.alloc(sizeof(TfqmrScal) * (size_t)nprob));
     HIPCHECK(B->partial.alloc(sizeof(double) * 2 * kMaxRedBlocks * (size_t)nprob));
     HIPCHECK(B->out.alloc(sizeof(double) * 2 * (size_t)nprob));
@@ -219,7 +221,7 @@ int batch_build(const heomq_params* params, int nprob, int device, bool workspac
 
 // Device vectors, scalars and host mirrors of one solve over P problems.
 struct TfqBufs {
-  double *x, *b, *w, *y1, *y2, *u1, *u2, *v, *dv, *ax, *xbest;
+  double *x, *b, *w, *y2, *u2, *aw, *v, *dv, *ax, *xbest;
   TfqmrScal* s;
   double *part, *outv;
   int32_t *mask, *mask2;
@@ -232,8 +234,8 @@ struct TfqBufs {
 TfqBufs carve_bufs(double* base, int64_t nt, TfqmrScal* s, double* part, double* outv, int32_t* mask, void* pinned,
                    int P) {
   TfqBufs V;
-  double** vec[11] = {&V.x, &V.b, &V.w, &V.y1, &V.y2, &V.u1, &V.u2, &V.v, &V.dv, &V.ax, &V.xbest};
-  for (int k = 0; k < 11; ++k) *vec[k] = base + (int64_t)k * nt;
+  double** vec[kTfqVecs] = {&V.x, &V.b, &V.w, &V.y2, &V.u2, &V.aw, &V.v, &V.dv, &V.ax, &V.xbest};
+  for (int k = 0; k < kTfqVecs; ++k) *vec[k] = base + (int64_t)k * nt;
   V.s = s;
   V.part = part;
   V.outv = outv;
@@ -277,10 +279,10 @@ int tfqmr_loop(Lay& L, const TfqBufs& V, hipGraphExec_t& chunk, hipStream_t st, 
       HIPCHECK(L.resid(V.b, V.ax, V.outv, V.part));
     }
     HIPCHECK(L.Ait(V.x, V.ax));
-    HIPCHECK(L.init_r(V.b, V.ax, V.w, V.y1, V.dv, V.s, V.part));
+    HIPCHECK(L.init_r(V.b, V.ax, V.w, V.y2, V.dv, V.s, V.part));
     if (L.any_scaled()) HIPCHECK(tfq_scale_targets(st, V.s, V.outv, P));
-    HIPCHECK(L.Ait(V.y1, V.ax));
-    HIPCHECK(L.init_v(V.ax, V.v, V.s, V.part, V.u1));
+    HIPCHECK(L.Ait(V.w, V.ax));   // into scratch: the problems that keep running keep their aw
+    HIPCHECK(L.init_v(V.ax, V.v, V.aw, V.u2, V.s, V.part));
     return HEOMQ_OK;
   };
   HIPCHECK(tfq_set_unscaled(st, V.s, nullptr, P));   // every problem starts with the scaled iteration
@@ -291,12 +293,11 @@ int tfqmr_loop(Lay& L, const TfqBufs& V, hipGraphExec_t& chunk, hipStream_t st, 
     HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t rec = hipSuccess;
     for (int k = 0; k < check_every && rec == hipSuccess; ++k) {
-      if ((rec = L.step1(V.y1, V.v, V.y2, V.w, V.u1, V.dv, V.s, V.part))) break;
+      if ((rec = L.step1(V.w, V.y2, V.v, V.aw, V.u2, V.dv, V.s, V.part))) break;
       if ((rec = L.Ait(V.y2, V.u2))) break;
       if ((rec = L.step2(V.x, V.w, V.u2, V.dv, V.y2, V.s, V.part))) break;
-      if ((rec = L.step3(V.x, V.dv, V.w, V.y2, V.y1, V.s))) break;
-      if ((rec = L.Ait(V.y1, V.u1))) break;
-      rec = L.vstep(V.u1, V.u2, V.v, V.s, V.part);
+      if ((rec = L.Ait(V.w, V.aw))) break;
+      rec = L.vstep(V.aw, V.u2, V.v, V.x, V.dv, V.s, V.part);
     }
     hipGraph_t g = nullptr;
     const hipError_t end = hipStreamEndCapture(st, &g);
@@ -485,24 +486,22 @@ struct BatchLayout {
   hipError_t resid(const double* b, const double* ax, double* out, double* part) {
     return vec_resid<double, double>(st, B->n, b, ax, out, part, bt);
   }
-  hipError_t init_r(const double* b, const double* ax, double* w, double* y1, double* d, TfqmrScal* s, double* part) {
-    return tfq_init_r<double>(st, B->n, b, ax, w, y1, d, s, part, bt);
+  hipError_t init_r(const double* b, const double* ax, double* w, double* y2, double* d, TfqmrScal* s, double* part) {
+    return tfq_init_r<double>(st, B->n, b, ax, w, y2, d, s, part, bt);
   }
-  hipError_t init_v(const double* u1, double* v, TfqmrScal* s, double* part, double* u1c) {
-    return tfq_init_v<double>(st, B->n, u1, v, s, part, bt, u1c);
+  hipError_t init_v(const double* ax, double* v, double* aw, double* u2, TfqmrScal* s, double* part) {
+    return tfq_init_v<double>(st, B->n, ax, v, aw, u2, s, part, bt);
   }
-  hipError_t step1(const double* y1, const double* v, double* y2, double* w, const double* u1, double* d,
+  hipError_t step1(double* w, double* y2, const double* v, const double* aw, const double* u2, double* d,
                    TfqmrScal* s, double* part) {
-    return tfq_step1<double>(st, B->n, y1, v, y2, w, u1, d, s, part, bt);
+    return tfq_step1<double>(st, B->n, w, y2, v, aw, u2, d, s, part, bt);
   }
   hipError_t step2(double* x, double* w, const double* u2, double* d, const double* y2, TfqmrScal* s, double* part) {
     return tfq_step2<double>(st, B->n, x, w, u2, d, y2, s, part, bt);
   }
-  hipError_t step3(double* x, const double* d, const double* w, const double* y2, double* y1, const TfqmrScal* s) {
-    return tfq_step3<double>(st, B->n, x, d, w, y2, y1, s, bt);
-  }
-  hipError_t vstep(const double* u1, const double* u2, double* v, TfqmrScal* s, double* part) {
-    return tfq_v<double>(st, B->n, u1, u2, v, s, part, bt);
+  hipError_t vstep(const double* aw, const double* u2, double* v, double* x, const double* d, TfqmrScal* s,
+                   double* part) {
+    return tfq_v<double>(st, B->n, aw, u2, v, x, d, s, part, bt);
   }
   hipError_t start(double* x, double* b) { return shape_start(B->ctx, op_it, x, b, B->n, B->nprob, st); }
   hipError_t herm(int, double* xq) { return launch_herm_packed<double>(B->ctx->pfwd, xq, st); }
@@ -541,24 +540,22 @@ struct SweepLayout {
   hipError_t resid(const double* b, const double* ax, double* out, double* part) {
     return vec_resid(st, W->cur, b, ax, out, part);
   }
-  hipError_t init_r(const double* b, const double* ax, double* w, double* y1, double* d, TfqmrScal* s, double* part) {
-    return tfq_init_r(st, W->cur, b, ax, w, y1, d, s, part);
+  hipError_t init_r(const double* b, const double* ax, double* w, double* y2, double* d, TfqmrScal* s, double* part) {
+    return tfq_init_r(st, W->cur, b, ax, w, y2, d, s, part);
   }
-  hipError_t init_v(const double* u1, double* v, TfqmrScal* s, double* part, double* u1c) {
-    return tfq_init_v(st, W->cur, u1, v, s, part, u1c);
+  hipError_t init_v(const double* ax, double* v, double* aw, double* u2, TfqmrScal* s, double* part) {
+    return tfq_init_v(st, W->cur, ax, v, aw, u2, s, part);
   }
-  hipError_t step1(const double* y1, const double* v, double* y2, double* w, const double* u1, double* d,
+  hipError_t step1(double* w, double* y2, const double* v, const double* aw, const double* u2, double* d,
                    TfqmrScal* s, double* part) {
-    return tfq_step1(st, W->cur, y1, v, y2, w, u1, d, s, part);
+    return tfq_step1(st, W->cur, w, y2, v, aw, u2, d, s, part);
   }
   hipError_t step2(double* x, double* w, const double* u2, double* d, const double* y2, TfqmrScal* s, double* part) {
     return tfq_step2(st, W->cur, x, w, u2, d, y2, s, part);
   }
-  hipError_t step3(double* x, const double* d, const double* w, const double* y2, double* y1, const TfqmrScal* s) {
-    return tfq_step3(st, W->cur, x, d, w, y2, y1, s);
-  }
-  hipError_t vstep(const double* u1, const double* u2, double* v, TfqmrScal* s, double* part) {
-    return tfq_v(st, W->cur, u1, u2, v, s, part);
+  hipError_t vstep(const double* aw, const double* u2, double* v, double* x, const double* d, TfqmrScal* s,
+                   double* part) {
+    return tfq_v(st, W->cur, aw, u2, v, x, d, s, part);
   }
   hipError_t start(double* x, double* b) {
     for (int g = 0; g < W->ng; ++g) {
@@ -755,7 +752,7 @@ int heomq_sweep_create(const heomq_params* params, int nprob, int device, heomq_
   W->m.qstart[ng] = q;
   W->nt = voff;
   HIPCHECK(hipSetDevice(device));
-  HIPCHECK(W->work.alloc(sizeof(double) * (size_t)W->nt * 11));
+  HIPCHECK(W->work.alloc(sizeof(double) * (size_t)W->nt * kTfqVecs));
   HIPCHECK(W->scal.alloc(sizeof(TfqmrScal) * (size_t)nprob));
   HIPCHECK(W->partial.alloc(sizeof(double) * 2 * (size_t)vb));
   HIPCHECK(W->out.alloc(sizeof(double) * 2 * (size_t)nprob));

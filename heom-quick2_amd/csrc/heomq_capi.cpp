@@ -1248,12 +1248,21 @@ int heomq_ctx_autotune(heomq_ctx* c, int reps, int32_t* forms, double* times_us)
   if (c->ptotal <= 0) return HEOMQ_OK;
   HIPCHECK(hipSetDevice(c->device));
   c->drop_graphs();   // captured chunks hold the operator's tables by value
-  DevBuf xb, yb;
+  DevBuf xb, yb, tb;
   const size_t bytes = sizeof(double) * 2 * (size_t)c->ptotal;
   HIPCHECK(xb.alloc(bytes));
   HIPCHECK(yb.alloc(bytes));
   hipStream_t st = c->solve_stream;
   HIPCHECK(vec_fill<double>(st, 2 * c->ptotal, xb.as<double>(), 1.0));
+  // the forward form is timed on the operator the steady solve iterates
+  // with: under block-Jacobi preconditioning (the default where H couples
+  // Fock states) the apply and its inverse, as one fused launch where the
+  // form allows it
+  const int want = c->precond >= 0 ? c->precond : (c->h_diagonal ? 1 : 2);
+  DevJacobi jc = c->pjac;
+  jc.ejac = c->prec_shift > 0 ? c->prec_shift : 1.0;   // timing only: any positive shift
+  const bool prec_fwd = want == 2 && jacobi_one_kernel(jc) && !c->trun.built;
+  if (prec_fwd) HIPCHECK(tb.alloc(bytes));
   hipEvent_t e0, e1;
   HIPCHECK(hipEventCreate(&e0));
   HIPCHECK(hipEventCreate(&e1));
@@ -1265,11 +1274,17 @@ int heomq_ctx_autotune(heomq_ctx* c, int reps, int32_t* forms, double* times_us)
       if ((f & 4) && !c->heavy_wave_ok) continue;
       set_apply_form(c, w, f);
       const DevPacked& op = w == 0 ? c->pfwd : c->ptrn;
+      auto one = [&]() -> hipError_t {
+        if (w == 1 || !prec_fwd) return launch_apply_packed_f64(op, xb.as<double>(), yb.as<double>(), true, st);
+        if (c->fuse_precond && apply_jac_fusable(op, jc))
+          return launch_apply_packed_jac(op, jc, xb.as<double>(), yb.as<double>(), true, st);
+        if (hipError_t e = launch_apply_packed_f64(op, xb.as<double>(), tb.as<double>(), true, st)) return e;
+        return launch_jacobi_1k(op, jc, tb.as<double>(), yb.as<double>(), false, st);
+      };
       hipError_t e = hipSuccess;
-      for (int k = 0; k < 2 && e == hipSuccess; ++k) e = launch_apply_packed_f64(op, xb.as<double>(), yb.as<double>(), true, st);
+      for (int k = 0; k < 2 && e == hipSuccess; ++k) e = one();
       if (e == hipSuccess) e = hipEventRecord(e0, st);
-      for (int k = 0; k < reps && e == hipSuccess; ++k)
-        e = launch_apply_packed_f64(op, xb.as<double>(), yb.as<double>(), true, st);
+      for (int k = 0; k < reps && e == hipSuccess; ++k) e = one();
       if (e == hipSuccess) e = hipEventRecord(e1, st);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float ms = 0.f;

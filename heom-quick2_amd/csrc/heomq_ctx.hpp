@@ -155,6 +155,9 @@ struct heomq_ctx {
   };
   std::vector<ChunkGraph> graphs;
   bool use_graphs = true;            // false: launch the chunk's kernels one by one
+  // block-Jacobi-preconditioned applies as one fused launch where the apply
+  // form allows it (else the apply, then the inverse)
+  bool fuse_precond = true;
   // left preconditioning of forward steady-state solves: -1 automatic (Jacobi
   // scaling when H is diagonal in the occupation basis, block Jacobi
   // otherwise), 0 none, 1 Jacobi scaling, 2 block Jacobi

@@ -318,5 +318,12 @@ inline bool jacobi_one_kernel(const DevJacobi& jc) { return jc.ado_blocks > 0 ||
 hipError_t launch_jacobi_1k(const DevPacked& op, const DevJacobi& jc, const double* in, double* out, bool accumulate,
                             hipStream_t s);
 constexpr int kJacobiWindow = 384;    // LDS window of the one-kernel form (complex elements); d <= 16 needs <= 328
+// y = M^-1 (Pi A x), the block-Jacobi-preconditioned forward apply, in one
+// launch (no A x in HBM).  Needs the sector-batched top tier and sliced heavy
+// rows (apply forms 2 and 6 without the wave form), one problem, no Jacobi
+// scaling, d <= 16 and the ADO-aligned inverse's tables.
+bool apply_jac_fusable(const DevPacked& op, const DevJacobi& jc);
+hipError_t launch_apply_packed_jac(const DevPacked& op, const DevJacobi& jc, const double* x, double* y, bool rdo,
+                                   hipStream_t s);
 
 }  // namespace heomq

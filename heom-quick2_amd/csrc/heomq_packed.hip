@@ -297,14 +297,27 @@ __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T
 template <typename T>
 __device__ void tier0_fixup_block(const DevPacked& op, const T* __restrict__ x, T* __restrict__ y, int rdo);
 
+// A thread's apply output held back for a fused preconditioner (instead of
+// being written to y): its value, the element's packed index (gbase + t), its
+// place among the block's elements (lbase + t, the ADO's slots contiguous),
+// the ADO's sector and rate.
+struct StageOut {
+  double2 v;
+  int64_t gbase;
+  int lbase, t, s;
+  double gr, gi;
+  bool act;
+};
+
 // ADOs below the top tier: one workgroup per ADO.  The block's 256 lanes are
 // laid out as (slice, slot): S = 256 / nnz slices, each walking every S-th
 // coupling of the ADO (entries and coefficients staged in LDS), two
 // couplings per step; the slices are summed through LDS in a fixed order.
 // Block 0 (the tier-0 ADO) then applies the trace row and Hermitianises.
-template <typename T, bool TRANS>
+template <typename T, bool TRANS, bool STAGE = false>
 __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
-                                  int64_t zero_at, int rdo, int bid, char* smem, bool scaled) {
+                                  int64_t zero_at, int rdo, int bid, char* smem, bool scaled,
+                                  StageOut* so = nullptr) {
   using V = typename P2<T>::type;
   constexpr int kStage = 256;
   T* red_r = reinterpret_cast<T*>(smem);
@@ -383,11 +396,25 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
       V o;
       o.x = sr;
       o.y = si;
-      reinterpret_cast<V*>(y)[base + t] = o;
+      if (STAGE) {   // one pass (nnz <= 256): slot t = threadIdx.x
+        so->v = make_double2((double)sr, (double)si);
+        so->act = true;
+      } else {
+        reinterpret_cast<V*>(y)[base + t] = o;
+      }
     }
     __syncthreads();
   }
-  if (a == 0) tier0_fixup_block<T>(op, x, y, rdo);   // block-uniform branch
+  if (STAGE) {
+    so->gbase = base;
+    so->lbase = 0;
+    so->t = threadIdx.x;
+    so->s = s;
+    so->gr = (double)gr;
+    so->gi = (double)gi;
+  } else if (a == 0) {
+    tier0_fixup_block<T>(op, x, y, rdo);   // block-uniform branch
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -614,9 +641,10 @@ __device__ inline void heavy_wave(const DevPacked& op, const Coefs cf_p, const T
 // block) read their neighbours.  Four levels of dependent loads in all.
 constexpr size_t kLOffHc = 256 * 2, kLOffX = kLOffHc + 256 * 16, kLSmem = kLOffX + 256 * 16;
 
-template <typename T, bool TRANS>
+template <typename T, bool TRANS, bool STAGE = false>
 __device__ inline void light_sector(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x,
-                                    T* __restrict__ y, int64_t zero_at, int bid, char* smem, bool scaled) {
+                                    T* __restrict__ y, int64_t zero_at, int bid, char* smem, bool scaled,
+                                    StageOut* so = nullptr) {
   using V = typename P2<T>::type;
   int16_t* s_hq = reinterpret_cast<int16_t*>(smem);
   double2* s_hc = reinterpret_cast<double2*>(smem + kLOffHc);
@@ -700,6 +728,17 @@ __device__ inline void light_sector(const DevPacked& op, const Coefs cf_p, const
     ai += di;
   }
   if (scaled) jacobi_scale<T>(op, gr, gi, csr, csi, ar, ai);
+  if (STAGE) {
+    so->v = make_double2((double)ar, (double)ai);
+    so->act = true;
+    so->gbase = base;
+    so->lbase = j * nnz;
+    so->t = t;
+    so->s = s;
+    so->gr = (double)gr;
+    so->gi = (double)gi;
+    return;
+  }
   V o;
   o.x = ar;
   o.y = ai;
@@ -1183,6 +1222,141 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
   xo[w0 + tid] = o;
 }
 
+// The block-Jacobi-preconditioned apply in one launch, y = M^-1 (Pi A x):
+// each workgroup of the apply holds whole ADOs (a heavy ADO, or a
+// sector-batched run of top-tier ADOs), so it keeps its outputs in registers,
+// reuses the apply's LDS for the four stages of k_jacobi_ado on them, and
+// writes only M^-1 A x.  The tier-0 workgroup applies the trace row and the
+// Hermitian tier-0 block to its staged outputs first, and averages each
+// element with the conjugate of its mirror after the inverse, as the two
+// kernels do.  Every element goes through the same arithmetic as in the
+// apply followed by k_jacobi_ado.
+template <bool RU>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+k_apply_packed_jac(const DevPacked op, const DevJacobi jc, const double* __restrict__ x, double* __restrict__ out,
+                   int rdo) {
+  static_assert(2 * 256 * sizeof(double2) + sizeof(UStore<RU>) + 16 * sizeof(double) <= kApplySmem,
+                "the inverse's stages fit the apply's LDS");
+  __shared__ __align__(16) char smem[kApplySmem];
+  const Coefs cf{op.rec, op.gam, op.hc_ell};
+  const int64_t hb = op.n_heavy;
+  const bool heavy = (int64_t)blockIdx.x < hb;
+  StageOut so;
+  so.act = false;
+  so.v = make_double2(0.0, 0.0);
+  so.lbase = 0;
+  so.t = 0;
+  if (heavy)
+    heavy_body<double, false, true>(op, cf, x, nullptr, -1, rdo, (int)blockIdx.x, smem, false, &so);
+  else
+    light_sector<double, false, true>(op, cf, x, nullptr, -1, (int)(blockIdx.x - hb), smem, false, &so);
+  __syncthreads();   // the apply's LDS is free from here
+  double2* xw = reinterpret_cast<double2*>(smem);
+  double2* tw = xw + 256;
+  UStore<RU>* us = reinterpret_cast<UStore<RU>*>(smem + 2 * 256 * sizeof(double2));
+  double* ew = reinterpret_cast<double*>(smem + 2 * 256 * sizeof(double2) + sizeof(UStore<RU>));
+  const int tid = threadIdx.x, d = op.d, DD = d * d;
+  const bool tier0 = heavy && blockIdx.x == 0;
+  xw[tid] = so.act ? so.v : make_double2(0.0, 0.0);
+  if (tid < DD) us->put(tid, reinterpret_cast<const double2*>(jc.u)[tid]);
+  if (tid < d) ew[tid] = jc.e[tid];
+  __syncthreads();
+  if (tier0) {   // block-uniform: the tier-0 fix-ups of the apply, on the staged block
+    const int16_t* pos = op.sec_pos + op.sec[0] * DD;
+    const int16_t* el = op.sec_elems + op.sec[0] * DD;
+    if (rdo && tid == 0) {
+      double sr = 0, si = 0;
+      for (int i = 0; i < d; ++i) {
+        const int q = pos[i + i * d];
+        if (q >= 0) { sr += x[2 * q]; si += x[2 * q + 1]; }
+      }
+      xw[op.trace_slot] = make_double2(sr, si);
+    }
+    __syncthreads();
+    if (op.herm0) {
+      const int n0 = (int)(op.off[1] - op.off[0]);
+      for (int t = tid; t < n0; t += blockDim.x) {
+        const int e = el[t];
+        const int i = e % d, j = e / d;
+        if (i > j) continue;
+        const int q = pos[j + i * d];
+        if (q < 0) continue;
+        const double2 a = xw[t], b = xw[q];
+        const double re = 0.5 * (a.x + b.x), im = 0.5 * (a.y - b.y);
+        xw[t] = make_double2(re, im);
+        xw[q] = make_double2(re, -im);
+      }
+    }
+    __syncthreads();
+  }
+  const int base = so.lbase;
+  JacDesc ds{};
+  if (so.act) ds = jc.desc[so.s * DD + so.t];
+  // X -> X U_c
+  if (so.act) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < ds.nq) {
+        const double2 p = us->mul(ds.c2[q] + ds.c * d, xw[base + ds.cslot[q]]);
+        acc.x += p.x;
+        acc.y += p.y;
+      }
+    tw[tid] = acc;
+  }
+  __syncthreads();
+  // -> (U_r^H .) / (shift - gamma_K + i (E_r - E_c))
+  if (so.act) {
+    double2 q = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p < ds.np) {
+        const double2 v = us->mulc(ds.r2[p] + ds.r * d, tw[base + ds.rslot[p]]);
+        q.x += v.x;
+        q.y += v.y;
+      }
+    const double wr = jc.ejac - so.gr, wi = (ew[ds.r] - ew[ds.c]) - so.gi;
+    const double den = wr * wr + wi * wi;
+    xw[tid] = make_double2((q.x * wr + q.y * wi) / den, (q.y * wr - q.x * wi) / den);
+  }
+  __syncthreads();
+  // -> . U_c^H
+  if (so.act) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < ds.nq) {
+        const double2 p = us->mulc(ds.c + ds.c2[q] * d, xw[base + ds.cslot[q]]);
+        acc.x += p.x;
+        acc.y += p.y;
+      }
+    tw[tid] = acc;
+  }
+  __syncthreads();
+  // -> U_r .
+  double2 z = make_double2(0.0, 0.0);
+  if (so.act) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p < ds.np) {
+        const double2 v = us->mul(ds.r + ds.r2[p] * d, tw[base + ds.rslot[p]]);
+        z.x += v.x;
+        z.y += v.y;
+      }
+  }
+  if (tier0) {   // tier 0: average with conj of the mirror element
+    xw[tid] = z;
+    __syncthreads();
+    const int tm = so.act ? op.sec_pos[so.s * DD + ds.c + ds.r * d] : -1;
+    if (tm >= 0) {
+      const double2 m = xw[tm];
+      z.x = 0.5 * (z.x + m.x);
+      z.y = 0.5 * (z.y - m.y);
+    }
+  }
+  if (so.act) reinterpret_cast<double2*>(out)[so.gbase + so.t] = z;
+}
+
 template <typename TP, typename TD>
 __global__ void k_unpack(const DevPacked op, const TP* __restrict__ packed, TD* __restrict__ dense) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1372,6 +1546,22 @@ hipError_t launch_jacobi_ado_f32(const DevPacked& op, const DevJacobi& jc, const
   else
     hipLaunchKernelGGL((k_jacobi_ado<false, float>), dim3((unsigned)jc.ado_blocks), dim3(256), 0, s, op, jc, in,
                        out, 0);
+  return hipGetLastError();
+}
+
+bool apply_jac_fusable(const DevPacked& op, const DevJacobi& jc) {
+  return op.lt_blk != nullptr && !op.heavy_wave && op.hg_off == nullptr && !op.transposed && op.precond == 0 &&
+         op.nprob == 1 && op.n_heavy > 0 && op.d <= 16 && jc.ado_blocks > 0 && jc.desc != nullptr;
+}
+
+hipError_t launch_apply_packed_jac(const DevPacked& op, const DevJacobi& jc, const double* x, double* y, bool rdo,
+                                   hipStream_t s) {
+  if (!apply_jac_fusable(op, jc) || (rdo && op.trace_slot < 0)) return hipErrorInvalidValue;
+  const int64_t blocks = op.n_heavy + op.n_lt_blocks;
+  if (jc.u_real)
+    hipLaunchKernelGGL((k_apply_packed_jac<true>), dim3((unsigned)blocks), dim3(256), 0, s, op, jc, x, y, rdo ? 1 : 0);
+  else
+    hipLaunchKernelGGL((k_apply_packed_jac<false>), dim3((unsigned)blocks), dim3(256), 0, s, op, jc, x, y, rdo ? 1 : 0);
   return hipGetLastError();
 }
 

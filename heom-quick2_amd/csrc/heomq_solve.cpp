@@ -46,6 +46,8 @@ hipError_t apply_packed_T<double>(heomq_ctx* c, bool trans, const double* x, dou
   if (prec != 2) return trans ? launch_apply_packed_f64(op, x, y, rdo, s) : packed_forward(c, op, x, y, rdo, s);
   DevJacobi jc = c->pjac;
   jc.ejac = c->prec_shift;
+  if (c->fuse_precond && !c->trun.built && apply_jac_fusable(op, jc))   // one launch, no A x in HBM
+    return launch_apply_packed_jac(op, jc, x, y, rdo, s);
   if (jacobi_one_kernel(jc)) {   // A x into the scratch, then y = M^-1 (A x) in one kernel
     if (hipError_t e = packed_forward(c, op, x, scratch, rdo, s)) return e;
     return launch_jacobi_1k(op, jc, scratch, y, false, s);
@@ -105,12 +107,12 @@ int64_t tpos_of(const heomq_ctx* c, int trace) {
 
 template <typename T>
 struct Work {
-  T *w, *y1, *y2, *u1, *u2, *v, *d;   // the Krylov state
+  T *w, *y2, *u2, *aw, *v, *d;        // the Krylov state (A is applied to w: y1, u1 are formed in the kernels)
   T *ax, *xbest;                      // true-residual product, best iterate so far
   T* pt = nullptr;                    // block-Jacobi preconditioning: eigenbasis scratch
   T* bp = nullptr;                    //   and the preconditioned right-hand side
 };
-constexpr int kWorkVecs = 9;
+constexpr int kWorkVecs = 8;
 
 template <typename T>
 int carve(DevBuf& buf, int64_t n, int count, std::vector<T*>& out) {
@@ -127,18 +129,17 @@ struct RunStats {
   double relres = 0;
 };
 
-// The TFQMR iteration, `iters` times: step1, A y2, step2, step3, A y1, v.
+// The TFQMR iteration, `iters` times: step1, A y2, step2, A w, v.
 template <typename T, typename Apply>
 hipError_t record_chunk(heomq_ctx* c, T* x, const Work<T>& W, int64_t n, hipStream_t st, Apply& A, int iters) {
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   double* part = c->partial.as<double>();
   for (int k = 0; k < iters; ++k) {
-    if (hipError_t e = tfq_step1<T>(st, n, W.y1, W.v, W.y2, W.w, W.u1, W.d, s, part)) return e;
+    if (hipError_t e = tfq_step1<T>(st, n, W.w, W.y2, W.v, W.aw, W.u2, W.d, s, part)) return e;
     if (hipError_t e = A(W.y2, W.u2)) return e;
     if (hipError_t e = tfq_step2<T>(st, n, x, W.w, W.u2, W.d, W.y2, s, part)) return e;
-    if (hipError_t e = tfq_step3<T>(st, n, x, W.d, W.w, W.y2, W.y1, s)) return e;
-    if (hipError_t e = A(W.y1, W.u1)) return e;
-    if (hipError_t e = tfq_v<T>(st, n, W.u1, W.u2, W.v, s, part)) return e;
+    if (hipError_t e = A(W.w, W.aw)) return e;
+    if (hipError_t e = tfq_v<T>(st, n, W.aw, W.u2, W.v, x, W.d, s, part)) return e;
   }
   return hipSuccess;
 }
@@ -254,7 +255,7 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
       HIPCHECK(Ait(x, W.ax));
       rs.applies++;
     }
-    HIPCHECK(tfq_init_r<T>(st, n, prec == 2 ? W.bp : b, W.ax, W.w, W.y1, W.d, s, part));
+    HIPCHECK(tfq_init_r<T>(st, n, prec == 2 ? W.bp : b, W.ax, W.w, W.y2, W.d, s, part));
     TfqmrScal& hs = *reinterpret_cast<TfqmrScal*>(static_cast<char*>(c->pinned) + 1024);
     if (prec) {   // the quasi-residual bound measures the scaled residual: scale the target with it
       HIPCHECK(hipMemcpyAsync(&hs, s, sizeof(TfqmrScal), hipMemcpyDeviceToHost, st));
@@ -263,9 +264,9 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
         HIPCHECK(vec_set_elem(st, reinterpret_cast<double*>(s), offsetof(TfqmrScal, target) / sizeof(double),
                               tol * std::sqrt(bb) * std::sqrt(hs.rr / rr)));
     }
-    HIPCHECK(Ait(W.y1, W.u1));
+    HIPCHECK(Ait(W.w, W.aw));
     rs.applies++;
-    HIPCHECK(tfq_init_v<T>(st, n, W.u1, W.v, s, part));
+    HIPCHECK(tfq_init_v<T>(st, n, W.aw, W.v, nullptr, W.u2, s, part));
     hipGraphExec_t chunk = nullptr;
     int no_progress = 0;
     double last_check = rr;
@@ -420,7 +421,7 @@ int steady_solve(heomq_ctx* c, double* x, const double* b, double tol, int maxit
   const int64_t n = packed ? 2 * c->ptotal : 2 * c->N * c->d * c->d;
   std::vector<double*> v;
   if (int rc = carve<double>(c->ws64, n, kWorkVecs + (packed ? 4 : 1), v)) return rc;
-  Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
+  Work<double> W{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
   double* bb = v[kWorkVecs];
   double* xs = packed && !x_packed ? v[kWorkVecs + 1] : x;
   if (packed) {
@@ -548,7 +549,7 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   std::vector<float*> v32;
   if (int rc = carve<float>(c->ws32, n, kWorkVecs + 4, v32)) return rc;
   if (stats) std::fill(stats, stats + HEOMQ_STATS_MIXED_LEN, 0.0);
-  Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6], v32[7], v32[8]};
+  Work<float> W{v32[0], v32[1], v32[2], v32[3], v32[4], v32[5], v32[6], v32[7]};
   float* x32 = v32[kWorkVecs];
   float* r32 = v32[kWorkVecs + 1];
   if (prec32 == 2) {
@@ -606,7 +607,7 @@ int heomq_solve_steady_mixed(heomq_ctx* c, double* x, const double* b, double to
   if (finish64) {
     std::vector<double*> w64;
     if (int e = carve<double>(c->ws64b, n, kWorkVecs + 2, w64)) return e;
-    Work<double> W64{w64[0], w64[1], w64[2], w64[3], w64[4], w64[5], w64[6], w64[7], w64[8]};
+    Work<double> W64{w64[0], w64[1], w64[2], w64[3], w64[4], w64[5], w64[6], w64[7]};
     if (prec64 == 2) {
       W64.pt = w64[kWorkVecs];
       W64.bp = w64[kWorkVecs + 1];
@@ -814,6 +815,13 @@ int heomq_ctx_set_presmooth(heomq_ctx* c, int enable) {
   return HEOMQ_OK;
 }
 
+int heomq_ctx_set_precond_fusion(heomq_ctx* c, int enable) {
+  if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
+  c->drop_graphs();   // captured chunks hold the preconditioned apply's kernels
+  c->fuse_precond = enable != 0;
+  return HEOMQ_OK;
+}
+
 int heomq_ctx_set_graphs(heomq_ctx* c, int enable) {
   if (!c) return fail(HEOMQ_BAD_ARG, "null argument");
   c->use_graphs = enable != 0;
@@ -853,7 +861,7 @@ int heomq_apply_packed_raw(heomq_ctx* c, const double* xp, double* yp, int trace
 int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int trace, int form, void* stream) {
   if (!c || !xp || !yp) return fail(HEOMQ_BAD_ARG, "null argument");
   if (trace != HEOMQ_TRACE_RDO && trace != HEOMQ_TRACE_NONE) return fail(HEOMQ_BAD_ARG, "packed apply: trace RDO or NONE");
-  if (form < 1 || form > 3) return fail(HEOMQ_BAD_ARG, "form must be 1, 2 or 3");
+  if (form < 1 || form > 4) return fail(HEOMQ_BAD_ARG, "form must be 1, 2, 3 or 4");
   if (xp == yp) return fail(HEOMQ_BAD_ARG, "x and y must differ");
   hipStream_t st = (hipStream_t)stream;
   const bool rdo = trace == HEOMQ_TRACE_RDO;
@@ -861,6 +869,13 @@ int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int t
   op.precond = 0;
   DevJacobi jc = c->pjac;
   jc.ejac = c->prec_shift;
+  if (form == 4) {
+    if (c->trun.built || !apply_jac_fusable(op, jc))
+      return fail(HEOMQ_UNSUPPORTED, "the fused preconditioned apply needs apply form 2 or 6 without the wave form "
+                                     "and the ADO-aligned inverse");
+    HIPCHECK(launch_apply_packed_jac(op, jc, xp, yp, rdo, st));
+    return HEOMQ_OK;
+  }
   std::vector<double*> v;
   if (int rc = carve<double>(c->ws64b, 2 * c->ptotal, 1, v)) return rc;
   HIPCHECK(packed_forward(c, op, xp, v[0], rdo, st));

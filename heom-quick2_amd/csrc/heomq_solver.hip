@@ -121,16 +121,24 @@ __device__ inline double shadow(int64_t k, uint64_t seed) {
   return (z >> 63) ? -1.0 : 1.0;
 }
 
-// r = b - Ax (stored in w and y1), d = 0; partials of ||r||^2 and rt.r.
+// The iteration is kept in a form that applies the operator to w instead of
+// y1 = w + beta y2: A y1 = A w + beta u2 (u2 = A y2), so y1 and u1 = A y1 are
+// formed where they are read and never stored.  One iteration is
+//   k_step1, A y2 -> u2, k_step2, A w -> aw, k_v
+// (Freund 1993's TFQMR; the same iterates as the y1 form in exact
+// arithmetic).  The state between iterations is w, y2, u2, aw, v, d, x.
+
+// r = b - Ax (stored in w), y2 = 0, d = 0; partials of ||r||^2 and rt.r.
+// With beta = 0 from the reset, the first k_step1 sees y1 = w = r.
 template <typename T, typename Loc>
 __global__ void k_init_r(const Loc loc, const T* __restrict__ b, const T* __restrict__ ax, T* __restrict__ w,
-                         T* __restrict__ y1, T* __restrict__ d, const TfqmrScal* s, double* partial) {
+                         T* __restrict__ y2, T* __restrict__ d, const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
   const Span sp = loc.span();
   s += sp.q;
   if (s->flag != kInitRequested) return;   // partials of this problem are not read
   const uint64_t seed = s->seed;
-  { const int64_t o = sp.off; b += o; ax += o; w += o; y1 += o; d += o; }
+  { const int64_t o = sp.off; b += o; ax += o; w += o; y2 += o; d += o; }
   double rr = 0, rrt = 0;
   SPAN_LOOP(i, sp) {
     const V bb = reinterpret_cast<const V*>(b)[i], aa = reinterpret_cast<const V*>(ax)[i];
@@ -138,10 +146,10 @@ __global__ void k_init_r(const Loc loc, const T* __restrict__ b, const T* __rest
     r.x = bb.x - aa.x;
     r.y = bb.y - aa.y;
     reinterpret_cast<V*>(w)[i] = r;
-    reinterpret_cast<V*>(y1)[i] = r;
     V z;
     z.x = 0;
     z.y = 0;
+    reinterpret_cast<V*>(y2)[i] = z;
     reinterpret_cast<V*>(d)[i] = z;
     rr += (double)r.x * r.x + (double)r.y * r.y;
     rrt += shadow(2 * i, seed) * r.x + shadow(2 * i + 1, seed) * r.y;
@@ -149,54 +157,67 @@ __global__ void k_init_r(const Loc loc, const T* __restrict__ b, const T* __rest
   span_partial(rr, rrt, partial, sp);
 }
 
-// v = u1; partial of rt.v
+// v = ax (= A r), aw = ax when aw is given (a batch restarts some problems
+// while the others keep their aw), u2 = 0; partial of rt.v
 template <typename T, typename Loc>
-__global__ void k_init_v(const Loc loc, const T* __restrict__ u1, T* __restrict__ v, T* __restrict__ u1_copy,
-                         const TfqmrScal* s, double* partial) {
+__global__ void k_init_v(const Loc loc, const T* __restrict__ ax, T* __restrict__ v, T* __restrict__ aw,
+                         T* __restrict__ u2, const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
   const Span sp = loc.span();
   s += sp.q;
   if (s->flag != kInitialising) return;
   const uint64_t seed = s->seed;
-  { const int64_t o = sp.off; u1 += o; v += o; if (u1_copy) u1_copy += o; }
+  { const int64_t o = sp.off; ax += o; v += o; if (aw) aw += o; u2 += o; }
   double sg = 0;
   SPAN_LOOP(i, sp) {
-    const V t = reinterpret_cast<const V*>(u1)[i];
+    const V t = reinterpret_cast<const V*>(ax)[i];
     reinterpret_cast<V*>(v)[i] = t;
-    if (u1_copy) reinterpret_cast<V*>(u1_copy)[i] = t;
+    if (aw) reinterpret_cast<V*>(aw)[i] = t;
+    V z;
+    z.x = 0;
+    z.y = 0;
+    reinterpret_cast<V*>(u2)[i] = z;
     sg += shadow(2 * i, seed) * t.x + shadow(2 * i + 1, seed) * t.y;
   }
   span_partial(sg, 0.0, partial, sp);
 }
 
 // First half of an iteration, fused with the next search direction:
-// y2 = y1 - alpha v;  w -= alpha u1;  d = y1 + dcoef d;  partial ||w||^2.
+// y1 = w + beta y2, u1 = aw + beta u2;  y2 = y1 - alpha v;  w -= alpha u1;
+// d = y1 + dcoef d;  partial ||w||^2.
 template <typename T, typename Loc>
-__global__ void k_step1(const Loc loc, const T* __restrict__ y1, const T* __restrict__ v, T* __restrict__ y2,
-                        T* __restrict__ w, const T* __restrict__ u1, T* __restrict__ d, const TfqmrScal* s,
+__global__ void k_step1(const Loc loc, T* __restrict__ w, T* __restrict__ y2, const T* __restrict__ v,
+                        const T* __restrict__ aw, const T* __restrict__ u2, T* __restrict__ d, const TfqmrScal* s,
                         double* partial) {
   using V = typename W2<T>::type;
   const Span sp = loc.span();
   s += sp.q;
   if (s->flag) return;
-  { const int64_t o = sp.off; y1 += o; v += o; y2 += o; w += o; u1 += o; d += o; }
-  const T alpha = (T)s->alpha, dcoef = (T)s->dcoef;
+  { const int64_t o = sp.off; w += o; y2 += o; v += o; aw += o; u2 += o; d += o; }
+  const T alpha = (T)s->alpha, dcoef = (T)s->dcoef, beta = (T)s->beta;
   double ww = 0;
   SPAN_LOOP(i, sp) {
-    const V a = reinterpret_cast<const V*>(y1)[i], b = reinterpret_cast<const V*>(v)[i];
-    V o;
-    o.x = a.x - alpha * b.x;
-    o.y = a.y - alpha * b.y;
-    reinterpret_cast<V*>(y2)[i] = o;
     V wi = reinterpret_cast<V*>(w)[i];
-    const V ui = reinterpret_cast<const V*>(u1)[i];
-    wi.x -= alpha * ui.x;
-    wi.y -= alpha * ui.y;
+    const V yo = reinterpret_cast<const V*>(y2)[i];
+    V y1;
+    y1.x = wi.x + beta * yo.x;
+    y1.y = wi.y + beta * yo.y;
+    const V a = reinterpret_cast<const V*>(aw)[i], u = reinterpret_cast<const V*>(u2)[i];
+    V u1;
+    u1.x = a.x + beta * u.x;
+    u1.y = a.y + beta * u.y;
+    const V b = reinterpret_cast<const V*>(v)[i];
+    V o;
+    o.x = y1.x - alpha * b.x;
+    o.y = y1.y - alpha * b.y;
+    reinterpret_cast<V*>(y2)[i] = o;
+    wi.x -= alpha * u1.x;
+    wi.y -= alpha * u1.y;
     reinterpret_cast<V*>(w)[i] = wi;
     const V di = reinterpret_cast<const V*>(d)[i];
     V dn;
-    dn.x = a.x + dcoef * di.x;
-    dn.y = a.y + dcoef * di.y;
+    dn.x = y1.x + dcoef * di.x;
+    dn.y = y1.y + dcoef * di.y;
     reinterpret_cast<V*>(d)[i] = dn;
     ww += (double)wi.x * wi.x + (double)wi.y * wi.y;
   }
@@ -238,17 +259,21 @@ __global__ void k_step2(const Loc loc, T* __restrict__ x, T* __restrict__ w, con
   span_partial(ww, rw, partial, sp);
 }
 
-// x += eta d;  y1 = w + beta y2 (not on the iteration that met the bound).
+// x += eta d (the second half step's update, also on the iteration that met
+// the bound); then v = u1 + beta (u2 + beta v) = aw + beta (2 u2 + beta v)
+// with u1 = A y1 = aw + beta u2; partial of rt.v.
 template <typename T, typename Loc>
-__global__ void k_step3(const Loc loc, T* __restrict__ x, const T* __restrict__ d, const T* __restrict__ w,
-                        const T* __restrict__ y2, T* __restrict__ y1, const TfqmrScal* s) {
+__global__ void k_v(const Loc loc, const T* __restrict__ aw, const T* __restrict__ u2, T* __restrict__ v,
+                    T* __restrict__ x, const T* __restrict__ d, const TfqmrScal* s, double* partial) {
   using V = typename W2<T>::type;
   const Span sp = loc.span();
   s += sp.q;
   if (s->flag) return;
-  { const int64_t o = sp.off; x += o; d += o; w += o; y2 += o; y1 += o; }
-  const T eta = (T)s->eta, beta = (T)s->beta;
+  const uint64_t seed = s->seed;
+  { const int64_t o = sp.off; aw += o; u2 += o; v += o; x += o; d += o; }
+  const T beta = (T)s->beta, eta = (T)s->eta;
   const bool next = s->stop == 0;
+  double sg = 0;
   SPAN_LOOP(i, sp) {
     V xi = reinterpret_cast<V*>(x)[i];
     const V di = reinterpret_cast<const V*>(d)[i];
@@ -256,35 +281,14 @@ __global__ void k_step3(const Loc loc, T* __restrict__ x, const T* __restrict__ 
     xi.y += eta * di.y;
     reinterpret_cast<V*>(x)[i] = xi;
     if (next) {
-      const V a = reinterpret_cast<const V*>(w)[i], b = reinterpret_cast<const V*>(y2)[i];
-      V o;
-      o.x = a.x + beta * b.x;
-      o.y = a.y + beta * b.y;
-      reinterpret_cast<V*>(y1)[i] = o;
+      const V a = reinterpret_cast<const V*>(aw)[i], b = reinterpret_cast<const V*>(u2)[i];
+      const V c = reinterpret_cast<const V*>(v)[i];
+      V t;
+      t.x = a.x + beta * ((T)2 * b.x + beta * c.x);
+      t.y = a.y + beta * ((T)2 * b.y + beta * c.y);
+      reinterpret_cast<V*>(v)[i] = t;
+      sg += shadow(2 * i, seed) * t.x + shadow(2 * i + 1, seed) * t.y;
     }
-  }
-}
-
-// v = u1 + beta (u2 + beta v); partial of rt.v
-template <typename T, typename Loc>
-__global__ void k_v(const Loc loc, const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ v,
-                    const TfqmrScal* s, double* partial) {
-  using V = typename W2<T>::type;
-  const Span sp = loc.span();
-  s += sp.q;
-  if (s->flag || s->stop) return;
-  const uint64_t seed = s->seed;
-  { const int64_t o = sp.off; u1 += o; u2 += o; v += o; }
-  const T beta = (T)s->beta;
-  double sg = 0;
-  SPAN_LOOP(i, sp) {
-    const V a = reinterpret_cast<const V*>(u1)[i], b = reinterpret_cast<const V*>(u2)[i];
-    const V c = reinterpret_cast<const V*>(v)[i];
-    V t;
-    t.x = a.x + beta * (b.x + beta * c.x);
-    t.y = a.y + beta * (b.y + beta * c.y);
-    reinterpret_cast<V*>(v)[i] = t;
-    sg += shadow(2 * i, seed) * t.x + shadow(2 * i + 1, seed) * t.y;
   }
   span_partial(sg, 0.0, partial, sp);
 }
@@ -363,7 +367,7 @@ __global__ void k_finalize(const Loc loc, int op, int nblk_uni, const double* __
     case F_HALF_RHO:   // second half step, then rho_new = rt.w = b
       half_step(s, a);
       s->iter += 1;
-      // bound met: the x update of this iteration still runs (k_step3); F_V then stops
+      // bound met: the x update of this iteration still runs (k_v); F_V then stops
       if (s->tau * sqrt(2.0 * s->iter + 1.0) <= s->target) { s->stop = 1; break; }
       if (s->rho == 0.0) { s->flag = 2; break; }
       s->beta = b / s->rho;
@@ -536,26 +540,26 @@ static hipError_t finalize(hipStream_t st, int op, int nblk, const double* parti
 
 // n counts reals; the vector kernels walk n/2 complex pairs of each problem
 template <typename T>
-hipError_t tfq_init_r(hipStream_t st, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, TfqmrScal* s,
+hipError_t tfq_init_r(hipStream_t st, int64_t n, const T* b, const T* ax, T* w, T* y2, T* d, TfqmrScal* s,
                       double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_init_r<T, UniLoc>), n2, b, ax, w, y1, d, s, partial);
+  LB((k_init_r<T, UniLoc>), n2, b, ax, w, y2, d, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_INIT_R, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
 template <typename T>
-hipError_t tfq_init_v(hipStream_t st, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial, Batch bt,
-                      T* u1_copy) {
+hipError_t tfq_init_v(hipStream_t st, int64_t n, const T* ax, T* v, T* aw, T* u2, TfqmrScal* s, double* partial,
+                      Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_init_v<T, UniLoc>), n2, u1, v, u1_copy, s, partial);
+  LB((k_init_v<T, UniLoc>), n2, ax, v, aw, u2, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_INIT_V, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
 template <typename T>
-hipError_t tfq_step1(hipStream_t st, int64_t n, const T* y1, const T* v, T* y2, T* w, const T* u1, T* d,
+hipError_t tfq_step1(hipStream_t st, int64_t n, T* w, T* y2, const T* v, const T* aw, const T* u2, T* d,
                      TfqmrScal* s, double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_step1<T, UniLoc>), n2, y1, v, y2, w, u1, d, s, partial);
+  LB((k_step1<T, UniLoc>), n2, w, y2, v, aw, u2, d, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_HALF, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
@@ -568,17 +572,10 @@ hipError_t tfq_step2(hipStream_t st, int64_t n, T* x, T* w, const T* u2, T* d, c
   return finalize(st, F_HALF_RHO, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
 template <typename T>
-hipError_t tfq_step3(hipStream_t st, int64_t n, T* x, const T* d, const T* w, const T* y2, T* y1,
-                     const TfqmrScal* s, Batch bt) {
+hipError_t tfq_v(hipStream_t st, int64_t n, const T* aw, const T* u2, T* v, T* x, const T* d, TfqmrScal* s,
+                 double* partial, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_step3<T, UniLoc>), n2, x, d, w, y2, y1, s);
-  return hipGetLastError();
-}
-template <typename T>
-hipError_t tfq_v(hipStream_t st, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial,
-                 Batch bt) {
-  const int64_t n2 = n / 2;
-  LB((k_v<T, UniLoc>), n2, u1, u2, v, s, partial);
+  LB((k_v<T, UniLoc>), n2, aw, u2, v, x, d, s, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_V, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
@@ -670,19 +667,19 @@ static hipError_t finalize_m(hipStream_t st, const Multi& m, int op, const doubl
   return hipGetLastError();
 }
 
-hipError_t tfq_init_r(hipStream_t st, const Multi& m, const double* b, const double* ax, double* w, double* y1,
+hipError_t tfq_init_r(hipStream_t st, const Multi& m, const double* b, const double* ax, double* w, double* y2,
                       double* d, TfqmrScal* s, double* partial) {
-  LM((k_init_r<double, MultiLoc>), b, ax, w, y1, d, s, partial);
+  LM((k_init_r<double, MultiLoc>), b, ax, w, y2, d, s, partial);
   return finalize_m(st, m, F_INIT_R, partial, s, nullptr);
 }
-hipError_t tfq_init_v(hipStream_t st, const Multi& m, const double* u1, double* v, TfqmrScal* s, double* partial,
-                      double* u1_copy) {
-  LM((k_init_v<double, MultiLoc>), u1, v, u1_copy, s, partial);
+hipError_t tfq_init_v(hipStream_t st, const Multi& m, const double* ax, double* v, double* aw, double* u2,
+                      TfqmrScal* s, double* partial) {
+  LM((k_init_v<double, MultiLoc>), ax, v, aw, u2, s, partial);
   return finalize_m(st, m, F_INIT_V, partial, s, nullptr);
 }
-hipError_t tfq_step1(hipStream_t st, const Multi& m, const double* y1, const double* v, double* y2, double* w,
-                     const double* u1, double* d, TfqmrScal* s, double* partial) {
-  LM((k_step1<double, MultiLoc>), y1, v, y2, w, u1, d, s, partial);
+hipError_t tfq_step1(hipStream_t st, const Multi& m, double* w, double* y2, const double* v, const double* aw,
+                     const double* u2, double* d, TfqmrScal* s, double* partial) {
+  LM((k_step1<double, MultiLoc>), w, y2, v, aw, u2, d, s, partial);
   return finalize_m(st, m, F_HALF, partial, s, nullptr);
 }
 hipError_t tfq_step2(hipStream_t st, const Multi& m, double* x, double* w, const double* u2, double* d,
@@ -690,14 +687,9 @@ hipError_t tfq_step2(hipStream_t st, const Multi& m, double* x, double* w, const
   LM((k_step2<double, MultiLoc>), x, w, u2, d, y2, s, partial);
   return finalize_m(st, m, F_HALF_RHO, partial, s, nullptr);
 }
-hipError_t tfq_step3(hipStream_t st, const Multi& m, double* x, const double* d, const double* w, const double* y2,
-                     double* y1, const TfqmrScal* s) {
-  LM((k_step3<double, MultiLoc>), x, d, w, y2, y1, s);
-  return hipGetLastError();
-}
-hipError_t tfq_v(hipStream_t st, const Multi& m, const double* u1, const double* u2, double* v, TfqmrScal* s,
-                 double* partial) {
-  LM((k_v<double, MultiLoc>), u1, u2, v, s, partial);
+hipError_t tfq_v(hipStream_t st, const Multi& m, const double* aw, const double* u2, double* v, double* x,
+                 const double* d, TfqmrScal* s, double* partial) {
+  LM((k_v<double, MultiLoc>), aw, u2, v, x, d, s, partial);
   return finalize_m(st, m, F_V, partial, s, nullptr);
 }
 hipError_t vec_resid(hipStream_t st, const Multi& m, const double* b, const double* ax, double* out,
@@ -709,14 +701,14 @@ hipError_t vec_resid(hipStream_t st, const Multi& m, const double* b, const doub
 #define INST(T)                                                                                          \
   template hipError_t tfq_init_r<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, T*, TfqmrScal*,     \
                                     double*, Batch);                                                    \
-  template hipError_t tfq_init_v<T>(hipStream_t, int64_t, const T*, T*, TfqmrScal*, double*, Batch, T*); \
-  template hipError_t tfq_step1<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, const T*, T*,       \
+  template hipError_t tfq_init_v<T>(hipStream_t, int64_t, const T*, T*, T*, T*, TfqmrScal*, double*,     \
+                                    Batch);                                                             \
+  template hipError_t tfq_step1<T>(hipStream_t, int64_t, T*, T*, const T*, const T*, const T*, T*,       \
                                    TfqmrScal*, double*, Batch);                                         \
   template hipError_t tfq_step2<T>(hipStream_t, int64_t, T*, T*, const T*, T*, const T*, TfqmrScal*,     \
                                    double*, Batch);                                                     \
-  template hipError_t tfq_step3<T>(hipStream_t, int64_t, T*, const T*, const T*, const T*, T*,           \
-                                   const TfqmrScal*, Batch);                                            \
-  template hipError_t tfq_v<T>(hipStream_t, int64_t, const T*, const T*, T*, TfqmrScal*, double*, Batch); \
+  template hipError_t tfq_v<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, const T*, TfqmrScal*,   \
+                               double*, Batch);                                                         \
   template hipError_t vec_fill<T>(hipStream_t, int64_t, T*, T);                                           \
   template hipError_t vec_randn<T>(hipStream_t, int64_t, T*, uint64_t);
 INST(double)

@@ -81,26 +81,28 @@ hipError_t vec_project(hipStream_t, int64_t nado, int d, T* x, const int8_t* par
                        bool hermitian);
 
 // One TFQMR iteration (Freund 1993) is: tfq_step1, A y2 -> u2, tfq_step2,
-// tfq_step3, A y1 -> u1, tfq_v.  Every reduction is followed by its one-block
-// finalize; the shadow vector is generated in the kernels.
+// A w -> aw, tfq_v.  The operator is applied to w rather than to y1 = w +
+// beta y2 (A y1 = aw + beta u2), so y1 and u1 = A y1 are formed inside
+// tfq_step1 and tfq_v and never stored.  A (re)start is tfq_init_r (r = b -
+// A x into w; y2 = d = 0), A w -> ax, tfq_init_v (v = ax, aw = ax when aw is
+// given; u2 = 0).  Every
+// reduction is followed by its one-block finalize; the shadow vector is
+// generated in the kernels.
 template <typename T>
-hipError_t tfq_init_r(hipStream_t, int64_t n, const T* b, const T* ax, T* w, T* y1, T* d, TfqmrScal* s,
+hipError_t tfq_init_r(hipStream_t, int64_t n, const T* b, const T* ax, T* w, T* y2, T* d, TfqmrScal* s,
                       double* partial, Batch bt = Batch{});
 template <typename T>
-hipError_t tfq_init_v(hipStream_t, int64_t n, const T* u1, T* v, TfqmrScal* s, double* partial,
-                      Batch bt = Batch{}, T* u1_copy = nullptr);
+hipError_t tfq_init_v(hipStream_t, int64_t n, const T* ax, T* v, T* aw, T* u2, TfqmrScal* s, double* partial,
+                      Batch bt = Batch{});
 template <typename T>
-hipError_t tfq_step1(hipStream_t, int64_t n, const T* y1, const T* v, T* y2, T* w, const T* u1, T* d,
+hipError_t tfq_step1(hipStream_t, int64_t n, T* w, T* y2, const T* v, const T* aw, const T* u2, T* d,
                      TfqmrScal* s, double* partial, Batch bt = Batch{});
 template <typename T>
 hipError_t tfq_step2(hipStream_t, int64_t n, T* x, T* w, const T* u2, T* d, const T* y2, TfqmrScal* s,
                      double* partial, Batch bt = Batch{});
 template <typename T>
-hipError_t tfq_step3(hipStream_t, int64_t n, T* x, const T* d, const T* w, const T* y2, T* y1,
-                     const TfqmrScal* s, Batch bt = Batch{});
-template <typename T>
-hipError_t tfq_v(hipStream_t, int64_t n, const T* u1, const T* u2, T* v, TfqmrScal* s, double* partial,
-                 Batch bt = Batch{});
+hipError_t tfq_v(hipStream_t, int64_t n, const T* aw, const T* u2, T* v, T* x, const T* d, TfqmrScal* s,
+                 double* partial, Batch bt = Batch{});
 template <typename T>
 hipError_t vec_fill(hipStream_t, int64_t n, T* v, T val);
 template <typename T>
@@ -120,18 +122,16 @@ hipError_t tfq_set_unscaled(hipStream_t, TfqmrScal* s, const int32_t* mask, int 
 hipError_t tfq_scale_targets(hipStream_t, TfqmrScal* s, const double* rr_true, int nprob);
 // The same steps over every problem of a Multi layout (FP64; partial holds
 // two doubles per flat-grid block, out two per problem).
-hipError_t tfq_init_r(hipStream_t, const Multi&, const double* b, const double* ax, double* w, double* y1, double* d,
+hipError_t tfq_init_r(hipStream_t, const Multi&, const double* b, const double* ax, double* w, double* y2, double* d,
                       TfqmrScal* s, double* partial);
-hipError_t tfq_init_v(hipStream_t, const Multi&, const double* u1, double* v, TfqmrScal* s, double* partial,
-                      double* u1_copy);
-hipError_t tfq_step1(hipStream_t, const Multi&, const double* y1, const double* v, double* y2, double* w,
-                     const double* u1, double* d, TfqmrScal* s, double* partial);
+hipError_t tfq_init_v(hipStream_t, const Multi&, const double* ax, double* v, double* aw, double* u2,
+                      TfqmrScal* s, double* partial);
+hipError_t tfq_step1(hipStream_t, const Multi&, double* w, double* y2, const double* v, const double* aw,
+                     const double* u2, double* d, TfqmrScal* s, double* partial);
 hipError_t tfq_step2(hipStream_t, const Multi&, double* x, double* w, const double* u2, double* d, const double* y2,
                      TfqmrScal* s, double* partial);
-hipError_t tfq_step3(hipStream_t, const Multi&, double* x, const double* d, const double* w, const double* y2,
-                     double* y1, const TfqmrScal* s);
-hipError_t tfq_v(hipStream_t, const Multi&, const double* u1, const double* u2, double* v, TfqmrScal* s,
-                 double* partial);
+hipError_t tfq_v(hipStream_t, const Multi&, const double* aw, const double* u2, double* v, double* x,
+                 const double* d, TfqmrScal* s, double* partial);
 hipError_t vec_resid(hipStream_t, const Multi&, const double* b, const double* ax, double* out, double* partial);
 hipError_t vec_set_elem(hipStream_t, double* v, int64_t off, double val);
 // A few scattered values written into every problem's copy of a vector.

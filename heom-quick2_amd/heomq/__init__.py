@@ -471,6 +471,11 @@ class Context:
         steady solves (default on)."""
         check(lib().heomq_ctx_set_presmooth(self._h, int(enable)))
 
+    def set_precond_fusion(self, enable: bool):
+        """Block-Jacobi-preconditioned applies as one fused launch where the
+        apply form allows it (default on; the same bits as the two kernels)."""
+        check(lib().heomq_ctx_set_precond_fusion(self._h, int(enable)))
+
     def set_trun_der(self, enable: bool = True):
         """The derivative terminator of the top tier ($jobinfo ltrun_der,
         calc_ax_tfqmr_omp.f90:362-478): the forward packed applies and the
@@ -488,6 +493,12 @@ class Context:
         clean = [None if np.isnan(v) else float(v) for v in times]
         return dict(forward=int(forms[0]), transposed=int(forms[1]),
                     times_us={"forward": clean[:8], "transposed": clean[8:]})
+
+    def apply_form(self):
+        """The packed apply's forms in use (forward, transposed)."""
+        forms = np.zeros(2, np.int32)
+        check(lib().heomq_ctx_get_apply_form(self._h, forms.ctypes.data_as(_lib.I32)))
+        return int(forms[0]), int(forms[1])
 
     def set_apply_form(self, forward: int, transposed: int | None = None):
         """Fix the packed apply's form (bits as autotune); returns the forms in
@@ -545,7 +556,8 @@ class Context:
 
     def apply_packed_precond(self, xp, yp=None, trace=TRACE_RDO, form=1):
         """y = M^-1 (Pi A x) on packed vectors with the block-Jacobi preconditioner
-        (form 1: the solver's one-kernel inverse; 2: two-kernel transform; 3: LDS-window kernel)."""
+        (form 1: apply, then the one-kernel inverse; 2: two-kernel transform; 3: LDS-window kernel;
+        4: apply and inverse fused in one launch)."""
         t = self.torch
         yp = self.packed_zeros() if yp is None else yp
         check(lib().heomq_apply_packed_precond(self._h, self._check_packed(xp, t.complex128),
@@ -752,7 +764,8 @@ class Context:
         bp = self._check_vec(b, t.complex128) if b is not None else None
         rc = lib().heomq_solve_steady(self._h, self._check_vec(x, t.complex128), bp, tol, maxiter, trace,
                                       int(transpose), _dp(stats), self._stream())
-        info = dict(iters=int(stats[0]), applies=int(stats[1]), relres=stats[2], seconds=stats[3], status=rc)
+        info = dict(iters=int(stats[0]), applies=int(stats[1]), relres=stats[2], seconds=stats[3], status=rc,
+                    precond=int(stats[4]))
         if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
             check(rc)
         return x, info

@@ -123,6 +123,7 @@ SIGNATURES = {
     "heomq_ctx_set_system_field": (C.c_int, [P, C.c_int, D, I32, D]),
     "heomq_ctx_field_shifts": (C.c_int, [P, C.c_double, D]),
     "heomq_ctx_set_presmooth": (C.c_int, [P, C.c_int]),
+    "heomq_ctx_set_precond_fusion": (C.c_int, [P, C.c_int]),
     "heomq_ctx_set_trun_der": (C.c_int, [P, C.c_int]),
     "heomq_ctx_set_apply_form": (C.c_int, [P, C.c_int, C.c_int]),
     "heomq_ctx_get_apply_form": (C.c_int, [P, I32]),

@@ -188,6 +188,13 @@ int heomq_ctx_set_trun_der(heomq_ctx* c, int enable);
  * (enable = 1, the default); 0 starts TFQMR from the guess as given.  No
  * counterpart in the reference. */
 int heomq_ctx_set_presmooth(heomq_ctx* c, int enable);
+/* Block-Jacobi-preconditioned packed applies (the steady solve's iteration,
+ * heomq_apply_packed_precond) run as one fused launch, the inverse applied to
+ * each workgroup's ADOs before anything is written, when the forward apply
+ * form has the sector-batched top tier and sliced heavy rows (enable = 1, the
+ * default; the results are the same bits); 0 always runs the apply, then the
+ * inverse.  No counterpart in the reference. */
+int heomq_ctx_set_precond_fusion(heomq_ctx* c, int enable);
 /* Left preconditioning of the forward steady-state TFQMR on the packed
  * layout: 0 none; 1 Jacobi (diagonal) scaling, default right-hand side only;
  * 2 block Jacobi, (shift - S_K)^-1 per ADO with S_K the same-tier operator
@@ -211,7 +218,11 @@ int heomq_ctx_set_shadow_seed(heomq_ctx* c, uint64_t seed);
  * workgroup, when every such ADO but the tier-0 one holds at most 16
  * elements; 0: one workgroup per ADO with the couplings sliced over the
  * lanes, top tier in packed-element order) over `reps` launches on this GPU,
- * for the forward and the transposed operator, and keeps the fastest.
+ * for the forward and the transposed operator, and keeps the fastest.  The
+ * forward forms are timed on the operator the steady solve iterates with:
+ * where that is block-Jacobi preconditioned (the default when H couples Fock
+ * states), the apply and its inverse, one fused launch in the forms that
+ * allow it (heomq_ctx_set_precond_fusion).
  * forms[0..1] (may be NULL) receive the choices, times_us[0..15] (may be
  * NULL) the times, form f of operator w at [8 w + f] (forms the layout does
  * not take are left untouched).  Every form computes the same operator.  No
@@ -259,7 +270,10 @@ int heomq_apply_packed_raw_f32(heomq_ctx* c, const float* xp, float* yp, int tra
  * of whole ADOs when d <= 16 and H couples groups of <= 4 states, else the
  * LDS-window kernel; HEOMQ_UNSUPPORTED when neither fits the layout); 2:
  * apply, then the two-kernel transform; 3: apply, then the LDS-window kernel
- * (when every ADO range a 256-element block touches fits its LDS).  All forms
+ * (when every ADO range a 256-element block touches fits its LDS); 4: one
+ * fused launch, the inverse applied in each apply workgroup to its whole
+ * ADOs (apply forms 2 and 6 without the wave form, d <= 16, the ADO-aligned
+ * inverse; the same bits as form 1), HEOMQ_UNSUPPORTED otherwise.  All forms
  * compute the same values.  No counterpart in the reference. */
 int heomq_apply_packed_precond(heomq_ctx* c, const double* xp, double* yp, int trace, int form, void* stream);
 /* The same operator as heomq_apply_packed_precond form 1, with no allocation:
@@ -305,8 +319,9 @@ int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host
  * ||b - A x||_2 <= tol * ||b||_2 on the (Re, Im) vector.  transpose = 1 solves
  * with A^T (the adjoint solve).  b may be NULL (then b = e_trace).
  * stats (HEOMQ_STATS_LEN): [0] iterations, [1] applies, [2] final relative
- * residual, [3] seconds, [4] 0, [5] length of the real vector iterated on,
- * [6..7] 0.  Returns HEOMQ_NOT_CONVERGED if maxiter hit.
+ * residual, [3] seconds, [4] the left preconditioner the iteration ended
+ * with (0 none, 1 Jacobi scaling, 2 block Jacobi), [5] length of the real
+ * vector iterated on, [6..7] 0.  Returns HEOMQ_NOT_CONVERGED if maxiter hit.
  * ------------------------------------------------------------------------- */
 int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol,
                        int maxiter, int trace, int transpose, double* stats,
