@@ -36,6 +36,7 @@ struct heomq_batch {
   heomq_model* shape = nullptr;   // problem 0's model (structure and shared host tables)
   heomq_ctx* ctx = nullptr;       // device tables of the shape
   DevBuf rec_all, gam_all, hq_union, hc_all, work, scal, partial, out, mask;
+  DevBuf scal2, partial2;         // TFQMR's mid-iteration scalar copies and second partial sums
   DevPacked bf;                   // batched forward operator
   std::vector<std::vector<double>> dbsqrt;   // per problem: lscale factors (1 unscaled), for the observables
   int64_t n = 0;                  // reals per problem vector (2 * packed elements)
@@ -71,6 +72,7 @@ struct heomq_sweep {
   int covered = 0;
   DevBuf pmap;                        // per shape: in-shape indices of the covered problems
   DevBuf work, scal, partial, out, mask;
+  DevBuf scal2, partial2;
   int64_t nt = 0;                     // reals of one vector over every problem
   hipStream_t stream = nullptr;
   hipGraphExec_t chunk = nullptr;
@@ -206,6 +208,8 @@ int batch_build(const heomq_params* params, int nprob, int device, bool workspac
     HIPCHECK(B->work.alloc(sizeof(double) * (size_t)B->n * nprob * kTfqVecs));
     HIPCHECK(B->scal.alloc(sizeof(TfqmrScal) * (size_t)nprob));
     HIPCHECK(B->partial.alloc(sizeof(double) * 2 * kMaxRedBlocks * (size_t)nprob));
+    HIPCHECK(B->scal2.alloc(sizeof(TfqmrScal) * (size_t)nprob));
+    HIPCHECK(B->partial2.alloc(sizeof(double) * 2 * kMaxRedBlocks * (size_t)nprob));
     HIPCHECK(B->out.alloc(sizeof(double) * 2 * (size_t)nprob));
     HIPCHECK(B->mask.alloc(2 * sizeof(int32_t) * (size_t)nprob));
     HIPCHECK(hipHostMalloc(&B->pinned,
@@ -222,8 +226,8 @@ int batch_build(const heomq_params* params, int nprob, int device, bool workspac
 // Device vectors, scalars and host mirrors of one solve over P problems.
 struct TfqBufs {
   double *x, *b, *w, *y2, *u2, *aw, *v, *dv, *ax, *xbest;
-  TfqmrScal* s;
-  double *part, *outv;
+  TfqmrScal *s, *s2;
+  double *part, *part2, *outv;
   int32_t *mask, *mask2;
   TfqmrScal* hs;   // pinned host copies
   double* hout;
@@ -231,13 +235,15 @@ struct TfqBufs {
   int64_t nt;      // reals of one vector over every problem
 };
 
-TfqBufs carve_bufs(double* base, int64_t nt, TfqmrScal* s, double* part, double* outv, int32_t* mask, void* pinned,
-                   int P) {
+TfqBufs carve_bufs(double* base, int64_t nt, TfqmrScal* s, TfqmrScal* s2, double* part, double* part2, double* outv,
+                   int32_t* mask, void* pinned, int P) {
   TfqBufs V;
   double** vec[kTfqVecs] = {&V.x, &V.b, &V.w, &V.y2, &V.u2, &V.aw, &V.v, &V.dv, &V.ax, &V.xbest};
   for (int k = 0; k < kTfqVecs; ++k) *vec[k] = base + (int64_t)k * nt;
   V.s = s;
+  V.s2 = s2;
   V.part = part;
+  V.part2 = part2;
   V.outv = outv;
   V.mask = mask;
   V.mask2 = mask + P;
@@ -295,9 +301,9 @@ int tfqmr_loop(Lay& L, const TfqBufs& V, hipGraphExec_t& chunk, hipStream_t st, 
     for (int k = 0; k < check_every && rec == hipSuccess; ++k) {
       if ((rec = L.step1(V.w, V.y2, V.v, V.aw, V.u2, V.dv, V.s, V.part))) break;
       if ((rec = L.Ait(V.y2, V.u2))) break;
-      if ((rec = L.step2(V.x, V.w, V.u2, V.dv, V.y2, V.s, V.part))) break;
+      if ((rec = L.step2(V.x, V.w, V.u2, V.dv, V.y2, V.s, V.s2, V.part, V.part2))) break;
       if ((rec = L.Ait(V.w, V.aw))) break;
-      rec = L.vstep(V.aw, V.u2, V.v, V.x, V.dv, V.s, V.part);
+      rec = L.vstep(V.aw, V.u2, V.v, V.x, V.dv, V.s, V.s2, V.part, V.part2);
     }
     hipGraph_t g = nullptr;
     const hipError_t end = hipStreamEndCapture(st, &g);
@@ -496,12 +502,13 @@ struct BatchLayout {
                    TfqmrScal* s, double* part) {
     return tfq_step1<double>(st, B->n, w, y2, v, aw, u2, d, s, part, bt);
   }
-  hipError_t step2(double* x, double* w, const double* u2, double* d, const double* y2, TfqmrScal* s, double* part) {
-    return tfq_step2<double>(st, B->n, x, w, u2, d, y2, s, part, bt);
+  hipError_t step2(double* x, double* w, const double* u2, double* d, const double* y2, TfqmrScal* s,
+                   TfqmrScal* s2, const double* part, double* part2) {
+    return tfq_step2<double>(st, B->n, x, w, u2, d, y2, s, s2, part, part2, bt);
   }
   hipError_t vstep(const double* aw, const double* u2, double* v, double* x, const double* d, TfqmrScal* s,
-                   double* part) {
-    return tfq_v<double>(st, B->n, aw, u2, v, x, d, s, part, bt);
+                   const TfqmrScal* s2, double* part, const double* part2) {
+    return tfq_v<double>(st, B->n, aw, u2, v, x, d, s, s2, part, part2, bt);
   }
   hipError_t start(double* x, double* b) { return shape_start(B->ctx, op_it, x, b, B->n, B->nprob, st); }
   hipError_t herm(int, double* xq) { return launch_herm_packed<double>(B->ctx->pfwd, xq, st); }
@@ -550,12 +557,13 @@ struct SweepLayout {
                    TfqmrScal* s, double* part) {
     return tfq_step1(st, W->cur, w, y2, v, aw, u2, d, s, part);
   }
-  hipError_t step2(double* x, double* w, const double* u2, double* d, const double* y2, TfqmrScal* s, double* part) {
-    return tfq_step2(st, W->cur, x, w, u2, d, y2, s, part);
+  hipError_t step2(double* x, double* w, const double* u2, double* d, const double* y2, TfqmrScal* s,
+                   TfqmrScal* s2, const double* part, double* part2) {
+    return tfq_step2(st, W->cur, x, w, u2, d, y2, s, s2, part, part2);
   }
   hipError_t vstep(const double* aw, const double* u2, double* v, double* x, const double* d, TfqmrScal* s,
-                   double* part) {
-    return tfq_v(st, W->cur, aw, u2, v, x, d, s, part);
+                   const TfqmrScal* s2, double* part, const double* part2) {
+    return tfq_v(st, W->cur, aw, u2, v, x, d, s, s2, part, part2);
   }
   hipError_t start(double* x, double* b) {
     for (int g = 0; g < W->ng; ++g) {
@@ -660,8 +668,9 @@ int heomq_batch_solve(heomq_batch* B, double tol, int maxiter, double* rdo, doub
   BatchLayout L{B, B->stream, B->bf, B->bf, Batch{P, B->n}};
   L.op_it.precond = B->precond ? 1 : 0;   // b = e_trace sits on the unscaled trace row
   L.op0.precond = 0;
-  const TfqBufs V = carve_bufs(B->work.as<double>(), B->n * P, B->scal.as<TfqmrScal>(), B->partial.as<double>(),
-                               B->out.as<double>(), B->mask.as<int32_t>(), B->pinned, P);
+  const TfqBufs V = carve_bufs(B->work.as<double>(), B->n * P, B->scal.as<TfqmrScal>(), B->scal2.as<TfqmrScal>(),
+                               B->partial.as<double>(), B->partial2.as<double>(), B->out.as<double>(),
+                               B->mask.as<int32_t>(), B->pinned, P);
   SolveOut so;
   if (int rc = tfqmr_loop(L, V, B->chunk, B->stream, tol, maxiter, so)) return rc;
   const int nal = c->nalf * c->nspin, noc = c->norbs * c->nspin;
@@ -755,6 +764,8 @@ int heomq_sweep_create(const heomq_params* params, int nprob, int device, heomq_
   HIPCHECK(W->work.alloc(sizeof(double) * (size_t)W->nt * kTfqVecs));
   HIPCHECK(W->scal.alloc(sizeof(TfqmrScal) * (size_t)nprob));
   HIPCHECK(W->partial.alloc(sizeof(double) * 2 * (size_t)vb));
+  HIPCHECK(W->scal2.alloc(sizeof(TfqmrScal) * (size_t)nprob));
+  HIPCHECK(W->partial2.alloc(sizeof(double) * 2 * (size_t)vb));
   HIPCHECK(W->out.alloc(sizeof(double) * 2 * (size_t)nprob));
   HIPCHECK(W->mask.alloc(2 * sizeof(int32_t) * (size_t)nprob));
   HIPCHECK(W->pmap.alloc(sizeof(int32_t) * (size_t)nprob));
@@ -842,8 +853,9 @@ int heomq_sweep_solve(heomq_sweep* W, double tol, int maxiter, double* rdo, doub
   W->cur0 = W->am0;
   W->covered = P;
   SweepLayout L{W, W->stream};
-  const TfqBufs V = carve_bufs(W->work.as<double>(), W->nt, W->scal.as<TfqmrScal>(), W->partial.as<double>(),
-                               W->out.as<double>(), W->mask.as<int32_t>(), W->pinned, P);
+  const TfqBufs V = carve_bufs(W->work.as<double>(), W->nt, W->scal.as<TfqmrScal>(), W->scal2.as<TfqmrScal>(),
+                               W->partial.as<double>(), W->partial2.as<double>(), W->out.as<double>(),
+                               W->mask.as<int32_t>(), W->pinned, P);
   SolveOut so;
   if (int rc = tfqmr_loop(L, V, W->chunk, W->stream, tol, maxiter, so)) return rc;
   for (int q = 0; q < P; ++q) {

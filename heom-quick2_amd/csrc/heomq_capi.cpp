@@ -1231,6 +1231,9 @@ int heomq_ctx_create(const heomq_model* h, int device, heomq_ctx** out) {
   HIPCHECK(c->scal.alloc(sizeof(TfqmrScal) * 2));
   HIPCHECK(hipMemset(c->scal.p, 0, sizeof(TfqmrScal) * 2));
   HIPCHECK(c->partial.alloc(sizeof(double) * 2 * kMaxRedBlocks));
+  HIPCHECK(c->scal2.alloc(sizeof(TfqmrScal)));
+  HIPCHECK(hipMemset(c->scal2.p, 0, sizeof(TfqmrScal)));
+  HIPCHECK(c->partial2.alloc(sizeof(double) * 2 * kMaxRedBlocks));
   HIPCHECK(c->red.alloc(sizeof(double) * 4 + sizeof(unsigned) * 4));
   HIPCHECK(hipMemset(c->red.p, 0, c->red.bytes));
   HIPCHECK(hipDeviceSynchronize());

@@ -133,13 +133,15 @@ struct RunStats {
 template <typename T, typename Apply>
 hipError_t record_chunk(heomq_ctx* c, T* x, const Work<T>& W, int64_t n, hipStream_t st, Apply& A, int iters) {
   TfqmrScal* s = c->scal.as<TfqmrScal>();
+  TfqmrScal* s2 = c->scal2.as<TfqmrScal>();
   double* part = c->partial.as<double>();
+  double* part2 = c->partial2.as<double>();
   for (int k = 0; k < iters; ++k) {
     if (hipError_t e = tfq_step1<T>(st, n, W.w, W.y2, W.v, W.aw, W.u2, W.d, s, part)) return e;
     if (hipError_t e = A(W.y2, W.u2)) return e;
-    if (hipError_t e = tfq_step2<T>(st, n, x, W.w, W.u2, W.d, W.y2, s, part)) return e;
+    if (hipError_t e = tfq_step2<T>(st, n, x, W.w, W.u2, W.d, W.y2, s, s2, part, part2)) return e;
     if (hipError_t e = A(W.w, W.aw)) return e;
-    if (hipError_t e = tfq_v<T>(st, n, W.aw, W.u2, W.v, x, W.d, s, part)) return e;
+    if (hipError_t e = tfq_v<T>(st, n, W.aw, W.u2, W.v, x, W.d, s, s2, part, part2)) return e;
   }
   return hipSuccess;
 }

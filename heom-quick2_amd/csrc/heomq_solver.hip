@@ -110,6 +110,27 @@ __device__ inline void span_partial(double a, double b, double* partial, const S
   }
 }
 
+// A reduction's phase 2 folded into the kernel that consumes it: every
+// workgroup sums its problem's partial sums (nblk pairs) in the order of
+// k_finalize's 256-thread fold, so every workgroup (and the separate
+// finalize it replaces) gets the same bits.  Every thread receives (a, b).
+__device__ inline void fold_partials(const double* __restrict__ partial, int64_t nblk, double& a, double& b) {
+  __shared__ double fa, fb;
+  double x = 0, y = 0;
+  for (int64_t t = threadIdx.x; t < nblk; t += blockDim.x) {
+    x += partial[2 * t];
+    y += partial[2 * t + 1];
+  }
+  block_sum2(x, y);
+  if (threadIdx.x == 0) {
+    fa = x;
+    fb = y;
+  }
+  __syncthreads();
+  a = fa;
+  b = fb;
+}
+
 // Shadow vector of TFQMR, generated where it is used instead of stored: a
 // +-1 sign per real element from a splitmix64 hash of its index (the same
 // bits for any grid and any call).
@@ -119,6 +140,34 @@ __device__ inline double shadow(int64_t k, uint64_t seed) {
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
   return (z >> 63) ? -1.0 : 1.0;
+}
+
+// One quasi-minimisation half step from ||w||^2 = a.
+__device__ inline void half_step(TfqmrScal* s, double a) {
+  const double theta = sqrt(a) / s->tau;
+  const double c = 1.0 / sqrt(1.0 + theta * theta);
+  s->tau = s->tau * theta * c;
+  s->theta = theta;
+  s->eta = c * c * s->alpha;
+  s->dcoef = theta * theta * s->eta / s->alpha;
+}
+
+// The second half step of an iteration, then rho_new = rt.w = b: the bound
+// met sets stop (the x update of the iteration still runs, k_v; the final
+// finalize then stops), rho = 0 is a breakdown.
+__device__ inline void half_step_rho(TfqmrScal* s, double a, double b) {
+  half_step(s, a);
+  s->iter += 1;
+  if (s->tau * sqrt(2.0 * s->iter + 1.0) <= s->target) {
+    s->stop = 1;
+    return;
+  }
+  if (s->rho == 0.0) {
+    s->flag = 2;
+    return;
+  }
+  s->beta = b / s->rho;
+  s->rho = b;
 }
 
 // The iteration is kept in a form that applies the operator to w instead of
@@ -224,18 +273,32 @@ __global__ void k_step1(const Loc loc, T* __restrict__ w, T* __restrict__ y2, co
   span_partial(ww, 0.0, partial, sp);
 }
 
-// Second half: x += eta d;  w -= alpha u2;  d = y2 + dcoef d;  partials of
-// ||w||^2 and rt.w (w is final for this iteration).
+// Second half, with the first half step's scalars folded in: (theta, tau,
+// eta, dcoef) from step 1's partial sums of ||w||^2 (part_in), written to the
+// problem's second scalar copy s_out by its first workgroup (s_out mirrors s
+// for a stopped or frozen problem); then x += eta d;  w -= alpha u2;
+// d = y2 + dcoef d;  partials of ||w||^2 and rt.w (w is final for this
+// iteration).
 template <typename T, typename Loc>
 __global__ void k_step2(const Loc loc, T* __restrict__ x, T* __restrict__ w, const T* __restrict__ u2,
-                        T* __restrict__ d, const T* __restrict__ y2, const TfqmrScal* s, double* partial) {
+                        T* __restrict__ d, const T* __restrict__ y2, const TfqmrScal* s, TfqmrScal* s_out,
+                        const double* __restrict__ part_in, double* partial) {
   using V = typename W2<T>::type;
   const Span sp = loc.span();
   s += sp.q;
-  if (s->flag) return;
-  const uint64_t seed = s->seed;
+  s_out += sp.q;
+  TfqmrScal t = *s;
+  if (t.flag) {   // uniform over the problem's workgroups
+    if (sp.blk == 0 && threadIdx.x == 0) *s_out = t;
+    return;
+  }
+  double a, unused;
+  fold_partials(part_in + 2 * (sp.pidx - sp.blk), sp.nblk, a, unused);
+  half_step(&t, a);
+  if (sp.blk == 0 && threadIdx.x == 0) *s_out = t;
+  const uint64_t seed = t.seed;
   { const int64_t o = sp.off; x += o; w += o; u2 += o; d += o; y2 += o; }
-  const T alpha = (T)s->alpha, dcoef = (T)s->dcoef, eta = (T)s->eta;
+  const T alpha = (T)t.alpha, dcoef = (T)t.dcoef, eta = (T)t.eta;
   double ww = 0, rw = 0;
   SPAN_LOOP(i, sp) {
     const V di = reinterpret_cast<const V*>(d)[i];
@@ -259,20 +322,31 @@ __global__ void k_step2(const Loc loc, T* __restrict__ x, T* __restrict__ w, con
   span_partial(ww, rw, partial, sp);
 }
 
-// x += eta d (the second half step's update, also on the iteration that met
-// the bound); then v = u1 + beta (u2 + beta v) = aw + beta (2 u2 + beta v)
-// with u1 = A y1 = aw + beta u2; partial of rt.v.
+// The second half step's scalars folded in: from step 2's partial sums
+// (part_in) and the scalar copy step 2 wrote (s_in), the half step, the stop
+// test and beta, written back to the problem's scalars s_out by its first
+// workgroup.  Then x += eta d (also on the iteration that met the bound);
+// v = u1 + beta (u2 + beta v) = aw + beta (2 u2 + beta v) with u1 = A y1 =
+// aw + beta u2; partial of rt.v.
 template <typename T, typename Loc>
 __global__ void k_v(const Loc loc, const T* __restrict__ aw, const T* __restrict__ u2, T* __restrict__ v,
-                    T* __restrict__ x, const T* __restrict__ d, const TfqmrScal* s, double* partial) {
+                    T* __restrict__ x, const T* __restrict__ d, const TfqmrScal* s_in, TfqmrScal* s_out,
+                    const double* __restrict__ part_in, double* partial) {
   using V = typename W2<T>::type;
   const Span sp = loc.span();
-  s += sp.q;
-  if (s->flag) return;
-  const uint64_t seed = s->seed;
+  s_in += sp.q;
+  s_out += sp.q;
+  TfqmrScal t = *s_in;
+  if (t.flag) return;   // s_out already holds this state
+  double a, b;
+  fold_partials(part_in + 2 * (sp.pidx - sp.blk), sp.nblk, a, b);
+  half_step_rho(&t, a, b);
+  if (sp.blk == 0 && threadIdx.x == 0) *s_out = t;
+  if (t.flag) return;   // breakdown: no update (uniform over the problem's workgroups)
+  const uint64_t seed = t.seed;
   { const int64_t o = sp.off; aw += o; u2 += o; v += o; x += o; d += o; }
-  const T beta = (T)s->beta, eta = (T)s->eta;
-  const bool next = s->stop == 0;
+  const T beta = (T)t.beta, eta = (T)t.eta;
+  const bool next = t.stop == 0;
   double sg = 0;
   SPAN_LOOP(i, sp) {
     V xi = reinterpret_cast<V*>(x)[i];
@@ -281,13 +355,13 @@ __global__ void k_v(const Loc loc, const T* __restrict__ aw, const T* __restrict
     xi.y += eta * di.y;
     reinterpret_cast<V*>(x)[i] = xi;
     if (next) {
-      const V a = reinterpret_cast<const V*>(aw)[i], b = reinterpret_cast<const V*>(u2)[i];
+      const V a2 = reinterpret_cast<const V*>(aw)[i], b2 = reinterpret_cast<const V*>(u2)[i];
       const V c = reinterpret_cast<const V*>(v)[i];
-      V t;
-      t.x = a.x + beta * ((T)2 * b.x + beta * c.x);
-      t.y = a.y + beta * ((T)2 * b.y + beta * c.y);
-      reinterpret_cast<V*>(v)[i] = t;
-      sg += shadow(2 * i, seed) * t.x + shadow(2 * i + 1, seed) * t.y;
+      V r;
+      r.x = a2.x + beta * ((T)2 * b2.x + beta * c.x);
+      r.y = a2.y + beta * ((T)2 * b2.y + beta * c.y);
+      reinterpret_cast<V*>(v)[i] = r;
+      sg += shadow(2 * i, seed) * r.x + shadow(2 * i + 1, seed) * r.y;
     }
   }
   span_partial(sg, 0.0, partial, sp);
@@ -312,15 +386,6 @@ __global__ void k_resid(const Loc loc, const TB* __restrict__ b, const TA* __res
 // scalars (Freund 1993; Saad, Iterative Methods 7.4.2).
 enum FinalizeOp { F_INIT_R = 0, F_INIT_V, F_HALF, F_HALF_RHO, F_V, F_RESID };
 
-// One quasi-minimisation half step from ||w||^2 = a.
-__device__ inline void half_step(TfqmrScal* s, double a) {
-  const double theta = sqrt(a) / s->tau;
-  const double c = 1.0 / sqrt(1.0 + theta * theta);
-  s->tau = s->tau * theta * c;
-  s->theta = theta;
-  s->eta = c * c * s->alpha;
-  s->dcoef = theta * theta * s->eta / s->alpha;
-}
 
 // Block q finalizes problem q (one block for a single problem; the covered
 // problems of a multi-size launch in order); nblk_uni is the blocks per
@@ -365,13 +430,7 @@ __global__ void k_finalize(const Loc loc, int op, int nblk_uni, const double* __
       half_step(s, a);
       break;
     case F_HALF_RHO:   // second half step, then rho_new = rt.w = b
-      half_step(s, a);
-      s->iter += 1;
-      // bound met: the x update of this iteration still runs (k_v); F_V then stops
-      if (s->tau * sqrt(2.0 * s->iter + 1.0) <= s->target) { s->stop = 1; break; }
-      if (s->rho == 0.0) { s->flag = 2; break; }
-      s->beta = b / s->rho;
-      s->rho = b;
+      half_step_rho(s, a, b);
       break;
     case F_V:
       if (s->stop) { s->flag = 1; break; }
@@ -560,22 +619,20 @@ hipError_t tfq_step1(hipStream_t st, int64_t n, T* w, T* y2, const T* v, const T
                      TfqmrScal* s, double* partial, Batch bt) {
   const int64_t n2 = n / 2;
   LB((k_step1<T, UniLoc>), n2, w, y2, v, aw, u2, d, s, partial);
-  if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_HALF, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
+  return hipGetLastError();
 }
 template <typename T>
 hipError_t tfq_step2(hipStream_t st, int64_t n, T* x, T* w, const T* u2, T* d, const T* y2, TfqmrScal* s,
-                     double* partial, Batch bt) {
+                     TfqmrScal* s2, const double* partial, double* partial2, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_step2<T, UniLoc>), n2, x, w, u2, d, y2, s, partial);
-  if (hipError_t e = hipGetLastError()) return e;
-  return finalize(st, F_HALF_RHO, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
+  LB((k_step2<T, UniLoc>), n2, x, w, u2, d, y2, s, s2, partial, partial2);
+  return hipGetLastError();
 }
 template <typename T>
 hipError_t tfq_v(hipStream_t st, int64_t n, const T* aw, const T* u2, T* v, T* x, const T* d, TfqmrScal* s,
-                 double* partial, Batch bt) {
+                 const TfqmrScal* s2, double* partial, const double* partial2, Batch bt) {
   const int64_t n2 = n / 2;
-  LB((k_v<T, UniLoc>), n2, aw, u2, v, x, d, s, partial);
+  LB((k_v<T, UniLoc>), n2, aw, u2, v, x, d, s2, s, partial2, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_V, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
 }
@@ -680,16 +737,16 @@ hipError_t tfq_init_v(hipStream_t st, const Multi& m, const double* ax, double* 
 hipError_t tfq_step1(hipStream_t st, const Multi& m, double* w, double* y2, const double* v, const double* aw,
                      const double* u2, double* d, TfqmrScal* s, double* partial) {
   LM((k_step1<double, MultiLoc>), w, y2, v, aw, u2, d, s, partial);
-  return finalize_m(st, m, F_HALF, partial, s, nullptr);
+  return hipGetLastError();
 }
 hipError_t tfq_step2(hipStream_t st, const Multi& m, double* x, double* w, const double* u2, double* d,
-                     const double* y2, TfqmrScal* s, double* partial) {
-  LM((k_step2<double, MultiLoc>), x, w, u2, d, y2, s, partial);
-  return finalize_m(st, m, F_HALF_RHO, partial, s, nullptr);
+                     const double* y2, TfqmrScal* s, TfqmrScal* s2, const double* partial, double* partial2) {
+  LM((k_step2<double, MultiLoc>), x, w, u2, d, y2, s, s2, partial, partial2);
+  return hipGetLastError();
 }
 hipError_t tfq_v(hipStream_t st, const Multi& m, const double* aw, const double* u2, double* v, double* x,
-                 const double* d, TfqmrScal* s, double* partial) {
-  LM((k_v<double, MultiLoc>), aw, u2, v, x, d, s, partial);
+                 const double* d, TfqmrScal* s, const TfqmrScal* s2, double* partial, const double* partial2) {
+  LM((k_v<double, MultiLoc>), aw, u2, v, x, d, s2, s, partial2, partial);
   return finalize_m(st, m, F_V, partial, s, nullptr);
 }
 hipError_t vec_resid(hipStream_t st, const Multi& m, const double* b, const double* ax, double* out,
@@ -706,9 +763,9 @@ hipError_t vec_resid(hipStream_t st, const Multi& m, const double* b, const doub
   template hipError_t tfq_step1<T>(hipStream_t, int64_t, T*, T*, const T*, const T*, const T*, T*,       \
                                    TfqmrScal*, double*, Batch);                                         \
   template hipError_t tfq_step2<T>(hipStream_t, int64_t, T*, T*, const T*, T*, const T*, TfqmrScal*,     \
-                                   double*, Batch);                                                     \
+                                   TfqmrScal*, const double*, double*, Batch);                          \
   template hipError_t tfq_v<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, const T*, TfqmrScal*,   \
-                               double*, Batch);                                                         \
+                               const TfqmrScal*, double*, const double*, Batch);                        \
   template hipError_t vec_fill<T>(hipStream_t, int64_t, T*, T);                                           \
   template hipError_t vec_randn<T>(hipStream_t, int64_t, T*, uint64_t);
 INST(double)

@@ -85,8 +85,11 @@ hipError_t vec_project(hipStream_t, int64_t nado, int d, T* x, const int8_t* par
 // beta y2 (A y1 = aw + beta u2), so y1 and u1 = A y1 are formed inside
 // tfq_step1 and tfq_v and never stored.  A (re)start is tfq_init_r (r = b -
 // A x into w; y2 = d = 0), A w -> ax, tfq_init_v (v = ax, aw = ax when aw is
-// given; u2 = 0).  Every
-// reduction is followed by its one-block finalize; the shadow vector is
+// given; u2 = 0).  The first two reductions of an iteration are folded by the
+// kernels that consume them (tfq_step2 folds tfq_step1's partial sums and
+// keeps the half step's scalars in a second copy s2; tfq_v folds tfq_step2's
+// partial sums in partial2 and writes the scalars back to s); tfq_v's own
+// reduction is followed by a one-block finalize.  The shadow vector is
 // generated in the kernels.
 template <typename T>
 hipError_t tfq_init_r(hipStream_t, int64_t n, const T* b, const T* ax, T* w, T* y2, T* d, TfqmrScal* s,
@@ -99,10 +102,10 @@ hipError_t tfq_step1(hipStream_t, int64_t n, T* w, T* y2, const T* v, const T* a
                      TfqmrScal* s, double* partial, Batch bt = Batch{});
 template <typename T>
 hipError_t tfq_step2(hipStream_t, int64_t n, T* x, T* w, const T* u2, T* d, const T* y2, TfqmrScal* s,
-                     double* partial, Batch bt = Batch{});
+                     TfqmrScal* s2, const double* partial, double* partial2, Batch bt = Batch{});
 template <typename T>
 hipError_t tfq_v(hipStream_t, int64_t n, const T* aw, const T* u2, T* v, T* x, const T* d, TfqmrScal* s,
-                 double* partial, Batch bt = Batch{});
+                 const TfqmrScal* s2, double* partial, const double* partial2, Batch bt = Batch{});
 template <typename T>
 hipError_t vec_fill(hipStream_t, int64_t n, T* v, T val);
 template <typename T>
@@ -129,9 +132,9 @@ hipError_t tfq_init_v(hipStream_t, const Multi&, const double* ax, double* v, do
 hipError_t tfq_step1(hipStream_t, const Multi&, double* w, double* y2, const double* v, const double* aw,
                      const double* u2, double* d, TfqmrScal* s, double* partial);
 hipError_t tfq_step2(hipStream_t, const Multi&, double* x, double* w, const double* u2, double* d, const double* y2,
-                     TfqmrScal* s, double* partial);
+                     TfqmrScal* s, TfqmrScal* s2, const double* partial, double* partial2);
 hipError_t tfq_v(hipStream_t, const Multi&, const double* aw, const double* u2, double* v, double* x,
-                 const double* d, TfqmrScal* s, double* partial);
+                 const double* d, TfqmrScal* s, const TfqmrScal* s2, double* partial, const double* partial2);
 hipError_t vec_resid(hipStream_t, const Multi&, const double* b, const double* ax, double* out, double* partial);
 hipError_t vec_set_elem(hipStream_t, double* v, int64_t off, double val);
 // A few scattered values written into every problem's copy of a vector.
