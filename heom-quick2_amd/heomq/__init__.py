@@ -252,6 +252,10 @@ class Model:
         self.d, self.M, self.L, self.N, self.ncor, self.nopr = (int(v) for v in info[:6])
         self.n_lower, self.n_upper = int(info[6]), int(info[7])
         self.first = first[: self.L + 2].copy()
+        # orbitals and leads from the model itself (a model loaded from the
+        # reference's arrays has no parameters): d = 4**norbs, nopr = 2 norbs nspin nalf
+        self.norbs = int(round(np.log(self.d) / np.log(4)))
+        self.nalf = self.nopr // (2 * self.norbs * 2)
         _LIVE_MODELS.add(self)
 
     def close(self):
@@ -315,7 +319,7 @@ class Model:
 
     def operators(self):
         Hf = np.zeros(self.d * self.d, np.complex128)
-        nops = self.nopr // (2 * (self.params.nalf if self.params else 2))
+        nops = self.nopr // (2 * self.nalf)
         ams = np.zeros(self.d * self.d * nops, np.float64)
         check(lib().heomq_model_operators(self._h, _dp(Hf.view(np.float64)), _dp(ams)))
         H = Hf.reshape(self.d, self.d).T.copy()                     # column-major -> H[i, j]
@@ -829,8 +833,7 @@ class Context:
 
     def field_shifts(self, tt: float) -> np.ndarray:
         """eshift(alpha, s; tt) of the lead field in eV, (nalf, nspin); zeros without one."""
-        p = self.model.params
-        nalf = p.nalf if p else 2
+        nalf = self.model.nalf
         es = np.zeros(nalf * 2)
         check(lib().heomq_ctx_field_shifts(self._h, float(tt), _dp(es)))
         return es.reshape(nalf, 2)
@@ -864,11 +867,8 @@ class Context:
         return xp, dict(ncheby=int(stats[0]), delta=stats[1], packed=True)
 
     def observables(self, x):
-        p = self.model.params
-        nalf = p.nalf if p else 2
-        norbs = p.norbs if p else 1
-        cur = np.zeros((nalf, 2))
-        occ = np.zeros((norbs, 2))
+        cur = np.zeros((self.model.nalf, 2))   # current(nalf, nspin) and occupation(norbs, nspin)
+        occ = np.zeros((self.model.norbs, 2))
         check(lib().heomq_observables(self._h, self._check_vec(x, self.torch.complex128), _dp(cur), _dp(occ),
                                       self._stream()))
         return cur, occ

@@ -45,8 +45,8 @@ def current_weights(model: Model, ctx: Context, lead: int = 0):
     spin, from the tier-1 lookups and c operators (td/calcj.f90:15-37,68)."""
     torch = ctx.torch
     M, d = model.M, model.d
-    nspin, norbs = 2, model.params.norbs
-    nalf = model.params.nalf
+    nspin, norbs = 2, model.norbs
+    nalf = model.nalf
     ncor = M // (2 * nspin * norbs * nalf)
     _, upper = model.coupling()
     _, A = model.operators()                      # A[m*nspin + s][i, j] = c_{ms}(i, j)

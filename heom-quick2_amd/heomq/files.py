@@ -70,8 +70,7 @@ class FileFormatError(ValueError):
 
 def header_of(model, nunk: int | None = None) -> StateHeader:
     """The header the reference writes for a hierarchy like `model`'s."""
-    p = model.params
-    return StateHeader(int(p.norbs), 2, int(model.ncor), int(model.L) + 1, int(p.nalf),
+    return StateHeader(int(model.norbs), 2, int(model.ncor), int(model.L) + 1, int(model.nalf),
                        int(model.N if nunk is None else nunk))
 
 

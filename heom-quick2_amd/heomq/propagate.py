@@ -71,8 +71,7 @@ def _shifts(ctx, t: float):
 
 def record(ctx, x, t: float, runits: int = 0):
     """(curr.data values, popu.data values) of state x at time t."""
-    p = ctx.model.params
-    nalf = p.nalf
+    nalf = ctx.model.nalf
     cur, occ = ctx.observables(x)
     occu, occd = float(occ[:, 0].sum()), float(occ[:, 1].sum())
     oc = occu + occd

@@ -148,6 +148,24 @@ def test_reference_loader_round_trip(pp, op):
     assert np.abs(m_ref.coefs()[2] - m.coefs()[2]).max() <= 1e-13 * np.abs(m.coefs()[2]).max()
 
 
+def test_model_knows_its_orbitals_and_leads_without_parameters():
+    # a model loaded from the reference's arrays carries no parameters: the
+    # sizes of its observables (current (nalf, nspin), occupation (norbs,
+    # nspin)) come from the model itself, for one, two and three orbitals
+    three = O.Params(norbs=3, nspin=2, nalf=2, nmats=1, bandwidth=(10.0, 10.0),
+                     linewidth=((0.1, 0.08, 0.12), (0.1, 0.08, 0.12)), temperature=(0.2, 0.2),
+                     engyshift=((0.1, 0.1), (-0.1, -0.1)), ham="siam", hpar=dict(eup=-0.5, edown=-0.5, uu=1.0),
+                     ntop=1)
+    for op, norbs in ((O.siam_params(nk=2, ntop=2), 1), (O.dqd_params(nk=2, ntop=2), 2), (three, 3)):
+        o = O.Model(op)
+        m = heomq.Model.from_reference(**reference_arrays(o))
+        assert m.params is None and (m.norbs, m.nalf, m.d) == (norbs, 2, 4 ** norbs)
+        H, A = m.operators()
+        assert A.shape == (2 * norbs, m.d, m.d)
+    m = heomq.Model(heomq.siam(nk=2, ntop=2))
+    assert (m.norbs, m.nalf) == (1, 2)
+
+
 def test_reference_loader_rejects_shifted_indextable():
     # indextable without tier 0's empty record (ball2drawer.f90:40-44) shifts
     # every tuple; the loader must refuse it instead of reading past the end
