@@ -192,6 +192,13 @@ struct ApplyMulti {
 static_assert(sizeof(ApplyMulti) <= 4000, "the multi-shape apply's kernel argument must stay under 4 KB");
 // workgroups of one problem's packed apply
 int64_t apply_blocks(const DevPacked& op);
+// One Chebyshev term (td/chebyshev.f90) in one launch: y = L p1 (forward, no
+// trace row, no Hermitian fix-up, one problem) is not stored; each element
+// updates p += a y, then x += b p.  The same arithmetic as the apply followed
+// by the two vector passes.
+bool apply_cheb_fusable(const DevPacked& op);
+hipError_t launch_apply_packed_cheb(const DevPacked& op, const double* p1, double* p, double* x, double a, double b,
+                                    hipStream_t s);
 // every shape needs a heavy part (top tier >= 1), whose block 0 does the
 // tier-0 fix-up, and its trace slot
 hipError_t launch_apply_packed_multi_f64(const ApplyMulti& am, const double* x, double* y, bool rdo, hipStream_t s);

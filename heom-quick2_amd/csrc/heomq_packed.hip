@@ -46,6 +46,28 @@ __device__ inline typename P2<T>::type ldw(const typename P2<T>::type* __restric
   return ld<T, TRANS>(xv, g, zero_at);
 }
 
+// A Chebyshev term applied where the apply writes its output (no y in
+// memory): with y = L p1 for element g,  p[g] += a y;  x[g] += b p[g]
+// (td/chebyshev.f90's P_k = P_{k-2} + (2 / delta) L P_{k-1} over P_{k-2},
+// then x += 2 J_k P_k), the same arithmetic as the two vector passes.
+struct ChebEpi {
+  double* p;
+  double* x;
+  double a, b;
+};
+__device__ inline void cheb_put(const ChebEpi* ep, int64_t g, double re, double im) {
+  double2* pv = reinterpret_cast<double2*>(ep->p) + g;
+  double2* xv = reinterpret_cast<double2*>(ep->x) + g;
+  double2 q = *pv;
+  q.x += ep->a * re;
+  q.y += ep->a * im;
+  *pv = q;
+  double2 w = *xv;
+  w.x += ep->b * q.x;
+  w.y += ep->b * q.y;
+  *xv = w;
+}
+
 // The coefficient arrays of one problem of a batch.
 struct Coefs {
   const DevRec* rec;
@@ -233,9 +255,9 @@ constexpr size_t kHeavyOffRi = 256 * 8, kHeavyOffEn = 2 * 256 * 8, kHeavyOffCf =
                  kHeavyOffX = kHeavyOffCf + 256 * sizeof(CoefLR), kHeavySmem = kHeavyOffX + 256 * 16;
 constexpr size_t kApplySmem0 = kLightSmem > kHeavySmem ? kLightSmem : kHeavySmem;
 
-template <typename T, bool TRANS, int U>
+template <typename T, bool TRANS, int U, bool EPI = false>
 __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
-                                  int64_t zero_at, int bid, char* smem, bool scaled) {
+                                  int64_t zero_at, int bid, char* smem, bool scaled, const ChebEpi* ep = nullptr) {
   using V = typename P2<T>::type;
   int64_t* s_off = reinterpret_cast<int64_t*>(smem);
   int64_t* s_p0 = reinterpret_cast<int64_t*>(smem + kLightOffP0);
@@ -288,6 +310,10 @@ __device__ inline void light_body(const DevPacked& op, const Coefs cf_p, const T
     coupling_block<T, TRANS, U>(en, cf, nv, op.maps, xv, slot, zero_at, ar, ai);
   }
   if (scaled) jacobi_scale<T>(op, (T)s_gam[l].x, (T)s_gam[l].y, csr, csi, ar, ai);
+  if (EPI) {
+    cheb_put(ep, t, (double)ar, (double)ai);
+    return;
+  }
   V o;
   o.x = ar;
   o.y = ai;
@@ -314,10 +340,10 @@ struct StageOut {
 // coupling of the ADO (entries and coefficients staged in LDS), two
 // couplings per step; the slices are summed through LDS in a fixed order.
 // Block 0 (the tier-0 ADO) then applies the trace row and Hermitianises.
-template <typename T, bool TRANS, bool STAGE = false>
+template <typename T, bool TRANS, bool STAGE = false, bool EPI = false>
 __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
                                   int64_t zero_at, int rdo, int bid, char* smem, bool scaled,
-                                  StageOut* so = nullptr) {
+                                  StageOut* so = nullptr, const ChebEpi* ep = nullptr) {
   using V = typename P2<T>::type;
   constexpr int kStage = 256;
   T* red_r = reinterpret_cast<T*>(smem);
@@ -399,6 +425,8 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
       if (STAGE) {   // one pass (nnz <= 256): slot t = threadIdx.x
         so->v = make_double2((double)sr, (double)si);
         so->act = true;
+      } else if (EPI) {
+        cheb_put(ep, base + t, (double)sr, (double)si);
       } else {
         reinterpret_cast<V*>(y)[base + t] = o;
       }
@@ -412,7 +440,7 @@ __device__ inline void heavy_body(const DevPacked& op, const Coefs cf_p, const T
     so->s = s;
     so->gr = (double)gr;
     so->gi = (double)gi;
-  } else if (a == 0) {
+  } else if (a == 0 && !EPI) {   // (with the epilogue no trace row and no Hermitian fix-up: host check)
     tier0_fixup_block<T>(op, x, y, rdo);   // block-uniform branch
   }
 }
@@ -440,9 +468,10 @@ constexpr size_t kGOffB = 256 * sizeof(HWgt<double>);                           
 constexpr size_t kGOffG = kGOffB + 2 * 16 * (size_t)kHeavyPairs;                   // run table
 constexpr size_t kGSmem = kGOffG + 16 * (size_t)(kHeavyGroups + 1);
 
-template <typename T, bool TRANS>
+template <typename T, bool TRANS, bool EPI = false>
 __device__ inline void heavy_grouped(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x,
-                                     T* __restrict__ y, int64_t zero_at, int rdo, int bid, char* smem, bool scaled) {
+                                     T* __restrict__ y, int64_t zero_at, int rdo, int bid, char* smem, bool scaled,
+                                     const ChebEpi* ep = nullptr) {
   using V = typename P2<T>::type;
   HWgt<T>* s_w = reinterpret_cast<HWgt<T>*>(smem);
   V* s_B = reinterpret_cast<V*>(smem + kGOffB);
@@ -564,7 +593,10 @@ __device__ inline void heavy_grouped(const DevPacked& op, const Coefs cf_p, cons
       V o;
       o.x = sr;
       o.y = si;
-      reinterpret_cast<V*>(y)[base + t] = o;
+      if (EPI)
+        cheb_put(ep, base + t, (double)sr, (double)si);
+      else
+        reinterpret_cast<V*>(y)[base + t] = o;
     }
     __syncthreads();
   }
@@ -577,9 +609,9 @@ __device__ inline void heavy_grouped(const DevPacked& op, const Coefs cf_p, cons
 // of two w, so 64 / w slices walk the coupling list (entries, records and
 // gather maps from global memory, two couplings in flight); the slices are
 // summed by shuffles in a fixed tree, with no LDS and no barrier.
-template <typename T, bool TRANS>
+template <typename T, bool TRANS, bool EPI = false>
 __device__ inline void heavy_wave(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x, T* __restrict__ y,
-                                  int64_t zero_at, int64_t a, bool scaled) {
+                                  int64_t zero_at, int64_t a, bool scaled, const ChebEpi* ep = nullptr) {
   using V = typename P2<T>::type;
   const int lane = threadIdx.x & 63;
   const int64_t base = op.off[a];
@@ -625,6 +657,10 @@ __device__ inline void heavy_wave(const DevPacked& op, const Coefs cf_p, const T
   }
   if (slice == 0 && act) {
     if (scaled) jacobi_scale<T>(op, gr, gi, csr, csi, ar, ai);
+    if (EPI) {
+      cheb_put(ep, base + t, (double)ar, (double)ai);
+      return;
+    }
     V o;
     o.x = ar;
     o.y = ai;
@@ -641,10 +677,10 @@ __device__ inline void heavy_wave(const DevPacked& op, const Coefs cf_p, const T
 // block) read their neighbours.  Four levels of dependent loads in all.
 constexpr size_t kLOffHc = 256 * 2, kLOffX = kLOffHc + 256 * 16, kLSmem = kLOffX + 256 * 16;
 
-template <typename T, bool TRANS, bool STAGE = false>
+template <typename T, bool TRANS, bool STAGE = false, bool EPI = false>
 __device__ inline void light_sector(const DevPacked& op, const Coefs cf_p, const T* __restrict__ x,
                                     T* __restrict__ y, int64_t zero_at, int bid, char* smem, bool scaled,
-                                    StageOut* so = nullptr) {
+                                    StageOut* so = nullptr, const ChebEpi* ep = nullptr) {
   using V = typename P2<T>::type;
   int16_t* s_hq = reinterpret_cast<int16_t*>(smem);
   double2* s_hc = reinterpret_cast<double2*>(smem + kLOffHc);
@@ -739,6 +775,10 @@ __device__ inline void light_sector(const DevPacked& op, const Coefs cf_p, const
     so->gi = (double)gi;
     return;
   }
+  if (EPI) {
+    cheb_put(ep, base + t, (double)ar, (double)ai);
+    return;
+  }
   V o;
   o.x = ar;
   o.y = ai;
@@ -754,9 +794,10 @@ constexpr size_t kApplySmem1 = kGSmem > kLSmem ? kGSmem : kLSmem;
 constexpr size_t kApplySmem = kApplySmem0 > kApplySmem1 ? kApplySmem0 : kApplySmem1;
 
 // Block bx of problem p of the apply (the kernels below supply the two).
-template <typename T, bool TRANS, int U>
+template <typename T, bool TRANS, int U, bool EPI = false>
 __device__ inline void apply_block(const DevPacked& op, int64_t p, int64_t bx, const T* __restrict__ x,
-                                   T* __restrict__ y, int64_t zero_at, int rdo, char* smem) {
+                                   T* __restrict__ y, int64_t zero_at, int rdo, char* smem,
+                                   const ChebEpi* ep = nullptr) {
   if (op.frozen_flag && op.frozen_flag[p * op.frozen_stride] == 3) return;   // finished: block-uniform, no barrier yet
   const bool scaled = op.precond && !(op.frozen_flag && op.frozen_flag[p * op.frozen_stride + 2]);
   x += 2 * p * op.vstride;
@@ -766,18 +807,18 @@ __device__ inline void apply_block(const DevPacked& op, int64_t p, int64_t bx, c
   if (bx < hb) {   // block-uniform
     if (op.heavy_wave && bx > 0) {   // four ADOs of small sectors, a wave each (no barrier)
       const int64_t a = 1 + (bx - 1) * 4 + (threadIdx.x >> 6);
-      if (a < op.n_heavy) heavy_wave<T, TRANS>(op, cf, x, y, zero_at, a, scaled);
+      if (a < op.n_heavy) heavy_wave<T, TRANS, EPI>(op, cf, x, y, zero_at, a, scaled, ep);
       return;
     }
     const int a = (int)bx;
     if (op.hg_off && op.hg_off[a + 1] > op.hg_off[a])
-      heavy_grouped<T, TRANS>(op, cf, x, y, zero_at, rdo, a, smem, scaled);
+      heavy_grouped<T, TRANS, EPI>(op, cf, x, y, zero_at, rdo, a, smem, scaled, ep);
     else
-      heavy_body<T, TRANS>(op, cf, x, y, zero_at, rdo, a, smem, scaled);
+      heavy_body<T, TRANS, false, EPI>(op, cf, x, y, zero_at, rdo, a, smem, scaled, nullptr, ep);
   } else if (op.lt_blk) {
-    light_sector<T, TRANS>(op, cf, x, y, zero_at, (int)(bx - hb), smem, scaled);
+    light_sector<T, TRANS, false, EPI>(op, cf, x, y, zero_at, (int)(bx - hb), smem, scaled, nullptr, ep);
   } else {
-    light_body<T, TRANS, U>(op, cf, x, y, zero_at, (int)(bx - hb), smem, scaled);
+    light_body<T, TRANS, U, EPI>(op, cf, x, y, zero_at, (int)(bx - hb), smem, scaled, ep);
   }
 }
 
@@ -786,6 +827,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
 k_apply_packed(const DevPacked op, const T* __restrict__ x, T* __restrict__ y, int64_t zero_at, int rdo) {
   __shared__ __align__(16) char smem[kApplySmem];
   apply_block<T, TRANS, U>(op, blockIdx.y, blockIdx.x, x, y, zero_at, rdo, smem);   // grid y: problem of a batch
+}
+
+// One Chebyshev term in one launch: the forward apply of p1 (no trace row,
+// no Hermitian fix-up) with each output element folded into the recurrence
+// and the sum at once (ChebEpi); L p1 itself is never stored.
+template <int U>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+k_apply_packed_cheb(const DevPacked op, const double* __restrict__ p1, const ChebEpi ep) {
+  __shared__ __align__(16) char smem[kApplySmem];
+  apply_block<double, false, U, true>(op, 0, blockIdx.x, p1, nullptr, -1, 0, smem, &ep);
 }
 
 // Forward applies of problems of several shapes in one flat grid (the shapes
@@ -1457,6 +1508,23 @@ hipError_t launch_apply_packed_f64(const DevPacked& op, const double* x, double*
 int64_t apply_blocks(const DevPacked& op) {
   const int64_t light = op.total - op.light_begin;
   return heavy_blocks(op) + (op.lt_blk ? op.n_lt_blocks : (light > 0 ? (light + 255) / 256 : 0));
+}
+
+bool apply_cheb_fusable(const DevPacked& op) {
+  return !op.transposed && op.nprob == 1 && op.herm0 == 0 && op.precond == 0 && op.n_heavy > 0;
+}
+
+hipError_t launch_apply_packed_cheb(const DevPacked& op, const double* p1, double* p, double* x, double a, double b,
+                                    hipStream_t s) {
+  if (!apply_cheb_fusable(op)) return hipErrorInvalidValue;
+  const int64_t blocks = apply_blocks(op);
+  if (blocks <= 0) return hipSuccess;
+  const ChebEpi ep{p, x, a, b};
+  if (op.max_light_couplings <= 1)
+    hipLaunchKernelGGL((k_apply_packed_cheb<1>), dim3((unsigned)blocks), dim3(256), 0, s, op, p1, ep);
+  else
+    hipLaunchKernelGGL((k_apply_packed_cheb<2>), dim3((unsigned)blocks), dim3(256), 0, s, op, p1, ep);
+  return hipGetLastError();
 }
 
 hipError_t launch_apply_packed_multi_f64(const ApplyMulti& am, const double* x, double* y, bool rdo, hipStream_t s) {

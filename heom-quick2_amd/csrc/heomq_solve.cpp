@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <mutex>
 #include <vector>
@@ -1047,10 +1048,13 @@ int cheby_setup(heomq_ctx* c, double dt, std::vector<double>& J, double& delta) 
 
 // nstep Chebyshev steps of td/chebyshev.f90:126-200 on n reals: P_0 = x,
 // P_1 = L x / delta, P_k = P_{k-2} + (2 / delta) L P_{k-1}, x = sum_k c_k J_k P_k
-// (c_0 = 1, else 2); apply(in, out) is L.
+// (c_0 = 1, else 2); apply(in, out) is L.  term(p1, p, x, a, b), when given,
+// does one k >= 2 term in one go (p += a L p1, then x += b p); without it
+// the term is the apply and two vector passes.
 template <typename Apply>
 int cheby_steps(double* x, int64_t n, double* pm2, double* pm1, double* lp, const std::vector<double>& J,
-                double delta, int nstep, hipStream_t st, Apply&& apply) {
+                double delta, int nstep, hipStream_t st, Apply&& apply,
+                const std::function<hipError_t(const double*, double*, double*, double, double)>& term = {}) {
   const int ncheby = (int)J.size();
   for (int step = 0; step < nstep; ++step) {
     HIPCHECK(hipMemcpyAsync(pm2, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));   // P0
@@ -1059,10 +1063,14 @@ int cheby_steps(double* x, int64_t n, double* pm2, double* pm1, double* lp, cons
     HIPCHECK(vec_lincomb3(st, n, pm1, 1.0 / delta, lp, 0.0, nullptr, 0.0, nullptr));        // P1
     if (ncheby > 1) HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, 2.0 * J[1], pm1, 0.0, nullptr));
     for (int k = 2; k < ncheby; ++k) {
-      if (int rc = apply(pm1, lp)) return rc;
       // P_k = P_{k-2} + (2/delta) L P_{k-1}, written over P_{k-2}
-      HIPCHECK(vec_lincomb3(st, n, pm2, 1.0, pm2, 2.0 / delta, lp, 0.0, nullptr));
-      HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, 2.0 * J[k], pm2, 0.0, nullptr));
+      if (term) {
+        HIPCHECK(term(pm1, pm2, x, 2.0 / delta, 2.0 * J[k]));
+      } else {
+        if (int rc = apply(pm1, lp)) return rc;
+        HIPCHECK(vec_lincomb3(st, n, pm2, 1.0, pm2, 2.0 / delta, lp, 0.0, nullptr));
+        HIPCHECK(vec_lincomb3(st, n, x, 1.0, x, 2.0 * J[k], pm2, 0.0, nullptr));
+      }
       std::swap(pm1, pm2);
     }
   }
@@ -1088,7 +1096,12 @@ int cheby_packed_core(heomq_ctx* c, double* xp, const std::vector<double>& J, do
     HIPCHECK(launch_apply_packed_f64(op, in, out, false, st));
     return HEOMQ_OK;
   };
-  return cheby_steps(xp, 2 * c->ptotal, v[0], v[1], v[2], J, delta, nstep, st, apply);
+  std::function<hipError_t(const double*, double*, double*, double, double)> term;
+  if (c->fuse_precond && apply_cheb_fusable(op))   // the term in one launch (the same switch as the solver's fusion)
+    term = [&](const double* p1, double* p, double* x, double a, double b) {
+      return launch_apply_packed_cheb(op, p1, p, x, a, b, st);
+    };
+  return cheby_steps(xp, 2 * c->ptotal, v[0], v[1], v[2], J, delta, nstep, st, apply, term);
 }
 
 }  // namespace

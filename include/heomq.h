@@ -193,7 +193,9 @@ int heomq_ctx_set_presmooth(heomq_ctx* c, int enable);
  * each workgroup's ADOs before anything is written, when the forward apply
  * form has the sector-batched top tier and sliced heavy rows (enable = 1, the
  * default; the results are the same bits); 0 always runs the apply, then the
- * inverse.  No counterpart in the reference. */
+ * inverse.  The same switch fuses each Chebyshev term of the packed
+ * propagators (the apply's output folded into the recurrence and the sum,
+ * one launch, the same bits).  No counterpart in the reference. */
 int heomq_ctx_set_precond_fusion(heomq_ctx* c, int enable);
 /* Left preconditioning of the forward steady-state TFQMR on the packed
  * layout: 0 none; 1 Jacobi (diagonal) scaling, default right-hand side only;

@@ -230,6 +230,28 @@ def test_chebyshev_in_sector_takes_the_packed_layout(name):
     assert not info["packed"]
 
 
+@pytest.mark.parametrize("name", ["siam_nk3_L3", "dqd_nk2_L2"])
+def test_chebyshev_fused_term_is_the_two_pass_result(name):
+    """Each Chebyshev term k >= 2 runs as one launch (the apply's output folded
+    into P_k = P_{k-2} + (2/delta) L P_{k-1} and x += 2 J_k P_k) by default:
+    the same bits as the apply followed by the two vector passes."""
+    m, ctx, o, torch = setup(name)
+    x = O.physical_projection(o, O.seeded_ado_vector(o, 46))
+    xp = ctx.pack(to_dev(torch, x))
+    a = xp.clone()
+    ctx.chebyshev_packed(a, 0.02, 3)
+    ctx.set_precond_fusion(False)
+    try:
+        b = xp.clone()
+        ctx.chebyshev_packed(b, 0.02, 3)
+    finally:
+        ctx.set_precond_fusion(True)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    xo, _, _ = O.chebyshev(o, x, 0.02, 3)
+    assert rel(from_dev(ctx.unpack(a)), xo) <= 1e-12
+
+
 def test_packed_propagation_conserves_trace_and_hermiticity():
     """Size-independent properties of the propagation at the two-impurity
     workload's full size (d=16, Nk=6, L=3, 117,321 ADOs) from a seeded random
