@@ -428,6 +428,7 @@ def sweep(args):
     value = len(pts) * args.steps / elapsed
     # plain FP64 per-point rate: one point of each shape solved alone, weighted by the shape's share
     t_plain, max_err = 0.0, 0.0
+    plain_ms = {}
     for k, idx in groups.items():
         i = idx[0]
         ctx = heomq.Context(heomq.Model(heomq.siam(**pts[i])))
@@ -436,7 +437,9 @@ def sweep(args):
         t1 = time.perf_counter()
         x, st = ctx.solve_steady(tol=tol)
         torch.cuda.synchronize()
-        t_plain += (time.perf_counter() - t1) * len(idx)
+        dt_one = time.perf_counter() - t1
+        plain_ms[f"nk{k[0]}_L{k[1]}"] = round(dt_one * 1e3, 3)
+        t_plain += dt_one * len(idx)
         max_err = max(max_err, float(np.abs(heomq_rdo(x) - rdo[i]).max()))
         ctx.close()
     plain_rate = len(pts) / t_plain
@@ -450,6 +453,7 @@ def sweep(args):
                    **({"sweeps": max(1, args.sweeps)} if args.driver == "sweep" else {}),
                    "shapes": {f"nk{k[0]}_L{k[1]}": len(v) for k, v in sorted(groups.items())}},
         "plain_fp64_points_per_s": round(plain_rate, 3),
+        "plain_ms_per_point_by_shape": plain_ms,
         "speedup_vs_plain": round(value / plain_rate, 2),
         "rdo_err_vs_plain": max_err,
         "iters_mean": float(iters.mean()), "iters_max": float(iters.max()), "relres_max": float(relres.max()),
