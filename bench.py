@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=5, help="timed CPU TFQMR iterations (median)")
     ap.add_argument("--apply-only", action="store_true",
                     help="profiling: a 64 MiB calibration copy, then --apply-reps applies; no solve, no JSON")
+    ap.add_argument("--nk", type=int, default=None, help="--apply-only: Pade terms of the profiled hierarchy")
+    ap.add_argument("--ntop", type=int, default=None, help="--apply-only: top tier of the profiled hierarchy")
     ap.add_argument("--apply-kernel", default="packed",
                     choices=["packed", "dense", "precond", "precond_fused", "f32", "transpose", "spa", "rk4",
                              "chebyshev", "bicg"],
@@ -120,6 +122,10 @@ def main():
         args.lscale = 1 if args.workload == "two_impurity" else 0
     params, wname = workload(args.workload, bool(args.lscale))
     if args.apply_only:
+        if args.ntop is not None or args.nk is not None:   # another hierarchy of the same model, profiling only
+            import dataclasses
+            params = dataclasses.replace(params, ntop=args.ntop if args.ntop is not None else params.ntop,
+                                         nmats=args.nk if args.nk is not None else params.nmats)
         return apply_only(params, args.apply_reps, args.apply_kernel)
     if args.tol is None:
         # scaled: 3e-11 on the scaled residual holds every step's RDO error under 1e-10;
