@@ -116,6 +116,7 @@ struct heomq_ctx {
   std::vector<int32_t> lower_drawers;
   // solver workspace
   DevBuf ws64, ws64b, ws32, ws_io, scal, partial, red;
+  DevBuf partial_apply;     // per-workgroup sums of the fused TFQMR iteration's operator launches
   DevBuf scal2, partial2;   // TFQMR's mid-iteration scalar copy and second partial-sum buffer
   DevBuf prop_flag;                  // off-sector test of a dense state (propagators)
   DevBuf ws_host;                    // dense state for the host-pointer solve

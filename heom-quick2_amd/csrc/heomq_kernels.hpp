@@ -333,4 +333,18 @@ bool apply_jac_fusable(const DevPacked& op, const DevJacobi& jc);
 hipError_t launch_apply_packed_jac(const DevPacked& op, const DevJacobi& jc, const double* x, double* y, bool rdo,
                                    hipStream_t s);
 
+// A TFQMR iteration's vector work carried by the preconditioned launch (kind
+// 1: out = u2, w -= alpha u2, partials (||w||^2, rt.w, rt.u2) per workgroup;
+// kind 2: out = aw, partial rt.aw per workgroup); s is the problem's device
+// scalars.  apply_jac_blocks is the launch's workgroup count.
+struct TfqmrScal;
+struct TfqEpi {
+  const TfqmrScal* s = nullptr;
+  double* w = nullptr;
+  double* partial = nullptr;
+};
+int64_t apply_jac_blocks(const DevPacked& op);
+hipError_t launch_apply_packed_jac_tfq(const DevPacked& op, const DevJacobi& jc, const double* x, double* y, bool rdo,
+                                       int kind, const TfqEpi& ep, hipStream_t s);
+
 }  // namespace heomq

@@ -13,6 +13,8 @@
 #include <algorithm>
 
 #include "heomq_kernels.hpp"
+#include "heomq_shadow.hpp"
+#include "heomq_solver.hpp"
 
 namespace heomq {
 
@@ -1282,10 +1284,17 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
 // element with the conjugate of its mirror after the inverse, as the two
 // kernels do.  Every element goes through the same arithmetic as in the
 // apply followed by k_jacobi_ado.
-template <bool RU>
+//
+// EP > 0 carries a TFQMR iteration's vector work on the outputs (one
+// problem): EP 1, out = u2 = M^-1 A y2, then w -= alpha u2 and per-workgroup
+// partials (||w||^2, rt.w, rt.u2) at ep.partial[3 blockIdx.x ..]; EP 2, out =
+// aw = M^-1 A w and the partial rt.aw at ep.partial[blockIdx.x].  Nothing
+// runs once the problem's flag is set.
+template <bool RU, int EP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
 k_apply_packed_jac(const DevPacked op, const DevJacobi jc, const double* __restrict__ x, double* __restrict__ out,
-                   int rdo) {
+                   int rdo, const TfqEpi ep) {
+  if (EP && ep.s->flag) return;   // uniform over the grid
   static_assert(2 * 256 * sizeof(double2) + sizeof(UStore<RU>) + 16 * sizeof(double) <= kApplySmem,
                 "the inverse's stages fit the apply's LDS");
   __shared__ __align__(16) char smem[kApplySmem];
@@ -1405,7 +1414,59 @@ k_apply_packed_jac(const DevPacked op, const DevJacobi jc, const double* __restr
       z.y = 0.5 * (z.y - m.y);
     }
   }
-  if (so.act) reinterpret_cast<double2*>(out)[so.gbase + so.t] = z;
+  if (EP == 0) {
+    if (so.act) reinterpret_cast<double2*>(out)[so.gbase + so.t] = z;
+    return;
+  }
+  double r0 = 0, r1 = 0, r2 = 0;
+  if (so.act) {
+    const int64_t g = so.gbase + so.t;
+    reinterpret_cast<double2*>(out)[g] = z;
+    const uint64_t seed = ep.s->seed;
+    const double s0 = shadow(2 * g, seed), s1 = shadow(2 * g + 1, seed);
+    if (EP == 1) {
+      const double alpha = ep.s->alpha;
+      double2 wi = reinterpret_cast<double2*>(ep.w)[g];
+      wi.x -= alpha * z.x;
+      wi.y -= alpha * z.y;
+      reinterpret_cast<double2*>(ep.w)[g] = wi;
+      r0 = wi.x * wi.x + wi.y * wi.y;
+      r1 = s0 * wi.x + s1 * wi.y;
+      r2 = s0 * z.x + s1 * z.y;
+    } else {
+      r0 = s0 * z.x + s1 * z.y;
+    }
+  }
+  __shared__ double red[3][4];
+  for (int off = 32; off > 0; off >>= 1) {
+    r0 += __shfl_down(r0, off, 64);
+    if (EP == 1) {
+      r1 += __shfl_down(r1, off, 64);
+      r2 += __shfl_down(r2, off, 64);
+    }
+  }
+  const int lane = tid & 63, wid = tid >> 6;
+  if (lane == 0) {
+    red[0][wid] = r0;
+    red[1][wid] = r1;
+    red[2][wid] = r2;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int k = 0; k < 4; ++k) {
+      a += red[0][k];
+      b += red[1][k];
+      c += red[2][k];
+    }
+    if (EP == 1) {
+      ep.partial[3 * blockIdx.x] = a;
+      ep.partial[3 * blockIdx.x + 1] = b;
+      ep.partial[3 * blockIdx.x + 2] = c;
+    } else {
+      ep.partial[blockIdx.x] = a;
+    }
+  }
 }
 
 template <typename TP, typename TD>
@@ -1625,11 +1686,33 @@ bool apply_jac_fusable(const DevPacked& op, const DevJacobi& jc) {
 hipError_t launch_apply_packed_jac(const DevPacked& op, const DevJacobi& jc, const double* x, double* y, bool rdo,
                                    hipStream_t s) {
   if (!apply_jac_fusable(op, jc) || (rdo && op.trace_slot < 0)) return hipErrorInvalidValue;
-  const int64_t blocks = op.n_heavy + op.n_lt_blocks;
+  const int64_t blocks = apply_jac_blocks(op);
+  const TfqEpi none{};
   if (jc.u_real)
-    hipLaunchKernelGGL((k_apply_packed_jac<true>), dim3((unsigned)blocks), dim3(256), 0, s, op, jc, x, y, rdo ? 1 : 0);
+    hipLaunchKernelGGL((k_apply_packed_jac<true, 0>), dim3((unsigned)blocks), dim3(256), 0, s, op, jc, x, y,
+                       rdo ? 1 : 0, none);
   else
-    hipLaunchKernelGGL((k_apply_packed_jac<false>), dim3((unsigned)blocks), dim3(256), 0, s, op, jc, x, y, rdo ? 1 : 0);
+    hipLaunchKernelGGL((k_apply_packed_jac<false, 0>), dim3((unsigned)blocks), dim3(256), 0, s, op, jc, x, y,
+                       rdo ? 1 : 0, none);
+  return hipGetLastError();
+}
+
+int64_t apply_jac_blocks(const DevPacked& op) { return op.n_heavy + op.n_lt_blocks; }
+
+hipError_t launch_apply_packed_jac_tfq(const DevPacked& op, const DevJacobi& jc, const double* x, double* y, bool rdo,
+                                       int kind, const TfqEpi& ep, hipStream_t s) {
+  if (!apply_jac_fusable(op, jc) || (rdo && op.trace_slot < 0) || (kind != 1 && kind != 2) || !ep.s ||
+      !ep.partial || (kind == 1 && !ep.w))
+    return hipErrorInvalidValue;
+  const dim3 grid((unsigned)apply_jac_blocks(op));
+  const int r = rdo ? 1 : 0;
+  if (jc.u_real) {
+    if (kind == 1) hipLaunchKernelGGL((k_apply_packed_jac<true, 1>), grid, dim3(256), 0, s, op, jc, x, y, r, ep);
+    else hipLaunchKernelGGL((k_apply_packed_jac<true, 2>), grid, dim3(256), 0, s, op, jc, x, y, r, ep);
+  } else {
+    if (kind == 1) hipLaunchKernelGGL((k_apply_packed_jac<false, 1>), grid, dim3(256), 0, s, op, jc, x, y, r, ep);
+    else hipLaunchKernelGGL((k_apply_packed_jac<false, 2>), grid, dim3(256), 0, s, op, jc, x, y, r, ep);
+  }
   return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 #include <limits>
 #include <mutex>
 #include <vector>
@@ -128,11 +129,57 @@ int carve(DevBuf& buf, int64_t n, int count, std::vector<T*>& out) {
 struct RunStats {
   int iters = 0, applies = 0;
   double relres = 0;
+  bool fused = false;   // some iterations ran with their vector work in the operator launches
 };
 
-// The TFQMR iteration, `iters` times: step1, A y2, step2, A w, v.
+// Whether the block-Jacobi iteration can run with its vector work carried by
+// the fused preconditioned launches (FP64, forward operator, one problem);
+// HEOMQ_TFQ_FUSED=0 keeps the five-kernel iteration for comparison.
+bool tfq_fusable(heomq_ctx* c, bool trans, bool rdo) {
+  static const bool off = [] {
+    const char* e = std::getenv("HEOMQ_TFQ_FUSED");
+    return e && e[0] == '0';
+  }();
+  if (off || trans || !c->fuse_precond || c->trun.built) return false;
+  DevPacked op = c->pfwd;
+  op.precond = 0;
+  DevJacobi jc = c->pjac;
+  return apply_jac_fusable(op, jc) && (!rdo || op.trace_slot >= 0);
+}
+
+// The fused iteration, `iters` times: step 1 (with the previous iteration's
+// deferred x, d and v work), M^-1 A y2 with step 2, M^-1 A w with the rt.aw
+// sums, and the one-block fold of all three reductions.
+hipError_t record_fused(heomq_ctx* c, double* x, const Work<double>& W, int64_t n, hipStream_t st, bool rdo,
+                        int iters) {
+  TfqmrScal* s = c->scal.as<TfqmrScal>();
+  double* part = c->partial.as<double>();
+  DevPacked op = c->pfwd;
+  op.precond = 0;
+  DevJacobi jc = c->pjac;
+  jc.ejac = c->prec_shift;
+  const int na = (int)apply_jac_blocks(op);
+  double* pa = c->partial_apply.as<double>();
+  if (!pa || c->partial_apply.bytes < sizeof(double) * 4 * (size_t)na) return hipErrorInvalidValue;
+  for (int k = 0; k < iters; ++k) {
+    if (hipError_t e = tfq_step1_fused(st, n, W.w, W.y2, W.v, W.aw, W.u2, W.d, x, s, part)) return e;
+    if (hipError_t e = launch_apply_packed_jac_tfq(op, jc, W.y2, W.u2, rdo, 1, TfqEpi{s, W.w, pa}, st)) return e;
+    if (hipError_t e = launch_apply_packed_jac_tfq(op, jc, W.w, W.aw, rdo, 2, TfqEpi{s, nullptr, pa + 3 * na}, st))
+      return e;
+    if (hipError_t e = tfq_fold(st, n, part, pa, na, s)) return e;
+  }
+  return hipSuccess;
+}
+
+// The TFQMR iteration, `iters` times: step1, A y2, step2, A w, v (fused >= 0:
+// the fused iteration, with the trace row when fused is 1).
 template <typename T, typename Apply>
-hipError_t record_chunk(heomq_ctx* c, T* x, const Work<T>& W, int64_t n, hipStream_t st, Apply& A, int iters) {
+hipError_t record_chunk(heomq_ctx* c, T* x, const Work<T>& W, int64_t n, hipStream_t st, Apply& A, int iters,
+                        int fused = -1) {
+  if (fused >= 0) {
+    if constexpr (std::is_same<T, double>::value) return record_fused(c, x, W, n, st, fused == 1, iters);
+    return hipErrorInvalidValue;
+  }
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   TfqmrScal* s2 = c->scal2.as<TfqmrScal>();
   double* part = c->partial.as<double>();
@@ -153,7 +200,7 @@ hipError_t record_chunk(heomq_ctx* c, T* x, const Work<T>& W, int64_t n, hipStre
 // convergence or breakdown does nothing.
 template <typename T, typename Apply>
 int chunk_graph(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool trans, int64_t tpos, int mode,
-                hipStream_t st, Apply& A, int iters, hipGraphExec_t* out) {
+                hipStream_t st, Apply& A, int iters, hipGraphExec_t* out, int fused = -1) {
   for (const auto& g : c->graphs)
     if (g.x == x && g.b == b && g.work == W.w && g.n == n && g.tpos == tpos && g.mode == mode &&
         g.trans == (int)trans && g.bytes == (int)sizeof(T) && g.stream == st) {
@@ -163,7 +210,7 @@ int chunk_graph(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, boo
   if (std::getenv("HEOMQ_SOLVER_LOG"))
     std::fprintf(stderr, "tfqmr: capturing a chunk graph (mode %d, %zu cached)\n", mode, c->graphs.size());
   HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  const hipError_t rec = record_chunk<T>(c, x, W, n, st, A, iters);
+  const hipError_t rec = record_chunk<T>(c, x, W, n, st, A, iters, fused);
   hipGraph_t graph = nullptr;
   const hipError_t end = hipStreamEndCapture(st, &graph);
   if (rec != hipSuccess || end != hipSuccess) {
@@ -208,6 +255,9 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
                                  "ADO-aligned inverse)");
     HIPCHECK(precond_rhs(c, b, W.bp, W.pt, st));
   }
+  bool fuse_ok = false;
+  if constexpr (std::is_same<T, double>::value) fuse_ok = mode == 1 && prec == 2 && tfq_fusable(c, trans, tpos >= 0);
+  if (fuse_ok) HIPCHECK(c->partial_apply.alloc(sizeof(double) * 4 * (size_t)apply_jac_blocks(c->pfwd)));
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   double* part = c->partial.as<double>();
   double* red = c->red.as<double>();
@@ -273,20 +323,30 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
     hipGraphExec_t chunk = nullptr;
     int no_progress = 0;
     double last_check = rr;
+    // the fused iteration leaves x one deferred update behind: flush() before x is read
+    const int fused = (prec == 2 && fuse_ok) ? (tpos >= 0 ? 1 : 0) : -1;
+    rs.fused = rs.fused || fused >= 0;
+    auto flush = [&]() -> hipError_t {
+      if constexpr (std::is_same<T, double>::value)
+        if (fused >= 0) return tfq_flush(st, n, x, W.d, W.y2, s);
+      return hipSuccess;
+    };
     if (c->use_graphs)
-      if (int rc = chunk_graph<T>(c, x, b, W, n, trans, tpos, mode + 2 * (int)prec, st, Ait, check_every, &chunk))
+      if (int rc = chunk_graph<T>(c, x, b, W, n, trans, tpos, mode + 2 * (int)prec + 8 * (fused >= 0), st, Ait,
+                                  check_every, &chunk, fused))
         return rc;
     for (int replay = 1;; ++replay) {
       if (chunk)
         HIPCHECK(hipGraphLaunch(chunk, st));
       else
-        HIPCHECK((record_chunk<T>(c, x, W, n, st, Ait, check_every)));
+        HIPCHECK((record_chunk<T>(c, x, W, n, st, Ait, check_every, fused)));
       HIPCHECK(hipMemcpyAsync(&hs, s, sizeof(TfqmrScal), hipMemcpyDeviceToHost, st));
       HIPCHECK(hipStreamSynchronize(st));
       rs.iters = base_iters + hs.iter;
       rs.applies += 2 * check_every;
       if (hs.flag != 0 || rs.iters >= maxiter) break;
       if (replay % verify_every == 0) {   // converged already, drifting away or stagnating?
+        HIPCHECK(flush());
         double r2 = 0, b2 = 0;
         if (int rc = true_resid(r2, b2)) return rc;
         if (log)
@@ -304,6 +364,7 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
         if (no_progress >= 2) break;
       }
     }
+    HIPCHECK(flush());
     base_iters = rs.iters;
     if (hs.flag == 2 && ++stalls > 3) return fail(HEOMQ_BREAKDOWN, "TFQMR breakdown");
   }
@@ -476,6 +537,7 @@ int steady_solve(heomq_ctx* c, double* x, const double* b, double tol, int maxit
     stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     stats[4] = prec;
     stats[5] = (double)n;
+    stats[6] = rs.fused ? 1.0 : 0.0;
   }
   return rc;
 }

@@ -8,6 +8,7 @@
 
 #include <cstdint>
 
+#include "heomq_shadow.hpp"
 #include "heomq_solver.hpp"
 
 namespace heomq {
@@ -131,16 +132,6 @@ __device__ inline void fold_partials(const double* __restrict__ partial, int64_t
   b = fb;
 }
 
-// Shadow vector of TFQMR, generated where it is used instead of stored: a
-// +-1 sign per real element from a splitmix64 hash of its index (the same
-// bits for any grid and any call).
-__device__ inline double shadow(int64_t k, uint64_t seed) {
-  uint64_t z = 0x5EED1234ull + 0xD1B54A32D192ED03ull * seed + 0x9E3779B97F4A7C15ull * (uint64_t)(k + 1);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z ^= z >> 31;
-  return (z >> 63) ? -1.0 : 1.0;
-}
 
 // One quasi-minimisation half step from ||w||^2 = a.
 __device__ inline void half_step(TfqmrScal* s, double a) {
@@ -367,6 +358,130 @@ __global__ void k_v(const Loc loc, const T* __restrict__ aw, const T* __restrict
   span_partial(sg, 0.0, partial, sp);
 }
 
+// ---------------------------------------------------------------------------
+// The iteration fused into the operator launches (FP64, one problem; the
+// operator side is k_apply_packed_jac's TFQMR epilogues)
+// ---------------------------------------------------------------------------
+
+// The previous iteration's deferred work, then step 1: d_b = cy y2 + cd d;
+// x += eta_a d + eta_b d_b; v = aw + beta (2 u2 + beta v); y1 = w + beta y2,
+// u1 = aw + beta u2; y2 = y1 - alpha v; w -= alpha u1; d = y1 + dcoef d_b;
+// partials of ||w||^2 and rt.v.  After a (re)start beta = 0 and v = aw, so
+// the v update rewrites the same values.
+__global__ void k_step1_fused(int64_t n2, double* __restrict__ w, double* __restrict__ y2, double* __restrict__ v,
+                              const double* __restrict__ aw, const double* __restrict__ u2, double* __restrict__ d,
+                              double* __restrict__ x, const TfqmrScal* s, double* partial) {
+  const TfqmrScal t = *s;
+  if (t.flag) return;
+  const double alpha = t.alpha, beta = t.beta, dcoef = t.dcoef, ea = t.eta_a, eb = t.eta_b, cy = t.cy, cd = t.cd;
+  const uint64_t seed = t.seed;
+  double ww = 0, sv = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 yo = reinterpret_cast<const double2*>(y2)[i];
+    const double2 da = reinterpret_cast<const double2*>(d)[i];
+    double2 db;
+    db.x = cy * yo.x + cd * da.x;
+    db.y = cy * yo.y + cd * da.y;
+    double2 xi = reinterpret_cast<double2*>(x)[i];
+    xi.x += ea * da.x + eb * db.x;
+    xi.y += ea * da.y + eb * db.y;
+    reinterpret_cast<double2*>(x)[i] = xi;
+    const double2 a = reinterpret_cast<const double2*>(aw)[i], u = reinterpret_cast<const double2*>(u2)[i];
+    double2 vi = reinterpret_cast<double2*>(v)[i];
+    vi.x = a.x + beta * (2.0 * u.x + beta * vi.x);
+    vi.y = a.y + beta * (2.0 * u.y + beta * vi.y);
+    reinterpret_cast<double2*>(v)[i] = vi;
+    sv += shadow(2 * i, seed) * vi.x + shadow(2 * i + 1, seed) * vi.y;
+    double2 wi = reinterpret_cast<double2*>(w)[i];
+    double2 y1;
+    y1.x = wi.x + beta * yo.x;
+    y1.y = wi.y + beta * yo.y;
+    double2 o;
+    o.x = y1.x - alpha * vi.x;
+    o.y = y1.y - alpha * vi.y;
+    reinterpret_cast<double2*>(y2)[i] = o;
+    wi.x -= alpha * (a.x + beta * u.x);
+    wi.y -= alpha * (a.y + beta * u.y);
+    reinterpret_cast<double2*>(w)[i] = wi;
+    double2 dn;
+    dn.x = y1.x + dcoef * db.x;
+    dn.y = y1.y + dcoef * db.y;
+    reinterpret_cast<double2*>(d)[i] = dn;
+    ww += wi.x * wi.x + wi.y * wi.y;
+  }
+  block_partial(ww, sv, partial);
+}
+
+// One workgroup: the iteration's reductions in a fixed order, both half
+// steps, the stop test, beta, sigma and alpha.  pv: (||w||^2, rt.v) of
+// k_step1_fused's nv blocks; pa: (||w||^2, rt.w, rt.u2) of the first
+// operator launch's na blocks, then rt.aw of the second's.  sigma =
+// rt.(aw + beta (2 u2 + beta v)) from the sums over aw, u2 and the stored v:
+// the v the next k_step1_fused forms, one step from a directly summed rt.v.
+__global__ void k_tfq_fold(const double* __restrict__ pv, int nv, const double* __restrict__ pa, int na,
+                           TfqmrScal* s) {
+  if (s->flag) return;
+  double a1 = 0, sv = 0, a2 = 0, rw = 0, ru = 0, ra = 0;
+  for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+    a1 += pv[2 * t];
+    sv += pv[2 * t + 1];
+  }
+  for (int t = threadIdx.x; t < na; t += blockDim.x) {
+    a2 += pa[3 * t];
+    rw += pa[3 * t + 1];
+    ru += pa[3 * t + 2];
+    ra += pa[3 * na + t];
+  }
+  block_sum2(a1, sv);
+  __syncthreads();
+  block_sum2(a2, rw);
+  __syncthreads();
+  block_sum2(ru, ra);
+  if (threadIdx.x != 0) return;
+  TfqmrScal t = *s;
+  half_step(&t, a1);
+  t.eta_a = t.eta;
+  t.cy = 1.0;
+  t.cd = t.dcoef;
+  half_step_rho(&t, a2, rw);
+  t.eta_b = t.flag ? 0.0 : t.eta;   // a breakdown keeps x at the first half step
+  if (t.stop) {
+    t.flag = 1;
+  } else if (!t.flag) {
+    const double sg = ra + t.beta * (2.0 * ru + t.beta * sv);
+    t.sigma = sg;
+    if (sg == 0.0) {
+      t.flag = 2;
+    } else {
+      t.alpha = t.rho / sg;
+      t.dcoef = t.theta * t.theta * t.eta / t.alpha;
+    }
+  }
+  *s = t;
+}
+
+// x += eta_a d + eta_b (cy y2 + cd d): the pending update of the last fused
+// iteration; k_tfq_flush_scal then clears eta_a and eta_b, so that a second
+// flush or the next k_step1_fused adds nothing more to x.
+__global__ void k_tfq_flush(int64_t n2, double* __restrict__ x, const double* __restrict__ d,
+                            const double* __restrict__ y2, const TfqmrScal* s) {
+  const double ea = s->eta_a, eb = s->eta_b, cy = s->cy, cd = s->cd;
+  if (ea == 0.0 && eb == 0.0) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 da = reinterpret_cast<const double2*>(d)[i], yo = reinterpret_cast<const double2*>(y2)[i];
+    double2 xi = reinterpret_cast<double2*>(x)[i];
+    xi.x += ea * da.x + eb * (cy * yo.x + cd * da.x);
+    xi.y += ea * da.y + eb * (cy * yo.y + cd * da.y);
+    reinterpret_cast<double2*>(x)[i] = xi;
+  }
+}
+__global__ void k_tfq_flush_scal(TfqmrScal* s) {
+  if (threadIdx.x == 0) {
+    s->eta_a = 0;
+    s->eta_b = 0;
+  }
+}
+
 // partials of ||b - ax||^2 and ||b||^2 (FP64)
 template <typename TB, typename TA, typename Loc>
 __global__ void k_resid(const Loc loc, const TB* __restrict__ b, const TA* __restrict__ ax, double* partial) {
@@ -417,6 +532,9 @@ __global__ void k_finalize(const Loc loc, int op, int nblk_uni, const double* __
       s->eta = 0;
       s->dcoef = 0;
       s->stop = 0;
+      s->eta_a = s->eta_b = 0;
+      s->cy = 0;
+      s->cd = 1;
       s->flag = (b == 0.0 && a > 0.0) ? 2 : kInitialising;
       break;
     case F_INIT_V:
@@ -533,6 +651,9 @@ __global__ void k_tfq_reset(TfqmrScal* s, double target, const int32_t* mask, in
   q->seed = fresh ? first_seed : q->seed + 1;
   q->rho = q->sigma = q->alpha = q->beta = 0;
   q->theta = q->tau = q->eta = q->dcoef = 0;
+  q->eta_a = q->eta_b = 0;
+  q->cy = 0;
+  q->cd = 1;
   q->rr = 0;
   q->target = target;
   q->iter = 0;
@@ -635,6 +756,24 @@ hipError_t tfq_v(hipStream_t st, int64_t n, const T* aw, const T* u2, T* v, T* x
   LB((k_v<T, UniLoc>), n2, aw, u2, v, x, d, s2, s, partial2, partial);
   if (hipError_t e = hipGetLastError()) return e;
   return finalize(st, F_V, grid_for<T>(n2), partial, s, nullptr, bt.nprob);
+}
+hipError_t tfq_step1_fused(hipStream_t st, int64_t n, double* w, double* y2, double* v, const double* aw,
+                           const double* u2, double* d, double* x, const TfqmrScal* s, double* pv) {
+  const int64_t n2 = n / 2;
+  hipLaunchKernelGGL(k_step1_fused, dim3(grid_for<double>(n2)), dim3(kRedThreads), 0, st, n2, w, y2, v, aw, u2, d,
+                     x, s, pv);
+  return hipGetLastError();
+}
+hipError_t tfq_fold(hipStream_t st, int64_t n, const double* pv, const double* pa, int na, TfqmrScal* s) {
+  hipLaunchKernelGGL(k_tfq_fold, dim3(1), dim3(kRedThreads), 0, st, pv, grid_for<double>(n / 2), pa, na, s);
+  return hipGetLastError();
+}
+hipError_t tfq_flush(hipStream_t st, int64_t n, double* x, const double* d, const double* y2, TfqmrScal* s) {
+  const int64_t n2 = n / 2;
+  hipLaunchKernelGGL(k_tfq_flush, dim3(grid_for<double>(n2)), dim3(kRedThreads), 0, st, n2, x, d, y2, s);
+  if (hipError_t e = hipGetLastError()) return e;
+  hipLaunchKernelGGL(k_tfq_flush_scal, dim3(1), dim3(64), 0, st, s);
+  return hipGetLastError();
 }
 template <typename T>
 hipError_t vec_fill(hipStream_t st, int64_t n, T* v, T val) {

@@ -55,6 +55,10 @@ struct TfqmrScal {
   int32_t stop;           // bound met this iteration: finish its x update, then flag = 1
   int32_t unscaled;       // batches: 1 = this problem iterates without the Jacobi scaling
   uint64_t seed;          // shadow vector of this (re)start: a new one at every restart
+  // Deferred updates of the iteration fused into the operator launches: the
+  // previous iteration's x += eta_a d_a + eta_b d_b with d_b = cy y2 + cd d_a
+  // runs at the start of the next tfq_step1_fused (or in tfq_flush).
+  double eta_a, eta_b, cy, cd;
 };
 
 // BiCG scalars (bicg/solvebicg.f90 notation: rho = <s, r>, alpha, beta).
@@ -106,6 +110,26 @@ hipError_t tfq_step2(hipStream_t, int64_t n, T* x, T* w, const T* u2, T* d, cons
 template <typename T>
 hipError_t tfq_v(hipStream_t, int64_t n, const T* aw, const T* u2, T* v, T* x, const T* d, TfqmrScal* s,
                  const TfqmrScal* s2, double* partial, const double* partial2, Batch bt = Batch{});
+
+// The same iteration with the vector work of tfq_step2 and tfq_v moved into
+// the two block-Jacobi-preconditioned operator launches and the next
+// iteration's first kernel (FP64, one problem):
+//   tfq_step1_fused   the previous iteration's deferred x update, d_b and v
+//                     (v = aw + beta (2 u2 + beta v)), then step 1; partials
+//                     of ||w||^2 and rt.v (vector grid, pv)
+//   M^-1 A y2 -> u2   with w -= alpha u2 and partials of ||w||^2, rt.w,
+//                     rt.u2 per operator workgroup (pa, three per block)
+//   M^-1 A w -> aw    with partials of rt.aw (pa + 3 na, one per block)
+//   tfq_fold          both half steps, the stop test, beta, and
+//                     sigma = rt.v_new = rt.aw + beta (2 rt.u2 + beta rt.v)
+//                     from the stored v's direct sum, alpha = rho / sigma.
+// x lags by the deferred update between iterations: tfq_flush completes it
+// (a no-op when nothing is pending) before x is read.
+hipError_t tfq_step1_fused(hipStream_t, int64_t n, double* w, double* y2, double* v, const double* aw,
+                           const double* u2, double* d, double* x, const TfqmrScal* s, double* pv);
+hipError_t tfq_fold(hipStream_t, int64_t n, const double* pv, const double* pa, int na, TfqmrScal* s);
+hipError_t tfq_flush(hipStream_t, int64_t n, double* x, const double* d, const double* y2, TfqmrScal* s);
+
 template <typename T>
 hipError_t vec_fill(hipStream_t, int64_t n, T* v, T val);
 template <typename T>

@@ -769,7 +769,7 @@ class Context:
         rc = lib().heomq_solve_steady(self._h, self._check_vec(x, t.complex128), bp, tol, maxiter, trace,
                                       int(transpose), _dp(stats), self._stream())
         info = dict(iters=int(stats[0]), applies=int(stats[1]), relres=stats[2], seconds=stats[3], status=rc,
-                    precond=int(stats[4]))
+                    precond=int(stats[4]), fused_iteration=bool(stats[6]))
         if rc not in (_lib.OK, _lib.NOT_CONVERGED) or (rc == _lib.NOT_CONVERGED and raise_on_fail):
             check(rc)
         return x, info

@@ -323,7 +323,11 @@ int heomq_calcderiv(heomq_ctx* c, const double* rhotmp_host, double* rhorhs_host
  * stats (HEOMQ_STATS_LEN): [0] iterations, [1] applies, [2] final relative
  * residual, [3] seconds, [4] the left preconditioner the iteration ended
  * with (0 none, 1 Jacobi scaling, 2 block Jacobi), [5] length of the real
- * vector iterated on, [6..7] 0.  Returns HEOMQ_NOT_CONVERGED if maxiter hit.
+ * vector iterated on, [6] 1 when block-Jacobi iterations ran with their
+ * vector work carried by the fused preconditioned launches (FP64 forward
+ * solve, fusable apply form, fusion on: heomq_ctx_set_precond_fusion; the
+ * same iterates in exact arithmetic, reductions in another order), [7] 0.
+ * Returns HEOMQ_NOT_CONVERGED if maxiter hit.
  * ------------------------------------------------------------------------- */
 int heomq_solve_steady(heomq_ctx* c, double* x, const double* b, double tol,
                        int maxiter, int trace, int transpose, double* stats,
