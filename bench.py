@@ -209,7 +209,7 @@ def main():
             roofline = {"bound": "hbm", "achieved": round(a_f, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(a_f / HBM_PEAK_GBS, 4),
                         "traffic": measured_traffic(args.workload, "precond_apply"),
-                        "kernel": "k_apply_packed_jac<true> (packed apply + block-Jacobi inverse, one launch)",
+                        "kernel": "k_apply_packed_jac<true, 0> (packed apply + block-Jacobi inverse, one launch)",
                         "t_apply_us": round(t_fused * 1e6, 2), "form": tuned["forward"], "bytes_alg": bytes_alg,
                         "packed_elems": P, "plain_apply": plain}
 

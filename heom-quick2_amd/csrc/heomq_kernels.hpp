@@ -334,8 +334,9 @@ hipError_t launch_apply_packed_jac(const DevPacked& op, const DevJacobi& jc, con
                                    hipStream_t s);
 
 // A TFQMR iteration's vector work carried by the preconditioned launch (kind
-// 1: out = u2, w -= alpha u2, partials (||w||^2, rt.w, rt.u2) per workgroup;
-// kind 2: out = aw, partial rt.aw per workgroup); s is the problem's device
+// 1: out = u2, w -= alpha u2, partials ||w||^2, rt.w, rt.u2 per workgroup,
+// three arrays of the workgroup count; kind 2: out = aw, partial rt.aw per
+// workgroup); s is the problem's device
 // scalars.  apply_jac_blocks is the launch's workgroup count.
 struct TfqmrScal;
 struct TfqEpi {

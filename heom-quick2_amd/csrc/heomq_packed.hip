@@ -1287,7 +1287,8 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
 //
 // EP > 0 carries a TFQMR iteration's vector work on the outputs (one
 // problem): EP 1, out = u2 = M^-1 A y2, then w -= alpha u2 and per-workgroup
-// partials (||w||^2, rt.w, rt.u2) at ep.partial[3 blockIdx.x ..]; EP 2, out =
+// partials ||w||^2, rt.w and rt.u2 at ep.partial[k gridDim.x + blockIdx.x],
+// k = 0, 1, 2; EP 2, out =
 // aw = M^-1 A w and the partial rt.aw at ep.partial[blockIdx.x].  Nothing
 // runs once the problem's flag is set.
 template <bool RU, int EP>
@@ -1460,9 +1461,9 @@ k_apply_packed_jac(const DevPacked op, const DevJacobi jc, const double* __restr
       c += red[2][k];
     }
     if (EP == 1) {
-      ep.partial[3 * blockIdx.x] = a;
-      ep.partial[3 * blockIdx.x + 1] = b;
-      ep.partial[3 * blockIdx.x + 2] = c;
+      ep.partial[blockIdx.x] = a;
+      ep.partial[gridDim.x + blockIdx.x] = b;
+      ep.partial[2 * gridDim.x + blockIdx.x] = c;
     } else {
       ep.partial[blockIdx.x] = a;
     }

@@ -412,32 +412,53 @@ __global__ void k_step1_fused(int64_t n2, double* __restrict__ w, double* __rest
   block_partial(ww, sv, partial);
 }
 
-// One workgroup: the iteration's reductions in a fixed order, both half
-// steps, the stop test, beta, sigma and alpha.  pv: (||w||^2, rt.v) of
-// k_step1_fused's nv blocks; pa: (||w||^2, rt.w, rt.u2) of the first
-// operator launch's na blocks, then rt.aw of the second's.  sigma =
-// rt.(aw + beta (2 u2 + beta v)) from the sums over aw, u2 and the stored v:
-// the v the next k_step1_fused forms, one step from a directly summed rt.v.
-__global__ void k_tfq_fold(const double* __restrict__ pv, int nv, const double* __restrict__ pa, int na,
-                           TfqmrScal* s) {
+// One workgroup of kFoldThreads: the iteration's reductions in a fixed order,
+// both half steps, the stop test, beta, sigma and alpha.  pv: (||w||^2, rt.v)
+// of k_step1_fused's nv blocks, interleaved; pa: ||w||^2, rt.w and rt.u2 of
+// the first operator launch's na blocks, then rt.aw of the second's, each an
+// array of na.  sigma = rt.(aw + beta (2 u2 + beta v)) from the sums over aw,
+// u2 and the stored v: the v the next k_step1_fused forms, one step from a
+// directly summed rt.v.
+constexpr int kFoldThreads = 1024;
+__global__ void __launch_bounds__(kFoldThreads) k_tfq_fold(const double* __restrict__ pv, int nv,
+                                                           const double* __restrict__ pa, int na, TfqmrScal* s) {
   if (s->flag) return;
-  double a1 = 0, sv = 0, a2 = 0, rw = 0, ru = 0, ra = 0;
-  for (int t = threadIdx.x; t < nv; t += blockDim.x) {
-    a1 += pv[2 * t];
-    sv += pv[2 * t + 1];
+  double v[6] = {0, 0, 0, 0, 0, 0};   // ||w||^2 and rt.v of step 1; ||w||^2, rt.w, rt.u2, rt.aw
+  for (int t = threadIdx.x; t < nv; t += kFoldThreads) {
+    v[0] += pv[2 * t];
+    v[1] += pv[2 * t + 1];
   }
-  for (int t = threadIdx.x; t < na; t += blockDim.x) {
-    a2 += pa[3 * t];
-    rw += pa[3 * t + 1];
-    ru += pa[3 * t + 2];
-    ra += pa[3 * na + t];
+  // four rows per thread and pass, all sixteen loads issued before the sums
+  for (int t0 = threadIdx.x; t0 < na; t0 += 4 * kFoldThreads) {
+    double l[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int t = t0 + u * kFoldThreads;
+        l[u][k] = t < na ? pa[(int64_t)k * na + t] : 0.0;
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[2 + k] += l[u][k];
   }
-  block_sum2(a1, sv);
+  __shared__ double red[6][kFoldThreads / 64];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k][wid] = v[k];
   __syncthreads();
-  block_sum2(a2, rw);
-  __syncthreads();
-  block_sum2(ru, ra);
   if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    v[k] = 0;
+    for (int q = 0; q < kFoldThreads / 64; ++q) v[k] += red[k][q];
+  }
+  const double a1 = v[0], sv = v[1], a2 = v[2], rw = v[3], ru = v[4], ra = v[5];
   TfqmrScal t = *s;
   half_step(&t, a1);
   t.eta_a = t.eta;
@@ -765,7 +786,7 @@ hipError_t tfq_step1_fused(hipStream_t st, int64_t n, double* w, double* y2, dou
   return hipGetLastError();
 }
 hipError_t tfq_fold(hipStream_t st, int64_t n, const double* pv, const double* pa, int na, TfqmrScal* s) {
-  hipLaunchKernelGGL(k_tfq_fold, dim3(1), dim3(kRedThreads), 0, st, pv, grid_for<double>(n / 2), pa, na, s);
+  hipLaunchKernelGGL(k_tfq_fold, dim3(1), dim3(kFoldThreads), 0, st, pv, grid_for<double>(n / 2), pa, na, s);
   return hipGetLastError();
 }
 hipError_t tfq_flush(hipStream_t st, int64_t n, double* x, const double* d, const double* y2, TfqmrScal* s) {

@@ -118,7 +118,7 @@ hipError_t tfq_v(hipStream_t, int64_t n, const T* aw, const T* u2, T* v, T* x, c
 //                     (v = aw + beta (2 u2 + beta v)), then step 1; partials
 //                     of ||w||^2 and rt.v (vector grid, pv)
 //   M^-1 A y2 -> u2   with w -= alpha u2 and partials of ||w||^2, rt.w,
-//                     rt.u2 per operator workgroup (pa, three per block)
+//                     rt.u2 per operator workgroup (pa: three arrays of na)
 //   M^-1 A w -> aw    with partials of rt.aw (pa + 3 na, one per block)
 //   tfq_fold          both half steps, the stop test, beta, and
 //                     sigma = rt.v_new = rt.aw + beta (2 rt.u2 + beta rt.v)
