@@ -257,7 +257,11 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
   }
   bool fuse_ok = false;
   if constexpr (std::is_same<T, double>::value) fuse_ok = mode == 1 && prec == 2 && tfq_fusable(c, trans, tpos >= 0);
-  if (fuse_ok) HIPCHECK(c->partial_apply.alloc(sizeof(double) * 4 * (size_t)apply_jac_blocks(c->pfwd)));
+  if (fuse_ok) {   // captured chunks hold the buffer's address: a new buffer retires them
+    const void* before = c->partial_apply.p;
+    HIPCHECK(c->partial_apply.alloc(sizeof(double) * 4 * (size_t)apply_jac_blocks(c->pfwd)));
+    if (c->partial_apply.p != before) c->drop_graphs();
+  }
   TfqmrScal* s = c->scal.as<TfqmrScal>();
   double* part = c->partial.as<double>();
   double* red = c->red.as<double>();
