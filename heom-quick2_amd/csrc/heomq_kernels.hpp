@@ -336,13 +336,16 @@ hipError_t launch_apply_packed_jac(const DevPacked& op, const DevJacobi& jc, con
 // A TFQMR iteration's vector work carried by the preconditioned launch (kind
 // 1: out = u2, w -= alpha u2, partials ||w||^2, rt.w, rt.u2 per workgroup,
 // three arrays of the workgroup count; kind 2: out = aw, partial rt.aw per
-// workgroup); s is the problem's device
+// workgroup, and kind 1's arrays summed in runs of 16 workgroups into
+// prev_out, three arrays of ceil(count / 16)); s is the problem's device
 // scalars.  apply_jac_blocks is the launch's workgroup count.
 struct TfqmrScal;
 struct TfqEpi {
   const TfqmrScal* s = nullptr;
   double* w = nullptr;
   double* partial = nullptr;
+  const double* prev = nullptr;   // kind 2: kind 1's three arrays, summed in runs of 16 into prev_out
+  double* prev_out = nullptr;
 };
 int64_t apply_jac_blocks(const DevPacked& op);
 hipError_t launch_apply_packed_jac_tfq(const DevPacked& op, const DevJacobi& jc, const double* x, double* y, bool rdo,

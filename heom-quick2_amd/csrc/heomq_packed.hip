@@ -1288,7 +1288,8 @@ __global__ __launch_bounds__(256) void k_jacobi_ado(const DevPacked op, const De
 // EP > 0 carries a TFQMR iteration's vector work on the outputs (one
 // problem): EP 1, out = u2 = M^-1 A y2, then w -= alpha u2 and per-workgroup
 // partials ||w||^2, rt.w and rt.u2 at ep.partial[k gridDim.x + blockIdx.x],
-// k = 0, 1, 2; EP 2, out =
+// k = 0, 1, 2; EP 2 also sums those (ep.prev) in runs of 16 workgroups into
+// ep.prev_out[k ceil(gridDim.x / 16) + blockIdx.x]; EP 2, out =
 // aw = M^-1 A w and the partial rt.aw at ep.partial[blockIdx.x].  Nothing
 // runs once the problem's flag is set.
 template <bool RU, int EP>
@@ -1466,6 +1467,15 @@ k_apply_packed_jac(const DevPacked op, const DevJacobi jc, const double* __restr
       ep.partial[2 * gridDim.x + blockIdx.x] = c;
     } else {
       ep.partial[blockIdx.x] = a;
+    }
+  }
+  if (EP == 2) {   // the first launch's sums, 16 workgroups' worth per workgroup here, for the fold
+    const int na = (int)gridDim.x, ng = (na + 15) >> 4;
+    if ((int)blockIdx.x < ng && tid < 64) {   // wave 0: lanes (k, j), k = tid / 16 < 3 the sum
+      const int k = tid >> 4, j = tid & 15, idx = (int)blockIdx.x * 16 + j;
+      double v = (k < 3 && idx < na) ? ep.prev[(int64_t)k * na + idx] : 0.0;
+      for (int off = 8; off > 0; off >>= 1) v += __shfl_down(v, off, 16);
+      if (k < 3 && j == 0) ep.prev_out[(int64_t)k * ng + blockIdx.x] = v;
     }
   }
 }
@@ -1703,7 +1713,7 @@ int64_t apply_jac_blocks(const DevPacked& op) { return op.n_heavy + op.n_lt_bloc
 hipError_t launch_apply_packed_jac_tfq(const DevPacked& op, const DevJacobi& jc, const double* x, double* y, bool rdo,
                                        int kind, const TfqEpi& ep, hipStream_t s) {
   if (!apply_jac_fusable(op, jc) || (rdo && op.trace_slot < 0) || (kind != 1 && kind != 2) || !ep.s ||
-      !ep.partial || (kind == 1 && !ep.w))
+      !ep.partial || (kind == 1 && !ep.w) || (kind == 2 && (!ep.prev || !ep.prev_out)))
     return hipErrorInvalidValue;
   const dim3 grid((unsigned)apply_jac_blocks(op));
   const int r = rdo ? 1 : 0;

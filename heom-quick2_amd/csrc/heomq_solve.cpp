@@ -147,6 +147,10 @@ bool tfq_fusable(heomq_ctx* c, bool trans, bool rdo) {
   return apply_jac_fusable(op, jc) && (!rdo || op.trace_slot >= 0);
 }
 
+// Sums of the fused iteration's operator launches over na workgroups: four
+// arrays of na, then three of ceil(na / 16).
+size_t tfq_fused_sums(int64_t na) { return 4 * (size_t)na + 3 * (size_t)((na + 15) / 16); }
+
 // The fused iteration, `iters` times: step 1 (with the previous iteration's
 // deferred x, d and v work), M^-1 A y2 with step 2, M^-1 A w with the rt.aw
 // sums, and the one-block fold of all three reductions.
@@ -160,11 +164,12 @@ hipError_t record_fused(heomq_ctx* c, double* x, const Work<double>& W, int64_t 
   jc.ejac = c->prec_shift;
   const int na = (int)apply_jac_blocks(op);
   double* pa = c->partial_apply.as<double>();
-  if (!pa || c->partial_apply.bytes < sizeof(double) * 4 * (size_t)na) return hipErrorInvalidValue;
+  if (!pa || c->partial_apply.bytes < sizeof(double) * tfq_fused_sums(na)) return hipErrorInvalidValue;
   for (int k = 0; k < iters; ++k) {
     if (hipError_t e = tfq_step1_fused(st, n, W.w, W.y2, W.v, W.aw, W.u2, W.d, x, s, part)) return e;
     if (hipError_t e = launch_apply_packed_jac_tfq(op, jc, W.y2, W.u2, rdo, 1, TfqEpi{s, W.w, pa}, st)) return e;
-    if (hipError_t e = launch_apply_packed_jac_tfq(op, jc, W.w, W.aw, rdo, 2, TfqEpi{s, nullptr, pa + 3 * na}, st))
+    if (hipError_t e = launch_apply_packed_jac_tfq(op, jc, W.w, W.aw, rdo, 2,
+                                                   TfqEpi{s, nullptr, pa + 3 * na, pa, pa + 4 * na}, st))
       return e;
     if (hipError_t e = tfq_fold(st, n, part, pa, na, s)) return e;
   }
@@ -259,7 +264,7 @@ int tfqmr_run(heomq_ctx* c, T* x, const T* b, const Work<T>& W, int64_t n, bool 
   if constexpr (std::is_same<T, double>::value) fuse_ok = mode == 1 && prec == 2 && tfq_fusable(c, trans, tpos >= 0);
   if (fuse_ok) {   // captured chunks hold the buffer's address: a new buffer retires them
     const void* before = c->partial_apply.p;
-    HIPCHECK(c->partial_apply.alloc(sizeof(double) * 4 * (size_t)apply_jac_blocks(c->pfwd)));
+    HIPCHECK(c->partial_apply.alloc(sizeof(double) * tfq_fused_sums(apply_jac_blocks(c->pfwd))));
     if (c->partial_apply.p != before) c->drop_graphs();
   }
   TfqmrScal* s = c->scal.as<TfqmrScal>();

@@ -415,8 +415,9 @@ __global__ void k_step1_fused(int64_t n2, double* __restrict__ w, double* __rest
 // One workgroup of kFoldThreads: the iteration's reductions in a fixed order,
 // both half steps, the stop test, beta, sigma and alpha.  pv: (||w||^2, rt.v)
 // of k_step1_fused's nv blocks, interleaved; pa: ||w||^2, rt.w and rt.u2 of
-// the first operator launch's na blocks, then rt.aw of the second's, each an
-// array of na.  sigma = rt.(aw + beta (2 u2 + beta v)) from the sums over aw,
+// the first operator launch's na blocks and rt.aw of the second's, each an
+// array of na, then the first three summed in runs of 16 blocks by the second
+// launch (three arrays of ceil(na / 16)), which is what the fold reads.  sigma = rt.(aw + beta (2 u2 + beta v)) from the sums over aw,
 // u2 and the stored v: the v the next k_step1_fused forms, one step from a
 // directly summed rt.v.
 constexpr int kFoldThreads = 1024;
@@ -428,20 +429,20 @@ __global__ void __launch_bounds__(kFoldThreads) k_tfq_fold(const double* __restr
     v[0] += pv[2 * t];
     v[1] += pv[2 * t + 1];
   }
-  // four rows per thread and pass, all sixteen loads issued before the sums
+  // the first launch's three sums, already summed in runs of 16 workgroups by
+  // the second launch (ng per array), then the second's rt.aw (na)
+  const int ng = (na + 15) >> 4;
+  const double* q = pa + 4 * (int64_t)na;
+  for (int t = threadIdx.x; t < ng; t += kFoldThreads)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[2 + k] += q[(int64_t)k * ng + t];
+  const double* paw = pa + 3 * (int64_t)na;
   for (int t0 = threadIdx.x; t0 < na; t0 += 4 * kFoldThreads) {
-    double l[4][4];
+    double l[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 4; ++u) l[u] = t0 + u * kFoldThreads < na ? paw[t0 + u * kFoldThreads] : 0.0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int t = t0 + u * kFoldThreads;
-        l[u][k] = t < na ? pa[(int64_t)k * na + t] : 0.0;
-      }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) v[2 + k] += l[u][k];
+    for (int u = 0; u < 4; ++u) v[5] += l[u];
   }
   __shared__ double red[6][kFoldThreads / 64];
 #pragma unroll

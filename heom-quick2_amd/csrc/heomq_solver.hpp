@@ -119,7 +119,8 @@ hipError_t tfq_v(hipStream_t, int64_t n, const T* aw, const T* u2, T* v, T* x, c
 //                     of ||w||^2 and rt.v (vector grid, pv)
 //   M^-1 A y2 -> u2   with w -= alpha u2 and partials of ||w||^2, rt.w,
 //                     rt.u2 per operator workgroup (pa: three arrays of na)
-//   M^-1 A w -> aw    with partials of rt.aw (pa + 3 na, one per block)
+//   M^-1 A w -> aw    with partials of rt.aw (pa + 3 na, one per block), and
+//                     the first launch's sums in runs of 16 (pa + 4 na)
 //   tfq_fold          both half steps, the stop test, beta, and
 //                     sigma = rt.v_new = rt.aw + beta (2 rt.u2 + beta rt.v)
 //                     from the stored v's direct sum, alpha = rho / sigma.
