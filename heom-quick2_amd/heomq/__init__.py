@@ -761,7 +761,10 @@ class Context:
 
     def solve_steady(self, x=None, b=None, tol=1e-12, maxiter=20000, trace=TRACE_RDO, transpose=False,
                      raise_on_fail=True):
-        """TFQMR steady state on the device; returns (x, stats dict)."""
+        """TFQMR steady state on the device; returns (x, stats dict): iters,
+        applies, relres, seconds, status, precond (the left preconditioner
+        the iteration ended with) and fused_iteration (block-Jacobi
+        iterations ran with their vector work in the operator launches)."""
         t = self.torch
         x = self.initial_guess() if x is None else x
         stats = np.zeros(_lib.STATS_LEN)
